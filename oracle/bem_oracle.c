@@ -756,3 +756,17 @@ int orc_zgesv(int nn, zc *a, zc *b)
     }
     return 0;
 }
+
+/* The NPG x NPG Gauss points and weights of one element, in the loop order
+ * of Nonsingd.for:108-119 (JG outer, IG inner).  Exposed for tests. */
+int orc_quad_rule(const double co[4][3], const double *gi, const double *ome,
+                  int npg, double *xg, double *w)
+{
+    for (int jg = 0; jg < npg; ++jg)
+        for (int ig = 0; ig < npg; ++ig) {
+            int q = jg * npg + ig;
+            if (quad_point(co, gi[ig], ome[ig], gi[jg], ome[jg], &xg[3 * q], &w[q]))
+                return ORC_DEGENERATE;
+        }
+    return ORC_OK;
+}

@@ -111,10 +111,16 @@ class Oracle:
         xp, xg, rn = (np.ascontiguousarray(v, dtype=np.float64) for v in (xp, xg, rn))
         zu = np.zeros(9, dtype=np.complex128)
         zt = np.zeros(9, dtype=np.complex128)
-        f = self.lib.orc_solfundif if self.kind == "c" else self.lib.ref_solfundif
-        f.argtypes = [_dp, _dp, _dp, _Z, _Z, _Z] + [C.c_double] * 6 + [_dp, _dp]
-        f(_p(xp), _p(xg), _p(rn), _z(zge), _z(zcs), _z(zcp), c1, c2, c3, c4, pi, fr,
-          _p(zu), _p(zt))
+        if self.kind == "c":
+            f = self.lib.orc_solfundif
+            f.argtypes = [_dp, _dp, _dp, _Z, _Z, _Z] + [C.c_double] * 6 + [_dp, _dp]
+            f(_p(xp), _p(xg), _p(rn), _z(zge), _z(zcs), _z(zcp), c1, c2, c3, c4, pi, fr,
+              _p(zu), _p(zt))
+        else:
+            zm = np.array([zge, zcs, zcp], dtype=np.complex128)
+            f = self.lib.ref_solfundif
+            f.argtypes = [_dp, _dp, _dp, _dp] + [C.c_double] * 6 + [_dp, _dp]
+            f(_p(xp), _p(xg), _p(rn), _p(zm), c1, c2, c3, c4, pi, fr, _p(zu), _p(zt))
         return _fmat(zu), _fmat(zt)
 
     def solfune(self, xp, xg, eta, c3, c4):
@@ -208,11 +214,12 @@ class Oracle:
                 raise ArithmeticError("degenerate Jacobian")
         else:
             co_f = self._co(co)
+            zm = np.array([zge, zcs, zcp], dtype=np.complex128)
             f = self.lib.ref_sigmaec
-            f.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, _Z, C.c_double, _Z, _Z,
-                          C.c_int, _dp, _dp, _dp, _dp]
-            f(_p(co_f), _p(xp), _p(rn), pi, fr, _z(zge), rho, _z(zcs), _z(zcp), npg,
-              _p(gi), _p(ome), _p(zd), _p(zs))
+            f.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, _dp, C.c_double, C.c_int,
+                          _dp, _dp, _dp, _dp]
+            f(_p(co_f), _p(xp), _p(rn), pi, fr, _p(zm), rho, npg, _p(gi), _p(ome), _p(zd),
+              _p(zs))
         return (zd.reshape(3, 3, 3).transpose(2, 1, 0).copy(),
                 zs.reshape(3, 3, 3).transpose(2, 1, 0).copy())
 
@@ -344,3 +351,17 @@ class Oracle:
         u, t, zdsol, zssol = self.interec(mesh, dyn, x, zdfi, fr, zmat, consts[0])
         return dict(hestdiag=hd, gestdiag=gd, consts=consts, zmat=zmat, fr=fr, zh=zh,
                     zg=zg, zfi=zfi, x=x, u=u, t=t, zdsol=zdsol, zssol=zssol)
+
+
+def quad_rule(orc: "Oracle", co, gi, ome):
+    """Gauss points (NPG^2, 3) and weights OME*OME*DET of one element."""
+    npg = len(gi)
+    xg = np.zeros(3 * npg * npg)
+    w = np.zeros(npg * npg)
+    co_c = np.ascontiguousarray(co, dtype=np.float64)
+    gi, ome = np.ascontiguousarray(gi), np.ascontiguousarray(ome)
+    f = orc.lib.orc_quad_rule
+    f.argtypes = [_dp, _dp, _dp, C.c_int, _dp, _dp]
+    if f(_p(co_c), _p(gi), _p(ome), npg, _p(xg), _p(w)):
+        raise ArithmeticError("degenerate Jacobian")
+    return xg.reshape(-1, 3), w

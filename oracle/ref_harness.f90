@@ -100,15 +100,15 @@ contains
                  delta, pi, fr)
   end subroutine
 
-  subroutine ref_solfundif(xp, xg, rn, zge, zcs, zcp, c1, c2, c3, c4, pi, fr, zu, zt) &
+  subroutine ref_solfundif(xp, xg, rn, zm, c1, c2, c3, c4, pi, fr, zu, zt) &
       bind(C, name='ref_solfundif')
     real(c_double) :: xp(3), xg(3), rn(3)
-    complex(c_double_complex), value :: zge, zcs, zcp
+    complex(c_double_complex) :: zm(3)
     real(c_double), value :: c1, c2, c3, c4, pi, fr
     complex(c_double_complex) :: zu(3,3), zt(3,3)
     real(8) :: delta(3,3)
     call identity3(delta)
-    call SOLFUNDIF(zu, zt, xp(1), xp(2), xp(3), xg(1), xg(2), xg(3), rn, zge, zcs, zcp, &
+    call SOLFUNDIF(zu, zt, xp(1), xp(2), xp(3), xg(1), xg(2), xg(3), rn, zm(1), zm(2), zm(3), &
                    c1, c2, c3, c4, delta, pi, fr)
   end subroutine
 
@@ -162,18 +162,21 @@ contains
                   gi, ome, npg)
   end subroutine
 
-  subroutine ref_sigmaec(co, xp, rn, pi, fr, zge, rho, zcs, zcp, npg, gi, ome, zd, zs) &
+  ! The material (zge, zcs, zcp) comes by reference: with more than eight
+  ! floating-point arguments a by-value complex would straddle registers and
+  ! stack, which C and Fortran compilers need not lay out alike.
+  subroutine ref_sigmaec(co, xp, rn, pi, fr, zm, rho, npg, gi, ome, zd, zs) &
       bind(C, name='ref_sigmaec')
     real(c_double) :: co(4,3), xp(3), rn(3)
     real(c_double), value :: pi, fr, rho
-    complex(c_double_complex), value :: zge, zcs, zcp
+    complex(c_double_complex) :: zm(3)
     integer(c_int), value :: npg
     real(c_double) :: gi(npg), ome(npg)
     complex(c_double_complex) :: zd(3,3,3), zs(3,3,3)
     real(8) :: delta(3,3)
     call set_units()
     call identity3(delta)
-    call SIGMAEC(co, xp(1), xp(2), xp(3), rn, delta, pi, fr, zge, rho, zcs, zcp, npg, &
+    call SIGMAEC(co, xp(1), xp(2), xp(3), rn, delta, pi, fr, zm(1), rho, zm(2), zm(3), npg, &
                  zd, zs, gi, ome)
   end subroutine
 
