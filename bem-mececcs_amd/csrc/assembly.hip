@@ -1,0 +1,748 @@
+// assembly.hip -- H/G influence-matrix assembly kernels for MI355X (gfx950).
+//
+// FP64 throughout.  Kernels and the reference routines whose results they
+// reproduce:
+//   k_element_setup  NORMVEC (Normvec.for:4-39), centroids (Inputece.for:100-102)
+//                    and the per-element Gauss geometry of Nonsingd.for:108-176
+//   k_static_hdiag   GHMATECE off-diagonal loop + RIGID_BODY
+//                    (Ghmatece.for:92-150, 258-282): the static H row sums,
+//                    without ever storing the static H matrix
+//   k_gest_diag      GEST_SINGULAR / SINGGE / NONSINGE_G (Ghmatece.for:288-323,
+//                    Singge.for:11-202)
+//   k_dyn_diag       SING_DE / SOLFUNDIF + HESTdiag/GESTdiag (Ghmatecd.for:139-153,
+//                    Sing_de.for, Solfundif.for)
+//   k_dyn_offdiag    NONSINGD / SOLFUND (Ghmatecd.for:155-162, Nonsingd.for,
+//                    Solfund.for), the KODE column swap (Ghmatecd.for:168,
+//                    316-333) and the fused RHS ZFI = ZG ZDFI (Ghmatecd.for:172)
+//   k_interior       INTEREC displacements and SIGMAEC stresses
+//                    (Interec.for:86-135, 188-283, Sigmaec.for)
+#include "bemx_common.hpp"
+
+namespace bemx {
+
+// ---------------------------------------------------------------------------
+// Element setup: normal, centroid, and the NPG x NPG Gauss points and weights
+// of every element, stored twice: element-major gp[j*npg2+q] (wave-uniform
+// reads in the assembly) and point-major gpT[q*n+j] (coalesced per-lane reads
+// in the interior kernel).  use_given != 0 keeps the caller's normals and
+// centroids (the reference driver computes them once, Main.for:120).
+// ---------------------------------------------------------------------------
+__global__ void k_element_setup(int n, const int* __restrict__ cone, const double* __restrict__ cx,
+                                const double* __restrict__ cy, const double* __restrict__ cz,
+                                const double* __restrict__ gi, const double* __restrict__ ome,
+                                int npg, int use_given, ElemData* __restrict__ elem,
+                                GaussPt* __restrict__ gp, GaussPt* __restrict__ gpT,
+                                int* __restrict__ status) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double co[4][3];
+    for (int k = 0; k < 4; ++k) {
+        int p = cone[k * n + j] - 1;
+        co[k][0] = cx[p];
+        co[k][1] = cy[p];
+        co[k][2] = cz[p];
+    }
+    if (!use_given) {
+        double a = (co[1][1] - co[0][1]) * (co[2][2] - co[0][2]) -
+                   (co[1][2] - co[0][2]) * (co[2][1] - co[0][1]);
+        double b = (co[1][2] - co[0][2]) * (co[2][0] - co[0][0]) -
+                   (co[1][0] - co[0][0]) * (co[2][2] - co[0][2]);
+        double c = (co[1][0] - co[0][0]) * (co[2][1] - co[0][1]) -
+                   (co[1][1] - co[0][1]) * (co[2][0] - co[0][0]);
+        double r = sqrt(a * a + b * b + c * c);
+        ElemData e;
+        e.nx = a / r;
+        e.ny = b / r;
+        e.nz = c / r;
+        e.cx = (co[0][0] + co[1][0] + co[2][0] + co[3][0]) * 0.25;
+        e.cy = (co[0][1] + co[1][1] + co[2][1] + co[3][1]) * 0.25;
+        e.cz = (co[0][2] + co[1][2] + co[2][2] + co[3][2]) * 0.25;
+        e.pad0 = e.pad1 = 0.0;
+        elem[j] = e;
+    }
+    int npg2 = npg * npg;
+    int bad = 0;
+    for (int jg = 0; jg < npg; ++jg) {
+        double eta = gi[jg], sp = 1.0 + eta, sm = 1.0 - eta;
+        for (int ig = 0; ig < npg; ++ig) {
+            double xi = gi[ig], rp = 1.0 + xi, rm = 1.0 - xi;
+            double dxi[4] = {-0.25 * sm, 0.25 * sm, 0.25 * sp, -0.25 * sp};
+            double det_[4] = {-0.25 * rm, -0.25 * rp, 0.25 * rp, 0.25 * rm};
+            double f[4] = {0.25 * rm * sm, 0.25 * rp * sm, 0.25 * rp * sp, 0.25 * rm * sp};
+            double j1[3], j2[3], xg[3];
+            for (int d = 0; d < 3; ++d) {
+                double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+                for (int k = 0; k < 4; ++k) {
+                    s1 += dxi[k] * co[k][d];
+                    s2 += det_[k] * co[k][d];
+                    s3 += co[k][d] * f[k];
+                }
+                j1[d] = s1;
+                j2[d] = s2;
+                xg[d] = s3;
+            }
+            double a = j1[1] * j2[2] - j2[1] * j1[2];
+            double b = j2[0] * j1[2] - j1[0] * j2[2];
+            double c = j1[0] * j2[1] - j2[0] * j1[1];
+            double det = sqrt(a * a + b * b + c * c);
+            if (det < 1.0e-5) bad = 1;
+            GaussPt g{xg[0], xg[1], xg[2], ome[ig] * ome[jg] * det};
+            int q = jg * npg + ig;
+            gp[(size_t)j * npg2 + q] = g;
+            gpT[(size_t)q * n + j] = g;
+        }
+    }
+    if (bad) atomicOr(status, ST_DEGENERATE);
+}
+
+// ---------------------------------------------------------------------------
+// Static H rigid-body diagonal.  For collocation row I (one per lane) and the
+// elements J of this split: T^st summed over J != I and its Gauss points, in
+// the factored form  T_ij = C3 [C4 d_ij S0 + 3 S2_ij + C4 (A_ij - A_ji)]  with
+// S0 = sum w drn/R^2, S2 = sum w drn r_i r_j/R^2 and A_ij = sum_J n_i(J) S1_j(J),
+// S1 = sum w r_j/R^2  (Solfune.for:139-152).  part[s][I][9] gets this split's
+// sum; k_static_hdiag_reduce negates and sums the splits in a fixed order.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_static_hdiag(int n, int nbe, int npg2, int jper,
+                                                     const ElemData* __restrict__ elem,
+                                                     const GaussPt* __restrict__ gp, double c3,
+                                                     double c4, double* __restrict__ part) {
+    int I = blockIdx.x * 64 + threadIdx.x;
+    int Ic = I < nbe ? I : nbe - 1;
+    int j0 = blockIdx.y * jper;
+    int j1 = min(n, j0 + jper);
+    double xc = elem[Ic].cx, yc = elem[Ic].cy, zc = elem[Ic].cz;
+    double s0 = 0, s2[6] = {0, 0, 0, 0, 0, 0}, A[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int J = j0; J < j1; ++J) {
+        double nx = elem[J].nx, ny = elem[J].ny, nz = elem[J].nz;
+        const GaussPt* g = gp + (size_t)J * npg2;
+        double t0 = 0, t2[6] = {0, 0, 0, 0, 0, 0}, t1[3] = {0, 0, 0};
+        for (int q = 0; q < npg2; ++q) {
+            GaussPt p = g[q];
+            double r1 = p.x - xc, r2 = p.y - yc, r3 = p.z - zc;
+            double R = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
+            double iR = 1.0 / R;
+            double d1 = r1 * iR, d2 = r2 * iR, d3 = r3 * iR;
+            double wr = p.w * iR * iR;
+            double drn = d1 * nx + d2 * ny + d3 * nz;
+            double wd = wr * drn;
+            t0 += wd;
+            t2[0] += wd * d1 * d1;
+            t2[1] += wd * d2 * d1;
+            t2[2] += wd * d3 * d1;
+            t2[3] += wd * d2 * d2;
+            t2[4] += wd * d3 * d2;
+            t2[5] += wd * d3 * d3;
+            t1[0] += wr * d1;
+            t1[1] += wr * d2;
+            t1[2] += wr * d3;
+        }
+        if (J != I) {
+            s0 += t0;
+            for (int k = 0; k < 6; ++k) s2[k] += t2[k];
+            double nn[3] = {nx, ny, nz};
+            for (int b = 0; b < 3; ++b)
+                for (int a = 0; a < 3; ++a) A[a + 3 * b] += nn[a] * t1[b];
+        }
+    }
+    if (I >= nbe) return;
+    const int sidx[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};  // (i,j) column-major -> packed sym
+    double* out = part + ((size_t)blockIdx.y * nbe + I) * 9;
+    for (int b = 0; b < 3; ++b)
+        for (int a = 0; a < 3; ++a) {
+            double v = 3.0 * s2[sidx[a + 3 * b]] + c4 * (A[a + 3 * b] - A[b + 3 * a]);
+            if (a == b) v += c4 * s0;
+            out[a + 3 * b] = c3 * v;
+        }
+}
+
+__global__ void k_static_hdiag_reduce(int nbe, int nsplit, const double* __restrict__ part,
+                                      double* __restrict__ hdiag) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nbe * 9) return;
+    double s = 0.0;
+    for (int k = 0; k < nsplit; ++k) s += part[(size_t)k * nbe * 9 + t];
+    hdiag[t] = -s;
+}
+
+// ---------------------------------------------------------------------------
+// Static G diagonal: four quads (P_a, P_b, xm, xm) collapsed onto the
+// centroid, NPG x NPG Gauss each, Kelvin U^st = (C1/R)(C2 d_ij + r_i r_j).
+// One wavefront per element; lanes stride over the 4 NPG^2 points.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_gest_diag(int n, int nbe, const int* __restrict__ cone,
+                                                  const double* __restrict__ cx,
+                                                  const double* __restrict__ cy,
+                                                  const double* __restrict__ cz,
+                                                  const ElemData* __restrict__ elem,
+                                                  const double* __restrict__ gi,
+                                                  const double* __restrict__ ome, int npg,
+                                                  double c1, double c2,
+                                                  double* __restrict__ gdiag,
+                                                  int* __restrict__ status) {
+    int I = blockIdx.x;
+    int lane = threadIdx.x;
+    double P[4][3];
+    for (int k = 0; k < 4; ++k) {
+        int p = cone[k * n + I] - 1;
+        P[k][0] = cx[p];
+        P[k][1] = cy[p];
+        P[k][2] = cz[p];
+    }
+    double xm[3] = {elem[I].cx, elem[I].cy, elem[I].cz};
+    int npg2 = npg * npg;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    int bad = 0;
+    for (int t = lane; t < 4 * npg2; t += 64) {
+        int it = t / npg2, q = t - it * npg2;
+        int jg = q / npg, ig = q - jg * npg;
+        const double* pa = P[it];
+        const double* pb = P[(it + 1) & 3];
+        double eta = gi[jg], sp = 1.0 + eta, sm = 1.0 - eta;
+        double xi = gi[ig], rp = 1.0 + xi, rm = 1.0 - xi;
+        double dxi[4] = {-0.25 * sm, 0.25 * sm, 0.25 * sp, -0.25 * sp};
+        double det_[4] = {-0.25 * rm, -0.25 * rp, 0.25 * rp, 0.25 * rm};
+        double f[4] = {0.25 * rm * sm, 0.25 * rp * sm, 0.25 * rp * sp, 0.25 * rm * sp};
+        double j1[3], j2[3], xg[3];
+        for (int d = 0; d < 3; ++d) {
+            double co[4] = {pa[d], pb[d], xm[d], xm[d]};
+            double s1 = 0, s2 = 0, s3 = 0;
+            for (int k = 0; k < 4; ++k) {
+                s1 += dxi[k] * co[k];
+                s2 += det_[k] * co[k];
+                s3 += co[k] * f[k];
+            }
+            j1[d] = s1;
+            j2[d] = s2;
+            xg[d] = s3;
+        }
+        double a = j1[1] * j2[2] - j2[1] * j1[2];
+        double b = j2[0] * j1[2] - j1[0] * j2[2];
+        double c = j1[0] * j2[1] - j2[0] * j1[1];
+        double det = sqrt(a * a + b * b + c * c);
+        if (det < 1.0e-5) bad = 1;
+        double w = ome[ig] * ome[jg] * det;
+        double r1 = xg[0] - xm[0], r2 = xg[1] - xm[1], r3 = xg[2] - xm[2];
+        double R = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
+        double iR = 1.0 / R;
+        double d1 = r1 * iR, d2 = r2 * iR, d3 = r3 * iR;
+        double wc = w * c1 * iR;
+        acc[0] += wc * (c2 + d1 * d1);
+        acc[1] += wc * (d2 * d1);
+        acc[2] += wc * (d3 * d1);
+        acc[3] += wc * (c2 + d2 * d2);
+        acc[4] += wc * (d3 * d2);
+        acc[5] += wc * (c2 + d3 * d3);
+    }
+    for (int k = 0; k < 6; ++k) acc[k] = wave_sum(acc[k]);
+    if (bad) atomicOr(status, ST_DEGENERATE);
+    if (lane == 0) {
+        double* o = gdiag + (size_t)I * 9;
+        o[0] = acc[0]; o[1] = acc[1]; o[2] = acc[2];
+        o[3] = acc[1]; o[4] = acc[3]; o[5] = acc[4];
+        o[6] = acc[2]; o[7] = acc[4]; o[8] = acc[5];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dynamic scalar functions at distance R (Solfund.for:29-53):
+//   chi, and with WithPsi also psi; and A = psi' - chi/R, B = 4 chi/R - 2 chi',
+//   C = (c1-2)(A + B/2 - 3 chi/R) - 2 chi/R.
+// 1/z is formed as (1/k)(1/R).
+// ---------------------------------------------------------------------------
+template <bool WithPsi>
+__device__ inline void dyn_scalars(const DynConst& K, double R, double iR, cd& psi, cd& chi,
+                                   cd& A, cd& B, cd& C) {
+    cd Es = exp_over_r(K.ks, R, iR);
+    cd Ep = exp_over_r(K.kp, R, iR);
+    double iR2 = iR * iR;
+    cd as = K.as * iR, as2 = K.as2 * iR2;
+    cd ap = K.ap * iR, ap2 = K.ap2 * iR2;
+    cd zs = K.ks * R, zp = K.kp * R;
+    // chi = (1 + 3/zs^2 - 3/zs) Es - c2 (1 + 3/zp^2 - 3/zp) Ep
+    cd fs = mk(1.0 + 3.0 * (as2.x - as.x), 3.0 * (as2.y - as.y));
+    cd fp = mk(1.0 + 3.0 * (ap2.x - ap.x), 3.0 * (ap2.y - ap.y));
+    cd Epc = K.c2 * Ep;
+    chi = fs * Es - fp * Epc;
+    if (WithPsi) {
+        cd gs = mk(1.0 + as2.x - as.x, as2.y - as.y);
+        cd gp = ap2 - ap;
+        psi = gs * Es - gp * Epc;
+    }
+    // psi' R = (-2 + zs + 3/zs - 3/zs^2) Es - c2 (-1 + 3/zp - 3/zp^2) Ep
+    cd hs = mk(-2.0 + zs.x + 3.0 * (as.x - as2.x), zs.y + 3.0 * (as.y - as2.y));
+    cd hp = mk(-1.0 + 3.0 * (ap.x - ap2.x), 3.0 * (ap.y - ap2.y));
+    cd psid = (hs * Es - hp * Epc) * iR;
+    // chi' R = (zs - 4 + 9/zs - 9/zs^2) Es - c2 (zp - 4 + 9/zp - 9/zp^2) Ep
+    cd ks_ = mk(zs.x - 4.0 + 9.0 * (as.x - as2.x), zs.y + 9.0 * (as.y - as2.y));
+    cd kp_ = mk(zp.x - 4.0 + 9.0 * (ap.x - ap2.x), zp.y + 9.0 * (ap.y - ap2.y));
+    cd chid = (ks_ * Es - kp_ * Epc) * iR;
+    cd chr = chi * iR;
+    A = psid - chr;
+    B = 4.0 * chr - 2.0 * chid;
+    C = K.c1m2 * (A + 0.5 * B - 3.0 * chr) - 2.0 * chr;
+}
+
+// Writes the 3x3 block of column element J for row element I after the KODE
+// swap (Ghmatecd.for:316-333 as called at :168): for a KODE=0 column c,
+// H'(:,c) = -G(:,c) and G'(:,c) = -mu* H(:,c).  Returns G' . zdfi_J in b.
+__device__ inline void swap_and_rhs(const cd H[9], const cd G[9], int flag, const cd* zdfi3,
+                                    cd zge, cd Hout[9], cd b[3], bool need_rhs) {
+    for (int k = 0; k < 3; ++k) {
+        bool sw = (flag >> (1 + k)) & 1;
+        cd t = need_rhs ? zdfi3[k] : mk(0.0, 0.0);
+        for (int i = 0; i < 3; ++i) {
+            cd h = H[i + 3 * k];
+            cd g = G[i + 3 * k];
+            Hout[i + 3 * k] = sw ? -g : h;
+            cd gn = sw ? -(h * zge) : g;
+            b[i] = b[i] + gn * t;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dynamic diagonal blocks: plain Gauss of (dynamic - static) kernels over the
+// element itself, collocation at its centroid, plus HESTdiag/GESTdiag.
+// One wavefront per element, one lane per Gauss point.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_dyn_diag(int nbe, int npg2, const ElemData* __restrict__ elem,
+                                                 const GaussPt* __restrict__ gp,
+                                                 const double* __restrict__ hdiag,
+                                                 const double* __restrict__ gdiag,
+                                                 const int* __restrict__ colflag,
+                                                 const cd* __restrict__ zdfi, DynConst K,
+                                                 cd* __restrict__ zh, size_t ldzh,
+                                                 cd* __restrict__ rhs_diag) {
+    int I = blockIdx.x;
+    int lane = threadIdx.x;
+    ElemData e = elem[I];
+    double nv[3] = {e.nx, e.ny, e.nz};
+    double accH[18], accG[18];
+    for (int k = 0; k < 18; ++k) accH[k] = accG[k] = 0.0;
+    for (int q = lane; q < npg2; q += 64) {
+        GaussPt p = gp[(size_t)I * npg2 + q];
+        double r[3] = {p.x - e.cx, p.y - e.cy, p.z - e.cz};
+        double R = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        double iR = 1.0 / R;
+        double rd[3] = {r[0] * iR, r[1] * iR, r[2] * iR};
+        double drn = rd[0] * nv[0] + rd[1] * nv[1] + rd[2] * nv[2];
+        cd psi, chi, A, B, C;
+        dyn_scalars<true>(K, R, iR, psi, chi, A, B, C);
+        double w = p.w;
+        double ust = K.c1s * iR, tst = K.c3s * iR * iR;
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) {
+                double dij = (i == j) ? 1.0 : 0.0;
+                // Solfundif.for:373-383
+                cd u = K.c0 * (psi * dij - chi * (rd[j] * rd[i]));
+                u.x -= ust * (K.c2s * dij + rd[i] * rd[j]);
+                cd t = K.q4pi * (A * (drn * dij + rd[j] * nv[i]) + B * (rd[i] * rd[j] * drn) +
+                                 C * (rd[i] * nv[j]));
+                t.x -= tst * (drn * (K.c4s * dij + 3.0 * rd[i] * rd[j]) +
+                              K.c4s * (rd[j] * nv[i] - rd[i] * nv[j]));
+                int k = i + 3 * j;
+                accH[2 * k] += t.x * w;
+                accH[2 * k + 1] += t.y * w;
+                accG[2 * k] += u.x * w;
+                accG[2 * k + 1] += u.y * w;
+            }
+    }
+    for (int k = 0; k < 18; ++k) {
+        accH[k] = wave_sum(accH[k]);
+        accG[k] = wave_sum(accG[k]);
+    }
+    if (lane != 0) return;
+    cd H[9], G[9], Ho[9], b[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+    for (int k = 0; k < 9; ++k) {
+        H[k] = mk(accH[2 * k] + hdiag[(size_t)I * 9 + k], accH[2 * k + 1]);
+        G[k] = mk(accG[2 * k] + gdiag[(size_t)I * 9 + k], accG[2 * k + 1]);
+    }
+    swap_and_rhs(H, G, colflag[I], zdfi + 3 * I, K.zge, Ho, b, true);
+    for (int k = 0; k < 3; ++k)
+        for (int i = 0; i < 3; ++i) zh[(size_t)(3 * I + k) * ldzh + 3 * I + i] = Ho[i + 3 * k];
+    for (int i = 0; i < 3; ++i) rhs_diag[3 * I + i] = b[i];
+}
+
+// ---------------------------------------------------------------------------
+// Off-diagonal dynamic blocks.  A workgroup of WAVES wavefronts covers
+// 64*WAVES collocation rows (one per lane) and a tile of JT column elements
+// whose Gauss points are staged in LDS and read by every lane at the same
+// address (broadcast).  Per element J the lane accumulates H in the factored
+// form  H_ij = d_ij SD + n_i SP_j + SRQ_ij  (SD = sum A' drn, SP = sum A' r_j,
+// SRQ = sum r_i (B' drn r_j + C' n_j), primes = times w/(4 pi)), which is
+// the Gauss sum of Solfund.for:58-59 with n fixed per element.  Only elements
+// flagged needs-G (nonzero traction or a KODE=0 column) also integrate U; G
+// is never stored: it is folded into the RHS partial rhs_part[tile][row].
+// ---------------------------------------------------------------------------
+template <bool NeedG>
+__device__ inline void offdiag_element(const DynConst& K, const GaussPt* __restrict__ sg, int npg2,
+                                       double xc, double yc, double zc, const double nv[3],
+                                       cd H[9], cd G[9]) {
+    cd SD = mk(0, 0), SP[3], SRQ[9], Gpsi = mk(0, 0), Gchi[6];
+    for (int k = 0; k < 3; ++k) SP[k] = mk(0, 0);
+    for (int k = 0; k < 9; ++k) SRQ[k] = mk(0, 0);
+    for (int k = 0; k < 6; ++k) Gchi[k] = mk(0, 0);
+    for (int q = 0; q < npg2; ++q) {
+        GaussPt p = sg[q];
+        double r1 = p.x - xc, r2 = p.y - yc, r3 = p.z - zc;
+        double R = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
+        double iR = 1.0 / R;
+        double rd[3] = {r1 * iR, r2 * iR, r3 * iR};
+        double drn = rd[0] * nv[0] + rd[1] * nv[1] + rd[2] * nv[2];
+        cd psi, chi, A, B, C;
+        dyn_scalars<NeedG>(K, R, iR, psi, chi, A, B, C);
+        double wq = p.w * K.q4pi;
+        cd Ap = A * wq, Bp = B * (wq * drn), Cp = C * wq;
+        SD = SD + Ap * drn;
+        cd Q[3];
+        for (int j = 0; j < 3; ++j) {
+            SP[j] = SP[j] + Ap * rd[j];
+            Q[j] = Bp * rd[j] + Cp * nv[j];
+        }
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) SRQ[i + 3 * j] = SRQ[i + 3 * j] + Q[j] * rd[i];
+        if (NeedG) {
+            Gpsi = Gpsi + psi * p.w;
+            cd cw = chi * p.w;
+            Gchi[0] = Gchi[0] + cw * (rd[0] * rd[0]);
+            Gchi[1] = Gchi[1] + cw * (rd[1] * rd[0]);
+            Gchi[2] = Gchi[2] + cw * (rd[2] * rd[0]);
+            Gchi[3] = Gchi[3] + cw * (rd[1] * rd[1]);
+            Gchi[4] = Gchi[4] + cw * (rd[2] * rd[1]);
+            Gchi[5] = Gchi[5] + cw * (rd[2] * rd[2]);
+        }
+    }
+    const int sidx[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) {
+            cd h = SRQ[i + 3 * j] + SP[j] * nv[i];
+            if (i == j) h = h + SD;
+            H[i + 3 * j] = h;
+            if (NeedG) {
+                cd g = -Gchi[sidx[i + 3 * j]];
+                if (i == j) g = g + Gpsi;
+                G[i + 3 * j] = K.c0 * g;
+            }
+        }
+}
+
+template <int WAVES, int JT>
+__global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag(int nbe, int npg2,
+                                                            const ElemData* __restrict__ elem,
+                                                            const GaussPt* __restrict__ gp,
+                                                            const int* __restrict__ colflag,
+                                                            const cd* __restrict__ zdfi,
+                                                            DynConst K, cd* __restrict__ zh,
+                                                            size_t ldzh,
+                                                            cd* __restrict__ rhs_part) {
+    extern __shared__ GaussPt sgp[];
+    const int j0 = blockIdx.y * JT;
+    const int jn = min(JT, nbe - j0);
+    for (int t = threadIdx.x; t < jn * npg2; t += 64 * WAVES) sgp[t] = gp[(size_t)j0 * npg2 + t];
+    __syncthreads();
+    const int I = blockIdx.x * (64 * WAVES) + threadIdx.x;
+    const bool rowok = I < nbe;
+    const int Ic = rowok ? I : nbe - 1;
+    const double xc = elem[Ic].cx, yc = elem[Ic].cy, zc = elem[Ic].cz;
+    cd b[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+    for (int jj = 0; jj < jn; ++jj) {
+        const int J = j0 + jj;
+        const int flag = colflag[J];
+        const double nv[3] = {elem[J].nx, elem[J].ny, elem[J].nz};
+        cd H[9], G[9], Ho[9];
+        const bool needg = flag & 1;
+        if (needg)
+            offdiag_element<true>(K, sgp + jj * npg2, npg2, xc, yc, zc, nv, H, G);
+        else
+            offdiag_element<false>(K, sgp + jj * npg2, npg2, xc, yc, zc, nv, H, G);
+        const bool own = rowok && (I != J);
+        if (needg) {
+            cd bb[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+            swap_and_rhs(H, G, flag, zdfi + 3 * J, K.zge, Ho, bb, true);
+            if (own)
+                for (int i = 0; i < 3; ++i) b[i] = b[i] + bb[i];
+        } else {
+            for (int k = 0; k < 9; ++k) Ho[k] = H[k];
+        }
+        if (own) {
+            for (int k = 0; k < 3; ++k) {
+                cd* col = zh + (size_t)(3 * J + k) * ldzh + 3 * I;
+                col[0] = Ho[3 * k];
+                col[1] = Ho[3 * k + 1];
+                col[2] = Ho[3 * k + 2];
+            }
+        }
+    }
+    if (rowok) {
+        cd* o = rhs_part + (size_t)blockIdx.y * 3 * nbe + 3 * I;
+        o[0] = b[0];
+        o[1] = b[1];
+        o[2] = b[2];
+    }
+}
+
+// ZFI = diagonal part + sum of tile partials, tiles summed in order.
+__global__ void k_rhs_reduce(int nn, int ntiles, const cd* __restrict__ part,
+                             const cd* __restrict__ diag, cd* __restrict__ zfi) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nn) return;
+    cd s = diag[r];
+    for (int t = 0; t < ntiles; ++t) s = s + part[(size_t)t * nn + r];
+    zfi[r] = s;
+}
+
+// Column flags from KODE and ZDFI: bit 0 = element needs G (nonzero traction
+// or a KODE=0 column), bit 1+k = column 3J+k has KODE = 0.
+__global__ void k_colflags(int nbe, const int* __restrict__ kode, const cd* __restrict__ zdfi,
+                           int* __restrict__ flag) {
+    int J = blockIdx.x * blockDim.x + threadIdx.x;
+    if (J >= nbe) return;
+    int f = 0;
+    for (int k = 0; k < 3; ++k) {
+        cd t = zdfi[3 * J + k];
+        bool sw = kode[3 * J + k] == 0;
+        if (sw) f |= (1 << (1 + k)) | 1;
+        if (t.x != 0.0 || t.y != 0.0) f |= 1;
+    }
+    flag[J] = f;
+}
+
+// INTEREC's un-swap (Interec.for:59-67): for KODE = 0, tmp = ZFI mu*,
+// ZFI = ZDFI, ZDFI = tmp.
+__global__ void k_unswap(int nn, const int* __restrict__ kode, cd* __restrict__ zfi,
+                         cd* __restrict__ zdfi, cd zge) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn || kode[i] != 0) return;
+    cd t = zfi[i] * zge;
+    zfi[i] = zdfi[i];
+    zdfi[i] = t;
+}
+
+// ---------------------------------------------------------------------------
+// Interior points.  One wavefront per (point k, block of 64 elements), one
+// lane per element J.  Per Gauss point the displacement kernels (Solfund) and
+// the stress kernels (Sigmaec.for:139-198) are contracted at once with the
+// element's traction t = ZDFI_J and displacement u = ZFI_J:
+//   du_i     += w [ (G t)_i - (T u)_i ]
+//   sigma_ab += w [ sum_m t_m ZD(m,a,b) - u_m ZS(m,a,b) ]   (symmetric in a,b)
+// and the wave reduces its 64 elements in a fixed order into part[k][blk].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_interior(int n, int nbe, int npg2, const ElemData* __restrict__ elem,
+                                                 const GaussPt* __restrict__ gpT,
+                                                 const double* __restrict__ cxi,
+                                                 const double* __restrict__ cyi,
+                                                 const double* __restrict__ czi,
+                                                 const cd* __restrict__ zfi,
+                                                 const cd* __restrict__ zdfi, DynConst K,
+                                                 cd* __restrict__ part) {
+    const int k = blockIdx.y;
+    const int J = blockIdx.x * 64 + threadIdx.x;
+    const bool ok = J < nbe;
+    const int Jc = ok ? J : nbe - 1;
+    const double xk = cxi[k], yk = cyi[k], zk = czi[k];
+    const double nv[3] = {elem[Jc].nx, elem[Jc].ny, elem[Jc].nz};
+    cd t[3], u[3];
+    for (int m = 0; m < 3; ++m) {
+        t[m] = ok ? zdfi[3 * J + m] : mk(0, 0);
+        u[m] = ok ? zfi[3 * J + m] : mk(0, 0);
+    }
+    const cd nu = u[0] * nv[0] + u[1] * nv[1] + u[2] * nv[2];
+    cd du[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+    cd sg[6];
+    for (int e = 0; e < 6; ++e) sg[e] = mk(0, 0);
+    const cd mu4 = K.zge * K.q4pi;
+    for (int q = 0; q < npg2; ++q) {
+        GaussPt p = gpT[(size_t)q * n + Jc];
+        double r1 = p.x - xk, r2 = p.y - yk, r3 = p.z - zk;
+        double R = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
+        double iR = 1.0 / R, iR2 = iR * iR;
+        double rd[3] = {r1 * iR, r2 * iR, r3 * iR};
+        double drn = rd[0] * nv[0] + rd[1] * nv[1] + rd[2] * nv[2];
+        // psi, chi and their first derivatives, as in dyn_scalars
+        cd Es = exp_over_r(K.ks, R, iR);
+        cd Ep = exp_over_r(K.kp, R, iR);
+        cd as = K.as * iR, as2 = K.as2 * iR2, ap = K.ap * iR, ap2 = K.ap2 * iR2;
+        cd zs = K.ks * R, zp = K.kp * R;
+        cd Epc = K.c2 * Ep;
+        cd psi = mk(1.0 + as2.x - as.x, as2.y - as.y) * Es - (ap2 - ap) * Epc;
+        cd chi = mk(1.0 + 3.0 * (as2.x - as.x), 3.0 * (as2.y - as.y)) * Es -
+                 mk(1.0 + 3.0 * (ap2.x - ap.x), 3.0 * (ap2.y - ap.y)) * Epc;
+        cd psid = (mk(-2.0 + zs.x + 3.0 * (as.x - as2.x), zs.y + 3.0 * (as.y - as2.y)) * Es -
+                   mk(-1.0 + 3.0 * (ap.x - ap2.x), 3.0 * (ap.y - ap2.y)) * Epc) * iR;
+        cd chid = (mk(zs.x - 4.0 + 9.0 * (as.x - as2.x), zs.y + 9.0 * (as.y - as2.y)) * Es -
+                   mk(zp.x - 4.0 + 9.0 * (ap.x - ap2.x), zp.y + 9.0 * (ap.y - ap2.y)) * Epc) * iR;
+        cd chr = chi * iR;
+        cd A = psid - chr;
+        cd B = 4.0 * chr - 2.0 * chid;
+        cd C = K.c1m2 * (A + 0.5 * B - 3.0 * chr) - 2.0 * chr;
+        // second derivatives (Sigmaec.for:146-159); q1 = -zp, q2 = -zs,
+        // exp(-q) / (q^2 R^3) = (exp(z)/R) (1/z^2) / R^2
+        cd q1 = -zp, q2 = -zs;
+        cd q1s = q1 * q1, q2s = q2 * q2;
+        double iR4 = iR2 * iR2;
+        cd e1 = Epc * K.ap2 * iR4;  // c2 exp(zp) / (zp^2 R^3) . R^0
+        cd e2 = Es * K.as2 * iR4;
+        cd p1a = mk(12.0 + 12.0 * q1.x, 12.0 * q1.y) + 5.0 * q1s + q1s * q1;
+        cd p2a = mk(12.0 + 12.0 * q2.x, 12.0 * q2.y) + 7.0 * q2s + 3.0 * (q2s * q2) + q2s * q2s;
+        cd p1b = mk(36.0 + 36.0 * q1.x, 36.0 * q1.y) + 17.0 * q1s + 5.0 * (q1s * q1) + q1s * q1s;
+        cd p2b = mk(36.0 + 36.0 * q2.x, 36.0 * q2.y) + 17.0 * q2s + 5.0 * (q2s * q2) + q2s * q2s;
+        cd fhidrr = e2 * p2a - e1 * p1a;
+        cd cappadrr = e2 * p2b - e1 * p1b;
+        // Sigmaec.for:161-176
+        cd cr = chi * iR, cr2 = chi * iR2, cdr = chid * iR, fdr = psid * iR;
+        cd cte1 = 2.0 * cr;
+        cd cte2 = K.lm * (psid - chid - cte1);
+        cd cte3 = psid - cr;
+        cd cte4 = 2.0 * (chid - cte1);
+        cd cte5 = 4.0 * (cappadrr - 5.0 * cdr + 8.0 * cr2);
+        cd cte6 = fhidrr - fdr - 3.0 * cdr + 6.0 * cr2;
+        cd cte7 = 2.0 * cdr - 4.0 * cr2;
+        cd cte8 = K.lm * (cappadrr + cdr - 4.0 * cr2 - fhidrr + fdr);
+        cd cte9 = 2.0 * (cte7 + cte8);
+        cd cte10 = 4.0 * cr2 + 4.0 * (K.lm * (cdr + 2.0 * cr2 - fdr));
+        cd cte11 = (K.lm * K.lm) * (cappadrr + 4.0 * cdr + 2.0 * cr2 - fhidrr - 2.0 * fdr);
+        cd cte12 = 2.0 * (fdr - cr2);
+        // contractions
+        cd rt = t[0] * rd[0] + t[1] * rd[1] + t[2] * rd[2];
+        cd ru = u[0] * rd[0] + u[1] * rd[1] + u[2] * rd[2];
+        double w = p.w;
+        cd c0w = K.c0 * w;
+        cd Tw = A * (K.q4pi * w);
+        cd Bw = B * (K.q4pi * w * drn);
+        cd Cw = C * (K.q4pi * w);
+        for (int i = 0; i < 3; ++i) {
+            cd gt = c0w * (psi * t[i] - chi * (rd[i] * rt));
+            cd tu = Tw * (u[i] * drn + ru * nv[i]) + Bw * (ru * rd[i]) + Cw * (nu * rd[i]);
+            du[i] = du[i] + gt - tu;
+        }
+        cd dA = (cte1 - cte2) * rt;  // coefficient of d_ab
+        cd sA = (cte9 * ru) * drn + (cte10 + cte11) * nu;
+        int e = 0;
+        for (int a = 0; a < 3; ++a)
+            for (int b = a; b < 3; ++b, ++e) {
+                double rab = rd[a] * rd[b];
+                cd zd = -cte3 * (t[b] * rd[a] + t[a] * rd[b]) + cte4 * (rt * rab);
+                cd zsv = (cte5 * (ru * rab) - cte6 * (u[b] * rd[a] + u[a] * rd[b])) * drn +
+                         cte9 * (nu * rab) - cte6 * (ru * (rd[a] * nv[b] + rd[b] * nv[a])) -
+                         cte12 * (u[a] * nv[b] + u[b] * nv[a]);
+                if (a == b) {
+                    zd = zd + dA;
+                    zsv = zsv + sA;
+                }
+                sg[e] = sg[e] + (zd * K.q4pi - zsv * mu4) * w;
+            }
+    }
+    double v[18];
+    for (int i = 0; i < 3; ++i) {
+        v[2 * i] = ok ? du[i].x : 0.0;
+        v[2 * i + 1] = ok ? du[i].y : 0.0;
+    }
+    for (int e = 0; e < 6; ++e) {
+        v[6 + 2 * e] = ok ? sg[e].x : 0.0;
+        v[7 + 2 * e] = ok ? sg[e].y : 0.0;
+    }
+    for (int i = 0; i < 18; ++i) v[i] = wave_sum(v[i]);
+    if (threadIdx.x == 0) {
+        cd* o = part + ((size_t)k * gridDim.x + blockIdx.x) * 9;
+        for (int i = 0; i < 9; ++i) o[i] = mk(v[2 * i], v[2 * i + 1]);
+    }
+}
+
+// Sum the element blocks of each interior point in order; expand the
+// symmetric stress to the 9 entries in the order (11,21,31,12,22,32,13,23,33)
+// of Interec.for:227-280.
+__global__ void k_interior_reduce(int l, int nblk, const cd* __restrict__ part,
+                                  cd* __restrict__ zdsol, cd* __restrict__ zssol) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= l) return;
+    cd s[9];
+    for (int i = 0; i < 9; ++i) s[i] = mk(0, 0);
+    for (int b = 0; b < nblk; ++b)
+        for (int i = 0; i < 9; ++i) s[i] = s[i] + part[((size_t)k * nblk + b) * 9 + i];
+    for (int i = 0; i < 3; ++i) zdsol[3 * k + i] = s[i];
+    // packed (00,01,02,11,12,22) at s[3..8]
+    const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+    for (int e = 0; e < 9; ++e) zssol[9 * k + e] = s[3 + map[e]];
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (called from bemx_api.hip).
+// ---------------------------------------------------------------------------
+hipError_t launch_element_setup(hipStream_t st, int n, const int* cone, const double* cx,
+                                const double* cy, const double* cz, const double* gi,
+                                const double* ome, int npg, int use_given, ElemData* elem,
+                                GaussPt* gp, GaussPt* gpT, int* status) {
+    hipLaunchKernelGGL(k_element_setup, dim3((n + 127) / 128), dim3(128), 0, st, n, cone, cx, cy,
+                       cz, gi, ome, npg, use_given, elem, gp, gpT, status);
+    return hipGetLastError();
+}
+
+hipError_t launch_static_hdiag(hipStream_t st, int n, int nbe, int npg2, const ElemData* elem,
+                               const GaussPt* gp, double c3, double c4, double* part, int nsplit,
+                               double* hdiag) {
+    int jper = (n + nsplit - 1) / nsplit;
+    hipLaunchKernelGGL(k_static_hdiag, dim3((nbe + 63) / 64, nsplit), dim3(64), 0, st, n, nbe,
+                       npg2, jper, elem, gp, c3, c4, part);
+    hipLaunchKernelGGL(k_static_hdiag_reduce, dim3((nbe * 9 + 255) / 256), dim3(256), 0, st, nbe,
+                       nsplit, part, hdiag);
+    return hipGetLastError();
+}
+
+hipError_t launch_gest_diag(hipStream_t st, int n, int nbe, const int* cone, const double* cx,
+                            const double* cy, const double* cz, const ElemData* elem,
+                            const double* gi, const double* ome, int npg, double c1, double c2,
+                            double* gdiag, int* status) {
+    hipLaunchKernelGGL(k_gest_diag, dim3(nbe), dim3(64), 0, st, n, nbe, cone, cx, cy, cz, elem,
+                       gi, ome, npg, c1, c2, gdiag, status);
+    return hipGetLastError();
+}
+
+constexpr int OD_WAVES = 4;
+constexpr int OD_JT = 8;
+
+int offdiag_tiles(int nbe) { return (nbe + OD_JT - 1) / OD_JT; }
+
+hipError_t launch_dyn_assembly(hipStream_t st, int nbe, int npg2, const ElemData* elem,
+                               const GaussPt* gp, const double* hdiag, const double* gdiag,
+                               const int* kode, const cd* zdfi, int* colflag, const DynConst& K,
+                               cd* zh, size_t ldzh, cd* rhs_part, cd* rhs_diag, cd* zfi,
+                               hipEvent_t ev_od0, hipEvent_t ev_od1) {
+    hipLaunchKernelGGL(k_colflags, dim3((nbe + 255) / 256), dim3(256), 0, st, nbe, kode, zdfi,
+                       colflag);
+    hipLaunchKernelGGL(k_dyn_diag, dim3(nbe), dim3(64), 0, st, nbe, npg2, elem, gp, hdiag, gdiag,
+                       colflag, zdfi, K, zh, ldzh, rhs_diag);
+    int tiles = offdiag_tiles(nbe);
+    size_t lds = (size_t)OD_JT * npg2 * sizeof(GaussPt);
+    if (ev_od0) hipEventRecord(ev_od0, st);
+    hipLaunchKernelGGL((k_dyn_offdiag<OD_WAVES, OD_JT>),
+                       dim3((nbe + 64 * OD_WAVES - 1) / (64 * OD_WAVES), tiles),
+                       dim3(64 * OD_WAVES), lds, st, nbe, npg2, elem, gp, colflag, zdfi, K, zh,
+                       ldzh, rhs_part);
+    if (ev_od1) hipEventRecord(ev_od1, st);
+    hipLaunchKernelGGL(k_rhs_reduce, dim3((3 * nbe + 255) / 256), dim3(256), 0, st, 3 * nbe,
+                       tiles, rhs_part, rhs_diag, zfi);
+    return hipGetLastError();
+}
+
+hipError_t launch_unswap(hipStream_t st, int nn, const int* kode, cd* zfi, cd* zdfi, cd zge) {
+    hipLaunchKernelGGL(k_unswap, dim3((nn + 255) / 256), dim3(256), 0, st, nn, kode, zfi, zdfi,
+                       zge);
+    return hipGetLastError();
+}
+
+int interior_blocks(int nbe) { return (nbe + 63) / 64; }
+
+hipError_t launch_interior(hipStream_t st, int n, int nbe, int npg2, int l, const ElemData* elem,
+                           const GaussPt* gpT, const double* cxi, const double* cyi,
+                           const double* czi, const cd* zfi, const cd* zdfi, const DynConst& K,
+                           cd* part, cd* zdsol, cd* zssol) {
+    int nblk = interior_blocks(nbe);
+    hipLaunchKernelGGL(k_interior, dim3(nblk, l), dim3(64), 0, st, n, nbe, npg2, elem, gpT, cxi,
+                       cyi, czi, zfi, zdfi, K, part);
+    hipLaunchKernelGGL(k_interior_reduce, dim3((l + 63) / 64), dim3(64), 0, st, l, nblk, part,
+                       zdsol, zssol);
+    return hipGetLastError();
+}
+
+}  // namespace bemx

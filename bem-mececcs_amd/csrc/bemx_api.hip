@@ -1,0 +1,555 @@
+// bemx_api.hip -- host side of libbemx.so: the device context, the C ABI of
+// include/bemx.h and the orchestration of the assembly, solve and interior
+// kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bemx.h"
+#include "bemx_common.hpp"
+#include "bemx_lu.hpp"
+
+using namespace bemx;
+
+namespace bemx {
+hipError_t launch_element_setup(hipStream_t, int, const int*, const double*, const double*,
+                                const double*, const double*, const double*, int, int,
+                                ElemData*, GaussPt*, GaussPt*, int*);
+hipError_t launch_static_hdiag(hipStream_t, int, int, int, const ElemData*, const GaussPt*,
+                               double, double, double*, int, double*);
+hipError_t launch_gest_diag(hipStream_t, int, int, const int*, const double*, const double*,
+                            const double*, const ElemData*, const double*, const double*, int,
+                            double, double, double*, int*);
+hipError_t launch_dyn_assembly(hipStream_t, int, int, const ElemData*, const GaussPt*,
+                               const double*, const double*, const int*, const cd*, int*,
+                               const DynConst&, cd*, size_t, cd*, cd*, cd*, hipEvent_t,
+                               hipEvent_t);
+int offdiag_tiles(int nbe);
+hipError_t launch_unswap(hipStream_t, int, const int*, cd*, cd*, cd);
+int interior_blocks(int nbe);
+hipError_t launch_interior(hipStream_t, int, int, int, int, const ElemData*, const GaussPt*,
+                           const double*, const double*, const double*, const cd*, const cd*,
+                           const DynConst&, cd*, cd*, cd*);
+}  // namespace bemx
+
+// A grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 256);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+struct bemx_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    std::string err;
+    // mesh
+    int n = 0, nbe = 0, np = 0, npg = 0;
+    bool mesh_ok = false;
+    DevBuf cone, cx, cy, cz, gi, ome, elem, gp, gpT, status;
+    // static pass
+    bool static_ok = false;
+    double consts[5] = {0, 0, 0, 0, 0};  // pi, c1..c4
+    DevBuf hdiag, gdiag, hpart;
+    // dynamic data
+    bool dyn_ok = false;
+    DevBuf kode, zdfi, colflag, rhs_part, rhs_diag, zfi, zh;
+    bool zh_valid = false;     // zh holds the current assembled matrix
+    const void* zh_host = nullptr;  // host array the last GHMATECD copied ZH to
+    // interior
+    DevBuf ipts, ipart, zdsol, zssol, ifi, idfi, ikode;
+    // solver
+    bemx::LuWork lu;
+    // ev0..ev1: whole solve; ev2..ev3: off-diagonal assembly kernel;
+    // ev1 of assembly end is ev4
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr;
+    double last[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+namespace {
+
+int fail(bemx_ctx* c, int code, const char* fmt, const char* a = "", long b = 0) {
+    char buf[512];
+    snprintf(buf, sizeof buf, fmt, a, b);
+    c->err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? BEMX_ERR_NOMEM : BEMX_ERR_HIP, \
+                        "%s (line %ld)", hipGetErrorString(e_), (long)__LINE__);       \
+    } while (0)
+
+int check_status(bemx_ctx* c) {
+    int s = 0;
+    HIPCHK(c, hipMemcpyAsync(&s, c->status.p, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (s & ST_DEGENERATE)
+        return fail(c, BEMX_ERR_DEGENERATE, "%sdegenerate element: Jacobian below 1e-5", "", 0);
+    if (s & ST_SYNC_TIMEOUT)
+        return fail(c, BEMX_ERR_TIMEOUT, "%sdevice-side wait timed out", "", 0);
+    return BEMX_OK;
+}
+
+DynConst make_dyn_const(const bemx_ctx* c, cd zge, cd zcs, cd zcp, double fr, double rho) {
+    DynConst K;
+    const double pi = c->consts[0];
+    cd zwi = mk(0.0, fr);
+    K.ks = -(zwi / zcs);
+    K.kp = -(zwi / zcp);
+    K.as = cinv(K.ks);
+    K.ap = cinv(K.kp);
+    K.as2 = K.as * K.as;
+    K.ap2 = K.ap * K.ap;
+    cd r = zcs / zcp;
+    K.c2 = r * r;
+    cd r1 = zcp / zcs;
+    K.c1m2 = r1 * r1 - mk(2.0, 0.0);
+    K.c0 = cinv(mk(4.0 * pi, 0.0) * zge);
+    K.zge = zge;
+    K.lm = ((zcp * zcp) * rho - 2.0 * zge) / zge;
+    K.q4pi = 1.0 / (4.0 * pi);
+    K.c1s = c->consts[1];
+    K.c2s = c->consts[2];
+    K.c3s = c->consts[3];
+    K.c4s = c->consts[4];
+    return K;
+}
+
+cd ld_c(const double* p) { return mk(p[0], p[1]); }
+
+}  // namespace
+
+extern "C" {
+
+int bemx_create(int device, bemx_ctx** out) {
+    if (!out) return BEMX_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return BEMX_ERR_HIP;
+    if (hipSetDevice(device) != hipSuccess) return BEMX_ERR_HIP;
+    bemx_ctx* c = new bemx_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess ||
+        hipEventCreate(&c->ev4) != hipSuccess || c->status.ensure(sizeof(int)) != hipSuccess) {
+        delete c;
+        return BEMX_ERR_HIP;
+    }
+    *out = c;
+    return BEMX_OK;
+}
+
+int bemx_destroy(bemx_ctx* c) {
+    if (!c) return BEMX_ERR_ARG;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->st);
+    DevBuf* bufs[] = {&c->cone, &c->cx, &c->cy, &c->cz, &c->gi, &c->ome, &c->elem,
+                      &c->gp, &c->gpT, &c->status, &c->hdiag, &c->gdiag, &c->hpart,
+                      &c->kode, &c->zdfi, &c->colflag, &c->rhs_part, &c->rhs_diag,
+                      &c->zfi, &c->zh, &c->ipts, &c->ipart, &c->zdsol, &c->zssol,
+                      &c->ifi, &c->idfi, &c->ikode};
+    for (DevBuf* b : bufs) b->release();
+    c->lu.release();
+    hipEvent_t evs[] = {c->ev0, c->ev1, c->ev2, c->ev3, c->ev4};
+    for (hipEvent_t e : evs)
+        if (e) hipEventDestroy(e);
+    hipStreamDestroy(c->st);
+    delete c;
+    return BEMX_OK;
+}
+
+const char* bemx_last_error(const bemx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int bemx_gauleg(int npg, double* x, double* w) {
+    // Newton iteration on P_n from the Chebyshev-like initial guess, stopping
+    // at |dz| <= 3e-14, as Gauleg.for:9-36.
+    if (npg <= 0 || !x || !w) return BEMX_ERR_ARG;
+    int m = (npg + 1) / 2;
+    for (int i = 1; i <= m; ++i) {
+        double z = std::cos(3.141592654 * (i - 0.25) / (npg + 0.5)), z1, pp, p1, p2, p3;
+        do {
+            p1 = 1.0;
+            p2 = 0.0;
+            for (int j = 1; j <= npg; ++j) {
+                p3 = p2;
+                p2 = p1;
+                p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+            }
+            pp = npg * (z * p1 - p2) / (z * z - 1.0);
+            z1 = z;
+            z = z1 - p1 / pp;
+        } while (std::fabs(z - z1) > 3e-14);
+        x[i - 1] = -z;
+        x[npg - i] = z;
+        w[i - 1] = 2.0 / ((1.0 - z * z) * pp * pp);
+        w[npg - i] = w[i - 1];
+    }
+    return BEMX_OK;
+}
+
+int bemx_send_shared_data(bemx_ctx* c, const double* cx, const double* cy, const double* cz,
+                          const double* cxm, const double* cym, const double* czm,
+                          const double* etas, const double* gi, const double* ome,
+                          const int* cone, int np, int npg, int n, int nbe) {
+    if (!c) return BEMX_ERR_ARG;
+    if (!cx || !cy || !cz || !cone || np <= 0 || npg <= 0 || npg > 16 || n <= 0 || nbe <= 0 ||
+        nbe > n)
+        return fail(c, BEMX_ERR_ARG, "%sbad mesh sizes", "", 0);
+    for (long k = 0; k < 4L * n; ++k)
+        if (cone[k] < 1 || cone[k] > np)
+            return fail(c, BEMX_ERR_ARG, "%sconnectivity entry %ld outside 1..NP", "", k);
+    bool given = cxm && cym && czm && etas;
+    HIPCHK(c, hipSetDevice(c->device));
+    c->mesh_ok = c->static_ok = c->dyn_ok = c->zh_valid = false;
+    c->n = n;
+    c->nbe = nbe;
+    c->np = np;
+    c->npg = npg;
+    int npg2 = npg * npg;
+    std::vector<double> g(npg), w(npg);
+    if (gi && ome) {
+        g.assign(gi, gi + npg);
+        w.assign(ome, ome + npg);
+    } else {
+        bemx_gauleg(npg, g.data(), w.data());
+    }
+    HIPCHK(c, c->cone.ensure(sizeof(int) * 4 * n));
+    HIPCHK(c, c->cx.ensure(sizeof(double) * np));
+    HIPCHK(c, c->cy.ensure(sizeof(double) * np));
+    HIPCHK(c, c->cz.ensure(sizeof(double) * np));
+    HIPCHK(c, c->gi.ensure(sizeof(double) * npg));
+    HIPCHK(c, c->ome.ensure(sizeof(double) * npg));
+    HIPCHK(c, c->elem.ensure(sizeof(ElemData) * n));
+    HIPCHK(c, c->gp.ensure(sizeof(GaussPt) * (size_t)n * npg2));
+    HIPCHK(c, c->gpT.ensure(sizeof(GaussPt) * (size_t)n * npg2));
+    HIPCHK(c, hipMemcpyAsync(c->cone.p, cone, sizeof(int) * 4 * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->cx.p, cx, sizeof(double) * np, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->cy.p, cy, sizeof(double) * np, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->cz.p, cz, sizeof(double) * np, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->gi.p, g.data(), sizeof(double) * npg, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->ome.p, w.data(), sizeof(double) * npg, hipMemcpyHostToDevice, c->st));
+    if (given) {
+        std::vector<ElemData> e(n);
+        for (int j = 0; j < n; ++j)
+            e[j] = ElemData{etas[3 * j], etas[3 * j + 1], etas[3 * j + 2], cxm[j], cym[j], czm[j],
+                            0.0, 0.0};
+        HIPCHK(c, hipMemcpyAsync(c->elem.p, e.data(), sizeof(ElemData) * n,
+                                 hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+    }
+    HIPCHK(c, hipMemsetAsync(c->status.p, 0, sizeof(int), c->st));
+    HIPCHK(c, launch_element_setup(c->st, n, c->cone.as<int>(), c->cx.as<double>(),
+                                   c->cy.as<double>(), c->cz.as<double>(), c->gi.as<double>(),
+                                   c->ome.as<double>(), npg, given ? 1 : 0, c->elem.as<ElemData>(),
+                                   c->gp.as<GaussPt>(), c->gpT.as<GaussPt>(), c->status.as<int>()));
+    int s = check_status(c);
+    if (s) return s;
+    c->mesh_ok = true;
+    return BEMX_OK;
+}
+
+int bemx_deallocate_shared_data(bemx_ctx* c) {
+    if (!c) return BEMX_ERR_ARG;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->st);
+    DevBuf* bufs[] = {&c->cone, &c->cx, &c->cy, &c->cz, &c->gi, &c->ome, &c->elem, &c->gp,
+                      &c->gpT, &c->hdiag, &c->gdiag, &c->hpart, &c->zh};
+    for (DevBuf* b : bufs) b->release();
+    c->mesh_ok = c->static_ok = c->zh_valid = false;
+    return BEMX_OK;
+}
+
+int bemx_ghmatece(bemx_ctx* c, double ge, double rnu, double* hestdiag, double* gestdiag,
+                  double* delta, double* pi, double* c1, double* c2, double* c3, double* c4) {
+    if (!c) return BEMX_ERR_ARG;
+    if (!c->mesh_ok) return fail(c, BEMX_ERR_STATE, "%sGHMATECE before the mesh was sent", "", 0);
+    if (!(ge > 0.0) || !(rnu < 0.5))
+        return fail(c, BEMX_ERR_ARG, "%sneeds G > 0 and nu < 1/2", "", 0);
+    HIPCHK(c, hipSetDevice(c->device));
+    // Ghmatece.for:55-63
+    const double PI = 4.0 * std::atan(1.0);
+    c->consts[0] = PI;
+    c->consts[1] = 1.0 / (16.0 * PI * ge * (1.0 - rnu));
+    c->consts[2] = 3.0 - (4.0 * rnu);
+    c->consts[3] = -1.0 / (8.0 * PI * (1.0 - rnu));
+    c->consts[4] = 1.0 - (2.0 * rnu);
+    const int nbe = c->nbe, n = c->n, npg2 = c->npg * c->npg;
+    // splits of the element range so that the grid holds >= 2048 waves
+    int rows = (nbe + 63) / 64;
+    int nsplit = std::max(1, std::min((2048 + rows - 1) / rows, (n + 15) / 16));
+    HIPCHK(c, c->hdiag.ensure(sizeof(double) * 9 * nbe));
+    HIPCHK(c, c->gdiag.ensure(sizeof(double) * 9 * nbe));
+    HIPCHK(c, c->hpart.ensure(sizeof(double) * 9 * (size_t)nbe * nsplit));
+    HIPCHK(c, hipMemsetAsync(c->status.p, 0, sizeof(int), c->st));
+    HIPCHK(c, launch_static_hdiag(c->st, n, nbe, npg2, c->elem.as<ElemData>(),
+                                  c->gp.as<GaussPt>(), c->consts[3], c->consts[4],
+                                  c->hpart.as<double>(), nsplit, c->hdiag.as<double>()));
+    HIPCHK(c, launch_gest_diag(c->st, n, nbe, c->cone.as<int>(), c->cx.as<double>(),
+                               c->cy.as<double>(), c->cz.as<double>(), c->elem.as<ElemData>(),
+                               c->gi.as<double>(), c->ome.as<double>(), c->npg, c->consts[1],
+                               c->consts[2], c->gdiag.as<double>(), c->status.as<int>()));
+    if (hestdiag)
+        HIPCHK(c, hipMemcpyAsync(hestdiag, c->hdiag.p, sizeof(double) * 9 * nbe,
+                                 hipMemcpyDeviceToHost, c->st));
+    if (gestdiag)
+        HIPCHK(c, hipMemcpyAsync(gestdiag, c->gdiag.p, sizeof(double) * 9 * nbe,
+                                 hipMemcpyDeviceToHost, c->st));
+    int s = check_status(c);
+    if (s) return s;
+    if (delta)
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) delta[i + 3 * j] = (i == j) ? 1.0 : 0.0;
+    if (pi) *pi = c->consts[0];
+    if (c1) *c1 = c->consts[1];
+    if (c2) *c2 = c->consts[2];
+    if (c3) *c3 = c->consts[3];
+    if (c4) *c4 = c->consts[4];
+    c->static_ok = true;
+    return BEMX_OK;
+}
+
+int bemx_send_dyn_data(bemx_ctx* c, const double* zdfi, const int* kode) {
+    if (!c) return BEMX_ERR_ARG;
+    if (!c->mesh_ok) return fail(c, BEMX_ERR_STATE, "%sdynamic data before the mesh", "", 0);
+    if (!zdfi || !kode) return fail(c, BEMX_ERR_ARG, "%snull ZDFI/KODE", "", 0);
+    HIPCHK(c, hipSetDevice(c->device));
+    size_t nn = 3 * (size_t)c->nbe;
+    HIPCHK(c, c->zdfi.ensure(sizeof(cd) * nn));
+    HIPCHK(c, c->kode.ensure(sizeof(int) * nn));
+    HIPCHK(c, hipMemcpyAsync(c->zdfi.p, zdfi, sizeof(cd) * nn, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->kode.p, kode, sizeof(int) * nn, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->dyn_ok = true;
+    return BEMX_OK;
+}
+
+// Assembly of ZH and ZFI on the device (no host copies).
+static int assemble_dev(bemx_ctx* c, const DynConst& K) {
+    const int nbe = c->nbe;
+    const size_t nn = 3 * (size_t)nbe;
+    const int tiles = offdiag_tiles(nbe);
+    HIPCHK(c, c->colflag.ensure(sizeof(int) * nbe));
+    HIPCHK(c, c->rhs_part.ensure(sizeof(cd) * nn * tiles));
+    HIPCHK(c, c->rhs_diag.ensure(sizeof(cd) * nn));
+    HIPCHK(c, c->zfi.ensure(sizeof(cd) * nn));
+    HIPCHK(c, c->zh.ensure(sizeof(cd) * nn * nn));
+    c->zh_valid = false;
+    HIPCHK(c, launch_dyn_assembly(c->st, nbe, c->npg * c->npg, c->elem.as<ElemData>(),
+                                  c->gp.as<GaussPt>(), c->hdiag.as<double>(),
+                                  c->gdiag.as<double>(), c->kode.as<int>(), c->zdfi.as<cd>(),
+                                  c->colflag.as<int>(), K, c->zh.as<cd>(), nn,
+                                  c->rhs_part.as<cd>(), c->rhs_diag.as<cd>(), c->zfi.as<cd>(),
+                                  c->ev2, c->ev3));
+    HIPCHK(c, hipEventRecord(c->ev4, c->st));
+    c->zh_valid = true;
+    return BEMX_OK;
+}
+
+int bemx_ghmatecd(bemx_ctx* c, const double* zge, const double* zcs, const double* zcp,
+                  double c1, double c2, double c3, double c4, double fr,
+                  const double* hestdiag, const double* gestdiag, double* zfi, double* zh) {
+    if (!c) return BEMX_ERR_ARG;
+    if (!c->mesh_ok || !c->dyn_ok)
+        return fail(c, BEMX_ERR_STATE, "%sGHMATECD needs the mesh and the dynamic data", "", 0);
+    if (!zge || !zcs || !zcp || !(fr != 0.0))
+        return fail(c, BEMX_ERR_ARG, "%sGHMATECD needs the material and a nonzero frequency", "", 0);
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nbe = c->nbe;
+    const size_t nn = 3 * (size_t)nbe;
+    if (hestdiag || gestdiag) {
+        if (!(hestdiag && gestdiag))
+            return fail(c, BEMX_ERR_ARG, "%spass both HESTdiag and GESTdiag, or neither", "", 0);
+        HIPCHK(c, c->hdiag.ensure(sizeof(double) * 9 * nbe));
+        HIPCHK(c, c->gdiag.ensure(sizeof(double) * 9 * nbe));
+        HIPCHK(c, hipMemcpyAsync(c->hdiag.p, hestdiag, sizeof(double) * 9 * nbe,
+                                 hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipMemcpyAsync(c->gdiag.p, gestdiag, sizeof(double) * 9 * nbe,
+                                 hipMemcpyHostToDevice, c->st));
+        const double PI = 4.0 * std::atan(1.0);
+        c->consts[0] = PI;
+        c->consts[1] = c1;
+        c->consts[2] = c2;
+        c->consts[3] = c3;
+        c->consts[4] = c4;
+    } else if (!c->static_ok) {
+        return fail(c, BEMX_ERR_STATE, "%sno static diagonals: call GHMATECE first", "", 0);
+    } else {
+        c->consts[1] = c1;
+        c->consts[2] = c2;
+        c->consts[3] = c3;
+        c->consts[4] = c4;
+    }
+    DynConst K = make_dyn_const(c, ld_c(zge), ld_c(zcs), ld_c(zcp), fr, 1.0);
+    int s = assemble_dev(c, K);
+    if (s) return s;
+    if (zfi) HIPCHK(c, hipMemcpyAsync(zfi, c->zfi.p, sizeof(cd) * nn, hipMemcpyDeviceToHost, c->st));
+    if (zh) HIPCHK(c, hipMemcpyAsync(zh, c->zh.p, sizeof(cd) * nn * nn, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->zh_host = zh;
+    return BEMX_OK;
+}
+
+int bemx_linsolve(bemx_ctx* c, int nn, double* zh, double* zfi, int mode, long long* info) {
+    if (!c) return BEMX_ERR_ARG;
+    if (info) *info = 0;
+    if (nn <= 0 || !zfi) return fail(c, BEMX_ERR_ARG, "%sbad LINSOLVE arguments", "", 0);
+    if (mode != BEMX_MODE_FP64 && mode != BEMX_MODE_MIXED)
+        return fail(c, BEMX_ERR_ARG, "%sunknown solver mode %ld", "", mode);
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t n = nn;
+    bool resident = c->zh_valid && (size_t)3 * c->nbe == n && (zh == nullptr || zh == c->zh_host);
+    if (!resident) {
+        if (!zh) return fail(c, BEMX_ERR_STATE, "%sno matrix on the device or host", "", 0);
+        HIPCHK(c, c->zh.ensure(sizeof(cd) * n * n));
+        HIPCHK(c, hipMemcpyAsync(c->zh.p, zh, sizeof(cd) * n * n, hipMemcpyHostToDevice, c->st));
+    }
+    HIPCHK(c, c->zfi.ensure(sizeof(cd) * n));
+    HIPCHK(c, hipMemcpyAsync(c->zfi.p, zfi, sizeof(cd) * n, hipMemcpyHostToDevice, c->st));
+    long long inf = 0;
+    double st[4] = {0, 0, 0, 0};
+    int s = bemx::lu_solve(c->lu, c->st, (int)n, c->zh.as<cd>(), n, c->zfi.as<cd>(), 1, mode,
+                           &inf, st, c->err);
+    c->zh_valid = false;  // the factorisation overwrote the matrix
+    double last[8] = {st[0], st[1], st[2], 0, 0, 0, 0, 0};
+    memcpy(c->last, last, sizeof last);
+    if (info) *info = inf;
+    if (s) return s;
+    HIPCHK(c, hipMemcpyAsync(zfi, c->zfi.p, sizeof(cd) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return BEMX_OK;
+}
+
+int bemx_interec(bemx_ctx* c, double* zfi, double* zdfi, const int* kode, int l,
+                 const double* cxi, const double* cyi, const double* czi, const double* zge,
+                 const double* zcs, const double* zcp, double fr, double rho, double* zdsol,
+                 double* zssol) {
+    if (!c) return BEMX_ERR_ARG;
+    if (!c->mesh_ok) return fail(c, BEMX_ERR_STATE, "%sINTEREC before the mesh", "", 0);
+    if (!zfi || !zdfi || !kode || l < 0 || !zge || !zcs || !zcp || !(fr != 0.0))
+        return fail(c, BEMX_ERR_ARG, "%sbad INTEREC arguments", "", 0);
+    const int nbe = c->nbe;
+    const size_t nn = 3 * (size_t)nbe;
+    // un-swap on the host arrays, as the reference does in place
+    cd z = ld_c(zge);
+    for (size_t i = 0; i < nn; ++i) {
+        if (kode[i] != 0) continue;
+        cd f = mk(zfi[2 * i], zfi[2 * i + 1]) * z;
+        zfi[2 * i] = zdfi[2 * i];
+        zfi[2 * i + 1] = zdfi[2 * i + 1];
+        zdfi[2 * i] = f.x;
+        zdfi[2 * i + 1] = f.y;
+    }
+    if (l == 0) return BEMX_OK;  // Interec.for:71
+    if (!cxi || !cyi || !czi || !zdsol || !zssol)
+        return fail(c, BEMX_ERR_ARG, "%snull interior arrays", "", 0);
+    HIPCHK(c, hipSetDevice(c->device));
+    int nblk = interior_blocks(nbe);
+    HIPCHK(c, c->ipts.ensure(sizeof(double) * 3 * l));
+    HIPCHK(c, c->ipart.ensure(sizeof(cd) * 9 * (size_t)l * nblk));
+    HIPCHK(c, c->zdsol.ensure(sizeof(cd) * 3 * l));
+    HIPCHK(c, c->zssol.ensure(sizeof(cd) * 9 * l));
+    HIPCHK(c, c->ifi.ensure(sizeof(cd) * nn));
+    HIPCHK(c, c->idfi.ensure(sizeof(cd) * nn));
+    double* pts = c->ipts.as<double>();
+    HIPCHK(c, hipMemcpyAsync(pts, cxi, sizeof(double) * l, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(pts + l, cyi, sizeof(double) * l, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(pts + 2 * l, czi, sizeof(double) * l, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->ifi.p, zfi, sizeof(cd) * nn, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->idfi.p, zdfi, sizeof(cd) * nn, hipMemcpyHostToDevice, c->st));
+    DynConst K = make_dyn_const(c, ld_c(zge), ld_c(zcs), ld_c(zcp), fr, rho);
+    if (c->consts[0] == 0.0) {  // INTEREC only needs pi from the static pass
+        c->consts[0] = 4.0 * std::atan(1.0);
+        K = make_dyn_const(c, ld_c(zge), ld_c(zcs), ld_c(zcp), fr, rho);
+    }
+    HIPCHK(c, launch_interior(c->st, c->n, nbe, c->npg * c->npg, l, c->elem.as<ElemData>(),
+                              c->gpT.as<GaussPt>(), pts, pts + l, pts + 2 * l, c->ifi.as<cd>(),
+                              c->idfi.as<cd>(), K, c->ipart.as<cd>(), c->zdsol.as<cd>(),
+                              c->zssol.as<cd>()));
+    HIPCHK(c, hipMemcpyAsync(zdsol, c->zdsol.p, sizeof(cd) * 3 * l, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(zssol, c->zssol.p, sizeof(cd) * 9 * l, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return BEMX_OK;
+}
+
+int bemx_solve(bemx_ctx* c, const double* zge, const double* zcs, const double* zcp, double fr,
+               int mode, double* zfi_out, double* stats) {
+    if (!c) return BEMX_ERR_ARG;
+    if (!c->mesh_ok || !c->dyn_ok || !c->static_ok)
+        return fail(c, BEMX_ERR_STATE, "%ssolve needs mesh, static pass and dynamic data", "", 0);
+    if (!zge || !zcs || !zcp || !(fr != 0.0))
+        return fail(c, BEMX_ERR_ARG, "%sbad material or frequency", "", 0);
+    if (mode != BEMX_MODE_FP64 && mode != BEMX_MODE_MIXED)
+        return fail(c, BEMX_ERR_ARG, "%sunknown solver mode %ld", "", mode);
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t nn = 3 * (size_t)c->nbe;
+    DynConst K = make_dyn_const(c, ld_c(zge), ld_c(zcs), ld_c(zcp), fr, 1.0);
+    HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    int s = assemble_dev(c, K);
+    if (s) return s;
+    long long inf = 0;
+    double st[4] = {0, 0, 0, 0};
+    s = bemx::lu_solve(c->lu, c->st, (int)nn, c->zh.as<cd>(), nn, c->zfi.as<cd>(), 1, mode, &inf,
+                       st, c->err);
+    c->zh_valid = false;
+    if (s) return s;
+    HIPCHK(c, hipEventRecord(c->ev1, c->st));
+    if (zfi_out)
+        HIPCHK(c, hipMemcpyAsync(zfi_out, c->zfi.p, sizeof(cd) * nn, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    float ms = 0.f, ms_od = 0.f, ms_as = 0.f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    hipEventElapsedTime(&ms_od, c->ev2, c->ev3);
+    hipEventElapsedTime(&ms_as, c->ev0, c->ev4);
+    double last[8] = {st[0], st[1], st[2], ms, ms_od, ms_as, ms - ms_as, 0.0};
+    memcpy(c->last, last, sizeof last);
+    if (stats) {
+        stats[0] = st[0];
+        stats[1] = st[1];
+        stats[2] = st[2];
+        stats[3] = ms;
+    }
+    return BEMX_OK;
+}
+
+int bemx_last_stats(const bemx_ctx* c, double* out) {
+    if (!c || !out) return BEMX_ERR_ARG;
+    memcpy(out, c->last, sizeof c->last);
+    return BEMX_OK;
+}
+
+int bemx_device_buffers(bemx_ctx* c, void** zh, void** zfi, void** stream) {
+    if (!c) return BEMX_ERR_ARG;
+    if (zh) *zh = c->zh.p;
+    if (zfi) *zfi = c->zfi.p;
+    if (stream) *stream = (void*)c->st;
+    return BEMX_OK;
+}
+
+}  // extern "C"

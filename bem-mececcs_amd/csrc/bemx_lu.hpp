@@ -1,0 +1,31 @@
+// bemx_lu.hpp -- dense complex solve on the device (LINSOLVE).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "bemx_common.hpp"
+
+namespace bemx {
+
+// Device workspace of the factorisation (grow-only).
+struct LuWork {
+    void* buf[16] = {nullptr};
+    size_t cap[16] = {0};
+    void* get(int slot, size_t bytes, hipError_t* e);
+    void release();
+};
+
+// Solves A x = b in place (A overwritten by its factors, b by x).
+// mode: 0 = FP64 partial-pivot LU; 1 = FP32 LU + FP64 GMRES-IR.
+// stats: [GMRES iterations, final ||r||/||b||, 1 if the mixed path fell back
+// to FP64, 0].
+// FP32 LU (split-BF16x3 MFMA update) preconditioning FP64 GMRES-IR; A is
+// read only, b is overwritten with x on success and left intact otherwise.
+int gmres_ir(LuWork& w, hipStream_t st, int n, const cd* A, size_t lda, cd* b, double stats[4],
+             std::string& err);
+
+int lu_solve(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, cd* b, int nrhs, int mode,
+             long long* info, double stats[4], std::string& err);
+
+}  // namespace bemx

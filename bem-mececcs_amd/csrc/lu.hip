@@ -1,0 +1,599 @@
+// lu.hip -- dense complex solve of ZH x = ZFI on MI355X (LINSOLVE).
+//
+// Plain FP64 path (ZGESV semantics, Linsolve.for:68-109): right-looking
+// blocked LU with partial pivoting, panel width NB = 64.
+//   k_panel_fp64   the m x NB panel held in LDS across P co-resident
+//                  workgroups; per column one inter-workgroup exchange of the
+//                  local pivot candidates (largest |re|+|im|, lowest row on
+//                  ties, as LAPACK's IZAMAX), then the row swap and the rank-1
+//                  update in LDS.  Also emits the panel's row permutation.
+//   k_rowperm      applies that permutation to the trailing columns (the
+//                  factored L columns are left in place: the forward solve
+//                  applies each panel's permutation before using its L block).
+//   k_trsm_u12     U12 = L11^-1 A12, one column per thread.
+//   k_zgemm_mfma   A22 -= L21 U12 as four real FP64 MFMA products
+//                  (v_mfma_f64_16x16x4f64): Cr -= Ar Br - Ai Bi,
+//                  Ci -= Ar Bi + Ai Br.
+// The solve walks the panels forward (permute, L11 solve, L21 update) and
+// backward (U11 solve, U update) with one fused kernel per step.
+#include <hip/hip_cooperative_groups.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "../../include/bemx.h"
+#include "bemx_lu.hpp"
+
+namespace bemx {
+
+void* LuWork::get(int slot, size_t bytes, hipError_t* e) {
+    *e = hipSuccess;
+    if (bytes <= cap[slot] && buf[slot]) return buf[slot];
+    if (buf[slot]) hipFree(buf[slot]);
+    buf[slot] = nullptr;
+    cap[slot] = 0;
+    *e = hipMalloc(&buf[slot], bytes ? bytes : 256);
+    if (*e == hipSuccess) cap[slot] = bytes;
+    return buf[slot];
+}
+
+void LuWork::release() {
+    for (int i = 0; i < 16; ++i) {
+        if (buf[i]) hipFree(buf[i]);
+        buf[i] = nullptr;
+        cap[i] = 0;
+    }
+}
+
+constexpr int NB = 64;          // panel width (FP64 path)
+constexpr int PANEL_ROWS = 64;  // rows of the panel per workgroup (64 x 64 x 16 B = 64 KB LDS)
+constexpr int SPIN_LIMIT = 1 << 22;
+
+struct Cand {
+    double val;
+    int row;
+    int pad;
+};
+
+__device__ inline double cabs1(cd a) { return fabs(a.x) + fabs(a.y); }
+
+// (val, row) ordering: larger val wins; equal val -> smaller row.
+__device__ inline bool better(double v, int r, double bv, int br) {
+    return v > bv || (v == bv && r < br);
+}
+
+// Bounded wait until *cnt >= target.  Gives up (and flags ST_SYNC_TIMEOUT)
+// after SPIN_LIMIT polls, or at once if an earlier wait already gave up, so
+// every wave of the grid always runs to its end.
+__device__ inline void wait_count(int* cnt, int target, int* status) {
+    int it = 0;
+    while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (++it > SPIN_LIMIT ||
+            (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+             ST_SYNC_TIMEOUT)) {
+            atomicOr(status, ST_SYNC_TIMEOUT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Panel factorisation.  Workgroup p owns panel rows [r0, r1) (global row
+// indices, r0 = k0 + p*PANEL_ROWS) held column-major in LDS.  For column j
+// (diagonal row d = k0 + j):
+//   1. local best candidate among owned rows >= d
+//   2. publish (val,row) and the candidate row; the owner of row d publishes
+//      row d; release; arrive on counter j; wait for all P arrivals
+//   3. every workgroup picks the same winner, reads the pivot row, swaps the
+//      contents of rows d and piv where it owns them, scales column j and
+//      applies the rank-1 update to columns j+1.. of its rows below d.
+// At the end workgroup 0 turns the pivot sequence into the list of rows
+// whose content changed (dst) and where it came from (src).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t lda, int n, int k0,
+                                                    int nb, int* __restrict__ ipiv,
+                                                    int* __restrict__ counters,
+                                                    Cand* __restrict__ slots,
+                                                    cd* __restrict__ cand, cd* __restrict__ drow,
+                                                    int* __restrict__ perm,
+                                                    long long* __restrict__ info,
+                                                    int* __restrict__ status) {
+    __shared__ cd sA[NB * PANEL_ROWS];
+    __shared__ cd urow[NB];
+    __shared__ double rv[4];
+    __shared__ int rr[4];
+    __shared__ int s_piv, s_wp;
+    __shared__ double s_pval;
+    const int P = gridDim.x, p = blockIdx.x, tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int r0 = k0 + p * PANEL_ROWS;
+    const int r1 = min(n, r0 + PANEL_ROWS);
+    const int nr = r1 - r0;
+    for (int t = tid; t < nr * nb; t += 256) {
+        int c = t / nr, r = t - c * nr;
+        sA[c * PANEL_ROWS + r] = A[(size_t)(k0 + c) * lda + r0 + r];
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        const int d = k0 + j;
+        // 1. local candidate
+        double bv = -1.0;
+        int br = 0x7fffffff;
+        if (tid < nr && r0 + tid >= d) {
+            bv = cabs1(sA[j * PANEL_ROWS + tid]);
+            br = r0 + tid;
+        }
+        for (int off = 32; off >= 1; off >>= 1) {
+            double ov = __shfl_xor(bv, off, 64);
+            int orr = __shfl_xor(br, off, 64);
+            if (better(ov, orr, bv, br)) {
+                bv = ov;
+                br = orr;
+            }
+        }
+        if (lane == 0) {
+            rv[wave] = bv;
+            rr[wave] = br;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            for (int w = 1; w < 4; ++w)
+                if (better(rv[w], rr[w], bv, br)) {
+                    bv = rv[w];
+                    br = rr[w];
+                }
+            rv[0] = bv;
+            rr[0] = br;
+            slots[(size_t)j * P + p] = Cand{bv, br, 0};
+        }
+        __syncthreads();
+        // 2. publish rows
+        const int myrow = rr[0];
+        if (myrow != 0x7fffffff && tid < nb)
+            cand[((size_t)j * P + p) * NB + tid] = sA[tid * PANEL_ROWS + (myrow - r0)];
+        if (d >= r0 && d < r1 && tid < nb) drow[(size_t)j * NB + tid] = sA[tid * PANEL_ROWS + (d - r0)];
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) {
+            atomicAdd(&counters[j], 1);
+            wait_count(&counters[j], P, status);
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        // 3. winner (same on every workgroup)
+        if (wave == 0) {
+            double wv = -1.0;
+            int wr = 0x7fffffff, wp = 0;
+            for (int q = lane; q < P; q += 64) {
+                Cand c = slots[(size_t)j * P + q];
+                if (better(c.val, c.row, wv, wr)) {
+                    wv = c.val;
+                    wr = c.row;
+                    wp = q;
+                }
+            }
+            for (int off = 32; off >= 1; off >>= 1) {
+                double ov = __shfl_xor(wv, off, 64);
+                int orr = __shfl_xor(wr, off, 64);
+                int op = __shfl_xor(wp, off, 64);
+                if (better(ov, orr, wv, wr)) {
+                    wv = ov;
+                    wr = orr;
+                    wp = op;
+                }
+            }
+            if (lane == 0) {
+                s_piv = wr;
+                s_pval = wv;
+                s_wp = wp;
+            }
+        }
+        __syncthreads();
+        const int piv = s_piv;
+        const double pval = s_pval;
+        if (tid < nb) urow[tid] = cand[((size_t)j * P + s_wp) * NB + tid];
+        if (p == 0 && tid == 0) {
+            ipiv[d] = piv;
+            if (pval == 0.0 && *info == 0) *info = d + 1;
+        }
+        __syncthreads();
+        if (piv != d) {
+            if (d >= r0 && d < r1 && tid < nb) sA[tid * PANEL_ROWS + (d - r0)] = urow[tid];
+            if (piv >= r0 && piv < r1 && tid < nb)
+                sA[tid * PANEL_ROWS + (piv - r0)] = drow[(size_t)j * NB + tid];
+        }
+        __syncthreads();
+        // 4. eliminate below d
+        cd u = urow[j];
+        cd inv = (pval != 0.0) ? cinv(u) : mk(0.0, 0.0);
+        if (tid < nr && r0 + tid > d && pval != 0.0) {
+            cd l = sA[j * PANEL_ROWS + tid] * inv;
+            sA[j * PANEL_ROWS + tid] = l;
+        }
+        __syncthreads();
+        const int rest = nb - j - 1;
+        for (int t = tid; t < nr * rest; t += 256) {
+            int c = j + 1 + t / nr, r = t - (t / nr) * nr;
+            if (r0 + r > d) sA[c * PANEL_ROWS + r] = sA[c * PANEL_ROWS + r] - sA[j * PANEL_ROWS + r] * urow[c];
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < nr * nb; t += 256) {
+        int c = t / nr, r = t - c * nr;
+        A[(size_t)(k0 + c) * lda + r0 + r] = sA[c * PANEL_ROWS + r];
+    }
+    if (p == 0 && tid == 0) {
+        // content[i] = original row now at position key[i]
+        int key[2 * NB], content[2 * NB], cnt = 0;
+        auto find = [&](int row) {
+            for (int i = 0; i < cnt; ++i)
+                if (key[i] == row) return i;
+            key[cnt] = row;
+            content[cnt] = row;
+            return cnt++;
+        };
+        for (int j = 0; j < nb; ++j) {
+            int a = find(k0 + j), b = find(ipiv[k0 + j]);
+            int t = content[a];
+            content[a] = content[b];
+            content[b] = t;
+        }
+        int m = 0;
+        for (int i = 0; i < cnt; ++i)
+            if (content[i] != key[i]) {
+                perm[1 + m] = key[i];
+                perm[1 + 2 * NB + m] = content[i];
+                ++m;
+            }
+        perm[0] = m;
+    }
+}
+
+// Row permutation of columns [c0, c0+nc): row dst[i] <- old row src[i].
+// perm = {m, dst[0..2NB), src[0..2NB)}.  One workgroup handles 4 columns.
+__global__ __launch_bounds__(256) void k_rowperm(cd* __restrict__ A, size_t lda, int c0, int nc,
+                                                 const int* __restrict__ perm) {
+    const int m = perm[0];
+    const int c = c0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int s = threadIdx.x & 63;
+    cd v[2];
+    const bool colok = c < c0 + nc;
+    for (int h = 0; h < 2; ++h) {
+        int i = s + 64 * h;
+        if (colok && i < m) v[h] = A[(size_t)c * lda + perm[1 + 2 * NB + i]];
+    }
+    __syncthreads();
+    for (int h = 0; h < 2; ++h) {
+        int i = s + 64 * h;
+        if (colok && i < m) A[(size_t)c * lda + perm[1 + i]] = v[h];
+    }
+}
+
+// U12 = L11^-1 A12 for the NB rows k0.. of columns [c0, c0+nc).
+__global__ __launch_bounds__(256) void k_trsm_u12(cd* __restrict__ A, size_t lda, int k0, int nb,
+                                                  int c0, int nc) {
+    __shared__ cd L[NB * NB];
+    for (int t = threadIdx.x; t < nb * nb; t += 256) {
+        int c = t / nb, r = t - c * nb;
+        L[c * NB + r] = A[(size_t)(k0 + c) * lda + k0 + r];
+    }
+    __syncthreads();
+    const int c = c0 + blockIdx.x * 256 + threadIdx.x;
+    if (c >= c0 + nc) return;
+    cd x[NB];
+    cd* col = A + (size_t)c * lda + k0;
+#pragma unroll
+    for (int r = 0; r < NB; ++r) x[r] = r < nb ? col[r] : mk(0, 0);
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+#pragma unroll
+        for (int r = i + 1; r < NB; ++r)
+            if (r < nb) x[r] = x[r] - L[i * NB + r] * x[i];
+    }
+#pragma unroll
+    for (int r = 0; r < NB; ++r)
+        if (r < nb) col[r] = x[r];
+}
+
+// ---------------------------------------------------------------------------
+// C[m x nc] -= A[m x K] B[K x nc], complex FP64, column-major, K <= NB.
+// 256 threads = 4 waves; a workgroup owns a 64 x 64 tile of C, each wave a
+// 32 x 32 quarter made of 2 x 2 MFMA 16x16 blocks with separate real and
+// imaginary accumulators.  The K dimension is staged through LDS in chunks
+// of 32 as separate real/imaginary planes.
+// MFMA 16x16x4 f64 operand layout: A[i][k] in lane i + 16k, B[k][j] in lane
+// j + 16k, D[i][j] in lane j + 16(i/4), register i%4.
+// ---------------------------------------------------------------------------
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int GB = 64;   // tile rows/cols
+constexpr int GK = 32;   // K chunk
+
+__global__ __launch_bounds__(256) void k_zgemm_mfma(int m, int nc, int K, const cd* __restrict__ A,
+                                                    size_t lda, const cd* __restrict__ B,
+                                                    size_t ldb, cd* __restrict__ C, size_t ldc) {
+    __shared__ double sAr[GK][GB + 1], sAi[GK][GB + 1];
+    __shared__ double sBr[GK][GB + 1], sBi[GK][GB + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int bm = blockIdx.x * GB, bn = blockIdx.y * GB;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    v4d accR[2][2], accI[2][2];
+    // load C into the accumulators
+    for (int bi = 0; bi < 2; ++bi)
+        for (int bj = 0; bj < 2; ++bj) {
+            int col = bn + wn + 16 * bj + li;
+            for (int r = 0; r < 4; ++r) {
+                int row = bm + wm + 16 * bi + 4 * lk + r;
+                cd v = (row < m && col < nc) ? C[(size_t)col * ldc + row] : mk(0, 0);
+                accR[bi][bj][r] = v.x;
+                accI[bi][bj][r] = v.y;
+            }
+        }
+    for (int kc = 0; kc < K; kc += GK) {
+        const int kn = min(GK, K - kc);
+        __syncthreads();
+        for (int t = tid; t < GK * GB; t += 256) {
+            int k = t / GB, i = t - k * GB;
+            cd v = (k < kn && bm + i < m) ? A[(size_t)(kc + k) * lda + bm + i] : mk(0, 0);
+            sAr[k][i] = v.x;
+            sAi[k][i] = v.y;
+        }
+        for (int t = tid; t < GK * GB; t += 256) {
+            int j = t / GK, k = t - j * GK;
+            cd v = (k < kn && bn + j < nc) ? B[(size_t)(bn + j) * ldb + kc + k] : mk(0, 0);
+            sBr[k][j] = v.x;
+            sBi[k][j] = v.y;
+        }
+        __syncthreads();
+        for (int k = 0; k < GK; k += 4) {
+            const int kk = k + lk;
+            double ar[2], ai[2], br[2], bi_[2];
+            for (int b = 0; b < 2; ++b) {
+                ar[b] = sAr[kk][wm + 16 * b + li];
+                ai[b] = sAi[kk][wm + 16 * b + li];
+                br[b] = sBr[kk][wn + 16 * b + li];
+                bi_[b] = sBi[kk][wn + 16 * b + li];
+            }
+            for (int x = 0; x < 2; ++x)
+                for (int y = 0; y < 2; ++y) {
+                    accR[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[x], br[y], accR[x][y], 0, 0, 0);
+                    accR[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], bi_[y], accR[x][y], 0, 0, 0);
+                    accI[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[x], bi_[y], accI[x][y], 0, 0, 0);
+                    accI[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[x], br[y], accI[x][y], 0, 0, 0);
+                }
+        }
+    }
+    for (int bi = 0; bi < 2; ++bi)
+        for (int bj = 0; bj < 2; ++bj) {
+            int col = bn + wn + 16 * bj + li;
+            for (int r = 0; r < 4; ++r) {
+                int row = bm + wm + 16 * bi + 4 * lk + r;
+                if (row < m && col < nc) C[(size_t)col * ldc + row] = mk(accR[bi][bj][r], accI[bi][bj][r]);
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Solve with the factors.  Forward step for panel k: apply the panel's row
+// permutation to b, y = L11^-1 b[k0:k0+nb] (every workgroup recomputes the
+// small block solve), then b[r] -= L21[r,:] y for the workgroup's rows.
+// Workgroup 0 also writes y back.  Backward step for panel k (last first):
+// x = U11^-1 b[k0:k0+nb], b[r] -= U[r, k0:k0+nb] x for rows r < k0.
+// One thread per row; b holds nrhs = 1 column.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fwd_step(const cd* __restrict__ A, size_t lda, int n,
+                                                  int k0, int nb, const int* __restrict__ perm,
+                                                  cd* __restrict__ b, cd* __restrict__ btmp) {
+    __shared__ cd y[NB];
+    __shared__ cd L[NB * NB];
+    const int tid = threadIdx.x;
+    for (int t = tid; t < nb * nb; t += 256) {
+        int c = t / nb, r = t - c * nb;
+        L[c * NB + r] = A[(size_t)(k0 + c) * lda + k0 + r];
+    }
+    // the permuted block of b (rows k0..k0+nb), read from the pre-step copy
+    if (tid < nb) {
+        int row = k0 + tid;
+        int m = perm[0], src = row;
+        for (int i = 0; i < m; ++i)
+            if (perm[1 + i] == row) src = perm[1 + 2 * NB + i];
+        y[tid] = btmp[src];
+    }
+    __syncthreads();
+    if (tid < 64) {  // unit-lower solve, lane r owns y[r]
+        cd yr = tid < nb ? y[tid] : mk(0, 0);
+        for (int i = 0; i < nb; ++i) {
+            cd yi = mk(__shfl(yr.x, i, 64), __shfl(yr.y, i, 64));
+            if (tid > i && tid < nb) yr = yr - L[i * NB + tid] * yi;
+        }
+        if (tid < nb) y[tid] = yr;
+    }
+    __syncthreads();
+    const int row = k0 + nb + blockIdx.x * 256 + tid;
+    if (blockIdx.x == 0 && tid < nb) b[k0 + tid] = y[tid];
+    if (row < n) {
+        int m = perm[0], src = row;
+        for (int i = 0; i < m; ++i)
+            if (perm[1 + i] == row) src = perm[1 + 2 * NB + i];
+        cd s = btmp[src];
+        for (int i = 0; i < nb; ++i) s = s - A[(size_t)(k0 + i) * lda + row] * y[i];
+        b[row] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bwd_step(const cd* __restrict__ A, size_t lda, int k0,
+                                                  int nb, cd* __restrict__ b,
+                                                  const cd* __restrict__ btmp) {
+    __shared__ cd x[NB];
+    __shared__ cd U[NB * NB];
+    const int tid = threadIdx.x;
+    for (int t = tid; t < nb * nb; t += 256) {
+        int c = t / nb, r = t - c * nb;
+        U[c * NB + r] = A[(size_t)(k0 + c) * lda + k0 + r];
+    }
+    if (tid < nb) x[tid] = btmp[k0 + tid];
+    __syncthreads();
+    if (tid < 64) {  // upper solve, lane r owns x[r]
+        cd xr = tid < nb ? x[tid] : mk(0, 0);
+        for (int i = nb - 1; i >= 0; --i) {
+            if (tid == i) xr = xr * cinv(U[i * NB + i]);
+            cd xi = mk(__shfl(xr.x, i, 64), __shfl(xr.y, i, 64));
+            if (tid < i) xr = xr - U[i * NB + tid] * xi;
+        }
+        if (tid < nb) x[tid] = xr;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && tid < nb) b[k0 + tid] = x[tid];
+    const int row = blockIdx.x * 256 + tid;
+    if (row < k0) {
+        cd s = btmp[row];
+        for (int i = 0; i < nb; ++i) s = s - A[(size_t)(k0 + i) * lda + row] * x[i];
+        b[row] = s;
+    }
+}
+
+__global__ void k_fill_int(int* p, int v, size_t cnt) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < cnt) p[i] = v;
+}
+
+__global__ void k_copy_cd(const cd* __restrict__ s, cd* __restrict__ d, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+
+// ---------------------------------------------------------------------------
+// Host orchestration.
+// ---------------------------------------------------------------------------
+#define LU_CHK(expr)                                                        \
+    do {                                                                    \
+        hipError_t e_ = (expr);                                             \
+        if (e_ != hipSuccess) {                                             \
+            err = std::string("LU: ") + hipGetErrorString(e_);              \
+            return e_ == hipErrorOutOfMemory ? BEMX_ERR_NOMEM : BEMX_ERR_HIP; \
+        }                                                                   \
+    } while (0)
+
+enum { W_IPIV, W_CNT, W_SLOTS, W_CAND, W_DROW, W_PERM, W_INFO, W_STATUS, W_BTMP };
+
+static int factor_fp64(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, std::string& err) {
+    hipError_t e;
+    const int npanel = (n + NB - 1) / NB;
+    const int pmax = (n + PANEL_ROWS - 1) / PANEL_ROWS;
+    int* ipiv = (int*)w.get(W_IPIV, sizeof(int) * n, &e);
+    LU_CHK(e);
+    int* cnt = (int*)w.get(W_CNT, sizeof(int) * (size_t)npanel * NB, &e);
+    LU_CHK(e);
+    Cand* slots = (Cand*)w.get(W_SLOTS, sizeof(Cand) * (size_t)NB * pmax, &e);
+    LU_CHK(e);
+    cd* cand = (cd*)w.get(W_CAND, sizeof(cd) * (size_t)NB * pmax * NB, &e);
+    LU_CHK(e);
+    cd* drow = (cd*)w.get(W_DROW, sizeof(cd) * NB * NB, &e);
+    LU_CHK(e);
+    int* perm = (int*)w.get(W_PERM, sizeof(int) * (size_t)npanel * (1 + 4 * NB), &e);
+    LU_CHK(e);
+    long long* info = (long long*)w.get(W_INFO, sizeof(long long), &e);
+    LU_CHK(e);
+    int* status = (int*)w.get(W_STATUS, sizeof(int), &e);
+    LU_CHK(e);
+    size_t ncnt = (size_t)npanel * NB;
+    hipLaunchKernelGGL(k_fill_int, dim3((ncnt + 255) / 256), dim3(256), 0, st, cnt, 0, ncnt);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, (int*)info, 0, 2);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, status, 0, 1);
+    int dev = 0, ncu = 0, perCU = 0;
+    LU_CHK(hipGetDevice(&dev));
+    LU_CHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    LU_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_panel_fp64, 256, 0));
+    for (int pk = 0; pk < npanel; ++pk) {
+        const int k0 = pk * NB;
+        const int nb = std::min(NB, n - k0);
+        const int m = n - k0;
+        const int P = (m + PANEL_ROWS - 1) / PANEL_ROWS;
+        if (P > ncu * perCU) {
+            err = "LU panel needs more co-resident workgroups than the device holds";
+            return BEMX_ERR_ARG;
+        }
+        int* cntk = cnt + (size_t)pk * NB;
+        int* permk = perm + (size_t)pk * (1 + 4 * NB);
+        void* args[] = {&A, &lda, (void*)&n, (void*)&k0, (void*)&nb, &ipiv, &cntk, &slots, &cand,
+                        &drow, &permk, &info, &status};
+        LU_CHK(hipLaunchCooperativeKernel((const void*)k_panel_fp64, dim3(P), dim3(256), args, 0,
+                                          st));
+        const int c0 = k0 + nb, nc = n - c0;
+        if (nc <= 0) continue;
+        hipLaunchKernelGGL(k_rowperm, dim3((nc + 3) / 4), dim3(256), 0, st, A, lda, c0, nc, permk);
+        hipLaunchKernelGGL(k_trsm_u12, dim3((nc + 255) / 256), dim3(256), 0, st, A, lda, k0, nb,
+                           c0, nc);
+        const int mr = n - c0;
+        hipLaunchKernelGGL(k_zgemm_mfma, dim3((mr + GB - 1) / GB, (nc + GB - 1) / GB), dim3(256), 0,
+                           st, mr, nc, nb, A + (size_t)k0 * lda + c0, lda,
+                           A + (size_t)c0 * lda + k0, lda, A + (size_t)c0 * lda + c0, lda);
+        LU_CHK(hipGetLastError());
+    }
+    return BEMX_OK;
+}
+
+static int solve_fp64(LuWork& w, hipStream_t st, int n, const cd* A, size_t lda, cd* b,
+                      std::string& err) {
+    hipError_t e;
+    const int npanel = (n + NB - 1) / NB;
+    int* perm = (int*)w.get(W_PERM, sizeof(int) * (size_t)npanel * (1 + 4 * NB), &e);
+    LU_CHK(e);
+    cd* btmp = (cd*)w.get(W_BTMP, sizeof(cd) * n, &e);
+    LU_CHK(e);
+    for (int pk = 0; pk < npanel; ++pk) {
+        const int k0 = pk * NB, nb = std::min(NB, n - k0);
+        hipLaunchKernelGGL(k_copy_cd, dim3((n + 255) / 256), dim3(256), 0, st, b, btmp, n);
+        const int rows = std::max(1, n - k0 - nb);
+        hipLaunchKernelGGL(k_fwd_step, dim3((rows + 255) / 256), dim3(256), 0, st, A, lda, n, k0,
+                           nb, perm + (size_t)pk * (1 + 4 * NB), b, btmp);
+    }
+    for (int pk = npanel - 1; pk >= 0; --pk) {
+        const int k0 = pk * NB, nb = std::min(NB, n - k0);
+        hipLaunchKernelGGL(k_copy_cd, dim3((n + 255) / 256), dim3(256), 0, st, b, btmp, n);
+        const int rows = std::max(1, k0);
+        hipLaunchKernelGGL(k_bwd_step, dim3((rows + 255) / 256), dim3(256), 0, st, A, lda, k0, nb,
+                           b, btmp);
+    }
+    LU_CHK(hipGetLastError());
+    return BEMX_OK;
+}
+
+int lu_solve(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, cd* b, int nrhs, int mode,
+             long long* info, double stats[4], std::string& err) {
+    if (nrhs != 1) {
+        err = "LU: one right-hand side per call";
+        return BEMX_ERR_ARG;
+    }
+    if (stats) stats[0] = stats[1] = stats[2] = stats[3] = 0.0;
+    if (mode == BEMX_MODE_MIXED) {
+        int sm = gmres_ir(w, st, n, A, lda, b, stats, err);
+        if (sm != BEMX_ERR_NOCONV && sm != BEMX_ERR_SINGULAR) return sm;
+        // the plain FP64 path is the fallback; A and b are still intact
+        if (stats) stats[2] = 1.0;
+    }
+    int s = factor_fp64(w, st, n, A, lda, err);
+    if (s) return s;
+    long long hinfo = 0;
+    int hstatus = 0;
+    hipError_t e;
+    LU_CHK(hipMemcpyAsync(&hinfo, w.get(W_INFO, sizeof(long long), &e), sizeof(long long),
+                          hipMemcpyDeviceToHost, st));
+    LU_CHK(hipMemcpyAsync(&hstatus, w.get(W_STATUS, sizeof(int), &e), sizeof(int),
+                          hipMemcpyDeviceToHost, st));
+    LU_CHK(hipStreamSynchronize(st));
+    if (hstatus & ST_SYNC_TIMEOUT) {
+        err = "LU panel: inter-workgroup wait timed out";
+        return BEMX_ERR_TIMEOUT;
+    }
+    if (info) *info = hinfo;
+    if (hinfo > 0) {
+        err = "LU: exactly singular pivot (INFO > 0)";
+        return BEMX_ERR_SINGULAR;
+    }
+    return solve_fp64(w, st, n, A, lda, b, err);
+}
+
+}  // namespace bemx

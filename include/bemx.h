@@ -1,0 +1,129 @@
+/*
+ * bemx.h -- C ABI of the MI355X-native BEM hot path (libbemx.so).
+ *
+ * Drop-in replacement for the reference's four assembly/solve subroutines
+ * and the device entry points they sat on (giulianobelinassi/BEM-MECECCS,
+ * src/kernels/*.cu).  Plain pointers and sizes only.  Arrays use the
+ * reference's Fortran layouts:
+ *   - real arrays are double (FPREC=double build)
+ *   - complex arrays / scalars are interleaved (re, im) doubles
+ *   - CONE(N,4) column-major int32 with 1-based node ids
+ *   - ETAS(3,N), HESTdiag/GESTdiag(3,3,NBE) column-major
+ *   - ZH(3NBE,3NBE) column-major, leading dimension 3NBE
+ * Every entry point returns a bemx_status; nothing inside the library exits
+ * the process (the reference exit(1)s on CUDA/MAGMA errors,
+ * kernels/shared.cu:33-41, Linsolve_cu.cu:10-17).  A context may be used by
+ * one host thread at a time; separate contexts are independent.
+ */
+#ifndef BEMX_H
+#define BEMX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bemx_ctx bemx_ctx;
+
+typedef enum {
+    BEMX_OK = 0,
+    BEMX_ERR_DEGENERATE = 1, /* Jacobian DET < 1e-5 (Nonsingd.for:152-158); the
+                                reference's GPU status "Matriz Singular" */
+    BEMX_ERR_SINGULAR = 2,   /* exactly zero pivot (LAPACK INFO > 0) */
+    BEMX_ERR_ARG = 3,        /* invalid argument / size */
+    BEMX_ERR_HIP = 4,        /* HIP runtime error */
+    BEMX_ERR_NOMEM = 5,      /* device allocation failed / admission refused */
+    BEMX_ERR_STATE = 6,      /* entry point called out of order */
+    BEMX_ERR_NOCONV = 7,     /* iterative refinement did not reach tolerance */
+    BEMX_ERR_TIMEOUT = 8     /* a bounded device-side wait gave up */
+} bemx_status;
+
+/* Solver modes for LINSOLVE / bemx_solve. */
+#define BEMX_MODE_FP64 0  /* partial-pivot LU in FP64 (ZGESV semantics) */
+#define BEMX_MODE_MIXED 1 /* FP32 LU (split-BF16x3 MFMA update) + FP64 GMRES-IR */
+
+/* Context on one device.  No reference counterpart: replaces the module
+ * globals of kernels/shared.cu:6-31. */
+int bemx_create(int device, bemx_ctx **ctx);
+int bemx_destroy(bemx_ctx *ctx);
+const char *bemx_last_error(const bemx_ctx *ctx);
+
+/* Gauss-Legendre rule on [-1,1].  Replaces GAULEG (Gauleg.for:9-36). */
+int bemx_gauleg(int npg, double *gi, double *ome);
+
+/* Upload the mesh and build its device geometry (normals, centroids, Gauss
+ * points).  Replaces send_shared_data_to_gpu_ (kernels/shared.cu:84-165).
+ * cxm/cym/czm/etas may be NULL: the device then computes them as NORMVEC
+ * (Normvec.for:4-39) and INPUTECE (Inputece.for:100-102) do; gi/ome may be
+ * NULL: bemx_gauleg is used. */
+int bemx_send_shared_data(bemx_ctx *ctx, const double *cx, const double *cy, const double *cz,
+                          const double *cxm, const double *cym, const double *czm,
+                          const double *etas, const double *gi, const double *ome,
+                          const int *cone, int np, int npg, int n, int nbe);
+
+/* Release the mesh.  Replaces deallocate_shared_gpu_data_ (kernels/shared.cu:184-214). */
+int bemx_deallocate_shared_data(bemx_ctx *ctx);
+
+/* Static pass.  Replaces GHMATECE (Ghmatece.for:9-256) = cuda_ghmatece_
+ * (kernels/Ghmatece_cu.cu:264-371) + GEST_SINGULAR + cuda_send_gest_data_
+ * (:373-380).  Outputs (any may be NULL): HESTdiag(3,3,NBE), GESTdiag(3,3,NBE),
+ * DELTA(3,3), PI, C1..C4.  The diagonals stay resident for bemx_ghmatecd. */
+int bemx_ghmatece(bemx_ctx *ctx, double ge, double rnu, double *hestdiag, double *gestdiag,
+                  double *delta, double *pi, double *c1, double *c2, double *c3, double *c4);
+
+/* Boundary data of the dynamic problem: ZDFI(3NBE) complex, KODE(3NBE).
+ * Replaces send_dyn_data_gpu_ (kernels/shared.cu:168-182). */
+int bemx_send_dyn_data(bemx_ctx *ctx, const double *zdfi, const int *kode);
+
+/* Dynamic assembly at one frequency.  Replaces GHMATECD (Ghmatecd.for:11-239)
+ * = cuda_ghmatecd_ (kernels/Ghmatecd_cu.cu:305-483): ZH (3NBE x 3NBE) after
+ * the KODE column swap, and ZFI = ZG ZDFI.  zge/zcs/zcp point to complex
+ * scalars.  hestdiag/gestdiag (host) may be NULL to use the ones resident
+ * from bemx_ghmatece; c1..c4 are the static constants.  zh may be NULL: the
+ * matrix then stays on the device only (bemx_linsolve uses it). */
+int bemx_ghmatecd(bemx_ctx *ctx, const double *zge, const double *zcs, const double *zcp,
+                  double c1, double c2, double c3, double c4, double fr,
+                  const double *hestdiag, const double *gestdiag, double *zfi, double *zh);
+
+/* Dense complex solve ZH x = ZFI, x returned in zfi.  Replaces LINSOLVE
+ * (Linsolve.for:1-66) = LINSOLVE_CPU/ZGESV (:68-109) and cuda_linsolve_
+ * (kernels/Linsolve_cu.cu:53-126).  If zh is NULL, or is the host array the
+ * last bemx_ghmatecd wrote, the device-resident matrix is factored (the
+ * reference hands device_zh from GHMATECD to LINSOLVE the same way);
+ * otherwise zh is uploaded.  mode: BEMX_MODE_FP64 or BEMX_MODE_MIXED.
+ * info (may be NULL): LAPACK-style (0, or k > 0 for a zero pivot at k). */
+int bemx_linsolve(bemx_ctx *ctx, int nn, double *zh, double *zfi, int mode, long long *info);
+
+/* Interior points.  Replaces INTEREC (Interec.for:9-288) = the KODE
+ * un-swap of ZFI/ZDFI (in place, as the reference), cuda_interec1_
+ * (kernels/Interec1_cu.cu:44-237) for ZDSOL(3L) and the SIGMAEC stress loop
+ * (Interec.for:186-283) for ZSSOL(9L). */
+int bemx_interec(bemx_ctx *ctx, double *zfi, double *zdfi, const int *kode, int l,
+                 const double *cxi, const double *cyi, const double *czi, const double *zge,
+                 const double *zcs, const double *zcp, double fr, double rho, double *zdsol,
+                 double *zssol);
+
+/* Device-resident solve of one frequency: assembly + factor (+ refine) +
+ * solve, with the boundary data from bemx_send_dyn_data.  No reference
+ * counterpart (it is GHMATECD + LINSOLVE without the host round trip of ZH);
+ * used by the sweep, the serving loop and the benchmark.  zfi_out (host,
+ * may be NULL) receives the solution; stats (may be NULL, 4 doubles):
+ * refinement iterations, final relative residual, backward error estimate,
+ * assembly+solve device time in ms. */
+int bemx_solve(bemx_ctx *ctx, const double *zge, const double *zcs, const double *zcp,
+               double fr, int mode, double *zfi_out, double *stats);
+
+/* Statistics of the last bemx_linsolve / bemx_solve (8 doubles): GMRES
+ * iterations, final ||r||/||b||, 1 if the mixed path fell back to FP64,
+ * solve device time (ms), off-diagonal assembly kernel time (ms), whole
+ * assembly time (ms), factor+refine+solve time (ms), 0.  Times are filled by
+ * bemx_solve only.  No reference counterpart (the reference prints
+ * per-routine wall times, Ghmatecd.for:183-184). */
+int bemx_last_stats(const bemx_ctx *ctx, double *out);
+
+/* Device buffer addresses of the last solve (for zero-copy consumers). */
+int bemx_device_buffers(bemx_ctx *ctx, void **zh, void **zfi, void **hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BEMX_H */
