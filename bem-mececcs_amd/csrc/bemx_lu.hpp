@@ -10,8 +10,9 @@ namespace bemx {
 
 // Device workspace of the factorisation (grow-only).
 struct LuWork {
-    void* buf[16] = {nullptr};
-    size_t cap[16] = {0};
+    static constexpr int SLOTS = 32;  // slots 0..8: FP64 path, 9..: mixed path
+    void* buf[SLOTS] = {nullptr};
+    size_t cap[SLOTS] = {0};
     void* get(int slot, size_t bytes, hipError_t* e);
     void release();
 };

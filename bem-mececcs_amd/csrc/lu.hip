@@ -39,7 +39,7 @@ void* LuWork::get(int slot, size_t bytes, hipError_t* e) {
 }
 
 void LuWork::release() {
-    for (int i = 0; i < 16; ++i) {
+    for (int i = 0; i < SLOTS; ++i) {
         if (buf[i]) hipFree(buf[i]);
         buf[i] = nullptr;
         cap[i] = 0;
