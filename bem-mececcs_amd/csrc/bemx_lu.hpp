@@ -13,6 +13,11 @@ struct LuWork {
     static constexpr int SLOTS = 32;  // slots 0..8: FP64 path, 9..: mixed path
     void* buf[SLOTS] = {nullptr};
     size_t cap[SLOTS] = {0};
+    // Flags of the cooperative triangular solves hold the epoch of the
+    // launch that set them; epochs only grow, so the array is cleared once,
+    // when it is (re)allocated, and never inside a solve.
+    int epoch = 0;
+    void* flags_init = nullptr;
     void* get(int slot, size_t bytes, hipError_t* e);
     void release();
 };

@@ -10,7 +10,7 @@
 //   k_rowperm      applies that permutation to the trailing columns (the
 //                  factored L columns are left in place: the forward solve
 //                  applies each panel's permutation before using its L block).
-//   k_trsm_u12     U12 = L11^-1 A12, one column per thread.
+//   k_trsm_lower_cols  U12 = L11^-1 A12, one wavefront per column.
 //   k_zgemm_mfma   A22 -= L21 U12 as four real FP64 MFMA products
 //                  (v_mfma_f64_16x16x4f64): Cr -= Ar Br - Ai Bi,
 //                  Ci -= Ar Bi + Ai Br.
@@ -24,6 +24,7 @@
 
 #include "../../include/bemx.h"
 #include "bemx_lu.hpp"
+#include "bemx_lu_kernels.hpp"
 
 namespace bemx {
 
@@ -271,32 +272,6 @@ __global__ __launch_bounds__(256) void k_rowperm(cd* __restrict__ A, size_t lda,
     }
 }
 
-// U12 = L11^-1 A12 for the NB rows k0.. of columns [c0, c0+nc).
-__global__ __launch_bounds__(256) void k_trsm_u12(cd* __restrict__ A, size_t lda, int k0, int nb,
-                                                  int c0, int nc) {
-    __shared__ cd L[NB * NB];
-    for (int t = threadIdx.x; t < nb * nb; t += 256) {
-        int c = t / nb, r = t - c * nb;
-        L[c * NB + r] = A[(size_t)(k0 + c) * lda + k0 + r];
-    }
-    __syncthreads();
-    const int c = c0 + blockIdx.x * 256 + threadIdx.x;
-    if (c >= c0 + nc) return;
-    cd x[NB];
-    cd* col = A + (size_t)c * lda + k0;
-#pragma unroll
-    for (int r = 0; r < NB; ++r) x[r] = r < nb ? col[r] : mk(0, 0);
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-#pragma unroll
-        for (int r = i + 1; r < NB; ++r)
-            if (r < nb) x[r] = x[r] - L[i * NB + r] * x[i];
-    }
-#pragma unroll
-    for (int r = 0; r < NB; ++r)
-        if (r < nb) col[r] = x[r];
-}
-
 // ---------------------------------------------------------------------------
 // C[m x nc] -= A[m x K] B[K x nc], complex FP64, column-major, K <= NB.
 // 256 threads = 4 waves; a workgroup owns a 64 x 64 tile of C, each wave a
@@ -304,7 +279,8 @@ __global__ __launch_bounds__(256) void k_trsm_u12(cd* __restrict__ A, size_t lda
 // imaginary accumulators.  The K dimension is staged through LDS in chunks
 // of 32 as separate real/imaginary planes.
 // MFMA 16x16x4 f64 operand layout: A[i][k] in lane i + 16k, B[k][j] in lane
-// j + 16k, D[i][j] in lane j + 16(i/4), register i%4.
+// j + 16k, D[i][j] in lane j + 16(i%4), register i/4 (unlike the f32-output
+// 16x16 forms, whose D rows run 4 per lane group).
 // ---------------------------------------------------------------------------
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int GB = 64;   // tile rows/cols
@@ -325,7 +301,7 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(int m, int nc, int K, const 
         for (int bj = 0; bj < 2; ++bj) {
             int col = bn + wn + 16 * bj + li;
             for (int r = 0; r < 4; ++r) {
-                int row = bm + wm + 16 * bi + 4 * lk + r;
+                int row = bm + wm + 16 * bi + 4 * r + lk;
                 cd v = (row < m && col < nc) ? C[(size_t)col * ldc + row] : mk(0, 0);
                 accR[bi][bj][r] = v.x;
                 accI[bi][bj][r] = v.y;
@@ -369,7 +345,7 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(int m, int nc, int K, const 
         for (int bj = 0; bj < 2; ++bj) {
             int col = bn + wn + 16 * bj + li;
             for (int r = 0; r < 4; ++r) {
-                int row = bm + wm + 16 * bi + 4 * lk + r;
+                int row = bm + wm + 16 * bi + 4 * r + lk;
                 if (row < m && col < nc) C[(size_t)col * ldc + row] = mk(accR[bi][bj][r], accI[bi][bj][r]);
             }
         }
@@ -524,8 +500,9 @@ static int factor_fp64(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, std:
         const int c0 = k0 + nb, nc = n - c0;
         if (nc <= 0) continue;
         hipLaunchKernelGGL(k_rowperm, dim3((nc + 3) / 4), dim3(256), 0, st, A, lda, c0, nc, permk);
-        hipLaunchKernelGGL(k_trsm_u12, dim3((nc + 255) / 256), dim3(256), 0, st, A, lda, k0, nb,
-                           c0, nc);
+        hipLaunchKernelGGL((k_trsm_lower_cols<cd, false>),
+                           dim3((nc + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG), dim3(256), 0, st,
+                           A, lda, k0, nb, c0, nc, (const int*)nullptr);
         const int mr = n - c0;
         hipLaunchKernelGGL(k_zgemm_mfma, dim3((mr + GB - 1) / GB, (nc + GB - 1) / GB), dim3(256), 0,
                            st, mr, nc, nb, A + (size_t)k0 * lda + c0, lda,

@@ -3,14 +3,18 @@
 // FP64 GMRES iterative refinement (GMRES-IR) against the FP64 matrix.
 //
 // Factorisation (blocks of NBM = 64 columns, A32 = (complex float) A64):
-//   k_diag_getrf32  partial pivoting inside the 64 x 64 diagonal block
-//                   (LDS), and the explicit inverses L11^-1, U11^-1
-//   k_u12_inv       U12 = L11^-1 P A12   (one column per thread)
-//   k_l21_inv       L21 = A21 U11^-1     (one row per thread)
-//   k_cgemm_bf16x3  A22 -= L21 U12; every FP32 operand x is split into
-//                   hi = bf16(x), lo = bf16(x - hi) and each real product is
-//                   hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16 with
-//                   FP32 accumulation; complex = 4 real products.
+//   k_diag_getrf32     partial pivoting inside the 64 x 64 diagonal block (LDS)
+//   k_trsm_lower_cols  U12 = L11^-1 P A12   (one wavefront per column)
+//   k_trsm_upper_rows  L21 = A21 U11^-1     (one wavefront per row, tile in LDS)
+//   k_cgemm_bf16x3     A22 -= L21 U12; every FP32 operand x is split into
+//                      hi = bf16(x), lo = bf16(x - hi) and each real product is
+//                      hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16 with
+//                      FP32 accumulation; complex = 4 real products.
+// Preconditioner M^-1 v (FP64 vectors, FP32 factors): k_trsv_lower_coop and
+// k_trsv_upper_coop, one cooperative launch each; workgroups own 64-row
+// blocks, accumulate the contributions of finished blocks as their flags
+// come up and publish their own block, so the dependency chain costs one
+// small GEMV and one in-register solve per block instead of one launch.
 // Pivoting is restricted to the diagonal block: the factors only
 // precondition.  The accuracy of the answer comes from the refinement, whose
 // residual b - A64 x is formed in FP64 with the FP64 matrix; if it stalls the
@@ -22,23 +26,13 @@
 
 #include "../../include/bemx.h"
 #include "bemx_lu.hpp"
+#include "bemx_lu_kernels.hpp"
 
 namespace bemx {
 
-struct cf {
-    float x, y;
-};
-__device__ inline cf mkf(float x, float y) { return cf{x, y}; }
-__device__ inline cf operator+(cf a, cf b) { return cf{a.x + b.x, a.y + b.y}; }
-__device__ inline cf operator-(cf a, cf b) { return cf{a.x - b.x, a.y - b.y}; }
-__device__ inline cf operator*(cf a, cf b) { return cf{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
-__device__ inline cf cinvf(cf a) {
-    float d = 1.0f / (a.x * a.x + a.y * a.y);
-    return cf{a.x * d, -a.y * d};
-}
-__device__ inline cd up(cf a) { return cd{(double)a.x, (double)a.y}; }
-
 constexpr int NBM = 64;
+static_assert(NBM == TB, "the mixed LU block is the triangular-kernel block");
+constexpr int FLAG_SPIN_LIMIT = 1 << 22;
 
 __global__ void k_cvt_z2c(const cd* __restrict__ a, size_t lda, cf* __restrict__ o, size_t ldo,
                           int n) {
@@ -50,14 +44,12 @@ __global__ void k_cvt_z2c(const cd* __restrict__ a, size_t lda, cf* __restrict__
     o[c * ldo + r] = cf{(float)v.x, (float)v.y};
 }
 
-// Diagonal block: GEPP within its nb rows, then the inverses.
-// lperm[i] = block row whose content ends at block row i.
+// Diagonal block: GEPP within its nb rows (largest |re|+|im|, lowest row on
+// ties).  lperm[i] = block row whose content ends at block row i.
 __global__ __launch_bounds__(256) void k_diag_getrf32(cf* __restrict__ A, size_t lda, int k0,
                                                       int nb, int* __restrict__ lperm,
-                                                      cf* __restrict__ Linv, cf* __restrict__ Uinv,
                                                       int* __restrict__ status) {
     __shared__ cf S[NBM * NBM];
-    __shared__ cf X[NBM * NBM];
     __shared__ int pos[NBM];
     __shared__ int s_p;
     const int tid = threadIdx.x;
@@ -120,73 +112,159 @@ __global__ __launch_bounds__(256) void k_diag_getrf32(cf* __restrict__ A, size_t
         A[(size_t)(k0 + c) * lda + k0 + r] = S[c * NBM + r];
     }
     if (tid < nb) lperm[tid] = pos[tid];
-    // L^-1 (unit lower): column c by forward substitution, thread c
-    if (tid < nb) {
-        const int c = tid;
-        for (int r = 0; r < nb; ++r) X[c * NBM + r] = mkf(r == c ? 1.f : 0.f, 0.f);
-        for (int i = c; i < nb; ++i) {
-            cf xi = X[c * NBM + i];
-            for (int r = i + 1; r < nb; ++r) X[c * NBM + r] = X[c * NBM + r] - S[i * NBM + r] * xi;
-        }
-    }
-    __syncthreads();
-    for (int t = tid; t < nb * nb; t += 256) Linv[t] = X[(t / nb) * NBM + t % nb];
-    __syncthreads();
-    // U^-1 (upper): column c by back substitution, thread c
-    if (tid < nb) {
-        const int c = tid;
-        for (int r = 0; r < nb; ++r) X[c * NBM + r] = mkf(r == c ? 1.f : 0.f, 0.f);
-        for (int i = c; i >= 0; --i) {
-            cf xi = X[c * NBM + i] * cinvf(S[i * NBM + i]);
-            X[c * NBM + i] = xi;
-            for (int r = 0; r < i; ++r) X[c * NBM + r] = X[c * NBM + r] - S[i * NBM + r] * xi;
-        }
-    }
-    __syncthreads();
-    for (int t = tid; t < nb * nb; t += 256) Uinv[t] = X[(t / nb) * NBM + t % nb];
 }
 
-// U12 = L11^-1 P A12 for columns [c0, c0+nc), one column per thread.
-__global__ __launch_bounds__(256) void k_u12_inv(cf* __restrict__ A, size_t lda, int k0, int nb,
-                                                 int c0, int nc, const int* __restrict__ lperm,
-                                                 const cf* __restrict__ Linv) {
-    __shared__ cf Li[NBM * NBM];
-    __shared__ int lp[NBM];
-    for (int t = threadIdx.x; t < nb * nb; t += 256) Li[(t / nb) * NBM + t % nb] = Linv[t];
-    if (threadIdx.x < nb) lp[threadIdx.x] = lperm[threadIdx.x];
+// L21 = A21 U11^-1 for rows [r0, r0+nr): a workgroup stages 32 rows x nb
+// columns in LDS; one wavefront per row, lane c = column c:
+// x_c = (a_c - sum_{i<c} x_i U_ic) / U_cc.
+constexpr int TRR_ROWS = 32;
+__global__ __launch_bounds__(256) void k_trsm_upper_rows(cf* __restrict__ A, size_t lda, int k0,
+                                                         int nb, int r0, int nr) {
+    __shared__ cf Ut[NBM * NBM];  // Ut[i*NBM + c] = U[i][c]
+    __shared__ cf T[NBM * (TRR_ROWS + 1)];
+    __shared__ cf dinv[NBM];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int t = tid; t < nb * nb; t += 256) {
+        int c = t / nb, i = t - c * nb;
+        Ut[i * NBM + c] = A[(size_t)(k0 + c) * lda + k0 + i];
+    }
+    const int rb = r0 + blockIdx.x * TRR_ROWS;
+    const int rn = min(TRR_ROWS, r0 + nr - rb);
+    for (int t = tid; t < nb * TRR_ROWS; t += 256) {
+        int c = t / TRR_ROWS, rho = t - c * TRR_ROWS;
+        T[c * (TRR_ROWS + 1) + rho] =
+            rho < rn ? A[(size_t)(k0 + c) * lda + rb + rho] : mkf(0.f, 0.f);
+    }
     __syncthreads();
-    const int c = c0 + blockIdx.x * 256 + threadIdx.x;
-    if (c >= c0 + nc) return;
-    cf* col = A + (size_t)c * lda + k0;
-    cf x[NBM];
-#pragma unroll
-    for (int r = 0; r < NBM; ++r) x[r] = r < nb ? col[lp[r < nb ? r : 0]] : mkf(0, 0);
-    for (int r = 0; r < nb; ++r) {
-        cf s = mkf(0, 0);
-#pragma unroll
-        for (int i = 0; i < NBM; ++i)
-            if (i <= r) s = s + Li[i * NBM + r] * x[i];
-        col[r] = s;
+    if (tid < nb) dinv[tid] = cinvf(Ut[tid * NBM + tid]);
+    __syncthreads();
+    for (int rho = wave; rho < rn; rho += 4) {
+        cf a = lane < nb ? T[lane * (TRR_ROWS + 1) + rho] : mkf(0.f, 0.f);
+        cf acc = mkf(0.f, 0.f), xf = mkf(0.f, 0.f);
+        for (int c = 0; c < nb; ++c) {
+            cf xv = (a - acc) * dinv[c];
+            cf xb = shfl_c(xv, c);
+            if (lane == c) xf = xb;
+            if (lane > c && lane < nb) acc = acc + xb * Ut[c * NBM + lane];
+        }
+        if (lane < nb) T[lane * (TRR_ROWS + 1) + rho] = xf;
+    }
+    __syncthreads();
+    for (int t = tid; t < nb * TRR_ROWS; t += 256) {
+        int c = t / TRR_ROWS, rho = t - c * TRR_ROWS;
+        if (rho < rn) A[(size_t)(k0 + c) * lda + rb + rho] = T[c * (TRR_ROWS + 1) + rho];
     }
 }
 
-// L21 = A21 U11^-1 for rows [r0, r0+nr), one row per thread.
-__global__ __launch_bounds__(256) void k_l21_inv(cf* __restrict__ A, size_t lda, int k0, int nb,
-                                                 int r0, int nr, const cf* __restrict__ Uinv) {
-    __shared__ cf Ui[NBM * NBM];
-    for (int t = threadIdx.x; t < nb * nb; t += 256) Ui[(t / nb) * NBM + t % nb] = Uinv[t];
-    __syncthreads();
-    const int r = r0 + blockIdx.x * 256 + threadIdx.x;
-    if (r >= r0 + nr) return;
-    cf x[NBM];
-#pragma unroll
-    for (int i = 0; i < NBM; ++i) x[i] = i < nb ? A[(size_t)(k0 + i) * lda + r] : mkf(0, 0);
-    for (int c = 0; c < nb; ++c) {
-        cf s = mkf(0, 0);
-#pragma unroll
-        for (int i = 0; i < NBM; ++i)
-            if (i <= c) s = s + x[i] * Ui[c * NBM + i];
-        A[(size_t)(k0 + c) * lda + r] = s;
+// Bounded wait until *flag >= target (see wait_count in lu.hip).
+__device__ inline void wait_ge(int* flag, int target, int* status) {
+    int it = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (++it > FLAG_SPIN_LIMIT ||
+            (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+             ST_SYNC_TIMEOUT)) {
+            atomicOr(status, ST_SYNC_TIMEOUT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+__device__ inline void publish(int* flag, int value) {
+    __hip_atomic_store(flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// y = L^-1 P b, block forward substitution.  Workgroup w owns blocks
+// w, w+P, ...; for its block it accumulates b - sum_k L[blk,k] y_k over the
+// finished blocks k < blk (waiting on flags[k] == epoch), applies the block
+// permutation, solves with the unit lower L_bb (wave 0, lane = row) and
+// publishes y_blk.  Blocks only wait on lower-numbered blocks, and every
+// workgroup is resident (cooperative launch), so the chain always drains.
+__global__ __launch_bounds__(256) void k_trsv_lower_coop(const cf* __restrict__ A, size_t lda,
+                                                         int n, int nblk,
+                                                         const int* __restrict__ lperm,
+                                                         const cd* __restrict__ b,
+                                                         cd* __restrict__ y, int* __restrict__ flags,
+                                                         int epoch, int* __restrict__ status) {
+    __shared__ cd acc[NBM];
+    __shared__ cd part[4][NBM];
+    const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
+    for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        const int r0 = blk * NBM, nb = min(NBM, n - r0);
+        if (tid < NBM) acc[tid] = tid < nb ? b[r0 + tid] : mk(0, 0);
+        __syncthreads();
+        for (int k = 0; k < blk; ++k) {
+            if (tid == 0) wait_ge(&flags[k], epoch, status);
+            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            cd s = mk(0, 0);
+            if (lane < nb) {
+                const cf* Lc = A + (size_t)(k * NBM + 16 * q) * lda + r0 + lane;
+                const cd* yk = y + k * NBM + 16 * q;
+#pragma unroll 4
+                for (int i = 0; i < 16; ++i) s = s + up(Lc[(size_t)i * lda]) * yk[i];
+            }
+            part[q][lane] = s;
+            __syncthreads();
+            if (tid < nb) acc[tid] = acc[tid] - ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]));
+            __syncthreads();
+        }
+        if (q == 0) {
+            cd v = lane < nb ? acc[lperm[(size_t)blk * NBM + lane]] : mk(0, 0);
+            for (int i = 0; i < nb - 1; ++i) {
+                cd vi = shfl_c(v, i);
+                if (lane > i && lane < nb) v = v - up(A[(size_t)(r0 + i) * lda + r0 + lane]) * vi;
+            }
+            if (lane < nb) y[r0 + lane] = v;
+        }
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) publish(&flags[blk], epoch);
+    }
+}
+
+// x = U^-1 y, block backward substitution (blocks in reverse order).
+__global__ __launch_bounds__(256) void k_trsv_upper_coop(const cf* __restrict__ A, size_t lda,
+                                                         int n, int nblk,
+                                                         const cd* __restrict__ yv,
+                                                         cd* __restrict__ x, int* __restrict__ flags,
+                                                         int epoch, int* __restrict__ status) {
+    __shared__ cd acc[NBM];
+    __shared__ cd part[4][NBM];
+    const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
+    for (int bi = blockIdx.x; bi < nblk; bi += gridDim.x) {
+        const int blk = nblk - 1 - bi;
+        const int r0 = blk * NBM, nb = min(NBM, n - r0);
+        if (tid < NBM) acc[tid] = tid < nb ? yv[r0 + tid] : mk(0, 0);
+        __syncthreads();
+        for (int k = nblk - 1; k > blk; --k) {
+            if (tid == 0) wait_ge(&flags[k], epoch, status);
+            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const int kb = min(NBM, n - k * NBM);
+            cd s = mk(0, 0);
+            if (lane < nb) {
+                const cf* Uc = A + (size_t)(k * NBM + 16 * q) * lda + r0 + lane;
+                const cd* xk = x + k * NBM + 16 * q;
+                for (int i = 0; i < 16 && 16 * q + i < kb; ++i) s = s + up(Uc[(size_t)i * lda]) * xk[i];
+            }
+            part[q][lane] = s;
+            __syncthreads();
+            if (tid < nb) acc[tid] = acc[tid] - ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]));
+            __syncthreads();
+        }
+        if (q == 0) {
+            cd v = lane < nb ? acc[lane] : mk(0, 0);
+            for (int i = nb - 1; i >= 0; --i) {
+                if (lane == i) v = v * cinv(up(A[(size_t)(r0 + i) * lda + r0 + i]));
+                cd vi = shfl_c(v, i);
+                if (lane < i) v = v - up(A[(size_t)(r0 + i) * lda + r0 + lane]) * vi;
+            }
+            if (lane < nb) x[r0 + lane] = v;
+        }
+        __threadfence();
+        __syncthreads();
+        if (tid == 0) publish(&flags[blk], epoch);
     }
 }
 
@@ -305,58 +383,6 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16x3(int m, int nc, int K,
 }
 
 // ---------------------------------------------------------------------------
-// Preconditioner solve M z = v with the FP32 factors, FP64 vectors.
-// Forward step k: y_k = L11^-1 P_k w_k, w[r] -= L21[r,:] y_k (r >= k0+nb);
-// backward step k: x_k = U11^-1 y_k, y[r] -= U[r, block k] x_k (r < k0).
-// The steps write disjoint rows from the ones they read, so no copy of the
-// vector is needed between launches.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_mfwd(const cf* __restrict__ A, size_t lda, int n, int k0,
-                                              int nb, const int* __restrict__ lperm,
-                                              const cf* __restrict__ Linv, cd* __restrict__ w,
-                                              cd* __restrict__ y) {
-    __shared__ cd bk[NBM], yk[NBM];
-    const int tid = threadIdx.x;
-    if (tid < nb) bk[tid] = w[k0 + lperm[tid]];
-    __syncthreads();
-    if (tid < nb) {
-        cd s = mk(0, 0);
-        for (int i = 0; i <= tid; ++i) s = s + up(Linv[i * nb + tid]) * bk[i];
-        yk[tid] = s;
-    }
-    __syncthreads();
-    if (blockIdx.x == 0 && tid < nb) y[k0 + tid] = yk[tid];
-    const int row = k0 + nb + blockIdx.x * 256 + tid;
-    if (row < n) {
-        cd s = w[row];
-        for (int i = 0; i < nb; ++i) s = s - up(A[(size_t)(k0 + i) * lda + row]) * yk[i];
-        w[row] = s;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_mbwd(const cf* __restrict__ A, size_t lda, int k0, int nb,
-                                              const cf* __restrict__ Uinv, cd* __restrict__ y,
-                                              cd* __restrict__ x) {
-    __shared__ cd bk[NBM], xk[NBM];
-    const int tid = threadIdx.x;
-    if (tid < nb) bk[tid] = y[k0 + tid];
-    __syncthreads();
-    if (tid < nb) {
-        cd s = mk(0, 0);
-        for (int i = tid; i < nb; ++i) s = s + up(Uinv[i * nb + tid]) * bk[i];
-        xk[tid] = s;
-    }
-    __syncthreads();
-    if (blockIdx.x == 0 && tid < nb) x[k0 + tid] = xk[tid];
-    const int row = blockIdx.x * 256 + tid;
-    if (row < k0) {
-        cd s = y[row];
-        for (int i = 0; i < nb; ++i) s = s - up(A[(size_t)(k0 + i) * lda + row]) * xk[i];
-        y[row] = s;
-    }
-}
-
-// ---------------------------------------------------------------------------
 // FP64 vector kernels of the refinement.
 // ---------------------------------------------------------------------------
 constexpr int MV_COLS = 128;  // columns per GEMV chunk
@@ -462,14 +488,15 @@ __global__ void k_fill_i(int* p, int v, int cnt) {
     } while (0)
 
 // Workspace slots 9.. of LuWork (0..8 belong to the FP64 path).
-enum { M_A32 = 9, M_PERM, M_INV, M_STATUS, M_VEC, M_V, M_PART, M_HOST };
+enum { M_A32 = 9, M_PERM, M_FLAGS, M_STATUS, M_VEC, M_V, M_PART, M_HOST, M_PTMP };
 
 struct MixedFactor {
     cf* A = nullptr;
     int* lperm = nullptr;
-    cf* Linv = nullptr;
-    cf* Uinv = nullptr;
-    int n = 0, nblk = 0;
+    int* flags = nullptr;
+    int* status = nullptr;
+    int* epoch = nullptr;   // host-side counter living in LuWork
+    int n = 0, nblk = 0, coop = 1;
 };
 
 static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda,
@@ -482,29 +509,40 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     MX_CHK(e);
     F.lperm = (int*)w.get(M_PERM, sizeof(int) * (size_t)nblk * NBM, &e);
     MX_CHK(e);
-    cf* inv = (cf*)w.get(M_INV, sizeof(cf) * (size_t)nblk * NBM * NBM * 2, &e);
+    const size_t cap_before = w.cap[M_FLAGS];
+    F.flags = (int*)w.get(M_FLAGS, sizeof(int) * (size_t)nblk, &e);
     MX_CHK(e);
-    F.Linv = inv;
-    F.Uinv = inv + (size_t)nblk * NBM * NBM;
-    int* status = (int*)w.get(M_STATUS, sizeof(int), &e);
+    if (w.flags_init != (void*)F.flags || w.cap[M_FLAGS] != cap_before) {
+        // fresh flag array: clear it once
+        hipLaunchKernelGGL(k_fill_i, dim3((nblk + 255) / 256), dim3(256), 0, st, F.flags, 0, nblk);
+        w.flags_init = F.flags;
+    }
+    F.epoch = &w.epoch;
+    F.status = (int*)w.get(M_STATUS, sizeof(int), &e);
     MX_CHK(e);
-    hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, st, status, 0, 1);
+    hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, st, F.status, 0, 1);
+    int dev = 0, ncu = 0, per = 0;
+    MX_CHK(hipGetDevice(&dev));
+    MX_CHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trsv_lower_coop, 256, 0));
+    int per2 = 0;
+    MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_trsv_upper_coop, 256, 0));
+    F.coop = std::max(1, std::min(nblk, ncu * std::min(per, per2)));
     size_t tot = (size_t)n * n;
     hipLaunchKernelGGL(k_cvt_z2c, dim3((tot + 255) / 256), dim3(256), 0, st, A64, lda, F.A,
                        (size_t)n, n);
     for (int k = 0; k < nblk; ++k) {
         const int k0 = k * NBM, nb = std::min(NBM, n - k0);
-        cf* Li = F.Linv + (size_t)k * NBM * NBM;
-        cf* Ui = F.Uinv + (size_t)k * NBM * NBM;
         int* lp = F.lperm + (size_t)k * NBM;
         hipLaunchKernelGGL(k_diag_getrf32, dim3(1), dim3(256), 0, st, F.A, (size_t)n, k0, nb, lp,
-                           Li, Ui, status);
+                           F.status);
         const int c0 = k0 + nb, rest = n - c0;
         if (rest <= 0) continue;
-        hipLaunchKernelGGL(k_u12_inv, dim3((rest + 255) / 256), dim3(256), 0, st, F.A, (size_t)n,
-                           k0, nb, c0, rest, lp, Li);
-        hipLaunchKernelGGL(k_l21_inv, dim3((rest + 255) / 256), dim3(256), 0, st, F.A, (size_t)n,
-                           k0, nb, c0, rest, Ui);
+        hipLaunchKernelGGL((k_trsm_lower_cols<cf, true>),
+                           dim3((rest + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG), dim3(256), 0,
+                           st, F.A, (size_t)n, k0, nb, c0, rest, (const int*)lp);
+        hipLaunchKernelGGL(k_trsm_upper_rows, dim3((rest + TRR_ROWS - 1) / TRR_ROWS), dim3(256),
+                           0, st, F.A, (size_t)n, k0, nb, c0, rest);
         hipLaunchKernelGGL(k_cgemm_bf16x3, dim3((rest + CT - 1) / CT, (rest + CT - 1) / CT),
                            dim3(256), 0, st, rest, rest, nb, F.A + (size_t)k0 * n + c0, (size_t)n,
                            F.A + (size_t)c0 * n + k0, (size_t)n, F.A + (size_t)c0 * n + c0,
@@ -512,7 +550,7 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     }
     MX_CHK(hipGetLastError());
     int hs = 0;
-    MX_CHK(hipMemcpyAsync(&hs, status, sizeof(int), hipMemcpyDeviceToHost, st));
+    MX_CHK(hipMemcpyAsync(&hs, F.status, sizeof(int), hipMemcpyDeviceToHost, st));
     MX_CHK(hipStreamSynchronize(st));
     if (hs & ST_PIVOT_ZERO) {
         err = "mixed solve: zero pivot in a diagonal block";
@@ -521,24 +559,24 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     return BEMX_OK;
 }
 
-// z = M^-1 v   (tmp: work vector of n)
-static void precond(hipStream_t st, const MixedFactor& F, const cd* v, cd* z, cd* tmp) {
-    const int n = F.n;
-    hipLaunchKernelGGL(k_copy_z, dim3((n + 255) / 256), dim3(256), 0, st, v, tmp, n);
-    for (int k = 0; k < F.nblk; ++k) {
-        const int k0 = k * NBM, nb = std::min(NBM, n - k0);
-        const int rows = std::max(1, n - k0 - nb);
-        hipLaunchKernelGGL(k_mfwd, dim3((rows + 255) / 256), dim3(256), 0, st, F.A, (size_t)n, n,
-                           k0, nb, F.lperm + (size_t)k * NBM, F.Linv + (size_t)k * NBM * NBM, tmp,
-                           z);
-    }
-    for (int k = F.nblk - 1; k >= 0; --k) {
-        const int k0 = k * NBM, nb = std::min(NBM, n - k0);
-        const int rows = std::max(1, k0);
-        hipLaunchKernelGGL(k_mbwd, dim3((rows + 255) / 256), dim3(256), 0, st, F.A, (size_t)n, k0,
-                           nb, F.Uinv + (size_t)k * NBM * NBM, z, tmp);
-    }
-    hipLaunchKernelGGL(k_copy_z, dim3((n + 255) / 256), dim3(256), 0, st, tmp, z, n);
+// z = M^-1 v = U^-1 L^-1 P v   (tmp: work vector of n; z may alias v)
+static hipError_t precond(hipStream_t st, const MixedFactor& F, const cd* v, cd* z, cd* tmp) {
+    int n = F.n, nblk = F.nblk;
+    size_t lda = (size_t)n;
+    const cf* A = F.A;
+    const int* lp = F.lperm;
+    int* flags = F.flags;
+    int* status = F.status;
+    int e1 = ++*F.epoch;
+    void* a1[] = {(void*)&A, &lda, &n, &nblk, (void*)&lp, (void*)&v, &tmp, &flags, &e1, &status};
+    hipError_t e = hipLaunchCooperativeKernel((const void*)k_trsv_lower_coop, dim3(F.coop),
+                                              dim3(256), a1, 0, st);
+    if (e != hipSuccess) return e;
+    int e2 = ++*F.epoch;
+    const cd* tv = tmp;
+    void* a2[] = {(void*)&A, &lda, &n, &nblk, (void*)&tv, &z, &flags, &e2, &status};
+    return hipLaunchCooperativeKernel((const void*)k_trsv_upper_coop, dim3(F.coop), dim3(256), a2,
+                                      0, st);
 }
 
 static void gemv(hipStream_t st, int n, const cd* A, size_t lda, const cd* x, const cd* b, cd* r,
@@ -592,7 +630,7 @@ int gmres_ir(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda, cd* b,
     // norms for the backward error: ||A||_inf estimate via |A| row sums is
     // costly; use ||b|| and the residual relative to ||A x|| instead.
     hipLaunchKernelGGL(k_copy_z, dim3(nb256), dim3(256), 0, st, b, bsave, n);
-    precond(st, F, bsave, x, tmp);  // x0 = M^-1 b
+    MX_CHK(precond(st, F, bsave, x, tmp));  // x0 = M^-1 b
     std::vector<zx> h;
     double nrm2 = 0, bnorm = 0, relres = 1.0;
     int it_total = 0;
@@ -615,7 +653,7 @@ int gmres_ir(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda, cd* b,
             break;
         }
         // GMRES on M^-1 A d = M^-1 r
-        precond(st, F, r, V, tmp);
+        MX_CHK(precond(st, F, r, V, tmp));
         double beta2 = 0;
         dots(V, 0, V, h, beta2);
         double beta = std::sqrt(beta2);
@@ -632,7 +670,7 @@ int gmres_ir(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda, cd* b,
         for (; k < MR; ++k) {
             ++it_total;
             gemv(st, n, A64, lda, V + (size_t)k * n, nullptr, wv, part, 0);
-            precond(st, F, wv, wv, tmp);
+            MX_CHK(precond(st, F, wv, wv, tmp));
             // classical Gram-Schmidt, two passes
             std::vector<zx> hcol(k + 1, zx(0, 0));
             double wn2 = 0;
@@ -693,6 +731,13 @@ int gmres_ir(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda, cd* b,
         hipLaunchKernelGGL(k_combine, dim3(nb256), dim3(256), 0, st, n, V, (size_t)n, k, hdev, x);
     }
     MX_CHK(hipGetLastError());
+    int hst = 0;
+    MX_CHK(hipMemcpyAsync(&hst, F.status, sizeof(int), hipMemcpyDeviceToHost, st));
+    MX_CHK(hipStreamSynchronize(st));
+    if (hst & ST_SYNC_TIMEOUT) {
+        err = "mixed solve: a device-side wait of the triangular solves timed out";
+        return BEMX_ERR_TIMEOUT;
+    }
     if (stats) {
         stats[0] = it_total;
         stats[1] = relres;
