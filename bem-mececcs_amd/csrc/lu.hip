@@ -48,7 +48,7 @@ void LuWork::release() {
 }
 
 constexpr int NB = 64;          // panel width (FP64 path)
-constexpr int PANEL_ROWS = 64;  // rows of the panel per workgroup (64 x 64 x 16 B = 64 KB LDS)
+constexpr int PANEL_ROWS = 128;  // rows of the panel per workgroup (128 x 64 x 16 B = 128 KB LDS)
 constexpr int SPIN_LIMIT = 1 << 22;
 
 struct Cand {
@@ -69,7 +69,8 @@ __device__ inline bool better(double v, int r, double bv, int br) {
 // every wave of the grid always runs to its end.
 __device__ inline void wait_count(int* cnt, int target, int* status) {
     int it = 0;
-    while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    // relaxed polls; the caller issues one acquire fence after the wait
+    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
         if (++it > SPIN_LIMIT ||
             (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
              ST_SYNC_TIMEOUT)) {
