@@ -29,6 +29,7 @@ EXPORTS = [
     "bemx_create", "bemx_destroy", "bemx_last_error", "bemx_gauleg", "bemx_send_shared_data",
     "bemx_deallocate_shared_data", "bemx_ghmatece", "bemx_send_dyn_data", "bemx_ghmatecd",
     "bemx_linsolve", "bemx_interec", "bemx_solve", "bemx_device_buffers", "bemx_last_stats",
+    "bemx_send_load_cases",
 ]
 
 
@@ -54,6 +55,7 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
     lib.bemx_deallocate_shared_data.argtypes = [vp]
     lib.bemx_ghmatece.argtypes = [vp, C.c_double, C.c_double] + [_dp] * 8
     lib.bemx_send_dyn_data.argtypes = [vp, _dp, _ip]
+    lib.bemx_send_load_cases.argtypes = [vp, C.c_int, _dp, _ip]
     lib.bemx_ghmatecd.argtypes = [vp, _dp, _dp, _dp] + [C.c_double] * 5 + [_dp] * 4
     lib.bemx_linsolve.argtypes = [vp, C.c_int, _dp, _dp, C.c_int, C.POINTER(C.c_longlong)]
     lib.bemx_interec.argtypes = [vp, _dp, _dp, _ip, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -177,6 +179,18 @@ class Context:
         kode = np.ascontiguousarray(kode, dtype=np.int32)
         code = self._lib.bemx_send_dyn_data(self.h, _p(zdfi.view(np.float64)), _pi(kode))
         self._check(code, "send_dyn_data")
+        self.nrhs = 1
+
+    def send_load_cases(self, zdfi, kode):
+        """zdfi: (3NBE, nrhs) complex, one load case per column (1..4)."""
+        zdfi = np.asfortranarray(np.asarray(zdfi, dtype=np.complex128).reshape(len(kode), -1))
+        kode = np.ascontiguousarray(kode, dtype=np.int32)
+        nrhs = zdfi.shape[1]
+        code = self._lib.bemx_send_load_cases(self.h, nrhs,
+                                              _p(zdfi.reshape(-1, order="F").view(np.float64)),
+                                              _pi(kode))
+        self._check(code, "send_load_cases")
+        self.nrhs = nrhs
 
     # GHMATECD (Ghmatecd.for:11-239)
     def ghmatecd(self, zmat, fr: float, consts=None, hestdiag=None, gestdiag=None,
@@ -185,7 +199,8 @@ class Context:
         nn = 3 * self.mesh.nbe
         consts = self.consts if consts is None else consts
         zge, zcs, zcp = (_cz(v) for v in zmat)
-        zfi = np.zeros(nn, dtype=np.complex128)
+        nr = getattr(self, "nrhs", 1)
+        zfi = np.zeros(nn * nr, dtype=np.complex128)
         zh = np.zeros((nn, nn), dtype=np.complex128, order="F") if want_zh else None
         hd = gd = None
         if hestdiag is not None:
@@ -198,7 +213,7 @@ class Context:
                                        if want_zh else None)
         self._check(code, "ghmatecd")
         self._last_zh = zh
-        return zh, zfi
+        return zh, (zfi if nr == 1 else zfi.reshape(nn, nr, order="F"))
 
     # LINSOLVE (Linsolve.for:1-66)
     def linsolve(self, zh, zfi, mode: int = MODE_FP64):
@@ -244,13 +259,18 @@ class Context:
 
     # device-resident solve of one frequency
     def solve(self, zmat, fr: float, mode: int = MODE_FP64, want_x: bool = True):
+        """-> (x, stats); x is (3NBE,) for one load case, (3NBE, nrhs) for
+        several (bemx_send_load_cases)."""
         nn = 3 * self.mesh.nbe
+        nr = getattr(self, "nrhs", 1)
         zge, zcs, zcp = (_cz(v) for v in zmat)
-        x = np.zeros(nn, dtype=np.complex128) if want_x else None
+        x = np.zeros(nn * nr, dtype=np.complex128) if want_x else None
         stats = np.zeros(4)
         code = self._lib.bemx_solve(self.h, _p(zge), _p(zcs), _p(zcp), fr, mode,
                                     _p(x.view(np.float64)) if want_x else None, _p(stats))
         self._check(code, "solve")
+        if want_x and nr > 1:
+            x = x.reshape(nn, nr, order="F")
         return x, stats
 
 

@@ -11,8 +11,9 @@
 //                    Singge.for:11-202)
 //   k_dyn_diag       SING_DE / SOLFUNDIF + HESTdiag/GESTdiag (Ghmatecd.for:139-153,
 //                    Sing_de.for, Solfundif.for)
-//   k_dyn_offdiag    NONSINGD / SOLFUND (Ghmatecd.for:155-162, Nonsingd.for,
-//                    Solfund.for), the KODE column swap (Ghmatecd.for:168,
+//   k_dyn_offdiag_h  NONSINGD / SOLFUND (Ghmatecd.for:155-162, Nonsingd.for,
+//                    Solfund.for) for the H part of every column
+//   k_dyn_offdiag_g  the G part of the KODE-swapped columns (Ghmatecd.for:168,
 //                    316-333) and the fused RHS ZFI = ZG ZDFI (Ghmatecd.for:172)
 //   k_interior       INTEREC displacements and SIGMAEC stresses
 //                    (Interec.for:86-135, 188-283, Sigmaec.for)
@@ -283,37 +284,36 @@ __device__ inline void dyn_scalars(const DynConst& K, double R, double iR, cd& p
     C = K.c1m2 * (A + 0.5 * B - 3.0 * chr) - 2.0 * chr;
 }
 
-// Writes the 3x3 block of column element J for row element I after the KODE
-// swap (Ghmatecd.for:316-333 as called at :168): for a KODE=0 column c,
-// H'(:,c) = -G(:,c) and G'(:,c) = -mu* H(:,c).  Returns G' . zdfi_J in b.
-__device__ inline void swap_and_rhs(const cd H[9], const cd G[9], int flag, const cd* zdfi3,
-                                    cd zge, cd Hout[9], cd b[3], bool need_rhs) {
+// KODE column swap of one 3x3 block (Ghmatecd.for:316-333 as called at
+// :168): for a KODE=0 column c, H'(:,c) = -G(:,c) and G'(:,c) = -mu* H(:,c).
+__device__ inline void swap_cols(const cd H[9], const cd G[9], int flag, cd zge, cd Hout[9],
+                                 cd Gout[9]) {
     for (int k = 0; k < 3; ++k) {
         bool sw = (flag >> (1 + k)) & 1;
-        cd t = need_rhs ? zdfi3[k] : mk(0.0, 0.0);
         for (int i = 0; i < 3; ++i) {
-            cd h = H[i + 3 * k];
-            cd g = G[i + 3 * k];
+            cd h = H[i + 3 * k], g = G[i + 3 * k];
             Hout[i + 3 * k] = sw ? -g : h;
-            cd gn = sw ? -(h * zge) : g;
-            b[i] = b[i] + gn * t;
+            Gout[i + 3 * k] = sw ? -(h * zge) : g;
         }
     }
 }
 
+constexpr int MAXRHS = 4;  // load cases per assembly
+
 // ---------------------------------------------------------------------------
 // Dynamic diagonal blocks: plain Gauss of (dynamic - static) kernels over the
 // element itself, collocation at its centroid, plus HESTdiag/GESTdiag.
-// One wavefront per element, one lane per Gauss point.
+// One wavefront per element, one lane per Gauss point.  Writes the swapped
+// block and, per load case c, G'_II ZDFI_c(I) into rhs_diag[c].
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_dyn_diag(int nbe, int npg2, const ElemData* __restrict__ elem,
                                                  const GaussPt* __restrict__ gp,
                                                  const double* __restrict__ hdiag,
                                                  const double* __restrict__ gdiag,
                                                  const int* __restrict__ colflag,
-                                                 const cd* __restrict__ zdfi, DynConst K,
-                                                 cd* __restrict__ zh, size_t ldzh,
-                                                 cd* __restrict__ rhs_diag) {
+                                                 const cd* __restrict__ zdfi, int nrhs,
+                                                 size_t ldr, DynConst K, cd* __restrict__ zh,
+                                                 size_t ldzh, cd* __restrict__ rhs_diag) {
     int I = blockIdx.x;
     int lane = threadIdx.x;
     ElemData e = elem[I];
@@ -353,15 +353,29 @@ __global__ __launch_bounds__(64) void k_dyn_diag(int nbe, int npg2, const ElemDa
         accG[k] = wave_sum(accG[k]);
     }
     if (lane != 0) return;
-    cd H[9], G[9], Ho[9], b[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
+    cd H[9], G[9], Ho[9], Gn[9];
     for (int k = 0; k < 9; ++k) {
         H[k] = mk(accH[2 * k] + hdiag[(size_t)I * 9 + k], accH[2 * k + 1]);
         G[k] = mk(accG[2 * k] + gdiag[(size_t)I * 9 + k], accG[2 * k + 1]);
     }
-    swap_and_rhs(H, G, colflag[I], zdfi + 3 * I, K.zge, Ho, b, true);
+    swap_cols(H, G, colflag[I], K.zge, Ho, Gn);
     for (int k = 0; k < 3; ++k)
         for (int i = 0; i < 3; ++i) zh[(size_t)(3 * I + k) * ldzh + 3 * I + i] = Ho[i + 3 * k];
-    for (int i = 0; i < 3; ++i) rhs_diag[3 * I + i] = b[i];
+    for (int c = 0; c < nrhs; ++c) {
+        const cd* t = zdfi + c * ldr + 3 * I;
+        for (int i = 0; i < 3; ++i)
+            rhs_diag[c * ldr + 3 * I + i] = Gn[i] * t[0] + Gn[i + 3] * t[1] + Gn[i + 6] * t[2];
+    }
+}
+
+// 1/sqrt(x) to FP64 accuracy: v_rsq_f64 refined by two Newton steps.
+__device__ inline double rsqrt_refined(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double h = x * y;
+    y = fma(y * 0.5, fma(-h, y, 1.0), y);
+    h = x * y;
+    y = fma(y * 0.5, fma(-h, y, 1.0), y);
+    return y;
 }
 
 // ---------------------------------------------------------------------------
@@ -371,9 +385,8 @@ __global__ __launch_bounds__(64) void k_dyn_diag(int nbe, int npg2, const ElemDa
 // address (broadcast).  Per element J the lane accumulates H in the factored
 // form  H_ij = d_ij SD + n_i SP_j + SRQ_ij  (SD = sum A' drn, SP = sum A' r_j,
 // SRQ = sum r_i (B' drn r_j + C' n_j), primes = times w/(4 pi)), which is
-// the Gauss sum of Solfund.for:58-59 with n fixed per element.  Only elements
-// flagged needs-G (nonzero traction or a KODE=0 column) also integrate U; G
-// is never stored: it is folded into the RHS partial rhs_part[tile][row].
+// the Gauss sum of Solfund.for:58-59 with n fixed per element.  With NeedG
+// it also integrates U (psi, chi) into G.
 // ---------------------------------------------------------------------------
 template <bool NeedG>
 __device__ inline void offdiag_element(const DynConst& K, const GaussPt* __restrict__ sg, int npg2,
@@ -386,8 +399,8 @@ __device__ inline void offdiag_element(const DynConst& K, const GaussPt* __restr
     for (int q = 0; q < npg2; ++q) {
         GaussPt p = sg[q];
         double r1 = p.x - xc, r2 = p.y - yc, r3 = p.z - zc;
-        double R = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
-        double iR = 1.0 / R;
+        double iR = rsqrt_refined(r1 * r1 + r2 * r2 + r3 * r3);
+        double R = (r1 * r1 + r2 * r2 + r3 * r3) * iR;
         double rd[3] = {r1 * iR, r2 * iR, r3 * iR};
         double drn = rd[0] * nv[0] + rd[1] * nv[1] + rd[2] * nv[2];
         cd psi, chi, A, B, C;
@@ -427,15 +440,15 @@ __device__ inline void offdiag_element(const DynConst& K, const GaussPt* __restr
         }
 }
 
+// H of every off-diagonal block; columns with KODE = 0 are left to
+// k_dyn_offdiag_g, which writes their swapped content.
 template <int WAVES, int JT>
-__global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag(int nbe, int npg2,
-                                                            const ElemData* __restrict__ elem,
-                                                            const GaussPt* __restrict__ gp,
-                                                            const int* __restrict__ colflag,
-                                                            const cd* __restrict__ zdfi,
-                                                            DynConst K, cd* __restrict__ zh,
-                                                            size_t ldzh,
-                                                            cd* __restrict__ rhs_part) {
+__global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag_h(int nbe, int npg2,
+                                                              const ElemData* __restrict__ elem,
+                                                              const GaussPt* __restrict__ gp,
+                                                              const int* __restrict__ colflag,
+                                                              DynConst K, cd* __restrict__ zh,
+                                                              size_t ldzh) {
     extern __shared__ GaussPt sgp[];
     const int j0 = blockIdx.y * JT;
     const int jn = min(JT, nbe - j0);
@@ -445,67 +458,132 @@ __global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag(int nbe, int npg2,
     const bool rowok = I < nbe;
     const int Ic = rowok ? I : nbe - 1;
     const double xc = elem[Ic].cx, yc = elem[Ic].cy, zc = elem[Ic].cz;
-    cd b[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
     for (int jj = 0; jj < jn; ++jj) {
         const int J = j0 + jj;
         const int flag = colflag[J];
         const double nv[3] = {elem[J].nx, elem[J].ny, elem[J].nz};
-        cd H[9], G[9], Ho[9];
-        const bool needg = flag & 1;
-        if (needg)
-            offdiag_element<true>(K, sgp + jj * npg2, npg2, xc, yc, zc, nv, H, G);
-        else
-            offdiag_element<false>(K, sgp + jj * npg2, npg2, xc, yc, zc, nv, H, G);
-        const bool own = rowok && (I != J);
-        if (needg) {
-            cd bb[3] = {mk(0, 0), mk(0, 0), mk(0, 0)};
-            swap_and_rhs(H, G, flag, zdfi + 3 * J, K.zge, Ho, bb, true);
-            if (own)
-                for (int i = 0; i < 3; ++i) b[i] = b[i] + bb[i];
-        } else {
-            for (int k = 0; k < 9; ++k) Ho[k] = H[k];
-        }
-        if (own) {
+        cd H[9], G[9];
+        offdiag_element<false>(K, sgp + jj * npg2, npg2, xc, yc, zc, nv, H, G);
+        if (rowok && I != J) {
             for (int k = 0; k < 3; ++k) {
+                if ((flag >> (1 + k)) & 1) continue;
                 cd* col = zh + (size_t)(3 * J + k) * ldzh + 3 * I;
-                col[0] = Ho[3 * k];
-                col[1] = Ho[3 * k + 1];
-                col[2] = Ho[3 * k + 2];
+                col[0] = H[3 * k];
+                col[1] = H[3 * k + 1];
+                col[2] = H[3 * k + 2];
             }
         }
     }
-    if (rowok) {
-        cd* o = rhs_part + (size_t)blockIdx.y * 3 * nbe + 3 * I;
-        o[0] = b[0];
-        o[1] = b[1];
-        o[2] = b[2];
-    }
 }
 
-// ZFI = diagonal part + sum of tile partials, tiles summed in order.
-__global__ void k_rhs_reduce(int nn, int ntiles, const cd* __restrict__ part,
-                             const cd* __restrict__ diag, cd* __restrict__ zfi) {
+// Flagged elements (nonzero traction in some load case, or a KODE = 0
+// column): H and G, the swapped KODE = 0 columns, and per load case c the
+// partial RHS  sum_J G'_IJ ZDFI_c(J)  of this chunk of the flagged list into
+// rhs_part[chunk][c][row].
+template <int WAVES, int JTG>
+__global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag_g(
+    int nbe, int npg2, const ElemData* __restrict__ elem, const GaussPt* __restrict__ gp,
+    const int* __restrict__ colflag, const int* __restrict__ flist, const int* __restrict__ fcount,
+    const cd* __restrict__ zdfi, int nrhs, size_t ldr, DynConst K, cd* __restrict__ zh,
+    size_t ldzh, cd* __restrict__ rhs_part) {
+    extern __shared__ GaussPt sgp[];
+    const int cnt = *fcount;
+    const int f0 = blockIdx.y * JTG;
+    if (f0 >= cnt) return;  // uniform over the workgroup
+    const int fn = min(JTG, cnt - f0);
+    for (int t = threadIdx.x; t < fn * npg2; t += 64 * WAVES) {
+        int jj = t / npg2, q = t - jj * npg2;
+        sgp[t] = gp[(size_t)flist[f0 + jj] * npg2 + q];
+    }
+    __syncthreads();
+    const int I = blockIdx.x * (64 * WAVES) + threadIdx.x;
+    const bool rowok = I < nbe;
+    const int Ic = rowok ? I : nbe - 1;
+    const double xc = elem[Ic].cx, yc = elem[Ic].cy, zc = elem[Ic].cz;
+    cd b[MAXRHS][3];
+    for (int c = 0; c < MAXRHS; ++c) b[c][0] = b[c][1] = b[c][2] = mk(0, 0);
+    for (int jj = 0; jj < fn; ++jj) {
+        const int J = flist[f0 + jj];
+        const int flag = colflag[J];
+        const double nv[3] = {elem[J].nx, elem[J].ny, elem[J].nz};
+        cd H[9], G[9], Ho[9], Gn[9];
+        offdiag_element<true>(K, sgp + jj * npg2, npg2, xc, yc, zc, nv, H, G);
+        swap_cols(H, G, flag, K.zge, Ho, Gn);
+        if (!rowok || I == J) continue;
+        for (int k = 0; k < 3; ++k) {
+            if (!((flag >> (1 + k)) & 1)) continue;
+            cd* col = zh + (size_t)(3 * J + k) * ldzh + 3 * I;
+            col[0] = Ho[3 * k];
+            col[1] = Ho[3 * k + 1];
+            col[2] = Ho[3 * k + 2];
+        }
+        for (int c = 0; c < nrhs; ++c) {
+            const cd* t = zdfi + c * ldr + 3 * J;
+            cd t0 = t[0], t1 = t[1], t2 = t[2];
+            for (int i = 0; i < 3; ++i) b[c][i] = b[c][i] + Gn[i] * t0 + Gn[i + 3] * t1 + Gn[i + 6] * t2;
+        }
+    }
+    if (rowok)
+        for (int c = 0; c < nrhs; ++c) {
+            cd* o = rhs_part + ((size_t)blockIdx.y * nrhs + c) * ldr + 3 * I;
+            o[0] = b[c][0];
+            o[1] = b[c][1];
+            o[2] = b[c][2];
+        }
+}
+
+// ZFI_c = diagonal part + the chunk partials of the flagged list, chunks
+// summed in order (deterministic).
+__global__ void k_rhs_reduce(int nn, int nrhs, size_t ldr, int jtg, const int* __restrict__ fcount,
+                             const cd* __restrict__ part, const cd* __restrict__ diag,
+                             cd* __restrict__ zfi) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nn) return;
-    cd s = diag[r];
-    for (int t = 0; t < ntiles; ++t) s = s + part[(size_t)t * nn + r];
-    zfi[r] = s;
+    int c = blockIdx.y;
+    if (r >= nn || c >= nrhs) return;
+    const int chunks = (*fcount + jtg - 1) / jtg;
+    cd s = diag[c * ldr + r];
+    for (int t = 0; t < chunks; ++t) s = s + part[((size_t)t * nrhs + c) * ldr + r];
+    zfi[c * ldr + r] = s;
 }
 
-// Column flags from KODE and ZDFI: bit 0 = element needs G (nonzero traction
-// or a KODE=0 column), bit 1+k = column 3J+k has KODE = 0.
-__global__ void k_colflags(int nbe, const int* __restrict__ kode, const cd* __restrict__ zdfi,
-                           int* __restrict__ flag) {
-    int J = blockIdx.x * blockDim.x + threadIdx.x;
-    if (J >= nbe) return;
-    int f = 0;
-    for (int k = 0; k < 3; ++k) {
-        cd t = zdfi[3 * J + k];
-        bool sw = kode[3 * J + k] == 0;
-        if (sw) f |= (1 << (1 + k)) | 1;
-        if (t.x != 0.0 || t.y != 0.0) f |= 1;
+// Column flags from KODE and the load cases (bit 0: the element needs G --
+// nonzero traction in some case or a KODE = 0 column; bit 1+k: column 3J+k
+// has KODE = 0) and the ascending list of flagged elements.  One workgroup.
+__global__ __launch_bounds__(256) void k_colflags(int nbe, const int* __restrict__ kode,
+                                                  const cd* __restrict__ zdfi, int nrhs, size_t ldr,
+                                                  int* __restrict__ flag, int* __restrict__ flist,
+                                                  int* __restrict__ fcount) {
+    __shared__ int wcnt[4];
+    __shared__ int base;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) base = 0;
+    __syncthreads();
+    for (int j0 = 0; j0 < nbe; j0 += 256) {
+        const int J = j0 + tid;
+        int f = 0;
+        if (J < nbe) {
+            for (int k = 0; k < 3; ++k) {
+                if (kode[3 * J + k] == 0) f |= (1 << (1 + k)) | 1;
+                for (int c = 0; c < nrhs; ++c) {
+                    cd t = zdfi[c * ldr + 3 * J + k];
+                    if (t.x != 0.0 || t.y != 0.0) f |= 1;
+                }
+            }
+            flag[J] = f;
+        }
+        const bool on = f & 1;
+        unsigned long long m = __ballot(on);
+        int below = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wcnt[wave] = __popcll(m);
+        __syncthreads();
+        int off = base;
+        for (int w = 0; w < wave; ++w) off += wcnt[w];
+        if (on) flist[off + below] = J;
+        __syncthreads();
+        if (tid == 0) base += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        __syncthreads();
     }
-    flag[J] = f;
+    if (tid == 0) *fcount = base;
 }
 
 // INTEREC's un-swap (Interec.for:59-67): for KODE = 0, tmp = ZFI mu*,
@@ -700,28 +778,35 @@ hipError_t launch_gest_diag(hipStream_t st, int n, int nbe, const int* cone, con
 
 constexpr int OD_WAVES = 4;
 constexpr int OD_JT = 8;
+constexpr int OG_JT = 4;
 
-int offdiag_tiles(int nbe) { return (nbe + OD_JT - 1) / OD_JT; }
+int offdiag_tiles(int nbe) { return (nbe + OG_JT - 1) / OG_JT; }
 
+// Dynamic assembly of ZH and of ZFI for nrhs load cases (zdfi, rhs_diag,
+// zfi: nrhs columns of leading dimension ldr = 3 NBE).  flist/fcount: NBE + 1
+// ints of scratch; rhs_part: offdiag_tiles(nbe) * nrhs * ldr complex.
 hipError_t launch_dyn_assembly(hipStream_t st, int nbe, int npg2, const ElemData* elem,
                                const GaussPt* gp, const double* hdiag, const double* gdiag,
-                               const int* kode, const cd* zdfi, int* colflag, const DynConst& K,
-                               cd* zh, size_t ldzh, cd* rhs_part, cd* rhs_diag, cd* zfi,
-                               hipEvent_t ev_od0, hipEvent_t ev_od1) {
-    hipLaunchKernelGGL(k_colflags, dim3((nbe + 255) / 256), dim3(256), 0, st, nbe, kode, zdfi,
-                       colflag);
+                               const int* kode, const cd* zdfi, int nrhs, int* colflag,
+                               int* flist, const DynConst& K, cd* zh, size_t ldzh, cd* rhs_part,
+                               cd* rhs_diag, cd* zfi, hipEvent_t ev_od0, hipEvent_t ev_od1) {
+    const size_t ldr = 3 * (size_t)nbe;
+    int* fcount = flist + nbe;
+    hipLaunchKernelGGL(k_colflags, dim3(1), dim3(256), 0, st, nbe, kode, zdfi, nrhs, ldr, colflag,
+                       flist, fcount);
     hipLaunchKernelGGL(k_dyn_diag, dim3(nbe), dim3(64), 0, st, nbe, npg2, elem, gp, hdiag, gdiag,
-                       colflag, zdfi, K, zh, ldzh, rhs_diag);
-    int tiles = offdiag_tiles(nbe);
-    size_t lds = (size_t)OD_JT * npg2 * sizeof(GaussPt);
+                       colflag, zdfi, nrhs, ldr, K, zh, ldzh, rhs_diag);
+    const int rowblk = (nbe + 64 * OD_WAVES - 1) / (64 * OD_WAVES);
     if (ev_od0) hipEventRecord(ev_od0, st);
-    hipLaunchKernelGGL((k_dyn_offdiag<OD_WAVES, OD_JT>),
-                       dim3((nbe + 64 * OD_WAVES - 1) / (64 * OD_WAVES), tiles),
-                       dim3(64 * OD_WAVES), lds, st, nbe, npg2, elem, gp, colflag, zdfi, K, zh,
-                       ldzh, rhs_part);
+    hipLaunchKernelGGL((k_dyn_offdiag_h<OD_WAVES, OD_JT>), dim3(rowblk, (nbe + OD_JT - 1) / OD_JT),
+                       dim3(64 * OD_WAVES), (size_t)OD_JT * npg2 * sizeof(GaussPt), st, nbe, npg2,
+                       elem, gp, colflag, K, zh, ldzh);
     if (ev_od1) hipEventRecord(ev_od1, st);
-    hipLaunchKernelGGL(k_rhs_reduce, dim3((3 * nbe + 255) / 256), dim3(256), 0, st, 3 * nbe,
-                       tiles, rhs_part, rhs_diag, zfi);
+    hipLaunchKernelGGL((k_dyn_offdiag_g<OD_WAVES, OG_JT>), dim3(rowblk, offdiag_tiles(nbe)),
+                       dim3(64 * OD_WAVES), (size_t)OG_JT * npg2 * sizeof(GaussPt), st, nbe, npg2,
+                       elem, gp, colflag, flist, fcount, zdfi, nrhs, ldr, K, zh, ldzh, rhs_part);
+    hipLaunchKernelGGL(k_rhs_reduce, dim3((3 * nbe + 255) / 256, nrhs), dim3(256), 0, st, 3 * nbe,
+                       nrhs, ldr, OG_JT, fcount, rhs_part, rhs_diag, zfi);
     return hipGetLastError();
 }
 

@@ -25,8 +25,8 @@ hipError_t launch_gest_diag(hipStream_t, int, int, const int*, const double*, co
                             const double*, const ElemData*, const double*, const double*, int,
                             double, double, double*, int*);
 hipError_t launch_dyn_assembly(hipStream_t, int, int, const ElemData*, const GaussPt*,
-                               const double*, const double*, const int*, const cd*, int*,
-                               const DynConst&, cd*, size_t, cd*, cd*, cd*, hipEvent_t,
+                               const double*, const double*, const int*, const cd*, int, int*,
+                               int*, const DynConst&, cd*, size_t, cd*, cd*, cd*, hipEvent_t,
                                hipEvent_t);
 int offdiag_tiles(int nbe);
 hipError_t launch_unswap(hipStream_t, int, const int*, cd*, cd*, cd);
@@ -74,7 +74,8 @@ struct bemx_ctx {
     DevBuf hdiag, gdiag, hpart;
     // dynamic data
     bool dyn_ok = false;
-    DevBuf kode, zdfi, colflag, rhs_part, rhs_diag, zfi, zh;
+    DevBuf kode, zdfi, colflag, flist, rhs_part, rhs_diag, zfi, zh;
+    int nrhs = 1;              // load cases (columns of zdfi / zfi)
     bool zh_valid = false;     // zh holds the current assembled matrix
     const void* zh_host = nullptr;  // host array the last GHMATECD copied ZH to
     // interior
@@ -172,7 +173,7 @@ int bemx_destroy(bemx_ctx* c) {
     hipStreamSynchronize(c->st);
     DevBuf* bufs[] = {&c->cone, &c->cx, &c->cy, &c->cz, &c->gi, &c->ome, &c->elem,
                       &c->gp, &c->gpT, &c->status, &c->hdiag, &c->gdiag, &c->hpart,
-                      &c->kode, &c->zdfi, &c->colflag, &c->rhs_part, &c->rhs_diag,
+                      &c->kode, &c->zdfi, &c->colflag, &c->flist, &c->rhs_part, &c->rhs_diag,
                       &c->zfi, &c->zh, &c->ipts, &c->ipart, &c->zdsol, &c->zssol,
                       &c->ifi, &c->idfi, &c->ikode};
     for (DevBuf* b : bufs) b->release();
@@ -335,19 +336,26 @@ int bemx_ghmatece(bemx_ctx* c, double ge, double rnu, double* hestdiag, double* 
     return BEMX_OK;
 }
 
-int bemx_send_dyn_data(bemx_ctx* c, const double* zdfi, const int* kode) {
+int bemx_send_load_cases(bemx_ctx* c, int nrhs, const double* zdfi, const int* kode) {
     if (!c) return BEMX_ERR_ARG;
     if (!c->mesh_ok) return fail(c, BEMX_ERR_STATE, "%sdynamic data before the mesh", "", 0);
     if (!zdfi || !kode) return fail(c, BEMX_ERR_ARG, "%snull ZDFI/KODE", "", 0);
+    if (nrhs < 1 || nrhs > 4) return fail(c, BEMX_ERR_ARG, "%s1..4 load cases, got %ld", "", nrhs);
     HIPCHK(c, hipSetDevice(c->device));
     size_t nn = 3 * (size_t)c->nbe;
-    HIPCHK(c, c->zdfi.ensure(sizeof(cd) * nn));
+    c->nrhs = nrhs;
+    HIPCHK(c, c->zdfi.ensure(sizeof(cd) * nn * nrhs));
     HIPCHK(c, c->kode.ensure(sizeof(int) * nn));
-    HIPCHK(c, hipMemcpyAsync(c->zdfi.p, zdfi, sizeof(cd) * nn, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->zdfi.p, zdfi, sizeof(cd) * nn * nrhs, hipMemcpyHostToDevice,
+                             c->st));
     HIPCHK(c, hipMemcpyAsync(c->kode.p, kode, sizeof(int) * nn, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->dyn_ok = true;
     return BEMX_OK;
+}
+
+int bemx_send_dyn_data(bemx_ctx* c, const double* zdfi, const int* kode) {
+    return bemx_send_load_cases(c, 1, zdfi, kode);
 }
 
 // Assembly of ZH and ZFI on the device (no host copies).
@@ -355,16 +363,18 @@ static int assemble_dev(bemx_ctx* c, const DynConst& K) {
     const int nbe = c->nbe;
     const size_t nn = 3 * (size_t)nbe;
     const int tiles = offdiag_tiles(nbe);
+    const int nr = c->nrhs;
     HIPCHK(c, c->colflag.ensure(sizeof(int) * nbe));
-    HIPCHK(c, c->rhs_part.ensure(sizeof(cd) * nn * tiles));
-    HIPCHK(c, c->rhs_diag.ensure(sizeof(cd) * nn));
-    HIPCHK(c, c->zfi.ensure(sizeof(cd) * nn));
+    HIPCHK(c, c->flist.ensure(sizeof(int) * (nbe + 1)));
+    HIPCHK(c, c->rhs_part.ensure(sizeof(cd) * nn * tiles * nr));
+    HIPCHK(c, c->rhs_diag.ensure(sizeof(cd) * nn * nr));
+    HIPCHK(c, c->zfi.ensure(sizeof(cd) * nn * nr));
     HIPCHK(c, c->zh.ensure(sizeof(cd) * nn * nn));
     c->zh_valid = false;
     HIPCHK(c, launch_dyn_assembly(c->st, nbe, c->npg * c->npg, c->elem.as<ElemData>(),
                                   c->gp.as<GaussPt>(), c->hdiag.as<double>(),
-                                  c->gdiag.as<double>(), c->kode.as<int>(), c->zdfi.as<cd>(),
-                                  c->colflag.as<int>(), K, c->zh.as<cd>(), nn,
+                                  c->gdiag.as<double>(), c->kode.as<int>(), c->zdfi.as<cd>(), nr,
+                                  c->colflag.as<int>(), c->flist.as<int>(), K, c->zh.as<cd>(), nn,
                                   c->rhs_part.as<cd>(), c->rhs_diag.as<cd>(), c->zfi.as<cd>(),
                                   c->ev2, c->ev3));
     HIPCHK(c, hipEventRecord(c->ev4, c->st));
@@ -409,7 +419,9 @@ int bemx_ghmatecd(bemx_ctx* c, const double* zge, const double* zcs, const doubl
     DynConst K = make_dyn_const(c, ld_c(zge), ld_c(zcs), ld_c(zcp), fr, 1.0);
     int s = assemble_dev(c, K);
     if (s) return s;
-    if (zfi) HIPCHK(c, hipMemcpyAsync(zfi, c->zfi.p, sizeof(cd) * nn, hipMemcpyDeviceToHost, c->st));
+    if (zfi)
+        HIPCHK(c, hipMemcpyAsync(zfi, c->zfi.p, sizeof(cd) * nn * c->nrhs, hipMemcpyDeviceToHost,
+                                 c->st));
     if (zh) HIPCHK(c, hipMemcpyAsync(zh, c->zh.p, sizeof(cd) * nn * nn, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->zh_host = zh;
@@ -515,13 +527,14 @@ int bemx_solve(bemx_ctx* c, const double* zge, const double* zcs, const double* 
     if (s) return s;
     long long inf = 0;
     double st[4] = {0, 0, 0, 0};
-    s = bemx::lu_solve(c->lu, c->st, (int)nn, c->zh.as<cd>(), nn, c->zfi.as<cd>(), 1, mode, &inf,
-                       st, c->err);
+    s = bemx::lu_solve(c->lu, c->st, (int)nn, c->zh.as<cd>(), nn, c->zfi.as<cd>(), c->nrhs, mode,
+                       &inf, st, c->err);
     c->zh_valid = false;
     if (s) return s;
     HIPCHK(c, hipEventRecord(c->ev1, c->st));
     if (zfi_out)
-        HIPCHK(c, hipMemcpyAsync(zfi_out, c->zfi.p, sizeof(cd) * nn, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync(zfi_out, c->zfi.p, sizeof(cd) * nn * c->nrhs,
+                                 hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     float ms = 0.f, ms_od = 0.f, ms_as = 0.f;
     hipEventElapsedTime(&ms, c->ev0, c->ev1);

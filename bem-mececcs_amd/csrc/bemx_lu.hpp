@@ -22,14 +22,17 @@ struct LuWork {
     void release();
 };
 
-// Solves A x = b in place (A overwritten by its factors, b by x).
+// Solves A X = B in place for nrhs columns of B (A overwritten by its
+// factors, B by X).
 // mode: 0 = FP64 partial-pivot LU; 1 = FP32 LU + FP64 GMRES-IR.
 // stats: [GMRES iterations, final ||r||/||b||, 1 if the mixed path fell back
 // to FP64, 0].
 // FP32 LU (split-BF16x3 MFMA update) preconditioning FP64 GMRES-IR; A is
 // read only, b is overwritten with x on success and left intact otherwise.
-int gmres_ir(LuWork& w, hipStream_t st, int n, const cd* A, size_t lda, cd* b, double stats[4],
-             std::string& err);
+// b holds nrhs columns of n; *nsolved receives how many leading columns
+// were solved when an error is returned.
+int gmres_ir(LuWork& w, hipStream_t st, int n, const cd* A, size_t lda, cd* b, int nrhs,
+             double stats[4], std::string& err, int* nsolved);
 
 int lu_solve(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, cd* b, int nrhs, int mode,
              long long* info, double stats[4], std::string& err);

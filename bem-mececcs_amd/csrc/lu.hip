@@ -541,15 +541,17 @@ static int solve_fp64(LuWork& w, hipStream_t st, int n, const cd* A, size_t lda,
 
 int lu_solve(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, cd* b, int nrhs, int mode,
              long long* info, double stats[4], std::string& err) {
-    if (nrhs != 1) {
-        err = "LU: one right-hand side per call";
+    if (nrhs < 1) {
+        err = "LU: no right-hand side";
         return BEMX_ERR_ARG;
     }
     if (stats) stats[0] = stats[1] = stats[2] = stats[3] = 0.0;
+    int first = 0;  // first column the FP64 path still has to solve
     if (mode == BEMX_MODE_MIXED) {
-        int sm = gmres_ir(w, st, n, A, lda, b, stats, err);
+        int sm = gmres_ir(w, st, n, A, lda, b, nrhs, stats, err, &first);
         if (sm != BEMX_ERR_NOCONV && sm != BEMX_ERR_SINGULAR) return sm;
-        // the plain FP64 path is the fallback; A and b are still intact
+        // the plain FP64 path is the fallback; A and the unsolved columns of
+        // b are still intact
         if (stats) stats[2] = 1.0;
     }
     int s = factor_fp64(w, st, n, A, lda, err);
@@ -571,7 +573,11 @@ int lu_solve(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, cd* b, int nrh
         err = "LU: exactly singular pivot (INFO > 0)";
         return BEMX_ERR_SINGULAR;
     }
-    return solve_fp64(w, st, n, A, lda, b, err);
+    for (int j = first; j < nrhs; ++j) {
+        s = solve_fp64(w, st, n, A, lda, b + (size_t)j * n, err);
+        if (s) return s;
+    }
+    return BEMX_OK;
 }
 
 }  // namespace bemx

@@ -590,11 +590,9 @@ static void gemv(hipStream_t st, int n, const cd* A, size_t lda, const cd* x, co
 
 typedef std::complex<double> zx;
 
-int gmres_ir(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda, cd* b, double stats[4],
-             std::string& err) {
-    MixedFactor F;
-    int s = factor_mixed(w, st, n, A64, lda, F, err);
-    if (s) return s;
+// GMRES-IR of one right-hand side against the FP32 factors F.
+static int refine_one(LuWork& w, hipStream_t st, const MixedFactor& F, int n, const cd* A64,
+                      size_t lda, cd* b, int* iters, double* relres_out, std::string& err) {
     hipError_t e;
     const int MR = 12;        // GMRES restart length
     const int MAXOUT = 12;    // refinement steps
@@ -738,12 +736,8 @@ int gmres_ir(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda, cd* b,
         err = "mixed solve: a device-side wait of the triangular solves timed out";
         return BEMX_ERR_TIMEOUT;
     }
-    if (stats) {
-        stats[0] = it_total;
-        stats[1] = relres;
-        stats[2] = 0.0;
-        stats[3] = 0.0;
-    }
+    *iters = it_total;
+    *relres_out = relres;
     if (!converged) {
         MX_CHK(hipStreamSynchronize(st));
         err = "mixed solve: GMRES-IR did not reach the FP64 residual level";
@@ -751,6 +745,30 @@ int gmres_ir(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda, cd* b,
     }
     hipLaunchKernelGGL(k_copy_z, dim3(nb256), dim3(256), 0, st, x, b, n);
     MX_CHK(hipStreamSynchronize(st));
+    return BEMX_OK;
+}
+
+int gmres_ir(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda, cd* b, int nrhs,
+             double stats[4], std::string& err, int* nsolved) {
+    *nsolved = 0;
+    MixedFactor F;
+    int s = factor_mixed(w, st, n, A64, lda, F, err);
+    if (s) return s;
+    int iters = 0;
+    double worst = 0.0;
+    for (int j = 0; j < nrhs; ++j) {
+        int it = 0;
+        double rr = 0.0;
+        s = refine_one(w, st, F, n, A64, lda, b + (size_t)j * n, &it, &rr, err);
+        iters += it;
+        worst = std::max(worst, rr);
+        if (stats) {
+            stats[0] = iters;
+            stats[1] = worst;
+        }
+        if (s) return s;
+        *nsolved = j + 1;
+    }
     return BEMX_OK;
 }
 

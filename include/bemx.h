@@ -74,6 +74,13 @@ int bemx_ghmatece(bemx_ctx *ctx, double ge, double rnu, double *hestdiag, double
  * Replaces send_dyn_data_gpu_ (kernels/shared.cu:168-182). */
 int bemx_send_dyn_data(bemx_ctx *ctx, const double *zdfi, const int *kode);
 
+/* Boundary data of nrhs (1..4) load cases on the same mesh and KODE:
+ * ZDFI(3NBE, nrhs) complex, column-major.  Each load case is a column of
+ * ZDFI as send_dyn_data_gpu_ (kernels/shared.cu:168-182) takes one; the
+ * reference solves one load case per run, so bemx_ghmatecd then returns
+ * ZFI(3NBE, nrhs) and bemx_solve solves all columns with one factorisation. */
+int bemx_send_load_cases(bemx_ctx *ctx, int nrhs, const double *zdfi, const int *kode);
+
 /* Dynamic assembly at one frequency.  Replaces GHMATECD (Ghmatecd.for:11-239)
  * = cuda_ghmatecd_ (kernels/Ghmatecd_cu.cu:305-483): ZH (3NBE x 3NBE) after
  * the KODE column swap, and ZFI = ZG ZDFI.  zge/zcs/zcp point to complex
@@ -106,9 +113,10 @@ int bemx_interec(bemx_ctx *ctx, double *zfi, double *zdfi, const int *kode, int 
  * solve, with the boundary data from bemx_send_dyn_data.  No reference
  * counterpart (it is GHMATECD + LINSOLVE without the host round trip of ZH);
  * used by the sweep, the serving loop and the benchmark.  zfi_out (host,
- * may be NULL) receives the solution; stats (may be NULL, 4 doubles):
- * refinement iterations, final relative residual, backward error estimate,
- * assembly+solve device time in ms. */
+ * may be NULL) receives the solution, 3NBE x nrhs for the load cases of
+ * bemx_send_load_cases (one factorisation serves all of them); stats (may
+ * be NULL, 4 doubles): GMRES iterations, worst final ||r||/||b||, 1 if the
+ * mixed path fell back to FP64, assembly+solve device time in ms. */
 int bemx_solve(bemx_ctx *ctx, const double *zge, const double *zcs, const double *zcp,
                double fr, int mode, double *zfi_out, double *stats);
 

@@ -110,3 +110,31 @@ def test_device_resident_solve_matches(ctx, orc):
     rel = np.max(np.abs(x - ref["x"])) / np.max(np.abs(ref["x"]))
     assert rel < 1e-12, rel
     assert stats[3] > 0.0
+
+
+@pytest.mark.parametrize("mode", [api.MODE_FP64, api.MODE_MIXED])
+def test_load_cases_share_one_factorisation(ctx, mode):
+    """Three load cases (random central traction patches, some KODE=0
+    displacement columns) solved together match the same cases solved one at
+    a time, column by column: one factorisation, several right-hand sides."""
+    mesh, dyn = decks.flat_patch_problem(12, 11, n_interior=0, freqs=(1.5,), load_cases=3,
+                                         seed=7)
+    kode = dyn.kode.copy()
+    kode[3 * 5:3 * 5 + 3] = 0      # element 6 fully prescribed displacement
+    kode[3 * 40 + 1] = 0           # one mixed element
+    cases = dyn.load_cases.T.astype(np.complex128)
+    cases[kode == 0, :] = 0.01 * (1 + 1j)
+    ctx.send_shared_data(mesh)
+    ctx.ghmatece(mesh.ge, mesh.rnu)
+    zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
+    ctx.send_load_cases(cases, kode)
+    _, zfi_all = ctx.ghmatecd(zmat, 1.5, want_zh=False)
+    x_all, _ = ctx.solve(zmat, 1.5, mode)
+    assert x_all.shape == (3 * mesh.nbe, 3)
+    for j in range(3):
+        ctx.send_dyn_data(cases[:, j], kode)
+        _, zfi = ctx.ghmatecd(zmat, 1.5, want_zh=False)
+        assert np.max(np.abs(zfi_all[:, j] - zfi)) <= 1e-13 * np.max(np.abs(zfi))
+        x, _ = ctx.solve(zmat, 1.5, mode)
+        rel = np.max(np.abs(x_all[:, j] - x)) / np.max(np.abs(x))
+        assert rel < (1e-13 if mode == api.MODE_FP64 else 1e-10), (j, rel)
