@@ -18,6 +18,10 @@ struct LuWork {
     // when it is (re)allocated, and never inside a solve.
     int epoch = 0;
     void* flags_init = nullptr;
+    // second stream and events of the look-ahead factorisation (lazy)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_panel = nullptr, ev_bulk = nullptr;
+    hipError_t ensure_side();
     void* get(int slot, size_t bytes, hipError_t* e);
     void release();
 };

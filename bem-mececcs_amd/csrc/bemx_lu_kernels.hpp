@@ -30,37 +30,92 @@ __device__ inline cf zero_of<cf>() { return mkf(0.f, 0.f); }
 __device__ inline cd shfl_c(cd v, int src) { return mk(__shfl(v.x, src, 64), __shfl(v.y, src, 64)); }
 __device__ inline cf shfl_c(cf v, int src) { return mkf(__shfl(v.x, src, 64), __shfl(v.y, src, 64)); }
 
-constexpr int TB = 64;            // triangular block size (rows of L11 / U11)
-constexpr int TRSM_COLS_PER_WG = 16;
+// Split-BF16 planes of a factor panel, the operands of the mixed LU's
+// trailing update: plane 0 re_hi, 1 re_lo, 2 im_hi, 3 im_lo, with
+// hi = bf16(x), lo = bf16(x - hi); element (index, k) of plane pl at
+// pl*pstride + index*64 + k, index = global row (L21) or column (U12).
+__device__ inline void split_bf16(float v, __bf16& hi, __bf16& lo) {
+    hi = (__bf16)v;
+    lo = (__bf16)(v - (float)hi);
+}
+__device__ inline void store_planes(__bf16* __restrict__ pl, size_t pstride, size_t idx, cf v) {
+    __bf16 h, l;
+    split_bf16(v.x, h, l);
+    pl[idx] = h;
+    pl[pstride + idx] = l;
+    split_bf16(v.y, h, l);
+    pl[2 * pstride + idx] = h;
+    pl[3 * pstride + idx] = l;
+}
 
-// X = L11^-1 (P X) for the TB rows k0.. of columns [c0, c0+nc), in place.
-// L11 = unit lower triangle of A[k0:k0+nb, k0:k0+nb].  One wavefront per
-// column, lane r = block row r; with PERM the column is first gathered by
-// the block permutation lperm (row r <- row lperm[r]).
+constexpr int TB = 64;            // triangular block size (rows of L11 / U11)
+constexpr int TRSM_COLS_PER_WG = 64;
+
+// X = L11^-1 (P X) for the TB rows k0.. of columns [c0, c0+nc), in place
+// (and, for complex float with planes != NULL, split into the bf16 planes).
+// L11 = unit lower triangle of A[k0:k0+nb, k0:k0+nb].  A workgroup of 256
+// threads stages L11 and its 64 columns (gathered by the block permutation
+// lperm, row r <- row lperm[r], with PERM) in LDS with all loads in flight;
+// then wave 0 solves, one lane per column, in four chunks of 16 rows: the
+// chunk is held in registers, updated with the solved rows of the earlier
+// chunks and solved in place.  L11 reads are wave-uniform LDS broadcasts,
+// the column reads conflict-free.
 template <class T, bool PERM>
 __global__ __launch_bounds__(256) void k_trsm_lower_cols(T* __restrict__ A, size_t lda, int k0,
                                                          int nb, int c0, int nc,
-                                                         const int* __restrict__ lperm) {
-    __shared__ T Ls[TB * TB];
-    __shared__ int lp[TB];
-    for (int t = threadIdx.x; t < nb * nb; t += 256) {
-        int c = t / nb, r = t - c * nb;
-        Ls[c * TB + r] = A[(size_t)(k0 + c) * lda + k0 + r];
+                                                         const int* __restrict__ lperm,
+                                                         __bf16* __restrict__ planes,
+                                                         size_t pstride) {
+    constexpr int XS = TB + 1;  // padded row stride: the transposing staging is not bank-bound
+    __shared__ T Ls[TB * TB];   // Ls[i*TB + r] = L11[r][i]
+    __shared__ T X[TB * XS];    // X[r*XS + lane] = row r of the lane's column
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+#pragma unroll 4
+    for (int t = tid; t < TB * TB; t += 256) {
+        int i = t >> 6, r = t & 63;
+        Ls[t] = (i < nb && r < nb) ? A[(size_t)(k0 + i) * lda + k0 + r] : zero_of<T>();
     }
-    if (PERM && threadIdx.x < nb) lp[threadIdx.x] = lperm[threadIdx.x];
+    const int cb = c0 + blockIdx.x * TRSM_COLS_PER_WG;
+    const int ncols = min(TRSM_COLS_PER_WG, c0 + nc - cb);
+    {
+        // lane = row of column j (coalesced), wave w takes columns w, w+4, ...
+        const int src = (lane < nb) ? (PERM ? lperm[lane] : lane) : 0;
+#pragma unroll 4
+        for (int j = w; j < TRSM_COLS_PER_WG; j += 4)
+            X[lane * XS + j] = (lane < nb && j < ncols) ? A[(size_t)(cb + j) * lda + k0 + src]
+                                                       : zero_of<T>();
+    }
     __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int cbeg = c0 + blockIdx.x * TRSM_COLS_PER_WG;
-    const int cend = min(c0 + nc, cbeg + TRSM_COLS_PER_WG);
-    for (int c = cbeg + wave; c < cend; c += 4) {
-        T* col = A + (size_t)c * lda + k0;
-        T x = zero_of<T>();
-        if (lane < nb) x = col[PERM ? lp[lane] : lane];
-        for (int i = 0; i < nb - 1; ++i) {
-            T xi = shfl_c(x, i);
-            if (lane > i && lane < nb) x = x - Ls[i * TB + lane] * xi;
+    if (w == 0) {
+#pragma unroll
+        for (int g = 0; g < TB; g += 16) {
+            T a[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) a[r] = X[(g + r) * XS + lane];
+            for (int i = 0; i < g; ++i) {
+                const T xi = X[i * XS + lane];
+#pragma unroll
+                for (int r = 0; r < 16; ++r) a[r] = a[r] - Ls[i * TB + g + r] * xi;
+            }
+#pragma unroll
+            for (int i = 0; i < 15; ++i) {
+#pragma unroll
+                for (int r = i + 1; r < 16; ++r) a[r] = a[r] - Ls[(g + i) * TB + g + r] * a[i];
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) X[(g + r) * XS + lane] = a[r];
         }
-        if (lane < nb) col[lane] = x;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int j = w; j < TRSM_COLS_PER_WG; j += 4) {
+        if (lane < nb && j < ncols) A[(size_t)(cb + j) * lda + k0 + lane] = X[lane * XS + j];
+        if constexpr (sizeof(T) == sizeof(cf)) {
+            // U12 columns also go out split (planes; k = lane), when asked
+            if (planes && j < ncols)
+                store_planes(planes, pstride, (size_t)(cb + j) * TB + lane,
+                             lane < nb ? X[lane * XS + j] : zero_of<T>());
+        }
     }
 }
 

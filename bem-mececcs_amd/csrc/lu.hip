@@ -10,7 +10,7 @@
 //   k_rowperm      applies that permutation to the trailing columns (the
 //                  factored L columns are left in place: the forward solve
 //                  applies each panel's permutation before using its L block).
-//   k_trsm_lower_cols  U12 = L11^-1 A12, one wavefront per column.
+//   k_trsm_lower_cols  U12 = L11^-1 A12, one lane per column.
 //   k_zgemm_mfma   A22 -= L21 U12 as four real FP64 MFMA products
 //                  (v_mfma_f64_16x16x4f64): Cr -= Ar Br - Ai Bi,
 //                  Ci -= Ar Bi + Ai Br.
@@ -45,6 +45,20 @@ void LuWork::release() {
         buf[i] = nullptr;
         cap[i] = 0;
     }
+    if (side) hipStreamDestroy(side);
+    if (ev_panel) hipEventDestroy(ev_panel);
+    if (ev_bulk) hipEventDestroy(ev_bulk);
+    side = nullptr;
+    ev_panel = ev_bulk = nullptr;
+    flags_init = nullptr;
+}
+
+hipError_t LuWork::ensure_side() {
+    hipError_t e = hipSuccess;
+    if (!side) e = hipStreamCreateWithFlags(&side, hipStreamNonBlocking);
+    if (e == hipSuccess && !ev_panel) e = hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming);
+    if (e == hipSuccess && !ev_bulk) e = hipEventCreateWithFlags(&ev_bulk, hipEventDisableTiming);
+    return e;
 }
 
 constexpr int NB = 64;          // panel width (FP64 path)
@@ -503,7 +517,8 @@ static int factor_fp64(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, std:
         hipLaunchKernelGGL(k_rowperm, dim3((nc + 3) / 4), dim3(256), 0, st, A, lda, c0, nc, permk);
         hipLaunchKernelGGL((k_trsm_lower_cols<cd, false>),
                            dim3((nc + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG), dim3(256), 0, st,
-                           A, lda, k0, nb, c0, nc, (const int*)nullptr);
+                           A, lda, k0, nb, c0, nc, (const int*)nullptr, (__bf16*)nullptr,
+                           (size_t)0);
         const int mr = n - c0;
         hipLaunchKernelGGL(k_zgemm_mfma, dim3((mr + GB - 1) / GB, (nc + GB - 1) / GB), dim3(256), 0,
                            st, mr, nc, nb, A + (size_t)k0 * lda + c0, lda,

@@ -3,13 +3,17 @@
 // FP64 GMRES iterative refinement (GMRES-IR) against the FP64 matrix.
 //
 // Factorisation (blocks of NBM = 64 columns, A32 = (complex float) A64):
-//   k_diag_getrf32     partial pivoting inside the 64 x 64 diagonal block (LDS)
-//   k_trsm_lower_cols  U12 = L11^-1 P A12   (one wavefront per column)
-//   k_trsm_upper_rows  L21 = A21 U11^-1     (one wavefront per row, tile in LDS)
-//   k_cgemm_bf16x3     A22 -= L21 U12; every FP32 operand x is split into
-//                      hi = bf16(x), lo = bf16(x - hi) and each real product is
+//   k_diag_getrf32     partial pivoting inside the 64 x 64 diagonal block
+//                      (registers: lane = row, wave = 16 columns)
+//   k_trsm_lower_cols  U12 = L11^-1 P A12   (one lane per column)
+//   k_trsm_upper_rows  L21 = A21 U11^-1     (one lane per row)
+//   k_cgemm_planes     A22 -= L21 U12; every FP32 operand x is split into
+//                      hi = bf16(x), lo = bf16(x - hi) (by the two triangular
+//                      kernels, into bf16 planes) and each real product is
 //                      hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16 with
 //                      FP32 accumulation; complex = 4 real products.
+// The factorisation looks one block ahead over two streams: the next
+// diagonal block is factored while the bulk update of the current step runs.
 // Preconditioner M^-1 v (FP64 vectors, FP32 factors): k_trsv_lower_coop and
 // k_trsv_upper_coop, one cooperative launch each; workgroups own 64-row
 // blocks, accumulate the contributions of finished blocks as their flags
@@ -44,122 +48,171 @@ __global__ void k_cvt_z2c(const cd* __restrict__ a, size_t lda, cf* __restrict__
     o[c * ldo + r] = cf{(float)v.x, (float)v.y};
 }
 
-// Diagonal block: GEPP within its nb rows (largest |re|+|im|, lowest row on
-// ties).  lperm[i] = block row whose content ends at block row i.
+// Wave-uniform copy of lane src's complex float.
+__device__ inline cf readlane_f(cf v, int src) {
+    return mkf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), src)),
+               __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), src)));
+}
+
+// Diagonal block: GEPP within its nb rows (largest |re|+|im|, lowest block
+// row on ties), held in registers: wave q owns columns [16q, 16q+16), lane r
+// owns block row r.  Rows are not moved while factoring: a row is marked
+// used when it becomes the pivot of step j (its rank), and the factors are
+// written back in rank order.  lperm[i] = block row whose content ends at
+// block row i.  One barrier per step: the wave owning column j+1 applies
+// step j to that column first and publishes the pivot and multipliers of
+// step j+1 (double-buffered in LDS) before finishing its step-j update.
 __global__ __launch_bounds__(256) void k_diag_getrf32(cf* __restrict__ A, size_t lda, int k0,
                                                       int nb, int* __restrict__ lperm,
                                                       int* __restrict__ status) {
-    __shared__ cf S[NBM * NBM];
-    __shared__ int pos[NBM];
-    __shared__ int s_p;
-    const int tid = threadIdx.x;
-    for (int t = tid; t < nb * nb; t += 256) {
-        int c = t / nb, r = t - c * nb;
-        S[c * NBM + r] = A[(size_t)(k0 + c) * lda + k0 + r];
+    __shared__ cf lcol[2][NBM];
+    __shared__ int s_piv[2];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int q = __builtin_amdgcn_readfirstlane(tid >> 6);
+    cf v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int c = 16 * q + i;
+        v[i] = (lane < nb && c < nb) ? A[(size_t)(k0 + c) * lda + k0 + lane] : mkf(0.f, 0.f);
     }
-    if (tid < NBM) pos[tid] = tid;
+    bool used = lane >= nb;
+    int rank = lane;  // rows >= nb keep their place
+    // pivot of column (wave-local index) ii; multipliers stored in v[ii]
+    auto search = [&](cf& col, int buf) {
+        float bv = used ? -1.f : fabsf(col.x) + fabsf(col.y);
+        int br = lane;
+        for (int off = 32; off >= 1; off >>= 1) {
+            float ov = __shfl_xor(bv, off, 64);
+            int orr = __shfl_xor(br, off, 64);
+            if (ov > bv || (ov == bv && orr < br)) {
+                bv = ov;
+                br = orr;
+            }
+        }
+        const int p = __builtin_amdgcn_readfirstlane(br);
+        const cf pv = readlane_f(col, p);
+        const bool zero = pv.x == 0.f && pv.y == 0.f;
+        const cf inv = zero ? mkf(0.f, 0.f) : cinvf(pv);
+        cf l = mkf(0.f, 0.f);
+        if (!used && lane != p) {
+            l = col * inv;
+            col = l;
+        }
+        lcol[buf][lane] = l;
+        if (lane == 0) {
+            s_piv[buf] = p;
+            if (zero) atomicOr(status, ST_PIVOT_ZERO);
+        }
+    };
+    if (q == 0 && nb > 0) search(v[0], 0);
     __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        if (tid < 64) {
-            float bv = -1.f;
-            int br = 1 << 30;
-            if (tid >= j && tid < nb) {
-                cf v = S[j * NBM + tid];
-                bv = fabsf(v.x) + fabsf(v.y);
-                br = tid;
-            }
-            for (int off = 32; off >= 1; off >>= 1) {
-                float ov = __shfl_xor(bv, off, 64);
-                int orr = __shfl_xor(br, off, 64);
-                if (ov > bv || (ov == bv && orr < br)) {
-                    bv = ov;
-                    br = orr;
+    for (int jc = 0; jc < 4; ++jc) {
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+            const int j = 16 * jc + jj;
+            if (j < nb) {
+                const int p = __builtin_amdgcn_readfirstlane(s_piv[j & 1]);
+                const cf l = lcol[j & 1][lane];
+                if (lane == p) {
+                    used = true;
+                    rank = j;
                 }
-            }
-            if (tid == 0) {
-                s_p = br;
-                if (bv == 0.f) atomicOr(status, ST_PIVOT_ZERO);
-            }
-        }
-        __syncthreads();
-        const int p = s_p;
-        if (p != j) {
-            for (int c = tid; c < nb; c += 256) {
-                cf t = S[c * NBM + j];
-                S[c * NBM + j] = S[c * NBM + p];
-                S[c * NBM + p] = t;
-            }
-            if (tid == 0) {
-                int t = pos[j];
-                pos[j] = pos[p];
-                pos[p] = t;
+                const int qn = jj == 15 ? jc + 1 : jc;  // owner of column j+1
+                const int in = (jj + 1) & 15;
+                const bool owner = q == qn && j + 1 < nb;
+                if (owner) {
+                    v[in] = v[in] - l * readlane_f(v[in], p);
+                    search(v[in], (j + 1) & 1);
+                }
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    if (16 * q + i > j && !(owner && i == in)) v[i] = v[i] - l * readlane_f(v[i], p);
+                }
+                __syncthreads();
             }
         }
-        __syncthreads();
-        cf u = S[j * NBM + j];
-        cf inv = (u.x != 0.f || u.y != 0.f) ? cinvf(u) : mkf(0.f, 0.f);
-        if (tid > j && tid < nb) S[j * NBM + tid] = S[j * NBM + tid] * inv;
-        __syncthreads();
-        const int rest = nb - j - 1;
-        for (int t = tid; t < rest * rest; t += 256) {
-            int c = j + 1 + t / rest, r = j + 1 + t % rest;
-            S[c * NBM + r] = S[c * NBM + r] - S[j * NBM + r] * S[c * NBM + j];
-        }
-        __syncthreads();
     }
-    for (int t = tid; t < nb * nb; t += 256) {
-        int c = t / nb, r = t - c * nb;
-        A[(size_t)(k0 + c) * lda + k0 + r] = S[c * NBM + r];
+    if (lane < nb) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int c = 16 * q + i;
+            if (c < nb) A[(size_t)(k0 + c) * lda + k0 + rank] = v[i];
+        }
+        if (q == 0) lperm[rank] = lane;
     }
-    if (tid < nb) lperm[tid] = pos[tid];
 }
 
-// L21 = A21 U11^-1 for rows [r0, r0+nr): a workgroup stages 32 rows x nb
-// columns in LDS; one wavefront per row, lane c = column c:
-// x_c = (a_c - sum_{i<c} x_i U_ic) / U_cc.
-constexpr int TRR_ROWS = 32;
+// L21 = A21 U11^-1 for rows [r0, r0+nr).  256 threads stage U11 and 64
+// rows in LDS with all loads in flight; wave 0 then solves, one lane per
+// row, left to right in four chunks of 16 columns (chunk in registers,
+// updated with the solved columns of the earlier chunks, then solved in
+// place).  U11 reads are wave-uniform broadcasts.  The result goes back to
+// A and, split, to the planes la.
+constexpr int TRR_ROWS = 64;
 __global__ __launch_bounds__(256) void k_trsm_upper_rows(cf* __restrict__ A, size_t lda, int k0,
-                                                         int nb, int r0, int nr) {
-    __shared__ cf Ut[NBM * NBM];  // Ut[i*NBM + c] = U[i][c]
-    __shared__ cf T[NBM * (TRR_ROWS + 1)];
+                                                         int nb, int r0, int nr,
+                                                         __bf16* __restrict__ la, size_t pstride) {
+    constexpr int XS = NBM + 1;
+    __shared__ cf Us[NBM * NBM];  // Us[c2*NBM + c] = U11[c][c2]
+    __shared__ cf X[NBM * XS];    // X[c*XS + lane] = column c of the lane's row
     __shared__ cf dinv[NBM];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int t = tid; t < nb * nb; t += 256) {
-        int c = t / nb, i = t - c * nb;
-        Ut[i * NBM + c] = A[(size_t)(k0 + c) * lda + k0 + i];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+#pragma unroll 4
+    for (int t = tid; t < NBM * NBM; t += 256) {
+        int c2 = t >> 6, c = t & 63;
+        Us[t] = (c2 < nb && c < nb) ? A[(size_t)(k0 + c2) * lda + k0 + c] : mkf(0.f, 0.f);
     }
     const int rb = r0 + blockIdx.x * TRR_ROWS;
-    const int rn = min(TRR_ROWS, r0 + nr - rb);
-    for (int t = tid; t < nb * TRR_ROWS; t += 256) {
-        int c = t / TRR_ROWS, rho = t - c * TRR_ROWS;
-        T[c * (TRR_ROWS + 1) + rho] =
-            rho < rn ? A[(size_t)(k0 + c) * lda + rb + rho] : mkf(0.f, 0.f);
+    const bool ok = rb + lane < r0 + nr;
+#pragma unroll 4
+    for (int c = w; c < NBM; c += 4)
+        X[c * XS + lane] = (ok && c < nb) ? A[(size_t)(k0 + c) * lda + rb + lane] : mkf(0.f, 0.f);
+    __syncthreads();
+    if (tid < NBM) {
+        cf u = Us[tid * NBM + tid];
+        dinv[tid] = (u.x != 0.f || u.y != 0.f) ? cinvf(u) : mkf(0.f, 0.f);
     }
     __syncthreads();
-    if (tid < nb) dinv[tid] = cinvf(Ut[tid * NBM + tid]);
-    __syncthreads();
-    for (int rho = wave; rho < rn; rho += 4) {
-        cf a = lane < nb ? T[lane * (TRR_ROWS + 1) + rho] : mkf(0.f, 0.f);
-        cf acc = mkf(0.f, 0.f), xf = mkf(0.f, 0.f);
-        for (int c = 0; c < nb; ++c) {
-            cf xv = (a - acc) * dinv[c];
-            cf xb = shfl_c(xv, c);
-            if (lane == c) xf = xb;
-            if (lane > c && lane < nb) acc = acc + xb * Ut[c * NBM + lane];
+    if (w == 0) {
+#pragma unroll
+        for (int g = 0; g < NBM; g += 16) {
+            cf a[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) a[c] = X[(g + c) * XS + lane];
+            for (int i = 0; i < g; ++i) {
+                const cf xi = X[i * XS + lane];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) a[c] = a[c] - xi * Us[(g + c) * NBM + i];
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                a[c] = a[c] * dinv[g + c];
+#pragma unroll
+                for (int c2 = c + 1; c2 < 16; ++c2) a[c2] = a[c2] - a[c] * Us[(g + c2) * NBM + g + c];
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c) X[(g + c) * XS + lane] = a[c];
         }
-        if (lane < nb) T[lane * (TRR_ROWS + 1) + rho] = xf;
     }
     __syncthreads();
-    for (int t = tid; t < nb * TRR_ROWS; t += 256) {
-        int c = t / TRR_ROWS, rho = t - c * TRR_ROWS;
-        if (rho < rn) A[(size_t)(k0 + c) * lda + rb + rho] = T[c * (TRR_ROWS + 1) + rho];
+#pragma unroll 4
+    for (int c = w; c < NBM; c += 4)
+        if (ok && c < nb) A[(size_t)(k0 + c) * lda + rb + lane] = X[c * XS + lane];
+    // planes: t -> (row rr, k), k fastest, so the 2-byte stores are coalesced
+#pragma unroll 4
+    for (int t = tid; t < TRR_ROWS * NBM; t += 256) {
+        const int rr = t >> 6, k = t & 63;
+        if (rb + rr < r0 + nr)
+            store_planes(la, pstride, (size_t)(rb + rr) * NBM + k,
+                         k < nb ? X[k * XS + rr] : mkf(0.f, 0.f));
     }
 }
 
-// Bounded wait until *flag >= target (see wait_count in lu.hip).
+// Bounded wait until *flag >= target (see wait_count in lu.hip); relaxed
+// polls, the caller issues one acquire fence after the wait.
 __device__ inline void wait_ge(int* flag, int target, int* status) {
     int it = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
         if (++it > FLAG_SPIN_LIMIT ||
             (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
              ST_SYNC_TIMEOUT)) {
@@ -174,183 +227,233 @@ __device__ inline void publish(int* flag, int value) {
     __hip_atomic_store(flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Wave-uniform copy of lane src's complex double (v_readlane, no LDS).
+__device__ inline cd readlane_c(cd v, int src) {
+    long long bx = __double_as_longlong(v.x), by = __double_as_longlong(v.y);
+    int xl = __builtin_amdgcn_readlane((int)bx, src), xh = __builtin_amdgcn_readlane((int)(bx >> 32), src);
+    int yl = __builtin_amdgcn_readlane((int)by, src), yh = __builtin_amdgcn_readlane((int)(by >> 32), src);
+    return mk(__longlong_as_double(((long long)xh << 32) | (unsigned)xl),
+              __longlong_as_double(((long long)yh << 32) | (unsigned)yl));
+}
+
+// Wave-level wait for a block flag: every lane polls the same address (one
+// transaction), then one agent-scope acquire.
+__device__ inline void wave_wait_ge(int* flag, int target, int* status) {
+    wait_ge(flag, target, status);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
 // y = L^-1 P b, block forward substitution.  Workgroup w owns blocks
-// w, w+P, ...; for its block it accumulates b - sum_k L[blk,k] y_k over the
-// finished blocks k < blk (waiting on flags[k] == epoch), applies the block
-// permutation, solves with the unit lower L_bb (wave 0, lane = row) and
-// publishes y_blk.  Blocks only wait on lower-numbered blocks, and every
-// workgroup is resident (cooperative launch), so the chain always drains.
+// w, w+P, ...  For its block, wave q accumulates columns [16q, 16q+16) of
+// every finished block k < blk (waiting on flags[k] == epoch itself, so the
+// four waves never stall each other); one barrier joins the four partial
+// sums, wave 0 applies the block permutation, solves with the unit lower
+// L_bb staged in LDS before any wait, and publishes y_blk.  Blocks only wait
+// on lower-numbered blocks and every workgroup is resident (cooperative
+// launch), so the chain always drains.
 __global__ __launch_bounds__(256) void k_trsv_lower_coop(const cf* __restrict__ A, size_t lda,
                                                          int n, int nblk,
                                                          const int* __restrict__ lperm,
                                                          const cd* __restrict__ b,
                                                          cd* __restrict__ y, int* __restrict__ flags,
                                                          int epoch, int* __restrict__ status) {
-    __shared__ cd acc[NBM];
+    __shared__ cf Ld[NBM * NBM];  // Ld[i*NBM + r] = L_bb[r][i]
     __shared__ cd part[4][NBM];
     const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
     for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
         const int r0 = blk * NBM, nb = min(NBM, n - r0);
-        if (tid < NBM) acc[tid] = tid < nb ? b[r0 + tid] : mk(0, 0);
-        __syncthreads();
-        for (int k = 0; k < blk; ++k) {
-            if (tid == 0) wait_ge(&flags[k], epoch, status);
-            __syncthreads();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            cd s = mk(0, 0);
-            if (lane < nb) {
-                const cf* Lc = A + (size_t)(k * NBM + 16 * q) * lda + r0 + lane;
-                const cd* yk = y + k * NBM + 16 * q;
-#pragma unroll 4
-                for (int i = 0; i < 16; ++i) s = s + up(Lc[(size_t)i * lda]) * yk[i];
-            }
-            part[q][lane] = s;
-            __syncthreads();
-            if (tid < nb) acc[tid] = acc[tid] - ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]));
-            __syncthreads();
+        for (int t = tid; t < nb * NBM; t += 256) {
+            int i = t >> 6, r = t & 63;
+            Ld[t] = r < nb ? A[(size_t)(r0 + i) * lda + r0 + r] : mkf(0.f, 0.f);
         }
+        cd s = mk(0, 0);
+        const int row = r0 + min(lane, nb - 1);
+        for (int k = 0; k < blk; ++k) {
+            const int c0 = k * NBM + 16 * q;
+            const cf* Lc = A + (size_t)c0 * lda + row;
+            cf l[16];  // the factor tile does not depend on the flag: load it first
+#pragma unroll
+            for (int i = 0; i < 16; ++i) l[i] = Lc[(size_t)i * lda];
+            wave_wait_ge(&flags[k], epoch, status);
+            const cd yl = y[c0 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) s = s + up(l[i]) * readlane_c(yl, i);
+        }
+        part[q][lane] = s;
+        __syncthreads();
         if (q == 0) {
-            cd v = lane < nb ? acc[lperm[(size_t)blk * NBM + lane]] : mk(0, 0);
+            cd v = mk(0, 0);
+            if (lane < nb) {
+                const int src = lperm[(size_t)blk * NBM + lane];
+                v = b[r0 + src] - ((part[0][src] + part[1][src]) + (part[2][src] + part[3][src]));
+            }
+#pragma unroll 8
             for (int i = 0; i < nb - 1; ++i) {
-                cd vi = shfl_c(v, i);
-                if (lane > i && lane < nb) v = v - up(A[(size_t)(r0 + i) * lda + r0 + lane]) * vi;
+                cd vi = readlane_c(v, i);
+                if (lane > i) v = v - up(Ld[i * NBM + lane]) * vi;
             }
             if (lane < nb) y[r0 + lane] = v;
+            __threadfence();
+            if (lane == 0) publish(&flags[blk], epoch);
         }
-        __threadfence();
         __syncthreads();
-        if (tid == 0) publish(&flags[blk], epoch);
     }
 }
 
-// x = U^-1 y, block backward substitution (blocks in reverse order).
+// x = U^-1 y, block backward substitution (blocks in reverse order), same
+// scheme as k_trsv_lower_coop; U_bb and the reciprocals of its diagonal are
+// staged in LDS before any wait.
 __global__ __launch_bounds__(256) void k_trsv_upper_coop(const cf* __restrict__ A, size_t lda,
                                                          int n, int nblk,
                                                          const cd* __restrict__ yv,
                                                          cd* __restrict__ x, int* __restrict__ flags,
                                                          int epoch, int* __restrict__ status) {
-    __shared__ cd acc[NBM];
+    __shared__ cf Ud[NBM * NBM];  // Ud[i*NBM + r] = U_bb[r][i]
+    __shared__ cd dinv[NBM];
     __shared__ cd part[4][NBM];
     const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
     for (int bi = blockIdx.x; bi < nblk; bi += gridDim.x) {
         const int blk = nblk - 1 - bi;
         const int r0 = blk * NBM, nb = min(NBM, n - r0);
-        if (tid < NBM) acc[tid] = tid < nb ? yv[r0 + tid] : mk(0, 0);
-        __syncthreads();
-        for (int k = nblk - 1; k > blk; --k) {
-            if (tid == 0) wait_ge(&flags[k], epoch, status);
-            __syncthreads();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            const int kb = min(NBM, n - k * NBM);
-            cd s = mk(0, 0);
-            if (lane < nb) {
-                const cf* Uc = A + (size_t)(k * NBM + 16 * q) * lda + r0 + lane;
-                const cd* xk = x + k * NBM + 16 * q;
-                for (int i = 0; i < 16 && 16 * q + i < kb; ++i) s = s + up(Uc[(size_t)i * lda]) * xk[i];
-            }
-            part[q][lane] = s;
-            __syncthreads();
-            if (tid < nb) acc[tid] = acc[tid] - ((part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]));
-            __syncthreads();
+        for (int t = tid; t < nb * NBM; t += 256) {
+            int i = t >> 6, r = t & 63;
+            Ud[t] = r < nb ? A[(size_t)(r0 + i) * lda + r0 + r] : mkf(0.f, 0.f);
         }
+        if (tid < nb) dinv[tid] = cinv(up(A[(size_t)(r0 + tid) * lda + r0 + tid]));
+        cd s = mk(0, 0);
+        const int row = r0 + min(lane, nb - 1);
+        for (int k = nblk - 1; k > blk; --k) {
+            const int kb = min(NBM, n - k * NBM);
+            const int c0 = k * NBM + 16 * q;
+            const int cnt = min(16, kb - 16 * q);  // columns of this wave in block k
+            const cf* Uc = A + (size_t)c0 * lda + row;
+            if (cnt == 16) {
+                cf u[16];  // the factor tile does not depend on the flag: load it first
+#pragma unroll
+                for (int i = 0; i < 16; ++i) u[i] = Uc[(size_t)i * lda];
+                wave_wait_ge(&flags[k], epoch, status);
+                const cd yl = x[c0 + (lane & 15)];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) s = s + up(u[i]) * readlane_c(yl, i);
+            } else {
+                wave_wait_ge(&flags[k], epoch, status);
+                if (cnt <= 0) continue;
+                const cd yl = x[c0 + min(lane & 15, cnt - 1)];
+                for (int i = 0; i < cnt; ++i) s = s + up(Uc[(size_t)i * lda]) * readlane_c(yl, i);
+            }
+        }
+        part[q][lane] = s;
+        __syncthreads();
         if (q == 0) {
-            cd v = lane < nb ? acc[lane] : mk(0, 0);
+            cd v = lane < nb ? yv[r0 + lane] - ((part[0][lane] + part[1][lane]) +
+                                                (part[2][lane] + part[3][lane]))
+                             : mk(0, 0);
+#pragma unroll 8
             for (int i = nb - 1; i >= 0; --i) {
-                if (lane == i) v = v * cinv(up(A[(size_t)(r0 + i) * lda + r0 + i]));
-                cd vi = shfl_c(v, i);
-                if (lane < i) v = v - up(A[(size_t)(r0 + i) * lda + r0 + lane]) * vi;
+                if (lane == i) v = v * dinv[i];
+                cd vi = readlane_c(v, i);
+                if (lane < i) v = v - up(Ud[i * NBM + lane]) * vi;
             }
             if (lane < nb) x[r0 + lane] = v;
+            __threadfence();
+            if (lane == 0) publish(&flags[blk], epoch);
         }
-        __threadfence();
         __syncthreads();
-        if (tid == 0) publish(&flags[blk], epoch);
     }
 }
 
 // ---------------------------------------------------------------------------
-// C[m x nc] -= A[m x K] B[K x nc], complex FP32, K <= 64, split-BF16x3 MFMA.
+// C[m x nc] -= L21 U12, complex FP32 with K = 64, split-BF16x3 on MFMA.  The
+// operands come pre-split as bf16 planes (written by the two triangular
+// kernels), so a lane loads its 8 consecutive k of one plane with a single
+// 16-byte global load, straight into the MFMA operand layout; no LDS.
 // Workgroup tile 64 x 64, 4 waves of 32 x 32 (2 x 2 MFMA 16x16 blocks).
-// LDS holds the bf16 hi/lo planes of A ([row][k]) and B ([col][k]) so a lane
-// reads its 8 consecutive k with one 16-byte load.
+// Each real product x*y is xh*yh + xh*yl + xl*yh (FP32 accumulation);
+// complex = 4 real products, 12 MFMA per 16x16x32 complex block.
 // v_mfma_f32_16x16x32_bf16 layout: A[i][k]: lane i + 16 (k/8), element k%8;
 // B[k][j]: lane j + 16 (k/8), element k%8; D[i][j]: lane j + 16 (i/4), reg i%4.
+// Tiles are assigned so that consecutive workgroups of one XCD (ids that
+// differ by the XCD count) sweep neighbouring tiles and share operands in
+// that XCD's L2.
 // ---------------------------------------------------------------------------
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 constexpr int CT = 64;
-constexpr int CK = 64;
-constexpr int KP = CK + 8;  // padded k stride (bf16 elements)
+constexpr int NXCD = 8;
 
-__device__ inline void split_bf16(float v, __bf16& hi, __bf16& lo) {
-    hi = (__bf16)v;
-    lo = (__bf16)(v - (float)hi);
+__device__ inline bf8 neg_bf8(bf8 x) {
+    u4 u = __builtin_bit_cast(u4, x);
+    u ^= 0x80008000u;
+    return __builtin_bit_cast(bf8, u);
 }
 
-__global__ __launch_bounds__(256) void k_cgemm_bf16x3(int m, int nc, int K,
-                                                      const cf* __restrict__ A, size_t lda,
-                                                      const cf* __restrict__ B, size_t ldb,
+__global__ __launch_bounds__(256) void k_cgemm_planes(int m, int nc, const __bf16* __restrict__ la,
+                                                      int ar0, const __bf16* __restrict__ ub,
+                                                      int bc0, size_t pstride,
                                                       cf* __restrict__ C, size_t ldc) {
-    // planes: 0 re_hi, 1 re_lo, 2 im_hi, 3 im_lo
-    __shared__ __attribute__((aligned(16))) __bf16 sA[4][CT * KP];
-    __shared__ __attribute__((aligned(16))) __bf16 sB[4][CT * KP];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int bm = blockIdx.x * CT, bn = blockIdx.y * CT;
-    for (int t = tid; t < CK * CT; t += 256) {
-        int k = t / CT, i = t - k * CT;
-        cf v = (k < K && bm + i < m) ? A[(size_t)k * lda + bm + i] : cf{0.f, 0.f};
-        __bf16 h, l;
-        split_bf16(v.x, h, l);
-        sA[0][i * KP + k] = h;
-        sA[1][i * KP + k] = l;
-        split_bf16(v.y, h, l);
-        sA[2][i * KP + k] = h;
-        sA[3][i * KP + k] = l;
-    }
-    for (int t = tid; t < CK * CT; t += 256) {
-        int j = t / CK, k = t - j * CK;
-        cf v = (k < K && bn + j < nc) ? B[(size_t)(bn + j) * ldb + k] : cf{0.f, 0.f};
-        __bf16 h, l;
-        split_bf16(v.x, h, l);
-        sB[0][j * KP + k] = h;
-        sB[1][j * KP + k] = l;
-        split_bf16(v.y, h, l);
-        sB[2][j * KP + k] = h;
-        sB[3][j * KP + k] = l;
-    }
-    __syncthreads();
+    const int gx = (m + CT - 1) / CT, gy = (nc + CT - 1) / CT;
+    const int tiles = gx * gy;
+    // XCD-aware tile order: workgroup id b runs on XCD b % NXCD; give each
+    // XCD a contiguous range of tiles.
+    const int per = (tiles + NXCD - 1) / NXCD;
+    const int b = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
+    if (b >= tiles) return;
+    const int bm = (b % gx) * CT, bn = (b / gx) * CT;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     const int li = lane & 15, lk = (lane >> 4) * 8;
     f4 accR[2][2], accI[2][2];
+    // C tile into the accumulators: a lane's 4 D registers are 4 consecutive
+    // rows of one column, 32 contiguous bytes (two 16-byte loads when aligned)
+#pragma unroll
     for (int x = 0; x < 2; ++x)
+#pragma unroll
         for (int y = 0; y < 2; ++y) {
-            int col = bn + wn + 16 * y + li;
-            for (int r = 0; r < 4; ++r) {
-                int row = bm + wm + 16 * x + (lane >> 4) * 4 + r;
-                cf v = (row < m && col < nc) ? C[(size_t)col * ldc + row] : cf{0.f, 0.f};
-                accR[x][y][r] = v.x;
-                accI[x][y][r] = v.y;
+            const int col = bn + wn + 16 * y + li;
+            const int row0 = bm + wm + 16 * x + (lane >> 4) * 4;
+            const cf* cp = C + (size_t)col * ldc + row0;
+            if (col < nc && row0 + 3 < m && ((uintptr_t)cp & 15) == 0) {
+                const float4 v0 = *(const float4*)cp, v1 = *(const float4*)(cp + 2);
+                accR[x][y] = f4{v0.x, v0.z, v1.x, v1.z};
+                accI[x][y] = f4{v0.y, v0.w, v1.y, v1.w};
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    cf v = (row0 + r < m && col < nc) ? cp[r] : cf{0.f, 0.f};
+                    accR[x][y][r] = v.x;
+                    accI[x][y][r] = v.y;
+                }
             }
         }
-    for (int k = 0; k < CK; k += 32) {
-        bf8 arh[2], arl[2], aih[2], ail[2], narh[2], narl[2], naih[2], nail[2];
+    size_t aoff[2], boff[2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) aoff[x] = (size_t)(ar0 + min(bm + wm + 16 * x + li, m - 1)) * CT + lk;
+#pragma unroll
+    for (int y = 0; y < 2; ++y) boff[y] = (size_t)(bc0 + min(bn + wn + 16 * y + li, nc - 1)) * CT + lk;
+#pragma unroll
+    for (int k = 0; k < CT; k += 32) {
+        bf8 narh[2], narl[2], aih[2], naih[2], nail[2], ail[2];
+#pragma unroll
         for (int x = 0; x < 2; ++x) {
-            int off = (wm + 16 * x + li) * KP + k + lk;
-            arh[x] = *(const bf8*)&sA[0][off];
-            arl[x] = *(const bf8*)&sA[1][off];
-            aih[x] = *(const bf8*)&sA[2][off];
-            ail[x] = *(const bf8*)&sA[3][off];
-            narh[x] = -arh[x];
-            narl[x] = -arl[x];
-            naih[x] = -aih[x];
-            nail[x] = -ail[x];
+            const __bf16* p = la + aoff[x] + k;
+            bf8 arh = *(const bf8*)p;
+            bf8 arl = *(const bf8*)(p + pstride);
+            aih[x] = *(const bf8*)(p + 2 * pstride);
+            ail[x] = *(const bf8*)(p + 3 * pstride);
+            narh[x] = neg_bf8(arh);
+            narl[x] = neg_bf8(arl);
+            naih[x] = neg_bf8(aih[x]);
+            nail[x] = neg_bf8(ail[x]);
         }
+#pragma unroll
         for (int y = 0; y < 2; ++y) {
-            int off = (wn + 16 * y + li) * KP + k + lk;
-            bf8 brh = *(const bf8*)&sB[0][off];
-            bf8 brl = *(const bf8*)&sB[1][off];
-            bf8 bih = *(const bf8*)&sB[2][off];
-            bf8 bil = *(const bf8*)&sB[3][off];
+            const __bf16* p = ub + boff[y] + k;
+            const bf8 brh = *(const bf8*)p;
+            const bf8 brl = *(const bf8*)(p + pstride);
+            const bf8 bih = *(const bf8*)(p + 2 * pstride);
+            const bf8 bil = *(const bf8*)(p + 3 * pstride);
+#pragma unroll
             for (int x = 0; x < 2; ++x) {
                 f4 r = accR[x][y], i = accI[x][y];
                 // Cr -= Ar Br - Ai Bi
@@ -372,14 +475,29 @@ __global__ __launch_bounds__(256) void k_cgemm_bf16x3(int m, int nc, int K,
             }
         }
     }
+#pragma unroll
     for (int x = 0; x < 2; ++x)
+#pragma unroll
         for (int y = 0; y < 2; ++y) {
-            int col = bn + wn + 16 * y + li;
-            for (int r = 0; r < 4; ++r) {
-                int row = bm + wm + 16 * x + (lane >> 4) * 4 + r;
-                if (row < m && col < nc) C[(size_t)col * ldc + row] = cf{accR[x][y][r], accI[x][y][r]};
+            const int col = bn + wn + 16 * y + li;
+            const int row0 = bm + wm + 16 * x + (lane >> 4) * 4;
+            cf* cp = C + (size_t)col * ldc + row0;
+            const f4 r = accR[x][y], i = accI[x][y];
+            if (col < nc && row0 + 3 < m && ((uintptr_t)cp & 15) == 0) {
+                *(float4*)cp = float4{r[0], i[0], r[1], i[1]};
+                *(float4*)(cp + 2) = float4{r[2], i[2], r[3], i[3]};
+            } else if (col < nc) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (row0 + q < m) cp[q] = cf{r[q], i[q]};
             }
         }
+}
+
+// Number of workgroups of k_cgemm_planes for an m x nc update.
+static int cgemm_groups(int m, int nc) {
+    const int tiles = ((m + CT - 1) / CT) * ((nc + CT - 1) / CT);
+    return ((tiles + NXCD - 1) / NXCD) * NXCD;
 }
 
 // ---------------------------------------------------------------------------
@@ -488,7 +606,7 @@ __global__ void k_fill_i(int* p, int v, int cnt) {
     } while (0)
 
 // Workspace slots 9.. of LuWork (0..8 belong to the FP64 path).
-enum { M_A32 = 9, M_PERM, M_FLAGS, M_STATUS, M_VEC, M_V, M_PART, M_HOST, M_PTMP };
+enum { M_A32 = 9, M_PERM, M_FLAGS, M_STATUS, M_VEC, M_V, M_PART, M_HOST, M_PTMP, M_PLANES };
 
 struct MixedFactor {
     cf* A = nullptr;
@@ -497,6 +615,7 @@ struct MixedFactor {
     int* status = nullptr;
     int* epoch = nullptr;   // host-side counter living in LuWork
     int n = 0, nblk = 0, coop = 1;
+    size_t ld = 0;  // leading dimension of A
 };
 
 static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda,
@@ -505,7 +624,8 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     const int nblk = (n + NBM - 1) / NBM;
     F.n = n;
     F.nblk = nblk;
-    F.A = (cf*)w.get(M_A32, sizeof(cf) * (size_t)n * n, &e);
+    F.ld = ((size_t)n + 31) & ~(size_t)31;  // 16-byte aligned 4-row groups
+    F.A = (cf*)w.get(M_A32, sizeof(cf) * F.ld * n, &e);
     MX_CHK(e);
     F.lperm = (int*)w.get(M_PERM, sizeof(int) * (size_t)nblk * NBM, &e);
     MX_CHK(e);
@@ -528,26 +648,58 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     int per2 = 0;
     MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_trsv_upper_coop, 256, 0));
     F.coop = std::max(1, std::min(nblk, ncu * std::min(per, per2)));
+    MX_CHK(w.ensure_side());
+    // bf16 planes of L21 and U12, double-buffered by step parity: step k+1
+    // writes one buffer while the bulk update of step k still reads the other
+    const size_t pstride = (size_t)n * NBM;
+    __bf16* planes = (__bf16*)w.get(M_PLANES, sizeof(__bf16) * pstride * 4 * 4, &e);
+    MX_CHK(e);
     size_t tot = (size_t)n * n;
+    const size_t ld = F.ld;
     hipLaunchKernelGGL(k_cvt_z2c, dim3((tot + 255) / 256), dim3(256), 0, st, A64, lda, F.A,
-                       (size_t)n, n);
+                       ld, n);
+    // Look-ahead over two streams.  Step k (block columns: k, N = k+1, R =
+    // the rest):
+    //   st:   getrf(k), L21(k); [wait bulk(k-1)] U12(k) on N, update of N
+    //   side: [wait L21(k)] U12(k) on R, update of R (the bulk)
+    // so getrf(k+1) and L21(k+1) run while the bulk update of step k does.
+    const hipStream_t sd = w.side;
+    bool bulk_pending = false;
     for (int k = 0; k < nblk; ++k) {
         const int k0 = k * NBM, nb = std::min(NBM, n - k0);
         int* lp = F.lperm + (size_t)k * NBM;
-        hipLaunchKernelGGL(k_diag_getrf32, dim3(1), dim3(256), 0, st, F.A, (size_t)n, k0, nb, lp,
+        __bf16* la = planes + (size_t)(k & 1) * 8 * pstride;
+        __bf16* ub = la + 4 * pstride;
+        hipLaunchKernelGGL(k_diag_getrf32, dim3(1), dim3(256), 0, st, F.A, ld, k0, nb, lp,
                            F.status);
         const int c0 = k0 + nb, rest = n - c0;
-        if (rest <= 0) continue;
-        hipLaunchKernelGGL((k_trsm_lower_cols<cf, true>),
-                           dim3((rest + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG), dim3(256), 0,
-                           st, F.A, (size_t)n, k0, nb, c0, rest, (const int*)lp);
+        if (rest <= 0) break;
         hipLaunchKernelGGL(k_trsm_upper_rows, dim3((rest + TRR_ROWS - 1) / TRR_ROWS), dim3(256),
-                           0, st, F.A, (size_t)n, k0, nb, c0, rest);
-        hipLaunchKernelGGL(k_cgemm_bf16x3, dim3((rest + CT - 1) / CT, (rest + CT - 1) / CT),
-                           dim3(256), 0, st, rest, rest, nb, F.A + (size_t)k0 * n + c0, (size_t)n,
-                           F.A + (size_t)c0 * n + k0, (size_t)n, F.A + (size_t)c0 * n + c0,
-                           (size_t)n);
+                           0, st, F.A, ld, k0, nb, c0, rest, la, pstride);
+        const int nn1 = std::min(NBM, rest), nr = rest - nn1;
+        if (nr > 0) MX_CHK(hipEventRecord(w.ev_panel, st));
+        if (bulk_pending) MX_CHK(hipStreamWaitEvent(st, w.ev_bulk, 0));
+        hipLaunchKernelGGL((k_trsm_lower_cols<cf, true>),
+                           dim3((nn1 + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG), dim3(256), 0,
+                           st, F.A, ld, k0, nb, c0, nn1, (const int*)lp, ub, pstride);
+        hipLaunchKernelGGL(k_cgemm_planes, dim3(cgemm_groups(rest, nn1)), dim3(256), 0, st, rest,
+                           nn1, (const __bf16*)la, c0, (const __bf16*)ub, c0, pstride,
+                           F.A + c0 * ld + c0, ld);
+        bulk_pending = false;
+        if (nr > 0) {
+            const int cr = c0 + nn1;
+            MX_CHK(hipStreamWaitEvent(sd, w.ev_panel, 0));
+            hipLaunchKernelGGL((k_trsm_lower_cols<cf, true>),
+                               dim3((nr + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG), dim3(256), 0,
+                               sd, F.A, ld, k0, nb, cr, nr, (const int*)lp, ub, pstride);
+            hipLaunchKernelGGL(k_cgemm_planes, dim3(cgemm_groups(rest, nr)), dim3(256), 0, sd,
+                               rest, nr, (const __bf16*)la, c0, (const __bf16*)ub, cr, pstride,
+                               F.A + cr * ld + c0, ld);
+            MX_CHK(hipEventRecord(w.ev_bulk, sd));
+            bulk_pending = true;
+        }
     }
+    if (bulk_pending) MX_CHK(hipStreamWaitEvent(st, w.ev_bulk, 0));
     MX_CHK(hipGetLastError());
     int hs = 0;
     MX_CHK(hipMemcpyAsync(&hs, F.status, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -562,7 +714,7 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
 // z = M^-1 v = U^-1 L^-1 P v   (tmp: work vector of n; z may alias v)
 static hipError_t precond(hipStream_t st, const MixedFactor& F, const cd* v, cd* z, cd* tmp) {
     int n = F.n, nblk = F.nblk;
-    size_t lda = (size_t)n;
+    size_t lda = F.ld;
     const cf* A = F.A;
     const int* lp = F.lperm;
     int* flags = F.flags;
