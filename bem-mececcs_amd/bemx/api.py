@@ -29,7 +29,8 @@ EXPORTS = [
     "bemx_create", "bemx_destroy", "bemx_last_error", "bemx_gauleg", "bemx_send_shared_data",
     "bemx_deallocate_shared_data", "bemx_ghmatece", "bemx_send_dyn_data", "bemx_ghmatecd",
     "bemx_linsolve", "bemx_interec", "bemx_solve", "bemx_device_buffers", "bemx_last_stats",
-    "bemx_send_load_cases",
+    "bemx_send_load_cases", "bemx_material", "bemx_server_create", "bemx_server_submit",
+    "bemx_server_wait", "bemx_server_destroy", "bemx_request_footprint",
 ]
 
 
@@ -63,6 +64,15 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
     lib.bemx_solve.argtypes = [vp, _dp, _dp, _dp, C.c_double, C.c_int, _dp, _dp]
     lib.bemx_device_buffers.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
     lib.bemx_last_stats.argtypes = [vp, _dp]
+    lib.bemx_material.argtypes = [C.c_double] * 4 + [_dp] * 3
+    lib.bemx_server_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(vp)]
+    lib.bemx_server_submit.argtypes = ([vp, _dp, _dp, _dp, _ip] + [C.c_int] * 4 +
+                                       [C.c_double] * 5 + [C.c_int, _dp, _ip,
+                                                           C.POINTER(C.c_longlong)])
+    lib.bemx_server_wait.argtypes = [vp, C.c_longlong, _dp, _dp, C.c_char_p, C.c_int]
+    lib.bemx_server_destroy.argtypes = [vp]
+    lib.bemx_request_footprint.argtypes = [C.c_int, C.c_int, C.c_int]
+    lib.bemx_request_footprint.restype = C.c_double
     return lib
 
 
@@ -272,6 +282,76 @@ class Context:
         if want_x and nr > 1:
             x = x.reshape(nn, nr, order="F")
         return x, stats
+
+
+class Server:
+    """Request loop on one device (bemx_server_*): continuous batching over
+    ``nslots`` worker slots with HBM-footprint admission."""
+
+    def __init__(self, device: int = 0, nslots: int = 2, hbm_budget: float = 0.0,
+                 mode: int = MODE_MIXED):
+        self._lib = lib()
+        h = C.c_void_p()
+        code = self._lib.bemx_server_create(device, nslots, float(hbm_budget), mode, C.byref(h))
+        if code:
+            raise BemxError(code, "bemx_server_create(nslots=%d)" % nslots)
+        self.h = h
+        self._sizes = {}
+
+    def close(self):
+        if self.h:
+            self._lib.bemx_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, mesh, kode, zdfi, zmat_params, fr: float) -> int:
+        """zdfi: (3NBE,) or (3NBE, nrhs) complex load cases; zmat_params =
+        (ge, rnu, dam, rho).  Returns the request id."""
+        zdfi = np.asfortranarray(np.asarray(zdfi, dtype=np.complex128).reshape(len(kode), -1))
+        kode = np.ascontiguousarray(kode, dtype=np.int32)
+        cx, cy, cz = (np.ascontiguousarray(v, dtype=np.float64) for v in (mesh.cx, mesh.cy, mesh.cz))
+        cone = mesh.cone_f()  # column-major CONE(N,4) memory
+        ge, rnu, dam, rho = (float(v) for v in zmat_params)
+        rid = C.c_longlong(0)
+        code = self._lib.bemx_server_submit(
+            self.h, _p(cx), _p(cy), _p(cz), _pi(cone), len(cx), mesh.n, mesh.nbe, mesh.npg,
+            ge, rnu, dam, rho, float(fr), zdfi.shape[1],
+            _p(zdfi.reshape(-1, order="F").view(np.float64)), _pi(kode), C.byref(rid))
+        if code:
+            raise BemxError(code, "bemx_server_submit")
+        self._sizes[rid.value] = zdfi.shape
+        return rid.value
+
+    def wait(self, rid: int):
+        """-> (x (3NBE,) or (3NBE, nrhs), info dict)."""
+        shape = self._sizes.pop(rid)
+        x = np.zeros(shape[0] * shape[1], dtype=np.complex128)
+        info = np.zeros(10)
+        msg = C.create_string_buffer(512)
+        code = self._lib.bemx_server_wait(self.h, rid, _p(x.view(np.float64)), _p(info), msg, 512)
+        if code:
+            raise BemxError(code, "request %d: %s" % (rid, msg.value.decode(errors="replace")))
+        if shape[1] > 1:
+            x = x.reshape(shape, order="F")
+        keys = ("latency_ms", "queue_ms", "service_ms", "gmres_iters", "relres", "fell_back",
+                "slot", "device_ms", "start_ms", "done_ms")
+        return x, dict(zip(keys, info.tolist()))
+
+
+def request_footprint(nbe: int, nrhs: int = 1, mode: int = MODE_MIXED) -> int:
+    """Bytes the request loop charges a request against its HBM budget."""
+    return int(lib().bemx_request_footprint(nbe, nrhs, mode))
+
+
+def material_c(ge: float, rnu: float, dam: float, rho: float) -> np.ndarray:
+    """bemx_material: the library's own ZGE/ZCS/ZCP."""
+    out = [np.zeros(2) for _ in range(3)]
+    code = lib().bemx_material(ge, rnu, dam, rho, *[_p(o) for o in out])
+    if code:
+        raise BemxError(code, "bemx_material")
+    return np.array([complex(o[0], o[1]) for o in out])
 
 
 def solve_problem(ctx: Context, mesh, dyn, freq_index: int = 0, mode: int = MODE_FP64):

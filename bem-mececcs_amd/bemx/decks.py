@@ -252,6 +252,22 @@ def central_elements(nx: int, ny: int, w: int = 2) -> List[int]:
     return [i + nx * j for j in range(j0, j0 + w) for i in range(i0, i0 + w)]
 
 
+def random_load_cases(nx: int, ny: int, nbe: int, count: int, rng) -> np.ndarray:
+    """``count`` random central traction patches, (count, 3 NBE): a w x w
+    block of elements (w in 1..3) near the centre of the nx x ny patch
+    loaded with unit traction along one axis."""
+    cases = np.zeros((count, 3 * nbe), dtype=np.float64)
+    for c in range(count):
+        w = int(rng.integers(1, 4))
+        i0 = int(rng.integers(max(0, nx // 2 - 4), min(nx - w, nx // 2 + 3) + 1))
+        j0 = int(rng.integers(max(0, ny // 2 - 4), min(ny - w, ny // 2 + 3) + 1))
+        comp = int(rng.integers(0, 3))
+        for j in range(j0, j0 + w):
+            for i in range(i0, i0 + w):
+                cases[c, 3 * (i + nx * j) + comp] = 1.0
+    return cases
+
+
 def flat_patch_problem(nx: int, ny: int, half: float = 5.0, npg: int = 8,
                        ge: float = 1.0, rnu: float = 0.25, dam: float = 0.01,
                        rho: float = 1.0, freqs: Sequence[float] = (1.0,),
@@ -276,17 +292,7 @@ def flat_patch_problem(nx: int, ny: int, half: float = 5.0, npg: int = 8,
                           (n_interior, 3))
     else:
         pts = np.zeros((0, 3))
-    cases = None
-    if load_cases > 0:
-        cases = np.zeros((load_cases, 3 * nbe), dtype=np.float64)
-        for c in range(load_cases):
-            w = int(rng.integers(1, 4))
-            i0 = int(rng.integers(max(0, nx // 2 - 4), min(nx - w, nx // 2 + 3) + 1))
-            j0 = int(rng.integers(max(0, ny // 2 - 4), min(ny - w, ny // 2 + 3) + 1))
-            comp = int(rng.integers(0, 3))
-            for j in range(j0, j0 + w):
-                for i in range(i0, i0 + w):
-                    cases[c, 3 * (i + nx * j) + comp] = 1.0
+    cases = random_load_cases(nx, ny, nbe, load_cases, rng) if load_cases > 0 else None
     dyn = Dynamic("DYNAMIC DATA " + mesh.title, pts.shape[0], float(dam), float(rho),
                   np.asarray(freqs, dtype=np.float64), kode, bc, pts[:, 0].copy(),
                   pts[:, 1].copy(), pts[:, 2].copy(), cases)
@@ -315,3 +321,33 @@ def config_problem(name: str, seed: int = 0):
         ge = float(rng.uniform(0.5, 2.0))
     return flat_patch_problem(cfg["nx"], cfg["ny"], cfg["half"], cfg["npg"], ge=ge,
                               rnu=rnu, freqs=freqs, seed=seed)
+
+
+# Serving stream: mesh size classes of the request mix.
+SERVE_CLASSES = ((16, 15), (24, 22), (32, 30), (48, 45), (64, 60))
+
+
+def serving_stream(n_requests: int = 2000, rate: float = 20.0, seed: int = 0,
+                   classes=SERVE_CLASSES):
+    """The request mix of the serving workload: Poisson arrivals at ``rate``
+    requests/s, a mesh class drawn uniformly from ``classes`` (flat patches,
+    half-width 5, 8 x 8 Gauss), omega ~ U[0.1, 10], damping ~ U[0.005, 0.05]
+    and 1..4 load cases (random central traction patches).  Returns
+    (meshes, kodes, requests) with requests a list of dicts (t_arrival, cls,
+    omega, dam, zdfi (3 NBE, nrhs))."""
+    rng = np.random.default_rng(seed)
+    meshes = [flat_patch(nx, ny) for (nx, ny) in classes]
+    kodes = [np.ones(3 * m.nbe, dtype=np.int32) for m in meshes]
+    t = 0.0
+    reqs = []
+    for _ in range(n_requests):
+        t += float(rng.exponential(1.0 / rate))
+        c = int(rng.integers(0, len(classes)))
+        nx, ny = classes[c]
+        omega = float(rng.uniform(0.1, 10.0))
+        dam = float(rng.uniform(0.005, 0.05))
+        nrhs = int(rng.integers(1, 5))
+        cases = random_load_cases(nx, ny, meshes[c].nbe, nrhs, rng)
+        reqs.append(dict(t_arrival=t, cls=c, omega=omega, dam=dam,
+                         zdfi=np.asfortranarray(cases.T.astype(np.complex128))))
+    return meshes, kodes, reqs

@@ -14,11 +14,11 @@ CSRC = os.path.join(HERE, "csrc")
 FSRC = os.path.join(HERE, "fortran")
 OUT = os.path.join(HERE, "bemx", "libbemx.so")
 FOUT = os.path.join(HERE, "bemx", "bemx_main")
-SOURCES = ["bemx_api.hip", "assembly.hip", "lu.hip", "lu_mixed.hip"]
+SOURCES = ["bemx_api.hip", "assembly.hip", "lu.hip", "lu_mixed.hip", "bemx_serve.hip"]
 FSOURCES = ["bemx_iface.f90", "bemx_surface.f90", "bemx_main.f90"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
-FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-Werror=return-type",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function"]
 

@@ -22,6 +22,11 @@ struct LuWork {
     hipStream_t side = nullptr;
     hipEvent_t ev_panel = nullptr, ev_bulk = nullptr;
     hipError_t ensure_side();
+    // graph of the mixed factorisation and the key it was captured for
+    bool use_graphs = true;
+    hipGraphExec_t gexec = nullptr;
+    const void* gkey[8] = {nullptr};
+    LuWork();
     void* get(int slot, size_t bytes, hipError_t* e);
     void release();
 };

@@ -54,12 +54,12 @@ constexpr int TRSM_COLS_PER_WG = 64;
 // X = L11^-1 (P X) for the TB rows k0.. of columns [c0, c0+nc), in place
 // (and, for complex float with planes != NULL, split into the bf16 planes).
 // L11 = unit lower triangle of A[k0:k0+nb, k0:k0+nb].  A workgroup of 256
-// threads stages L11 and its 64 columns (gathered by the block permutation
-// lperm, row r <- row lperm[r], with PERM) in LDS with all loads in flight;
-// then wave 0 solves, one lane per column, in four chunks of 16 rows: the
-// chunk is held in registers, updated with the solved rows of the earlier
-// chunks and solved in place.  L11 reads are wave-uniform LDS broadcasts,
-// the column reads conflict-free.
+// threads loads L11 and its 64 columns (gathered by the block permutation
+// lperm, row r <- row lperm[r], with PERM) with all 32 loads of a thread in
+// flight, then solves one lane per column, by chunks of 16 rows: wave 0
+// solves chunk g in registers, then the four waves update the later chunks
+// with it in parallel.  L11 reads are wave-uniform LDS broadcasts, the
+// column reads conflict-free.
 template <class T, bool PERM>
 __global__ __launch_bounds__(256) void k_trsm_lower_cols(T* __restrict__ A, size_t lda, int k0,
                                                          int nb, int c0, int nc,
@@ -70,33 +70,34 @@ __global__ __launch_bounds__(256) void k_trsm_lower_cols(T* __restrict__ A, size
     __shared__ T Ls[TB * TB];   // Ls[i*TB + r] = L11[r][i]
     __shared__ T X[TB * XS];    // X[r*XS + lane] = row r of the lane's column
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-#pragma unroll 4
-    for (int t = tid; t < TB * TB; t += 256) {
-        int i = t >> 6, r = t & 63;
-        Ls[t] = (i < nb && r < nb) ? A[(size_t)(k0 + i) * lda + k0 + r] : zero_of<T>();
-    }
     const int cb = c0 + blockIdx.x * TRSM_COLS_PER_WG;
     const int ncols = min(TRSM_COLS_PER_WG, c0 + nc - cb);
     {
+        T lv[16], xv[16];
         // lane = row of column j (coalesced), wave w takes columns w, w+4, ...
         const int src = (lane < nb) ? (PERM ? lperm[lane] : lane) : 0;
-#pragma unroll 4
-        for (int j = w; j < TRSM_COLS_PER_WG; j += 4)
-            X[lane * XS + j] = (lane < nb && j < ncols) ? A[(size_t)(cb + j) * lda + k0 + src]
-                                                       : zero_of<T>();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int t = tid + 256 * q, i = t >> 6, r = t & 63;
+            lv[q] = (i < nb && r < nb) ? A[(size_t)(k0 + i) * lda + k0 + r] : zero_of<T>();
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int j = w + 4 * q;
+            xv[q] = (lane < nb && j < ncols) ? A[(size_t)(cb + j) * lda + k0 + src] : zero_of<T>();
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) Ls[tid + 256 * q] = lv[q];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) X[lane * XS + w + 4 * q] = xv[q];
     }
     __syncthreads();
-    if (w == 0) {
 #pragma unroll
-        for (int g = 0; g < TB; g += 16) {
+    for (int g = 0; g < TB; g += 16) {
+        if (w == 0) {
             T a[16];
 #pragma unroll
             for (int r = 0; r < 16; ++r) a[r] = X[(g + r) * XS + lane];
-            for (int i = 0; i < g; ++i) {
-                const T xi = X[i * XS + lane];
-#pragma unroll
-                for (int r = 0; r < 16; ++r) a[r] = a[r] - Ls[i * TB + g + r] * xi;
-            }
 #pragma unroll
             for (int i = 0; i < 15; ++i) {
 #pragma unroll
@@ -105,8 +106,23 @@ __global__ __launch_bounds__(256) void k_trsm_lower_cols(T* __restrict__ A, size
 #pragma unroll
             for (int r = 0; r < 16; ++r) X[(g + r) * XS + lane] = a[r];
         }
+        __syncthreads();
+        const int g2 = g + 16 * (1 + w);  // the chunk this wave updates
+        if (g2 < TB) {
+            T a[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) a[r] = X[(g2 + r) * XS + lane];
+#pragma unroll 4
+            for (int i = 0; i < 16; ++i) {
+                const T xi = X[(g + i) * XS + lane];
+#pragma unroll
+                for (int r = 0; r < 16; ++r) a[r] = a[r] - Ls[(g + i) * TB + g2 + r] * xi;
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) X[(g2 + r) * XS + lane] = a[r];
+        }
+        __syncthreads();
     }
-    __syncthreads();
 #pragma unroll 4
     for (int j = w; j < TRSM_COLS_PER_WG; j += 4) {
         if (lane < nb && j < ncols) A[(size_t)(cb + j) * lda + k0 + lane] = X[lane * XS + j];

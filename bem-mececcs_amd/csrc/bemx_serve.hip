@@ -1,0 +1,316 @@
+// bemx_serve.hip -- device-resident request loop for BEM solves on one GPU.
+//
+// Continuous batching: requests (a mesh, a material, a frequency and 1..4
+// load cases) are queued as they arrive.  NSLOTS worker threads each own a
+// device context (its own streams and grow-only workspace); a free worker
+// takes the oldest queued request as soon as its HBM footprint fits in what
+// the budget has left, runs assembly + factorisation + refinement for it
+// with bemx_solve and hands the solution back.  Requests that arrive while
+// others are on the GPU are admitted as slots and memory free up, and the
+// work of different slots overlaps on the device: while one solve sits in
+// the latency-bound chain of its factorisation, another's bulk update fills
+// the CUs.  A slot keeps the mesh and static pass (GHMATECE) of the last
+// mesh it served, so a stream of requests on one mesh (a frequency sweep)
+// pays the static pass once per slot.  Each request's arithmetic is the
+// same whichever slot runs it and whatever runs beside it.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <condition_variable>
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/bemx.h"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double ms_between(clk::time_point a, clk::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+struct Request {
+    long long id = 0;
+    std::vector<double> cx, cy, cz;
+    std::vector<int> cone;
+    int np = 0, n = 0, nbe = 0, npg = 0;
+    double ge = 0, rnu = 0, dam = 0, rho = 0, fr = 0;
+    int nrhs = 1;
+    std::vector<double> zdfi;  // interleaved complex, 3NBE x nrhs
+    std::vector<int> kode;
+    uint64_t mesh_key = 0;
+    size_t footprint = 0;
+    clk::time_point t_submit, t_start, t_done;
+    bool done = false;
+    int status = BEMX_OK;
+    std::string err;
+    std::vector<double> x;
+    double stats[4] = {0, 0, 0, 0};
+    int slot = -1;
+};
+
+}  // namespace
+
+struct bemx_server {
+    int device = 0;
+    clk::time_point t0 = clk::now();
+    int mode = BEMX_MODE_MIXED;
+    size_t budget = 0, in_use = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> queue;
+    std::map<long long, std::unique_ptr<Request>> reqs;
+    long long next_id = 1;
+    bool stop = false;
+    std::vector<std::thread> workers;
+    std::vector<bemx_ctx*> ctxs;
+    std::string err;
+};
+
+namespace {
+
+// Device bytes one request keeps live while it runs: ZH (complex128), the
+// FP32 copy, bf16 panels and the refinement vectors of the mixed path, and
+// the per-chunk partial right-hand sides of the assembly.
+size_t request_footprint(int nbe, int nrhs, int mode) {
+    const size_t nn = 3 * (size_t)nbe;
+    const size_t ld32 = (nn + 31) & ~(size_t)31;
+    size_t b = nn * nn * 16;                                    // ZH
+    b += (size_t)((nbe + 3) / 4) * nn * nrhs * 16;               // assembly partial RHS
+    b += nn * nrhs * 16 * 4;                                     // ZDFI, ZFI, diagonal RHS
+    if (mode == BEMX_MODE_MIXED) {
+        b += ld32 * nn * 8;                                      // FP32 factors
+        b += nn * 64 * 2 * 16;                                   // bf16 panels
+        b += nn * 16 * (5 + 13) + ((nn + 127) / 128) * nn * 16;  // GMRES vectors, GEMV partials
+    } else {
+        b += nn * 16 * 2 + nn * 64 * 16;                         // solve vector, panel candidates
+    }
+    return b;
+}
+
+void run_request(bemx_server* s, bemx_ctx* c, uint64_t* slot_key, Request* r) {
+    int st = BEMX_OK;
+    if (*slot_key != r->mesh_key) {
+        *slot_key = 0;
+        st = bemx_send_shared_data(c, r->cx.data(), r->cy.data(), r->cz.data(), nullptr, nullptr,
+                                   nullptr, nullptr, nullptr, nullptr, r->cone.data(), r->np,
+                                   r->npg, r->n, r->nbe);
+        if (!st)
+            st = bemx_ghmatece(c, r->ge, r->rnu, nullptr, nullptr, nullptr, nullptr, nullptr,
+                               nullptr, nullptr, nullptr);
+        if (!st) *slot_key = r->mesh_key;
+    }
+    double zge[2], zcs[2], zcp[2];
+    if (!st) st = bemx_material(r->ge, r->rnu, r->dam, r->rho, zge, zcs, zcp);
+    if (!st) st = bemx_send_load_cases(c, r->nrhs, r->zdfi.data(), r->kode.data());
+    if (!st) {
+        r->x.assign((size_t)6 * r->nbe * r->nrhs, 0.0);
+        st = bemx_solve(c, zge, zcs, zcp, r->fr, s->mode, r->x.data(), r->stats);
+    }
+    r->status = st;
+    if (st) r->err = bemx_last_error(c);
+}
+
+void worker_main(bemx_server* s, int slot) {
+    bemx_ctx* c = s->ctxs[slot];
+    uint64_t key = 0;
+    for (;;) {
+        Request* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            // FIFO admission: the oldest request runs as soon as a slot is
+            // free and its footprint fits in the budget left
+            s->cv_work.wait(lk, [&] {
+                return s->stop ||
+                       (!s->queue.empty() && s->in_use + s->queue.front()->footprint <= s->budget);
+            });
+            if (s->stop) return;
+            r = s->queue.front();
+            s->queue.pop_front();
+            s->in_use += r->footprint;
+            r->slot = slot;
+            r->t_start = clk::now();
+        }
+        run_request(s, c, &key, r);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->in_use -= r->footprint;
+            r->t_done = clk::now();
+            r->done = true;
+        }
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int bemx_material(double ge, double rnu, double dam, double rho, double* zge, double* zcs,
+                  double* zcp) {
+    // mu* = G (1 + 2i DAM), cs = sqrt(mu*/rho), cp = cs sqrt((2-2nu)/(1-2nu))
+    if (!zge || !zcs || !zcp || !(rho > 0.0) || rnu >= 0.5) return BEMX_ERR_ARG;
+    const std::complex<double> g(ge, ge * 2.0 * dam);
+    const std::complex<double> cs = std::sqrt(g / rho);
+    const std::complex<double> cp = cs * std::sqrt((2.0 - 2.0 * rnu) / (1.0 - 2.0 * rnu));
+    zge[0] = g.real();
+    zge[1] = g.imag();
+    zcs[0] = cs.real();
+    zcs[1] = cs.imag();
+    zcp[0] = cp.real();
+    zcp[1] = cp.imag();
+    return BEMX_OK;
+}
+
+double bemx_request_footprint(int nbe, int nrhs, int mode) {
+    if (nbe <= 0 || nrhs < 1) return 0.0;
+    return (double)request_footprint(nbe, nrhs, mode);
+}
+
+int bemx_server_create(int device, int nslots, double hbm_budget_bytes, int mode,
+                       bemx_server** out) {
+    if (!out || nslots < 1 || nslots > 16) return BEMX_ERR_ARG;
+    if (mode != BEMX_MODE_FP64 && mode != BEMX_MODE_MIXED) return BEMX_ERR_ARG;
+    *out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) return BEMX_ERR_HIP;
+    size_t freeb = 0, totb = 0;
+    if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return BEMX_ERR_HIP;
+    auto* s = new bemx_server();
+    s->device = device;
+    s->mode = mode;
+    s->budget = hbm_budget_bytes > 0 ? (size_t)hbm_budget_bytes : (size_t)(0.8 * (double)freeb);
+    for (int i = 0; i < nslots; ++i) {
+        bemx_ctx* c = nullptr;
+        int st = bemx_create(device, &c);
+        if (st) {
+            for (bemx_ctx* d : s->ctxs) bemx_destroy(d);
+            delete s;
+            return st;
+        }
+        s->ctxs.push_back(c);
+    }
+    for (int i = 0; i < nslots; ++i) s->workers.emplace_back(worker_main, s, i);
+    *out = s;
+    return BEMX_OK;
+}
+
+int bemx_server_submit(bemx_server* s, const double* cx, const double* cy, const double* cz,
+                       const int* cone, int np, int n, int nbe, int npg, double ge, double rnu,
+                       double dam, double rho, double fr, int nrhs, const double* zdfi,
+                       const int* kode, long long* id) {
+    if (!s || !cx || !cy || !cz || !cone || !zdfi || !kode || !id) return BEMX_ERR_ARG;
+    if (np <= 0 || n <= 0 || nbe <= 0 || nbe > n || npg <= 0 || nrhs < 1 || nrhs > 4 ||
+        !(fr != 0.0))
+        return BEMX_ERR_ARG;
+    auto r = std::make_unique<Request>();
+    r->cx.assign(cx, cx + np);
+    r->cy.assign(cy, cy + np);
+    r->cz.assign(cz, cz + np);
+    r->cone.assign(cone, cone + 4 * (size_t)n);
+    r->np = np;
+    r->n = n;
+    r->nbe = nbe;
+    r->npg = npg;
+    r->ge = ge;
+    r->rnu = rnu;
+    r->dam = dam;
+    r->rho = rho;
+    r->fr = fr;
+    r->nrhs = nrhs;
+    r->zdfi.assign(zdfi, zdfi + (size_t)6 * nbe * nrhs);
+    r->kode.assign(kode, kode + (size_t)3 * nbe);
+    uint64_t h = 1469598103934665603ull;
+    int dims[4] = {np, n, nbe, npg};
+    double mat[2] = {ge, rnu};
+    h = fnv1a(h, dims, sizeof dims);
+    h = fnv1a(h, mat, sizeof mat);
+    h = fnv1a(h, cx, sizeof(double) * np);
+    h = fnv1a(h, cy, sizeof(double) * np);
+    h = fnv1a(h, cz, sizeof(double) * np);
+    h = fnv1a(h, cone, sizeof(int) * 4 * (size_t)n);
+    r->mesh_key = h ? h : 1;
+    r->footprint = request_footprint(nbe, nrhs, s->mode);
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return BEMX_ERR_STATE;
+    if (r->footprint > s->budget) {
+        s->err = "request footprint exceeds the HBM budget";
+        return BEMX_ERR_NOMEM;
+    }
+    r->id = s->next_id++;
+    r->t_submit = clk::now();
+    *id = r->id;
+    s->queue.push_back(r.get());
+    s->reqs[r->id] = std::move(r);
+    s->cv_work.notify_all();
+    return BEMX_OK;
+}
+
+int bemx_server_wait(bemx_server* s, long long id, double* x, double* info, char* msg,
+                     int msglen) {
+    if (!s) return BEMX_ERR_ARG;
+    std::unique_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(id);
+        if (it == s->reqs.end()) return BEMX_ERR_ARG;
+        Request* p = it->second.get();
+        s->cv_done.wait(lk, [&] { return p->done || s->stop; });
+        if (!p->done) return BEMX_ERR_STATE;
+        r = std::move(it->second);
+        s->reqs.erase(it);
+    }
+    if (x && !r->status) memcpy(x, r->x.data(), sizeof(double) * r->x.size());
+    if (info) {
+        info[0] = ms_between(r->t_submit, r->t_done);
+        info[1] = ms_between(r->t_submit, r->t_start);
+        info[2] = ms_between(r->t_start, r->t_done);
+        info[3] = r->stats[0];
+        info[4] = r->stats[1];
+        info[5] = r->stats[2];
+        info[6] = r->slot;
+        info[7] = r->stats[3];
+        info[8] = ms_between(s->t0, r->t_start);
+        info[9] = ms_between(s->t0, r->t_done);
+    }
+    if (msg && msglen > 0) {
+        strncpy(msg, r->err.c_str(), (size_t)msglen - 1);
+        msg[msglen - 1] = 0;
+    }
+    return r->status;
+}
+
+int bemx_server_destroy(bemx_server* s) {
+    if (!s) return BEMX_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    s->cv_done.notify_all();
+    for (std::thread& t : s->workers) t.join();
+    for (bemx_ctx* c : s->ctxs) bemx_destroy(c);
+    delete s;
+    return BEMX_OK;
+}
+
+}  // extern "C"

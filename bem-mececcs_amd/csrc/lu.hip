@@ -19,6 +19,7 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -45,12 +46,21 @@ void LuWork::release() {
         buf[i] = nullptr;
         cap[i] = 0;
     }
+    if (gexec) hipGraphExecDestroy(gexec);
+    gexec = nullptr;
     if (side) hipStreamDestroy(side);
     if (ev_panel) hipEventDestroy(ev_panel);
     if (ev_bulk) hipEventDestroy(ev_bulk);
     side = nullptr;
     ev_panel = ev_bulk = nullptr;
     flags_init = nullptr;
+}
+
+LuWork::LuWork() {
+    // BEMX_GRAPHS=0 replaces the graph replay of the mixed factorisation by
+    // eager launches (for A/B timing)
+    const char* g = getenv("BEMX_GRAPHS");
+    use_graphs = !(g && g[0] == '0');
 }
 
 hipError_t LuWork::ensure_side() {

@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstring>
 #include <vector>
 
 #include "../../include/bemx.h"
@@ -142,11 +143,11 @@ __global__ __launch_bounds__(256) void k_diag_getrf32(cf* __restrict__ A, size_t
     }
 }
 
-// L21 = A21 U11^-1 for rows [r0, r0+nr).  256 threads stage U11 and 64
-// rows in LDS with all loads in flight; wave 0 then solves, one lane per
-// row, left to right in four chunks of 16 columns (chunk in registers,
-// updated with the solved columns of the earlier chunks, then solved in
-// place).  U11 reads are wave-uniform broadcasts.  The result goes back to
+// L21 = A21 U11^-1 for rows [r0, r0+nr).  256 threads load U11 and 64
+// rows with all 32 loads of a thread in flight; the rows are then solved one
+// lane per row, left to right by chunks of 16 columns: wave 0 solves chunk g
+// in registers, then the four waves update the later chunks with it in
+// parallel.  U11 reads are wave-uniform broadcasts.  The result goes back to
 // A and, split, to the planes la.
 constexpr int TRR_ROWS = 64;
 __global__ __launch_bounds__(256) void k_trsm_upper_rows(cf* __restrict__ A, size_t lda, int k0,
@@ -157,33 +158,37 @@ __global__ __launch_bounds__(256) void k_trsm_upper_rows(cf* __restrict__ A, siz
     __shared__ cf X[NBM * XS];    // X[c*XS + lane] = column c of the lane's row
     __shared__ cf dinv[NBM];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-#pragma unroll 4
-    for (int t = tid; t < NBM * NBM; t += 256) {
-        int c2 = t >> 6, c = t & 63;
-        Us[t] = (c2 < nb && c < nb) ? A[(size_t)(k0 + c2) * lda + k0 + c] : mkf(0.f, 0.f);
-    }
     const int rb = r0 + blockIdx.x * TRR_ROWS;
     const bool ok = rb + lane < r0 + nr;
-#pragma unroll 4
-    for (int c = w; c < NBM; c += 4)
-        X[c * XS + lane] = (ok && c < nb) ? A[(size_t)(k0 + c) * lda + rb + lane] : mkf(0.f, 0.f);
+    {
+        cf uv[16], xv[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int t = tid + 256 * q, c2 = t >> 6, c = t & 63;
+            uv[q] = (c2 < nb && c < nb) ? A[(size_t)(k0 + c2) * lda + k0 + c] : mkf(0.f, 0.f);
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int c = w + 4 * q;
+            xv[q] = (ok && c < nb) ? A[(size_t)(k0 + c) * lda + rb + lane] : mkf(0.f, 0.f);
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) Us[tid + 256 * q] = uv[q];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) X[(w + 4 * q) * XS + lane] = xv[q];
+    }
     __syncthreads();
     if (tid < NBM) {
         cf u = Us[tid * NBM + tid];
         dinv[tid] = (u.x != 0.f || u.y != 0.f) ? cinvf(u) : mkf(0.f, 0.f);
     }
     __syncthreads();
-    if (w == 0) {
 #pragma unroll
-        for (int g = 0; g < NBM; g += 16) {
+    for (int g = 0; g < NBM; g += 16) {
+        if (w == 0) {
             cf a[16];
 #pragma unroll
             for (int c = 0; c < 16; ++c) a[c] = X[(g + c) * XS + lane];
-            for (int i = 0; i < g; ++i) {
-                const cf xi = X[i * XS + lane];
-#pragma unroll
-                for (int c = 0; c < 16; ++c) a[c] = a[c] - xi * Us[(g + c) * NBM + i];
-            }
 #pragma unroll
             for (int c = 0; c < 16; ++c) {
                 a[c] = a[c] * dinv[g + c];
@@ -193,8 +198,23 @@ __global__ __launch_bounds__(256) void k_trsm_upper_rows(cf* __restrict__ A, siz
 #pragma unroll
             for (int c = 0; c < 16; ++c) X[(g + c) * XS + lane] = a[c];
         }
+        __syncthreads();
+        const int g2 = g + 16 * (1 + w);  // the chunk this wave updates
+        if (g2 < NBM) {
+            cf a[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) a[c] = X[(g2 + c) * XS + lane];
+#pragma unroll 4
+            for (int i = 0; i < 16; ++i) {
+                const cf xi = X[(g + i) * XS + lane];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) a[c] = a[c] - xi * Us[(g2 + c) * NBM + g + i];
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c) X[(g2 + c) * XS + lane] = a[c];
+        }
+        __syncthreads();
     }
-    __syncthreads();
 #pragma unroll 4
     for (int c = w; c < NBM; c += 4)
         if (ok && c < nb) A[(size_t)(k0 + c) * lda + rb + lane] = X[c * XS + lane];
@@ -618,42 +638,16 @@ struct MixedFactor {
     size_t ld = 0;  // leading dimension of A
 };
 
-static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda,
-                        MixedFactor& F, std::string& err) {
-    hipError_t e;
-    const int nblk = (n + NBM - 1) / NBM;
-    F.n = n;
-    F.nblk = nblk;
-    F.ld = ((size_t)n + 31) & ~(size_t)31;  // 16-byte aligned 4-row groups
-    F.A = (cf*)w.get(M_A32, sizeof(cf) * F.ld * n, &e);
-    MX_CHK(e);
-    F.lperm = (int*)w.get(M_PERM, sizeof(int) * (size_t)nblk * NBM, &e);
-    MX_CHK(e);
-    const size_t cap_before = w.cap[M_FLAGS];
-    F.flags = (int*)w.get(M_FLAGS, sizeof(int) * (size_t)nblk, &e);
-    MX_CHK(e);
-    if (w.flags_init != (void*)F.flags || w.cap[M_FLAGS] != cap_before) {
-        // fresh flag array: clear it once
-        hipLaunchKernelGGL(k_fill_i, dim3((nblk + 255) / 256), dim3(256), 0, st, F.flags, 0, nblk);
-        w.flags_init = F.flags;
-    }
-    F.epoch = &w.epoch;
-    F.status = (int*)w.get(M_STATUS, sizeof(int), &e);
-    MX_CHK(e);
+// The launch sequence of the factorisation (no host synchronisation, no
+// allocation), so it can be captured into a graph.
+static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda,
+                                 const MixedFactor& F, __bf16* planes, size_t pstride) {
+    hipError_t first = hipSuccess;
+    auto chk = [&](hipError_t e) {
+        if (first == hipSuccess && e != hipSuccess) first = e;
+    };
+    const int nblk = F.nblk;
     hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, st, F.status, 0, 1);
-    int dev = 0, ncu = 0, per = 0;
-    MX_CHK(hipGetDevice(&dev));
-    MX_CHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trsv_lower_coop, 256, 0));
-    int per2 = 0;
-    MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_trsv_upper_coop, 256, 0));
-    F.coop = std::max(1, std::min(nblk, ncu * std::min(per, per2)));
-    MX_CHK(w.ensure_side());
-    // bf16 planes of L21 and U12, double-buffered by step parity: step k+1
-    // writes one buffer while the bulk update of step k still reads the other
-    const size_t pstride = (size_t)n * NBM;
-    __bf16* planes = (__bf16*)w.get(M_PLANES, sizeof(__bf16) * pstride * 4 * 4, &e);
-    MX_CHK(e);
     size_t tot = (size_t)n * n;
     const size_t ld = F.ld;
     hipLaunchKernelGGL(k_cvt_z2c, dim3((tot + 255) / 256), dim3(256), 0, st, A64, lda, F.A,
@@ -677,8 +671,8 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
         hipLaunchKernelGGL(k_trsm_upper_rows, dim3((rest + TRR_ROWS - 1) / TRR_ROWS), dim3(256),
                            0, st, F.A, ld, k0, nb, c0, rest, la, pstride);
         const int nn1 = std::min(NBM, rest), nr = rest - nn1;
-        if (nr > 0) MX_CHK(hipEventRecord(w.ev_panel, st));
-        if (bulk_pending) MX_CHK(hipStreamWaitEvent(st, w.ev_bulk, 0));
+        if (nr > 0) chk(hipEventRecord(w.ev_panel, st));
+        if (bulk_pending) chk(hipStreamWaitEvent(st, w.ev_bulk, 0));
         hipLaunchKernelGGL((k_trsm_lower_cols<cf, true>),
                            dim3((nn1 + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG), dim3(256), 0,
                            st, F.A, ld, k0, nb, c0, nn1, (const int*)lp, ub, pstride);
@@ -688,18 +682,82 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
         bulk_pending = false;
         if (nr > 0) {
             const int cr = c0 + nn1;
-            MX_CHK(hipStreamWaitEvent(sd, w.ev_panel, 0));
+            chk(hipStreamWaitEvent(sd, w.ev_panel, 0));
             hipLaunchKernelGGL((k_trsm_lower_cols<cf, true>),
                                dim3((nr + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG), dim3(256), 0,
                                sd, F.A, ld, k0, nb, cr, nr, (const int*)lp, ub, pstride);
             hipLaunchKernelGGL(k_cgemm_planes, dim3(cgemm_groups(rest, nr)), dim3(256), 0, sd,
                                rest, nr, (const __bf16*)la, c0, (const __bf16*)ub, cr, pstride,
                                F.A + cr * ld + c0, ld);
-            MX_CHK(hipEventRecord(w.ev_bulk, sd));
+            chk(hipEventRecord(w.ev_bulk, sd));
             bulk_pending = true;
         }
     }
-    if (bulk_pending) MX_CHK(hipStreamWaitEvent(st, w.ev_bulk, 0));
+    if (bulk_pending) chk(hipStreamWaitEvent(st, w.ev_bulk, 0));
+    chk(hipGetLastError());
+    return first;
+}
+
+static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda,
+                        MixedFactor& F, std::string& err) {
+    hipError_t e;
+    const int nblk = (n + NBM - 1) / NBM;
+    F.n = n;
+    F.nblk = nblk;
+    F.ld = ((size_t)n + 31) & ~(size_t)31;  // 16-byte aligned 4-row groups
+    F.A = (cf*)w.get(M_A32, sizeof(cf) * F.ld * n, &e);
+    MX_CHK(e);
+    F.lperm = (int*)w.get(M_PERM, sizeof(int) * (size_t)nblk * NBM, &e);
+    MX_CHK(e);
+    const size_t cap_before = w.cap[M_FLAGS];
+    F.flags = (int*)w.get(M_FLAGS, sizeof(int) * (size_t)nblk, &e);
+    MX_CHK(e);
+    if (w.flags_init != (void*)F.flags || w.cap[M_FLAGS] != cap_before) {
+        // fresh flag array: clear it once
+        hipLaunchKernelGGL(k_fill_i, dim3((nblk + 255) / 256), dim3(256), 0, st, F.flags, 0, nblk);
+        w.flags_init = F.flags;
+    }
+    F.epoch = &w.epoch;
+    F.status = (int*)w.get(M_STATUS, sizeof(int), &e);
+    MX_CHK(e);
+    int dev = 0, ncu = 0, per = 0;
+    MX_CHK(hipGetDevice(&dev));
+    MX_CHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trsv_lower_coop, 256, 0));
+    int per2 = 0;
+    MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_trsv_upper_coop, 256, 0));
+    F.coop = std::max(1, std::min(nblk, ncu * std::min(per, per2)));
+    MX_CHK(w.ensure_side());
+    // bf16 planes of L21 and U12, double-buffered by step parity: step k+1
+    // writes one buffer while the bulk update of step k still reads the other
+    const size_t pstride = (size_t)n * NBM;
+    __bf16* planes = (__bf16*)w.get(M_PLANES, sizeof(__bf16) * pstride * 4 * 4, &e);
+    MX_CHK(e);
+    // The launch sequence depends only on n and the buffer addresses: it is
+    // captured once into a graph and replayed while those stay the same.
+    const void* key[8] = {(const void*)(size_t)n, A64, (const void*)lda, F.A, planes, F.lperm,
+                          F.status, w.side};
+    if (w.use_graphs) {
+        if (!w.gexec || memcmp(key, w.gkey, sizeof key) != 0) {
+            if (w.gexec) MX_CHK(hipGraphExecDestroy(w.gexec));
+            w.gexec = nullptr;
+            hipGraph_t g = nullptr;
+            MX_CHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            const hipError_t e1 = enqueue_factor(w, st, n, A64, lda, F, planes, pstride);
+            const hipError_t e2 = hipStreamEndCapture(st, &g);
+            if (e1 != hipSuccess || e2 != hipSuccess) {
+                if (g) hipGraphDestroy(g);
+                MX_CHK(e1 != hipSuccess ? e1 : e2);
+            }
+            const hipError_t e3 = hipGraphInstantiate(&w.gexec, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            MX_CHK(e3);
+            memcpy(w.gkey, key, sizeof key);
+        }
+        MX_CHK(hipGraphLaunch(w.gexec, st));
+    } else {
+        MX_CHK(enqueue_factor(w, st, n, A64, lda, F, planes, pstride));
+    }
     MX_CHK(hipGetLastError());
     int hs = 0;
     MX_CHK(hipMemcpyAsync(&hs, F.status, sizeof(int), hipMemcpyDeviceToHost, st));

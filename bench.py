@@ -1,22 +1,37 @@
 """Benchmark: BEM solves/s and p50/p99 latency, mixed precision vs plain FP64.
 
-Workload: the 48 x 45 flat free-surface patch (2160 boundary elements + 5
-enclosing, 6480 complex DOF), mu = 1, nu = 0.25, rho = 1, damping 0.01,
-omega = 2.0, 8 x 8 Gauss points, unit vertical traction on the central 2 x 2
-elements.  One step = one solve at one frequency, everything resident in HBM
-before the timed region: dynamic assembly of ZH and the RHS (FP64), factor,
-refine, solve.  The mixed path (FP32 LU with split-BF16x3 MFMA trailing
-update + FP64 GMRES-IR) is the headline `value`; the plain FP64 partial-pivot
-path is timed beside it, and every mixed result is compared with the FP64
-result of the same step (the accuracy contract is <= 1e-10 relative).
+Workloads (--workload):
+  solve  (default) the 48 x 45 flat free-surface patch (2160 boundary
+         elements + 5 enclosing, 6480 complex DOF), mu = 1, nu = 0.25,
+         rho = 1, damping 0.01, omega = 2.0, 8 x 8 Gauss points, unit
+         vertical traction on the central 2 x 2 elements.  A step is one
+         solve at one frequency with everything resident in HBM before the
+         timed region: dynamic assembly of ZH and the RHS (FP64), factor,
+         refine, solve.  Single-solve latency is measured on one context
+         (mixed and FP64 interleaved, every mixed result compared with the
+         FP64 result of the same step); `value` is the throughput of the
+         request loop (bemx_server) running the same solves on --slots
+         concurrent slots.
+  sweep  the same mesh, 256 frequencies log-spaced in [0.1, 10] rad/s, each
+         a full assembly + factor + refine, through the request loop.
+  serve  2000 requests with Poisson arrivals: meshes from {16x15, 24x22,
+         32x30, 48x45, 64x60}, omega ~ U[0.1, 10], damping ~ U[0.005, 0.05],
+         1..4 load cases; p50/p99 of submit-to-done latency.
+  large  96 x 90 patch over [-10, 10]^2 (8640 elements, 25920 complex DOF),
+         omega = 0.5, 6 x 6 Gauss, nu and mu drawn by seed; mixed path with
+         the plain FP64 path as reference.
+The mixed path (FP32 LU with split-BF16x3 MFMA trailing update + FP64
+GMRES-IR) is the headline; the plain FP64 partial-pivot path is measured
+beside it and the accuracy contract is <= 1e-10 relative to FP64.
 
-The CPU baseline is the oracle (C, OpenMP, FP64, LAPACK zgesv via numpy) on
-the same problem, one solve, on the host's cores.
+The CPU baseline (default workload) is the oracle (C, OpenMP, FP64, LAPACK
+zgesv via numpy) on the same problem, one solve, on the host's cores.
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -34,75 +49,108 @@ METRIC = ("BEM solves/s + p50/p99 latency at ≤1e-10 err vs FP64, mixed vs plai
 FLOPS_PER_TRIPLE = 242
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
+DTYPE = "f64 assembly + f32/bf16x3 factor + f64 refine"
 
 
 def percentile(v, q):
     return float(np.percentile(np.asarray(v), q)) if len(v) else None
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py runs on one GPU: the solve is not sharded across devices "
-              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
-        sys.exit(2)
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
-    from bemx import api, decks
 
-    mesh, dyn = decks.config_problem(args.config)
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def base_record(args, workload, value, ms_per_step, steps):
+    return {
+        "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": DTYPE,
+        "data": "synthetic (generated flat-patch meshes)",
+        "config": dict(workload, parallelism="single GPU, %d request-loop slots" % args.slots),
+    }
+
+
+def stream_through_server(api, mode, slots, items, label):
+    """Submit every item at once to a request loop of `slots` slots; items
+    are (mesh, kode, zdfi, (ge, rnu, dam, rho), omega).  Returns (x list,
+    info list, wall seconds)."""
+    srv = api.Server(0, nslots=slots, mode=mode)
+    try:
+        t0 = time.perf_counter()
+        ids = [srv.submit(*it) for it in items]
+        outs = [srv.wait(i) for i in ids]
+        wall = time.perf_counter() - t0
+    finally:
+        srv.close()
+    svc = [o[1]["service_ms"] for o in outs]
+    log("%s: %d solves in %.3f s on %d slots (%.2f solves/s), service p50 %.2f ms" %
+        (label, len(items), wall, slots, len(items) / wall, percentile(svc, 50)))
+    return [o[0] for o in outs], [o[1] for o in outs], wall
+
+
+# ---------------------------------------------------------------------------
+def workload_solve(args, api, decks):
+    mesh, dyn = decks.config_problem("c2")
     fr = float(dyn.freqs[0])
     zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
     ctx = api.Context(0)
     ctx.send_shared_data(mesh)
     ctx.ghmatece(mesh.ge, mesh.rnu)
     ctx.send_dyn_data(dyn.bc.astype(np.complex128), dyn.kode)
-    print("workload %s: nbe=%d n=%d dof=%d npg=%d omega=%g" %
-          (args.config, mesh.nbe, mesh.n, 3 * mesh.nbe, mesh.npg, fr), file=sys.stderr, flush=True)
-
+    log("workload c2: nbe=%d n=%d dof=%d npg=%d omega=%g" %
+        (mesh.nbe, mesh.n, 3 * mesh.nbe, mesh.npg, fr))
     for _ in range(args.warmup):
         ctx.solve(zmat, fr, api.MODE_FP64)
         ctx.solve(zmat, fr, api.MODE_MIXED)
-    print("warmup done", file=sys.stderr, flush=True)
+    log("warmup done")
 
-    # ---- timed: mixed path (headline) and plain FP64 path, interleaved
+    # ---- single-solve latency: mixed and plain FP64, interleaved
     lat_mixed, lat_fp64, errs, iters, fallbacks, offdiag_ms = [], [], [], [], 0, []
-    t_mixed = t_fp64 = 0.0
     for step in range(args.steps):
         t0 = time.perf_counter()
         x64, _ = ctx.solve(zmat, fr, api.MODE_FP64)
         t1 = time.perf_counter()
         st64 = ctx.last_stats()
-        t2 = time.perf_counter()
         xm, _ = ctx.solve(zmat, fr, api.MODE_MIXED)
-        t3 = time.perf_counter()
+        t2 = time.perf_counter()
         stm = ctx.last_stats()
-        t_fp64 += t1 - t0
-        t_mixed += t3 - t2
         lat_fp64.append((t1 - t0) * 1e3)
-        lat_mixed.append((t3 - t2) * 1e3)
+        lat_mixed.append((t2 - t1) * 1e3)
         offdiag_ms.extend([st64["offdiag_ms"], stm["offdiag_ms"]])
         iters.append(stm["gmres_iters"])
         fallbacks += int(stm["fell_back"])
-        errs.append(float(np.max(np.abs(xm - x64)) / np.max(np.abs(x64))))
-        print("step %d: fp64 %.2f ms (assembly %.2f, factor+solve %.2f), mixed %.2f ms "
-              "(assembly %.2f, factor+refine %.2f, %d GMRES it), rel err %.2e" %
-              (step, lat_fp64[-1], st64["assembly_ms"], st64["factor_ms"], lat_mixed[-1],
-               stm["assembly_ms"], stm["factor_ms"], stm["gmres_iters"], errs[-1]),
-              file=sys.stderr, flush=True)
+        errs.append(rel_err(xm, x64))
+        log("step %d: fp64 %.2f ms (assembly %.2f, factor+solve %.2f), mixed %.2f ms "
+            "(assembly %.2f, factor+refine %.2f, %d GMRES it), rel err %.2e" %
+            (step, lat_fp64[-1], st64["assembly_ms"], st64["factor_ms"], lat_mixed[-1],
+             stm["assembly_ms"], stm["factor_ms"], stm["gmres_iters"], errs[-1]))
+    ctx.close()
 
-    value = args.steps / t_mixed
-    fp64_value = args.steps / t_fp64
+    # ---- throughput: the same solves through the request loop
+    item = (mesh, dyn.kode, dyn.bc.astype(np.complex128), (mesh.ge, mesh.rnu, dyn.dam, dyn.rho),
+            fr)
+    n_stream = max(args.steps, 2 * args.slots)
+    stream_through_server(api, api.MODE_MIXED, args.slots, [item] * args.slots, "warm mixed")
+    xs, infos, wall = stream_through_server(api, api.MODE_MIXED, args.slots, [item] * n_stream,
+                                            "mixed stream")
+    n64 = max(2, n_stream // 2)
+    stream_through_server(api, api.MODE_FP64, args.slots, [item] * args.slots, "warm fp64")
+    x64s, _, wall64 = stream_through_server(api, api.MODE_FP64, args.slots, [item] * n64,
+                                            "fp64 stream")
+    errs.extend(rel_err(x, x64s[0]) for x in xs)
+    value = n_stream / wall
+    fp64_value = n64 / wall64
+
     triples = mesh.nbe * mesh.nbe * mesh.npg ** 2
     od = float(np.mean(offdiag_ms))
     achieved = FLOPS_PER_TRIPLE * triples / (od * 1e-3) / 1e12
 
     cpu = None
+    err_oracle = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
@@ -111,36 +159,27 @@ def main():
         t0 = time.perf_counter()
         ref = orc.solve_problem(mesh, dyn)
         tcpu = time.perf_counter() - t0
-        err_oracle = float(np.max(np.abs(xm - ref["x"])) / np.max(np.abs(ref["x"])))
+        err_oracle = rel_err(xm, ref["x"])
         cpu = {"value": 1.0 / tcpu, "unit": "solves/s", "cores": threads, "kind": "port",
                "sample": "one full solve of the same workload (GHMATECE, GHMATECD, LAPACK "
                          "zgesv, INTEREC at 64 points) by the C oracle with OpenMP on %d "
                          "threads: %.1f s" % (threads, tcpu)}
-        print("cpu oracle solve %.1f s; mixed vs oracle rel err %.2e" % (tcpu, err_oracle),
-              file=sys.stderr, flush=True)
-    else:
-        err_oracle = None
+        log("cpu oracle solve %.1f s; mixed vs oracle rel err %.2e" % (tcpu, err_oracle))
 
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * t_mixed / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64 assembly + f32/bf16x3 factor + f64 refine",
-        "data": "synthetic (generated flat-patch mesh)",
-        "config": {"workload": "flat patch 48x45 (2160 BE + 5 EE, 6480 complex DOF), "
-                               "omega=2.0, 8x8 Gauss, one frequency per solve",
-                   "parallelism": "single GPU"},
+    out = base_record(args, {"workload": "flat patch 48x45 (2160 BE + 5 EE, 6480 complex DOF), "
+                                         "omega=2.0, 8x8 Gauss, one frequency per solve"},
+                      value, 1e3 / value, n_stream)
+    out.update({
         "p50_ms": percentile(lat_mixed, 50),
         "p99_ms": percentile(lat_mixed, 99),
+        "latency_note": "p50/p99: one solve alone on one context; value: request-loop "
+                        "throughput on %d slots" % args.slots,
+        "stream_service_p50_ms": percentile([i["service_ms"] for i in infos], 50),
+        "stream_service_p99_ms": percentile([i["service_ms"] for i in infos], 99),
+        "single_context_solves_per_s": 1e3 / float(np.mean(lat_mixed)),
         "fp64": {"value": fp64_value, "unit": "solves/s", "p50_ms": percentile(lat_fp64, 50),
-                 "p99_ms": percentile(lat_fp64, 99)},
+                 "p99_ms": percentile(lat_fp64, 99),
+                 "single_context_solves_per_s": 1e3 / float(np.mean(lat_fp64))},
         "speedup_mixed_vs_fp64": value / fp64_value,
         "max_rel_err_mixed_vs_fp64": max(errs),
         "rel_err_mixed_vs_oracle": err_oracle,
@@ -153,8 +192,160 @@ def main():
                      "avg_ms": od, "flops_per_triple": FLOPS_PER_TRIPLE,
                      "triples": triples},
         "cpu_baseline": cpu,
-    }
-    print(json.dumps(out))
+    })
+    return out
+
+
+def workload_sweep(args, api, decks):
+    mesh, dyn = decks.config_problem("c4")
+    freqs = [float(f) for f in dyn.freqs][: args.nfreq]
+    p = (mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
+    bc = dyn.bc.astype(np.complex128)
+    items = [(mesh, dyn.kode, bc, p, f) for f in freqs]
+    stream_through_server(api, api.MODE_MIXED, args.slots, items[: args.slots], "warm mixed")
+    xs, infos, wall = stream_through_server(api, api.MODE_MIXED, args.slots, items, "mixed sweep")
+    x64, _, wall64 = stream_through_server(api, api.MODE_FP64, args.slots, items, "fp64 sweep")
+    errs = [rel_err(a, b) for a, b in zip(xs, x64)]
+    svc = [i["service_ms"] for i in infos]
+    value = len(items) / wall
+    out = base_record(args, {"workload": "frequency sweep: flat patch 48x45 (6480 complex DOF), "
+                                         "%d frequencies log-spaced in [0.1, 10]" % len(items)},
+                      value, 1e3 / value, len(items))
+    out.update({
+        "p50_ms": percentile(svc, 50), "p99_ms": percentile(svc, 99),
+        "latency_note": "per-frequency service time (assembly + factor + refine) in the loop",
+        "fp64": {"value": len(items) / wall64, "unit": "solves/s"},
+        "speedup_mixed_vs_fp64": value / (len(items) / wall64),
+        "max_rel_err_mixed_vs_fp64": max(errs),
+        "gmres_iters_max": int(max(i["gmres_iters"] for i in infos)),
+        "fallbacks_to_fp64": int(sum(i["fell_back"] for i in infos)),
+    })
+    return out
+
+
+def workload_serve(args, api, decks):
+    meshes, kodes, reqs = decks.serving_stream(args.requests, args.rate, seed=0)
+    srv = api.Server(0, nslots=args.slots, mode=api.MODE_MIXED)
+    # warm every size class on every slot's workspace (4 load cases: the largest)
+    warm = []
+    for c, m in enumerate(meshes):
+        z = np.zeros((3 * m.nbe, 4), np.complex128)
+        z[np.arange(4), np.arange(4)] = 1.0
+        for _ in range(args.slots):
+            warm.append(srv.submit(m, kodes[c], z, (m.ge, m.rnu, 0.01, 1.0), 1.0))
+    for w in warm:
+        srv.wait(w)
+    log("warm done")
+    ids = [None] * len(reqs)
+
+    def feeder():
+        t0 = time.perf_counter()
+        for k, r in enumerate(reqs):
+            dt = r["t_arrival"] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            m = meshes[r["cls"]]
+            ids[k] = srv.submit(m, kodes[r["cls"]], r["zdfi"], (m.ge, m.rnu, r["dam"], 1.0),
+                                r["omega"])
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=feeder)
+    th.start()
+    outs = []
+    for k in range(len(reqs)):
+        while ids[k] is None:
+            time.sleep(0.0005)
+        outs.append(srv.wait(ids[k]))
+        if (k + 1) % 250 == 0:
+            log("%d/%d requests done" % (k + 1, len(reqs)))
+    th.join()
+    wall = time.perf_counter() - t0
+    srv.close()
+    lat = [o[1]["latency_ms"] for o in outs]
+    # accuracy: a sample of requests re-solved alone on the plain FP64 path
+    errs = []
+    ctx = api.Context(0)
+    for k in range(0, len(reqs), max(1, len(reqs) // args.check)):
+        r = reqs[k]
+        m = meshes[r["cls"]]
+        ctx.send_shared_data(m)
+        ctx.ghmatece(m.ge, m.rnu)
+        ctx.send_load_cases(r["zdfi"], kodes[r["cls"]])
+        x64, _ = ctx.solve(api.material(m.ge, m.rnu, r["dam"], 1.0), r["omega"], api.MODE_FP64)
+        xm = outs[k][0]
+        errs.append(rel_err(xm.reshape(x64.shape, order="F"), x64))
+    ctx.close()
+    value = len(reqs) / wall
+    out = base_record(args, {"workload": "serving stream: %d requests, Poisson arrivals at %.1f/s, "
+                                         "meshes 16x15..64x60 (240-3840 BE), omega U[0.1,10], "
+                                         "damping U[0.005,0.05], 1-4 load cases" %
+                                         (len(reqs), args.rate)},
+                      value, 1e3 / value, len(reqs))
+    out.update({
+        "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+        "latency_note": "submit-to-done latency per request (queueing included)",
+        "max_rel_err_mixed_vs_fp64": max(errs), "checked_requests": len(errs),
+        "fallbacks_to_fp64": int(sum(o[1]["fell_back"] for o in outs)),
+        "offered_rate": args.rate,
+    })
+    return out
+
+
+def workload_large(args, api, decks):
+    mesh, dyn = decks.config_problem("c3", seed=args.seed)
+    fr = float(dyn.freqs[0])
+    zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
+    ctx = api.Context(0)
+    ctx.send_shared_data(mesh)
+    ctx.ghmatece(mesh.ge, mesh.rnu)
+    ctx.send_dyn_data(dyn.bc.astype(np.complex128), dyn.kode)
+    log("workload c3: nbe=%d dof=%d nu=%.3f mu=%.3f" % (mesh.nbe, 3 * mesh.nbe, mesh.rnu, mesh.ge))
+    ctx.solve(zmat, fr, api.MODE_MIXED)
+    lat = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        xm, _ = ctx.solve(zmat, fr, api.MODE_MIXED)
+        lat.append((time.perf_counter() - t0) * 1e3)
+        stm = ctx.last_stats()
+        log("mixed %.1f ms (assembly %.1f, factor+refine %.1f, %d GMRES it)" %
+            (lat[-1], stm["assembly_ms"], stm["factor_ms"], stm["gmres_iters"]))
+    t0 = time.perf_counter()
+    x64, _ = ctx.solve(zmat, fr, api.MODE_FP64)
+    t64 = time.perf_counter() - t0
+    ctx.close()
+    value = 1e3 / float(np.mean(lat))
+    out = base_record(args, {"workload": "large single solve: flat patch 96x90 over [-10,10]^2 "
+                                         "(8640 BE, 25920 complex DOF), omega=0.5, 6x6 Gauss"},
+                      value, float(np.mean(lat)), args.steps)
+    out.update({"p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+                "fp64": {"value": 1.0 / t64, "unit": "solves/s"},
+                "speedup_mixed_vs_fp64": value * t64,
+                "max_rel_err_mixed_vs_fp64": rel_err(xm, x64)})
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="solve", choices=["solve", "sweep", "serve", "large"])
+    ap.add_argument("--slots", type=int, default=2, help="request-loop slots")
+    ap.add_argument("--nfreq", type=int, default=256, help="sweep: number of frequencies")
+    ap.add_argument("--requests", type=int, default=2000, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=20.0, help="serve: arrivals per second")
+    ap.add_argument("--check", type=int, default=20, help="serve: requests re-solved in FP64")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: the solve is not sharded across devices "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    from bemx import api, decks
+    fn = {"solve": workload_solve, "sweep": workload_sweep, "serve": workload_serve,
+          "large": workload_large}[args.workload]
+    print(json.dumps(fn(args, api, decks)))
 
 
 if __name__ == "__main__":

@@ -3,7 +3,7 @@
  *
  * Drop-in replacement for the reference's four assembly/solve subroutines
  * and the device entry points they sat on (giulianobelinassi/BEM-MECECCS,
- * src/kernels/*.cu).  Plain pointers and sizes only.  Arrays use the
+ * the .cu files of src/kernels).  Plain pointers and sizes only.  Arrays use the
  * reference's Fortran layouts:
  *   - real arrays are double (FPREC=double build)
  *   - complex arrays / scalars are interleaved (re, im) doubles
@@ -130,6 +130,45 @@ int bemx_last_stats(const bemx_ctx *ctx, double *out);
 
 /* Device buffer addresses of the last solve (for zero-copy consumers). */
 int bemx_device_buffers(bemx_ctx *ctx, void **zh, void **zfi, void **hip_stream);
+
+/* Complex material constants: mu* = G(1 + 2i DAM), cs = sqrt(mu* / rho),
+ * cp = cs sqrt((2-2nu)/(1-2nu)).  Replaces the INPUTECD lines that form
+ * ZGE/ZCS/ZCP (Inputecd.for:166-168).  Outputs are complex (re, im). */
+int bemx_material(double ge, double rnu, double dam, double rho, double *zge, double *zcs,
+                  double *zcp);
+
+/* ---- Request loop (continuous batching on one GPU) ----------------------
+ * No reference counterpart: the reference solves one deck per process
+ * (Main.for).  nslots worker slots (1..16), each a context with its own
+ * streams, take queued requests oldest first as soon as the request's HBM
+ * footprint (ZH + factor workspace + assembly partials) fits in
+ * hbm_budget_bytes (<= 0: 80% of the free HBM at creation); their work
+ * overlaps on the device.  mode: BEMX_MODE_FP64 or BEMX_MODE_MIXED for
+ * every request.  Mesh arrays as bemx_send_shared_data, material as
+ * bemx_material, ZDFI(3NBE, nrhs) / KODE(3NBE) as bemx_send_load_cases;
+ * everything is copied at submit.  A request larger than the whole budget
+ * is refused with BEMX_ERR_NOMEM. */
+typedef struct bemx_server bemx_server;
+/* Device bytes a request of nbe elements and nrhs load cases is admitted
+ * with (what the budget is charged while it runs). */
+double bemx_request_footprint(int nbe, int nrhs, int mode);
+int bemx_server_create(int device, int nslots, double hbm_budget_bytes, int mode,
+                       bemx_server **srv);
+int bemx_server_submit(bemx_server *srv, const double *cx, const double *cy, const double *cz,
+                       const int *cone, int np, int n, int nbe, int npg, double ge, double rnu,
+                       double dam, double rho, double fr, int nrhs, const double *zdfi,
+                       const int *kode, long long *id);
+/* Blocks until request id is done; returns its status.  x (may be NULL):
+ * 3NBE x nrhs complex solution.  info (may be NULL, 10 doubles): latency
+ * ms (submit to done), queue ms, service ms, GMRES iterations, worst
+ * relative residual, 1 if it fell back to FP64, slot, device ms of the
+ * solve, start and done times in ms since the server was created.  msg
+ * (may be NULL) receives the error text.  Each id can be waited once. */
+int bemx_server_wait(bemx_server *srv, long long id, double *x, double *info, char *msg,
+                     int msglen);
+/* Stops the workers after their current request; requests still queued
+ * are dropped.  Must not run concurrently with bemx_server_wait. */
+int bemx_server_destroy(bemx_server *srv);
 
 #ifdef __cplusplus
 }

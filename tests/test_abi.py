@@ -12,7 +12,7 @@ from bemx import api, decks
 
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "bemx.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(bemx_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|double|const char \*)\s*(bemx_\w+)\s*\(", text, re.M)))
 
 
 def test_header_and_python_agree():
@@ -49,6 +49,27 @@ def test_create_without_device_fails_cleanly():
 def test_material_matches_reference_formula(orc):
     for args in ((1.0, 0.25, 0.01, 1.0), (1.7, 0.31, 0.03, 2.2)):
         np.testing.assert_allclose(api.material(*args), orc.material(*args), rtol=1e-15)
+
+
+def test_library_material_matches_reference_formula(orc):
+    """bemx_material (host code of the library) against the oracle's
+    restatement of Inputecd.for:166-168."""
+    for args in ((1.0, 0.25, 0.01, 1.0), (1.7, 0.31, 0.03, 2.2), (0.5, 0.2, 0.005, 1.3)):
+        np.testing.assert_allclose(api.material_c(*args), orc.material(*args), rtol=1e-15)
+
+
+def test_request_footprint_and_server_without_device():
+    """The admission footprint is at least ZH (16 nn^2 bytes) and grows with
+    the size; a request loop cannot be created without a device."""
+    import pytest
+    for nbe in (240, 2160):
+        nn = 3 * nbe
+        fp = api.request_footprint(nbe, 1, api.MODE_MIXED)
+        assert fp >= 16 * nn * nn + 8 * nn * nn
+        assert api.request_footprint(nbe, 4, api.MODE_MIXED) > fp
+    assert api.request_footprint(2160) > api.request_footprint(240)
+    with pytest.raises(api.BemxError):
+        api.Server(99, 2)
 
 
 def test_deck_round_trip(tmp_path):

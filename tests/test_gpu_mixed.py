@@ -7,6 +7,8 @@ back to the plain FP64 path, in a handful of GMRES iterations (a broken
 BF16x3 product or pivot bookkeeping shows up as a useless preconditioner:
 many iterations or a fallback).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -95,3 +97,29 @@ def test_mixed_bem_solve_matches_fp64_oracle(ctx, orc, case):
     rel = np.max(np.abs(x - ref["x"])) / np.max(np.abs(ref["x"]))
     assert rel < TOL, rel
     assert stats[2] == 0.0  # no fallback
+
+
+@pytest.mark.skipif(os.environ.get("BEMX_GRAPHS") == "0", reason="graphs switched off")
+def test_graph_replays_match_and_recapture_on_size_change():
+    """The mixed factorisation is captured into a graph once per (size,
+    buffers) and replayed: replays of the same system, with eager work and a
+    solve of another size (a re-capture) in between, give bitwise the same
+    answer as the first (captured) run, and every one is within 1e-10 of the
+    FP64 solution."""
+    c = api.Context(0)
+    try:
+        a, b = _rand_system(300, 7)
+        ref = np.linalg.solve(a, b)
+        first = c.linsolve(a, b, api.MODE_MIXED)
+        a2, b2 = _rand_system(200, 8)
+        for rep in range(4):
+            x = c.linsolve(a, b, api.MODE_MIXED)
+            assert np.array_equal(x, first), rep
+            assert np.max(np.abs(x - ref)) / np.max(np.abs(ref)) < TOL
+            if rep == 1:
+                y = c.linsolve(a2, b2, api.MODE_MIXED)  # other size: re-capture
+                yr = np.linalg.solve(a2, b2)
+                assert np.max(np.abs(y - yr)) / np.max(np.abs(yr)) < TOL
+            c.linsolve(a, b, api.MODE_FP64)  # eager work between replays
+    finally:
+        c.close()
