@@ -107,14 +107,19 @@ __device__ inline void wait_count(int* cnt, int target, int* status) {
 
 // ---------------------------------------------------------------------------
 // Panel factorisation.  Workgroup p owns panel rows [r0, r1) (global row
-// indices, r0 = k0 + p*PANEL_ROWS) held column-major in LDS.  For column j
-// (diagonal row d = k0 + j):
-//   1. local best candidate among owned rows >= d
-//   2. publish (val,row) and the candidate row; the owner of row d publishes
-//      row d; release; arrive on counter j; wait for all P arrivals
-//   3. every workgroup picks the same winner, reads the pivot row, swaps the
-//      contents of rows d and piv where it owns them, scales column j and
-//      applies the rank-1 update to columns j+1.. of its rows below d.
+// indices, r0 = k0 + p*PANEL_ROWS) held column-major in LDS.  Column j
+// (diagonal row d = k0 + j) costs one inter-workgroup exchange:
+//   wait   all P workgroups have published their candidate for column j
+//          (best |re|+|im| among owned rows >= d, lowest row on ties, its
+//          complete row) and the owner of row d has published row d;
+//   pick   every workgroup reads the P candidates, picks the same winner and
+//          reads the winner's row and row d (one round of global reads);
+//   apply  swap rows d and piv where owned, scale column j, update column
+//          j+1 (only), find the local candidate for column j+1, complete
+//          that row and row d+1 for the columns after j+1 and publish them;
+//   then   the rest of the rank-1 update of columns j+2.. runs while the
+//          other workgroups reach their own publish (look-ahead by one
+//          column: the exchange of column j+1 overlaps this update).
 // At the end workgroup 0 turns the pivot sequence into the list of rows
 // whose content changed (dst) and where it came from (src).
 // ---------------------------------------------------------------------------
@@ -127,11 +132,11 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
                                                     long long* __restrict__ info,
                                                     int* __restrict__ status) {
     __shared__ cd sA[NB * PANEL_ROWS];
-    __shared__ cd urow[NB];
+    __shared__ cd urow[NB], dbuf[NB];
     __shared__ double rv[4];
     __shared__ int rr[4];
-    __shared__ int s_piv, s_wp;
-    __shared__ double s_pval;
+    __shared__ int s_piv, s_wp, s_cand;
+    __shared__ double s_pval, s_cval;
     const int P = gridDim.x, p = blockIdx.x, tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int r0 = k0 + p * PANEL_ROWS;
@@ -142,13 +147,13 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
         sA[c * PANEL_ROWS + r] = A[(size_t)(k0 + c) * lda + r0 + r];
     }
     __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        const int d = k0 + j;
-        // 1. local candidate
+
+    // local candidate of column jj among owned rows >= dmin -> s_cand, s_cval
+    auto local_cand = [&](int jj, int dmin) {
         double bv = -1.0;
         int br = 0x7fffffff;
-        if (tid < nr && r0 + tid >= d) {
-            bv = cabs1(sA[j * PANEL_ROWS + tid]);
+        if (tid < nr && r0 + tid >= dmin) {
+            bv = cabs1(sA[jj * PANEL_ROWS + tid]);
             br = r0 + tid;
         }
         for (int off = 32; off >= 1; off >>= 1) {
@@ -170,25 +175,37 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
                     bv = rv[w];
                     br = rr[w];
                 }
-            rv[0] = bv;
-            rr[0] = br;
-            slots[(size_t)j * P + p] = Cand{bv, br, 0};
+            s_cand = br;
+            s_cval = bv;
         }
         __syncthreads();
-        // 2. publish rows
-        const int myrow = rr[0];
+    };
+    // publish the candidate row and row k0+jj (complete for column jj on) of step jj
+    auto publish = [&](int jj) {
+        const int myrow = s_cand;
         if (myrow != 0x7fffffff && tid < nb)
-            cand[((size_t)j * P + p) * NB + tid] = sA[tid * PANEL_ROWS + (myrow - r0)];
-        if (d >= r0 && d < r1 && tid < nb) drow[(size_t)j * NB + tid] = sA[tid * PANEL_ROWS + (d - r0)];
+            cand[((size_t)jj * P + p) * NB + tid] = sA[tid * PANEL_ROWS + (myrow - r0)];
+        const int d = k0 + jj;
+        if (d >= r0 && d < r1 && tid >= 64 && tid < 64 + nb)
+            drow[(size_t)jj * NB + tid - 64] = sA[(tid - 64) * PANEL_ROWS + (d - r0)];
         __threadfence();
         __syncthreads();
         if (tid == 0) {
-            atomicAdd(&counters[j], 1);
-            wait_count(&counters[j], P, status);
+            slots[(size_t)jj * P + p] = Cand{s_cval, myrow, 0};
+            __threadfence();
+            atomicAdd(&counters[jj], 1);
         }
+    };
+
+    local_cand(0, k0);
+    publish(0);
+    const int r = tid & (PANEL_ROWS - 1), half = tid >> 7;  // update mapping: row, column parity
+    for (int j = 0; j < nb; ++j) {
+        const int d = k0 + j;
+        if (tid == 0) wait_count(&counters[j], P, status);
         __syncthreads();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        // 3. winner (same on every workgroup)
+        // pick: the winner (wave 0) and row d (wave 1) in one round of reads
         if (wave == 0) {
             double wv = -1.0;
             int wr = 0x7fffffff, wp = 0;
@@ -215,6 +232,8 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
                 s_pval = wv;
                 s_wp = wp;
             }
+        } else if (wave == 1 && lane < nb) {
+            dbuf[lane] = drow[(size_t)j * NB + lane];
         }
         __syncthreads();
         const int piv = s_piv;
@@ -225,30 +244,45 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
             if (pval == 0.0 && *info == 0) *info = d + 1;
         }
         __syncthreads();
+        // apply: swap, scale column j, update column j+1
         if (piv != d) {
             if (d >= r0 && d < r1 && tid < nb) sA[tid * PANEL_ROWS + (d - r0)] = urow[tid];
-            if (piv >= r0 && piv < r1 && tid < nb)
-                sA[tid * PANEL_ROWS + (piv - r0)] = drow[(size_t)j * NB + tid];
+            if (piv >= r0 && piv < r1 && tid < nb) sA[tid * PANEL_ROWS + (piv - r0)] = dbuf[tid];
         }
         __syncthreads();
-        // 4. eliminate below d
-        cd u = urow[j];
-        cd inv = (pval != 0.0) ? cinv(u) : mk(0.0, 0.0);
-        if (tid < nr && r0 + tid > d && pval != 0.0) {
-            cd l = sA[j * PANEL_ROWS + tid] * inv;
+        const cd inv = (pval != 0.0) ? cinv(urow[j]) : mk(0.0, 0.0);
+        const bool live = tid < nr && r0 + tid > d && pval != 0.0;
+        if (live) {
+            const cd l = sA[j * PANEL_ROWS + tid] * inv;
             sA[j * PANEL_ROWS + tid] = l;
+            if (j + 1 < nb) sA[(j + 1) * PANEL_ROWS + tid] = sA[(j + 1) * PANEL_ROWS + tid] - l * urow[j + 1];
         }
         __syncthreads();
-        const int rest = nb - j - 1;
-        for (int t = tid; t < nr * rest; t += 256) {
-            int c = j + 1 + t / nr, r = t - (t / nr) * nr;
-            if (r0 + r > d) sA[c * PANEL_ROWS + r] = sA[c * PANEL_ROWS + r] - sA[j * PANEL_ROWS + r] * urow[c];
+        int sp1 = -1, sp2 = -1;  // local rows already completed for columns j+2..
+        if (j + 1 < nb) {
+            local_cand(j + 1, d + 1);
+            const int cr = s_cand;
+            if (cr != 0x7fffffff) sp1 = cr - r0;
+            if (d + 1 >= r0 && d + 1 < r1 && d + 1 != cr) sp2 = d + 1 - r0;
+            // complete the candidate row (threads 0..63) and row d+1 (64..127)
+            const int sr = tid < 64 ? sp1 : (tid < 128 ? sp2 : -1);
+            const int c = j + 2 + lane;
+            if (sr >= 0 && c < nb && pval != 0.0)
+                sA[c * PANEL_ROWS + sr] = sA[c * PANEL_ROWS + sr] - sA[j * PANEL_ROWS + sr] * urow[c];
+            __syncthreads();
+            publish(j + 1);
+        }
+        // the rest of the rank-1 update (columns j+2..), overlapping the exchange
+        if (r < nr && r0 + r > d && r != sp1 && r != sp2 && pval != 0.0) {
+            const cd l = sA[j * PANEL_ROWS + r];
+            for (int c = j + 2 + half; c < nb; c += 2)
+                sA[c * PANEL_ROWS + r] = sA[c * PANEL_ROWS + r] - l * urow[c];
         }
         __syncthreads();
     }
     for (int t = tid; t < nr * nb; t += 256) {
-        int c = t / nr, r = t - c * nr;
-        A[(size_t)(k0 + c) * lda + r0 + r] = sA[c * PANEL_ROWS + r];
+        int c = t / nr, r2 = t - c * nr;
+        A[(size_t)(k0 + c) * lda + r0 + r2] = sA[c * PANEL_ROWS + r2];
     }
     if (p == 0 && tid == 0) {
         // content[i] = original row now at position key[i]

@@ -18,7 +18,9 @@
 // k_trsv_upper_coop, one cooperative launch each; workgroups own 64-row
 // blocks, accumulate the contributions of finished blocks as their flags
 // come up and publish their own block, so the dependency chain costs one
-// small GEMV and one in-register solve per block instead of one launch.
+// small GEMV and one product with the block's explicit inverse
+// (k_diag_inverses, computed on the side stream during the factorisation)
+// per block instead of one launch.
 // Pivoting is restricted to the diagonal block: the factors only
 // precondition.  The accuracy of the answer comes from the refinement, whose
 // residual b - A64 x is formed in FP64 with the FP64 matrix; if it stalls the
@@ -49,14 +51,29 @@ __global__ void k_cvt_z2c(const cd* __restrict__ a, size_t lda, cf* __restrict__
     o[c * ldo + r] = cf{(float)v.x, (float)v.y};
 }
 
+// Maximum over the wavefront with DPP (no LDS): prefix maxima inside each
+// row of 16 lanes (row_shr 1, 2, 4, 8), then across rows (row_bcast 15 into
+// rows 1 and 3, row_bcast 31 into rows 2 and 3); lane 63 ends with the
+// maximum.  Lanes whose DPP source lies outside the row take 0, the identity.
+__device__ inline unsigned wave_max_u32(unsigned v) {
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 // Wave-uniform copy of lane src's complex float.
 __device__ inline cf readlane_f(cf v, int src) {
     return mkf(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), src)),
                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), src)));
 }
 
-// Diagonal block: GEPP within its nb rows (largest |re|+|im|, lowest block
-// row on ties), held in registers: wave q owns columns [16q, 16q+16), lane r
+// Diagonal block: GEPP within its nb rows (largest |re|+|im| to 17 mantissa
+// bits, lowest block row on ties), held in registers: wave q owns columns
+// [16q, 16q+16), lane r
 // owns block row r.  Rows are not moved while factoring: a row is marked
 // used when it becomes the pivot of step j (its rank), and the factors are
 // written back in rank order.  lperm[i] = block row whose content ends at
@@ -80,17 +97,13 @@ __global__ __launch_bounds__(256) void k_diag_getrf32(cf* __restrict__ A, size_t
     int rank = lane;  // rows >= nb keep their place
     // pivot of column (wave-local index) ii; multipliers stored in v[ii]
     auto search = [&](cf& col, int buf) {
-        float bv = used ? -1.f : fabsf(col.x) + fabsf(col.y);
-        int br = lane;
-        for (int off = 32; off >= 1; off >>= 1) {
-            float ov = __shfl_xor(bv, off, 64);
-            int orr = __shfl_xor(br, off, 64);
-            if (ov > bv || (ov == bv && orr < br)) {
-                bv = ov;
-                br = orr;
-            }
-        }
-        const int p = __builtin_amdgcn_readfirstlane(br);
+        // key = |re|+|im| with its 6 lowest mantissa bits replaced by 63 - lane:
+        // the wave maximum picks the largest candidate (to 17 mantissa bits,
+        // ample for pivoting a preconditioner), lowest row on ties
+        const unsigned key =
+            used ? 0u
+                 : ((__float_as_uint(fabsf(col.x) + fabsf(col.y)) & ~63u) | (unsigned)(63 - lane));
+        const int p = 63 - (int)(wave_max_u32(key) & 63u);
         const cf pv = readlane_f(col, p);
         const bool zero = pv.x == 0.f && pv.y == 0.f;
         const cf inv = zero ? mkf(0.f, 0.f) : cinvf(pv);
@@ -247,6 +260,132 @@ __device__ inline void publish(int* flag, int value) {
     __hip_atomic_store(flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Explicit inverses of diagonal block k for the preconditioner's block
+// solves: Li = L11^-1 P (workgroup 0, one lane per column of P) and
+// Ui = U11^-1 (workgroup 1, one lane per row of I), column-major 64 x 64.
+// Solved like the two triangular kernels (16-wide chunks: wave 0 solves a
+// chunk in registers, the four waves update the later chunks).  A partial
+// last block is padded with the identity.
+__global__ __launch_bounds__(256) void k_diag_inverses(const cf* __restrict__ A, size_t lda,
+                                                       int k0, int nb,
+                                                       const int* __restrict__ lperm,
+                                                       cf* __restrict__ Li, cf* __restrict__ Ui) {
+    constexpr int XS = NBM + 1;
+    __shared__ cf M[NBM * NBM];  // M[c*NBM + r] = block[r][c]
+    __shared__ cf X[NBM * XS];
+    __shared__ cf dinv[NBM];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    {
+        cf mv[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int t = tid + 256 * q, c = t >> 6, r = t & 63;
+            mv[q] = (c < nb && r < nb) ? A[(size_t)(k0 + c) * lda + k0 + r]
+                                       : mkf(r == c ? 1.f : 0.f, 0.f);
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) M[tid + 256 * q] = mv[q];
+    }
+    if (blockIdx.x == 0) {
+        // X[r][col] = (P I)[r][col]: row r of P I is row lperm[r] of I
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int r = w + 4 * q;
+            const int src = r < nb ? lperm[r] : r;
+            X[r * XS + lane] = mkf(src == lane ? 1.f : 0.f, 0.f);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < NBM; g += 16) {
+            if (w == 0) {
+                cf a[16];
+#pragma unroll
+                for (int r = 0; r < 16; ++r) a[r] = X[(g + r) * XS + lane];
+#pragma unroll
+                for (int i = 0; i < 15; ++i) {
+#pragma unroll
+                    for (int r = i + 1; r < 16; ++r) a[r] = a[r] - M[(g + i) * NBM + g + r] * a[i];
+                }
+#pragma unroll
+                for (int r = 0; r < 16; ++r) X[(g + r) * XS + lane] = a[r];
+            }
+            __syncthreads();
+            const int g2 = g + 16 * (1 + w);
+            if (g2 < NBM) {
+                cf a[16];
+#pragma unroll
+                for (int r = 0; r < 16; ++r) a[r] = X[(g2 + r) * XS + lane];
+#pragma unroll 4
+                for (int i = 0; i < 16; ++i) {
+                    const cf xi = X[(g + i) * XS + lane];
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) a[r] = a[r] - M[(g + i) * NBM + g2 + r] * xi;
+                }
+#pragma unroll
+                for (int r = 0; r < 16; ++r) X[(g2 + r) * XS + lane] = a[r];
+            }
+            __syncthreads();
+        }
+        // Li column col = lane's column: Li[col*64 + r] = X[r][col]
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int col = w + 4 * q;
+            Li[col * NBM + lane] = X[lane * XS + col];
+        }
+    } else {
+        // rows of I, solved x U = e_r left to right (lane = row r)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int c = w + 4 * q;
+            X[c * XS + lane] = mkf(c == lane ? 1.f : 0.f, 0.f);
+        }
+        __syncthreads();
+        if (tid < NBM) {
+            const cf u = M[tid * NBM + tid];
+            dinv[tid] = (u.x != 0.f || u.y != 0.f) ? cinvf(u) : mkf(0.f, 0.f);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < NBM; g += 16) {
+            if (w == 0) {
+                cf a[16];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) a[c] = X[(g + c) * XS + lane];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    a[c] = a[c] * dinv[g + c];
+#pragma unroll
+                    for (int c2 = c + 1; c2 < 16; ++c2) a[c2] = a[c2] - a[c] * M[(g + c2) * NBM + g + c];
+                }
+#pragma unroll
+                for (int c = 0; c < 16; ++c) X[(g + c) * XS + lane] = a[c];
+            }
+            __syncthreads();
+            const int g2 = g + 16 * (1 + w);
+            if (g2 < NBM) {
+                cf a[16];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) a[c] = X[(g2 + c) * XS + lane];
+#pragma unroll 4
+                for (int i = 0; i < 16; ++i) {
+                    const cf xi = X[(g + i) * XS + lane];
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) a[c] = a[c] - xi * M[(g2 + c) * NBM + g + i];
+                }
+#pragma unroll
+                for (int c = 0; c < 16; ++c) X[(g2 + c) * XS + lane] = a[c];
+            }
+            __syncthreads();
+        }
+        // Ui[c*64 + r] = X[c][r]: row r = lane
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int c = w + 4 * q;
+            Ui[c * NBM + lane] = X[c * XS + lane];
+        }
+    }
+}
+
 // Wave-uniform copy of lane src's complex double (v_readlane, no LDS).
 __device__ inline cd readlane_c(cd v, int src) {
     long long bx = __double_as_longlong(v.x), by = __double_as_longlong(v.y);
@@ -263,29 +402,49 @@ __device__ inline void wave_wait_ge(int* flag, int target, int* status) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
+// Block diagonal step of the two block solves: out = Minv acc with the 64 x
+// 64 inverse of the diagonal block (column-major, the lane's 16 columns of
+// wave q preloaded in mv), acc in LDS; the four waves' partial products are
+// joined through LDS and wave 0 stores out and publishes the block flag.
+__device__ inline void diag_apply_publish(const cf mv[16], const cd* acc, cd (*part)[NBM],
+                                          int q, int lane, int nb, cd* out, int* flag,
+                                          int epoch) {
+    cd s = mk(0, 0);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s = s + up(mv[i]) * acc[16 * q + i];
+    part[q][lane] = s;
+    __syncthreads();
+    if (q == 0) {
+        const cd v = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+        if (lane < nb) out[lane] = v;
+        __threadfence();
+        if (lane == 0) publish(flag, epoch);
+    }
+}
+
 // y = L^-1 P b, block forward substitution.  Workgroup w owns blocks
 // w, w+P, ...  For its block, wave q accumulates columns [16q, 16q+16) of
 // every finished block k < blk (waiting on flags[k] == epoch itself, so the
-// four waves never stall each other); one barrier joins the four partial
-// sums, wave 0 applies the block permutation, solves with the unit lower
-// L_bb staged in LDS before any wait, and publishes y_blk.  Blocks only wait
-// on lower-numbered blocks and every workgroup is resident (cooperative
-// launch), so the chain always drains.
+// four waves never stall each other; the factor tile is loaded before the
+// wait); the partial sums are joined through LDS and the block is finished
+// as y_blk = (L_bb^-1 P_bb) acc with the precomputed inverse.  Blocks only
+// wait on lower-numbered blocks and every workgroup is resident
+// (cooperative launch), so the chain always drains.
 __global__ __launch_bounds__(256) void k_trsv_lower_coop(const cf* __restrict__ A, size_t lda,
                                                          int n, int nblk,
-                                                         const int* __restrict__ lperm,
+                                                         const cf* __restrict__ Linv,
                                                          const cd* __restrict__ b,
                                                          cd* __restrict__ y, int* __restrict__ flags,
                                                          int epoch, int* __restrict__ status) {
-    __shared__ cf Ld[NBM * NBM];  // Ld[i*NBM + r] = L_bb[r][i]
     __shared__ cd part[4][NBM];
+    __shared__ cd acc[NBM];
     const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
     for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
         const int r0 = blk * NBM, nb = min(NBM, n - r0);
-        for (int t = tid; t < nb * NBM; t += 256) {
-            int i = t >> 6, r = t & 63;
-            Ld[t] = r < nb ? A[(size_t)(r0 + i) * lda + r0 + r] : mkf(0.f, 0.f);
-        }
+        cf mv[16];  // this wave's 16 columns of the diagonal-block inverse
+        const cf* Mb = Linv + (size_t)blk * NBM * NBM + 16 * q * NBM + lane;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) mv[i] = Mb[i * NBM];
         cd s = mk(0, 0);
         const int row = r0 + min(lane, nb - 1);
         for (int k = 0; k < blk; ++k) {
@@ -301,45 +460,34 @@ __global__ __launch_bounds__(256) void k_trsv_lower_coop(const cf* __restrict__ 
         }
         part[q][lane] = s;
         __syncthreads();
-        if (q == 0) {
-            cd v = mk(0, 0);
-            if (lane < nb) {
-                const int src = lperm[(size_t)blk * NBM + lane];
-                v = b[r0 + src] - ((part[0][src] + part[1][src]) + (part[2][src] + part[3][src]));
-            }
-#pragma unroll 8
-            for (int i = 0; i < nb - 1; ++i) {
-                cd vi = readlane_c(v, i);
-                if (lane > i) v = v - up(Ld[i * NBM + lane]) * vi;
-            }
-            if (lane < nb) y[r0 + lane] = v;
-            __threadfence();
-            if (lane == 0) publish(&flags[blk], epoch);
-        }
+        if (q == 0)
+            acc[lane] = lane < nb ? b[r0 + lane] - ((part[0][lane] + part[1][lane]) +
+                                                    (part[2][lane] + part[3][lane]))
+                                  : mk(0, 0);
+        __syncthreads();
+        diag_apply_publish(mv, acc, part, q, lane, nb, y + r0, &flags[blk], epoch);
         __syncthreads();
     }
 }
 
 // x = U^-1 y, block backward substitution (blocks in reverse order), same
-// scheme as k_trsv_lower_coop; U_bb and the reciprocals of its diagonal are
-// staged in LDS before any wait.
+// scheme as k_trsv_lower_coop with x_blk = U_bb^-1 acc.
 __global__ __launch_bounds__(256) void k_trsv_upper_coop(const cf* __restrict__ A, size_t lda,
                                                          int n, int nblk,
+                                                         const cf* __restrict__ Uinv,
                                                          const cd* __restrict__ yv,
                                                          cd* __restrict__ x, int* __restrict__ flags,
                                                          int epoch, int* __restrict__ status) {
-    __shared__ cf Ud[NBM * NBM];  // Ud[i*NBM + r] = U_bb[r][i]
-    __shared__ cd dinv[NBM];
     __shared__ cd part[4][NBM];
+    __shared__ cd acc[NBM];
     const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
     for (int bi = blockIdx.x; bi < nblk; bi += gridDim.x) {
         const int blk = nblk - 1 - bi;
         const int r0 = blk * NBM, nb = min(NBM, n - r0);
-        for (int t = tid; t < nb * NBM; t += 256) {
-            int i = t >> 6, r = t & 63;
-            Ud[t] = r < nb ? A[(size_t)(r0 + i) * lda + r0 + r] : mkf(0.f, 0.f);
-        }
-        if (tid < nb) dinv[tid] = cinv(up(A[(size_t)(r0 + tid) * lda + r0 + tid]));
+        cf mv[16];
+        const cf* Mb = Uinv + (size_t)blk * NBM * NBM + 16 * q * NBM + lane;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) mv[i] = Mb[i * NBM];
         cd s = mk(0, 0);
         const int row = r0 + min(lane, nb - 1);
         for (int k = nblk - 1; k > blk; --k) {
@@ -364,20 +512,12 @@ __global__ __launch_bounds__(256) void k_trsv_upper_coop(const cf* __restrict__ 
         }
         part[q][lane] = s;
         __syncthreads();
-        if (q == 0) {
-            cd v = lane < nb ? yv[r0 + lane] - ((part[0][lane] + part[1][lane]) +
-                                                (part[2][lane] + part[3][lane]))
-                             : mk(0, 0);
-#pragma unroll 8
-            for (int i = nb - 1; i >= 0; --i) {
-                if (lane == i) v = v * dinv[i];
-                cd vi = readlane_c(v, i);
-                if (lane < i) v = v - up(Ud[i * NBM + lane]) * vi;
-            }
-            if (lane < nb) x[r0 + lane] = v;
-            __threadfence();
-            if (lane == 0) publish(&flags[blk], epoch);
-        }
+        if (q == 0)
+            acc[lane] = lane < nb ? yv[r0 + lane] - ((part[0][lane] + part[1][lane]) +
+                                                     (part[2][lane] + part[3][lane]))
+                                  : mk(0, 0);
+        __syncthreads();
+        diag_apply_publish(mv, acc, part, q, lane, nb, x + r0, &flags[blk], epoch);
         __syncthreads();
     }
 }
@@ -626,7 +766,7 @@ __global__ void k_fill_i(int* p, int v, int cnt) {
     } while (0)
 
 // Workspace slots 9.. of LuWork (0..8 belong to the FP64 path).
-enum { M_A32 = 9, M_PERM, M_FLAGS, M_STATUS, M_VEC, M_V, M_PART, M_HOST, M_PTMP, M_PLANES };
+enum { M_A32 = 9, M_PERM, M_FLAGS, M_STATUS, M_VEC, M_V, M_PART, M_HOST, M_PTMP, M_PLANES, M_INV };
 
 struct MixedFactor {
     cf* A = nullptr;
@@ -634,6 +774,7 @@ struct MixedFactor {
     int* flags = nullptr;
     int* status = nullptr;
     int* epoch = nullptr;   // host-side counter living in LuWork
+    cf* inv = nullptr;      // per block: L_bb^-1 P_bb, then (after nblk blocks) U_bb^-1
     int n = 0, nblk = 0, coop = 1;
     size_t ld = 0;  // leading dimension of A
 };
@@ -666,7 +807,14 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
         __bf16* ub = la + 4 * pstride;
         hipLaunchKernelGGL(k_diag_getrf32, dim3(1), dim3(256), 0, st, F.A, ld, k0, nb, lp,
                            F.status);
+        cf* li = F.inv + (size_t)k * NBM * NBM;
+        cf* ui = F.inv + (size_t)(nblk + k) * NBM * NBM;
         const int c0 = k0 + nb, rest = n - c0;
+        if (rest <= NBM) {
+            // no bulk update left to hide it behind: inverses on the main stream
+            hipLaunchKernelGGL(k_diag_inverses, dim3(2), dim3(256), 0, st, (const cf*)F.A, ld, k0,
+                               nb, (const int*)lp, li, ui);
+        }
         if (rest <= 0) break;
         hipLaunchKernelGGL(k_trsm_upper_rows, dim3((rest + TRR_ROWS - 1) / TRR_ROWS), dim3(256),
                            0, st, F.A, ld, k0, nb, c0, rest, la, pstride);
@@ -683,6 +831,8 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
         if (nr > 0) {
             const int cr = c0 + nn1;
             chk(hipStreamWaitEvent(sd, w.ev_panel, 0));
+            hipLaunchKernelGGL(k_diag_inverses, dim3(2), dim3(256), 0, sd, (const cf*)F.A, ld, k0,
+                               nb, (const int*)lp, li, ui);
             hipLaunchKernelGGL((k_trsm_lower_cols<cf, true>),
                                dim3((nr + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG), dim3(256), 0,
                                sd, F.A, ld, k0, nb, cr, nr, (const int*)lp, ub, pstride);
@@ -708,6 +858,8 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     F.A = (cf*)w.get(M_A32, sizeof(cf) * F.ld * n, &e);
     MX_CHK(e);
     F.lperm = (int*)w.get(M_PERM, sizeof(int) * (size_t)nblk * NBM, &e);
+    MX_CHK(e);
+    F.inv = (cf*)w.get(M_INV, sizeof(cf) * (size_t)nblk * NBM * NBM * 2, &e);
     MX_CHK(e);
     const size_t cap_before = w.cap[M_FLAGS];
     F.flags = (int*)w.get(M_FLAGS, sizeof(int) * (size_t)nblk, &e);
@@ -774,17 +926,18 @@ static hipError_t precond(hipStream_t st, const MixedFactor& F, const cd* v, cd*
     int n = F.n, nblk = F.nblk;
     size_t lda = F.ld;
     const cf* A = F.A;
-    const int* lp = F.lperm;
+    const cf* li = F.inv;
+    const cf* ui = F.inv + (size_t)nblk * NBM * NBM;
     int* flags = F.flags;
     int* status = F.status;
     int e1 = ++*F.epoch;
-    void* a1[] = {(void*)&A, &lda, &n, &nblk, (void*)&lp, (void*)&v, &tmp, &flags, &e1, &status};
+    void* a1[] = {(void*)&A, &lda, &n, &nblk, (void*)&li, (void*)&v, &tmp, &flags, &e1, &status};
     hipError_t e = hipLaunchCooperativeKernel((const void*)k_trsv_lower_coop, dim3(F.coop),
                                               dim3(256), a1, 0, st);
     if (e != hipSuccess) return e;
     int e2 = ++*F.epoch;
     const cd* tv = tmp;
-    void* a2[] = {(void*)&A, &lda, &n, &nblk, (void*)&tv, &z, &flags, &e2, &status};
+    void* a2[] = {(void*)&A, &lda, &n, &nblk, (void*)&ui, (void*)&tv, &z, &flags, &e2, &status};
     return hipLaunchCooperativeKernel((const void*)k_trsv_upper_coop, dim3(F.coop), dim3(256), a2,
                                       0, st);
 }
