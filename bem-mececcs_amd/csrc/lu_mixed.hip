@@ -563,66 +563,91 @@ __global__ __launch_bounds__(256) void k_cgemm_planes(int m, int nc, const __bf1
     const int bm = (b % gx) * CT, bn = (b / gx) * CT;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     const int li = lane & 15, lk = (lane >> 4) * 8;
+    // operand fragments of both k-steps first, so that they and the C tile
+    // below are in flight together: pa/pb[k-step][block][plane]
+    bf8 pa[2][2][4], pb[2][2][4];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+        const __bf16* p = la + (size_t)(ar0 + min(bm + wm + 16 * x + li, m - 1)) * CT + lk;
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+            for (int pl = 0; pl < 4; ++pl) pa[ks][x][pl] = *(const bf8*)(p + 32 * ks + pl * pstride);
+    }
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+        const __bf16* p = ub + (size_t)(bc0 + min(bn + wn + 16 * y + li, nc - 1)) * CT + lk;
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+            for (int pl = 0; pl < 4; ++pl) pb[ks][y][pl] = *(const bf8*)(p + 32 * ks + pl * pstride);
+    }
     f4 accR[2][2], accI[2][2];
     // C tile into the accumulators: a lane's 4 D registers are 4 consecutive
-    // rows of one column, 32 contiguous bytes (two 16-byte loads when aligned)
+    // rows of one column, 32 contiguous bytes.  Interior tiles of an aligned
+    // C take a branch-free path (all eight 16-byte loads in flight at once);
+    // edge tiles go element by element.
+    const bool fast = bm + CT <= m && bn + CT <= nc && (ldc & 3) == 0 &&
+                      ((uintptr_t)C & 31) == 0;
+    if (fast) {
+        float4 v0[2][2], v1[2][2];
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+        for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y) {
-            const int col = bn + wn + 16 * y + li;
-            const int row0 = bm + wm + 16 * x + (lane >> 4) * 4;
-            const cf* cp = C + (size_t)col * ldc + row0;
-            if (col < nc && row0 + 3 < m && ((uintptr_t)cp & 15) == 0) {
-                const float4 v0 = *(const float4*)cp, v1 = *(const float4*)(cp + 2);
-                accR[x][y] = f4{v0.x, v0.z, v1.x, v1.z};
-                accI[x][y] = f4{v0.y, v0.w, v1.y, v1.w};
-            } else {
+            for (int y = 0; y < 2; ++y) {
+                const cf* cp = C + (size_t)(bn + wn + 16 * y + li) * ldc + bm + wm + 16 * x +
+                               (lane >> 4) * 4;
+                v0[x][y] = *(const float4*)cp;
+                v1[x][y] = *(const float4*)(cp + 2);
+            }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                accR[x][y] = f4{v0[x][y].x, v0[x][y].z, v1[x][y].x, v1[x][y].z};
+                accI[x][y] = f4{v0[x][y].y, v0[x][y].w, v1[x][y].y, v1[x][y].w};
+            }
+    } else {
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                const int col = bn + wn + 16 * y + li;
+                const int row0 = bm + wm + 16 * x + (lane >> 4) * 4;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    cf v = (row0 + r < m && col < nc) ? cp[r] : cf{0.f, 0.f};
+                    cf v = (row0 + r < m && col < nc) ? C[(size_t)col * ldc + row0 + r]
+                                                      : cf{0.f, 0.f};
                     accR[x][y][r] = v.x;
                     accI[x][y][r] = v.y;
                 }
             }
-        }
-    size_t aoff[2], boff[2];
+    }
 #pragma unroll
-    for (int x = 0; x < 2; ++x) aoff[x] = (size_t)(ar0 + min(bm + wm + 16 * x + li, m - 1)) * CT + lk;
-#pragma unroll
-    for (int y = 0; y < 2; ++y) boff[y] = (size_t)(bc0 + min(bn + wn + 16 * y + li, nc - 1)) * CT + lk;
-#pragma unroll
-    for (int k = 0; k < CT; k += 32) {
-        bf8 narh[2], narl[2], aih[2], naih[2], nail[2], ail[2];
+    for (int ks = 0; ks < 2; ++ks) {
+        bf8 narh[2], narl[2], naih[2], nail[2];
 #pragma unroll
         for (int x = 0; x < 2; ++x) {
-            const __bf16* p = la + aoff[x] + k;
-            bf8 arh = *(const bf8*)p;
-            bf8 arl = *(const bf8*)(p + pstride);
-            aih[x] = *(const bf8*)(p + 2 * pstride);
-            ail[x] = *(const bf8*)(p + 3 * pstride);
-            narh[x] = neg_bf8(arh);
-            narl[x] = neg_bf8(arl);
-            naih[x] = neg_bf8(aih[x]);
-            nail[x] = neg_bf8(ail[x]);
+            narh[x] = neg_bf8(pa[ks][x][0]);
+            narl[x] = neg_bf8(pa[ks][x][1]);
+            naih[x] = neg_bf8(pa[ks][x][2]);
+            nail[x] = neg_bf8(pa[ks][x][3]);
         }
 #pragma unroll
         for (int y = 0; y < 2; ++y) {
-            const __bf16* p = ub + boff[y] + k;
-            const bf8 brh = *(const bf8*)p;
-            const bf8 brl = *(const bf8*)(p + pstride);
-            const bf8 bih = *(const bf8*)(p + 2 * pstride);
-            const bf8 bil = *(const bf8*)(p + 3 * pstride);
+            const bf8 brh = pb[ks][y][0], brl = pb[ks][y][1], bih = pb[ks][y][2],
+                      bil = pb[ks][y][3];
 #pragma unroll
             for (int x = 0; x < 2; ++x) {
+                const bf8 aih = pa[ks][x][2], ail = pa[ks][x][3];
                 f4 r = accR[x][y], i = accI[x][y];
                 // Cr -= Ar Br - Ai Bi
                 r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narh[x], brh, r, 0, 0, 0);
                 r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narh[x], brl, r, 0, 0, 0);
                 r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narl[x], brh, r, 0, 0, 0);
-                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih[x], bih, r, 0, 0, 0);
-                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih[x], bil, r, 0, 0, 0);
-                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail[x], bih, r, 0, 0, 0);
+                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, bih, r, 0, 0, 0);
+                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, bil, r, 0, 0, 0);
+                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail, bih, r, 0, 0, 0);
                 // Ci -= Ar Bi + Ai Br
                 i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narh[x], bih, i, 0, 0, 0);
                 i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narh[x], bil, i, 0, 0, 0);
@@ -635,23 +660,31 @@ __global__ __launch_bounds__(256) void k_cgemm_planes(int m, int nc, const __bf1
             }
         }
     }
+    if (fast) {
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+        for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y) {
-            const int col = bn + wn + 16 * y + li;
-            const int row0 = bm + wm + 16 * x + (lane >> 4) * 4;
-            cf* cp = C + (size_t)col * ldc + row0;
-            const f4 r = accR[x][y], i = accI[x][y];
-            if (col < nc && row0 + 3 < m && ((uintptr_t)cp & 15) == 0) {
+            for (int y = 0; y < 2; ++y) {
+                cf* cp = C + (size_t)(bn + wn + 16 * y + li) * ldc + bm + wm + 16 * x +
+                         (lane >> 4) * 4;
+                const f4 r = accR[x][y], i = accI[x][y];
                 *(float4*)cp = float4{r[0], i[0], r[1], i[1]};
                 *(float4*)(cp + 2) = float4{r[2], i[2], r[3], i[3]};
-            } else if (col < nc) {
+            }
+    } else {
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                const int col = bn + wn + 16 * y + li;
+                const int row0 = bm + wm + 16 * x + (lane >> 4) * 4;
+                if (col >= nc) continue;
+                const f4 r = accR[x][y], i = accI[x][y];
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    if (row0 + q < m) cp[q] = cf{r[q], i[q]};
+                    if (row0 + q < m) C[(size_t)col * ldc + row0 + q] = cf{r[q], i[q]};
             }
-        }
+    }
 }
 
 // Number of workgroups of k_cgemm_planes for an m x nc update.

@@ -48,6 +48,11 @@ METRIC = ("BEM solves/s + p50/p99 latency at ≤1e-10 err vs FP64, mixed vs plai
 # implementation detail and not counted.
 FLOPS_PER_TRIPLE = 242
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+# HBM bytes per launch of k_dyn_offdiag_h on the default workload, from the
+# rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a plain streaming copy of
+# known size) recorded in profiles/pmc_traffic_c2.md: 44.3 MB read + 671.7 MB
+# written (the ZH store itself is 671.8 MB).
+OFFDIAG_TRAFFIC_BYTES_C2 = 716.0e6
 HBM_PEAK_GBS = 8000.0
 DTYPE = "f64 assembly + f32/bf16x3 factor + f64 refine"
 
@@ -188,7 +193,8 @@ def workload_solve(args, api, decks):
         "roofline": {"kernel": "k_dyn_offdiag_h (dynamic H assembly)",
                      "pipe": "FP64 vector ALU", "bound": "mfma", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": OFFDIAG_TRAFFIC_BYTES_C2,
+                     "traffic_unit": "bytes per launch (rocprofv3 PMC)",
                      "avg_ms": od, "flops_per_triple": FLOPS_PER_TRIPLE,
                      "triples": triples},
         "cpu_baseline": cpu,
