@@ -61,16 +61,16 @@ constexpr int TRSM_COLS_PER_WG = 64;
 // with it in parallel.  L11 reads are wave-uniform LDS broadcasts, the
 // column reads conflict-free.
 template <class T, bool PERM>
-__global__ __launch_bounds__(256) void k_trsm_lower_cols(T* __restrict__ A, size_t lda, int k0,
-                                                         int nb, int c0, int nc,
-                                                         const int* __restrict__ lperm,
-                                                         __bf16* __restrict__ planes,
-                                                         size_t pstride) {
+__device__ __forceinline__ void trsm_lower_cols_body(int bid, T* __restrict__ A, size_t lda,
+                                                     int k0, int nb, int c0, int nc,
+                                                     const int* __restrict__ lperm,
+                                                     __bf16* __restrict__ planes,
+                                                     size_t pstride) {
     constexpr int XS = TB + 1;  // padded row stride: the transposing staging is not bank-bound
     __shared__ T Ls[TB * TB];   // Ls[i*TB + r] = L11[r][i]
     __shared__ T X[TB * XS];    // X[r*XS + lane] = row r of the lane's column
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int cb = c0 + blockIdx.x * TRSM_COLS_PER_WG;
+    const int cb = c0 + bid * TRSM_COLS_PER_WG;
     const int ncols = min(TRSM_COLS_PER_WG, c0 + nc - cb);
     {
         T lv[16], xv[16];
@@ -133,6 +133,15 @@ __global__ __launch_bounds__(256) void k_trsm_lower_cols(T* __restrict__ A, size
                              lane < nb ? X[lane * XS + j] : zero_of<T>());
         }
     }
+}
+
+template <class T, bool PERM>
+__global__ __launch_bounds__(256) void k_trsm_lower_cols(T* __restrict__ A, size_t lda, int k0,
+                                                         int nb, int c0, int nc,
+                                                         const int* __restrict__ lperm,
+                                                         __bf16* __restrict__ planes,
+                                                         size_t pstride) {
+    trsm_lower_cols_body<T, PERM>(blockIdx.x, A, lda, k0, nb, c0, nc, lperm, planes, pstride);
 }
 
 }  // namespace bemx

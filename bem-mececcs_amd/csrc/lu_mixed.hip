@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -163,15 +164,15 @@ __global__ __launch_bounds__(256) void k_diag_getrf32(cf* __restrict__ A, size_t
 // parallel.  U11 reads are wave-uniform broadcasts.  The result goes back to
 // A and, split, to the planes la.
 constexpr int TRR_ROWS = 64;
-__global__ __launch_bounds__(256) void k_trsm_upper_rows(cf* __restrict__ A, size_t lda, int k0,
-                                                         int nb, int r0, int nr,
-                                                         __bf16* __restrict__ la, size_t pstride) {
+__device__ __forceinline__ void trsm_upper_rows_body(int bid, cf* __restrict__ A, size_t lda,
+                                                     int k0, int nb, int r0, int nr,
+                                                     __bf16* __restrict__ la, size_t pstride) {
     constexpr int XS = NBM + 1;
     __shared__ cf Us[NBM * NBM];  // Us[c2*NBM + c] = U11[c][c2]
     __shared__ cf X[NBM * XS];    // X[c*XS + lane] = column c of the lane's row
     __shared__ cf dinv[NBM];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int rb = r0 + blockIdx.x * TRR_ROWS;
+    const int rb = r0 + bid * TRR_ROWS;
     const bool ok = rb + lane < r0 + nr;
     {
         cf uv[16], xv[16];
@@ -239,6 +240,12 @@ __global__ __launch_bounds__(256) void k_trsm_upper_rows(cf* __restrict__ A, siz
             store_planes(la, pstride, (size_t)(rb + rr) * NBM + k,
                          k < nb ? X[k * XS + rr] : mkf(0.f, 0.f));
     }
+}
+
+__global__ __launch_bounds__(256) void k_trsm_upper_rows(cf* __restrict__ A, size_t lda, int k0,
+                                                         int nb, int r0, int nr,
+                                                         __bf16* __restrict__ la, size_t pstride) {
+    trsm_upper_rows_body(blockIdx.x, A, lda, k0, nb, r0, nr, la, pstride);
 }
 
 // Bounded wait until *flag >= target (see wait_count in lu.hip); relaxed
@@ -548,18 +555,12 @@ __device__ inline bf8 neg_bf8(bf8 x) {
     return __builtin_bit_cast(bf8, u);
 }
 
-__global__ __launch_bounds__(256) void k_cgemm_planes(int m, int nc, const __bf16* __restrict__ la,
-                                                      int ar0, const __bf16* __restrict__ ub,
-                                                      int bc0, size_t pstride,
-                                                      cf* __restrict__ C, size_t ldc) {
+// One 64 x 64 tile b of the update (tile order: row tiles fastest).
+__device__ __forceinline__ void cgemm_tile(int b, int gx, int m, int nc,
+                                           const __bf16* __restrict__ la, int ar0,
+                                           const __bf16* __restrict__ ub, int bc0,
+                                           size_t pstride, cf* __restrict__ C, size_t ldc) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int gx = (m + CT - 1) / CT, gy = (nc + CT - 1) / CT;
-    const int tiles = gx * gy;
-    // XCD-aware tile order: workgroup id b runs on XCD b % NXCD; give each
-    // XCD a contiguous range of tiles.
-    const int per = (tiles + NXCD - 1) / NXCD;
-    const int b = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
-    if (b >= tiles) return;
     const int bm = (b % gx) * CT, bn = (b / gx) * CT;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     const int li = lane & 15, lk = (lane >> 4) * 8;
@@ -687,10 +688,43 @@ __global__ __launch_bounds__(256) void k_cgemm_planes(int m, int nc, const __bf1
     }
 }
 
-// Number of workgroups of k_cgemm_planes for an m x nc update.
-static int cgemm_groups(int m, int nc) {
+__global__ __launch_bounds__(256) void k_cgemm_planes(int m, int nc, const __bf16* __restrict__ la,
+                                                      int ar0, const __bf16* __restrict__ ub,
+                                                      int bc0, size_t pstride,
+                                                      cf* __restrict__ C, size_t ldc) {
+    const int gx = (m + CT - 1) / CT, gy = (nc + CT - 1) / CT;
+    const int tiles = gx * gy;
+    // XCD-aware tile order: workgroup id runs on XCD id % NXCD; give each XCD
+    // a contiguous range of tiles.  A grid smaller than the tile count loops
+    // (its size a multiple of NXCD keeps every id on its XCD), which leaves
+    // CUs to the kernels of the other stream.
+    const int per = (tiles + NXCD - 1) / NXCD;
+    for (int id = blockIdx.x; id < per * NXCD; id += gridDim.x) {
+        const int b = (id % NXCD) * per + id / NXCD;
+        if (b < tiles) cgemm_tile(b, gx, m, nc, la, ar0, ub, bc0, pstride, C, ldc);
+    }
+}
+
+// Number of workgroups of k_cgemm_planes for an m x nc update, at most
+// `cap` (a multiple of NXCD; 0 = one workgroup per tile).
+static int cgemm_groups(int m, int nc, int cap = 0) {
     const int tiles = ((m + CT - 1) / CT) * ((nc + CT - 1) / CT);
-    return ((tiles + NXCD - 1) / NXCD) * NXCD;
+    const int g = ((tiles + NXCD - 1) / NXCD) * NXCD;
+    return cap > 0 ? std::min(g, cap) : g;
+}
+
+// Workgroups of the bulk update on the side stream (BEMX_BULK_WGS; default
+// 0 = one per tile).  A cap keeps the bulk update from holding every CU, so
+// the panel kernels of the main stream find room sooner; measured on the
+// 6480-DOF solve, caps of 256..512 cost more in the bulk update than they
+// gained on the panel chain.
+static int bulk_wg_cap() {
+    static const int cap = [] {
+        const char* v = getenv("BEMX_BULK_WGS");
+        const int c = v ? atoi(v) : 0;
+        return c > 0 ? ((c + NXCD - 1) / NXCD) * NXCD : 0;
+    }();
+    return cap;
 }
 
 // ---------------------------------------------------------------------------
@@ -869,7 +903,7 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
             hipLaunchKernelGGL((k_trsm_lower_cols<cf, true>),
                                dim3((nr + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG), dim3(256), 0,
                                sd, F.A, ld, k0, nb, cr, nr, (const int*)lp, ub, pstride);
-            hipLaunchKernelGGL(k_cgemm_planes, dim3(cgemm_groups(rest, nr)), dim3(256), 0, sd,
+            hipLaunchKernelGGL(k_cgemm_planes, dim3(cgemm_groups(rest, nr, bulk_wg_cap())), dim3(256), 0, sd,
                                rest, nr, (const __bf16*)la, c0, (const __bf16*)ub, cr, pstride,
                                F.A + cr * ld + c0, ld);
             chk(hipEventRecord(w.ev_bulk, sd));

@@ -75,7 +75,8 @@ def base_record(args, workload, value, ms_per_step, steps):
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": DTYPE,
         "data": "synthetic (generated flat-patch meshes)",
-        "config": dict(workload, parallelism="single GPU, %d request-loop slots" % args.slots),
+        "config": dict(workload, parallelism="single GPU, %s" % (
+            "%d request-loop slots" % args.slots if args.slots else "one context")),
     }
 
 
@@ -95,6 +96,33 @@ def stream_through_server(api, mode, slots, items, label):
     log("%s: %d solves in %.3f s on %d slots (%.2f solves/s), service p50 %.2f ms" %
         (label, len(items), wall, slots, len(items) / wall, percentile(svc, 50)))
     return [o[0] for o in outs], [o[1] for o in outs], wall
+
+
+def one_context_sweep(api, mesh, dyn, freqs, mode, label):
+    """The frequencies of a sweep solved back to back on one context (no
+    request loop); returns (x list, info list, wall seconds)."""
+    ctx = api.Context(0)
+    try:
+        ctx.send_shared_data(mesh)
+        ctx.ghmatece(mesh.ge, mesh.rnu)
+        ctx.send_dyn_data(dyn.bc.astype(np.complex128), dyn.kode)
+        zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
+        ctx.solve(zmat, freqs[0], mode)  # warm
+        xs, infos = [], []
+        t0 = time.perf_counter()
+        for f in freqs:
+            t1 = time.perf_counter()
+            x, _ = ctx.solve(zmat, f, mode)
+            st = ctx.last_stats()
+            xs.append(x)
+            infos.append({"service_ms": (time.perf_counter() - t1) * 1e3,
+                          "gmres_iters": st["gmres_iters"], "fell_back": float(st["fell_back"])})
+        wall = time.perf_counter() - t0
+    finally:
+        ctx.close()
+    log("%s: %d solves in %.3f s on one context (%.2f solves/s)" %
+        (label, len(freqs), wall, len(freqs) / wall))
+    return xs, infos, wall
 
 
 # ---------------------------------------------------------------------------
@@ -208,9 +236,15 @@ def workload_sweep(args, api, decks):
     p = (mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
     bc = dyn.bc.astype(np.complex128)
     items = [(mesh, dyn.kode, bc, p, f) for f in freqs]
-    stream_through_server(api, api.MODE_MIXED, args.slots, items[: args.slots], "warm mixed")
-    xs, infos, wall = stream_through_server(api, api.MODE_MIXED, args.slots, items, "mixed sweep")
-    x64, _, wall64 = stream_through_server(api, api.MODE_FP64, args.slots, items, "fp64 sweep")
+    if args.slots == 0:
+        # no request loop: one context in this thread, frequencies back to back
+        xs, infos, wall = one_context_sweep(api, mesh, dyn, freqs, api.MODE_MIXED, "mixed sweep")
+        x64, _, wall64 = one_context_sweep(api, mesh, dyn, freqs, api.MODE_FP64, "fp64 sweep")
+    else:
+        stream_through_server(api, api.MODE_MIXED, args.slots, items[: args.slots], "warm mixed")
+        xs, infos, wall = stream_through_server(api, api.MODE_MIXED, args.slots, items,
+                                                "mixed sweep")
+        x64, _, wall64 = stream_through_server(api, api.MODE_FP64, args.slots, items, "fp64 sweep")
     errs = [rel_err(a, b) for a, b in zip(xs, x64)]
     svc = [i["service_ms"] for i in infos]
     value = len(items) / wall
@@ -336,7 +370,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="solve", choices=["solve", "sweep", "serve", "large"])
-    ap.add_argument("--slots", type=int, default=2, help="request-loop slots")
+    ap.add_argument("--slots", type=int, default=2,
+                    help="request-loop slots (sweep: 0 = one context, no request loop)")
     ap.add_argument("--nfreq", type=int, default=256, help="sweep: number of frequencies")
     ap.add_argument("--requests", type=int, default=2000, help="serve: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: arrivals per second")
