@@ -409,122 +409,224 @@ __device__ inline void wave_wait_ge(int* flag, int target, int* status) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-// Block diagonal step of the two block solves: out = Minv acc with the 64 x
-// 64 inverse of the diagonal block (column-major, the lane's 16 columns of
-// wave q preloaded in mv), acc in LDS; the four waves' partial products are
-// joined through LDS and wave 0 stores out and publishes the block flag.
-__device__ inline void diag_apply_publish(const cf mv[16], const cd* acc, cd (*part)[NBM],
-                                          int q, int lane, int nb, cd* out, int* flag,
-                                          int epoch) {
-    cd s = mk(0, 0);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) s = s + up(mv[i]) * acc[16 * q + i];
-    part[q][lane] = s;
+// ---------------------------------------------------------------------------
+// Preconditioner block solves over 128-row blocks (two factor blocks each),
+// which halves the number of block hand-offs of the dependency chain.
+// k_pair_inverses builds, for pair p = (blocks a = 2p, b = 2p+1), the
+// inverses of the 128 x 128 diagonal blocks of both triangular factors from
+// the 64 x 64 inverses and the coupling blocks:
+//   lower  [Li_a, 0; X, Li_b],  X = -Li_b L_ba Li_a
+//   upper  [Ui_a, Y; 0, Ui_b],  Y = -Ui_a U_ab Ui_b
+// (Li = L^-1 P of a block, Ui = U^-1; L_ba / U_ab as stored in A, rows of
+// block b in their unpermuted order, which is how the block solves use
+// them).  Column-major 128 x 128; a missing or partial block b contributes
+// zero coupling.
+// ---------------------------------------------------------------------------
+constexpr int NB2 = 2 * NBM;
+
+// Grid (2, pairs): x = 0 lower, 1 upper; y = pair.
+__global__ __launch_bounds__(256) void k_pair_inverses(const cf* __restrict__ A, size_t lda, int n,
+                                                       const cf* __restrict__ inv, int nblk,
+                                                       cf* __restrict__ Ml, cf* __restrict__ Mu) {
+    __shared__ cf S0[NBM * NBM], S1[NBM * NBM], T[NBM * NBM];
+    const int tid = threadIdx.x;
+    const int p = blockIdx.y;
+    const bool lower = blockIdx.x == 0;
+    const int a = 2 * p, b = 2 * p + 1;
+    const int a0 = a * NBM, b0 = b * NBM;
+    const int nbb = b < nblk ? min(NBM, n - b0) : 0;  // rows/columns of block b
+    const cf* Ia = inv + (size_t)(lower ? a : nblk + a) * NBM * NBM;
+    const cf* Ib = inv + (size_t)(lower ? b : nblk + b) * NBM * NBM;
+    cf* M = (lower ? Ml : Mu) + (size_t)p * NB2 * NB2;
+    // the two diagonal quadrants, and zero for the one that stays zero
+    for (int t = tid; t < NBM * NBM; t += 256) {
+        const int c = t >> 6, r = t & 63;
+        M[(size_t)c * NB2 + r] = Ia[t];
+        M[(size_t)(NBM + c) * NB2 + NBM + r] =
+            nbb > 0 ? Ib[t] : mkf(r == c ? 1.f : 0.f, 0.f);
+        if (lower) M[(size_t)(NBM + c) * NB2 + r] = mkf(0.f, 0.f);
+        else M[(size_t)c * NB2 + NBM + r] = mkf(0.f, 0.f);
+    }
+    if (nbb == 0) {
+        for (int t = tid; t < NBM * NBM; t += 256) {
+            const int c = t >> 6, r = t & 63;
+            if (lower) M[(size_t)c * NB2 + NBM + r] = mkf(0.f, 0.f);
+            else M[(size_t)(NBM + c) * NB2 + r] = mkf(0.f, 0.f);
+        }
+        return;
+    }
+    // lower: S0 = L_ba (rows b, cols a), S1 = Li_a;  T = S0 S1;  then X = -Li_b T
+    // upper: S0 = Ui_a, S1 = U_ab (rows a, cols b);  T = S0 S1;  then Y = -T Ui_b
+    for (int t = tid; t < NBM * NBM; t += 256) {
+        const int c = t >> 6, r = t & 63;
+        if (lower) {
+            S0[t] = r < nbb ? A[(size_t)(a0 + c) * lda + b0 + r] : mkf(0.f, 0.f);
+            S1[t] = Ia[t];
+        } else {
+            S0[t] = Ia[t];
+            S1[t] = c < nbb ? A[(size_t)(b0 + c) * lda + a0 + r] : mkf(0.f, 0.f);
+        }
+    }
     __syncthreads();
-    if (q == 0) {
-        const cd v = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
-        if (lane < nb) out[lane] = v;
-        __threadfence();
-        if (lane == 0) publish(flag, epoch);
+    // thread: column c = tid >> 2, rows r = (tid & 3) + 4 i
+    const int c = tid >> 2, rq = tid & 3;
+    {
+        cf acc[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[i] = mkf(0.f, 0.f);
+        for (int k = 0; k < NBM; ++k) {
+            const cf bk = S1[c * NBM + k];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[i] = acc[i] + S0[k * NBM + rq + 4 * i] * bk;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) T[c * NBM + rq + 4 * i] = acc[i];
+    }
+    __syncthreads();
+    {
+        // lower: X = -Li_b T (Li_b from global, column-major); upper: Y = -T Ui_b
+        cf acc[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[i] = mkf(0.f, 0.f);
+        for (int k = 0; k < NBM; ++k) {
+            if (lower) {
+                const cf bk = T[c * NBM + k];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) acc[i] = acc[i] + Ib[k * NBM + rq + 4 * i] * bk;
+            } else {
+                const cf bk = Ib[c * NBM + k];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) acc[i] = acc[i] + T[k * NBM + rq + 4 * i] * bk;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int r = rq + 4 * i;
+            const cf v = mkf(-acc[i].x, -acc[i].y);
+            if (lower) M[(size_t)c * NB2 + NBM + r] = v;
+            else M[(size_t)(NBM + c) * NB2 + r] = v;
+        }
     }
 }
 
-// y = L^-1 P b, block forward substitution.  Workgroup w owns blocks
-// w, w+P, ...  For its block, wave q accumulates columns [16q, 16q+16) of
-// every finished block k < blk (waiting on flags[k] == epoch itself, so the
-// four waves never stall each other; the factor tile is loaded before the
-// wait); the partial sums are joined through LDS and the block is finished
-// as y_blk = (L_bb^-1 P_bb) acc with the precomputed inverse.  Blocks only
-// wait on lower-numbered blocks and every workgroup is resident
-// (cooperative launch), so the chain always drains.
-__global__ __launch_bounds__(256) void k_trsv_lower_coop(const cf* __restrict__ A, size_t lda,
-                                                         int n, int nblk,
-                                                         const cf* __restrict__ Linv,
+// Block diagonal step of the 128-row block solves: out = Minv acc, the
+// eight waves each taking 32 columns of one row half; joined through LDS,
+// the two row halves stored by waves 0 and 4, one flag published.
+__device__ inline void diag128_apply_publish(const cf mv[32], const cd* acc, cd (*part)[NBM],
+                                             int w, int lane, int nb, cd* out, int* flag,
+                                             int epoch) {
+    const int rh = w >> 2, q = w & 3;
+    cd s = mk(0, 0);
+#pragma unroll
+    for (int i = 0; i < 32; ++i) s = s + up(mv[i]) * acc[32 * q + i];
+    part[w][lane] = s;
+    __syncthreads();
+    if (q == 0) {
+        const int row = rh * NBM + lane;
+        const cd v = (part[4 * rh][lane] + part[4 * rh + 1][lane]) +
+                     (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]);
+        if (row < nb) out[row] = v;
+        __threadfence();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) publish(flag, epoch);
+}
+
+// y = L^-1 P b over 128-row blocks.  Workgroup g owns blocks g, g+G, ...;
+// wave w = (row half rh, column quarter q) accumulates its 64 rows x 32
+// columns of every finished block as that block's flag comes up (factor
+// tile loaded before the wait), then y_blk = Minv acc.  Blocks only wait on
+// lower-numbered blocks and every workgroup is resident (cooperative
+// launch), so the chain always drains.
+__global__ __launch_bounds__(512) void k_trsv_lower_coop(const cf* __restrict__ A, size_t lda,
+                                                         int n, int nblk2,
+                                                         const cf* __restrict__ Minv,
                                                          const cd* __restrict__ b,
                                                          cd* __restrict__ y, int* __restrict__ flags,
                                                          int epoch, int* __restrict__ status) {
-    __shared__ cd part[4][NBM];
-    __shared__ cd acc[NBM];
-    const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
-    for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
-        const int r0 = blk * NBM, nb = min(NBM, n - r0);
-        cf mv[16];  // this wave's 16 columns of the diagonal-block inverse
-        const cf* Mb = Linv + (size_t)blk * NBM * NBM + 16 * q * NBM + lane;
+    __shared__ cd part[8][NBM];
+    __shared__ cd acc[NB2];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rh = w >> 2, q = w & 3;
+    for (int blk = blockIdx.x; blk < nblk2; blk += gridDim.x) {
+        const int r0 = blk * NB2, nb = min(NB2, n - r0);
+        const int row = rh * NBM + lane;
+        cf mv[32];
+        const cf* Mb = Minv + (size_t)blk * NB2 * NB2 + (size_t)(32 * q) * NB2 + row;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) mv[i] = Mb[i * NBM];
+        for (int i = 0; i < 32; ++i) mv[i] = Mb[(size_t)i * NB2];
         cd s = mk(0, 0);
-        const int row = r0 + min(lane, nb - 1);
+        const int rowc = r0 + min(row, nb - 1);
         for (int k = 0; k < blk; ++k) {
-            const int c0 = k * NBM + 16 * q;
-            const cf* Lc = A + (size_t)c0 * lda + row;
-            cf l[16];  // the factor tile does not depend on the flag: load it first
+            const int c0 = k * NB2 + 32 * q;
+            const cf* Lc = A + (size_t)c0 * lda + rowc;
+            cf l[32];  // the factor tile does not depend on the flag: load it first
 #pragma unroll
-            for (int i = 0; i < 16; ++i) l[i] = Lc[(size_t)i * lda];
+            for (int i = 0; i < 32; ++i) l[i] = Lc[(size_t)i * lda];
             wave_wait_ge(&flags[k], epoch, status);
-            const cd yl = y[c0 + (lane & 15)];
+            const cd yl = y[c0 + (lane & 31)];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) s = s + up(l[i]) * readlane_c(yl, i);
+            for (int i = 0; i < 32; ++i) s = s + up(l[i]) * readlane_c(yl, i);
         }
-        part[q][lane] = s;
+        part[w][lane] = s;
         __syncthreads();
         if (q == 0)
-            acc[lane] = lane < nb ? b[r0 + lane] - ((part[0][lane] + part[1][lane]) +
-                                                    (part[2][lane] + part[3][lane]))
-                                  : mk(0, 0);
+            acc[row] = row < nb ? b[r0 + row] - ((part[4 * rh][lane] + part[4 * rh + 1][lane]) +
+                                                 (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]))
+                                : mk(0, 0);
         __syncthreads();
-        diag_apply_publish(mv, acc, part, q, lane, nb, y + r0, &flags[blk], epoch);
+        diag128_apply_publish(mv, acc, part, w, lane, nb, y + r0, &flags[blk], epoch);
         __syncthreads();
     }
 }
 
-// x = U^-1 y, block backward substitution (blocks in reverse order), same
-// scheme as k_trsv_lower_coop with x_blk = U_bb^-1 acc.
-__global__ __launch_bounds__(256) void k_trsv_upper_coop(const cf* __restrict__ A, size_t lda,
-                                                         int n, int nblk,
-                                                         const cf* __restrict__ Uinv,
+// x = U^-1 y over 128-row blocks (in reverse order), same scheme.
+__global__ __launch_bounds__(512) void k_trsv_upper_coop(const cf* __restrict__ A, size_t lda,
+                                                         int n, int nblk2,
+                                                         const cf* __restrict__ Minv,
                                                          const cd* __restrict__ yv,
                                                          cd* __restrict__ x, int* __restrict__ flags,
                                                          int epoch, int* __restrict__ status) {
-    __shared__ cd part[4][NBM];
-    __shared__ cd acc[NBM];
-    const int tid = threadIdx.x, lane = tid & 63, q = tid >> 6;
-    for (int bi = blockIdx.x; bi < nblk; bi += gridDim.x) {
-        const int blk = nblk - 1 - bi;
-        const int r0 = blk * NBM, nb = min(NBM, n - r0);
-        cf mv[16];
-        const cf* Mb = Uinv + (size_t)blk * NBM * NBM + 16 * q * NBM + lane;
+    __shared__ cd part[8][NBM];
+    __shared__ cd acc[NB2];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rh = w >> 2, q = w & 3;
+    for (int bi = blockIdx.x; bi < nblk2; bi += gridDim.x) {
+        const int blk = nblk2 - 1 - bi;
+        const int r0 = blk * NB2, nb = min(NB2, n - r0);
+        const int row = rh * NBM + lane;
+        cf mv[32];
+        const cf* Mb = Minv + (size_t)blk * NB2 * NB2 + (size_t)(32 * q) * NB2 + row;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) mv[i] = Mb[i * NBM];
+        for (int i = 0; i < 32; ++i) mv[i] = Mb[(size_t)i * NB2];
         cd s = mk(0, 0);
-        const int row = r0 + min(lane, nb - 1);
-        for (int k = nblk - 1; k > blk; --k) {
-            const int kb = min(NBM, n - k * NBM);
-            const int c0 = k * NBM + 16 * q;
-            const int cnt = min(16, kb - 16 * q);  // columns of this wave in block k
-            const cf* Uc = A + (size_t)c0 * lda + row;
-            if (cnt == 16) {
-                cf u[16];  // the factor tile does not depend on the flag: load it first
+        const int rowc = r0 + min(row, nb - 1);
+        for (int k = nblk2 - 1; k > blk; --k) {
+            const int kb = min(NB2, n - k * NB2);
+            const int c0 = k * NB2 + 32 * q;
+            const int cnt = min(32, kb - 32 * q);  // columns of this wave in block k
+            const cf* Uc = A + (size_t)c0 * lda + rowc;
+            if (cnt == 32) {
+                cf u[32];  // the factor tile does not depend on the flag: load it first
 #pragma unroll
-                for (int i = 0; i < 16; ++i) u[i] = Uc[(size_t)i * lda];
+                for (int i = 0; i < 32; ++i) u[i] = Uc[(size_t)i * lda];
                 wave_wait_ge(&flags[k], epoch, status);
-                const cd yl = x[c0 + (lane & 15)];
+                const cd yl = x[c0 + (lane & 31)];
 #pragma unroll
-                for (int i = 0; i < 16; ++i) s = s + up(u[i]) * readlane_c(yl, i);
+                for (int i = 0; i < 32; ++i) s = s + up(u[i]) * readlane_c(yl, i);
             } else {
                 wave_wait_ge(&flags[k], epoch, status);
                 if (cnt <= 0) continue;
-                const cd yl = x[c0 + min(lane & 15, cnt - 1)];
+                const cd yl = x[c0 + min(lane & 31, cnt - 1)];
                 for (int i = 0; i < cnt; ++i) s = s + up(Uc[(size_t)i * lda]) * readlane_c(yl, i);
             }
         }
-        part[q][lane] = s;
+        part[w][lane] = s;
         __syncthreads();
         if (q == 0)
-            acc[lane] = lane < nb ? yv[r0 + lane] - ((part[0][lane] + part[1][lane]) +
-                                                     (part[2][lane] + part[3][lane]))
-                                  : mk(0, 0);
+            acc[row] = row < nb ? yv[r0 + row] - ((part[4 * rh][lane] + part[4 * rh + 1][lane]) +
+                                                  (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]))
+                                : mk(0, 0);
         __syncthreads();
-        diag_apply_publish(mv, acc, part, q, lane, nb, x + r0, &flags[blk], epoch);
+        diag128_apply_publish(mv, acc, part, w, lane, nb, x + r0, &flags[blk], epoch);
         __syncthreads();
     }
 }
@@ -833,7 +935,8 @@ __global__ void k_fill_i(int* p, int v, int cnt) {
     } while (0)
 
 // Workspace slots 9.. of LuWork (0..8 belong to the FP64 path).
-enum { M_A32 = 9, M_PERM, M_FLAGS, M_STATUS, M_VEC, M_V, M_PART, M_HOST, M_PTMP, M_PLANES, M_INV };
+enum { M_A32 = 9, M_PERM, M_FLAGS, M_STATUS, M_VEC, M_V, M_PART, M_HOST, M_PTMP, M_PLANES, M_INV,
+       M_MINV };
 
 struct MixedFactor {
     cf* A = nullptr;
@@ -842,7 +945,8 @@ struct MixedFactor {
     int* status = nullptr;
     int* epoch = nullptr;   // host-side counter living in LuWork
     cf* inv = nullptr;      // per block: L_bb^-1 P_bb, then (after nblk blocks) U_bb^-1
-    int n = 0, nblk = 0, coop = 1;
+    cf* minv = nullptr;     // per 128-row pair: lower inverse, then (after npair) upper
+    int n = 0, nblk = 0, npair = 0, coop = 1;
     size_t ld = 0;  // leading dimension of A
 };
 
@@ -911,6 +1015,9 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
         }
     }
     if (bulk_pending) chk(hipStreamWaitEvent(st, w.ev_bulk, 0));
+    // 128-row diagonal-block inverses of both factors for the block solves
+    hipLaunchKernelGGL(k_pair_inverses, dim3(2, F.npair), dim3(256), 0, st, (const cf*)F.A, ld,
+                       n, (const cf*)F.inv, nblk, F.minv, F.minv + (size_t)F.npair * NB2 * NB2);
     chk(hipGetLastError());
     return first;
 }
@@ -928,6 +1035,9 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     MX_CHK(e);
     F.inv = (cf*)w.get(M_INV, sizeof(cf) * (size_t)nblk * NBM * NBM * 2, &e);
     MX_CHK(e);
+    F.npair = (nblk + 1) / 2;
+    F.minv = (cf*)w.get(M_MINV, sizeof(cf) * (size_t)F.npair * NB2 * NB2 * 2, &e);
+    MX_CHK(e);
     const size_t cap_before = w.cap[M_FLAGS];
     F.flags = (int*)w.get(M_FLAGS, sizeof(int) * (size_t)nblk, &e);
     MX_CHK(e);
@@ -942,10 +1052,10 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     int dev = 0, ncu = 0, per = 0;
     MX_CHK(hipGetDevice(&dev));
     MX_CHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trsv_lower_coop, 256, 0));
+    MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trsv_lower_coop, 512, 0));
     int per2 = 0;
-    MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_trsv_upper_coop, 256, 0));
-    F.coop = std::max(1, std::min(nblk, ncu * std::min(per, per2)));
+    MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_trsv_upper_coop, 512, 0));
+    F.coop = std::max(1, std::min(F.npair, ncu * std::min(per, per2)));
     MX_CHK(w.ensure_side());
     // bf16 planes of L21 and U12, double-buffered by step parity: step k+1
     // writes one buffer while the bulk update of step k still reads the other
@@ -990,22 +1100,22 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
 
 // z = M^-1 v = U^-1 L^-1 P v   (tmp: work vector of n; z may alias v)
 static hipError_t precond(hipStream_t st, const MixedFactor& F, const cd* v, cd* z, cd* tmp) {
-    int n = F.n, nblk = F.nblk;
+    int n = F.n, nblk = F.npair;
     size_t lda = F.ld;
     const cf* A = F.A;
-    const cf* li = F.inv;
-    const cf* ui = F.inv + (size_t)nblk * NBM * NBM;
+    const cf* li = F.minv;
+    const cf* ui = F.minv + (size_t)F.npair * NB2 * NB2;
     int* flags = F.flags;
     int* status = F.status;
     int e1 = ++*F.epoch;
     void* a1[] = {(void*)&A, &lda, &n, &nblk, (void*)&li, (void*)&v, &tmp, &flags, &e1, &status};
     hipError_t e = hipLaunchCooperativeKernel((const void*)k_trsv_lower_coop, dim3(F.coop),
-                                              dim3(256), a1, 0, st);
+                                              dim3(512), a1, 0, st);
     if (e != hipSuccess) return e;
     int e2 = ++*F.epoch;
     const cd* tv = tmp;
     void* a2[] = {(void*)&A, &lda, &n, &nblk, (void*)&ui, (void*)&tv, &z, &flags, &e2, &status};
-    return hipLaunchCooperativeKernel((const void*)k_trsv_upper_coop, dim3(F.coop), dim3(256), a2,
+    return hipLaunchCooperativeKernel((const void*)k_trsv_upper_coop, dim3(F.coop), dim3(512), a2,
                                       0, st);
 }
 
