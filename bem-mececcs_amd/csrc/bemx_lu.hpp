@@ -46,4 +46,8 @@ int gmres_ir(LuWork& w, hipStream_t st, int n, const cd* A, size_t lda, cd* b, i
 int lu_solve(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, cd* b, int nrhs, int mode,
              long long* info, double stats[4], std::string& err);
 
+// Device bytes of the mixed factorisation's workspace for order n beyond the
+// FP32 copy of the matrix (bf16 planes, diagonal-block inverses).
+size_t mixed_factor_extra_bytes(int n);
+
 }  // namespace bemx

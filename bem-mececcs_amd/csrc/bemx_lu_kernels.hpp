@@ -13,8 +13,10 @@ __host__ __device__ inline cf operator-(cf a, cf b) { return cf{a.x - b.x, a.y -
 __host__ __device__ inline cf operator*(cf a, cf b) {
     return cf{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
-__host__ __device__ inline cf cinvf(cf a) {
-    float d = 1.0f / (a.x * a.x + a.y * a.y);
+// 1 / a for the FP32 factors, which only precondition: v_rcp_f32 (1 ulp)
+// of |a|^2 instead of a full-precision division.
+__device__ inline cf cinvf(cf a) {
+    const float d = __builtin_amdgcn_rcpf(a.x * a.x + a.y * a.y);
     return cf{a.x * d, -a.y * d};
 }
 __device__ inline cd up(cf a) { return cd{(double)a.x, (double)a.y}; }
@@ -60,15 +62,16 @@ constexpr int TRSM_COLS_PER_WG = 64;
 // solves chunk g in registers, then the four waves update the later chunks
 // with it in parallel.  L11 reads are wave-uniform LDS broadcasts, the
 // column reads conflict-free.
+// Ls (TB x TB) and X (TB x (TB + 1)) are LDS buffers of the calling kernel.
 template <class T, bool PERM>
 __device__ __forceinline__ void trsm_lower_cols_body(int bid, T* __restrict__ A, size_t lda,
                                                      int k0, int nb, int c0, int nc,
                                                      const int* __restrict__ lperm,
                                                      __bf16* __restrict__ planes,
-                                                     size_t pstride) {
+                                                     size_t pstride, T* __restrict__ Ls,
+                                                     T* __restrict__ X) {
     constexpr int XS = TB + 1;  // padded row stride: the transposing staging is not bank-bound
-    __shared__ T Ls[TB * TB];   // Ls[i*TB + r] = L11[r][i]
-    __shared__ T X[TB * XS];    // X[r*XS + lane] = row r of the lane's column
+    // Ls[i*TB + r] = L11[r][i];  X[r*XS + lane] = row r of the lane's column
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int cb = c0 + bid * TRSM_COLS_PER_WG;
     const int ncols = min(TRSM_COLS_PER_WG, c0 + nc - cb);
@@ -141,7 +144,10 @@ __global__ __launch_bounds__(256) void k_trsm_lower_cols(T* __restrict__ A, size
                                                          const int* __restrict__ lperm,
                                                          __bf16* __restrict__ planes,
                                                          size_t pstride) {
-    trsm_lower_cols_body<T, PERM>(blockIdx.x, A, lda, k0, nb, c0, nc, lperm, planes, pstride);
+    __shared__ T Ls[TB * TB];
+    __shared__ T X[TB * (TB + 1)];
+    trsm_lower_cols_body<T, PERM>(blockIdx.x, A, lda, k0, nb, c0, nc, lperm, planes, pstride, Ls,
+                                  X);
 }
 
 }  // namespace bemx

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "../../include/bemx.h"
+#include "bemx_lu.hpp"
 
 namespace {
 
@@ -99,7 +100,7 @@ size_t request_footprint(int nbe, int nrhs, int mode) {
     b += nn * nrhs * 16 * 4;                                     // ZDFI, ZFI, diagonal RHS
     if (mode == BEMX_MODE_MIXED) {
         b += ld32 * nn * 8;                                      // FP32 factors
-        b += nn * 64 * 2 * 16;                                   // bf16 panels
+        b += bemx::mixed_factor_extra_bytes((int)nn);                  // bf16 planes, inverses
         b += nn * 16 * (5 + 13) + ((nn + 127) / 128) * nn * 16;  // GMRES vectors, GEMV partials
     } else {
         b += nn * 16 * 2 + nn * 64 * 16;                         // solve vector, panel candidates

@@ -136,6 +136,7 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
     __shared__ double rv[4];
     __shared__ int rr[4];
     __shared__ int s_piv, s_wp, s_cand;
+    __shared__ int s_pivs[NB];  // pivot row of each column
     __shared__ double s_pval, s_cval;
     const int P = gridDim.x, p = blockIdx.x, tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -239,6 +240,7 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
         const int piv = s_piv;
         const double pval = s_pval;
         if (tid < nb) urow[tid] = cand[((size_t)j * P + s_wp) * NB + tid];
+        if (tid == 0) s_pivs[j] = piv;
         if (p == 0 && tid == 0) {
             ipiv[d] = piv;
             if (pval == 0.0 && *info == 0) *info = d + 1;
@@ -284,30 +286,62 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
         int c = t / nr, r2 = t - c * nr;
         A[(size_t)(k0 + c) * lda + r0 + r2] = sA[c * PANEL_ROWS + r2];
     }
-    if (p == 0 && tid == 0) {
-        // content[i] = original row now at position key[i]
-        int key[2 * NB], content[2 * NB], cnt = 0;
-        auto find = [&](int row) {
-            for (int i = 0; i < cnt; ++i)
-                if (key[i] == row) return i;
-            key[cnt] = row;
-            content[cnt] = row;
-            return cnt++;
+    if (p == 0 && wave == 0) {
+        // Replay the swaps on the rows they touch (at most 2 NB): slot i of
+        // the list lives in lane i % 64, register i / 64; key = position,
+        // content = original row now there.  Lookups are wave ballots, so
+        // the whole replay stays in registers.
+        int key[2] = {-1, -1}, content[2] = {-1, -1};
+        int cnt = 0;
+        auto find = [&](int row) -> int {
+            const unsigned long long b0 = __ballot(lane < cnt && key[0] == row);
+            const unsigned long long b1 = __ballot(lane + 64 < cnt && key[1] == row);
+            if (b0) return __ffsll((long long)b0) - 1;
+            if (b1) return 63 + __ffsll((long long)b1);
+            const int i = cnt++;
+            if (lane == (i & 63)) {
+                if (i < 64) {
+                    key[0] = row;
+                    content[0] = row;
+                } else {
+                    key[1] = row;
+                    content[1] = row;
+                }
+            }
+            return i;
+        };
+        auto get = [&](int i) {
+            return i < 64 ? __shfl(content[0], i, 64) : __shfl(content[1], i - 64, 64);
+        };
+        auto set = [&](int i, int v) {
+            if (lane == (i & 63)) {
+                if (i < 64)
+                    content[0] = v;
+                else
+                    content[1] = v;
+            }
         };
         for (int j = 0; j < nb; ++j) {
-            int a = find(k0 + j), b = find(ipiv[k0 + j]);
-            int t = content[a];
-            content[a] = content[b];
-            content[b] = t;
+            const int a = find(k0 + j), b = find(s_pivs[j]);
+            const int ca = get(a), cb = get(b);
+            set(a, cb);
+            set(b, ca);
         }
-        int m = 0;
-        for (int i = 0; i < cnt; ++i)
-            if (content[i] != key[i]) {
-                perm[1 + m] = key[i];
-                perm[1 + 2 * NB + m] = content[i];
-                ++m;
-            }
-        perm[0] = m;
+        const bool ch0 = lane < cnt && content[0] != key[0];
+        const bool ch1 = lane + 64 < cnt && content[1] != key[1];
+        const unsigned long long m0 = __ballot(ch0), m1 = __ballot(ch1);
+        const unsigned long long below = (1ull << lane) - 1;
+        const int n0 = __popcll(m0);
+        const int i0 = __popcll(m0 & below), i1 = n0 + __popcll(m1 & below);
+        if (ch0) {
+            perm[1 + i0] = key[0];
+            perm[1 + 2 * NB + i0] = content[0];
+        }
+        if (ch1) {
+            perm[1 + i1] = key[1];
+            perm[1 + 2 * NB + i1] = content[1];
+        }
+        if (lane == 0) perm[0] = n0 + __popcll(m1);
     }
 }
 

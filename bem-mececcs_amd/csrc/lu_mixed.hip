@@ -3,24 +3,27 @@
 // FP64 GMRES iterative refinement (GMRES-IR) against the FP64 matrix.
 //
 // Factorisation (blocks of NBM = 64 columns, A32 = (complex float) A64):
-//   k_diag_getrf32     partial pivoting inside the 64 x 64 diagonal block
-//                      (registers: lane = row, wave = 16 columns)
-//   k_trsm_lower_cols  U12 = L11^-1 P A12   (one lane per column)
-//   k_trsm_upper_rows  L21 = A21 U11^-1     (one lane per row)
-//   k_cgemm_planes     A22 -= L21 U12; every FP32 operand x is split into
-//                      hi = bf16(x), lo = bf16(x - hi) (by the two triangular
-//                      kernels, into bf16 planes) and each real product is
-//                      hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16 with
-//                      FP32 accumulation; complex = 4 real products.
-// The factorisation looks one block ahead over two streams: the next
-// diagonal block is factored while the bulk update of the current step runs.
+//   k_diag_getrf_panels  partial pivoting inside the 64 x 64 diagonal block
+//                      (registers: lane = row, wave = a panel of 16 columns)
+//   k_trsm_both        L21 = A21 U11^-1 (one lane per row) and
+//                      U12 = L11^-1 P A12 (one lane per column), one launch
+//   k_cgemm_update     A22 -= L21 U12 over K = 64 .. 512; every FP32 operand x
+//                      is split into hi = bf16(x), lo = bf16(x - hi) (by the
+//                      triangular kernels, into bf16 planes) and each real
+//                      product is hi*hi + hi*lo + lo*hi on
+//                      v_mfma_f32_16x16x32_bf16 with FP32 accumulation;
+//                      complex = 4 real products.
+// Blocks are factored in groups (enqueue_factor): inside a group the update
+// is applied to the group's own columns and rows only; the rest of the
+// matrix gets one update per group with K = 64 x group size, on a second
+// stream, while the next group is factored.
 // Preconditioner M^-1 v (FP64 vectors, FP32 factors): k_trsv_lower_coop and
-// k_trsv_upper_coop, one cooperative launch each; workgroups own 64-row
-// blocks, accumulate the contributions of finished blocks as their flags
-// come up and publish their own block, so the dependency chain costs one
-// small GEMV and one product with the block's explicit inverse
-// (k_diag_inverses, computed on the side stream during the factorisation)
-// per block instead of one launch.
+// k_trsv_upper_coop, one cooperative launch each; workgroups own 128-row
+// blocks (pairs of diagonal blocks), accumulate the contributions of
+// finished blocks as their flags come up and publish their own block, so the
+// dependency chain costs one small GEMV and one product with the block's
+// explicit inverse (k_diag_inverses, then k_pair_inverses, after the
+// factorisation) per 128 rows instead of one launch.
 // Pivoting is restricted to the diagonal block: the factors only
 // precondition.  The accuracy of the answer comes from the refinement, whose
 // residual b - A64 x is formed in FP64 with the FP64 matrix; if it stalls the
@@ -73,19 +76,24 @@ __device__ inline cf readlane_f(cf v, int src) {
 }
 
 // Diagonal block: GEPP within its nb rows (largest |re|+|im| to 17 mantissa
-// bits, lowest block row on ties), held in registers: wave q owns columns
-// [16q, 16q+16), lane r
-// owns block row r.  Rows are not moved while factoring: a row is marked
-// used when it becomes the pivot of step j (its rank), and the factors are
-// written back in rank order.  lperm[i] = block row whose content ends at
-// block row i.  One barrier per step: the wave owning column j+1 applies
-// step j to that column first and publishes the pivot and multipliers of
-// step j+1 (double-buffered in LDS) before finishing its step-j update.
-__global__ __launch_bounds__(256) void k_diag_getrf32(cf* __restrict__ A, size_t lda, int k0,
-                                                      int nb, int* __restrict__ lperm,
-                                                      int* __restrict__ status) {
-    __shared__ cf lcol[2][NBM];
-    __shared__ int s_piv[2];
+// bits, lowest block row on ties), held in registers, factored by panels of
+// 16 columns: wave q owns columns [16q, 16q+16), lane r owns block row r.
+// Rows are not moved while factoring: a row is marked used when it becomes
+// the pivot of step j (its rank), and the factors are written back in rank
+// order; lperm[i] = block row whose content ends at block row i.
+// Wave q factors its panel
+// alone (16 pivot steps, no barrier: search, multipliers and the update of
+// its remaining columns all stay in the wave), publishes the multipliers and
+// pivots in LDS, and after one barrier the waves to its right replay those
+// 16 steps on their own columns while wave q+1 factors the next panel.  The
+// chain is 4 panel factorisations and 3 replays with 3 barriers, instead of
+// a barrier per column.
+__global__ __launch_bounds__(256) void k_diag_getrf_panels(cf* __restrict__ A, size_t lda, int k0,
+                                                           int nb, int* __restrict__ lperm,
+                                                           int* __restrict__ status) {
+    __shared__ cf lmul[NBM][NBM];  // lmul[j][r]: multiplier of row r at step j
+    __shared__ int s_piv[NBM];
+    __shared__ int s_rank[NBM];
     const int tid = threadIdx.x, lane = tid & 63;
     const int q = __builtin_amdgcn_readfirstlane(tid >> 6);
     cf v[16];
@@ -95,58 +103,66 @@ __global__ __launch_bounds__(256) void k_diag_getrf32(cf* __restrict__ A, size_t
         v[i] = (lane < nb && c < nb) ? A[(size_t)(k0 + c) * lda + k0 + lane] : mkf(0.f, 0.f);
     }
     bool used = lane >= nb;
-    int rank = lane;  // rows >= nb keep their place
-    // pivot of column (wave-local index) ii; multipliers stored in v[ii]
-    auto search = [&](cf& col, int buf) {
-        // key = |re|+|im| with its 6 lowest mantissa bits replaced by 63 - lane:
-        // the wave maximum picks the largest candidate (to 17 mantissa bits,
-        // ample for pivoting a preconditioner), lowest row on ties
-        const unsigned key =
-            used ? 0u
-                 : ((__float_as_uint(fabsf(col.x) + fabsf(col.y)) & ~63u) | (unsigned)(63 - lane));
-        const int p = 63 - (int)(wave_max_u32(key) & 63u);
-        const cf pv = readlane_f(col, p);
-        const bool zero = pv.x == 0.f && pv.y == 0.f;
-        const cf inv = zero ? mkf(0.f, 0.f) : cinvf(pv);
-        cf l = mkf(0.f, 0.f);
-        if (!used && lane != p) {
-            l = col * inv;
-            col = l;
-        }
-        lcol[buf][lane] = l;
-        if (lane == 0) {
-            s_piv[buf] = p;
-            if (zero) atomicOr(status, ST_PIVOT_ZERO);
-        }
-    };
-    if (q == 0 && nb > 0) search(v[0], 0);
-    __syncthreads();
-    for (int jc = 0; jc < 4; ++jc) {
+    int rank = lane;
+    for (int pq = 0; pq < 4 && 16 * pq < nb; ++pq) {
+        if (q == pq) {
 #pragma unroll
-        for (int jj = 0; jj < 16; ++jj) {
-            const int j = 16 * jc + jj;
-            if (j < nb) {
-                const int p = __builtin_amdgcn_readfirstlane(s_piv[j & 1]);
-                const cf l = lcol[j & 1][lane];
+            for (int i = 0; i < 16; ++i) {
+                const int j = 16 * pq + i;
+                if (j < nb) {
+                    // key = |re|+|im| with its 6 lowest mantissa bits
+                    // replaced by 63 - lane: the wave maximum picks the
+                    // largest candidate, lowest row on ties
+                    const unsigned key =
+                        used ? 0u
+                             : ((__float_as_uint(fabsf(v[i].x) + fabsf(v[i].y)) & ~63u) |
+                                (unsigned)(63 - lane));
+                    const int p = 63 - (int)(wave_max_u32(key) & 63u);
+                    const cf pv = readlane_f(v[i], p);
+                    const bool zero = pv.x == 0.f && pv.y == 0.f;
+                    const cf inv = zero ? mkf(0.f, 0.f) : cinvf(pv);
+                    cf l = mkf(0.f, 0.f);
+                    if (!used && lane != p) {
+                        l = v[i] * inv;
+                        v[i] = l;
+                    }
+                    if (lane == p) {
+                        used = true;
+                        rank = j;
+                    }
+                    lmul[j][lane] = l;
+                    if (lane == 0) {
+                        s_piv[j] = p;
+                        if (zero) atomicOr(status, ST_PIVOT_ZERO);
+                    }
+#pragma unroll
+                    for (int c = i + 1; c < 16; ++c) v[c] = v[c] - l * readlane_f(v[c], p);
+                }
+            }
+        }
+        __syncthreads();
+        if (q > pq) {
+            // replay the 16 steps of panel pq on this wave's columns
+            for (int i = 0; i < 16; ++i) {
+                const int j = 16 * pq + i;
+                if (j >= nb) break;
+                const int p = __builtin_amdgcn_readfirstlane(s_piv[j]);
+                const cf l = lmul[j][lane];
                 if (lane == p) {
                     used = true;
                     rank = j;
                 }
-                const int qn = jj == 15 ? jc + 1 : jc;  // owner of column j+1
-                const int in = (jj + 1) & 15;
-                const bool owner = q == qn && j + 1 < nb;
-                if (owner) {
-                    v[in] = v[in] - l * readlane_f(v[in], p);
-                    search(v[in], (j + 1) & 1);
-                }
 #pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    if (16 * q + i > j && !(owner && i == in)) v[i] = v[i] - l * readlane_f(v[i], p);
-                }
-                __syncthreads();
+                for (int c = 0; c < 16; ++c) v[c] = v[c] - l * readlane_f(v[c], p);
             }
         }
     }
+    // the last wave with columns below nb has replayed every pivot: its
+    // ranks are the final ones
+    const int qlast = (nb - 1) >> 4;
+    if (q == qlast) s_rank[lane] = rank;
+    __syncthreads();
+    rank = s_rank[lane];
     if (lane < nb) {
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
@@ -166,11 +182,12 @@ __global__ __launch_bounds__(256) void k_diag_getrf32(cf* __restrict__ A, size_t
 constexpr int TRR_ROWS = 64;
 __device__ __forceinline__ void trsm_upper_rows_body(int bid, cf* __restrict__ A, size_t lda,
                                                      int k0, int nb, int r0, int nr,
-                                                     __bf16* __restrict__ la, size_t pstride) {
+                                                     __bf16* __restrict__ la, size_t pstride,
+                                                     cf* __restrict__ Us, cf* __restrict__ X,
+                                                     cf* __restrict__ dinv) {
     constexpr int XS = NBM + 1;
-    __shared__ cf Us[NBM * NBM];  // Us[c2*NBM + c] = U11[c][c2]
-    __shared__ cf X[NBM * XS];    // X[c*XS + lane] = column c of the lane's row
-    __shared__ cf dinv[NBM];
+    // LDS buffers of the calling kernel: Us[c2*NBM + c] = U11[c][c2] (NBM x
+    // NBM), X[c*XS + lane] = column c of the lane's row (NBM x XS), dinv[NBM]
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int rb = r0 + bid * TRR_ROWS;
     const bool ok = rb + lane < r0 + nr;
@@ -242,10 +259,25 @@ __device__ __forceinline__ void trsm_upper_rows_body(int bid, cf* __restrict__ A
     }
 }
 
-__global__ __launch_bounds__(256) void k_trsm_upper_rows(cf* __restrict__ A, size_t lda, int k0,
-                                                         int nb, int r0, int nr,
-                                                         __bf16* __restrict__ la, size_t pstride) {
-    trsm_upper_rows_body(blockIdx.x, A, lda, k0, nb, r0, nr, la, pstride);
+// L21 of rows [c0, c0+nr) and U12 of columns [c0, c0+nc) of the step at
+// k0, in one launch: the first ceil(nr/64) workgroups take row blocks, the
+// others column blocks (the two read only the diagonal block in common).
+__global__ __launch_bounds__(256) void k_trsm_both(cf* __restrict__ A, size_t lda, int k0, int nb,
+                                                   int c0, int nr, int nc,
+                                                   const int* __restrict__ lperm,
+                                                   __bf16* __restrict__ la,
+                                                   __bf16* __restrict__ ub, size_t pstride) {
+    // one set of LDS buffers serves whichever of the two solves this
+    // workgroup runs
+    __shared__ cf T1[NBM * NBM];
+    __shared__ cf T2[NBM * (NBM + 1)];
+    __shared__ cf dinv[NBM];
+    const int nrb = (nr + TRR_ROWS - 1) / TRR_ROWS;
+    if ((int)blockIdx.x < nrb)
+        trsm_upper_rows_body(blockIdx.x, A, lda, k0, nb, c0, nr, la, pstride, T1, T2, dinv);
+    else
+        trsm_lower_cols_body<cf, true>(blockIdx.x - nrb, A, lda, k0, nb, c0, nc, lperm, ub,
+                                       pstride, T1, T2);
 }
 
 // Bounded wait until *flag >= target (see wait_count in lu.hip); relaxed
@@ -274,9 +306,14 @@ __device__ inline void publish(int* flag, int value) {
 // chunk in registers, the four waves update the later chunks).  A partial
 // last block is padded with the identity.
 __global__ __launch_bounds__(256) void k_diag_inverses(const cf* __restrict__ A, size_t lda,
-                                                       int k0, int nb,
-                                                       const int* __restrict__ lperm,
-                                                       cf* __restrict__ Li, cf* __restrict__ Ui) {
+                                                       int n, const int* __restrict__ lperm_all,
+                                                       cf* __restrict__ inv) {
+    // block y of the grid: diagonal block y; x = 0: lower factor, x = 1: upper
+    const int blk = blockIdx.y, nblk = gridDim.y;
+    const int k0 = blk * NBM, nb = min(NBM, n - k0);
+    const int* __restrict__ lperm = lperm_all + (size_t)blk * NBM;
+    cf* __restrict__ Li = inv + (size_t)blk * NBM * NBM;
+    cf* __restrict__ Ui = inv + (size_t)(nblk + blk) * NBM * NBM;
     constexpr int XS = NBM + 1;
     __shared__ cf M[NBM * NBM];  // M[c*NBM + r] = block[r][c]
     __shared__ cf X[NBM * XS];
@@ -512,14 +549,34 @@ __global__ __launch_bounds__(256) void k_pair_inverses(const cf* __restrict__ A,
 // Block diagonal step of the 128-row block solves: out = Minv acc, the
 // eight waves each taking 32 columns of one row half; joined through LDS,
 // the two row halves stored by waves 0 and 4, one flag published.
+// sum_i f[i] * y_i over 32 columns (FP32 factor entries, FP64 vector) with
+// four independent accumulators; y_i is read from yb at wave-uniform
+// addresses (LDS broadcast).
+__device__ inline cd dot32(const cf (&f)[32], const cd* yb) {
+    cd s0 = mk(0, 0), s1 = mk(0, 0), s2 = mk(0, 0), s3 = mk(0, 0);
+#pragma unroll
+    for (int i = 0; i < 32; i += 4) {
+        s0 = s0 + up(f[i]) * yb[i];
+        s1 = s1 + up(f[i + 1]) * yb[i + 1];
+        s2 = s2 + up(f[i + 2]) * yb[i + 2];
+        s3 = s3 + up(f[i + 3]) * yb[i + 3];
+    }
+    return (s0 + s1) + (s2 + s3);
+}
+
 __device__ inline void diag128_apply_publish(const cf mv[32], const cd* acc, cd (*part)[NBM],
                                              int w, int lane, int nb, cd* out, int* flag,
                                              int epoch) {
     const int rh = w >> 2, q = w & 3;
-    cd s = mk(0, 0);
+    cd s0 = mk(0, 0), s1 = mk(0, 0), s2 = mk(0, 0), s3 = mk(0, 0);
 #pragma unroll
-    for (int i = 0; i < 32; ++i) s = s + up(mv[i]) * acc[32 * q + i];
-    part[w][lane] = s;
+    for (int i = 0; i < 32; i += 4) {
+        s0 = s0 + up(mv[i]) * acc[32 * q + i];
+        s1 = s1 + up(mv[i + 1]) * acc[32 * q + i + 1];
+        s2 = s2 + up(mv[i + 2]) * acc[32 * q + i + 2];
+        s3 = s3 + up(mv[i + 3]) * acc[32 * q + i + 3];
+    }
+    part[w][lane] = (s0 + s1) + (s2 + s3);
     __syncthreads();
     if (q == 0) {
         const int row = rh * NBM + lane;
@@ -546,6 +603,7 @@ __global__ __launch_bounds__(512) void k_trsv_lower_coop(const cf* __restrict__ 
                                                          int epoch, int* __restrict__ status) {
     __shared__ cd part[8][NBM];
     __shared__ cd acc[NB2];
+    __shared__ cd yb[8][32];  // per wave: the 32 entries of y it multiplies
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rh = w >> 2, q = w & 3;
     for (int blk = blockIdx.x; blk < nblk2; blk += gridDim.x) {
         const int r0 = blk * NB2, nb = min(NB2, n - r0);
@@ -563,9 +621,12 @@ __global__ __launch_bounds__(512) void k_trsv_lower_coop(const cf* __restrict__ 
 #pragma unroll
             for (int i = 0; i < 32; ++i) l[i] = Lc[(size_t)i * lda];
             wave_wait_ge(&flags[k], epoch, status);
-            const cd yl = y[c0 + (lane & 31)];
-#pragma unroll
-            for (int i = 0; i < 32; ++i) s = s + up(l[i]) * readlane_c(yl, i);
+            // lanes 0..31 stage y in the wave's own LDS slot; the wave's LDS
+            // operations complete in order, so its reads below see them
+            if (lane < 32) yb[w][lane] = y[c0 + lane];
+            __builtin_amdgcn_wave_barrier();
+            s = s + dot32(l, yb[w]);
+            __builtin_amdgcn_wave_barrier();
         }
         part[w][lane] = s;
         __syncthreads();
@@ -588,6 +649,7 @@ __global__ __launch_bounds__(512) void k_trsv_upper_coop(const cf* __restrict__ 
                                                          int epoch, int* __restrict__ status) {
     __shared__ cd part[8][NBM];
     __shared__ cd acc[NB2];
+    __shared__ cd yb[8][32];  // per wave: the 32 entries of x it multiplies
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rh = w >> 2, q = w & 3;
     for (int bi = blockIdx.x; bi < nblk2; bi += gridDim.x) {
         const int blk = nblk2 - 1 - bi;
@@ -609,9 +671,10 @@ __global__ __launch_bounds__(512) void k_trsv_upper_coop(const cf* __restrict__ 
 #pragma unroll
                 for (int i = 0; i < 32; ++i) u[i] = Uc[(size_t)i * lda];
                 wave_wait_ge(&flags[k], epoch, status);
-                const cd yl = x[c0 + (lane & 31)];
-#pragma unroll
-                for (int i = 0; i < 32; ++i) s = s + up(u[i]) * readlane_c(yl, i);
+                if (lane < 32) yb[w][lane] = x[c0 + lane];
+                __builtin_amdgcn_wave_barrier();
+                s = s + dot32(u, yb[w]);
+                __builtin_amdgcn_wave_barrier();
             } else {
                 wave_wait_ge(&flags[k], epoch, status);
                 if (cnt <= 0) continue;
@@ -632,18 +695,15 @@ __global__ __launch_bounds__(512) void k_trsv_upper_coop(const cf* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// C[m x nc] -= L21 U12, complex FP32 with K = 64, split-BF16x3 on MFMA.  The
-// operands come pre-split as bf16 planes (written by the two triangular
-// kernels), so a lane loads its 8 consecutive k of one plane with a single
-// 16-byte global load, straight into the MFMA operand layout; no LDS.
-// Workgroup tile 64 x 64, 4 waves of 32 x 32 (2 x 2 MFMA 16x16 blocks).
-// Each real product x*y is xh*yh + xh*yl + xl*yh (FP32 accumulation);
-// complex = 4 real products, 12 MFMA per 16x16x32 complex block.
+// Split-BF16x3 complex products on MFMA.  Each real product x*y is
+// xh*yh + xh*yl + xl*yh (FP32 accumulation); complex = 4 real products,
+// 12 MFMA per 16x16x32 complex block.  The operands come pre-split as bf16
+// planes (written by the triangular kernels), so a lane loads its 8
+// consecutive k of one plane with a single 16-byte global load, straight
+// into the MFMA operand layout.  Workgroup tile 64 x 64, 4 waves of 32 x 32
+// (2 x 2 MFMA 16x16 blocks).
 // v_mfma_f32_16x16x32_bf16 layout: A[i][k]: lane i + 16 (k/8), element k%8;
 // B[k][j]: lane j + 16 (k/8), element k%8; D[i][j]: lane j + 16 (i/4), reg i%4.
-// Tiles are assigned so that consecutive workgroups of one XCD (ids that
-// differ by the XCD count) sweep neighbouring tiles and share operands in
-// that XCD's L2.
 // ---------------------------------------------------------------------------
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -657,40 +717,95 @@ __device__ inline bf8 neg_bf8(bf8 x) {
     return __builtin_bit_cast(bf8, u);
 }
 
-// One 64 x 64 tile b of the update (tile order: row tiles fastest).
-__device__ __forceinline__ void cgemm_tile(int b, int gx, int m, int nc,
-                                           const __bf16* __restrict__ la, int ar0,
-                                           const __bf16* __restrict__ ub, int bc0,
-                                           size_t pstride, cf* __restrict__ C, size_t ldc) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int bm = (b % gx) * CT, bn = (b / gx) * CT;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-    const int li = lane & 15, lk = (lane >> 4) * 8;
-    // operand fragments of both k-steps first, so that they and the C tile
-    // below are in flight together: pa/pb[k-step][block][plane]
-    bf8 pa[2][2][4], pb[2][2][4];
+// ---------------------------------------------------------------------------
+// Trailing update over K = 64 * nks: C -= sum over chunks q of L21_q U12_q,
+// chunk q's planes at base + q * sstride (L21 planes first, U12 planes
+// 4 * pstride after them).  Up to two output regions per launch.  Tile order
+// inside a region: panels of UPD_PANEL column tiles, row tiles outer and the
+// panel's column tiles inner, so the workgroups resident on one XCD at a
+// time share a few L21 and U12 tiles in that XCD's L2.  Operands of k-step
+// ks+1 are loaded while the MFMAs of k-step ks run.
+// ---------------------------------------------------------------------------
+struct UpdRegion {
+    int m, nc;   // rows and columns of the region
+    int r0, c0;  // first row / column (absolute: the plane index)
+    cf* C;       // A(r0, c0)
+};
+constexpr int UPD_PANEL = 8;
+
+__device__ __forceinline__ void upd_load(bf8 (&fa)[2][4], bf8 (&fb)[2][4],
+                                         const __bf16* const (&pa)[2],
+                                         const __bf16* const (&pb)[2], size_t off,
+                                         size_t pstride) {
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int pl = 0; pl < 4; ++pl) fa[x][pl] = *(const bf8*)(pa[x] + off + pl * pstride);
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int pl = 0; pl < 4; ++pl) fb[y][pl] = *(const bf8*)(pb[y] + off + pl * pstride);
+}
+
+// acc -= A B for one 32-wide k-step (12 MFMA per complex 16 x 16 block)
+__device__ __forceinline__ void upd_mfma(f4 (&accR)[2][2], f4 (&accI)[2][2],
+                                         const bf8 (&fa)[2][4], const bf8 (&fb)[2][4]) {
+    bf8 narh[2], narl[2], naih[2], nail[2];
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
-        const __bf16* p = la + (size_t)(ar0 + min(bm + wm + 16 * x + li, m - 1)) * CT + lk;
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks)
-#pragma unroll
-            for (int pl = 0; pl < 4; ++pl) pa[ks][x][pl] = *(const bf8*)(p + 32 * ks + pl * pstride);
+        narh[x] = neg_bf8(fa[x][0]);
+        narl[x] = neg_bf8(fa[x][1]);
+        naih[x] = neg_bf8(fa[x][2]);
+        nail[x] = neg_bf8(fa[x][3]);
     }
 #pragma unroll
     for (int y = 0; y < 2; ++y) {
-        const __bf16* p = ub + (size_t)(bc0 + min(bn + wn + 16 * y + li, nc - 1)) * CT + lk;
+        const bf8 brh = fb[y][0], brl = fb[y][1], bih = fb[y][2], bil = fb[y][3];
 #pragma unroll
-        for (int ks = 0; ks < 2; ++ks)
-#pragma unroll
-            for (int pl = 0; pl < 4; ++pl) pb[ks][y][pl] = *(const bf8*)(p + 32 * ks + pl * pstride);
+        for (int x = 0; x < 2; ++x) {
+            const bf8 aih = fa[x][2], ail = fa[x][3];
+            f4 r = accR[x][y], i = accI[x][y];
+            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narh[x], brh, r, 0, 0, 0);
+            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narh[x], brl, r, 0, 0, 0);
+            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narl[x], brh, r, 0, 0, 0);
+            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, bih, r, 0, 0, 0);
+            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, bil, r, 0, 0, 0);
+            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail, bih, r, 0, 0, 0);
+            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narh[x], bih, i, 0, 0, 0);
+            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narh[x], bil, i, 0, 0, 0);
+            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narl[x], bih, i, 0, 0, 0);
+            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih[x], brh, i, 0, 0, 0);
+            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih[x], brl, i, 0, 0, 0);
+            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nail[x], brh, i, 0, 0, 0);
+            accR[x][y] = r;
+            accI[x][y] = i;
+        }
     }
+}
+
+__device__ __forceinline__ void upd_tile(const UpdRegion& R, int t, const __bf16* __restrict__ base,
+                                         size_t pstride, size_t sstride, int nks, size_t ldc) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int gx = (R.m + CT - 1) / CT, gy = (R.nc + CT - 1) / CT;
+    const int p = t / (gx * UPD_PANEL);
+    const int within = t - p * gx * UPD_PANEL;
+    const int pw = min(UPD_PANEL, gy - p * UPD_PANEL);
+    const int bm = (within / pw) * CT, bn = (p * UPD_PANEL + within % pw) * CT;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int li = lane & 15, lk = (lane >> 4) * 8;
+    const __bf16* pa[2];
+    const __bf16* pb[2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+        pa[x] = base + (size_t)(R.r0 + min(bm + wm + 16 * x + li, R.m - 1)) * CT + lk;
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+        pb[y] = base + 4 * pstride + (size_t)(R.c0 + min(bn + wn + 16 * y + li, R.nc - 1)) * CT + lk;
+    bf8 a0[2][4], b0[2][4], a1[2][4], b1[2][4];
+    upd_load(a0, b0, pa, pb, 0, pstride);
     f4 accR[2][2], accI[2][2];
-    // C tile into the accumulators: a lane's 4 D registers are 4 consecutive
-    // rows of one column, 32 contiguous bytes.  Interior tiles of an aligned
-    // C take a branch-free path (all eight 16-byte loads in flight at once);
-    // edge tiles go element by element.
-    const bool fast = bm + CT <= m && bn + CT <= nc && (ldc & 3) == 0 &&
+    cf* __restrict__ C = R.C;
+    const bool fast = bm + CT <= R.m && bn + CT <= R.nc && (ldc & 3) == 0 &&
                       ((uintptr_t)C & 31) == 0;
     if (fast) {
         float4 v0[2][2], v1[2][2];
@@ -719,49 +834,18 @@ __device__ __forceinline__ void cgemm_tile(int b, int gx, int m, int nc,
                 const int row0 = bm + wm + 16 * x + (lane >> 4) * 4;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    cf v = (row0 + r < m && col < nc) ? C[(size_t)col * ldc + row0 + r]
-                                                      : cf{0.f, 0.f};
+                    cf v = (row0 + r < R.m && col < R.nc) ? C[(size_t)col * ldc + row0 + r]
+                                                          : cf{0.f, 0.f};
                     accR[x][y][r] = v.x;
                     accI[x][y][r] = v.y;
                 }
             }
     }
-#pragma unroll
-    for (int ks = 0; ks < 2; ++ks) {
-        bf8 narh[2], narl[2], naih[2], nail[2];
-#pragma unroll
-        for (int x = 0; x < 2; ++x) {
-            narh[x] = neg_bf8(pa[ks][x][0]);
-            narl[x] = neg_bf8(pa[ks][x][1]);
-            naih[x] = neg_bf8(pa[ks][x][2]);
-            nail[x] = neg_bf8(pa[ks][x][3]);
-        }
-#pragma unroll
-        for (int y = 0; y < 2; ++y) {
-            const bf8 brh = pb[ks][y][0], brl = pb[ks][y][1], bih = pb[ks][y][2],
-                      bil = pb[ks][y][3];
-#pragma unroll
-            for (int x = 0; x < 2; ++x) {
-                const bf8 aih = pa[ks][x][2], ail = pa[ks][x][3];
-                f4 r = accR[x][y], i = accI[x][y];
-                // Cr -= Ar Br - Ai Bi
-                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narh[x], brh, r, 0, 0, 0);
-                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narh[x], brl, r, 0, 0, 0);
-                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narl[x], brh, r, 0, 0, 0);
-                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, bih, r, 0, 0, 0);
-                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, bil, r, 0, 0, 0);
-                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail, bih, r, 0, 0, 0);
-                // Ci -= Ar Bi + Ai Br
-                i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narh[x], bih, i, 0, 0, 0);
-                i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narh[x], bil, i, 0, 0, 0);
-                i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narl[x], bih, i, 0, 0, 0);
-                i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih[x], brh, i, 0, 0, 0);
-                i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih[x], brl, i, 0, 0, 0);
-                i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nail[x], brh, i, 0, 0, 0);
-                accR[x][y] = r;
-                accI[x][y] = i;
-            }
-        }
+    for (int q = 0; q < nks; ++q) {
+        upd_load(a1, b1, pa, pb, (size_t)q * sstride + 32, pstride);
+        upd_mfma(accR, accI, a0, b0);
+        if (q + 1 < nks) upd_load(a0, b0, pa, pb, (size_t)(q + 1) * sstride, pstride);
+        upd_mfma(accR, accI, a1, b1);
     }
     if (fast) {
 #pragma unroll
@@ -781,38 +865,241 @@ __device__ __forceinline__ void cgemm_tile(int b, int gx, int m, int nc,
             for (int y = 0; y < 2; ++y) {
                 const int col = bn + wn + 16 * y + li;
                 const int row0 = bm + wm + 16 * x + (lane >> 4) * 4;
-                if (col >= nc) continue;
+                if (col >= R.nc) continue;
                 const f4 r = accR[x][y], i = accI[x][y];
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (row0 + q < m) C[(size_t)col * ldc + row0 + q] = cf{r[q], i[q]};
+                for (int qq = 0; qq < 4; ++qq)
+                    if (row0 + qq < R.m) C[(size_t)col * ldc + row0 + qq] = cf{r[qq], i[qq]};
             }
     }
 }
 
-__global__ __launch_bounds__(256) void k_cgemm_planes(int m, int nc, const __bf16* __restrict__ la,
-                                                      int ar0, const __bf16* __restrict__ ub,
-                                                      int bc0, size_t pstride,
-                                                      cf* __restrict__ C, size_t ldc) {
-    const int gx = (m + CT - 1) / CT, gy = (nc + CT - 1) / CT;
-    const int tiles = gx * gy;
-    // XCD-aware tile order: workgroup id runs on XCD id % NXCD; give each XCD
-    // a contiguous range of tiles.  A grid smaller than the tile count loops
-    // (its size a multiple of NXCD keeps every id on its XCD), which leaves
-    // CUs to the kernels of the other stream.
+__device__ inline int upd_tiles(const UpdRegion& R) {
+    return R.m > 0 && R.nc > 0 ? ((R.m + CT - 1) / CT) * ((R.nc + CT - 1) / CT) : 0;
+}
+
+// Grid: any multiple of NXCD; workgroup id runs on XCD id % NXCD and each
+// XCD takes a contiguous range of the tiles of both regions.
+__global__ __launch_bounds__(256) void k_cgemm_update(UpdRegion r0, UpdRegion r1,
+                                                      const __bf16* __restrict__ base,
+                                                      size_t pstride, size_t sstride, int nks,
+                                                      size_t ldc) {
+    const int t0 = upd_tiles(r0), tiles = t0 + upd_tiles(r1);
     const int per = (tiles + NXCD - 1) / NXCD;
     for (int id = blockIdx.x; id < per * NXCD; id += gridDim.x) {
-        const int b = (id % NXCD) * per + id / NXCD;
-        if (b < tiles) cgemm_tile(b, gx, m, nc, la, ar0, ub, bc0, pstride, C, ldc);
+        const int t = (id % NXCD) * per + id / NXCD;
+        if (t < t0)
+            upd_tile(r0, t, base, pstride, sstride, nks, ldc);
+        else if (t < tiles)
+            upd_tile(r1, t - t0, base, pstride, sstride, nks, ldc);
     }
 }
 
-// Number of workgroups of k_cgemm_planes for an m x nc update, at most
-// `cap` (a multiple of NXCD; 0 = one workgroup per tile).
-static int cgemm_groups(int m, int nc, int cap = 0) {
-    const int tiles = ((m + CT - 1) / CT) * ((nc + CT - 1) / CT);
-    const int g = ((tiles + NXCD - 1) / NXCD) * NXCD;
-    return cap > 0 ? std::min(g, cap) : g;
+// ---------------------------------------------------------------------------
+// The same update with 128 x 128 workgroup tiles staged through LDS (used
+// for every update; k_cgemm_update above is its 64 x 64 register-only form).
+// 512 threads = 8 waves of 64 x 32 (4 x 2 MFMA blocks).  Per stage of 32 k
+// the workgroup copies the tile's L21 and U12 planes (2 x 4 planes x 128 x
+// 32 bf16 = 64 KB) into LDS once, so every operand byte is read from L2 once
+// per workgroup instead of once per wave; the next stage's planes are loaded
+// into registers while the MFMAs of the current one run.  In LDS a row of a
+// plane is 32 k = 64 bytes; its four 16-byte parts are stored XOR-swizzled
+// by (row / 4) % 4, so the 16 rows that a quarter-wave reads in one
+// ds_read_b128 hit distinct banks.  The products accumulate from zero and C
+// is loaded during the first stage and updated once at the end.
+// ---------------------------------------------------------------------------
+constexpr int BT = 128;            // workgroup tile
+constexpr int KS = 32;             // k per stage
+constexpr int PLN = BT * KS;       // bf16 per plane per operand per stage
+constexpr int UPD2_PANEL = 4;      // column tiles per panel of the tile order
+
+__device__ __forceinline__ int swz(int r, int part) { return r * KS + ((part ^ ((r >> 2) & 3)) << 3); }
+
+__device__ __forceinline__ void upd2_tile(const UpdRegion& R, int t, const __bf16* __restrict__ base,
+                                          size_t pstride, size_t sstride, int nks, size_t ldc,
+                                          __bf16* __restrict__ la, __bf16* __restrict__ lb) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int gx = (R.m + BT - 1) / BT, gy = (R.nc + BT - 1) / BT;
+    const int p = t / (gx * UPD2_PANEL);
+    const int within = t - p * gx * UPD2_PANEL;
+    const int pw = min(UPD2_PANEL, gy - p * UPD2_PANEL);
+    const int bm = (within / pw) * BT, bn = (p * UPD2_PANEL + within % pw) * BT;
+    // copy assignment: thread -> (row, 16-byte part) of every plane
+    const int crow = tid >> 2, cpart = tid & 3;
+    const __bf16* ga = base + (size_t)(R.r0 + min(bm + crow, R.m - 1)) * CT + cpart * 8;
+    const __bf16* gb = base + 4 * pstride + (size_t)(R.c0 + min(bn + crow, R.nc - 1)) * CT + cpart * 8;
+    const int coff = swz(crow, cpart);
+    // compute assignment: wave tile rows 64 wr.., columns 32 wc..
+    const int wr = w & 1, wc = w >> 1;
+    const int li = lane & 15, lq = lane >> 4;
+    u4 ra[4], rb[4];
+    auto gload = [&](int s) {
+        const size_t off = (size_t)(s >> 1) * sstride + (s & 1) * KS;
+#pragma unroll
+        for (int pl = 0; pl < 4; ++pl) {
+            ra[pl] = *(const u4*)(ga + off + pl * pstride);
+            rb[pl] = *(const u4*)(gb + off + pl * pstride);
+        }
+    };
+    gload(0);
+    // C of this wave's 64 x 32 block, in flight during the first stage
+    cf* __restrict__ C = R.C;
+    const bool fast = bm + BT <= R.m && bn + BT <= R.nc && (ldc & 3) == 0 &&
+                      ((uintptr_t)C & 31) == 0;
+    float4 c0[4][2], c1[4][2];
+    if (fast) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                const cf* cp = C + (size_t)(bn + 32 * wc + 16 * y + li) * ldc + bm + 64 * wr +
+                               16 * x + lq * 4;
+                c0[x][y] = *(const float4*)cp;
+                c1[x][y] = *(const float4*)(cp + 2);
+            }
+    }
+    f4 accR[4][2], accI[4][2];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            accR[x][y] = f4{0.f, 0.f, 0.f, 0.f};
+            accI[x][y] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+    const int nst = 2 * nks;
+    for (int s = 0; s < nst; ++s) {
+        __syncthreads();  // every wave is done with the previous stage
+#pragma unroll
+        for (int pl = 0; pl < 4; ++pl) {
+            *(u4*)(la + pl * PLN + coff) = ra[pl];
+            *(u4*)(lb + pl * PLN + coff) = rb[pl];
+        }
+        __syncthreads();
+        if (s + 1 < nst) gload(s + 1);
+        bf8 fb[2][4];
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int pl = 0; pl < 4; ++pl)
+                fb[y][pl] = *(const bf8*)(lb + pl * PLN + swz(32 * wc + 16 * y + li, lq));
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+            bf8 fa[4];
+#pragma unroll
+            for (int pl = 0; pl < 4; ++pl)
+                fa[pl] = *(const bf8*)(la + pl * PLN + swz(64 * wr + 16 * x + li, lq));
+            const bf8 naih = neg_bf8(fa[2]), nail = neg_bf8(fa[3]);
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                f4 r = accR[x][y], i = accI[x][y];
+                // + Ar Br - Ai Bi
+                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0], fb[y][0], r, 0, 0, 0);
+                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0], fb[y][1], r, 0, 0, 0);
+                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[1], fb[y][0], r, 0, 0, 0);
+                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, fb[y][2], r, 0, 0, 0);
+                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, fb[y][3], r, 0, 0, 0);
+                r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nail, fb[y][2], r, 0, 0, 0);
+                // + Ar Bi + Ai Br
+                i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0], fb[y][2], i, 0, 0, 0);
+                i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[0], fb[y][3], i, 0, 0, 0);
+                i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[1], fb[y][2], i, 0, 0, 0);
+                i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[2], fb[y][0], i, 0, 0, 0);
+                i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[2], fb[y][1], i, 0, 0, 0);
+                i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[3], fb[y][0], i, 0, 0, 0);
+                accR[x][y] = r;
+                accI[x][y] = i;
+            }
+        }
+    }
+    // C -= L21 U12
+    if (fast) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                cf* cp = C + (size_t)(bn + 32 * wc + 16 * y + li) * ldc + bm + 64 * wr + 16 * x +
+                         lq * 4;
+                const f4 r = accR[x][y], i = accI[x][y];
+                *(float4*)cp = float4{c0[x][y].x - r[0], c0[x][y].y - i[0], c0[x][y].z - r[1],
+                                      c0[x][y].w - i[1]};
+                *(float4*)(cp + 2) = float4{c1[x][y].x - r[2], c1[x][y].y - i[2],
+                                            c1[x][y].z - r[3], c1[x][y].w - i[3]};
+            }
+    } else {
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                const int col = bn + 32 * wc + 16 * y + li;
+                const int row0 = bm + 64 * wr + 16 * x + lq * 4;
+                if (col >= R.nc) continue;
+                const f4 r = accR[x][y], i = accI[x][y];
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq)
+                    if (row0 + qq < R.m) {
+                        cf* cp = C + (size_t)col * ldc + row0 + qq;
+                        const cf v = *cp;
+                        *cp = cf{v.x - r[qq], v.y - i[qq]};
+                    }
+            }
+    }
+}
+
+__device__ inline int upd2_tiles(const UpdRegion& R) {
+    return R.m > 0 && R.nc > 0 ? ((R.m + BT - 1) / BT) * ((R.nc + BT - 1) / BT) : 0;
+}
+
+// Grid: any multiple of NXCD (workgroup id runs on XCD id % NXCD; each XCD
+// takes a contiguous range of the tiles of both regions).
+__global__ __launch_bounds__(512) void k_cgemm_update2(UpdRegion r0, UpdRegion r1,
+                                                       const __bf16* __restrict__ base,
+                                                       size_t pstride, size_t sstride, int nks,
+                                                       size_t ldc) {
+    __shared__ __attribute__((aligned(16))) __bf16 lds_a[4 * PLN];
+    __shared__ __attribute__((aligned(16))) __bf16 lds_b[4 * PLN];
+    const int t0 = upd2_tiles(r0), tiles = t0 + upd2_tiles(r1);
+    const int per = (tiles + NXCD - 1) / NXCD;
+    for (int id = blockIdx.x; id < per * NXCD; id += gridDim.x) {
+        const int t = (id % NXCD) * per + id / NXCD;
+        if (t < t0)
+            upd2_tile(r0, t, base, pstride, sstride, nks, ldc, lds_a, lds_b);
+        else if (t < tiles)
+            upd2_tile(r1, t - t0, base, pstride, sstride, nks, ldc, lds_a, lds_b);
+    }
+}
+
+static int upd2_tiles_host(const UpdRegion& R) {
+    return R.m > 0 && R.nc > 0 ? ((R.m + BT - 1) / BT) * ((R.nc + BT - 1) / BT) : 0;
+}
+
+static int upd_tiles_host(const UpdRegion& R) {
+    return R.m > 0 && R.nc > 0 ? ((R.m + CT - 1) / CT) * ((R.nc + CT - 1) / CT) : 0;
+}
+
+// Launch of k_cgemm_update for regions a and b (b may be empty); nothing
+// when both are empty.
+static void launch_update(hipStream_t s, const UpdRegion& a, const UpdRegion& b,
+                          const __bf16* base, size_t pstride, size_t sstride, int nks,
+                          size_t ldc, int cap = 0) {
+    static const bool small = [] {
+        const char* v = getenv("BEMX_UPD_TILE");
+        return v && atoi(v) == 64;
+    }();
+    if (small) {
+        const int tiles = upd_tiles_host(a) + upd_tiles_host(b);
+        if (tiles == 0) return;
+        int g = ((tiles + NXCD - 1) / NXCD) * NXCD;
+        if (cap > 0) g = std::min(g, cap);
+        hipLaunchKernelGGL(k_cgemm_update, dim3(g), dim3(256), 0, s, a, b, base, pstride, sstride,
+                           nks, ldc);
+        return;
+    }
+    const int tiles = upd2_tiles_host(a) + upd2_tiles_host(b);
+    if (tiles == 0) return;
+    int g = ((tiles + NXCD - 1) / NXCD) * NXCD;
+    if (cap > 0) g = std::min(g, cap);
+    hipLaunchKernelGGL(k_cgemm_update2, dim3(g), dim3(512), 0, s, a, b, base, pstride, sstride,
+                       nks, ldc);
 }
 
 // Workgroups of the bulk update on the side stream (BEMX_BULK_WGS; default
@@ -950,8 +1237,40 @@ struct MixedFactor {
     size_t ld = 0;  // leading dimension of A
 };
 
+// Blocks per group of the factorisation (BEMX_GROUP, 1..8; default 4): the
+// trailing update outside a group runs once per group with K = 64 * group.
+static int group_blocks() {
+    static const int g = [] {
+        const char* v = getenv("BEMX_GROUP");
+        const int x = v ? atoi(v) : 4;
+        return std::min(8, std::max(1, x));
+    }();
+    return g;
+}
+
+size_t mixed_factor_extra_bytes(int n) {
+    const size_t nb = (size_t)(n + NBM - 1) / NBM, np = (nb + 1) / 2;
+    const size_t planes = sizeof(__bf16) * (size_t)n * NBM * 8 * 2 * group_blocks();
+    const size_t inv = sizeof(cf) * nb * NBM * NBM * 2 + sizeof(cf) * np * NB2 * NB2 * 2;
+    return planes + inv;
+}
+
 // The launch sequence of the factorisation (no host synchronisation, no
 // allocation), so it can be captured into a graph.
+//
+// Blocks of 64 columns are factored in groups of s = group_blocks() (W = 64 s
+// columns).  Step k of group g (its columns end at gend):
+//   st:   getrf(k); L21(k) on all rows below and U12(k) on all columns to the
+//         right (one launch, bf16 planes into the group's slot of k); then
+//         the update of the group's own columns (rows below k) and of the
+//         group's own rows (columns beyond gend), K = 64.
+// After the last step of group g, with K = W from the group's planes:
+//   st:   [wait Z(g-1)] X(g): the next group's columns, all rows below gend;
+//         Y(g): the next group's rows, the columns beyond it;
+//   side: Z(g): everything beyond the next group (the bulk),
+// so group g+1 is factored while the bulk update of group g runs.  Planes
+// are double-buffered by group parity: group g+1 writes the slots Z(g-1) read,
+// after the wait for Z(g-1).
 static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda,
                                  const MixedFactor& F, __bf16* planes, size_t pstride) {
     hipError_t first = hipSuccess;
@@ -959,62 +1278,61 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
         if (first == hipSuccess && e != hipSuccess) first = e;
     };
     const int nblk = F.nblk;
+    const int sg = group_blocks();
+    const size_t sstride = 8 * pstride;  // one slot: 4 L21 planes, 4 U12 planes
     hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, st, F.status, 0, 1);
     size_t tot = (size_t)n * n;
     const size_t ld = F.ld;
     hipLaunchKernelGGL(k_cvt_z2c, dim3((tot + 255) / 256), dim3(256), 0, st, A64, lda, F.A,
                        ld, n);
-    // Look-ahead over two streams.  Step k (block columns: k, N = k+1, R =
-    // the rest):
-    //   st:   getrf(k), L21(k); [wait bulk(k-1)] U12(k) on N, update of N
-    //   side: [wait L21(k)] U12(k) on R, update of R (the bulk)
-    // so getrf(k+1) and L21(k+1) run while the bulk update of step k does.
+    auto at = [&](int r, int c) { return F.A + (size_t)c * ld + r; };
+    const UpdRegion none{0, 0, 0, 0, nullptr};
     const hipStream_t sd = w.side;
     bool bulk_pending = false;
     for (int k = 0; k < nblk; ++k) {
+        const int g = k / sg, j = k - g * sg;
         const int k0 = k * NBM, nb = std::min(NBM, n - k0);
-        int* lp = F.lperm + (size_t)k * NBM;
-        __bf16* la = planes + (size_t)(k & 1) * 8 * pstride;
-        __bf16* ub = la + 4 * pstride;
-        hipLaunchKernelGGL(k_diag_getrf32, dim3(1), dim3(256), 0, st, F.A, ld, k0, nb, lp,
-                           F.status);
-        cf* li = F.inv + (size_t)k * NBM * NBM;
-        cf* ui = F.inv + (size_t)(nblk + k) * NBM * NBM;
         const int c0 = k0 + nb, rest = n - c0;
-        if (rest <= NBM) {
-            // no bulk update left to hide it behind: inverses on the main stream
-            hipLaunchKernelGGL(k_diag_inverses, dim3(2), dim3(256), 0, st, (const cf*)F.A, ld, k0,
-                               nb, (const int*)lp, li, ui);
-        }
+        const int gend = std::min(n, (g + 1) * sg * NBM);
+        int* lp = F.lperm + (size_t)k * NBM;
+        __bf16* gbase = planes + (size_t)(g & 1) * sg * sstride;
+        __bf16* la = gbase + (size_t)j * sstride;
+        __bf16* ub = la + 4 * pstride;
+        hipLaunchKernelGGL(k_diag_getrf_panels, dim3(1), dim3(256), 0, st, F.A, ld, k0, nb, lp,
+                           F.status);
         if (rest <= 0) break;
-        hipLaunchKernelGGL(k_trsm_upper_rows, dim3((rest + TRR_ROWS - 1) / TRR_ROWS), dim3(256),
-                           0, st, F.A, ld, k0, nb, c0, rest, la, pstride);
-        const int nn1 = std::min(NBM, rest), nr = rest - nn1;
-        if (nr > 0) chk(hipEventRecord(w.ev_panel, st));
+        const int nrb = (rest + TRR_ROWS - 1) / TRR_ROWS;
+        const int ncb = (rest + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG;
+        hipLaunchKernelGGL(k_trsm_both, dim3(nrb + ncb), dim3(256), 0, st, F.A, ld, k0, nb, c0,
+                           rest, rest, (const int*)lp, la, ub, pstride);
+        if (c0 < gend) {
+            // inside the group: its columns right of k, and its rows below k
+            const UpdRegion cs{rest, gend - c0, c0, c0, at(c0, c0)};
+            const UpdRegion rs{gend - c0, n - gend, c0, gend, at(c0, gend)};
+            launch_update(st, cs, n > gend ? rs : none, la, pstride, sstride, 1, ld);
+            continue;
+        }
+        // last block of group g: apply the group to the rest, K = 64 * sg
+        const int nend = std::min(n, gend + sg * NBM);
         if (bulk_pending) chk(hipStreamWaitEvent(st, w.ev_bulk, 0));
-        hipLaunchKernelGGL((k_trsm_lower_cols<cf, true>),
-                           dim3((nn1 + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG), dim3(256), 0,
-                           st, F.A, ld, k0, nb, c0, nn1, (const int*)lp, ub, pstride);
-        hipLaunchKernelGGL(k_cgemm_planes, dim3(cgemm_groups(rest, nn1)), dim3(256), 0, st, rest,
-                           nn1, (const __bf16*)la, c0, (const __bf16*)ub, c0, pstride,
-                           F.A + c0 * ld + c0, ld);
         bulk_pending = false;
-        if (nr > 0) {
-            const int cr = c0 + nn1;
+        const UpdRegion X{n - gend, nend - gend, gend, gend, at(gend, gend)};
+        const UpdRegion Y{nend - gend, n - nend, gend, nend, at(gend, nend)};
+        launch_update(st, X, n > nend ? Y : none, gbase, pstride, sstride, sg, ld);
+        if (n > nend) {
+            const UpdRegion Z{n - nend, n - nend, nend, nend, at(nend, nend)};
+            chk(hipEventRecord(w.ev_panel, st));
             chk(hipStreamWaitEvent(sd, w.ev_panel, 0));
-            hipLaunchKernelGGL(k_diag_inverses, dim3(2), dim3(256), 0, sd, (const cf*)F.A, ld, k0,
-                               nb, (const int*)lp, li, ui);
-            hipLaunchKernelGGL((k_trsm_lower_cols<cf, true>),
-                               dim3((nr + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG), dim3(256), 0,
-                               sd, F.A, ld, k0, nb, cr, nr, (const int*)lp, ub, pstride);
-            hipLaunchKernelGGL(k_cgemm_planes, dim3(cgemm_groups(rest, nr, bulk_wg_cap())), dim3(256), 0, sd,
-                               rest, nr, (const __bf16*)la, c0, (const __bf16*)ub, cr, pstride,
-                               F.A + cr * ld + c0, ld);
+            launch_update(sd, Z, none, gbase, pstride, sstride, sg, ld, bulk_wg_cap());
             chk(hipEventRecord(w.ev_bulk, sd));
             bulk_pending = true;
         }
     }
     if (bulk_pending) chk(hipStreamWaitEvent(st, w.ev_bulk, 0));
+    // explicit inverses of every diagonal block (L_bb^-1 P_bb and U_bb^-1),
+    // all blocks in one launch: a diagonal block is final once its getrf ran
+    hipLaunchKernelGGL(k_diag_inverses, dim3(2, nblk), dim3(256), 0, st, (const cf*)F.A, ld, n,
+                       (const int*)F.lperm, F.inv);
     // 128-row diagonal-block inverses of both factors for the block solves
     hipLaunchKernelGGL(k_pair_inverses, dim3(2, F.npair), dim3(256), 0, st, (const cf*)F.A, ld,
                        n, (const cf*)F.inv, nblk, F.minv, F.minv + (size_t)F.npair * NB2 * NB2);
@@ -1057,10 +1375,12 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_trsv_upper_coop, 512, 0));
     F.coop = std::max(1, std::min(F.npair, ncu * std::min(per, per2)));
     MX_CHK(w.ensure_side());
-    // bf16 planes of L21 and U12, double-buffered by step parity: step k+1
-    // writes one buffer while the bulk update of step k still reads the other
+    // bf16 planes of L21 and U12: one slot (8 planes) per block of a group,
+    // double-buffered by group parity (group g+1 writes one set while the bulk
+    // update of group g still reads the other)
     const size_t pstride = (size_t)n * NBM;
-    __bf16* planes = (__bf16*)w.get(M_PLANES, sizeof(__bf16) * pstride * 4 * 4, &e);
+    __bf16* planes = (__bf16*)w.get(
+        M_PLANES, sizeof(__bf16) * pstride * 8 * 2 * (size_t)group_blocks(), &e);
     MX_CHK(e);
     // The launch sequence depends only on n and the buffer addresses: it is
     // captured once into a graph and replayed while those stay the same.
@@ -1075,11 +1395,11 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
             const hipError_t e1 = enqueue_factor(w, st, n, A64, lda, F, planes, pstride);
             const hipError_t e2 = hipStreamEndCapture(st, &g);
             if (e1 != hipSuccess || e2 != hipSuccess) {
-                if (g) hipGraphDestroy(g);
+                if (g) (void)hipGraphDestroy(g);
                 MX_CHK(e1 != hipSuccess ? e1 : e2);
             }
             const hipError_t e3 = hipGraphInstantiate(&w.gexec, g, nullptr, nullptr, 0);
-            hipGraphDestroy(g);
+            (void)hipGraphDestroy(g);
             MX_CHK(e3);
             memcpy(w.gkey, key, sizeof key);
         }
