@@ -13,6 +13,15 @@ __host__ __device__ inline cf operator-(cf a, cf b) { return cf{a.x - b.x, a.y -
 __host__ __device__ inline cf operator*(cf a, cf b) {
     return cf{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
+// a - x y in four fused multiply-adds (the operator form costs six
+// instructions: a multiply, a fused multiply-add and a subtraction per part)
+__device__ inline cf cfms(cf a, cf x, cf y) {
+    return cf{fmaf(-x.x, y.x, fmaf(x.y, y.y, a.x)), fmaf(-x.x, y.y, fmaf(-x.y, y.x, a.y))};
+}
+__device__ inline cd cfms(cd a, cd x, cd y) {
+    return cd{fma(-x.x, y.x, fma(x.y, y.y, a.x)), fma(-x.x, y.y, fma(-x.y, y.x, a.y))};
+}
+
 // 1 / a for the FP32 factors, which only precondition: v_rcp_f32 (1 ulp)
 // of |a|^2 instead of a full-precision division.
 __device__ inline cf cinvf(cf a) {
@@ -104,7 +113,7 @@ __device__ __forceinline__ void trsm_lower_cols_body(int bid, T* __restrict__ A,
 #pragma unroll
             for (int i = 0; i < 15; ++i) {
 #pragma unroll
-                for (int r = i + 1; r < 16; ++r) a[r] = a[r] - Ls[(g + i) * TB + g + r] * a[i];
+                for (int r = i + 1; r < 16; ++r) a[r] = cfms(a[r], Ls[(g + i) * TB + g + r], a[i]);
             }
 #pragma unroll
             for (int r = 0; r < 16; ++r) X[(g + r) * XS + lane] = a[r];
@@ -119,7 +128,7 @@ __device__ __forceinline__ void trsm_lower_cols_body(int bid, T* __restrict__ A,
             for (int i = 0; i < 16; ++i) {
                 const T xi = X[(g + i) * XS + lane];
 #pragma unroll
-                for (int r = 0; r < 16; ++r) a[r] = a[r] - Ls[(g + i) * TB + g2 + r] * xi;
+                for (int r = 0; r < 16; ++r) a[r] = cfms(a[r], Ls[(g + i) * TB + g2 + r], xi);
             }
 #pragma unroll
             for (int r = 0; r < 16; ++r) X[(g2 + r) * XS + lane] = a[r];

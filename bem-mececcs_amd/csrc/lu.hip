@@ -189,12 +189,13 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
         const int d = k0 + jj;
         if (d >= r0 && d < r1 && tid >= 64 && tid < 64 + nb)
             drow[(size_t)jj * NB + tid - 64] = sA[(tid - 64) * PANEL_ROWS + (d - r0)];
-        __threadfence();
+        // the barrier completes this workgroup's stores; the release add of
+        // the count then makes them (and the slot) visible at device scope
+        // with a single L2 write-back
         __syncthreads();
         if (tid == 0) {
             slots[(size_t)jj * P + p] = Cand{s_cval, myrow, 0};
-            __threadfence();
-            atomicAdd(&counters[jj], 1);
+            __hip_atomic_fetch_add(&counters[jj], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
     };
 
@@ -257,7 +258,7 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
         if (live) {
             const cd l = sA[j * PANEL_ROWS + tid] * inv;
             sA[j * PANEL_ROWS + tid] = l;
-            if (j + 1 < nb) sA[(j + 1) * PANEL_ROWS + tid] = sA[(j + 1) * PANEL_ROWS + tid] - l * urow[j + 1];
+            if (j + 1 < nb) sA[(j + 1) * PANEL_ROWS + tid] = cfms(sA[(j + 1) * PANEL_ROWS + tid], l, urow[j + 1]);
         }
         __syncthreads();
         int sp1 = -1, sp2 = -1;  // local rows already completed for columns j+2..
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
             const int sr = tid < 64 ? sp1 : (tid < 128 ? sp2 : -1);
             const int c = j + 2 + lane;
             if (sr >= 0 && c < nb && pval != 0.0)
-                sA[c * PANEL_ROWS + sr] = sA[c * PANEL_ROWS + sr] - sA[j * PANEL_ROWS + sr] * urow[c];
+                sA[c * PANEL_ROWS + sr] = cfms(sA[c * PANEL_ROWS + sr], sA[j * PANEL_ROWS + sr], urow[c]);
             __syncthreads();
             publish(j + 1);
         }
@@ -278,7 +279,7 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
         if (r < nr && r0 + r > d && r != sp1 && r != sp2 && pval != 0.0) {
             const cd l = sA[j * PANEL_ROWS + r];
             for (int c = j + 2 + half; c < nb; c += 2)
-                sA[c * PANEL_ROWS + r] = sA[c * PANEL_ROWS + r] - l * urow[c];
+                sA[c * PANEL_ROWS + r] = cfms(sA[c * PANEL_ROWS + r], l, urow[c]);
         }
         __syncthreads();
     }

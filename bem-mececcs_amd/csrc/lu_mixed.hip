@@ -7,7 +7,7 @@
 //                      (registers: lane = row, wave = a panel of 16 columns)
 //   k_trsm_both        L21 = A21 U11^-1 (one lane per row) and
 //                      U12 = L11^-1 P A12 (one lane per column), one launch
-//   k_cgemm_update     A22 -= L21 U12 over K = 64 .. 512; every FP32 operand x
+//   k_cgemm_update2    A22 -= L21 U12 over K = 64 .. 512; every FP32 operand x
 //                      is split into hi = bf16(x), lo = bf16(x - hi) (by the
 //                      triangular kernels, into bf16 planes) and each real
 //                      product is hi*hi + hi*lo + lo*hi on
@@ -77,38 +77,42 @@ __device__ inline cf readlane_f(cf v, int src) {
 
 // Diagonal block: GEPP within its nb rows (largest |re|+|im| to 17 mantissa
 // bits, lowest block row on ties), held in registers, factored by panels of
-// 16 columns: wave q owns columns [16q, 16q+16), lane r owns block row r.
+// PW columns: wave q owns columns [PW q, PW q + PW), lane r owns block row r.
 // Rows are not moved while factoring: a row is marked used when it becomes
 // the pivot of step j (its rank), and the factors are written back in rank
 // order; lperm[i] = block row whose content ends at block row i.
 // Wave q factors its panel
-// alone (16 pivot steps, no barrier: search, multipliers and the update of
+// alone (PW pivot steps, no barrier: search, multipliers and the update of
 // its remaining columns all stay in the wave), publishes the multipliers and
 // pivots in LDS, and after one barrier the waves to its right replay those
-// 16 steps on their own columns while wave q+1 factors the next panel.  The
-// chain is 4 panel factorisations and 3 replays with 3 barriers, instead of
-// a barrier per column.
-__global__ __launch_bounds__(256) void k_diag_getrf_panels(cf* __restrict__ A, size_t lda, int k0,
-                                                           int nb, int* __restrict__ lperm,
-                                                           int* __restrict__ status) {
+// PW steps on their own columns while wave q+1 factors the next panel.  The
+// chain is 64/PW panel factorisations and 64/PW - 1 replays of one panel,
+// with one barrier per panel instead of one per column; narrower panels
+// shorten both (the replays of earlier panels run off the chain).
+template <int PW>
+__global__ __launch_bounds__(64 * (NBM / PW)) void k_diag_getrf_panels(cf* __restrict__ A,
+                                                                       size_t lda, int k0, int nb,
+                                                                       int* __restrict__ lperm,
+                                                                       int* __restrict__ status) {
+    constexpr int NW = NBM / PW;  // waves = panels
     __shared__ cf lmul[NBM][NBM];  // lmul[j][r]: multiplier of row r at step j
     __shared__ int s_piv[NBM];
     __shared__ int s_rank[NBM];
     const int tid = threadIdx.x, lane = tid & 63;
     const int q = __builtin_amdgcn_readfirstlane(tid >> 6);
-    cf v[16];
+    cf v[PW];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int c = 16 * q + i;
+    for (int i = 0; i < PW; ++i) {
+        const int c = PW * q + i;
         v[i] = (lane < nb && c < nb) ? A[(size_t)(k0 + c) * lda + k0 + lane] : mkf(0.f, 0.f);
     }
     bool used = lane >= nb;
     int rank = lane;
-    for (int pq = 0; pq < 4 && 16 * pq < nb; ++pq) {
+    for (int pq = 0; pq < NW && PW * pq < nb; ++pq) {
         if (q == pq) {
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int j = 16 * pq + i;
+            for (int i = 0; i < PW; ++i) {
+                const int j = PW * pq + i;
                 if (j < nb) {
                     // key = |re|+|im| with its 6 lowest mantissa bits
                     // replaced by 63 - lane: the wave maximum picks the
@@ -136,15 +140,15 @@ __global__ __launch_bounds__(256) void k_diag_getrf_panels(cf* __restrict__ A, s
                         if (zero) atomicOr(status, ST_PIVOT_ZERO);
                     }
 #pragma unroll
-                    for (int c = i + 1; c < 16; ++c) v[c] = v[c] - l * readlane_f(v[c], p);
+                    for (int c = i + 1; c < PW; ++c) v[c] = cfms(v[c], l, readlane_f(v[c], p));
                 }
             }
         }
         __syncthreads();
         if (q > pq) {
-            // replay the 16 steps of panel pq on this wave's columns
-            for (int i = 0; i < 16; ++i) {
-                const int j = 16 * pq + i;
+            // replay the PW steps of panel pq on this wave's columns
+            for (int i = 0; i < PW; ++i) {
+                const int j = PW * pq + i;
                 if (j >= nb) break;
                 const int p = __builtin_amdgcn_readfirstlane(s_piv[j]);
                 const cf l = lmul[j][lane];
@@ -153,24 +157,49 @@ __global__ __launch_bounds__(256) void k_diag_getrf_panels(cf* __restrict__ A, s
                     rank = j;
                 }
 #pragma unroll
-                for (int c = 0; c < 16; ++c) v[c] = v[c] - l * readlane_f(v[c], p);
+                for (int c = 0; c < PW; ++c) v[c] = cfms(v[c], l, readlane_f(v[c], p));
             }
         }
     }
     // the last wave with columns below nb has replayed every pivot: its
     // ranks are the final ones
-    const int qlast = (nb - 1) >> 4;
+    const int qlast = (nb - 1) / PW;
     if (q == qlast) s_rank[lane] = rank;
     __syncthreads();
     rank = s_rank[lane];
     if (lane < nb) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int c = 16 * q + i;
+        for (int i = 0; i < PW; ++i) {
+            const int c = PW * q + i;
             if (c < nb) A[(size_t)(k0 + c) * lda + k0 + rank] = v[i];
         }
         if (q == 0) lperm[rank] = lane;
     }
+}
+
+// Panel width of the diagonal-block factorisation (BEMX_GETRF_PW = 16, 8 or
+// 4; default 4: 16 waves of 4 columns).  Measured alone on MI355X: 38 us
+// with 16, 30 us with 8, 27 us with 4.
+static int getrf_pw() {
+    static const int w = [] {
+        const char* v = getenv("BEMX_GETRF_PW");
+        const int x = v ? atoi(v) : 4;
+        return (x == 16 || x == 8) ? x : 4;
+    }();
+    return w;
+}
+
+static void launch_getrf(hipStream_t st, cf* A, size_t ld, int k0, int nb, int* lp, int* status) {
+    const int pw = getrf_pw();
+    if (pw == 16)
+        hipLaunchKernelGGL(k_diag_getrf_panels<16>, dim3(1), dim3(256), 0, st, A, ld, k0, nb, lp,
+                           status);
+    else if (pw == 4)
+        hipLaunchKernelGGL(k_diag_getrf_panels<4>, dim3(1), dim3(1024), 0, st, A, ld, k0, nb, lp,
+                           status);
+    else
+        hipLaunchKernelGGL(k_diag_getrf_panels<8>, dim3(1), dim3(512), 0, st, A, ld, k0, nb, lp,
+                           status);
 }
 
 // L21 = A21 U11^-1 for rows [r0, r0+nr).  256 threads load U11 and 64
@@ -224,7 +253,7 @@ __device__ __forceinline__ void trsm_upper_rows_body(int bid, cf* __restrict__ A
             for (int c = 0; c < 16; ++c) {
                 a[c] = a[c] * dinv[g + c];
 #pragma unroll
-                for (int c2 = c + 1; c2 < 16; ++c2) a[c2] = a[c2] - a[c] * Us[(g + c2) * NBM + g + c];
+                for (int c2 = c + 1; c2 < 16; ++c2) a[c2] = cfms(a[c2], a[c], Us[(g + c2) * NBM + g + c]);
             }
 #pragma unroll
             for (int c = 0; c < 16; ++c) X[(g + c) * XS + lane] = a[c];
@@ -239,7 +268,7 @@ __device__ __forceinline__ void trsm_upper_rows_body(int bid, cf* __restrict__ A
             for (int i = 0; i < 16; ++i) {
                 const cf xi = X[(g + i) * XS + lane];
 #pragma unroll
-                for (int c = 0; c < 16; ++c) a[c] = a[c] - xi * Us[(g2 + c) * NBM + g + i];
+                for (int c = 0; c < 16; ++c) a[c] = cfms(a[c], xi, Us[(g2 + c) * NBM + g + i]);
             }
 #pragma unroll
             for (int c = 0; c < 16; ++c) X[(g2 + c) * XS + lane] = a[c];
@@ -348,7 +377,7 @@ __global__ __launch_bounds__(256) void k_diag_inverses(const cf* __restrict__ A,
 #pragma unroll
                 for (int i = 0; i < 15; ++i) {
 #pragma unroll
-                    for (int r = i + 1; r < 16; ++r) a[r] = a[r] - M[(g + i) * NBM + g + r] * a[i];
+                    for (int r = i + 1; r < 16; ++r) a[r] = cfms(a[r], M[(g + i) * NBM + g + r], a[i]);
                 }
 #pragma unroll
                 for (int r = 0; r < 16; ++r) X[(g + r) * XS + lane] = a[r];
@@ -363,7 +392,7 @@ __global__ __launch_bounds__(256) void k_diag_inverses(const cf* __restrict__ A,
                 for (int i = 0; i < 16; ++i) {
                     const cf xi = X[(g + i) * XS + lane];
 #pragma unroll
-                    for (int r = 0; r < 16; ++r) a[r] = a[r] - M[(g + i) * NBM + g2 + r] * xi;
+                    for (int r = 0; r < 16; ++r) a[r] = cfms(a[r], M[(g + i) * NBM + g2 + r], xi);
                 }
 #pragma unroll
                 for (int r = 0; r < 16; ++r) X[(g2 + r) * XS + lane] = a[r];
@@ -399,7 +428,7 @@ __global__ __launch_bounds__(256) void k_diag_inverses(const cf* __restrict__ A,
                 for (int c = 0; c < 16; ++c) {
                     a[c] = a[c] * dinv[g + c];
 #pragma unroll
-                    for (int c2 = c + 1; c2 < 16; ++c2) a[c2] = a[c2] - a[c] * M[(g + c2) * NBM + g + c];
+                    for (int c2 = c + 1; c2 < 16; ++c2) a[c2] = cfms(a[c2], a[c], M[(g + c2) * NBM + g + c]);
                 }
 #pragma unroll
                 for (int c = 0; c < 16; ++c) X[(g + c) * XS + lane] = a[c];
@@ -414,7 +443,7 @@ __global__ __launch_bounds__(256) void k_diag_inverses(const cf* __restrict__ A,
                 for (int i = 0; i < 16; ++i) {
                     const cf xi = X[(g + i) * XS + lane];
 #pragma unroll
-                    for (int c = 0; c < 16; ++c) a[c] = a[c] - xi * M[(g2 + c) * NBM + g + i];
+                    for (int c = 0; c < 16; ++c) a[c] = cfms(a[c], xi, M[(g2 + c) * NBM + g + i]);
                 }
 #pragma unroll
                 for (int c = 0; c < 16; ++c) X[(g2 + c) * XS + lane] = a[c];
@@ -549,17 +578,17 @@ __global__ __launch_bounds__(256) void k_pair_inverses(const cf* __restrict__ A,
 // Block diagonal step of the 128-row block solves: out = Minv acc, the
 // eight waves each taking 32 columns of one row half; joined through LDS,
 // the two row halves stored by waves 0 and 4, one flag published.
-// sum_i f[i] * y_i over 32 columns (FP32 factor entries, FP64 vector) with
-// four independent accumulators; y_i is read from yb at wave-uniform
-// addresses (LDS broadcast).
-__device__ inline cd dot32(const cf (&f)[32], const cd* yb) {
+// sum_i f[i] * y_i over 32 columns (FP32 factor entries, FP64 vector), y_i
+// held by lane i (i < 32) and broadcast with v_readlane; four independent
+// accumulators keep the FMA chains short.
+__device__ inline cd dot32(const cf (&f)[32], cd yl) {
     cd s0 = mk(0, 0), s1 = mk(0, 0), s2 = mk(0, 0), s3 = mk(0, 0);
 #pragma unroll
     for (int i = 0; i < 32; i += 4) {
-        s0 = s0 + up(f[i]) * yb[i];
-        s1 = s1 + up(f[i + 1]) * yb[i + 1];
-        s2 = s2 + up(f[i + 2]) * yb[i + 2];
-        s3 = s3 + up(f[i + 3]) * yb[i + 3];
+        s0 = s0 + up(f[i]) * readlane_c(yl, i);
+        s1 = s1 + up(f[i + 1]) * readlane_c(yl, i + 1);
+        s2 = s2 + up(f[i + 2]) * readlane_c(yl, i + 2);
+        s3 = s3 + up(f[i + 3]) * readlane_c(yl, i + 3);
     }
     return (s0 + s1) + (s2 + s3);
 }
@@ -583,8 +612,10 @@ __device__ inline void diag128_apply_publish(const cf mv[32], const cd* acc, cd 
         const cd v = (part[4 * rh][lane] + part[4 * rh + 1][lane]) +
                      (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]);
         if (row < nb) out[row] = v;
-        __threadfence();
     }
+    // the barrier completes these stores; the release store of the flag
+    // then makes them visible at device scope (one L2 write-back, not one
+    // per wave)
     __syncthreads();
     if (threadIdx.x == 0) publish(flag, epoch);
 }
@@ -603,7 +634,6 @@ __global__ __launch_bounds__(512) void k_trsv_lower_coop(const cf* __restrict__ 
                                                          int epoch, int* __restrict__ status) {
     __shared__ cd part[8][NBM];
     __shared__ cd acc[NB2];
-    __shared__ cd yb[8][32];  // per wave: the 32 entries of y it multiplies
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rh = w >> 2, q = w & 3;
     for (int blk = blockIdx.x; blk < nblk2; blk += gridDim.x) {
         const int r0 = blk * NB2, nb = min(NB2, n - r0);
@@ -621,12 +651,7 @@ __global__ __launch_bounds__(512) void k_trsv_lower_coop(const cf* __restrict__ 
 #pragma unroll
             for (int i = 0; i < 32; ++i) l[i] = Lc[(size_t)i * lda];
             wave_wait_ge(&flags[k], epoch, status);
-            // lanes 0..31 stage y in the wave's own LDS slot; the wave's LDS
-            // operations complete in order, so its reads below see them
-            if (lane < 32) yb[w][lane] = y[c0 + lane];
-            __builtin_amdgcn_wave_barrier();
-            s = s + dot32(l, yb[w]);
-            __builtin_amdgcn_wave_barrier();
+            s = s + dot32(l, y[c0 + (lane & 31)]);
         }
         part[w][lane] = s;
         __syncthreads();
@@ -649,7 +674,6 @@ __global__ __launch_bounds__(512) void k_trsv_upper_coop(const cf* __restrict__ 
                                                          int epoch, int* __restrict__ status) {
     __shared__ cd part[8][NBM];
     __shared__ cd acc[NB2];
-    __shared__ cd yb[8][32];  // per wave: the 32 entries of x it multiplies
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rh = w >> 2, q = w & 3;
     for (int bi = blockIdx.x; bi < nblk2; bi += gridDim.x) {
         const int blk = nblk2 - 1 - bi;
@@ -671,10 +695,7 @@ __global__ __launch_bounds__(512) void k_trsv_upper_coop(const cf* __restrict__ 
 #pragma unroll
                 for (int i = 0; i < 32; ++i) u[i] = Uc[(size_t)i * lda];
                 wave_wait_ge(&flags[k], epoch, status);
-                if (lane < 32) yb[w][lane] = x[c0 + lane];
-                __builtin_amdgcn_wave_barrier();
-                s = s + dot32(u, yb[w]);
-                __builtin_amdgcn_wave_barrier();
+                s = s + dot32(u, x[c0 + (lane & 31)]);
             } else {
                 wave_wait_ge(&flags[k], epoch, status);
                 if (cnt <= 0) continue;
@@ -720,184 +741,15 @@ __device__ inline bf8 neg_bf8(bf8 x) {
 // ---------------------------------------------------------------------------
 // Trailing update over K = 64 * nks: C -= sum over chunks q of L21_q U12_q,
 // chunk q's planes at base + q * sstride (L21 planes first, U12 planes
-// 4 * pstride after them).  Up to two output regions per launch.  Tile order
-// inside a region: panels of UPD_PANEL column tiles, row tiles outer and the
-// panel's column tiles inner, so the workgroups resident on one XCD at a
-// time share a few L21 and U12 tiles in that XCD's L2.  Operands of k-step
-// ks+1 are loaded while the MFMAs of k-step ks run.
+// 4 * pstride after them).  Up to two output regions per launch.
 // ---------------------------------------------------------------------------
 struct UpdRegion {
     int m, nc;   // rows and columns of the region
     int r0, c0;  // first row / column (absolute: the plane index)
     cf* C;       // A(r0, c0)
 };
-constexpr int UPD_PANEL = 8;
 
-__device__ __forceinline__ void upd_load(bf8 (&fa)[2][4], bf8 (&fb)[2][4],
-                                         const __bf16* const (&pa)[2],
-                                         const __bf16* const (&pb)[2], size_t off,
-                                         size_t pstride) {
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int pl = 0; pl < 4; ++pl) fa[x][pl] = *(const bf8*)(pa[x] + off + pl * pstride);
-#pragma unroll
-    for (int y = 0; y < 2; ++y)
-#pragma unroll
-        for (int pl = 0; pl < 4; ++pl) fb[y][pl] = *(const bf8*)(pb[y] + off + pl * pstride);
-}
-
-// acc -= A B for one 32-wide k-step (12 MFMA per complex 16 x 16 block)
-__device__ __forceinline__ void upd_mfma(f4 (&accR)[2][2], f4 (&accI)[2][2],
-                                         const bf8 (&fa)[2][4], const bf8 (&fb)[2][4]) {
-    bf8 narh[2], narl[2], naih[2], nail[2];
-#pragma unroll
-    for (int x = 0; x < 2; ++x) {
-        narh[x] = neg_bf8(fa[x][0]);
-        narl[x] = neg_bf8(fa[x][1]);
-        naih[x] = neg_bf8(fa[x][2]);
-        nail[x] = neg_bf8(fa[x][3]);
-    }
-#pragma unroll
-    for (int y = 0; y < 2; ++y) {
-        const bf8 brh = fb[y][0], brl = fb[y][1], bih = fb[y][2], bil = fb[y][3];
-#pragma unroll
-        for (int x = 0; x < 2; ++x) {
-            const bf8 aih = fa[x][2], ail = fa[x][3];
-            f4 r = accR[x][y], i = accI[x][y];
-            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narh[x], brh, r, 0, 0, 0);
-            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narh[x], brl, r, 0, 0, 0);
-            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narl[x], brh, r, 0, 0, 0);
-            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, bih, r, 0, 0, 0);
-            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aih, bil, r, 0, 0, 0);
-            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ail, bih, r, 0, 0, 0);
-            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narh[x], bih, i, 0, 0, 0);
-            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narh[x], bil, i, 0, 0, 0);
-            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(narl[x], bih, i, 0, 0, 0);
-            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih[x], brh, i, 0, 0, 0);
-            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih[x], brl, i, 0, 0, 0);
-            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nail[x], brh, i, 0, 0, 0);
-            accR[x][y] = r;
-            accI[x][y] = i;
-        }
-    }
-}
-
-__device__ __forceinline__ void upd_tile(const UpdRegion& R, int t, const __bf16* __restrict__ base,
-                                         size_t pstride, size_t sstride, int nks, size_t ldc) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int gx = (R.m + CT - 1) / CT, gy = (R.nc + CT - 1) / CT;
-    const int p = t / (gx * UPD_PANEL);
-    const int within = t - p * gx * UPD_PANEL;
-    const int pw = min(UPD_PANEL, gy - p * UPD_PANEL);
-    const int bm = (within / pw) * CT, bn = (p * UPD_PANEL + within % pw) * CT;
-    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-    const int li = lane & 15, lk = (lane >> 4) * 8;
-    const __bf16* pa[2];
-    const __bf16* pb[2];
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-        pa[x] = base + (size_t)(R.r0 + min(bm + wm + 16 * x + li, R.m - 1)) * CT + lk;
-#pragma unroll
-    for (int y = 0; y < 2; ++y)
-        pb[y] = base + 4 * pstride + (size_t)(R.c0 + min(bn + wn + 16 * y + li, R.nc - 1)) * CT + lk;
-    bf8 a0[2][4], b0[2][4], a1[2][4], b1[2][4];
-    upd_load(a0, b0, pa, pb, 0, pstride);
-    f4 accR[2][2], accI[2][2];
-    cf* __restrict__ C = R.C;
-    const bool fast = bm + CT <= R.m && bn + CT <= R.nc && (ldc & 3) == 0 &&
-                      ((uintptr_t)C & 31) == 0;
-    if (fast) {
-        float4 v0[2][2], v1[2][2];
-#pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-            for (int y = 0; y < 2; ++y) {
-                const cf* cp = C + (size_t)(bn + wn + 16 * y + li) * ldc + bm + wm + 16 * x +
-                               (lane >> 4) * 4;
-                v0[x][y] = *(const float4*)cp;
-                v1[x][y] = *(const float4*)(cp + 2);
-            }
-#pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-            for (int y = 0; y < 2; ++y) {
-                accR[x][y] = f4{v0[x][y].x, v0[x][y].z, v1[x][y].x, v1[x][y].z};
-                accI[x][y] = f4{v0[x][y].y, v0[x][y].w, v1[x][y].y, v1[x][y].w};
-            }
-    } else {
-#pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-            for (int y = 0; y < 2; ++y) {
-                const int col = bn + wn + 16 * y + li;
-                const int row0 = bm + wm + 16 * x + (lane >> 4) * 4;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    cf v = (row0 + r < R.m && col < R.nc) ? C[(size_t)col * ldc + row0 + r]
-                                                          : cf{0.f, 0.f};
-                    accR[x][y][r] = v.x;
-                    accI[x][y][r] = v.y;
-                }
-            }
-    }
-    for (int q = 0; q < nks; ++q) {
-        upd_load(a1, b1, pa, pb, (size_t)q * sstride + 32, pstride);
-        upd_mfma(accR, accI, a0, b0);
-        if (q + 1 < nks) upd_load(a0, b0, pa, pb, (size_t)(q + 1) * sstride, pstride);
-        upd_mfma(accR, accI, a1, b1);
-    }
-    if (fast) {
-#pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-            for (int y = 0; y < 2; ++y) {
-                cf* cp = C + (size_t)(bn + wn + 16 * y + li) * ldc + bm + wm + 16 * x +
-                         (lane >> 4) * 4;
-                const f4 r = accR[x][y], i = accI[x][y];
-                *(float4*)cp = float4{r[0], i[0], r[1], i[1]};
-                *(float4*)(cp + 2) = float4{r[2], i[2], r[3], i[3]};
-            }
-    } else {
-#pragma unroll
-        for (int x = 0; x < 2; ++x)
-#pragma unroll
-            for (int y = 0; y < 2; ++y) {
-                const int col = bn + wn + 16 * y + li;
-                const int row0 = bm + wm + 16 * x + (lane >> 4) * 4;
-                if (col >= R.nc) continue;
-                const f4 r = accR[x][y], i = accI[x][y];
-#pragma unroll
-                for (int qq = 0; qq < 4; ++qq)
-                    if (row0 + qq < R.m) C[(size_t)col * ldc + row0 + qq] = cf{r[qq], i[qq]};
-            }
-    }
-}
-
-__device__ inline int upd_tiles(const UpdRegion& R) {
-    return R.m > 0 && R.nc > 0 ? ((R.m + CT - 1) / CT) * ((R.nc + CT - 1) / CT) : 0;
-}
-
-// Grid: any multiple of NXCD; workgroup id runs on XCD id % NXCD and each
-// XCD takes a contiguous range of the tiles of both regions.
-__global__ __launch_bounds__(256) void k_cgemm_update(UpdRegion r0, UpdRegion r1,
-                                                      const __bf16* __restrict__ base,
-                                                      size_t pstride, size_t sstride, int nks,
-                                                      size_t ldc) {
-    const int t0 = upd_tiles(r0), tiles = t0 + upd_tiles(r1);
-    const int per = (tiles + NXCD - 1) / NXCD;
-    for (int id = blockIdx.x; id < per * NXCD; id += gridDim.x) {
-        const int t = (id % NXCD) * per + id / NXCD;
-        if (t < t0)
-            upd_tile(r0, t, base, pstride, sstride, nks, ldc);
-        else if (t < tiles)
-            upd_tile(r1, t - t0, base, pstride, sstride, nks, ldc);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// The same update with 128 x 128 workgroup tiles staged through LDS (used
-// for every update; k_cgemm_update above is its 64 x 64 register-only form).
+// k_cgemm_update2: 128 x 128 workgroup tiles staged through LDS.
 // 512 threads = 8 waves of 64 x 32 (4 x 2 MFMA blocks).  Per stage of 32 k
 // the workgroup copies the tile's L21 and U12 planes (2 x 4 planes x 128 x
 // 32 bf16 = 64 KB) into LDS once, so every operand byte is read from L2 once
@@ -906,8 +758,10 @@ __global__ __launch_bounds__(256) void k_cgemm_update(UpdRegion r0, UpdRegion r1
 // plane is 32 k = 64 bytes; its four 16-byte parts are stored XOR-swizzled
 // by (row / 4) % 4, so the 16 rows that a quarter-wave reads in one
 // ds_read_b128 hit distinct banks.  The products accumulate from zero and C
-// is loaded during the first stage and updated once at the end.
-// ---------------------------------------------------------------------------
+// is loaded during the first stage and updated once at the end.  Tile order
+// inside a region: panels of UPD2_PANEL column tiles, row tiles outer and the
+// panel's column tiles inner, so the workgroups resident on one XCD at a
+// time share a few L21 and U12 tiles in that XCD's L2.
 constexpr int BT = 128;            // workgroup tile
 constexpr int KS = 32;             // k per stage
 constexpr int PLN = BT * KS;       // bf16 per plane per operand per stage
@@ -1072,28 +926,12 @@ static int upd2_tiles_host(const UpdRegion& R) {
     return R.m > 0 && R.nc > 0 ? ((R.m + BT - 1) / BT) * ((R.nc + BT - 1) / BT) : 0;
 }
 
-static int upd_tiles_host(const UpdRegion& R) {
-    return R.m > 0 && R.nc > 0 ? ((R.m + CT - 1) / CT) * ((R.nc + CT - 1) / CT) : 0;
-}
 
-// Launch of k_cgemm_update for regions a and b (b may be empty); nothing
+// Launch of k_cgemm_update2 for regions a and b (b may be empty); nothing
 // when both are empty.
 static void launch_update(hipStream_t s, const UpdRegion& a, const UpdRegion& b,
                           const __bf16* base, size_t pstride, size_t sstride, int nks,
                           size_t ldc, int cap = 0) {
-    static const bool small = [] {
-        const char* v = getenv("BEMX_UPD_TILE");
-        return v && atoi(v) == 64;
-    }();
-    if (small) {
-        const int tiles = upd_tiles_host(a) + upd_tiles_host(b);
-        if (tiles == 0) return;
-        int g = ((tiles + NXCD - 1) / NXCD) * NXCD;
-        if (cap > 0) g = std::min(g, cap);
-        hipLaunchKernelGGL(k_cgemm_update, dim3(g), dim3(256), 0, s, a, b, base, pstride, sstride,
-                           nks, ldc);
-        return;
-    }
     const int tiles = upd2_tiles_host(a) + upd2_tiles_host(b);
     if (tiles == 0) return;
     int g = ((tiles + NXCD - 1) / NXCD) * NXCD;
@@ -1189,6 +1027,26 @@ __global__ void k_orth_update(int n, const cd* __restrict__ V, size_t ldv, int n
     out[row] = v * s;
 }
 
+// out[i] = sum over the nparts partial rows of entry i (i < nv1), in a fixed
+// order: the reduction of k_dots on the device, same result on every run.
+__global__ void k_dots_reduce(int nparts, int nv1, const cd* __restrict__ part,
+                              cd* __restrict__ out) {
+    const int i = threadIdx.x;
+    if (i >= nv1) return;
+    cd s = mk(0, 0);
+    for (int bk = 0; bk < nparts; ++bk) s = s + part[(size_t)bk * nv1 + i];
+    out[i] = s;
+}
+
+// out = w / sqrt(nrm2->x) (zero when the norm is zero)
+__global__ void k_scale_by_norm(int n, const cd* __restrict__ w, cd* __restrict__ out,
+                                const cd* __restrict__ nrm2) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    const double hn = sqrt(nrm2->x);
+    out[row] = hn > 0.0 ? w[row] * (1.0 / hn) : mk(0, 0);
+}
+
 // x += sum_i y_i V_i
 __global__ void k_combine(int n, const cd* __restrict__ V, size_t ldv, int nv,
                           const cd* __restrict__ y, cd* __restrict__ x) {
@@ -1237,12 +1095,14 @@ struct MixedFactor {
     size_t ld = 0;  // leading dimension of A
 };
 
-// Blocks per group of the factorisation (BEMX_GROUP, 1..8; default 4): the
-// trailing update outside a group runs once per group with K = 64 * group.
+// Blocks per group of the factorisation (BEMX_GROUP, 1..8; default 8): the
+// trailing update outside a group runs once per group with K = 64 * group
+// (measured on MI355X: 4 and 8 are equal at 6480 DOF, 8 is 11% faster at
+// 25920 DOF than 4).
 static int group_blocks() {
     static const int g = [] {
         const char* v = getenv("BEMX_GROUP");
-        const int x = v ? atoi(v) : 4;
+        const int x = v ? atoi(v) : 8;
         return std::min(8, std::max(1, x));
     }();
     return g;
@@ -1263,7 +1123,10 @@ size_t mixed_factor_extra_bytes(int n) {
 //   st:   getrf(k); L21(k) on all rows below and U12(k) on all columns to the
 //         right (one launch, bf16 planes into the group's slot of k); then
 //         the update of the group's own columns (rows below k) and of the
-//         group's own rows (columns beyond gend), K = 64.
+//         group's own rows (columns beyond gend), K = 64.  (A left-looking
+//         form, one K = 64 j update of block column and row k per step, was
+//         measured 0.7 ms slower per 6480-DOF solve: its 64-wide regions
+//         half-fill the 128-wide tiles.)
 // After the last step of group g, with K = W from the group's planes:
 //   st:   [wait Z(g-1)] X(g): the next group's columns, all rows below gend;
 //         Y(g): the next group's rows, the columns beyond it;
@@ -1298,8 +1161,7 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
         __bf16* gbase = planes + (size_t)(g & 1) * sg * sstride;
         __bf16* la = gbase + (size_t)j * sstride;
         __bf16* ub = la + 4 * pstride;
-        hipLaunchKernelGGL(k_diag_getrf_panels, dim3(1), dim3(256), 0, st, F.A, ld, k0, nb, lp,
-                           F.status);
+        launch_getrf(st, F.A, ld, k0, nb, lp, F.status);
         if (rest <= 0) break;
         const int nrb = (rest + TRR_ROWS - 1) / TRR_ROWS;
         const int ncb = (rest + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG;
@@ -1468,8 +1330,20 @@ static int refine_one(LuWork& w, hipStream_t st, const MixedFactor& F, int n, co
     cd* part = (cd*)w.get(M_PART, sizeof(cd) * std::max((size_t)nchunk * n,
                                                          (size_t)nb256 * (MR + 2)), &e);
     MX_CHK(e);
-    cd* hdev = (cd*)w.get(M_HOST, sizeof(cd) * (MR + 2), &e);
+    // device scratch of the Arnoldi step: projections of the two
+    // Gram-Schmidt passes and the norm (slots of MR + 2 entries each)
+    const int HS = MR + 2;
+    cd* hdev = (cd*)w.get(M_HOST, sizeof(cd) * HS * 3, &e);
     MX_CHK(e);
+    cd *h1 = hdev + HS, *h2 = hdev + 2 * HS;
+    std::vector<cd> hstep((size_t)HS * 3);
+    // k_dots over nv vectors + reduction into out (nv + 1 entries, the last
+    // one ||wvec||^2), all on the device
+    auto dots_dev = [&](const cd* Vb, int nv, const cd* wvec, cd* out) {
+        hipLaunchKernelGGL(k_dots, dim3(nb256), dim3(256), 0, st, n, Vb, (size_t)n, nv, wvec, part);
+        hipLaunchKernelGGL(k_dots_reduce, dim3(1), dim3(64), 0, st, nb256, nv + 1,
+                           (const cd*)part, out);
+    };
     std::vector<cd> hpart((size_t)nb256 * (MR + 2));
     auto dots = [&](const cd* Vb, int nv, const cd* wvec, std::vector<zx>& out, double& nrm2) -> int {
         hipLaunchKernelGGL(k_dots, dim3(nb256), dim3(256), 0, st, n, Vb, (size_t)n, nv, wvec, part);
@@ -1497,10 +1371,8 @@ static int refine_one(LuWork& w, hipStream_t st, const MixedFactor& F, int n, co
     bool converged = false;
     for (int outer = 0; outer < MAXOUT; ++outer) {
         gemv(st, n, A64, lda, x, bsave, r, part, 1);  // r = b - A x (FP64)
-        double rn2 = 0, xn2 = 0;
+        double rn2 = 0;
         dots(r, 0, r, h, rn2);
-        dots(x, 0, x, h, xn2);
-        (void)xn2;
         double prev = relres;
         relres = std::sqrt(rn2) / std::max(bnorm, 1e-300);
         // ||A x|| ~ ||b||, so ||r|| <= TOL ||b|| bounds the normwise backward
@@ -1529,29 +1401,25 @@ static int refine_one(LuWork& w, hipStream_t st, const MixedFactor& F, int n, co
             ++it_total;
             gemv(st, n, A64, lda, V + (size_t)k * n, nullptr, wv, part, 0);
             MX_CHK(precond(st, F, wv, wv, tmp));
-            // classical Gram-Schmidt, two passes
-            std::vector<zx> hcol(k + 1, zx(0, 0));
-            double wn2 = 0;
-            for (int pass = 0; pass < 2; ++pass) {
-                std::vector<zx> hh;
-                dots(V, k + 1, wv, hh, wn2);
-                std::vector<cd> hc(k + 1);
-                for (int i = 0; i <= k; ++i) {
-                    hcol[i] += hh[i];
-                    hc[i] = mk(hh[i].real(), hh[i].imag());
-                }
-                MX_CHK(hipMemcpyAsync(hdev, hc.data(), sizeof(cd) * (k + 1), hipMemcpyHostToDevice,
-                                      st));
-                hipLaunchKernelGGL(k_orth_update, dim3(nb256), dim3(256), 0, st, n, V, (size_t)n,
-                                   k + 1, hdev, wv, wv, 1.0);
-            }
-            dots(V, 0, wv, h, wn2);
-            double hn = std::sqrt(wn2);
-            for (int i = 0; i <= k; ++i) H[i][k] = hcol[i];
+            // classical Gram-Schmidt, two passes, and the normalisation, all
+            // on the device; one copy of the coefficients to the host
+            dots_dev(V, k + 1, wv, hdev);
+            hipLaunchKernelGGL(k_orth_update, dim3(nb256), dim3(256), 0, st, n, V, (size_t)n,
+                               k + 1, hdev, wv, wv, 1.0);
+            dots_dev(V, k + 1, wv, h1);
+            hipLaunchKernelGGL(k_orth_update, dim3(nb256), dim3(256), 0, st, n, V, (size_t)n,
+                               k + 1, h1, wv, wv, 1.0);
+            dots_dev(V, 0, wv, h2);
+            hipLaunchKernelGGL(k_scale_by_norm, dim3(nb256), dim3(256), 0, st, n, wv,
+                               V + (size_t)(k + 1) * n, h2);
+            MX_CHK(hipMemcpyAsync(hstep.data(), hdev, sizeof(cd) * HS * 3, hipMemcpyDeviceToHost,
+                                  st));
+            MX_CHK(hipStreamSynchronize(st));
+            const double hn = std::sqrt(hstep[2 * HS].x);
+            for (int i = 0; i <= k; ++i)
+                H[i][k] = (zx(0, 0) + zx(hstep[i].x, hstep[i].y)) +
+                          zx(hstep[HS + i].x, hstep[HS + i].y);
             H[k + 1][k] = hn;
-            if (hn > 0)
-                hipLaunchKernelGGL(k_orth_update, dim3(nb256), dim3(256), 0, st, n, V, (size_t)n, 0,
-                                   hdev, wv, V + (size_t)(k + 1) * n, 1.0 / hn);
             // Givens
             for (int i = 0; i < k; ++i) {
                 zx t = std::conj(cs[i]) * H[i][k] + std::conj(sn[i]) * H[i + 1][k];
