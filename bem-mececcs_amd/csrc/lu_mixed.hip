@@ -593,9 +593,11 @@ __device__ inline cd dot32(const cf (&f)[32], cd yl) {
     return (s0 + s1) + (s2 + s3);
 }
 
-__device__ inline void diag128_apply_publish(const cf mv[32], const cd* acc, cd (*part)[NBM],
-                                             int w, int lane, int nb, cd* out, int* flag,
-                                             int epoch) {
+// z = Minv acc for this wave's row (the rows of row half rh); the four
+// column quarters are summed through part.  Returns z in the waves of
+// quarter 0 (zero elsewhere); ends with a barrier, so part may be reused.
+__device__ inline cd diag128_apply(const cf mv[32], const cd* acc, cd (*part)[NBM], int w,
+                                   int lane) {
     const int rh = w >> 2, q = w & 3;
     cd s0 = mk(0, 0), s1 = mk(0, 0), s2 = mk(0, 0), s3 = mk(0, 0);
 #pragma unroll
@@ -607,17 +609,74 @@ __device__ inline void diag128_apply_publish(const cf mv[32], const cd* acc, cd 
     }
     part[w][lane] = (s0 + s1) + (s2 + s3);
     __syncthreads();
-    if (q == 0) {
-        const int row = rh * NBM + lane;
-        const cd v = (part[4 * rh][lane] + part[4 * rh + 1][lane]) +
-                     (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]);
-        if (row < nb) out[row] = v;
-    }
-    // the barrier completes these stores; the release store of the flag
-    // then makes them visible at device scope (one L2 write-back, not one
-    // per wave)
+    cd z = mk(0, 0);
+    if (q == 0)
+        z = (part[4 * rh][lane] + part[4 * rh + 1][lane]) +
+            (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]);
     __syncthreads();
-    if (threadIdx.x == 0) publish(flag, epoch);
+    return z;
+}
+
+// Coupling blocks for the block solves: lower, pair p >= 1,
+// Wl_p = Ml_p L_{p,p-1}; upper, pair p < npair - 1, Wu_p = Mu_p U_{p,p+1}
+// (Ml/Mu the 128 x 128 inverses from k_pair_inverses, L/U blocks of A as the
+// block solves read them; a partial pair's missing rows (lower) or columns
+// (upper) are zero).  With them a block's solution is
+//   y_p = Ml_p (b_p - sum_{k < p-1} L_{p,k} y_k) - Wl_p y_{p-1}
+// so only the last product waits for the previous block.  Grid (2, npair,
+// 4): z = 32-column quarter of W; 256 threads of 4 rows x 4 columns.
+__global__ __launch_bounds__(256) void k_pair_coupling(const cf* __restrict__ A, size_t lda,
+                                                       int n, int npair,
+                                                       const cf* __restrict__ Ml,
+                                                       const cf* __restrict__ Mu,
+                                                       cf* __restrict__ Wl, cf* __restrict__ Wu) {
+    __shared__ cf Ms[16][NB2];
+    __shared__ cf Bs[16][32];
+    const bool lower = blockIdx.x == 0;
+    const int p = blockIdx.y, cq = blockIdx.z;
+    if (lower ? p == 0 : p == npair - 1) return;
+    const int tid = threadIdx.x, tr = tid & 31, tc = tid >> 5;
+    const cf* M = (lower ? Ml : Mu) + (size_t)p * NB2 * NB2;
+    cf* W = (lower ? Wl : Wu) + (size_t)p * NB2 * NB2;
+    const int rb = p * NB2;                          // rows of B: pair p
+    const int cb = (lower ? p - 1 : p + 1) * NB2;    // columns of B
+    const int nbr = min(NB2, n - rb), nbc = min(NB2, n - cb);
+    cf acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = mkf(0.f, 0.f);
+    for (int k0 = 0; k0 < NB2; k0 += 16) {
+        for (int t = tid; t < 16 * NB2; t += 256) {
+            const int kk = t >> 7, r = t & (NB2 - 1);
+            Ms[kk][r] = M[(size_t)(k0 + kk) * NB2 + r];
+        }
+        for (int t = tid; t < 16 * 32; t += 256) {
+            const int kk = t & 15, c = t >> 4;
+            const int k = k0 + kk, col = cq * 32 + c;
+            Bs[kk][c] = (k < nbr && col < nbc) ? A[(size_t)(cb + col) * lda + rb + k]
+                                              : mkf(0.f, 0.f);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < 16; ++kk) {
+            cf m[4], bv[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) m[r] = Ms[kk][4 * tr + r];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) bv[c] = Bs[kk][4 * tc + c];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[r][c] = acc[r][c] + m[r] * bv[c];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            W[(size_t)(cq * 32 + 4 * tc + c) * NB2 + 4 * tr + r] = acc[r][c];
 }
 
 // y = L^-1 P b over 128-row blocks.  Workgroup g owns blocks g, g+G, ...;
@@ -629,6 +688,7 @@ __device__ inline void diag128_apply_publish(const cf mv[32], const cd* acc, cd 
 __global__ __launch_bounds__(512) void k_trsv_lower_coop(const cf* __restrict__ A, size_t lda,
                                                          int n, int nblk2,
                                                          const cf* __restrict__ Minv,
+                                                         const cf* __restrict__ Wc,
                                                          const cd* __restrict__ b,
                                                          cd* __restrict__ y, int* __restrict__ flags,
                                                          int epoch, int* __restrict__ status) {
@@ -644,7 +704,7 @@ __global__ __launch_bounds__(512) void k_trsv_lower_coop(const cf* __restrict__ 
         for (int i = 0; i < 32; ++i) mv[i] = Mb[(size_t)i * NB2];
         cd s = mk(0, 0);
         const int rowc = r0 + min(row, nb - 1);
-        for (int k = 0; k < blk; ++k) {
+        for (int k = 0; k < blk - 1; ++k) {
             const int c0 = k * NB2 + 32 * q;
             const cf* Lc = A + (size_t)c0 * lda + rowc;
             cf l[32];  // the factor tile does not depend on the flag: load it first
@@ -653,6 +713,12 @@ __global__ __launch_bounds__(512) void k_trsv_lower_coop(const cf* __restrict__ 
             wave_wait_ge(&flags[k], epoch, status);
             s = s + dot32(l, y[c0 + (lane & 31)]);
         }
+        cf wv[32];  // coupling to the previous block, loaded before its wait
+        if (blk > 0) {
+            const cf* Wb = Wc + (size_t)blk * NB2 * NB2 + (size_t)(32 * q) * NB2 + row;
+#pragma unroll
+            for (int i = 0; i < 32; ++i) wv[i] = Wb[(size_t)i * NB2];
+        }
         part[w][lane] = s;
         __syncthreads();
         if (q == 0)
@@ -660,8 +726,21 @@ __global__ __launch_bounds__(512) void k_trsv_lower_coop(const cf* __restrict__ 
                                                  (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]))
                                 : mk(0, 0);
         __syncthreads();
-        diag128_apply_publish(mv, acc, part, w, lane, nb, y + r0, &flags[blk], epoch);
+        cd z = diag128_apply(mv, acc, part, w, lane);
+        if (blk > 0) {
+            wave_wait_ge(&flags[blk - 1], epoch, status);
+            const cd t = dot32(wv, y[(blk - 1) * NB2 + 32 * q + (lane & 31)]);
+            part[w][lane] = t;
+            __syncthreads();
+            if (q == 0)
+                z = z - ((part[4 * rh][lane] + part[4 * rh + 1][lane]) +
+                         (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]));
+        }
+        if (q == 0 && row < nb) y[r0 + row] = z;
+        // the barrier completes these stores; the release store of the flag
+        // then makes them visible at device scope
         __syncthreads();
+        if (tid == 0) publish(&flags[blk], epoch);
     }
 }
 
@@ -669,6 +748,7 @@ __global__ __launch_bounds__(512) void k_trsv_lower_coop(const cf* __restrict__ 
 __global__ __launch_bounds__(512) void k_trsv_upper_coop(const cf* __restrict__ A, size_t lda,
                                                          int n, int nblk2,
                                                          const cf* __restrict__ Minv,
+                                                         const cf* __restrict__ Wc,
                                                          const cd* __restrict__ yv,
                                                          cd* __restrict__ x, int* __restrict__ flags,
                                                          int epoch, int* __restrict__ status) {
@@ -685,7 +765,7 @@ __global__ __launch_bounds__(512) void k_trsv_upper_coop(const cf* __restrict__ 
         for (int i = 0; i < 32; ++i) mv[i] = Mb[(size_t)i * NB2];
         cd s = mk(0, 0);
         const int rowc = r0 + min(row, nb - 1);
-        for (int k = nblk2 - 1; k > blk; --k) {
+        for (int k = nblk2 - 1; k > blk + 1; --k) {
             const int kb = min(NB2, n - k * NB2);
             const int c0 = k * NB2 + 32 * q;
             const int cnt = min(32, kb - 32 * q);  // columns of this wave in block k
@@ -703,6 +783,13 @@ __global__ __launch_bounds__(512) void k_trsv_upper_coop(const cf* __restrict__ 
                 for (int i = 0; i < cnt; ++i) s = s + up(Uc[(size_t)i * lda]) * readlane_c(yl, i);
             }
         }
+        const bool next = blk + 1 < nblk2;
+        cf wv[32];  // coupling to the next block, loaded before its wait
+        if (next) {
+            const cf* Wb = Wc + (size_t)blk * NB2 * NB2 + (size_t)(32 * q) * NB2 + row;
+#pragma unroll
+            for (int i = 0; i < 32; ++i) wv[i] = Wb[(size_t)i * NB2];
+        }
         part[w][lane] = s;
         __syncthreads();
         if (q == 0)
@@ -710,8 +797,21 @@ __global__ __launch_bounds__(512) void k_trsv_upper_coop(const cf* __restrict__ 
                                                   (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]))
                                 : mk(0, 0);
         __syncthreads();
-        diag128_apply_publish(mv, acc, part, w, lane, nb, x + r0, &flags[blk], epoch);
+        cd z = diag128_apply(mv, acc, part, w, lane);
+        if (next) {
+            wave_wait_ge(&flags[blk + 1], epoch, status);
+            // columns of a partial next block have zero coupling; their
+            // index is clamped into x
+            const cd t = dot32(wv, x[min((blk + 1) * NB2 + 32 * q + (lane & 31), n - 1)]);
+            part[w][lane] = t;
+            __syncthreads();
+            if (q == 0)
+                z = z - ((part[4 * rh][lane] + part[4 * rh + 1][lane]) +
+                         (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]));
+        }
+        if (q == 0 && row < nb) x[r0 + row] = z;
         __syncthreads();
+        if (tid == 0) publish(&flags[blk], epoch);
     }
 }
 
@@ -1081,7 +1181,7 @@ __global__ void k_fill_i(int* p, int v, int cnt) {
 
 // Workspace slots 9.. of LuWork (0..8 belong to the FP64 path).
 enum { M_A32 = 9, M_PERM, M_FLAGS, M_STATUS, M_VEC, M_V, M_PART, M_HOST, M_PTMP, M_PLANES, M_INV,
-       M_MINV };
+       M_MINV, M_WC };
 
 struct MixedFactor {
     cf* A = nullptr;
@@ -1091,6 +1191,7 @@ struct MixedFactor {
     int* epoch = nullptr;   // host-side counter living in LuWork
     cf* inv = nullptr;      // per block: L_bb^-1 P_bb, then (after nblk blocks) U_bb^-1
     cf* minv = nullptr;     // per 128-row pair: lower inverse, then (after npair) upper
+    cf* wc = nullptr;       // per pair: lower coupling Wl, then (after npair) upper Wu
     int n = 0, nblk = 0, npair = 0, coop = 1;
     size_t ld = 0;  // leading dimension of A
 };
@@ -1198,6 +1299,11 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
     // 128-row diagonal-block inverses of both factors for the block solves
     hipLaunchKernelGGL(k_pair_inverses, dim3(2, F.npair), dim3(256), 0, st, (const cf*)F.A, ld,
                        n, (const cf*)F.inv, nblk, F.minv, F.minv + (size_t)F.npair * NB2 * NB2);
+    // coupling blocks of consecutive pairs for the block solves
+    hipLaunchKernelGGL(k_pair_coupling, dim3(2, F.npair, 4), dim3(256), 0, st, (const cf*)F.A, ld,
+                       n, F.npair, (const cf*)F.minv,
+                       (const cf*)(F.minv + (size_t)F.npair * NB2 * NB2), F.wc,
+                       F.wc + (size_t)F.npair * NB2 * NB2);
     chk(hipGetLastError());
     return first;
 }
@@ -1217,6 +1323,8 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     MX_CHK(e);
     F.npair = (nblk + 1) / 2;
     F.minv = (cf*)w.get(M_MINV, sizeof(cf) * (size_t)F.npair * NB2 * NB2 * 2, &e);
+    MX_CHK(e);
+    F.wc = (cf*)w.get(M_WC, sizeof(cf) * (size_t)F.npair * NB2 * NB2 * 2, &e);
     MX_CHK(e);
     const size_t cap_before = w.cap[M_FLAGS];
     F.flags = (int*)w.get(M_FLAGS, sizeof(int) * (size_t)nblk, &e);
@@ -1287,16 +1395,20 @@ static hipError_t precond(hipStream_t st, const MixedFactor& F, const cd* v, cd*
     const cf* A = F.A;
     const cf* li = F.minv;
     const cf* ui = F.minv + (size_t)F.npair * NB2 * NB2;
+    const cf* wl = F.wc;
+    const cf* wu = F.wc + (size_t)F.npair * NB2 * NB2;
     int* flags = F.flags;
     int* status = F.status;
     int e1 = ++*F.epoch;
-    void* a1[] = {(void*)&A, &lda, &n, &nblk, (void*)&li, (void*)&v, &tmp, &flags, &e1, &status};
+    void* a1[] = {(void*)&A, &lda, &n, &nblk, (void*)&li, (void*)&wl, (void*)&v, &tmp, &flags, &e1,
+                  &status};
     hipError_t e = hipLaunchCooperativeKernel((const void*)k_trsv_lower_coop, dim3(F.coop),
                                               dim3(512), a1, 0, st);
     if (e != hipSuccess) return e;
     int e2 = ++*F.epoch;
     const cd* tv = tmp;
-    void* a2[] = {(void*)&A, &lda, &n, &nblk, (void*)&ui, (void*)&tv, &z, &flags, &e2, &status};
+    void* a2[] = {(void*)&A, &lda, &n, &nblk, (void*)&ui, (void*)&wu, (void*)&tv, &z, &flags, &e2,
+                  &status};
     return hipLaunchCooperativeKernel((const void*)k_trsv_upper_coop, dim3(F.coop), dim3(512), a2,
                                       0, st);
 }
