@@ -17,13 +17,13 @@
 // is applied to the group's own columns and rows only; the rest of the
 // matrix gets one update per group with K = 64 x group size, on a second
 // stream, while the next group is factored.
-// Preconditioner M^-1 v (FP64 vectors, FP32 factors): k_trsv_lower_coop and
-// k_trsv_upper_coop, one cooperative launch each; workgroups own 128-row
-// blocks (pairs of diagonal blocks), accumulate the contributions of
-// finished blocks as their flags come up and publish their own block, so the
-// dependency chain costs one small GEMV and one product with the block's
-// explicit inverse (k_diag_inverses, then k_pair_inverses, after the
-// factorisation) per 128 rows instead of one launch.
+// Preconditioner M^-1 v (FP64 vectors, FP32 factors): k_trsv_coop, one
+// cooperative launch for the lower and then the upper block solve;
+// workgroups own 128-row blocks (pairs of diagonal blocks), accumulate the
+// contributions of finished blocks as their flags come up and publish their
+// own block, so the dependency chain costs one product with a precomputed
+// coupling block (k_pair_coupling) per 128 rows; the explicit inverses come
+// from k_diag_inverses and k_pair_inverses after the factorisation.
 // Pivoting is restricted to the diagonal block: the factors only
 // precondition.  The accuracy of the answer comes from the refinement, whose
 // residual b - A64 x is formed in FP64 with the FP64 matrix; if it stalls the
@@ -685,15 +685,13 @@ __global__ __launch_bounds__(256) void k_pair_coupling(const cf* __restrict__ A,
 // tile loaded before the wait), then y_blk = Minv acc.  Blocks only wait on
 // lower-numbered blocks and every workgroup is resident (cooperative
 // launch), so the chain always drains.
-__global__ __launch_bounds__(512) void k_trsv_lower_coop(const cf* __restrict__ A, size_t lda,
-                                                         int n, int nblk2,
-                                                         const cf* __restrict__ Minv,
-                                                         const cf* __restrict__ Wc,
-                                                         const cd* __restrict__ b,
-                                                         cd* __restrict__ y, int* __restrict__ flags,
-                                                         int epoch, int* __restrict__ status) {
-    __shared__ cd part[8][NBM];
-    __shared__ cd acc[NB2];
+__device__ __forceinline__ void trsv_lower_body(const cf* __restrict__ A, size_t lda, int n,
+                                                int nblk2, const cf* __restrict__ Minv,
+                                                const cf* __restrict__ Wc,
+                                                const cd* __restrict__ b, cd* __restrict__ y,
+                                                int* __restrict__ flags, int epoch,
+                                                int* __restrict__ status, cd (*part)[NBM],
+                                                cd* acc) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rh = w >> 2, q = w & 3;
     for (int blk = blockIdx.x; blk < nblk2; blk += gridDim.x) {
         const int r0 = blk * NB2, nb = min(NB2, n - r0);
@@ -745,15 +743,16 @@ __global__ __launch_bounds__(512) void k_trsv_lower_coop(const cf* __restrict__ 
 }
 
 // x = U^-1 y over 128-row blocks (in reverse order), same scheme.
-__global__ __launch_bounds__(512) void k_trsv_upper_coop(const cf* __restrict__ A, size_t lda,
-                                                         int n, int nblk2,
-                                                         const cf* __restrict__ Minv,
-                                                         const cf* __restrict__ Wc,
-                                                         const cd* __restrict__ yv,
-                                                         cd* __restrict__ x, int* __restrict__ flags,
-                                                         int epoch, int* __restrict__ status) {
-    __shared__ cd part[8][NBM];
-    __shared__ cd acc[NB2];
+// ylow: epoch of the lower solve that produced yv; block blk's entries of
+// yv are read after its flag reached ylow (x may alias the lower solve's
+// right-hand side: block blk of it was read before that flag was set).
+__device__ __forceinline__ void trsv_upper_body(const cf* __restrict__ A, size_t lda, int n,
+                                                int nblk2, const cf* __restrict__ Minv,
+                                                const cf* __restrict__ Wc,
+                                                const cd* __restrict__ yv, cd* __restrict__ x,
+                                                int* __restrict__ flags, int epoch, int ylow,
+                                                int* __restrict__ status, cd (*part)[NBM],
+                                                cd* acc) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rh = w >> 2, q = w & 3;
     for (int bi = blockIdx.x; bi < nblk2; bi += gridDim.x) {
         const int blk = nblk2 - 1 - bi;
@@ -790,6 +789,7 @@ __global__ __launch_bounds__(512) void k_trsv_upper_coop(const cf* __restrict__ 
 #pragma unroll
             for (int i = 0; i < 32; ++i) wv[i] = Wb[(size_t)i * NB2];
         }
+        wave_wait_ge(&flags[blk], ylow, status);
         part[w][lane] = s;
         __syncthreads();
         if (q == 0)
@@ -813,6 +813,23 @@ __global__ __launch_bounds__(512) void k_trsv_upper_coop(const cf* __restrict__ 
         __syncthreads();
         if (tid == 0) publish(&flags[blk], epoch);
     }
+}
+
+// z = U^-1 L^-1 P v: the lower block solve, then the upper one, in one
+// cooperative launch (every workgroup resident; each waits only on blocks
+// that other resident workgroups finish).  tmp receives y = L^-1 P v.
+__global__ __launch_bounds__(512) void k_trsv_coop(const cf* __restrict__ A, size_t lda, int n,
+                                                   int nblk2, const cf* __restrict__ Ml,
+                                                   const cf* __restrict__ Wl,
+                                                   const cf* __restrict__ Mu,
+                                                   const cf* __restrict__ Wu,
+                                                   const cd* __restrict__ v, cd* __restrict__ tmp,
+                                                   cd* __restrict__ z, int* __restrict__ flags,
+                                                   int e1, int e2, int* __restrict__ status) {
+    __shared__ cd part[8][NBM];
+    __shared__ cd acc[NB2];
+    trsv_lower_body(A, lda, n, nblk2, Ml, Wl, v, tmp, flags, e1, status, part, acc);
+    trsv_upper_body(A, lda, n, nblk2, Mu, Wu, tmp, z, flags, e2, e1, status, part, acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1340,10 +1357,8 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     int dev = 0, ncu = 0, per = 0;
     MX_CHK(hipGetDevice(&dev));
     MX_CHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trsv_lower_coop, 512, 0));
-    int per2 = 0;
-    MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, k_trsv_upper_coop, 512, 0));
-    F.coop = std::max(1, std::min(F.npair, ncu * std::min(per, per2)));
+    MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trsv_coop, 512, 0));
+    F.coop = std::max(1, std::min(F.npair, ncu * per));
     MX_CHK(w.ensure_side());
     // bf16 planes of L21 and U12: one slot (8 planes) per block of a group,
     // double-buffered by group parity (group g+1 writes one set while the bulk
@@ -1400,17 +1415,10 @@ static hipError_t precond(hipStream_t st, const MixedFactor& F, const cd* v, cd*
     int* flags = F.flags;
     int* status = F.status;
     int e1 = ++*F.epoch;
-    void* a1[] = {(void*)&A, &lda, &n, &nblk, (void*)&li, (void*)&wl, (void*)&v, &tmp, &flags, &e1,
-                  &status};
-    hipError_t e = hipLaunchCooperativeKernel((const void*)k_trsv_lower_coop, dim3(F.coop),
-                                              dim3(512), a1, 0, st);
-    if (e != hipSuccess) return e;
     int e2 = ++*F.epoch;
-    const cd* tv = tmp;
-    void* a2[] = {(void*)&A, &lda, &n, &nblk, (void*)&ui, (void*)&wu, (void*)&tv, &z, &flags, &e2,
-                  &status};
-    return hipLaunchCooperativeKernel((const void*)k_trsv_upper_coop, dim3(F.coop), dim3(512), a2,
-                                      0, st);
+    void* a[] = {(void*)&A, &lda, &n, &nblk, (void*)&li, (void*)&wl, (void*)&ui, (void*)&wu,
+                 (void*)&v, &tmp, &z, &flags, &e1, &e2, &status};
+    return hipLaunchCooperativeKernel((const void*)k_trsv_coop, dim3(F.coop), dim3(512), a, 0, st);
 }
 
 static void gemv(hipStream_t st, int n, const cd* A, size_t lda, const cd* x, const cd* b, cd* r,

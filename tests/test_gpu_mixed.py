@@ -34,7 +34,10 @@ def _rand_system(n, seed, diag=4.0):
     return np.asfortranarray(a), b
 
 
-@pytest.mark.parametrize("n", [5, 64, 65, 130, 257, 600])
+# 1100 = two full groups of eight 64-column blocks and a partial one: the
+# update of everything beyond the next group (the bulk, on the second
+# stream) is non-empty only from n > 1024.
+@pytest.mark.parametrize("n", [5, 64, 65, 130, 257, 600, 1100])
 def test_mixed_random(ctx, n):
     print("n", n, flush=True)
     a, b = _rand_system(n, 100 + n)
