@@ -1052,20 +1052,26 @@ static void launch_update(hipStream_t s, const UpdRegion& a, const UpdRegion& b,
     const int tiles = upd2_tiles_host(a) + upd2_tiles_host(b);
     if (tiles == 0) return;
     int g = ((tiles + NXCD - 1) / NXCD) * NXCD;
-    if (cap > 0) g = std::min(g, cap);
+    // a cap only for updates of up to 16 tiles per capped workgroup: a
+    // larger one is the bottleneck itself and gets the whole device
+    if (cap > 0 && tiles <= 16 * cap) g = std::min(g, cap);
     hipLaunchKernelGGL(k_cgemm_update2, dim3(g), dim3(512), 0, s, a, b, base, pstride, sstride,
                        nks, ldc);
 }
 
 // Workgroups of the bulk update on the side stream (BEMX_BULK_WGS; default
-// 0 = one per tile).  A cap keeps the bulk update from holding every CU, so
-// the panel kernels of the main stream find room sooner; measured on the
-// 6480-DOF solve, caps of 256..512 cost more in the bulk update than they
-// gained on the panel chain.
+// 128, 0 = one per tile).  The bulk update then loops over its tiles on
+// half the CUs and leaves the rest to the critical chain of the next group
+// (getrf, triangular solves, in-group updates), which otherwise queues
+// behind thousands of bulk workgroups.  Updates of more than 16 tiles per
+// capped workgroup (the early bulk updates of large systems, which bound
+// the factorisation) are not capped.  Measured on the 6480-DOF solve:
+// 16.9 ms with one workgroup per tile, 16.1 ms with 128 (96: 16.4, 192:
+// 16.6, 64: 17.3), and 73.8 vs 70.1 solves/s through two request slots.
 static int bulk_wg_cap() {
     static const int cap = [] {
         const char* v = getenv("BEMX_BULK_WGS");
-        const int c = v ? atoi(v) : 0;
+        const int c = v ? atoi(v) : 128;
         return c > 0 ? ((c + NXCD - 1) / NXCD) * NXCD : 0;
     }();
     return cap;
