@@ -68,8 +68,8 @@ constexpr int TRSM_COLS_PER_WG = 64;
 // threads loads L11 and its 64 columns (gathered by the block permutation
 // lperm, row r <- row lperm[r], with PERM) with all 32 loads of a thread in
 // flight, then solves one lane per column, by chunks of 16 rows: wave 0
-// solves chunk g in registers, then the four waves update the later chunks
-// with it in parallel.  L11 reads are wave-uniform LDS broadcasts, the
+// solves chunk g in registers, then each of the four waves updates a
+// quarter of every later chunk with it.  L11 reads are wave-uniform LDS broadcasts, the
 // column reads conflict-free.
 // Ls (TB x TB) and X (TB x (TB + 1)) are LDS buffers of the calling kernel.
 template <class T, bool PERM>
@@ -119,19 +119,25 @@ __device__ __forceinline__ void trsm_lower_cols_body(int bid, T* __restrict__ A,
             for (int r = 0; r < 16; ++r) X[(g + r) * XS + lane] = a[r];
         }
         __syncthreads();
-        const int g2 = g + 16 * (1 + w);  // the chunk this wave updates
-        if (g2 < TB) {
-            T a[16];
+        // every wave updates rows 4w..4w+3 of each later chunk, so the next
+        // chunk is complete after a quarter of one chunk's update
+        if (g + 16 < TB) {
+            T xg[16];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) a[r] = X[(g2 + r) * XS + lane];
-#pragma unroll 4
-            for (int i = 0; i < 16; ++i) {
-                const T xi = X[(g + i) * XS + lane];
+            for (int i = 0; i < 16; ++i) xg[i] = X[(g + i) * XS + lane];
 #pragma unroll
-                for (int r = 0; r < 16; ++r) a[r] = cfms(a[r], Ls[(g + i) * TB + g2 + r], xi);
+            for (int g2 = g + 16; g2 < TB; g2 += 16) {
+                const int rw = g2 + 4 * w;
+                T a[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) a[r] = X[(rw + r) * XS + lane];
+#pragma unroll
+                for (int i = 0; i < 16; ++i)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) a[r] = cfms(a[r], Ls[(g + i) * TB + rw + r], xg[i]);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) X[(rw + r) * XS + lane] = a[r];
             }
-#pragma unroll
-            for (int r = 0; r < 16; ++r) X[(g2 + r) * XS + lane] = a[r];
         }
         __syncthreads();
     }

@@ -205,8 +205,8 @@ static void launch_getrf(hipStream_t st, cf* A, size_t ld, int k0, int nb, int* 
 // L21 = A21 U11^-1 for rows [r0, r0+nr).  256 threads load U11 and 64
 // rows with all 32 loads of a thread in flight; the rows are then solved one
 // lane per row, left to right by chunks of 16 columns: wave 0 solves chunk g
-// in registers, then the four waves update the later chunks with it in
-// parallel.  U11 reads are wave-uniform broadcasts.  The result goes back to
+// in registers, then each of the four waves updates a quarter of every
+// later chunk with it.  U11 reads are wave-uniform broadcasts.  The result goes back to
 // A and, split, to the planes la.
 constexpr int TRR_ROWS = 64;
 __device__ __forceinline__ void trsm_upper_rows_body(int bid, cf* __restrict__ A, size_t lda,
@@ -259,19 +259,25 @@ __device__ __forceinline__ void trsm_upper_rows_body(int bid, cf* __restrict__ A
             for (int c = 0; c < 16; ++c) X[(g + c) * XS + lane] = a[c];
         }
         __syncthreads();
-        const int g2 = g + 16 * (1 + w);  // the chunk this wave updates
-        if (g2 < NBM) {
-            cf a[16];
+        // every wave updates columns 4w..4w+3 of each later chunk, so the
+        // next chunk is complete after a quarter of one chunk's update
+        if (g + 16 < NBM) {
+            cf xg[16];
 #pragma unroll
-            for (int c = 0; c < 16; ++c) a[c] = X[(g2 + c) * XS + lane];
-#pragma unroll 4
-            for (int i = 0; i < 16; ++i) {
-                const cf xi = X[(g + i) * XS + lane];
+            for (int i = 0; i < 16; ++i) xg[i] = X[(g + i) * XS + lane];
 #pragma unroll
-                for (int c = 0; c < 16; ++c) a[c] = cfms(a[c], xi, Us[(g2 + c) * NBM + g + i]);
+            for (int g2 = g + 16; g2 < NBM; g2 += 16) {
+                const int cw = g2 + 4 * w;
+                cf a[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) a[c] = X[(cw + c) * XS + lane];
+#pragma unroll
+                for (int i = 0; i < 16; ++i)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) a[c] = cfms(a[c], xg[i], Us[(cw + c) * NBM + g + i]);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) X[(cw + c) * XS + lane] = a[c];
             }
-#pragma unroll
-            for (int c = 0; c < 16; ++c) X[(g2 + c) * XS + lane] = a[c];
         }
         __syncthreads();
     }
@@ -873,28 +879,34 @@ struct UpdRegion {
 // per workgroup instead of once per wave; the next stage's planes are loaded
 // into registers while the MFMAs of the current one run.  In LDS a row of a
 // plane is 32 k = 64 bytes; its four 16-byte parts are stored XOR-swizzled
-// by (row / 4) % 4, so the 16 rows that a quarter-wave reads in one
-// ds_read_b128 hit distinct banks.  The products accumulate from zero and C
+// (swz) so the rows one ds_read_b128 reads together hit distinct banks.  The products accumulate from zero and C
 // is loaded during the first stage and updated once at the end.  Tile order
-// inside a region: panels of UPD2_PANEL column tiles, row tiles outer and the
+// inside a region: panels of `panel` column tiles, row tiles outer and the
 // panel's column tiles inner, so the workgroups resident on one XCD at a
 // time share a few L21 and U12 tiles in that XCD's L2.
 constexpr int BT = 128;            // workgroup tile
 constexpr int KS = 32;             // k per stage
 constexpr int PLN = BT * KS;       // bf16 per plane per operand per stage
-constexpr int UPD2_PANEL = 4;      // column tiles per panel of the tile order
 
-__device__ __forceinline__ int swz(int r, int part) { return r * KS + ((part ^ ((r >> 2) & 3)) << 3); }
+// 16-byte part `part` of row r of a plane in LDS, XOR-swizzled by
+// ((r >> 1) ^ (r >> 2)) & 3: the rows that one ds_read_b128 serves together
+// (8 consecutive rows per 128 bytes, or 16 per 256 bytes) land in distinct
+// bank groups under either width.
+__device__ __forceinline__ int swz(int r, int part) {
+    return r * KS + ((part ^ (((r >> 1) ^ (r >> 2)) & 3)) << 3);
+}
 
+template <int NBUF>
 __device__ __forceinline__ void upd2_tile(const UpdRegion& R, int t, const __bf16* __restrict__ base,
                                           size_t pstride, size_t sstride, int nks, size_t ldc,
-                                          __bf16* __restrict__ la, __bf16* __restrict__ lb) {
+                                          __bf16* __restrict__ la, __bf16* __restrict__ lb,
+                                          int panel) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int gx = (R.m + BT - 1) / BT, gy = (R.nc + BT - 1) / BT;
-    const int p = t / (gx * UPD2_PANEL);
-    const int within = t - p * gx * UPD2_PANEL;
-    const int pw = min(UPD2_PANEL, gy - p * UPD2_PANEL);
-    const int bm = (within / pw) * BT, bn = (p * UPD2_PANEL + within % pw) * BT;
+    const int p = t / (gx * panel);
+    const int within = t - p * gx * panel;
+    const int pw = min(panel, gy - p * panel);
+    const int bm = (within / pw) * BT, bn = (p * panel + within % pw) * BT;
     // copy assignment: thread -> (row, 16-byte part) of every plane
     const int crow = tid >> 2, cpart = tid & 3;
     const __bf16* ga = base + (size_t)(R.r0 + min(bm + crow, R.m - 1)) * CT + cpart * 8;
@@ -903,6 +915,7 @@ __device__ __forceinline__ void upd2_tile(const UpdRegion& R, int t, const __bf1
     // compute assignment: wave tile rows 64 wr.., columns 32 wc..
     const int wr = w & 1, wc = w >> 1;
     const int li = lane & 15, lq = lane >> 4;
+    // planes of the next stage, loaded into registers while a stage runs
     u4 ra[4], rb[4];
     auto gload = [&](int s) {
         const size_t off = (size_t)(s >> 1) * sstride + (s & 1) * KS;
@@ -912,13 +925,21 @@ __device__ __forceinline__ void upd2_tile(const UpdRegion& R, int t, const __bf1
             rb[pl] = *(const u4*)(gb + off + pl * pstride);
         }
     };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int pl = 0; pl < 4; ++pl) {
+            *(u4*)(la + buf * 4 * PLN + pl * PLN + coff) = ra[pl];
+            *(u4*)(lb + buf * 4 * PLN + pl * PLN + coff) = rb[pl];
+        }
+    };
+    const int nst = 2 * nks;
     gload(0);
     // C of this wave's 64 x 32 block, in flight during the first stage
     cf* __restrict__ C = R.C;
     const bool fast = bm + BT <= R.m && bn + BT <= R.nc && (ldc & 3) == 0 &&
                       ((uintptr_t)C & 31) == 0;
     float4 c0[4][2], c1[4][2];
-    if (fast) {
+    auto cload = [&]() {
 #pragma unroll
         for (int x = 0; x < 4; ++x)
 #pragma unroll
@@ -928,7 +949,8 @@ __device__ __forceinline__ void upd2_tile(const UpdRegion& R, int t, const __bf1
                 c0[x][y] = *(const float4*)cp;
                 c1[x][y] = *(const float4*)(cp + 2);
             }
-    }
+    };
+    if (fast) cload();
     f4 accR[4][2], accI[4][2];
 #pragma unroll
     for (int x = 0; x < 4; ++x)
@@ -937,28 +959,21 @@ __device__ __forceinline__ void upd2_tile(const UpdRegion& R, int t, const __bf1
             accR[x][y] = f4{0.f, 0.f, 0.f, 0.f};
             accI[x][y] = f4{0.f, 0.f, 0.f, 0.f};
         }
-    const int nst = 2 * nks;
-    for (int s = 0; s < nst; ++s) {
-        __syncthreads();  // every wave is done with the previous stage
-#pragma unroll
-        for (int pl = 0; pl < 4; ++pl) {
-            *(u4*)(la + pl * PLN + coff) = ra[pl];
-            *(u4*)(lb + pl * PLN + coff) = rb[pl];
-        }
-        __syncthreads();
-        if (s + 1 < nst) gload(s + 1);
+    auto compute = [&](int buf) {
+        const __bf16* __restrict__ xa = la + buf * 4 * PLN;
+        const __bf16* __restrict__ xb = lb + buf * 4 * PLN;
         bf8 fb[2][4];
 #pragma unroll
         for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int pl = 0; pl < 4; ++pl)
-                fb[y][pl] = *(const bf8*)(lb + pl * PLN + swz(32 * wc + 16 * y + li, lq));
+                fb[y][pl] = *(const bf8*)(xb + pl * PLN + swz(32 * wc + 16 * y + li, lq));
 #pragma unroll
         for (int x = 0; x < 4; ++x) {
             bf8 fa[4];
 #pragma unroll
             for (int pl = 0; pl < 4; ++pl)
-                fa[pl] = *(const bf8*)(la + pl * PLN + swz(64 * wr + 16 * x + li, lq));
+                fa[pl] = *(const bf8*)(xa + pl * PLN + swz(64 * wr + 16 * x + li, lq));
             const bf8 naih = neg_bf8(fa[2]), nail = neg_bf8(fa[3]);
 #pragma unroll
             for (int y = 0; y < 2; ++y) {
@@ -980,6 +995,31 @@ __device__ __forceinline__ void upd2_tile(const UpdRegion& R, int t, const __bf1
                 accR[x][y] = r;
                 accI[x][y] = i;
             }
+        }
+    };
+    if (NBUF == 1) {
+        // one LDS buffer: two barriers per stage
+        for (int s = 0; s < nst; ++s) {
+            __syncthreads();  // every wave is done with the previous stage
+            lstore(0);
+            __syncthreads();
+            if (s + 1 < nst) gload(s + 1);
+            compute(0);
+        }
+    } else {
+        // two LDS buffers: stage s+1 is stored while other waves still
+        // compute stage s; one barrier per stage
+        __syncthreads();  // the previous tile is done with both buffers
+        lstore(0);
+        if (nst > 1) gload(1);
+        __syncthreads();
+        for (int s = 0; s < nst; ++s) {
+            compute(s & 1);
+            if (s + 1 < nst) {
+                lstore((s + 1) & 1);
+                if (s + 2 < nst) gload(s + 2);
+            }
+            __syncthreads();
         }
     }
     // C -= L21 U12
@@ -1022,20 +1062,21 @@ __device__ inline int upd2_tiles(const UpdRegion& R) {
 
 // Grid: any multiple of NXCD (workgroup id runs on XCD id % NXCD; each XCD
 // takes a contiguous range of the tiles of both regions).
+template <int NBUF>
 __global__ __launch_bounds__(512) void k_cgemm_update2(UpdRegion r0, UpdRegion r1,
                                                        const __bf16* __restrict__ base,
                                                        size_t pstride, size_t sstride, int nks,
-                                                       size_t ldc) {
-    __shared__ __attribute__((aligned(16))) __bf16 lds_a[4 * PLN];
-    __shared__ __attribute__((aligned(16))) __bf16 lds_b[4 * PLN];
+                                                       size_t ldc, int panel) {
+    __shared__ __attribute__((aligned(16))) __bf16 lds_a[NBUF * 4 * PLN];
+    __shared__ __attribute__((aligned(16))) __bf16 lds_b[NBUF * 4 * PLN];
     const int t0 = upd2_tiles(r0), tiles = t0 + upd2_tiles(r1);
     const int per = (tiles + NXCD - 1) / NXCD;
     for (int id = blockIdx.x; id < per * NXCD; id += gridDim.x) {
         const int t = (id % NXCD) * per + id / NXCD;
         if (t < t0)
-            upd2_tile(r0, t, base, pstride, sstride, nks, ldc, lds_a, lds_b);
+            upd2_tile<NBUF>(r0, t, base, pstride, sstride, nks, ldc, lds_a, lds_b, panel);
         else if (t < tiles)
-            upd2_tile(r1, t - t0, base, pstride, sstride, nks, ldc, lds_a, lds_b);
+            upd2_tile<NBUF>(r1, t - t0, base, pstride, sstride, nks, ldc, lds_a, lds_b, panel);
     }
 }
 
@@ -1055,8 +1096,18 @@ static void launch_update(hipStream_t s, const UpdRegion& a, const UpdRegion& b,
     // a cap only for updates of up to 16 tiles per capped workgroup: a
     // larger one is the bottleneck itself and gets the whole device
     if (cap > 0 && tiles <= 16 * cap) g = std::min(g, cap);
-    hipLaunchKernelGGL(k_cgemm_update2, dim3(g), dim3(512), 0, s, a, b, base, pstride, sstride,
-                       nks, ldc);
+    // 4 column tiles per panel of the tile order (2, 8 and 16 measured the
+    // same on the 25920-DOF solve)
+    static const int nbuf = [] {
+        const char* v = getenv("BEMX_UPD_LDSBUF");
+        return (v && atoi(v) == 1) ? 1 : 2;
+    }();
+    if (nbuf == 2)
+        hipLaunchKernelGGL(k_cgemm_update2<2>, dim3(g), dim3(512), 0, s, a, b, base, pstride,
+                           sstride, nks, ldc, 4);
+    else
+        hipLaunchKernelGGL(k_cgemm_update2<1>, dim3(g), dim3(512), 0, s, a, b, base, pstride,
+                           sstride, nks, ldc, 4);
 }
 
 // Workgroups of the bulk update on the side stream (BEMX_BULK_WGS; default
