@@ -2,28 +2,32 @@
 // runs as split-BF16x3 products on MFMA, used as the preconditioner of an
 // FP64 GMRES iterative refinement (GMRES-IR) against the FP64 matrix.
 //
-// Factorisation (blocks of NBM = 64 columns, A32 = (complex float) A64):
-//   k_diag_getrf_panels  partial pivoting inside the 64 x 64 diagonal block
-//                      (registers: lane = row, wave = a panel of 16 columns)
-//   k_trsm_both        L21 = A21 U11^-1 (one lane per row) and
-//                      U12 = L11^-1 P A12 (one lane per column), one launch
+// Factorisation (blocks of NBM = 64 columns, A32 = (complex float) A64), in
+// the block form A = L U with unit block-lower L and block-upper U whose
+// diagonal blocks are kept as explicit inverses:
+//   k_diag_gj / gj_inverse_body  inverse of the 64 x 64 diagonal block by
+//                      Gauss-Jordan with partial pivoting inside the block
+//                      (registers: lane = row, wave = a panel of columns)
+//   k_panel_inv        L21 = A21 Xinv (MFMA) for all rows below and the
+//                      split planes of U12 = A12 for all columns right
 //   k_cgemm_update2    A22 -= L21 U12 over K = 64 .. 512; every FP32 operand x
-//                      is split into hi = bf16(x), lo = bf16(x - hi) (by the
-//                      triangular kernels, into bf16 planes) and each real
+//                      is split into hi = bf16(x), lo = bf16(x - hi) (by
+//                      k_panel_inv, into bf16 planes) and each real
 //                      product is hi*hi + hi*lo + lo*hi on
 //                      v_mfma_f32_16x16x32_bf16 with FP32 accumulation;
 //                      complex = 4 real products.
 // Blocks are factored in groups (enqueue_factor): inside a group the update
-// is applied to the group's own columns and rows only; the rest of the
-// matrix gets one update per group with K = 64 x group size, on a second
-// stream, while the next group is factored.
+// is applied to the group's own columns and rows only, in the same launch as
+// the inversion of the next diagonal block (k_cgemm_update_inverse); the rest
+// of the matrix gets one update per group with K = 64 x group size, on a
+// second stream, while the next group is factored.
 // Preconditioner M^-1 v (FP64 vectors, FP32 factors): k_trsv_coop, one
 // cooperative launch for the lower and then the upper block solve;
 // workgroups own 128-row blocks (pairs of diagonal blocks), accumulate the
 // contributions of finished blocks as their flags come up and publish their
 // own block, so the dependency chain costs one product with a precomputed
-// coupling block (k_pair_coupling) per 128 rows; the explicit inverses come
-// from k_diag_inverses and k_pair_inverses after the factorisation.
+// coupling block (k_pair_coupling) per 128 rows; the 128-row inverses come
+// from k_pair_inverses after the factorisation.
 // Pivoting is restricted to the diagonal block: the factors only
 // precondition.  The accuracy of the answer comes from the refinement, whose
 // residual b - A64 x is formed in FP64 with the FP64 matrix; if it stalls the
@@ -75,244 +79,145 @@ __device__ inline cf readlane_f(cf v, int src) {
                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), src)));
 }
 
-// Diagonal block: GEPP within its nb rows (largest |re|+|im| to 17 mantissa
-// bits, lowest block row on ties), held in registers, factored by panels of
-// PW columns: wave q owns columns [PW q, PW q + PW), lane r owns block row r.
-// Rows are not moved while factoring: a row is marked used when it becomes
-// the pivot of step j (its rank), and the factors are written back in rank
-// order; lperm[i] = block row whose content ends at block row i.
-// Wave q factors its panel
-// alone (PW pivot steps, no barrier: search, multipliers and the update of
-// its remaining columns all stay in the wave), publishes the multipliers and
-// pivots in LDS, and after one barrier the waves to its right replay those
-// PW steps on their own columns while wave q+1 factors the next panel.  The
-// chain is 64/PW panel factorisations and 64/PW - 1 replays of one panel,
-// with one barrier per panel instead of one per column; narrower panels
-// shorten both (the replays of earlier panels run off the chain).
+// Inverse of the 64 x 64 diagonal block by Gauss-Jordan elimination with
+// partial pivoting (largest |re|+|im| to 17 mantissa bits among the rows not
+// yet pivots, lowest row on ties) on [A | I], held in registers with lane r
+// owning row r.  Rows are not moved and not normalised: step j picks pivot
+// row p_j (rank j) and subtracts l_r = a_rj / a_pj times row p_j from every
+// other row r (rows above the pivots too), so at the end the A part is the
+// pivots d_j at (p_j, j) and row j of A^-1 is row p_j of the identity part
+// divided by d_j.  A partial block (nb < 64) is padded with the identity,
+// so its inverse is padded likewise.
+// Wave q owns the A columns [PW q, PW q + PW) and factors them as a panel
+// alone (the searches and the panel updates stay in the wave), publishing
+// each step as soon as its panel is updated: multipliers (0 in the pivot
+// row), pivot row and 1/d_j in LDS, then the step count s_done with a
+// workgroup-scope release.  The waves apply published steps as they come
+// (polling s_done, sleeping between polls): each to its A columns the steps
+// of the panels to its left before its own panel, and the first half of the
+// waves, idle after their panels, all 64 steps to the identity columns
+// (2 PW each), which nothing on the chain waits for.  Issue priority: the
+// factoring wave over the A-column replays over the identity columns.
+// The body runs on 64 * NBM / PW threads; lmul (NBM x NBM: lmul[j NBM + r]),
+// s_inv, s_piv (NBM each) and s_done are LDS.  Xinv: column-major, leading
+// dimension NBM.
 template <int PW>
-__global__ __launch_bounds__(64 * (NBM / PW)) void k_diag_getrf_panels(cf* __restrict__ A,
-                                                                       size_t lda, int k0, int nb,
-                                                                       int* __restrict__ lperm,
-                                                                       int* __restrict__ status) {
+__device__ __forceinline__ void gj_inverse_body(const cf* __restrict__ A, size_t lda, int nb,
+                                                cf* __restrict__ Xinv, int* __restrict__ status,
+                                                cf* lmul, cf* s_inv, int* s_piv, int* s_done) {
     constexpr int NW = NBM / PW;  // waves = panels
-    __shared__ cf lmul[NBM][NBM];  // lmul[j][r]: multiplier of row r at step j
-    __shared__ int s_piv[NBM];
-    __shared__ int s_rank[NBM];
+    constexpr int EC = 2 * PW;    // identity columns of each of the first NW / 2 waves
     const int tid = threadIdx.x, lane = tid & 63;
     const int q = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int first = PW * q;  // this wave's first column
     cf v[PW];
 #pragma unroll
     for (int i = 0; i < PW; ++i) {
-        const int c = PW * q + i;
-        v[i] = (lane < nb && c < nb) ? A[(size_t)(k0 + c) * lda + k0 + lane] : mkf(0.f, 0.f);
+        const int c = first + i;
+        v[i] = (lane < nb && c < nb) ? A[(size_t)c * lda + lane] : mkf(lane == c ? 1.f : 0.f, 0.f);
     }
-    bool used = lane >= nb;
+    if (tid == 0) *s_done = 0;
+    __syncthreads();  // the count is zero before any wave polls it
+    int avail = 0;
+    auto wait_step = [&](int j) {
+        if (j < avail) return;
+        int spins = 0;
+        while ((avail = __hip_atomic_load(s_done, __ATOMIC_ACQUIRE,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP)) <= j) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > FLAG_SPIN_LIMIT) {
+                if (lane == 0) atomicOr(status, ST_SYNC_TIMEOUT);
+                avail = NBM;
+                break;
+            }
+        }
+    };
+    // the panels to the left, on this wave's A columns
+    bool used = false;
+    __builtin_amdgcn_s_setprio(1);
+    for (int j = 0; j < first; ++j) {
+        wait_step(j);
+        const int p = __builtin_amdgcn_readfirstlane(s_piv[j]);
+        const cf l = lmul[j * NBM + lane];
+        used = used || lane == p;
+#pragma unroll
+        for (int c = 0; c < PW; ++c) v[c] = cfms(v[c], l, readlane_f(v[c], p));
+    }
+    // this wave's panel
+    __builtin_amdgcn_s_setprio(2);
+    bool zero = false;
+#pragma unroll
+    for (int i = 0; i < PW; ++i) {
+        const int j = first + i;
+        // key = |re|+|im| with its 6 lowest mantissa bits replaced by 63 -
+        // lane: the wave maximum picks the largest candidate, lowest row on
+        // ties
+        const unsigned key = used ? 0u
+                                  : ((__float_as_uint(fabsf(v[i].x) + fabsf(v[i].y)) & ~63u) |
+                                     (unsigned)(63 - lane));
+        const int p = 63 - (int)(wave_max_u32(key) & 63u);
+        // branch-free from here to the next search (an exactly zero pivot
+        // gives zero multipliers and is reported)
+        const cf pv = readlane_f(v[i], p);
+        const float d2 = pv.x * pv.x + pv.y * pv.y;
+        zero |= d2 == 0.f;
+        const float rd = d2 > 0.f ? __builtin_amdgcn_rcpf(d2) : 0.f;
+        const cf inv = mkf(pv.x * rd, -pv.y * rd);
+        const bool piv = lane == p;
+        const cf l = piv ? mkf(0.f, 0.f) : v[i] * inv;
+        used = used || piv;
+        lmul[j * NBM + lane] = l;
+        if (lane == 0) {
+            s_piv[j] = p;
+            s_inv[j] = inv;
+        }
+#pragma unroll
+        for (int c = i + 1; c < PW; ++c) v[c] = cfms(v[c], l, readlane_f(v[c], p));
+        // publish after the panel update, when the LDS writes have long
+        // completed (the release waits for them); the empty asm keeps the
+        // updated columns ahead of that wait
+#pragma unroll
+        for (int c = i + 1; c < PW; ++c) asm volatile("" ::"v"(v[c].x), "v"(v[c].y) : "memory");
+        if (lane == 0)
+            __hip_atomic_store(s_done, j + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __builtin_amdgcn_s_setprio(0);
+    if (zero && lane == 0 && first < nb) atomicOr(status, ST_PIVOT_ZERO);
+    if (q >= NW / 2) return;
+    // identity columns [EC q, EC q + EC), all 64 steps: column m stays e_m
+    // (steps with another pivot row leave it alone) until row m is the
+    // pivot, so a column is updated only from that step on (act: the
+    // wave-uniform set of such columns)
+    cf e[EC];
+#pragma unroll
+    for (int i = 0; i < EC; ++i) e[i] = mkf(lane == EC * q + i ? 1.f : 0.f, 0.f);
     int rank = lane;
-    for (int pq = 0; pq < NW && PW * pq < nb; ++pq) {
-        if (q == pq) {
+    unsigned act = 0;
+    for (int j = 0; j < NBM; ++j) {
+        wait_step(j);
+        const int p = __builtin_amdgcn_readfirstlane(s_piv[j]);
+        const cf l = lmul[j * NBM + lane];
+        rank = lane == p ? j : rank;
+        const int m = p - EC * q;
+        if (m >= 0 && m < EC) act |= 1u << m;
 #pragma unroll
-            for (int i = 0; i < PW; ++i) {
-                const int j = PW * pq + i;
-                if (j < nb) {
-                    // key = |re|+|im| with its 6 lowest mantissa bits
-                    // replaced by 63 - lane: the wave maximum picks the
-                    // largest candidate, lowest row on ties
-                    const unsigned key =
-                        used ? 0u
-                             : ((__float_as_uint(fabsf(v[i].x) + fabsf(v[i].y)) & ~63u) |
-                                (unsigned)(63 - lane));
-                    const int p = 63 - (int)(wave_max_u32(key) & 63u);
-                    const cf pv = readlane_f(v[i], p);
-                    const bool zero = pv.x == 0.f && pv.y == 0.f;
-                    const cf inv = zero ? mkf(0.f, 0.f) : cinvf(pv);
-                    cf l = mkf(0.f, 0.f);
-                    if (!used && lane != p) {
-                        l = v[i] * inv;
-                        v[i] = l;
-                    }
-                    if (lane == p) {
-                        used = true;
-                        rank = j;
-                    }
-                    lmul[j][lane] = l;
-                    if (lane == 0) {
-                        s_piv[j] = p;
-                        if (zero) atomicOr(status, ST_PIVOT_ZERO);
-                    }
-#pragma unroll
-                    for (int c = i + 1; c < PW; ++c) v[c] = cfms(v[c], l, readlane_f(v[c], p));
-                }
-            }
-        }
-        __syncthreads();
-        if (q > pq) {
-            // replay the PW steps of panel pq on this wave's columns
-            for (int i = 0; i < PW; ++i) {
-                const int j = PW * pq + i;
-                if (j >= nb) break;
-                const int p = __builtin_amdgcn_readfirstlane(s_piv[j]);
-                const cf l = lmul[j][lane];
-                if (lane == p) {
-                    used = true;
-                    rank = j;
-                }
-#pragma unroll
-                for (int c = 0; c < PW; ++c) v[c] = cfms(v[c], l, readlane_f(v[c], p));
-            }
-        }
+        for (int c = 0; c < EC; ++c)
+            if ((act >> c) & 1u) e[c] = cfms(e[c], l, readlane_f(e[c], p));
     }
-    // the last wave with columns below nb has replayed every pivot: its
-    // ranks are the final ones
-    const int qlast = (nb - 1) / PW;
-    if (q == qlast) s_rank[lane] = rank;
-    __syncthreads();
-    rank = s_rank[lane];
-    if (lane < nb) {
+    const cf sc = s_inv[rank];
 #pragma unroll
-        for (int i = 0; i < PW; ++i) {
-            const int c = PW * q + i;
-            if (c < nb) A[(size_t)(k0 + c) * lda + k0 + rank] = v[i];
-        }
-        if (q == 0) lperm[rank] = lane;
-    }
+    for (int i = 0; i < EC; ++i) Xinv[(size_t)(EC * q + i) * NBM + rank] = e[i] * sc;
 }
 
-// Panel width of the diagonal-block factorisation (BEMX_GETRF_PW = 16, 8 or
-// 4; default 4: 16 waves of 4 columns).  Measured alone on MI355X: 38 us
-// with 16, 30 us with 8, 27 us with 4.
-static int getrf_pw() {
-    static const int w = [] {
-        const char* v = getenv("BEMX_GETRF_PW");
-        const int x = v ? atoi(v) : 4;
-        return (x == 16 || x == 8) ? x : 4;
-    }();
-    return w;
-}
-
-static void launch_getrf(hipStream_t st, cf* A, size_t ld, int k0, int nb, int* lp, int* status) {
-    const int pw = getrf_pw();
-    if (pw == 16)
-        hipLaunchKernelGGL(k_diag_getrf_panels<16>, dim3(1), dim3(256), 0, st, A, ld, k0, nb, lp,
-                           status);
-    else if (pw == 4)
-        hipLaunchKernelGGL(k_diag_getrf_panels<4>, dim3(1), dim3(1024), 0, st, A, ld, k0, nb, lp,
-                           status);
-    else
-        hipLaunchKernelGGL(k_diag_getrf_panels<8>, dim3(1), dim3(512), 0, st, A, ld, k0, nb, lp,
-                           status);
-}
-
-// L21 = A21 U11^-1 for rows [r0, r0+nr).  256 threads load U11 and 64
-// rows with all 32 loads of a thread in flight; the rows are then solved one
-// lane per row, left to right by chunks of 16 columns: wave 0 solves chunk g
-// in registers, then each of the four waves updates a quarter of every
-// later chunk with it.  U11 reads are wave-uniform broadcasts.  The result goes back to
-// A and, split, to the planes la.
-constexpr int TRR_ROWS = 64;
-__device__ __forceinline__ void trsm_upper_rows_body(int bid, cf* __restrict__ A, size_t lda,
-                                                     int k0, int nb, int r0, int nr,
-                                                     __bf16* __restrict__ la, size_t pstride,
-                                                     cf* __restrict__ Us, cf* __restrict__ X,
-                                                     cf* __restrict__ dinv) {
-    constexpr int XS = NBM + 1;
-    // LDS buffers of the calling kernel: Us[c2*NBM + c] = U11[c][c2] (NBM x
-    // NBM), X[c*XS + lane] = column c of the lane's row (NBM x XS), dinv[NBM]
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int rb = r0 + bid * TRR_ROWS;
-    const bool ok = rb + lane < r0 + nr;
-    {
-        cf uv[16], xv[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int t = tid + 256 * q, c2 = t >> 6, c = t & 63;
-            uv[q] = (c2 < nb && c < nb) ? A[(size_t)(k0 + c2) * lda + k0 + c] : mkf(0.f, 0.f);
-        }
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int c = w + 4 * q;
-            xv[q] = (ok && c < nb) ? A[(size_t)(k0 + c) * lda + rb + lane] : mkf(0.f, 0.f);
-        }
-#pragma unroll
-        for (int q = 0; q < 16; ++q) Us[tid + 256 * q] = uv[q];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) X[(w + 4 * q) * XS + lane] = xv[q];
-    }
-    __syncthreads();
-    if (tid < NBM) {
-        cf u = Us[tid * NBM + tid];
-        dinv[tid] = (u.x != 0.f || u.y != 0.f) ? cinvf(u) : mkf(0.f, 0.f);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int g = 0; g < NBM; g += 16) {
-        if (w == 0) {
-            cf a[16];
-#pragma unroll
-            for (int c = 0; c < 16; ++c) a[c] = X[(g + c) * XS + lane];
-#pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                a[c] = a[c] * dinv[g + c];
-#pragma unroll
-                for (int c2 = c + 1; c2 < 16; ++c2) a[c2] = cfms(a[c2], a[c], Us[(g + c2) * NBM + g + c]);
-            }
-#pragma unroll
-            for (int c = 0; c < 16; ++c) X[(g + c) * XS + lane] = a[c];
-        }
-        __syncthreads();
-        // every wave updates columns 4w..4w+3 of each later chunk, so the
-        // next chunk is complete after a quarter of one chunk's update
-        if (g + 16 < NBM) {
-            cf xg[16];
-#pragma unroll
-            for (int i = 0; i < 16; ++i) xg[i] = X[(g + i) * XS + lane];
-#pragma unroll
-            for (int g2 = g + 16; g2 < NBM; g2 += 16) {
-                const int cw = g2 + 4 * w;
-                cf a[4];
-#pragma unroll
-                for (int c = 0; c < 4; ++c) a[c] = X[(cw + c) * XS + lane];
-#pragma unroll
-                for (int i = 0; i < 16; ++i)
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) a[c] = cfms(a[c], xg[i], Us[(cw + c) * NBM + g + i]);
-#pragma unroll
-                for (int c = 0; c < 4; ++c) X[(cw + c) * XS + lane] = a[c];
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll 4
-    for (int c = w; c < NBM; c += 4)
-        if (ok && c < nb) A[(size_t)(k0 + c) * lda + rb + lane] = X[c * XS + lane];
-    // planes: t -> (row rr, k), k fastest, so the 2-byte stores are coalesced
-#pragma unroll 4
-    for (int t = tid; t < TRR_ROWS * NBM; t += 256) {
-        const int rr = t >> 6, k = t & 63;
-        if (rb + rr < r0 + nr)
-            store_planes(la, pstride, (size_t)(rb + rr) * NBM + k,
-                         k < nb ? X[k * XS + rr] : mkf(0.f, 0.f));
-    }
-}
-
-// L21 of rows [c0, c0+nr) and U12 of columns [c0, c0+nc) of the step at
-// k0, in one launch: the first ceil(nr/64) workgroups take row blocks, the
-// others column blocks (the two read only the diagonal block in common).
-__global__ __launch_bounds__(256) void k_trsm_both(cf* __restrict__ A, size_t lda, int k0, int nb,
-                                                   int c0, int nr, int nc,
-                                                   const int* __restrict__ lperm,
-                                                   __bf16* __restrict__ la,
-                                                   __bf16* __restrict__ ub, size_t pstride) {
-    // one set of LDS buffers serves whichever of the two solves this
-    // workgroup runs
-    __shared__ cf T1[NBM * NBM];
-    __shared__ cf T2[NBM * (NBM + 1)];
-    __shared__ cf dinv[NBM];
-    const int nrb = (nr + TRR_ROWS - 1) / TRR_ROWS;
-    if ((int)blockIdx.x < nrb)
-        trsm_upper_rows_body(blockIdx.x, A, lda, k0, nb, c0, nr, la, pstride, T1, T2, dinv);
-    else
-        trsm_lower_cols_body<cf, true>(blockIdx.x - nrb, A, lda, k0, nb, c0, nc, lperm, ub,
-                                       pstride, T1, T2);
+template <int PW>
+__global__ __launch_bounds__(64 * (NBM / PW)) void k_diag_gj(const cf* __restrict__ A, size_t lda,
+                                                             int nb, cf* __restrict__ Xinv,
+                                                             int* __restrict__ status) {
+    __shared__ cf lmul[NBM * NBM];
+    __shared__ cf s_inv[NBM];
+    __shared__ int s_piv[NBM];
+    __shared__ int s_done;
+    gj_inverse_body<PW>(A, lda, nb, Xinv, status, lmul, s_inv, s_piv, &s_done);
 }
 
 // Bounded wait until *flag >= target (see wait_count in lu.hip); relaxed
@@ -332,137 +237,6 @@ __device__ inline void wait_ge(int* flag, int target, int* status) {
 
 __device__ inline void publish(int* flag, int value) {
     __hip_atomic_store(flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Explicit inverses of diagonal block k for the preconditioner's block
-// solves: Li = L11^-1 P (workgroup 0, one lane per column of P) and
-// Ui = U11^-1 (workgroup 1, one lane per row of I), column-major 64 x 64.
-// Solved like the two triangular kernels (16-wide chunks: wave 0 solves a
-// chunk in registers, the four waves update the later chunks).  A partial
-// last block is padded with the identity.
-__global__ __launch_bounds__(256) void k_diag_inverses(const cf* __restrict__ A, size_t lda,
-                                                       int n, const int* __restrict__ lperm_all,
-                                                       cf* __restrict__ inv) {
-    // block y of the grid: diagonal block y; x = 0: lower factor, x = 1: upper
-    const int blk = blockIdx.y, nblk = gridDim.y;
-    const int k0 = blk * NBM, nb = min(NBM, n - k0);
-    const int* __restrict__ lperm = lperm_all + (size_t)blk * NBM;
-    cf* __restrict__ Li = inv + (size_t)blk * NBM * NBM;
-    cf* __restrict__ Ui = inv + (size_t)(nblk + blk) * NBM * NBM;
-    constexpr int XS = NBM + 1;
-    __shared__ cf M[NBM * NBM];  // M[c*NBM + r] = block[r][c]
-    __shared__ cf X[NBM * XS];
-    __shared__ cf dinv[NBM];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    {
-        cf mv[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int t = tid + 256 * q, c = t >> 6, r = t & 63;
-            mv[q] = (c < nb && r < nb) ? A[(size_t)(k0 + c) * lda + k0 + r]
-                                       : mkf(r == c ? 1.f : 0.f, 0.f);
-        }
-#pragma unroll
-        for (int q = 0; q < 16; ++q) M[tid + 256 * q] = mv[q];
-    }
-    if (blockIdx.x == 0) {
-        // X[r][col] = (P I)[r][col]: row r of P I is row lperm[r] of I
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int r = w + 4 * q;
-            const int src = r < nb ? lperm[r] : r;
-            X[r * XS + lane] = mkf(src == lane ? 1.f : 0.f, 0.f);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int g = 0; g < NBM; g += 16) {
-            if (w == 0) {
-                cf a[16];
-#pragma unroll
-                for (int r = 0; r < 16; ++r) a[r] = X[(g + r) * XS + lane];
-#pragma unroll
-                for (int i = 0; i < 15; ++i) {
-#pragma unroll
-                    for (int r = i + 1; r < 16; ++r) a[r] = cfms(a[r], M[(g + i) * NBM + g + r], a[i]);
-                }
-#pragma unroll
-                for (int r = 0; r < 16; ++r) X[(g + r) * XS + lane] = a[r];
-            }
-            __syncthreads();
-            const int g2 = g + 16 * (1 + w);
-            if (g2 < NBM) {
-                cf a[16];
-#pragma unroll
-                for (int r = 0; r < 16; ++r) a[r] = X[(g2 + r) * XS + lane];
-#pragma unroll 4
-                for (int i = 0; i < 16; ++i) {
-                    const cf xi = X[(g + i) * XS + lane];
-#pragma unroll
-                    for (int r = 0; r < 16; ++r) a[r] = cfms(a[r], M[(g + i) * NBM + g2 + r], xi);
-                }
-#pragma unroll
-                for (int r = 0; r < 16; ++r) X[(g2 + r) * XS + lane] = a[r];
-            }
-            __syncthreads();
-        }
-        // Li column col = lane's column: Li[col*64 + r] = X[r][col]
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int col = w + 4 * q;
-            Li[col * NBM + lane] = X[lane * XS + col];
-        }
-    } else {
-        // rows of I, solved x U = e_r left to right (lane = row r)
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int c = w + 4 * q;
-            X[c * XS + lane] = mkf(c == lane ? 1.f : 0.f, 0.f);
-        }
-        __syncthreads();
-        if (tid < NBM) {
-            const cf u = M[tid * NBM + tid];
-            dinv[tid] = (u.x != 0.f || u.y != 0.f) ? cinvf(u) : mkf(0.f, 0.f);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int g = 0; g < NBM; g += 16) {
-            if (w == 0) {
-                cf a[16];
-#pragma unroll
-                for (int c = 0; c < 16; ++c) a[c] = X[(g + c) * XS + lane];
-#pragma unroll
-                for (int c = 0; c < 16; ++c) {
-                    a[c] = a[c] * dinv[g + c];
-#pragma unroll
-                    for (int c2 = c + 1; c2 < 16; ++c2) a[c2] = cfms(a[c2], a[c], M[(g + c2) * NBM + g + c]);
-                }
-#pragma unroll
-                for (int c = 0; c < 16; ++c) X[(g + c) * XS + lane] = a[c];
-            }
-            __syncthreads();
-            const int g2 = g + 16 * (1 + w);
-            if (g2 < NBM) {
-                cf a[16];
-#pragma unroll
-                for (int c = 0; c < 16; ++c) a[c] = X[(g2 + c) * XS + lane];
-#pragma unroll 4
-                for (int i = 0; i < 16; ++i) {
-                    const cf xi = X[(g + i) * XS + lane];
-#pragma unroll
-                    for (int c = 0; c < 16; ++c) a[c] = cfms(a[c], xi, M[(g2 + c) * NBM + g + i]);
-                }
-#pragma unroll
-                for (int c = 0; c < 16; ++c) X[(g2 + c) * XS + lane] = a[c];
-            }
-            __syncthreads();
-        }
-        // Ui[c*64 + r] = X[c][r]: row r = lane
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int c = w + 4 * q;
-            Ui[c * NBM + lane] = X[c * XS + lane];
-        }
-    }
 }
 
 // Wave-uniform copy of lane src's complex double (v_readlane, no LDS).
@@ -485,14 +259,13 @@ __device__ inline void wave_wait_ge(int* flag, int target, int* status) {
 // Preconditioner block solves over 128-row blocks (two factor blocks each),
 // which halves the number of block hand-offs of the dependency chain.
 // k_pair_inverses builds, for pair p = (blocks a = 2p, b = 2p+1), the
-// inverses of the 128 x 128 diagonal blocks of both triangular factors from
-// the 64 x 64 inverses and the coupling blocks:
-//   lower  [Li_a, 0; X, Li_b],  X = -Li_b L_ba Li_a
+// inverses of the 128 x 128 diagonal blocks of both block factors from the
+// 64 x 64 diagonal-block inverses and the coupling blocks:
+//   lower  [I, 0; -L_ba, I]     (unit diagonal blocks)
 //   upper  [Ui_a, Y; 0, Ui_b],  Y = -Ui_a U_ab Ui_b
-// (Li = L^-1 P of a block, Ui = U^-1; L_ba / U_ab as stored in A, rows of
-// block b in their unpermuted order, which is how the block solves use
-// them).  Column-major 128 x 128; a missing or partial block b contributes
-// zero coupling.
+// (Ui = inverse of the block's diagonal block; L_ba / U_ab as stored in A).
+// Column-major 128 x 128; a missing or partial block b contributes zero
+// coupling.
 // ---------------------------------------------------------------------------
 constexpr int NB2 = 2 * NBM;
 
@@ -507,37 +280,39 @@ __global__ __launch_bounds__(256) void k_pair_inverses(const cf* __restrict__ A,
     const int a = 2 * p, b = 2 * p + 1;
     const int a0 = a * NBM, b0 = b * NBM;
     const int nbb = b < nblk ? min(NBM, n - b0) : 0;  // rows/columns of block b
-    const cf* Ia = inv + (size_t)(lower ? a : nblk + a) * NBM * NBM;
-    const cf* Ib = inv + (size_t)(lower ? b : nblk + b) * NBM * NBM;
     cf* M = (lower ? Ml : Mu) + (size_t)p * NB2 * NB2;
-    // the two diagonal quadrants, and zero for the one that stays zero
-    for (int t = tid; t < NBM * NBM; t += 256) {
-        const int c = t >> 6, r = t & 63;
-        M[(size_t)c * NB2 + r] = Ia[t];
-        M[(size_t)(NBM + c) * NB2 + NBM + r] =
-            nbb > 0 ? Ib[t] : mkf(r == c ? 1.f : 0.f, 0.f);
-        if (lower) M[(size_t)(NBM + c) * NB2 + r] = mkf(0.f, 0.f);
-        else M[(size_t)c * NB2 + NBM + r] = mkf(0.f, 0.f);
-    }
-    if (nbb == 0) {
+    if (lower) {
         for (int t = tid; t < NBM * NBM; t += 256) {
             const int c = t >> 6, r = t & 63;
-            if (lower) M[(size_t)c * NB2 + NBM + r] = mkf(0.f, 0.f);
-            else M[(size_t)(NBM + c) * NB2 + r] = mkf(0.f, 0.f);
+            const cf one = mkf(r == c ? 1.f : 0.f, 0.f);
+            M[(size_t)c * NB2 + r] = one;
+            M[(size_t)(NBM + c) * NB2 + NBM + r] = one;
+            M[(size_t)(NBM + c) * NB2 + r] = mkf(0.f, 0.f);
+            cf x = mkf(0.f, 0.f);
+            if (r < nbb) {
+                x = A[(size_t)(a0 + c) * lda + b0 + r];
+                x = mkf(-x.x, -x.y);
+            }
+            M[(size_t)c * NB2 + NBM + r] = x;
         }
         return;
     }
-    // lower: S0 = L_ba (rows b, cols a), S1 = Li_a;  T = S0 S1;  then X = -Li_b T
-    // upper: S0 = Ui_a, S1 = U_ab (rows a, cols b);  T = S0 S1;  then Y = -T Ui_b
+    const cf* Ia = inv + (size_t)a * NBM * NBM;
+    const cf* Ib = inv + (size_t)(b < nblk ? b : a) * NBM * NBM;
+    // upper: the two diagonal quadrants, zero below them
     for (int t = tid; t < NBM * NBM; t += 256) {
         const int c = t >> 6, r = t & 63;
-        if (lower) {
-            S0[t] = r < nbb ? A[(size_t)(a0 + c) * lda + b0 + r] : mkf(0.f, 0.f);
-            S1[t] = Ia[t];
-        } else {
-            S0[t] = Ia[t];
-            S1[t] = c < nbb ? A[(size_t)(b0 + c) * lda + a0 + r] : mkf(0.f, 0.f);
-        }
+        M[(size_t)c * NB2 + r] = Ia[t];
+        M[(size_t)(NBM + c) * NB2 + NBM + r] = nbb > 0 ? Ib[t] : mkf(r == c ? 1.f : 0.f, 0.f);
+        M[(size_t)c * NB2 + NBM + r] = mkf(0.f, 0.f);
+        if (nbb == 0) M[(size_t)(NBM + c) * NB2 + r] = mkf(0.f, 0.f);
+    }
+    if (nbb == 0) return;
+    // S0 = Ui_a, S1 = U_ab (rows a, cols b);  T = S0 S1;  then Y = -T Ui_b
+    for (int t = tid; t < NBM * NBM; t += 256) {
+        const int c = t >> 6, r = t & 63;
+        S0[t] = Ia[t];
+        S1[t] = c < nbb ? A[(size_t)(b0 + c) * lda + a0 + r] : mkf(0.f, 0.f);
     }
     __syncthreads();
     // thread: column c = tid >> 2, rows r = (tid & 3) + 4 i
@@ -556,28 +331,17 @@ __global__ __launch_bounds__(256) void k_pair_inverses(const cf* __restrict__ A,
     }
     __syncthreads();
     {
-        // lower: X = -Li_b T (Li_b from global, column-major); upper: Y = -T Ui_b
         cf acc[16];
 #pragma unroll
         for (int i = 0; i < 16; ++i) acc[i] = mkf(0.f, 0.f);
         for (int k = 0; k < NBM; ++k) {
-            if (lower) {
-                const cf bk = T[c * NBM + k];
+            const cf bk = Ib[c * NBM + k];
 #pragma unroll
-                for (int i = 0; i < 16; ++i) acc[i] = acc[i] + Ib[k * NBM + rq + 4 * i] * bk;
-            } else {
-                const cf bk = Ib[c * NBM + k];
-#pragma unroll
-                for (int i = 0; i < 16; ++i) acc[i] = acc[i] + T[k * NBM + rq + 4 * i] * bk;
-            }
+            for (int i = 0; i < 16; ++i) acc[i] = acc[i] + T[k * NBM + rq + 4 * i] * bk;
         }
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int r = rq + 4 * i;
-            const cf v = mkf(-acc[i].x, -acc[i].y);
-            if (lower) M[(size_t)c * NB2 + NBM + r] = v;
-            else M[(size_t)(NBM + c) * NB2 + r] = v;
-        }
+        for (int i = 0; i < 16; ++i)
+            M[(size_t)(NBM + c) * NB2 + rq + 4 * i] = mkf(-acc[i].x, -acc[i].y);
     }
 }
 
@@ -861,6 +625,130 @@ __device__ inline bf8 neg_bf8(bf8 x) {
     return __builtin_bit_cast(bf8, u);
 }
 
+// The block column and row of step k once the diagonal block's inverse Xinv
+// is known: L21 = A21 Xinv for the rows [c0, c0+nr) and U12 = A12 (the rows
+// of the block as they stand) for the columns [c0, c0+nc), in one launch:
+// the first ceil(nr/64) workgroups take 64-row blocks, the others 64-column
+// blocks.  Row blocks: Xinv is split into bf16 hi/lo planes in LDS (the MFMA
+// B operand); each wave takes 16 rows, its A operand (8 consecutive k of one
+// row per lane) loaded straight from A and split in registers, and computes
+// 16 x 64 with 2 x 4 complex 16x16x32 blocks of 12 MFMA (the split-BF16x3
+// product of the trailing update, FP32 accumulation).  The 64 x 64 result is
+// staged in LDS and stored to A and, split, to the planes la, both
+// coalesced.  Column blocks only split A12 into the planes ub.
+constexpr int TRR_ROWS = 64;
+__global__ __launch_bounds__(256) void k_panel_inv(cf* __restrict__ A, size_t lda, int k0, int c0,
+                                                   int nr, int nc, const cf* __restrict__ Xinv,
+                                                   __bf16* __restrict__ la,
+                                                   __bf16* __restrict__ ub, size_t pstride) {
+    constexpr int XS = NBM + 1;
+    constexpr int BK = NBM + 8;  // padded k stride of the LDS planes
+    __shared__ __attribute__((aligned(16))) __bf16 Bp[4][NBM * BK];  // [pl][col BK + k]
+    __shared__ cf X[NBM * XS];                                       // X[col XS + row]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int nrb = (nr + TRR_ROWS - 1) / TRR_ROWS;
+    if ((int)blockIdx.x >= nrb) {
+        const int cb = c0 + ((int)blockIdx.x - nrb) * NBM;
+#pragma unroll 4
+        for (int t = tid; t < NBM * NBM; t += 256) {
+            const int j = t >> 6, k = t & 63;
+            if (cb + j < c0 + nc)
+                store_planes(ub, pstride, (size_t)(cb + j) * NBM + k,
+                             A[(size_t)(cb + j) * lda + k0 + k]);
+        }
+        return;
+    }
+    const int rb = c0 + (int)blockIdx.x * TRR_ROWS;
+    const int li = lane & 15, lq = lane >> 4;
+    const int row = rb + 16 * w + li;
+    const bool rok = row < c0 + nr;
+    // A operand: k = 32 kc + 8 lq + e of this lane's row, loads in flight
+    // while Xinv is split into LDS
+    cf ax[2][8];
+#pragma unroll
+    for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            ax[kc][e] = rok ? A[(size_t)(k0 + 32 * kc + 8 * lq + e) * lda + row] : mkf(0.f, 0.f);
+#pragma unroll 4
+    for (int t = tid; t < NBM * NBM; t += 256) {
+        const int col = t >> 6, k = t & 63;  // Xinv[col 64 + k] = Xinv[k][col]
+        const cf x = Xinv[t];
+        __bf16 h, l;
+        split_bf16(x.x, h, l);
+        Bp[0][col * BK + k] = h;
+        Bp[1][col * BK + k] = l;
+        split_bf16(x.y, h, l);
+        Bp[2][col * BK + k] = h;
+        Bp[3][col * BK + k] = l;
+    }
+    bf8 fa[2][4];
+#pragma unroll
+    for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            __bf16 h, l;
+            split_bf16(ax[kc][e].x, h, l);
+            fa[kc][0][e] = h;
+            fa[kc][1][e] = l;
+            split_bf16(ax[kc][e].y, h, l);
+            fa[kc][2][e] = h;
+            fa[kc][3][e] = l;
+        }
+    __syncthreads();
+    f4 accR[4], accI[4];
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+        accR[y] = f4{0.f, 0.f, 0.f, 0.f};
+        accI[y] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int kc = 0; kc < 2; ++kc) {
+        const bf8 naih = neg_bf8(fa[kc][2]), nail = neg_bf8(fa[kc][3]);
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            bf8 fb[4];
+#pragma unroll
+            for (int pl = 0; pl < 4; ++pl)
+                fb[pl] = *(const bf8*)&Bp[pl][(16 * y + li) * BK + 32 * kc + 8 * lq];
+            f4 r = accR[y], i = accI[y];
+            // + Ar Br - Ai Bi
+            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[kc][0], fb[0], r, 0, 0, 0);
+            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[kc][0], fb[1], r, 0, 0, 0);
+            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[kc][1], fb[0], r, 0, 0, 0);
+            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, fb[2], r, 0, 0, 0);
+            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(naih, fb[3], r, 0, 0, 0);
+            r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nail, fb[2], r, 0, 0, 0);
+            // + Ar Bi + Ai Br
+            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[kc][0], fb[2], i, 0, 0, 0);
+            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[kc][0], fb[3], i, 0, 0, 0);
+            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[kc][1], fb[2], i, 0, 0, 0);
+            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[kc][2], fb[0], i, 0, 0, 0);
+            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[kc][2], fb[1], i, 0, 0, 0);
+            i = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[kc][3], fb[0], i, 0, 0, 0);
+            accR[y] = r;
+            accI[y] = i;
+        }
+    }
+    // D layout: lane -> column 16 y + li, rows 16 w + 4 lq + (0..3)
+#pragma unroll
+    for (int y = 0; y < 4; ++y)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            X[(16 * y + li) * XS + 16 * w + 4 * lq + e] = mkf(accR[y][e], accI[y][e]);
+    __syncthreads();
+    const bool ok = rb + lane < c0 + nr;
+#pragma unroll 4
+    for (int c = w; c < NBM; c += 4)
+        if (ok) A[(size_t)(k0 + c) * lda + rb + lane] = X[c * XS + lane];
+    // planes: t -> (row rr, k), k fastest, so the 2-byte stores are coalesced
+#pragma unroll 4
+    for (int t = tid; t < TRR_ROWS * NBM; t += 256) {
+        const int rr = t >> 6, k = t & 63;
+        if (rb + rr < c0 + nr) store_planes(la, pstride, (size_t)(rb + rr) * NBM + k, X[k * XS + rr]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Trailing update over K = 64 * nks: C -= sum over chunks q of L21_q U12_q,
 // chunk q's planes at base + q * sstride (L21 planes first, U12 planes
@@ -900,7 +788,7 @@ template <int NBUF>
 __device__ __forceinline__ void upd2_tile(const UpdRegion& R, int t, const __bf16* __restrict__ base,
                                           size_t pstride, size_t sstride, int nks, size_t ldc,
                                           __bf16* __restrict__ la, __bf16* __restrict__ lb,
-                                          int panel) {
+                                          int panel, bool skip_corner = false) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int gx = (R.m + BT - 1) / BT, gy = (R.nc + BT - 1) / BT;
     const int p = t / (gx * panel);
@@ -1022,8 +910,10 @@ __device__ __forceinline__ void upd2_tile(const UpdRegion& R, int t, const __bf1
             __syncthreads();
         }
     }
-    // C -= L21 U12
-    if (fast) {
+    // C -= L21 U12 (the tile's top-left 64 x 64 corner, waves 0 and 2, is
+    // left to another workgroup when skip_corner is set)
+    if (skip_corner && wr == 0 && wc < 2) {
+    } else if (fast) {
 #pragma unroll
         for (int x = 0; x < 4; ++x)
 #pragma unroll
@@ -1080,6 +970,43 @@ __global__ __launch_bounds__(512) void k_cgemm_update2(UpdRegion r0, UpdRegion r
     }
 }
 
+// Update of step k fused with the inversion of diagonal block k+1, which is
+// the top-left nb1 x nb1 corner of r0: workgroup 0 applies the update to
+// that corner itself (one partial tile) and inverts it with
+// gj_inverse_body<8> (LDS of the tile reused) into Xinv1; workgroups 1..
+// update r0 without that corner, and r1.  The next block's inversion thus
+// runs beside the update instead of after it.
+__global__ __launch_bounds__(512) void k_cgemm_update_inverse(UpdRegion r0, UpdRegion r1,
+                                                              const __bf16* __restrict__ base,
+                                                              size_t pstride, size_t sstride,
+                                                              int nks, size_t ldc, int panel,
+                                                              int nb1, cf* __restrict__ Xinv1,
+                                                              int* __restrict__ status) {
+    __shared__ __attribute__((aligned(16))) __bf16 lds_a[2 * 4 * PLN];
+    __shared__ __attribute__((aligned(16))) __bf16 lds_b[2 * 4 * PLN];
+    static_assert(sizeof(lds_a) >= sizeof(cf) * (NBM + 1) * NBM + sizeof(int) * (NBM + 1),
+                  "inversion scratch fits the tile buffer");
+    if (blockIdx.x == 0) {
+        const UpdRegion corner{nb1, nb1, r0.r0, r0.c0, r0.C};
+        upd2_tile<2>(corner, 0, base, pstride, sstride, nks, ldc, lds_a, lds_b, panel);
+        __syncthreads();  // the corner's stores before the inversion reads it
+        cf* lmul = (cf*)lds_a;
+        cf* s_inv = lmul + NBM * NBM;
+        int* s_piv = (int*)(s_inv + NBM);
+        gj_inverse_body<8>(r0.C, ldc, nb1, Xinv1, status, lmul, s_inv, s_piv, s_piv + NBM);
+        return;
+    }
+    const int t0 = upd2_tiles(r0), tiles = t0 + upd2_tiles(r1);
+    const int per = (tiles + NXCD - 1) / NXCD;
+    for (int id = blockIdx.x - 1; id < per * NXCD; id += gridDim.x - 1) {
+        const int t = (id % NXCD) * per + id / NXCD;
+        if (t < t0)
+            upd2_tile<2>(r0, t, base, pstride, sstride, nks, ldc, lds_a, lds_b, panel, t == 0);
+        else if (t < tiles)
+            upd2_tile<2>(r1, t - t0, base, pstride, sstride, nks, ldc, lds_a, lds_b, panel);
+    }
+}
+
 static int upd2_tiles_host(const UpdRegion& R) {
     return R.m > 0 && R.nc > 0 ? ((R.m + BT - 1) / BT) * ((R.nc + BT - 1) / BT) : 0;
 }
@@ -1108,6 +1035,17 @@ static void launch_update(hipStream_t s, const UpdRegion& a, const UpdRegion& b,
     else
         hipLaunchKernelGGL(k_cgemm_update2<1>, dim3(g), dim3(512), 0, s, a, b, base, pstride,
                            sstride, nks, ldc, 4);
+}
+
+// k_cgemm_update_inverse for regions a (non-empty; block k+1 of nb1 columns
+// at its top-left corner) and b.
+static void launch_update_inverse(hipStream_t s, const UpdRegion& a, const UpdRegion& b,
+                                  const __bf16* base, size_t pstride, size_t sstride, int nks,
+                                  size_t ldc, int nb1, cf* Xinv1, int* status) {
+    const int tiles = upd2_tiles_host(a) + upd2_tiles_host(b);
+    const int g = ((tiles + NXCD - 1) / NXCD) * NXCD;
+    hipLaunchKernelGGL(k_cgemm_update_inverse, dim3(1 + g), dim3(512), 0, s, a, b, base, pstride,
+                       sstride, nks, ldc, 4, nb1, Xinv1, status);
 }
 
 // Workgroups of the bulk update on the side stream (BEMX_BULK_WGS; default
@@ -1259,11 +1197,10 @@ enum { M_A32 = 9, M_PERM, M_FLAGS, M_STATUS, M_VEC, M_V, M_PART, M_HOST, M_PTMP,
 
 struct MixedFactor {
     cf* A = nullptr;
-    int* lperm = nullptr;
     int* flags = nullptr;
     int* status = nullptr;
     int* epoch = nullptr;   // host-side counter living in LuWork
-    cf* inv = nullptr;      // per block: L_bb^-1 P_bb, then (after nblk blocks) U_bb^-1
+    cf* inv = nullptr;      // per block: inverse of its diagonal block (column-major 64 x 64)
     cf* minv = nullptr;     // per 128-row pair: lower inverse, then (after npair) upper
     cf* wc = nullptr;       // per pair: lower coupling Wl, then (after npair) upper Wu
     int n = 0, nblk = 0, npair = 0, coop = 1;
@@ -1286,25 +1223,34 @@ static int group_blocks() {
 size_t mixed_factor_extra_bytes(int n) {
     const size_t nb = (size_t)(n + NBM - 1) / NBM, np = (nb + 1) / 2;
     const size_t planes = sizeof(__bf16) * (size_t)n * NBM * 8 * 2 * group_blocks();
-    const size_t inv = sizeof(cf) * nb * NBM * NBM * 2 + sizeof(cf) * np * NB2 * NB2 * 2;
+    const size_t inv = sizeof(cf) * nb * NBM * NBM + sizeof(cf) * np * NB2 * NB2 * 4;
     return planes + inv;
 }
 
 // The launch sequence of the factorisation (no host synchronisation, no
 // allocation), so it can be captured into a graph.
 //
+// Block LU in the form A = L U with unit block-lower L (L21 = A21 A11^-1 per
+// step) and block-upper U whose diagonal blocks are the Schur-complement
+// blocks A11 themselves, kept as their inverses Xinv (Gauss-Jordan with
+// partial pivoting inside the block), and whose off-diagonal blocks are the
+// rows A12 as they stand at their step: the Schur complements are those of
+// the partial-pivot-in-block LU, with no triangular solves on the chain.
 // Blocks of 64 columns are factored in groups of s = group_blocks() (W = 64 s
 // columns).  Step k of group g (its columns end at gend):
-//   st:   getrf(k); L21(k) on all rows below and U12(k) on all columns to the
-//         right (one launch, bf16 planes into the group's slot of k); then
-//         the update of the group's own columns (rows below k) and of the
-//         group's own rows (columns beyond gend), K = 64.  (A left-looking
-//         form, one K = 64 j update of block column and row k per step, was
-//         measured 0.7 ms slower per 6480-DOF solve: its 64-wide regions
-//         half-fill the 128-wide tiles.)
+//   st:   L21(k) = A21 Xinv(k) on all rows below and the planes of U12(k) on
+//         all columns to the right (k_panel_inv, bf16 planes into the
+//         group's slot of k); then the update of the group's own columns
+//         (rows below k) and of the group's own rows (columns beyond gend),
+//         K = 64, in one launch with the inversion of block k+1
+//         (k_cgemm_update_inverse; block 0 by k_diag_gj alone).  (A
+//         left-looking form, one K = 64 j update of block column and row k
+//         per step, was measured 0.7 ms slower per 6480-DOF solve: its
+//         64-wide regions half-fill the 128-wide tiles.)
 // After the last step of group g, with K = W from the group's planes:
 //   st:   [wait Z(g-1)] X(g): the next group's columns, all rows below gend;
-//         Y(g): the next group's rows, the columns beyond it;
+//         Y(g): the next group's rows, the columns beyond it; with the
+//         inversion of the next group's first block in the same launch;
 //   side: Z(g): everything beyond the next group (the bulk),
 // so group g+1 is factored while the bulk update of group g runs.  Planes
 // are double-buffered by group parity: group g+1 writes the slots Z(g-1) read,
@@ -1332,21 +1278,27 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
         const int k0 = k * NBM, nb = std::min(NBM, n - k0);
         const int c0 = k0 + nb, rest = n - c0;
         const int gend = std::min(n, (g + 1) * sg * NBM);
-        int* lp = F.lperm + (size_t)k * NBM;
+        cf* xinv = F.inv + (size_t)k * NBM * NBM;
         __bf16* gbase = planes + (size_t)(g & 1) * sg * sstride;
         __bf16* la = gbase + (size_t)j * sstride;
         __bf16* ub = la + 4 * pstride;
-        launch_getrf(st, F.A, ld, k0, nb, lp, F.status);
+        // block 0 alone; block k+1 is inverted inside step k's update
+        if (k == 0)
+            hipLaunchKernelGGL(k_diag_gj<4>, dim3(1), dim3(1024), 0, st, (const cf*)F.A, ld, nb,
+                               xinv, F.status);
         if (rest <= 0) break;
         const int nrb = (rest + TRR_ROWS - 1) / TRR_ROWS;
-        const int ncb = (rest + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG;
-        hipLaunchKernelGGL(k_trsm_both, dim3(nrb + ncb), dim3(256), 0, st, F.A, ld, k0, nb, c0,
-                           rest, rest, (const int*)lp, la, ub, pstride);
+        const int ncb = (rest + NBM - 1) / NBM;
+        hipLaunchKernelGGL(k_panel_inv, dim3(nrb + ncb), dim3(256), 0, st, F.A, ld, k0, c0, rest,
+                           rest, (const cf*)xinv, la, ub, pstride);
+        const int nb1 = std::min(NBM, rest);
+        cf* xinv1 = xinv + NBM * NBM;
         if (c0 < gend) {
             // inside the group: its columns right of k, and its rows below k
             const UpdRegion cs{rest, gend - c0, c0, c0, at(c0, c0)};
             const UpdRegion rs{gend - c0, n - gend, c0, gend, at(c0, gend)};
-            launch_update(st, cs, n > gend ? rs : none, la, pstride, sstride, 1, ld);
+            launch_update_inverse(st, cs, n > gend ? rs : none, la, pstride, sstride, 1, ld, nb1,
+                                  xinv1, F.status);
             continue;
         }
         // last block of group g: apply the group to the rest, K = 64 * sg
@@ -1355,7 +1307,8 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
         bulk_pending = false;
         const UpdRegion X{n - gend, nend - gend, gend, gend, at(gend, gend)};
         const UpdRegion Y{nend - gend, n - nend, gend, nend, at(gend, nend)};
-        launch_update(st, X, n > nend ? Y : none, gbase, pstride, sstride, sg, ld);
+        launch_update_inverse(st, X, n > nend ? Y : none, gbase, pstride, sstride, sg, ld, nb1,
+                              xinv1, F.status);
         if (n > nend) {
             const UpdRegion Z{n - nend, n - nend, nend, nend, at(nend, nend)};
             chk(hipEventRecord(w.ev_panel, st));
@@ -1366,10 +1319,6 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
         }
     }
     if (bulk_pending) chk(hipStreamWaitEvent(st, w.ev_bulk, 0));
-    // explicit inverses of every diagonal block (L_bb^-1 P_bb and U_bb^-1),
-    // all blocks in one launch: a diagonal block is final once its getrf ran
-    hipLaunchKernelGGL(k_diag_inverses, dim3(2, nblk), dim3(256), 0, st, (const cf*)F.A, ld, n,
-                       (const int*)F.lperm, F.inv);
     // 128-row diagonal-block inverses of both factors for the block solves
     hipLaunchKernelGGL(k_pair_inverses, dim3(2, F.npair), dim3(256), 0, st, (const cf*)F.A, ld,
                        n, (const cf*)F.inv, nblk, F.minv, F.minv + (size_t)F.npair * NB2 * NB2);
@@ -1391,9 +1340,7 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     F.ld = ((size_t)n + 31) & ~(size_t)31;  // 16-byte aligned 4-row groups
     F.A = (cf*)w.get(M_A32, sizeof(cf) * F.ld * n, &e);
     MX_CHK(e);
-    F.lperm = (int*)w.get(M_PERM, sizeof(int) * (size_t)nblk * NBM, &e);
-    MX_CHK(e);
-    F.inv = (cf*)w.get(M_INV, sizeof(cf) * (size_t)nblk * NBM * NBM * 2, &e);
+    F.inv = (cf*)w.get(M_INV, sizeof(cf) * (size_t)nblk * NBM * NBM, &e);
     MX_CHK(e);
     F.npair = (nblk + 1) / 2;
     F.minv = (cf*)w.get(M_MINV, sizeof(cf) * (size_t)F.npair * NB2 * NB2 * 2, &e);
@@ -1426,7 +1373,7 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     MX_CHK(e);
     // The launch sequence depends only on n and the buffer addresses: it is
     // captured once into a graph and replayed while those stay the same.
-    const void* key[8] = {(const void*)(size_t)n, A64, (const void*)lda, F.A, planes, F.lperm,
+    const void* key[8] = {(const void*)(size_t)n, A64, (const void*)lda, F.A, planes, F.inv,
                           F.status, w.side};
     if (w.use_graphs) {
         if (!w.gexec || memcmp(key, w.gkey, sizeof key) != 0) {
