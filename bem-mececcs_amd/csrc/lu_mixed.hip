@@ -5,9 +5,13 @@
 // Factorisation (blocks of NBM = 64 columns, A32 = (complex float) A64), in
 // the block form A = L U with unit block-lower L and block-upper U whose
 // diagonal blocks are kept as explicit inverses:
-//   k_diag_gj / gj_inverse_body  inverse of the 64 x 64 diagonal block by
-//                      Gauss-Jordan with partial pivoting inside the block
-//                      (registers: lane = row, wave = a panel of columns)
+//   k_diag_gj / gj_factor_body  Gauss-Jordan elimination of the 64 x 64
+//                      diagonal block with partial pivoting inside the block
+//                      (registers: lane = row, wave = a panel of columns),
+//                      recorded (multipliers, pivots)
+//   k_cgemm_update_inverse  the next block's elimination (workgroup 0) and
+//                      its inverse Xinv from the record (8 more workgroups)
+//                      beside the update of the current step
 //   k_panel_inv        L21 = A21 Xinv (MFMA) for all rows below and the
 //                      split planes of U12 = A12 for all columns right
 //   k_cgemm_update2    A22 -= L21 U12 over K = 64 .. 512; every FP32 operand x
@@ -18,7 +22,7 @@
 //                      complex = 4 real products.
 // Blocks are factored in groups (enqueue_factor): inside a group the update
 // is applied to the group's own columns and rows only, in the same launch as
-// the inversion of the next diagonal block (k_cgemm_update_inverse); the rest
+// the elimination of the next diagonal block (k_cgemm_update_inverse); the rest
 // of the matrix gets one update per group with K = 64 x group size, on a
 // second stream, while the next group is factored.
 // Preconditioner M^-1 v (FP64 vectors, FP32 factors): k_trsv_coop, one
@@ -79,34 +83,65 @@ __device__ inline cf readlane_f(cf v, int src) {
                __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), src)));
 }
 
-// Inverse of the 64 x 64 diagonal block by Gauss-Jordan elimination with
-// partial pivoting (largest |re|+|im| to 17 mantissa bits among the rows not
-// yet pivots, lowest row on ties) on [A | I], held in registers with lane r
-// owning row r.  Rows are not moved and not normalised: step j picks pivot
-// row p_j (rank j) and subtracts l_r = a_rj / a_pj times row p_j from every
-// other row r (rows above the pivots too), so at the end the A part is the
-// pivots d_j at (p_j, j) and row j of A^-1 is row p_j of the identity part
-// divided by d_j.  A partial block (nb < 64) is padded with the identity,
-// so its inverse is padded likewise.
+// Gauss-Jordan elimination of the 64 x 64 diagonal block with partial
+// pivoting (largest |re|+|im| to 17 mantissa bits among the rows not yet
+// pivots, lowest row on ties), held in registers with lane r owning row r.
+// Rows are not moved and not normalised: step j picks pivot row p_j (rank
+// j) and subtracts l_r = a_rj / a_pj times row p_j from every other row r
+// (rows above the pivots too, l_p = 0), so the A part ends as the pivots d_j
+// at (p_j, j).  The record of the elimination -- lmul[j NBM + r] = l_r of
+// step j, piv[j] = p_j, inv[j] = 1 / d_j -- determines A^-1: applied to the
+// identity it gives E, and row j of A^-1 is row p_j of E times inv[j]
+// (gj_identity_cols).  A partial block (nb < 64) is padded with the
+// identity, so its inverse is padded likewise.
 // Wave q owns the A columns [PW q, PW q + PW) and factors them as a panel
 // alone (the searches and the panel updates stay in the wave), publishing
-// each step as soon as its panel is updated: multipliers (0 in the pivot
-// row), pivot row and 1/d_j in LDS, then the step count s_done with a
-// workgroup-scope release.  The waves apply published steps as they come
-// (polling s_done, sleeping between polls): each to its A columns the steps
-// of the panels to its left before its own panel, and the first half of the
-// waves, idle after their panels, all 64 steps to the identity columns
-// (2 PW each), which nothing on the chain waits for.  Issue priority: the
-// factoring wave over the A-column replays over the identity columns.
-// The body runs on 64 * NBM / PW threads; lmul (NBM x NBM: lmul[j NBM + r]),
-// s_inv, s_piv (NBM each) and s_done are LDS.  Xinv: column-major, leading
-// dimension NBM.
+// each step as soon as its panel is updated: the record in LDS (and in rec,
+// global memory, for the kernels that build the inverse), then the step
+// count s_done with a workgroup-scope release.  The waves to its right apply
+// published steps to their columns as they come (polling s_done, sleeping
+// between polls), so when wave q publishes its last step, wave q+1 has that
+// one step left to apply before it factors the next panel: no barrier on
+// the chain.  The factoring wave raises its issue priority over the
+// replaying waves that share its SIMD.
+// The body runs on 64 * NBM / PW threads; lmul (NBM x NBM), s_inv, s_piv
+// (NBM each) and s_done are LDS.
+// Bounded wait until *flag >= target (see wait_count in lu.hip); relaxed
+// polls, the caller issues one acquire fence after the wait.
+__device__ inline void wait_ge(int* flag, int target, int* status) {
+    int it = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (++it > FLAG_SPIN_LIMIT ||
+            (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+             ST_SYNC_TIMEOUT)) {
+            atomicOr(status, ST_SYNC_TIMEOUT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+__device__ inline void publish(int* flag, int value) {
+    __hip_atomic_store(flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wave-level wait for a block flag: every lane polls the same address (one
+// transaction), then one agent-scope acquire.
+__device__ inline void wave_wait_ge(int* flag, int target, int* status) {
+    wait_ge(flag, target, status);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+constexpr size_t REC_BYTES = sizeof(cf) * (NBM * NBM + NBM) + sizeof(int) * NBM;
+__device__ inline cf* rec_lmul(char* rec) { return (cf*)rec; }
+__device__ inline cf* rec_inv(char* rec) { return (cf*)rec + NBM * NBM; }
+__device__ inline int* rec_piv(char* rec) { return (int*)((cf*)rec + NBM * NBM + NBM); }
+
 template <int PW>
-__device__ __forceinline__ void gj_inverse_body(const cf* __restrict__ A, size_t lda, int nb,
-                                                cf* __restrict__ Xinv, int* __restrict__ status,
-                                                cf* lmul, cf* s_inv, int* s_piv, int* s_done) {
-    constexpr int NW = NBM / PW;  // waves = panels
-    constexpr int EC = 2 * PW;    // identity columns of each of the first NW / 2 waves
+__device__ __forceinline__ void gj_factor_body(const cf* __restrict__ A, size_t lda, int nb,
+                                               char* __restrict__ rec, int* __restrict__ status,
+                                               cf* lmul, cf* s_inv, int* s_piv, int* s_done,
+                                               int* __restrict__ gpan = nullptr) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int q = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int first = PW * q;  // this wave's first column
@@ -118,25 +153,25 @@ __device__ __forceinline__ void gj_inverse_body(const cf* __restrict__ A, size_t
     }
     if (tid == 0) *s_done = 0;
     __syncthreads();  // the count is zero before any wave polls it
+    cf* __restrict__ g_lmul = rec_lmul(rec);
+    cf* __restrict__ g_inv = rec_inv(rec);
+    int* __restrict__ g_piv = rec_piv(rec);
+    // the panels to the left, as they are published
+    bool used = false;
     int avail = 0;
-    auto wait_step = [&](int j) {
-        if (j < avail) return;
-        int spins = 0;
-        while ((avail = __hip_atomic_load(s_done, __ATOMIC_ACQUIRE,
-                                          __HIP_MEMORY_SCOPE_WORKGROUP)) <= j) {
-            __builtin_amdgcn_s_sleep(1);
-            if (++spins > FLAG_SPIN_LIMIT) {
-                if (lane == 0) atomicOr(status, ST_SYNC_TIMEOUT);
-                avail = NBM;
-                break;
+    for (int j = 0; j < first; ++j) {
+        if (j >= avail) {
+            int spins = 0;
+            while ((avail = __hip_atomic_load(s_done, __ATOMIC_ACQUIRE,
+                                              __HIP_MEMORY_SCOPE_WORKGROUP)) <= j) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > FLAG_SPIN_LIMIT) {
+                    if (lane == 0) atomicOr(status, ST_SYNC_TIMEOUT);
+                    avail = NBM;
+                    break;
+                }
             }
         }
-    };
-    // the panels to the left, on this wave's A columns
-    bool used = false;
-    __builtin_amdgcn_s_setprio(1);
-    for (int j = 0; j < first; ++j) {
-        wait_step(j);
         const int p = __builtin_amdgcn_readfirstlane(s_piv[j]);
         const cf l = lmul[j * NBM + lane];
         used = used || lane == p;
@@ -167,9 +202,12 @@ __device__ __forceinline__ void gj_inverse_body(const cf* __restrict__ A, size_t
         const cf l = piv ? mkf(0.f, 0.f) : v[i] * inv;
         used = used || piv;
         lmul[j * NBM + lane] = l;
+        g_lmul[j * NBM + lane] = l;
         if (lane == 0) {
             s_piv[j] = p;
             s_inv[j] = inv;
+            g_piv[j] = p;
+            g_inv[j] = inv;
         }
 #pragma unroll
         for (int c = i + 1; c < PW; ++c) v[c] = cfms(v[c], l, readlane_f(v[c], p));
@@ -183,27 +221,55 @@ __device__ __forceinline__ void gj_inverse_body(const cf* __restrict__ A, size_t
     }
     __builtin_amdgcn_s_setprio(0);
     if (zero && lane == 0 && first < nb) atomicOr(status, ST_PIVOT_ZERO);
-    if (q >= NW / 2) return;
-    // identity columns [EC q, EC q + EC), all 64 steps: column m stays e_m
-    // (steps with another pivot row leave it alone) until row m is the
-    // pivot, so a column is updated only from that step on (act: the
-    // wave-uniform set of such columns)
-    cf e[EC];
+    // the panel's record is in memory for other workgroups (an agent-scope
+    // release of this wave's own record stores, after the next wave has
+    // already been handed the panel through LDS)
+    if (gpan && lane == 0) publish(&gpan[q], 1);
+}
+
+// Columns [c0e, c0e + EC) of E (the identity under the 64 recorded steps;
+// record in LDS), lane = row of E; rank = the step whose pivot is this
+// lane's row.  Column m stays e_m (steps with another pivot row leave it
+// alone) until row m is the pivot, so a column is updated only from that
+// step on (act: the wave-uniform set of such columns).  Row rank of A^-1 is
+// then e[] * inv[rank].
+template <int EC>
+__device__ __forceinline__ void gj_identity_cols(const cf* lmul, const int* piv, int c0e,
+                                                 cf (&e)[EC], int& rank) {
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int i = 0; i < EC; ++i) e[i] = mkf(lane == EC * q + i ? 1.f : 0.f, 0.f);
-    int rank = lane;
+    for (int i = 0; i < EC; ++i) e[i] = mkf(lane == c0e + i ? 1.f : 0.f, 0.f);
+    rank = lane;
     unsigned act = 0;
     for (int j = 0; j < NBM; ++j) {
-        wait_step(j);
-        const int p = __builtin_amdgcn_readfirstlane(s_piv[j]);
+        const int p = __builtin_amdgcn_readfirstlane(piv[j]);
         const cf l = lmul[j * NBM + lane];
         rank = lane == p ? j : rank;
-        const int m = p - EC * q;
+        const int m = p - c0e;
         if (m >= 0 && m < EC) act |= 1u << m;
 #pragma unroll
         for (int c = 0; c < EC; ++c)
             if ((act >> c) & 1u) e[c] = cfms(e[c], l, readlane_f(e[c], p));
     }
+}
+
+// The elimination of the diagonal block and, when the inverse is not built
+// from the record by the next step's k_panel_inv (the last block), the
+// inverse itself into Xinv (column-major, leading dimension NBM) after one
+// barrier, every wave taking 64 / waves columns.
+template <int PW>
+__device__ __forceinline__ void gj_block(const cf* __restrict__ A, size_t lda, int nb,
+                                         char* __restrict__ rec, cf* __restrict__ Xinv,
+                                         bool with_inverse, int* __restrict__ status, cf* lmul,
+                                         cf* s_inv, int* s_piv, int* s_done) {
+    gj_factor_body<PW>(A, lda, nb, rec, status, lmul, s_inv, s_piv, s_done);
+    if (!with_inverse) return;
+    __syncthreads();
+    constexpr int EC = PW;  // 64 / (64 / PW waves)
+    const int q = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    cf e[EC];
+    int rank;
+    gj_identity_cols<EC>(lmul, s_piv, EC * q, e, rank);
     const cf sc = s_inv[rank];
 #pragma unroll
     for (int i = 0; i < EC; ++i) Xinv[(size_t)(EC * q + i) * NBM + rank] = e[i] * sc;
@@ -211,32 +277,15 @@ __device__ __forceinline__ void gj_inverse_body(const cf* __restrict__ A, size_t
 
 template <int PW>
 __global__ __launch_bounds__(64 * (NBM / PW)) void k_diag_gj(const cf* __restrict__ A, size_t lda,
-                                                             int nb, cf* __restrict__ Xinv,
+                                                             int nb, char* __restrict__ rec,
+                                                             cf* __restrict__ Xinv,
+                                                             int with_inverse,
                                                              int* __restrict__ status) {
     __shared__ cf lmul[NBM * NBM];
     __shared__ cf s_inv[NBM];
     __shared__ int s_piv[NBM];
     __shared__ int s_done;
-    gj_inverse_body<PW>(A, lda, nb, Xinv, status, lmul, s_inv, s_piv, &s_done);
-}
-
-// Bounded wait until *flag >= target (see wait_count in lu.hip); relaxed
-// polls, the caller issues one acquire fence after the wait.
-__device__ inline void wait_ge(int* flag, int target, int* status) {
-    int it = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        if (++it > FLAG_SPIN_LIMIT ||
-            (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
-             ST_SYNC_TIMEOUT)) {
-            atomicOr(status, ST_SYNC_TIMEOUT);
-            break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-}
-
-__device__ inline void publish(int* flag, int value) {
-    __hip_atomic_store(flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    gj_block<PW>(A, lda, nb, rec, Xinv, with_inverse != 0, status, lmul, s_inv, s_piv, &s_done);
 }
 
 // Wave-uniform copy of lane src's complex double (v_readlane, no LDS).
@@ -246,13 +295,6 @@ __device__ inline cd readlane_c(cd v, int src) {
     int yl = __builtin_amdgcn_readlane((int)by, src), yh = __builtin_amdgcn_readlane((int)(by >> 32), src);
     return mk(__longlong_as_double(((long long)xh << 32) | (unsigned)xl),
               __longlong_as_double(((long long)yh << 32) | (unsigned)yl));
-}
-
-// Wave-level wait for a block flag: every lane polls the same address (one
-// transaction), then one agent-scope acquire.
-__device__ inline void wave_wait_ge(int* flag, int target, int* status) {
-    wait_ge(flag, target, status);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 // ---------------------------------------------------------------------------
@@ -838,7 +880,27 @@ __device__ __forceinline__ void upd2_tile(const UpdRegion& R, int t, const __bf1
                 c1[x][y] = *(const float4*)(cp + 2);
             }
     };
-    if (fast) cload();
+    if (fast) {
+        cload();
+    } else {
+        // partial tile: the same registers, element by element with bounds,
+        // all in flight together (loaded after the stores they would
+        // otherwise wait behind, they serialise)
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                const int col = bn + 32 * wc + 16 * y + li;
+                const int row0 = bm + 64 * wr + 16 * x + lq * 4;
+                cf t[4];
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq)
+                    t[qq] = (col < R.nc && row0 + qq < R.m) ? C[(size_t)col * ldc + row0 + qq]
+                                                            : mkf(0.f, 0.f);
+                c0[x][y] = float4{t[0].x, t[0].y, t[1].x, t[1].y};
+                c1[x][y] = float4{t[2].x, t[2].y, t[3].x, t[3].y};
+            }
+    }
     f4 accR[4][2], accI[4][2];
 #pragma unroll
     for (int x = 0; x < 4; ++x)
@@ -935,13 +997,12 @@ __device__ __forceinline__ void upd2_tile(const UpdRegion& R, int t, const __bf1
                 const int row0 = bm + 64 * wr + 16 * x + lq * 4;
                 if (col >= R.nc) continue;
                 const f4 r = accR[x][y], i = accI[x][y];
+                const cf v[4] = {mkf(c0[x][y].x, c0[x][y].y), mkf(c0[x][y].z, c0[x][y].w),
+                                 mkf(c1[x][y].x, c1[x][y].y), mkf(c1[x][y].z, c1[x][y].w)};
 #pragma unroll
                 for (int qq = 0; qq < 4; ++qq)
-                    if (row0 + qq < R.m) {
-                        cf* cp = C + (size_t)col * ldc + row0 + qq;
-                        const cf v = *cp;
-                        *cp = cf{v.x - r[qq], v.y - i[qq]};
-                    }
+                    if (row0 + qq < R.m)
+                        C[(size_t)col * ldc + row0 + qq] = cf{v[qq].x - r[qq], v[qq].y - i[qq]};
             }
     }
 }
@@ -970,35 +1031,79 @@ __global__ __launch_bounds__(512) void k_cgemm_update2(UpdRegion r0, UpdRegion r
     }
 }
 
-// Update of step k fused with the inversion of diagonal block k+1, which is
-// the top-left nb1 x nb1 corner of r0: workgroup 0 applies the update to
-// that corner itself (one partial tile) and inverts it with
-// gj_inverse_body<8> (LDS of the tile reused) into Xinv1; workgroups 1..
-// update r0 without that corner, and r1.  The next block's inversion thus
-// runs beside the update instead of after it.
+constexpr int XINV_WGS = 8;  // inverse-builder workgroups of k_cgemm_update_inverse
+constexpr int GJ_PANELS = NBM / 8;  // panels of gj_factor_body<8>
+
+// Update of step k fused with the elimination and inversion of diagonal
+// block k+1, which is the top-left nb1 x nb1 corner of r0.  Workgroup 0
+// applies the update to that corner itself (one partial tile) and eliminates
+// it with gj_factor_body<8> (LDS of the tile reused), the record into rec1,
+// raising gpan[q] once panel q of it is in memory; workgroups 1 .. ntile
+// update r0 without that corner, and r1; the last XINV_WGS workgroups build
+// the inverse Xinv1 from the record, one identity column per wave, panel by
+// panel as the flags come up (the column is updated only from its pivot
+// step on), so the inverse is done shortly after the elimination.  The waits
+// cannot block workgroup 0: at most XINV_WGS workgroups wait, far fewer
+// than the CUs.  The next block's elimination and inversion thus run beside
+// the update instead of after it.
 __global__ __launch_bounds__(512) void k_cgemm_update_inverse(UpdRegion r0, UpdRegion r1,
                                                               const __bf16* __restrict__ base,
                                                               size_t pstride, size_t sstride,
                                                               int nks, size_t ldc, int panel,
-                                                              int nb1, cf* __restrict__ Xinv1,
+                                                              int ntile, int nb1,
+                                                              char* __restrict__ rec1,
+                                                              int* __restrict__ gpan,
+                                                              cf* __restrict__ Xinv1,
                                                               int* __restrict__ status) {
     __shared__ __attribute__((aligned(16))) __bf16 lds_a[2 * 4 * PLN];
     __shared__ __attribute__((aligned(16))) __bf16 lds_b[2 * 4 * PLN];
     static_assert(sizeof(lds_a) >= sizeof(cf) * (NBM + 1) * NBM + sizeof(int) * (NBM + 1),
-                  "inversion scratch fits the tile buffer");
-    if (blockIdx.x == 0) {
+                  "elimination scratch fits the tile buffer");
+    const int bid = blockIdx.x;
+    if (bid == 0) {
         const UpdRegion corner{nb1, nb1, r0.r0, r0.c0, r0.C};
         upd2_tile<2>(corner, 0, base, pstride, sstride, nks, ldc, lds_a, lds_b, panel);
-        __syncthreads();  // the corner's stores before the inversion reads it
+        __syncthreads();  // the corner's stores before the elimination reads it
         cf* lmul = (cf*)lds_a;
         cf* s_inv = lmul + NBM * NBM;
         int* s_piv = (int*)(s_inv + NBM);
-        gj_inverse_body<8>(r0.C, ldc, nb1, Xinv1, status, lmul, s_inv, s_piv, s_piv + NBM);
+        gj_factor_body<8>(r0.C, ldc, nb1, rec1, status, lmul, s_inv, s_piv, s_piv + NBM, gpan);
+        return;
+    }
+    if (bid > ntile) {
+        // column m of the inverse from the record
+        const int lane = threadIdx.x & 63;
+        const int m = (bid - ntile - 1) * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+        const cf* __restrict__ g_lmul = rec_lmul(rec1);
+        const int* __restrict__ g_piv = rec_piv(rec1);
+        cf e = mkf(lane == m ? 1.f : 0.f, 0.f);
+        int rank = lane;
+        bool active = false;
+        for (int pn = 0; pn < GJ_PANELS; ++pn) {
+            wave_wait_ge(&gpan[pn], 1, status);
+            cf lv[8];
+            int pv[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                lv[i] = g_lmul[(8 * pn + i) * NBM + lane];
+                pv[i] = __builtin_amdgcn_readfirstlane(g_piv[8 * pn + i]);
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int p = pv[i];
+                rank = lane == p ? 8 * pn + i : rank;
+                active = active || p == m;
+                if (active) e = cfms(e, lv[i], readlane_f(e, p));
+            }
+        }
+        // row rank of A^-1: this lane's row of E times 1 / d_rank
+        const cf sc = rec_inv(rec1)[rank];
+        Xinv1[(size_t)m * NBM + rank] = e * sc;
         return;
     }
     const int t0 = upd2_tiles(r0), tiles = t0 + upd2_tiles(r1);
     const int per = (tiles + NXCD - 1) / NXCD;
-    for (int id = blockIdx.x - 1; id < per * NXCD; id += gridDim.x - 1) {
+    for (int id = bid - 1; id < per * NXCD; id += ntile) {
         const int t = (id % NXCD) * per + id / NXCD;
         if (t < t0)
             upd2_tile<2>(r0, t, base, pstride, sstride, nks, ldc, lds_a, lds_b, panel, t == 0);
@@ -1041,11 +1146,12 @@ static void launch_update(hipStream_t s, const UpdRegion& a, const UpdRegion& b,
 // at its top-left corner) and b.
 static void launch_update_inverse(hipStream_t s, const UpdRegion& a, const UpdRegion& b,
                                   const __bf16* base, size_t pstride, size_t sstride, int nks,
-                                  size_t ldc, int nb1, cf* Xinv1, int* status) {
+                                  size_t ldc, int nb1, char* rec1, int* gpan, cf* Xinv1,
+                                  int* status) {
     const int tiles = upd2_tiles_host(a) + upd2_tiles_host(b);
     const int g = ((tiles + NXCD - 1) / NXCD) * NXCD;
-    hipLaunchKernelGGL(k_cgemm_update_inverse, dim3(1 + g), dim3(512), 0, s, a, b, base, pstride,
-                       sstride, nks, ldc, 4, nb1, Xinv1, status);
+    hipLaunchKernelGGL(k_cgemm_update_inverse, dim3(1 + g + XINV_WGS), dim3(512), 0, s, a, b,
+                       base, pstride, sstride, nks, ldc, 4, g, nb1, rec1, gpan, Xinv1, status);
 }
 
 // Workgroups of the bulk update on the side stream (BEMX_BULK_WGS; default
@@ -1192,8 +1298,8 @@ __global__ void k_fill_i(int* p, int v, int cnt) {
     } while (0)
 
 // Workspace slots 9.. of LuWork (0..8 belong to the FP64 path).
-enum { M_A32 = 9, M_PERM, M_FLAGS, M_STATUS, M_VEC, M_V, M_PART, M_HOST, M_PTMP, M_PLANES, M_INV,
-       M_MINV, M_WC };
+enum { M_A32 = 9, M_REC, M_FLAGS, M_STATUS, M_VEC, M_V, M_PART, M_HOST, M_PTMP, M_PLANES, M_INV,
+       M_MINV, M_WC, M_GJF };
 
 struct MixedFactor {
     cf* A = nullptr;
@@ -1201,6 +1307,8 @@ struct MixedFactor {
     int* status = nullptr;
     int* epoch = nullptr;   // host-side counter living in LuWork
     cf* inv = nullptr;      // per block: inverse of its diagonal block (column-major 64 x 64)
+    char* rec = nullptr;    // two elimination records (steps k even / odd)
+    int* gjflag = nullptr;  // per block and panel: that part of its record is in memory
     cf* minv = nullptr;     // per 128-row pair: lower inverse, then (after npair) upper
     cf* wc = nullptr;       // per pair: lower coupling Wl, then (after npair) upper Wu
     int n = 0, nblk = 0, npair = 0, coop = 1;
@@ -1223,7 +1331,7 @@ static int group_blocks() {
 size_t mixed_factor_extra_bytes(int n) {
     const size_t nb = (size_t)(n + NBM - 1) / NBM, np = (nb + 1) / 2;
     const size_t planes = sizeof(__bf16) * (size_t)n * NBM * 8 * 2 * group_blocks();
-    const size_t inv = sizeof(cf) * nb * NBM * NBM + sizeof(cf) * np * NB2 * NB2 * 4;
+    const size_t inv = sizeof(cf) * nb * NBM * NBM + sizeof(cf) * np * NB2 * NB2 * 4 + 2 * REC_BYTES;
     return planes + inv;
 }
 
@@ -1242,7 +1350,7 @@ size_t mixed_factor_extra_bytes(int n) {
 //         all columns to the right (k_panel_inv, bf16 planes into the
 //         group's slot of k); then the update of the group's own columns
 //         (rows below k) and of the group's own rows (columns beyond gend),
-//         K = 64, in one launch with the inversion of block k+1
+//         K = 64, in one launch with the elimination of block k+1
 //         (k_cgemm_update_inverse; block 0 by k_diag_gj alone).  (A
 //         left-looking form, one K = 64 j update of block column and row k
 //         per step, was measured 0.7 ms slower per 6480-DOF solve: its
@@ -1250,7 +1358,7 @@ size_t mixed_factor_extra_bytes(int n) {
 // After the last step of group g, with K = W from the group's planes:
 //   st:   [wait Z(g-1)] X(g): the next group's columns, all rows below gend;
 //         Y(g): the next group's rows, the columns beyond it; with the
-//         inversion of the next group's first block in the same launch;
+//         elimination of the next group's first block in the same launch;
 //   side: Z(g): everything beyond the next group (the bulk),
 // so group g+1 is factored while the bulk update of group g runs.  Planes
 // are double-buffered by group parity: group g+1 writes the slots Z(g-1) read,
@@ -1265,6 +1373,10 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
     const int sg = group_blocks();
     const size_t sstride = 8 * pstride;  // one slot: 4 L21 planes, 4 U12 planes
     hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, st, F.status, 0, 1);
+    // elimination-record flags (per block and panel), raised by
+    // k_cgemm_update_inverse
+    hipLaunchKernelGGL(k_fill_i, dim3((nblk * GJ_PANELS + 255) / 256), dim3(256), 0, st, F.gjflag,
+                       0, nblk * GJ_PANELS);
     size_t tot = (size_t)n * n;
     const size_t ld = F.ld;
     hipLaunchKernelGGL(k_cvt_z2c, dim3((tot + 255) / 256), dim3(256), 0, st, A64, lda, F.A,
@@ -1282,10 +1394,12 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
         __bf16* gbase = planes + (size_t)(g & 1) * sg * sstride;
         __bf16* la = gbase + (size_t)j * sstride;
         __bf16* ub = la + 4 * pstride;
-        // block 0 alone; block k+1 is inverted inside step k's update
+        char* rec1 = F.rec + (size_t)((k + 1) & 1) * REC_BYTES;
+        // block 0 alone; block k+1 is eliminated and inverted inside step
+        // k's update
         if (k == 0)
             hipLaunchKernelGGL(k_diag_gj<4>, dim3(1), dim3(1024), 0, st, (const cf*)F.A, ld, nb,
-                               xinv, F.status);
+                               F.rec, xinv, 1, F.status);
         if (rest <= 0) break;
         const int nrb = (rest + TRR_ROWS - 1) / TRR_ROWS;
         const int ncb = (rest + NBM - 1) / NBM;
@@ -1293,12 +1407,13 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
                            rest, (const cf*)xinv, la, ub, pstride);
         const int nb1 = std::min(NBM, rest);
         cf* xinv1 = xinv + NBM * NBM;
+        int* gjf1 = F.gjflag + (size_t)(k + 1) * GJ_PANELS;
         if (c0 < gend) {
             // inside the group: its columns right of k, and its rows below k
             const UpdRegion cs{rest, gend - c0, c0, c0, at(c0, c0)};
             const UpdRegion rs{gend - c0, n - gend, c0, gend, at(c0, gend)};
             launch_update_inverse(st, cs, n > gend ? rs : none, la, pstride, sstride, 1, ld, nb1,
-                                  xinv1, F.status);
+                                  rec1, gjf1, xinv1, F.status);
             continue;
         }
         // last block of group g: apply the group to the rest, K = 64 * sg
@@ -1308,7 +1423,7 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
         const UpdRegion X{n - gend, nend - gend, gend, gend, at(gend, gend)};
         const UpdRegion Y{nend - gend, n - nend, gend, nend, at(gend, nend)};
         launch_update_inverse(st, X, n > nend ? Y : none, gbase, pstride, sstride, sg, ld, nb1,
-                              xinv1, F.status);
+                              rec1, gjf1, xinv1, F.status);
         if (n > nend) {
             const UpdRegion Z{n - nend, n - nend, nend, nend, at(nend, nend)};
             chk(hipEventRecord(w.ev_panel, st));
@@ -1341,6 +1456,10 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     F.A = (cf*)w.get(M_A32, sizeof(cf) * F.ld * n, &e);
     MX_CHK(e);
     F.inv = (cf*)w.get(M_INV, sizeof(cf) * (size_t)nblk * NBM * NBM, &e);
+    MX_CHK(e);
+    F.rec = (char*)w.get(M_REC, 2 * REC_BYTES, &e);
+    MX_CHK(e);
+    F.gjflag = (int*)w.get(M_GJF, sizeof(int) * (size_t)nblk * GJ_PANELS, &e);
     MX_CHK(e);
     F.npair = (nblk + 1) / 2;
     F.minv = (cf*)w.get(M_MINV, sizeof(cf) * (size_t)F.npair * NB2 * NB2 * 2, &e);
