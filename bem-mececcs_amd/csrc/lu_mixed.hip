@@ -679,66 +679,28 @@ __device__ inline bf8 neg_bf8(bf8 x) {
 // staged in LDS and stored to A and, split, to the planes la, both
 // coalesced.  Column blocks only split A12 into the planes ub.
 constexpr int TRR_ROWS = 64;
-__global__ __launch_bounds__(256) void k_panel_inv(cf* __restrict__ A, size_t lda, int k0, int c0,
-                                                   int nr, int nc, const cf* __restrict__ Xinv,
-                                                   __bf16* __restrict__ la,
-                                                   __bf16* __restrict__ ub, size_t pstride) {
-    constexpr int XS = NBM + 1;
-    constexpr int BK = NBM + 8;  // padded k stride of the LDS planes
-    __shared__ __attribute__((aligned(16))) __bf16 Bp[4][NBM * BK];  // [pl][col BK + k]
-    __shared__ cf X[NBM * XS];                                       // X[col XS + row]
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int nrb = (nr + TRR_ROWS - 1) / TRR_ROWS;
-    if ((int)blockIdx.x >= nrb) {
-        const int cb = c0 + ((int)blockIdx.x - nrb) * NBM;
-#pragma unroll 4
-        for (int t = tid; t < NBM * NBM; t += 256) {
-            const int j = t >> 6, k = t & 63;
-            if (cb + j < c0 + nc)
-                store_planes(ub, pstride, (size_t)(cb + j) * NBM + k,
-                             A[(size_t)(cb + j) * lda + k0 + k]);
-        }
-        return;
-    }
-    const int rb = c0 + (int)blockIdx.x * TRR_ROWS;
-    const int li = lane & 15, lq = lane >> 4;
-    const int row = rb + 16 * w + li;
-    const bool rok = row < c0 + nr;
-    // A operand: k = 32 kc + 8 lq + e of this lane's row, loads in flight
-    // while Xinv is split into LDS
-    cf ax[2][8];
-#pragma unroll
-    for (int kc = 0; kc < 2; ++kc)
-#pragma unroll
-        for (int e = 0; e < 8; ++e)
-            ax[kc][e] = rok ? A[(size_t)(k0 + 32 * kc + 8 * lq + e) * lda + row] : mkf(0.f, 0.f);
-#pragma unroll 4
-    for (int t = tid; t < NBM * NBM; t += 256) {
-        const int col = t >> 6, k = t & 63;  // Xinv[col 64 + k] = Xinv[k][col]
-        const cf x = Xinv[t];
-        __bf16 h, l;
-        split_bf16(x.x, h, l);
-        Bp[0][col * BK + k] = h;
-        Bp[1][col * BK + k] = l;
-        split_bf16(x.y, h, l);
-        Bp[2][col * BK + k] = h;
-        Bp[3][col * BK + k] = l;
-    }
-    bf8 fa[2][4];
-#pragma unroll
-    for (int kc = 0; kc < 2; ++kc)
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            __bf16 h, l;
-            split_bf16(ax[kc][e].x, h, l);
-            fa[kc][0][e] = h;
-            fa[kc][1][e] = l;
-            split_bf16(ax[kc][e].y, h, l);
-            fa[kc][2][e] = h;
-            fa[kc][3][e] = l;
-        }
-    __syncthreads();
-    f4 accR[4], accI[4];
+constexpr int PI_BK = NBM + 8;  // padded k stride of the LDS planes
+
+// Split one complex float into the four bf16 planes of an MFMA operand
+// (element e of lane's 8 consecutive k).
+__device__ __forceinline__ void split_into(bf8 (&f)[4], int e, cf x) {
+    __bf16 h, l;
+    split_bf16(x.x, h, l);
+    f[0][e] = h;
+    f[1][e] = l;
+    split_bf16(x.y, h, l);
+    f[2][e] = h;
+    f[3][e] = l;
+}
+
+// 16 rows x 64 columns, K = 64, of a complex product on MFMA: A operand fa
+// (rows 16 w.., this lane's 8 consecutive k per 32-wide chunk kc, split),
+// B operand from the LDS planes Bp (plane stride NBM PI_BK, element
+// [col PI_BK + k]); 12 MFMA per complex 16x16x32 block (split-BF16x3).
+// D layout: lane -> column 16 y + (lane & 15), rows 4 (lane >> 4) + (0..3).
+__device__ __forceinline__ void cmma_16x64(const bf8 (&fa)[2][4], const __bf16* __restrict__ Bp,
+                                           f4 (&accR)[4], f4 (&accI)[4]) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lq = lane >> 4;
 #pragma unroll
     for (int y = 0; y < 4; ++y) {
         accR[y] = f4{0.f, 0.f, 0.f, 0.f};
@@ -752,7 +714,7 @@ __global__ __launch_bounds__(256) void k_panel_inv(cf* __restrict__ A, size_t ld
             bf8 fb[4];
 #pragma unroll
             for (int pl = 0; pl < 4; ++pl)
-                fb[pl] = *(const bf8*)&Bp[pl][(16 * y + li) * BK + 32 * kc + 8 * lq];
+                fb[pl] = *(const bf8*)&Bp[pl * NBM * PI_BK + (16 * y + li) * PI_BK + 32 * kc + 8 * lq];
             f4 r = accR[y], i = accI[y];
             // + Ar Br - Ai Bi
             r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[kc][0], fb[0], r, 0, 0, 0);
@@ -772,13 +734,121 @@ __global__ __launch_bounds__(256) void k_panel_inv(cf* __restrict__ A, size_t ld
             accI[y] = i;
         }
     }
-    // D layout: lane -> column 16 y + li, rows 16 w + 4 lq + (0..3)
+}
+
+// Workgroup 0 of a step inside a group also applies the step to the next
+// diagonal block (corner = its size, 0 otherwise): that block has only this
+// step's update pending, its rows are the workgroup's rows, and U12 there is
+// A12 as it stands, so corner -= L21 A12 is one more 64 x 64 x 64 product
+// (the split-BF16x3 product of the trailing update, the same operands).
+__global__ __launch_bounds__(256) void k_panel_inv(cf* __restrict__ A, size_t lda, int k0, int c0,
+                                                   int nr, int nc, const cf* __restrict__ Xinv,
+                                                   __bf16* __restrict__ la,
+                                                   __bf16* __restrict__ ub, size_t pstride,
+                                                   int corner) {
+    constexpr int XS = NBM + 1;
+    __shared__ __attribute__((aligned(16))) __bf16 Bp[4 * NBM * PI_BK];  // [pl][col BK + k]
+    __shared__ cf X[NBM * XS];                                           // X[col XS + row]
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int nrb = (nr + TRR_ROWS - 1) / TRR_ROWS;
+    if ((int)blockIdx.x >= nrb) {
+        const int cb = c0 + ((int)blockIdx.x - nrb) * NBM;
+#pragma unroll 4
+        for (int t = tid; t < NBM * NBM; t += 256) {
+            const int j = t >> 6, k = t & 63;
+            if (cb + j < c0 + nc)
+                store_planes(ub, pstride, (size_t)(cb + j) * NBM + k,
+                             A[(size_t)(cb + j) * lda + k0 + k]);
+        }
+        return;
+    }
+    const int rb = c0 + (int)blockIdx.x * TRR_ROWS;
+    const int li = lane & 15, lq = lane >> 4;
+    const int row = rb + 16 * w + li;
+    const bool rok = row < c0 + nr;
+    const int cn = blockIdx.x == 0 ? corner : 0;  // uniform
+    // A operand: k = 32 kc + 8 lq + e of this lane's row, loads in flight
+    // while Xinv is split into LDS
+    cf ax[2][8];
+#pragma unroll
+    for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            ax[kc][e] = rok ? A[(size_t)(k0 + 32 * kc + 8 * lq + e) * lda + row] : mkf(0.f, 0.f);
+    // the corner's A12 block (uq[q]: column t >> 6, k = t & 63 of t = tid +
+    // 256 q) and its current values in the D layout, in flight as well
+    cf uq[16], cc[4][4];
+    if (cn > 0) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int t = tid + 256 * q, j = t >> 6, k = t & 63;
+            uq[q] = j < cn ? A[(size_t)(c0 + j) * lda + k0 + k] : mkf(0.f, 0.f);
+        }
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = 16 * w + 4 * lq + e, c = 16 * y + li;
+                cc[y][e] = (r < cn && c < cn) ? A[(size_t)(c0 + c) * lda + c0 + r] : mkf(0.f, 0.f);
+            }
+    }
+#pragma unroll 4
+    for (int t = tid; t < NBM * NBM; t += 256) {
+        const int col = t >> 6, k = t & 63;  // Xinv[col 64 + k] = Xinv[k][col]
+        const cf x = Xinv[t];
+        __bf16 h, l;
+        split_bf16(x.x, h, l);
+        Bp[0 * NBM * PI_BK + col * PI_BK + k] = h;
+        Bp[1 * NBM * PI_BK + col * PI_BK + k] = l;
+        split_bf16(x.y, h, l);
+        Bp[2 * NBM * PI_BK + col * PI_BK + k] = h;
+        Bp[3 * NBM * PI_BK + col * PI_BK + k] = l;
+    }
+    bf8 fa[2][4];
+#pragma unroll
+    for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) split_into(fa[kc], e, ax[kc][e]);
+    __syncthreads();
+    f4 accR[4], accI[4];
+    cmma_16x64(fa, Bp, accR, accI);
 #pragma unroll
     for (int y = 0; y < 4; ++y)
 #pragma unroll
         for (int e = 0; e < 4; ++e)
             X[(16 * y + li) * XS + 16 * w + 4 * lq + e] = mkf(accR[y][e], accI[y][e]);
     __syncthreads();
+    if (cn > 0) {
+        // corner -= L21 A12: A12 into the planes (Bp is free), L21 rows of
+        // this workgroup from X
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int t = tid + 256 * q, j = t >> 6, k = t & 63;
+            __bf16 h, l;
+            split_bf16(uq[q].x, h, l);
+            Bp[0 * NBM * PI_BK + j * PI_BK + k] = h;
+            Bp[1 * NBM * PI_BK + j * PI_BK + k] = l;
+            split_bf16(uq[q].y, h, l);
+            Bp[2 * NBM * PI_BK + j * PI_BK + k] = h;
+            Bp[3 * NBM * PI_BK + j * PI_BK + k] = l;
+        }
+#pragma unroll
+        for (int kc = 0; kc < 2; ++kc)
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                split_into(fa[kc], e, X[(32 * kc + 8 * lq + e) * XS + 16 * w + li]);
+        __syncthreads();
+        cmma_16x64(fa, Bp, accR, accI);
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = 16 * w + 4 * lq + e, c = 16 * y + li;
+                if (r < cn && c < cn)
+                    A[(size_t)(c0 + c) * lda + c0 + r] =
+                        mkf(cc[y][e].x - accR[y][e], cc[y][e].y - accI[y][e]);
+            }
+    }
     const bool ok = rb + lane < c0 + nr;
 #pragma unroll 4
     for (int c = w; c < NBM; c += 4)
@@ -1036,8 +1106,9 @@ constexpr int GJ_PANELS = NBM / 8;  // panels of gj_factor_body<8>
 
 // Update of step k fused with the elimination and inversion of diagonal
 // block k+1, which is the top-left nb1 x nb1 corner of r0.  Workgroup 0
-// applies the update to that corner itself (one partial tile) and eliminates
-// it with gj_factor_body<8> (LDS of the tile reused), the record into rec1,
+// applies the update to that corner itself (one partial tile; with
+// do_corner = 0 k_panel_inv has done it) and eliminates it with
+// gj_factor_body<8> (LDS of the tile reused), the record into rec1,
 // raising gpan[q] once panel q of it is in memory; workgroups 1 .. ntile
 // update r0 without that corner, and r1; the last XINV_WGS workgroups build
 // the inverse Xinv1 from the record, one identity column per wave, panel by
@@ -1051,6 +1122,7 @@ __global__ __launch_bounds__(512) void k_cgemm_update_inverse(UpdRegion r0, UpdR
                                                               size_t pstride, size_t sstride,
                                                               int nks, size_t ldc, int panel,
                                                               int ntile, int nb1,
+                                                              int do_corner,
                                                               char* __restrict__ rec1,
                                                               int* __restrict__ gpan,
                                                               cf* __restrict__ Xinv1,
@@ -1061,9 +1133,11 @@ __global__ __launch_bounds__(512) void k_cgemm_update_inverse(UpdRegion r0, UpdR
                   "elimination scratch fits the tile buffer");
     const int bid = blockIdx.x;
     if (bid == 0) {
-        const UpdRegion corner{nb1, nb1, r0.r0, r0.c0, r0.C};
-        upd2_tile<2>(corner, 0, base, pstride, sstride, nks, ldc, lds_a, lds_b, panel);
-        __syncthreads();  // the corner's stores before the elimination reads it
+        if (do_corner) {
+            const UpdRegion corner{nb1, nb1, r0.r0, r0.c0, r0.C};
+            upd2_tile<2>(corner, 0, base, pstride, sstride, nks, ldc, lds_a, lds_b, panel);
+            __syncthreads();  // the corner's stores before the elimination reads it
+        }
         cf* lmul = (cf*)lds_a;
         cf* s_inv = lmul + NBM * NBM;
         int* s_piv = (int*)(s_inv + NBM);
@@ -1146,12 +1220,13 @@ static void launch_update(hipStream_t s, const UpdRegion& a, const UpdRegion& b,
 // at its top-left corner) and b.
 static void launch_update_inverse(hipStream_t s, const UpdRegion& a, const UpdRegion& b,
                                   const __bf16* base, size_t pstride, size_t sstride, int nks,
-                                  size_t ldc, int nb1, char* rec1, int* gpan, cf* Xinv1,
-                                  int* status) {
+                                  size_t ldc, int nb1, bool do_corner, char* rec1, int* gpan,
+                                  cf* Xinv1, int* status) {
     const int tiles = upd2_tiles_host(a) + upd2_tiles_host(b);
     const int g = ((tiles + NXCD - 1) / NXCD) * NXCD;
     hipLaunchKernelGGL(k_cgemm_update_inverse, dim3(1 + g + XINV_WGS), dim3(512), 0, s, a, b,
-                       base, pstride, sstride, nks, ldc, 4, g, nb1, rec1, gpan, Xinv1, status);
+                       base, pstride, sstride, nks, ldc, 4, g, nb1, (int)do_corner, rec1, gpan,
+                       Xinv1, status);
 }
 
 // Workgroups of the bulk update on the side stream (BEMX_BULK_WGS; default
@@ -1403,17 +1478,19 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
         if (rest <= 0) break;
         const int nrb = (rest + TRR_ROWS - 1) / TRR_ROWS;
         const int ncb = (rest + NBM - 1) / NBM;
-        hipLaunchKernelGGL(k_panel_inv, dim3(nrb + ncb), dim3(256), 0, st, F.A, ld, k0, c0, rest,
-                           rest, (const cf*)xinv, la, ub, pstride);
         const int nb1 = std::min(NBM, rest);
+        // inside a group the next diagonal block is updated by k_panel_inv
+        const bool in_group = c0 < gend;
+        hipLaunchKernelGGL(k_panel_inv, dim3(nrb + ncb), dim3(256), 0, st, F.A, ld, k0, c0, rest,
+                           rest, (const cf*)xinv, la, ub, pstride, in_group ? nb1 : 0);
         cf* xinv1 = xinv + NBM * NBM;
         int* gjf1 = F.gjflag + (size_t)(k + 1) * GJ_PANELS;
-        if (c0 < gend) {
+        if (in_group) {
             // inside the group: its columns right of k, and its rows below k
             const UpdRegion cs{rest, gend - c0, c0, c0, at(c0, c0)};
             const UpdRegion rs{gend - c0, n - gend, c0, gend, at(c0, gend)};
             launch_update_inverse(st, cs, n > gend ? rs : none, la, pstride, sstride, 1, ld, nb1,
-                                  rec1, gjf1, xinv1, F.status);
+                                  false, rec1, gjf1, xinv1, F.status);
             continue;
         }
         // last block of group g: apply the group to the rest, K = 64 * sg
@@ -1423,7 +1500,7 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
         const UpdRegion X{n - gend, nend - gend, gend, gend, at(gend, gend)};
         const UpdRegion Y{nend - gend, n - nend, gend, nend, at(gend, nend)};
         launch_update_inverse(st, X, n > nend ? Y : none, gbase, pstride, sstride, sg, ld, nb1,
-                              rec1, gjf1, xinv1, F.status);
+                              true, rec1, gjf1, xinv1, F.status);
         if (n > nend) {
             const UpdRegion Z{n - nend, n - nend, nend, nend, at(nend, nend)};
             chk(hipEventRecord(w.ev_panel, st));
