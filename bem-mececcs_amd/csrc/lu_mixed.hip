@@ -107,13 +107,16 @@ __device__ inline cf readlane_f(cf v, int src) {
 // The body runs on 64 * NBM / PW threads; lmul (NBM x NBM), s_inv, s_piv
 // (NBM each) and s_done are LDS.
 // Bounded wait until *flag >= target (see wait_count in lu.hip); relaxed
-// polls, the caller issues one acquire fence after the wait.
+// polls, the caller issues one acquire fence after the wait.  Another
+// workgroup's time-out (status) is looked at every 64 polls only, so a poll
+// is one memory round trip.
 __device__ inline void wait_ge(int* flag, int target, int* status) {
     int it = 0;
     while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        if (++it > FLAG_SPIN_LIMIT ||
-            (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
-             ST_SYNC_TIMEOUT)) {
+        if ((++it & 63) == 0 &&
+            (it > FLAG_SPIN_LIMIT ||
+             (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+              ST_SYNC_TIMEOUT))) {
             atomicOr(status, ST_SYNC_TIMEOUT);
             break;
         }
@@ -491,16 +494,63 @@ __global__ __launch_bounds__(256) void k_pair_coupling(const cf* __restrict__ A,
             W[(size_t)(cq * 32 + 4 * tc + c) * NB2 + 4 * tr + r] = acc[r][c];
 }
 
-// y = L^-1 P b over 128-row blocks.  Workgroup g owns blocks g, g+G, ...;
+// Hand-off of a block's solution to the next blocks of the chain without a
+// separate flag: entry i also goes to ll[4 i .. 4 i + 3] as four 64-bit words
+// {32-bit half of re / im, tag}, each stored as one agent-scope atomic, and
+// the reader polls the words themselves until all four carry its tag (the
+// launch epoch), so seeing the data and reading it is one memory round trip
+// instead of flag, fence, data.
+typedef unsigned long long u64;
+__device__ inline void ll_put(u64* __restrict__ ll, int i, cd v, int tag) {
+    const u64 t = (u64)(unsigned)tag << 32;
+    const u64 bx = (u64)__double_as_longlong(v.x), by = (u64)__double_as_longlong(v.y);
+    u64* p = ll + 4 * (size_t)i;
+    __hip_atomic_store(p + 0, t | (bx & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 1, t | (bx >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 2, t | (by & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 3, t | (by >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Entry i of this lane once every lane of the wave has its entry with the
+// tag (bounded like wait_ge).
+__device__ inline cd ll_wait(u64* __restrict__ ll, int i, int tag, int* status) {
+    u64* p = ll + 4 * (size_t)i;
+    u64 w0, w1, w2, w3;
+    int it = 0;
+    for (;;) {
+        w0 = __hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        w1 = __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        w2 = __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        w3 = __hip_atomic_load(p + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned t = (unsigned)tag;
+        const bool ok = (unsigned)(w0 >> 32) == t && (unsigned)(w1 >> 32) == t &&
+                        (unsigned)(w2 >> 32) == t && (unsigned)(w3 >> 32) == t;
+        if (__all(ok)) break;
+        if ((++it & 63) == 0 &&
+            (it > FLAG_SPIN_LIMIT ||
+             (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+              ST_SYNC_TIMEOUT))) {
+            atomicOr(status, ST_SYNC_TIMEOUT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return mk(__longlong_as_double((long long)(((w1 & 0xffffffffull) << 32) | (w0 & 0xffffffffull))),
+              __longlong_as_double((long long)(((w3 & 0xffffffffull) << 32) | (w2 & 0xffffffffull))));
+}
+
+// y = L^-1 b over 128-row blocks.  Workgroup g owns blocks g, g+G, ...;
 // wave w = (row half rh, column quarter q) accumulates its 64 rows x 32
-// columns of every finished block as that block's flag comes up (factor
-// tile loaded before the wait), then y_blk = Minv acc.  Blocks only wait on
+// columns of every finished block as that block's flag (for the block two
+// back: its hand-off words) comes up (factor tile loaded before the wait),
+// then y_blk = Minv acc - W y_{blk-1} with the previous block's words.  Blocks only wait on
 // lower-numbered blocks and every workgroup is resident (cooperative
 // launch), so the chain always drains.
 __device__ __forceinline__ void trsv_lower_body(const cf* __restrict__ A, size_t lda, int n,
                                                 int nblk2, const cf* __restrict__ Minv,
                                                 const cf* __restrict__ Wc,
                                                 const cd* __restrict__ b, cd* __restrict__ y,
+                                                u64* __restrict__ yll,
                                                 int* __restrict__ flags, int epoch,
                                                 int* __restrict__ status, cd (*part)[NBM],
                                                 cd* acc) {
@@ -520,8 +570,16 @@ __device__ __forceinline__ void trsv_lower_body(const cf* __restrict__ A, size_t
             cf l[32];  // the factor tile does not depend on the flag: load it first
 #pragma unroll
             for (int i = 0; i < 32; ++i) l[i] = Lc[(size_t)i * lda];
-            wave_wait_ge(&flags[k], epoch, status);
-            s = s + dot32(l, y[c0 + (lane & 31)]);
+            // the block two back comes through the hand-off words too: the
+            // rest of this block's work before the coupling product hangs on it
+            cd yk;
+            if (k == blk - 2) {
+                yk = ll_wait(yll, c0 + (lane & 31), epoch, status);
+            } else {
+                wave_wait_ge(&flags[k], epoch, status);
+                yk = y[c0 + (lane & 31)];
+            }
+            s = s + dot32(l, yk);
         }
         cf wv[32];  // coupling to the previous block, loaded before its wait
         if (blk > 0) {
@@ -538,17 +596,21 @@ __device__ __forceinline__ void trsv_lower_body(const cf* __restrict__ A, size_t
         __syncthreads();
         cd z = diag128_apply(mv, acc, part, w, lane);
         if (blk > 0) {
-            wave_wait_ge(&flags[blk - 1], epoch, status);
-            const cd t = dot32(wv, y[(blk - 1) * NB2 + 32 * q + (lane & 31)]);
+            const cd yl = ll_wait(yll, (blk - 1) * NB2 + 32 * q + (lane & 31), epoch, status);
+            const cd t = dot32(wv, yl);
             part[w][lane] = t;
             __syncthreads();
             if (q == 0)
                 z = z - ((part[4 * rh][lane] + part[4 * rh + 1][lane]) +
                          (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]));
         }
-        if (q == 0 && row < nb) y[r0 + row] = z;
+        if (q == 0 && row < nb) {
+            ll_put(yll, r0 + row, z, epoch);  // the chain's hand-off first
+            y[r0 + row] = z;
+        }
         // the barrier completes these stores; the release store of the flag
-        // then makes them visible at device scope
+        // then makes them visible at device scope (for the blocks three and
+        // more ahead and the upper solve)
         __syncthreads();
         if (tid == 0) publish(&flags[blk], epoch);
     }
@@ -562,6 +624,7 @@ __device__ __forceinline__ void trsv_upper_body(const cf* __restrict__ A, size_t
                                                 int nblk2, const cf* __restrict__ Minv,
                                                 const cf* __restrict__ Wc,
                                                 const cd* __restrict__ yv, cd* __restrict__ x,
+                                                u64* __restrict__ yll,
                                                 int* __restrict__ flags, int epoch, int ylow,
                                                 int* __restrict__ status, cd (*part)[NBM],
                                                 cd* acc) {
@@ -585,12 +648,26 @@ __device__ __forceinline__ void trsv_upper_body(const cf* __restrict__ A, size_t
                 cf u[32];  // the factor tile does not depend on the flag: load it first
 #pragma unroll
                 for (int i = 0; i < 32; ++i) u[i] = Uc[(size_t)i * lda];
-                wave_wait_ge(&flags[k], epoch, status);
-                s = s + dot32(u, x[c0 + (lane & 31)]);
+                cd xk;  // the block two ahead through the hand-off words
+                if (k == blk + 2) {
+                    xk = ll_wait(yll, c0 + (lane & 31), epoch, status);
+                } else {
+                    wave_wait_ge(&flags[k], epoch, status);
+                    xk = x[c0 + (lane & 31)];
+                }
+                s = s + dot32(u, xk);
             } else {
-                wave_wait_ge(&flags[k], epoch, status);
-                if (cnt <= 0) continue;
-                const cd yl = x[c0 + min(lane & 31, cnt - 1)];
+                if (cnt <= 0) {
+                    wave_wait_ge(&flags[k], epoch, status);
+                    continue;
+                }
+                cd yl;
+                if (k == blk + 2) {
+                    yl = ll_wait(yll, c0 + min(lane & 31, cnt - 1), epoch, status);
+                } else {
+                    wave_wait_ge(&flags[k], epoch, status);
+                    yl = x[c0 + min(lane & 31, cnt - 1)];
+                }
                 for (int i = 0; i < cnt; ++i) s = s + up(Uc[(size_t)i * lda]) * readlane_c(yl, i);
             }
         }
@@ -611,17 +688,21 @@ __device__ __forceinline__ void trsv_upper_body(const cf* __restrict__ A, size_t
         __syncthreads();
         cd z = diag128_apply(mv, acc, part, w, lane);
         if (next) {
-            wave_wait_ge(&flags[blk + 1], epoch, status);
             // columns of a partial next block have zero coupling; their
             // index is clamped into x
-            const cd t = dot32(wv, x[min((blk + 1) * NB2 + 32 * q + (lane & 31), n - 1)]);
+            const cd xl =
+                ll_wait(yll, min((blk + 1) * NB2 + 32 * q + (lane & 31), n - 1), epoch, status);
+            const cd t = dot32(wv, xl);
             part[w][lane] = t;
             __syncthreads();
             if (q == 0)
                 z = z - ((part[4 * rh][lane] + part[4 * rh + 1][lane]) +
                          (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]));
         }
-        if (q == 0 && row < nb) x[r0 + row] = z;
+        if (q == 0 && row < nb) {
+            ll_put(yll, r0 + row, z, epoch);
+            x[r0 + row] = z;
+        }
         __syncthreads();
         if (tid == 0) publish(&flags[blk], epoch);
     }
@@ -636,12 +717,13 @@ __global__ __launch_bounds__(512) void k_trsv_coop(const cf* __restrict__ A, siz
                                                    const cf* __restrict__ Mu,
                                                    const cf* __restrict__ Wu,
                                                    const cd* __restrict__ v, cd* __restrict__ tmp,
-                                                   cd* __restrict__ z, int* __restrict__ flags,
-                                                   int e1, int e2, int* __restrict__ status) {
+                                                   cd* __restrict__ z, u64* __restrict__ yll,
+                                                   int* __restrict__ flags, int e1, int e2,
+                                                   int* __restrict__ status) {
     __shared__ cd part[8][NBM];
     __shared__ cd acc[NB2];
-    trsv_lower_body(A, lda, n, nblk2, Ml, Wl, v, tmp, flags, e1, status, part, acc);
-    trsv_upper_body(A, lda, n, nblk2, Mu, Wu, tmp, z, flags, e2, e1, status, part, acc);
+    trsv_lower_body(A, lda, n, nblk2, Ml, Wl, v, tmp, yll, flags, e1, status, part, acc);
+    trsv_upper_body(A, lda, n, nblk2, Mu, Wu, tmp, z, yll, flags, e2, e1, status, part, acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1374,7 +1456,7 @@ __global__ void k_fill_i(int* p, int v, int cnt) {
 
 // Workspace slots 9.. of LuWork (0..8 belong to the FP64 path).
 enum { M_A32 = 9, M_REC, M_FLAGS, M_STATUS, M_VEC, M_V, M_PART, M_HOST, M_PTMP, M_PLANES, M_INV,
-       M_MINV, M_WC, M_GJF };
+       M_MINV, M_WC, M_GJF, M_LL };
 
 struct MixedFactor {
     cf* A = nullptr;
@@ -1384,6 +1466,7 @@ struct MixedFactor {
     cf* inv = nullptr;      // per block: inverse of its diagonal block (column-major 64 x 64)
     char* rec = nullptr;    // two elimination records (steps k even / odd)
     int* gjflag = nullptr;  // per block and panel: that part of its record is in memory
+    unsigned long long* yll = nullptr;  // tagged hand-off words of the block solves
     cf* minv = nullptr;     // per 128-row pair: lower inverse, then (after npair) upper
     cf* wc = nullptr;       // per pair: lower coupling Wl, then (after npair) upper Wu
     int n = 0, nblk = 0, npair = 0, coop = 1;
@@ -1552,6 +1635,16 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
         w.flags_init = F.flags;
     }
     F.epoch = &w.epoch;
+    {
+        // tags are launch epochs (increasing from 1): only fresh memory needs
+        // clearing, old tags never match again
+        const size_t cap_ll = w.cap[M_LL];
+        F.yll = (u64*)w.get(M_LL, sizeof(u64) * 4 * (size_t)n, &e);
+        MX_CHK(e);
+        if (w.cap[M_LL] != cap_ll)
+            hipLaunchKernelGGL(k_fill_i, dim3((8 * n + 255) / 256), dim3(256), 0, st, (int*)F.yll, 0,
+                               8 * n);
+    }
     F.status = (int*)w.get(M_STATUS, sizeof(int), &e);
     MX_CHK(e);
     int dev = 0, ncu = 0, per = 0;
@@ -1616,8 +1709,9 @@ static hipError_t precond(hipStream_t st, const MixedFactor& F, const cd* v, cd*
     int* status = F.status;
     int e1 = ++*F.epoch;
     int e2 = ++*F.epoch;
+    u64* yll = F.yll;
     void* a[] = {(void*)&A, &lda, &n, &nblk, (void*)&li, (void*)&wl, (void*)&ui, (void*)&wu,
-                 (void*)&v, &tmp, &z, &flags, &e1, &e2, &status};
+                 (void*)&v, &tmp, &z, &yll, &flags, &e1, &e2, &status};
     return hipLaunchCooperativeKernel((const void*)k_trsv_coop, dim3(F.coop), dim3(512), a, 0, st);
 }
 
