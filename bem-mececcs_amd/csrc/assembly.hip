@@ -17,6 +17,8 @@
 //                    316-333) and the fused RHS ZFI = ZG ZDFI (Ghmatecd.for:172)
 //   k_interior       INTEREC displacements and SIGMAEC stresses
 //                    (Interec.for:86-135, 188-283, Sigmaec.for)
+#include <algorithm>
+
 #include "bemx_common.hpp"
 
 namespace bemx {
@@ -479,57 +481,73 @@ __global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag_h(int nbe, int npg2,
 // Flagged elements (nonzero traction in some load case, or a KODE = 0
 // column): H and G, the swapped KODE = 0 columns, and per load case c the
 // partial RHS  sum_J G'_IJ ZDFI_c(J)  of this chunk of the flagged list into
-// rhs_part[chunk][c][row].
-template <int WAVES, int JTG>
-__global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag_g(
+// rhs_part[chunk][c][row].  A workgroup covers 64 rows (one per lane) and
+// one chunk of JTG flagged elements, wave w taking element w of the chunk,
+// so the few flagged columns are spread over many workgroups; the waves'
+// RHS terms are summed through LDS in element order (a fixed order: the
+// same result on every run and in every batch).
+template <int JTG>
+__global__ __launch_bounds__(64 * JTG) void k_dyn_offdiag_g(
     int nbe, int npg2, const ElemData* __restrict__ elem, const GaussPt* __restrict__ gp,
     const int* __restrict__ colflag, const int* __restrict__ flist, const int* __restrict__ fcount,
     const cd* __restrict__ zdfi, int nrhs, size_t ldr, DynConst K, cd* __restrict__ zh,
     size_t ldzh, cd* __restrict__ rhs_part) {
     extern __shared__ GaussPt sgp[];
+    __shared__ cd rb[JTG][MAXRHS][3][64];
     const int cnt = *fcount;
-    const int f0 = blockIdx.y * JTG;
-    if (f0 >= cnt) return;  // uniform over the workgroup
-    const int fn = min(JTG, cnt - f0);
-    for (int t = threadIdx.x; t < fn * npg2; t += 64 * WAVES) {
-        int jj = t / npg2, q = t - jj * npg2;
-        sgp[t] = gp[(size_t)flist[f0 + jj] * npg2 + q];
-    }
-    __syncthreads();
-    const int I = blockIdx.x * (64 * WAVES) + threadIdx.x;
+    const int lane = threadIdx.x & 63, jj = threadIdx.x >> 6;
+    const int I = blockIdx.x * 64 + lane;
     const bool rowok = I < nbe;
     const int Ic = rowok ? I : nbe - 1;
     const double xc = elem[Ic].cx, yc = elem[Ic].cy, zc = elem[Ic].cz;
-    cd b[MAXRHS][3];
-    for (int c = 0; c < MAXRHS; ++c) b[c][0] = b[c][1] = b[c][2] = mk(0, 0);
-    for (int jj = 0; jj < fn; ++jj) {
-        const int J = flist[f0 + jj];
-        const int flag = colflag[J];
-        const double nv[3] = {elem[J].nx, elem[J].ny, elem[J].nz};
-        cd H[9], G[9], Ho[9], Gn[9];
-        offdiag_element<true>(K, sgp + jj * npg2, npg2, xc, yc, zc, nv, H, G);
-        swap_cols(H, G, flag, K.zge, Ho, Gn);
-        if (!rowok || I == J) continue;
-        for (int k = 0; k < 3; ++k) {
-            if (!((flag >> (1 + k)) & 1)) continue;
-            cd* col = zh + (size_t)(3 * J + k) * ldzh + 3 * I;
-            col[0] = Ho[3 * k];
-            col[1] = Ho[3 * k + 1];
-            col[2] = Ho[3 * k + 2];
+    // chunks blockIdx.y, + gridDim.y, ... of the flagged list (the grid is
+    // not sized by the flagged count, which only the device knows)
+    for (int chunk = blockIdx.y; chunk * JTG < cnt; chunk += gridDim.y) {
+        const int f0 = chunk * JTG;
+        const int fn = min(JTG, cnt - f0);
+        __syncthreads();  // the previous chunk is done with sgp and rb
+        for (int t = threadIdx.x; t < fn * npg2; t += 64 * JTG) {
+            int jf = t / npg2, q = t - jf * npg2;
+            sgp[t] = gp[(size_t)flist[f0 + jf] * npg2 + q];
         }
-        for (int c = 0; c < nrhs; ++c) {
-            const cd* t = zdfi + c * ldr + 3 * J;
-            cd t0 = t[0], t1 = t[1], t2 = t[2];
-            for (int i = 0; i < 3; ++i) b[c][i] = b[c][i] + Gn[i] * t0 + Gn[i + 3] * t1 + Gn[i + 6] * t2;
+        __syncthreads();
+        cd b[MAXRHS][3];
+        for (int c = 0; c < MAXRHS; ++c) b[c][0] = b[c][1] = b[c][2] = mk(0, 0);
+        if (jj < fn) {
+            const int J = flist[f0 + jj];
+            const int flag = colflag[J];
+            const double nv[3] = {elem[J].nx, elem[J].ny, elem[J].nz};
+            cd H[9], G[9], Ho[9], Gn[9];
+            offdiag_element<true>(K, sgp + jj * npg2, npg2, xc, yc, zc, nv, H, G);
+            swap_cols(H, G, flag, K.zge, Ho, Gn);
+            if (rowok && I != J) {
+                for (int k = 0; k < 3; ++k) {
+                    if (!((flag >> (1 + k)) & 1)) continue;
+                    cd* col = zh + (size_t)(3 * J + k) * ldzh + 3 * I;
+                    col[0] = Ho[3 * k];
+                    col[1] = Ho[3 * k + 1];
+                    col[2] = Ho[3 * k + 2];
+                }
+                for (int c = 0; c < nrhs; ++c) {
+                    const cd* t = zdfi + c * ldr + 3 * J;
+                    cd t0 = t[0], t1 = t[1], t2 = t[2];
+                    for (int i = 0; i < 3; ++i) b[c][i] = Gn[i] * t0 + Gn[i + 3] * t1 + Gn[i + 6] * t2;
+                }
+            }
         }
+        for (int c = 0; c < nrhs; ++c)
+            for (int i = 0; i < 3; ++i) rb[jj][c][i][lane] = b[c][i];
+        __syncthreads();
+        if (jj == 0 && rowok)
+            for (int c = 0; c < nrhs; ++c) {
+                cd* o = rhs_part + ((size_t)chunk * nrhs + c) * ldr + 3 * I;
+                for (int i = 0; i < 3; ++i) {
+                    cd acc = rb[0][c][i][lane];
+                    for (int w = 1; w < JTG; ++w) acc = acc + rb[w][c][i][lane];
+                    o[i] = acc;
+                }
+            }
     }
-    if (rowok)
-        for (int c = 0; c < nrhs; ++c) {
-            cd* o = rhs_part + ((size_t)blockIdx.y * nrhs + c) * ldr + 3 * I;
-            o[0] = b[c][0];
-            o[1] = b[c][1];
-            o[2] = b[c][2];
-        }
 }
 
 // ZFI_c = diagonal part + the chunk partials of the flagged list, chunks
@@ -801,13 +819,14 @@ hipError_t launch_dyn_assembly(hipStream_t st, int nbe, int npg2, const ElemData
     hipLaunchKernelGGL(k_dyn_diag, dim3(nbe), dim3(64), 0, st, nbe, npg2, elem, gp, hdiag, gdiag,
                        colflag, zdfi, nrhs, ldr, K, zh, ldzh, rhs_diag);
     const int rowblk = (nbe + 64 * OD_WAVES - 1) / (64 * OD_WAVES);
-    if (ev_od0) hipEventRecord(ev_od0, st);
+    if (ev_od0) (void)hipEventRecord(ev_od0, st);
     hipLaunchKernelGGL((k_dyn_offdiag_h<OD_WAVES, OD_JT>), dim3(rowblk, (nbe + OD_JT - 1) / OD_JT),
                        dim3(64 * OD_WAVES), (size_t)OD_JT * npg2 * sizeof(GaussPt), st, nbe, npg2,
                        elem, gp, colflag, K, zh, ldzh);
-    if (ev_od1) hipEventRecord(ev_od1, st);
-    hipLaunchKernelGGL((k_dyn_offdiag_g<OD_WAVES, OG_JT>), dim3(rowblk, offdiag_tiles(nbe)),
-                       dim3(64 * OD_WAVES), (size_t)OG_JT * npg2 * sizeof(GaussPt), st, nbe, npg2,
+    if (ev_od1) (void)hipEventRecord(ev_od1, st);
+    hipLaunchKernelGGL((k_dyn_offdiag_g<OG_JT>),
+                       dim3((nbe + 63) / 64, std::min(offdiag_tiles(nbe), 64)),
+                       dim3(64 * OG_JT), (size_t)OG_JT * npg2 * sizeof(GaussPt), st, nbe, npg2,
                        elem, gp, colflag, flist, fcount, zdfi, nrhs, ldr, K, zh, ldzh, rhs_part);
     hipLaunchKernelGGL(k_rhs_reduce, dim3((3 * nbe + 255) / 256, nrhs), dim3(256), 0, st, 3 * nbe,
                        nrhs, ldr, OG_JT, fcount, rhs_part, rhs_diag, zfi);
