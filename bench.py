@@ -50,9 +50,9 @@ FLOPS_PER_TRIPLE = 242
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 # HBM bytes per launch of k_dyn_offdiag_h on the default workload, from the
 # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a plain streaming copy of
-# known size) recorded in profiles/pmc_traffic_c2.md: 46.7 MB read + 686.5 MB
+# known size) recorded in profiles/pmc_traffic_c2.md: 49.2 MB read + 671.6 MB
 # written (the ZH store itself is 671.8 MB).
-OFFDIAG_TRAFFIC_BYTES_C2 = 733.2e6
+OFFDIAG_TRAFFIC_BYTES_C2 = 720.8e6
 HBM_PEAK_GBS = 8000.0
 DTYPE = "f64 assembly + f32/bf16x3 factor + f64 refine"
 
