@@ -26,12 +26,12 @@
 // of the matrix gets one update per group with K = 64 x group size, on a
 // second stream, while the next group is factored.
 // Preconditioner M^-1 v (FP64 vectors, FP32 factors): k_trsv_coop, one
-// cooperative launch for the lower and then the upper block solve;
-// workgroups own 128-row blocks (pairs of diagonal blocks), accumulate the
-// contributions of finished blocks as their flags come up and publish their
-// own block, so the dependency chain costs one product with a precomputed
-// coupling block (k_pair_coupling) per 128 rows; the 128-row inverses come
-// from k_pair_inverses after the factorisation.
+// cooperative launch for the lower and then the upper block solve; two
+// workgroups own each 128-row block (pair of diagonal blocks), 64 rows each,
+// accumulate the contributions of finished blocks as their flags come up and
+// publish their half, so the dependency chain costs one 64 x 128 product
+// with a precomputed coupling block (k_pair_coupling) per 128 rows; the
+// 128-row inverses come from k_pair_inverses after the factorisation.
 // Pivoting is restricted to the diagonal block: the factors only
 // precondition.  The accuracy of the answer comes from the refinement, whose
 // residual b - A64 x is formed in FP64 with the FP64 matrix; if it stalls the
@@ -390,47 +390,7 @@ __global__ __launch_bounds__(256) void k_pair_inverses(const cf* __restrict__ A,
     }
 }
 
-// Block diagonal step of the 128-row block solves: out = Minv acc, the
-// eight waves each taking 32 columns of one row half; joined through LDS,
-// the two row halves stored by waves 0 and 4, one flag published.
-// sum_i f[i] * y_i over 32 columns (FP32 factor entries, FP64 vector), y_i
-// held by lane i (i < 32) and broadcast with v_readlane; four independent
-// accumulators keep the FMA chains short.
-__device__ inline cd dot32(const cf (&f)[32], cd yl) {
-    cd s0 = mk(0, 0), s1 = mk(0, 0), s2 = mk(0, 0), s3 = mk(0, 0);
-#pragma unroll
-    for (int i = 0; i < 32; i += 4) {
-        s0 = s0 + up(f[i]) * readlane_c(yl, i);
-        s1 = s1 + up(f[i + 1]) * readlane_c(yl, i + 1);
-        s2 = s2 + up(f[i + 2]) * readlane_c(yl, i + 2);
-        s3 = s3 + up(f[i + 3]) * readlane_c(yl, i + 3);
-    }
-    return (s0 + s1) + (s2 + s3);
-}
 
-// z = Minv acc for this wave's row (the rows of row half rh); the four
-// column quarters are summed through part.  Returns z in the waves of
-// quarter 0 (zero elsewhere); ends with a barrier, so part may be reused.
-__device__ inline cd diag128_apply(const cf mv[32], const cd* acc, cd (*part)[NBM], int w,
-                                   int lane) {
-    const int rh = w >> 2, q = w & 3;
-    cd s0 = mk(0, 0), s1 = mk(0, 0), s2 = mk(0, 0), s3 = mk(0, 0);
-#pragma unroll
-    for (int i = 0; i < 32; i += 4) {
-        s0 = s0 + up(mv[i]) * acc[32 * q + i];
-        s1 = s1 + up(mv[i + 1]) * acc[32 * q + i + 1];
-        s2 = s2 + up(mv[i + 2]) * acc[32 * q + i + 2];
-        s3 = s3 + up(mv[i + 3]) * acc[32 * q + i + 3];
-    }
-    part[w][lane] = (s0 + s1) + (s2 + s3);
-    __syncthreads();
-    cd z = mk(0, 0);
-    if (q == 0)
-        z = (part[4 * rh][lane] + part[4 * rh + 1][lane]) +
-            (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]);
-    __syncthreads();
-    return z;
-}
 
 // Coupling blocks for the block solves: lower, pair p >= 1,
 // Wl_p = Ml_p L_{p,p-1}; upper, pair p < npair - 1, Wu_p = Mu_p U_{p,p+1}
@@ -539,172 +499,199 @@ __device__ inline cd ll_wait(u64* __restrict__ ll, int i, int tag, int* status) 
               __longlong_as_double((long long)(((w3 & 0xffffffffull) << 32) | (w2 & 0xffffffffull))));
 }
 
-// y = L^-1 b over 128-row blocks.  Workgroup g owns blocks g, g+G, ...;
-// wave w = (row half rh, column quarter q) accumulates its 64 rows x 32
-// columns of every finished block as that block's flag (for the block two
-// back: its hand-off words) comes up (factor tile loaded before the wait),
-// then y_blk = Minv acc - W y_{blk-1} with the previous block's words.  Blocks only wait on
-// lower-numbered blocks and every workgroup is resident (cooperative
+// sum_i f[i] * y_i over 16 columns (FP32 factor entries, FP64 vector), y_i
+// held by lane i (i < 16) and broadcast with v_readlane.
+__device__ inline cd dot16(const cf (&f)[16], cd yl) {
+    cd s0 = mk(0, 0), s1 = mk(0, 0), s2 = mk(0, 0), s3 = mk(0, 0);
+#pragma unroll
+    for (int i = 0; i < 16; i += 4) {
+        s0 = s0 + up(f[i]) * readlane_c(yl, i);
+        s1 = s1 + up(f[i + 1]) * readlane_c(yl, i + 1);
+        s2 = s2 + up(f[i + 2]) * readlane_c(yl, i + 2);
+        s3 = s3 + up(f[i + 3]) * readlane_c(yl, i + 3);
+    }
+    return (s0 + s1) + (s2 + s3);
+}
+
+// Sum over the 8 waves' partials of this lane's row, in a fixed order.
+__device__ inline cd sum8(const cd (*part)[NBM], int lane) {
+    return ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) +
+           ((part[4][lane] + part[5][lane]) + (part[6][lane] + part[7][lane]));
+}
+
+// Block solves over 128-row blocks, each block split between two workgroups
+// of 64 rows (one lane per row; wave w takes the 16 columns [16 w, 16 w + 16)
+// of every block it multiplies), so the product that waits on the previous
+// block is 64 x 128 per workgroup.  Workgroup 2 g + h owns row half h of
+// blocks g, g + G, ...  Per block: the contributions of the finished blocks
+// as their flags come up (the block two back through its hand-off words),
+// the halves of the accumulated right-hand side exchanged with the partner
+// workgroup through hand-off words (acc_ll), the product with the block's
+// 128 x 128 inverse, and the coupling to the previous block as its hand-off
+// words arrive; one reduction, then this half's solution goes out as
+// hand-off words and plain stores, and flags[2 blk + h] is raised.  Blocks
+// only wait on lower-numbered blocks and on the partner half, which
+// publishes before it waits, and every workgroup is resident (cooperative
 // launch), so the chain always drains.
 __device__ __forceinline__ void trsv_lower_body(const cf* __restrict__ A, size_t lda, int n,
                                                 int nblk2, const cf* __restrict__ Minv,
                                                 const cf* __restrict__ Wc,
                                                 const cd* __restrict__ b, cd* __restrict__ y,
-                                                u64* __restrict__ yll,
+                                                u64* __restrict__ yll, u64* __restrict__ acc_ll,
                                                 int* __restrict__ flags, int epoch,
                                                 int* __restrict__ status, cd (*part)[NBM],
-                                                cd* acc) {
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rh = w >> 2, q = w & 3;
-    for (int blk = blockIdx.x; blk < nblk2; blk += gridDim.x) {
+                                                cd* accs) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, hw = w >> 2;
+    const int h = blockIdx.x & 1, G = gridDim.x >> 1;
+    for (int blk = blockIdx.x >> 1; blk < nblk2; blk += G) {
         const int r0 = blk * NB2, nb = min(NB2, n - r0);
-        const int row = rh * NBM + lane;
-        cf mv[32];
-        const cf* Mb = Minv + (size_t)blk * NB2 * NB2 + (size_t)(32 * q) * NB2 + row;
-#pragma unroll
-        for (int i = 0; i < 32; ++i) mv[i] = Mb[(size_t)i * NB2];
-        cd s = mk(0, 0);
+        const int row = NBM * h + lane;  // row within the block
+        const bool rok = row < nb;
         const int rowc = r0 + min(row, nb - 1);
-        for (int k = 0; k < blk - 1; ++k) {
-            const int c0 = k * NB2 + 32 * q;
-            const cf* Lc = A + (size_t)c0 * lda + rowc;
-            cf l[32];  // the factor tile does not depend on the flag: load it first
+        cf mv[16];
+        const cf* Mb = Minv + (size_t)blk * NB2 * NB2 + (size_t)(16 * w) * NB2 + row;
 #pragma unroll
-            for (int i = 0; i < 32; ++i) l[i] = Lc[(size_t)i * lda];
-            // the block two back comes through the hand-off words too: the
-            // rest of this block's work before the coupling product hangs on it
+        for (int i = 0; i < 16; ++i) mv[i] = Mb[(size_t)i * NB2];
+        cd s = mk(0, 0);
+        for (int k = 0; k < blk - 1; ++k) {
+            const int c0 = k * NB2 + 16 * w;
+            const cf* Lc = A + (size_t)c0 * lda + rowc;
+            cf l[16];  // the factor tile does not depend on the flag: load it first
+#pragma unroll
+            for (int i = 0; i < 16; ++i) l[i] = Lc[(size_t)i * lda];
             cd yk;
             if (k == blk - 2) {
-                yk = ll_wait(yll, c0 + (lane & 31), epoch, status);
+                yk = ll_wait(yll, c0 + (lane & 15), epoch, status);
             } else {
-                wave_wait_ge(&flags[k], epoch, status);
-                yk = y[c0 + (lane & 31)];
+                wave_wait_ge(&flags[2 * k + hw], epoch, status);
+                yk = y[c0 + (lane & 15)];
             }
-            s = s + dot32(l, yk);
-        }
-        cf wv[32];  // coupling to the previous block, loaded before its wait
-        if (blk > 0) {
-            const cf* Wb = Wc + (size_t)blk * NB2 * NB2 + (size_t)(32 * q) * NB2 + row;
-#pragma unroll
-            for (int i = 0; i < 32; ++i) wv[i] = Wb[(size_t)i * NB2];
+            s = s + dot16(l, yk);
         }
         part[w][lane] = s;
         __syncthreads();
-        if (q == 0)
-            acc[row] = row < nb ? b[r0 + row] - ((part[4 * rh][lane] + part[4 * rh + 1][lane]) +
-                                                 (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]))
-                                : mk(0, 0);
-        __syncthreads();
-        cd z = diag128_apply(mv, acc, part, w, lane);
-        if (blk > 0) {
-            const cd yl = ll_wait(yll, (blk - 1) * NB2 + 32 * q + (lane & 31), epoch, status);
-            const cd t = dot32(wv, yl);
-            part[w][lane] = t;
-            __syncthreads();
-            if (q == 0)
-                z = z - ((part[4 * rh][lane] + part[4 * rh + 1][lane]) +
-                         (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]));
+        if (w == 0) {
+            const cd a = rok ? b[r0 + row] - sum8(part, lane) : mk(0, 0);
+            accs[row] = a;
+            ll_put(acc_ll, r0 + row, a, epoch);
+        } else if (w == 1) {
+            accs[NBM * (1 - h) + lane] =
+                ll_wait(acc_ll, r0 + NBM * (1 - h) + lane, epoch, status);
         }
-        if (q == 0 && row < nb) {
+        __syncthreads();
+        cd t = mk(0, 0);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) t = t + up(mv[i]) * accs[16 * w + i];
+        if (blk > 0) {
+            cf wv[16];  // coupling to the previous block
+            const cf* Wb = Wc + (size_t)blk * NB2 * NB2 + (size_t)(16 * w) * NB2 + row;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) wv[i] = Wb[(size_t)i * NB2];
+            t = t - dot16(wv, ll_wait(yll, (blk - 1) * NB2 + 16 * w + (lane & 15), epoch, status));
+        }
+        part[w][lane] = t;
+        __syncthreads();
+        if (w == 0 && rok) {
+            const cd z = sum8(part, lane);
             ll_put(yll, r0 + row, z, epoch);  // the chain's hand-off first
             y[r0 + row] = z;
         }
         // the barrier completes these stores; the release store of the flag
-        // then makes them visible at device scope (for the blocks three and
-        // more ahead and the upper solve)
+        // then makes them visible at device scope
         __syncthreads();
-        if (tid == 0) publish(&flags[blk], epoch);
+        if (tid == 0) publish(&flags[2 * blk + h], epoch);
     }
 }
 
 // x = U^-1 y over 128-row blocks (in reverse order), same scheme.
-// ylow: epoch of the lower solve that produced yv; block blk's entries of
-// yv are read after its flag reached ylow (x may alias the lower solve's
-// right-hand side: block blk of it was read before that flag was set).
+// ylow: epoch of the lower solve that produced yv; this half's entries of
+// yv are read after its lower flag reached ylow (x may alias the lower
+// solve's right-hand side: those entries were read before that flag was
+// set).
 __device__ __forceinline__ void trsv_upper_body(const cf* __restrict__ A, size_t lda, int n,
                                                 int nblk2, const cf* __restrict__ Minv,
                                                 const cf* __restrict__ Wc,
                                                 const cd* __restrict__ yv, cd* __restrict__ x,
-                                                u64* __restrict__ yll,
+                                                u64* __restrict__ yll, u64* __restrict__ acc_ll,
                                                 int* __restrict__ flags, int epoch, int ylow,
                                                 int* __restrict__ status, cd (*part)[NBM],
-                                                cd* acc) {
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, rh = w >> 2, q = w & 3;
-    for (int bi = blockIdx.x; bi < nblk2; bi += gridDim.x) {
+                                                cd* accs) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, hw = w >> 2;
+    const int h = blockIdx.x & 1, G = gridDim.x >> 1;
+    for (int bi = blockIdx.x >> 1; bi < nblk2; bi += G) {
         const int blk = nblk2 - 1 - bi;
         const int r0 = blk * NB2, nb = min(NB2, n - r0);
-        const int row = rh * NBM + lane;
-        cf mv[32];
-        const cf* Mb = Minv + (size_t)blk * NB2 * NB2 + (size_t)(32 * q) * NB2 + row;
-#pragma unroll
-        for (int i = 0; i < 32; ++i) mv[i] = Mb[(size_t)i * NB2];
-        cd s = mk(0, 0);
+        const int row = NBM * h + lane;
+        const bool rok = row < nb;
         const int rowc = r0 + min(row, nb - 1);
+        cf mv[16];
+        const cf* Mb = Minv + (size_t)blk * NB2 * NB2 + (size_t)(16 * w) * NB2 + row;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) mv[i] = Mb[(size_t)i * NB2];
+        cd s = mk(0, 0);
         for (int k = nblk2 - 1; k > blk + 1; --k) {
             const int kb = min(NB2, n - k * NB2);
-            const int c0 = k * NB2 + 32 * q;
-            const int cnt = min(32, kb - 32 * q);  // columns of this wave in block k
+            const int c0 = k * NB2 + 16 * w;
+            const int cnt = min(16, kb - 16 * w);  // columns of this wave in block k
+            if (cnt <= 0) continue;
             const cf* Uc = A + (size_t)c0 * lda + rowc;
-            if (cnt == 32) {
-                cf u[32];  // the factor tile does not depend on the flag: load it first
+            if (cnt == 16) {
+                cf u[16];  // the factor tile does not depend on the flag: load it first
 #pragma unroll
-                for (int i = 0; i < 32; ++i) u[i] = Uc[(size_t)i * lda];
-                cd xk;  // the block two ahead through the hand-off words
+                for (int i = 0; i < 16; ++i) u[i] = Uc[(size_t)i * lda];
+                cd xk;  // the block two ahead through its hand-off words
                 if (k == blk + 2) {
-                    xk = ll_wait(yll, c0 + (lane & 31), epoch, status);
+                    xk = ll_wait(yll, c0 + (lane & 15), epoch, status);
                 } else {
-                    wave_wait_ge(&flags[k], epoch, status);
-                    xk = x[c0 + (lane & 31)];
+                    wave_wait_ge(&flags[2 * k + hw], epoch, status);
+                    xk = x[c0 + (lane & 15)];
                 }
-                s = s + dot32(u, xk);
+                s = s + dot16(u, xk);
             } else {
-                if (cnt <= 0) {
-                    wave_wait_ge(&flags[k], epoch, status);
-                    continue;
-                }
-                cd yl;
+                cd xk;
                 if (k == blk + 2) {
-                    yl = ll_wait(yll, c0 + min(lane & 31, cnt - 1), epoch, status);
+                    xk = ll_wait(yll, c0 + min(lane & 15, cnt - 1), epoch, status);
                 } else {
-                    wave_wait_ge(&flags[k], epoch, status);
-                    yl = x[c0 + min(lane & 31, cnt - 1)];
+                    wave_wait_ge(&flags[2 * k + hw], epoch, status);
+                    xk = x[c0 + min(lane & 15, cnt - 1)];
                 }
-                for (int i = 0; i < cnt; ++i) s = s + up(Uc[(size_t)i * lda]) * readlane_c(yl, i);
+                for (int i = 0; i < cnt; ++i) s = s + up(Uc[(size_t)i * lda]) * readlane_c(xk, i);
             }
         }
-        const bool next = blk + 1 < nblk2;
-        cf wv[32];  // coupling to the next block, loaded before its wait
-        if (next) {
-            const cf* Wb = Wc + (size_t)blk * NB2 * NB2 + (size_t)(32 * q) * NB2 + row;
-#pragma unroll
-            for (int i = 0; i < 32; ++i) wv[i] = Wb[(size_t)i * NB2];
-        }
-        wave_wait_ge(&flags[blk], ylow, status);
+        wave_wait_ge(&flags[2 * blk + h], ylow, status);
         part[w][lane] = s;
         __syncthreads();
-        if (q == 0)
-            acc[row] = row < nb ? yv[r0 + row] - ((part[4 * rh][lane] + part[4 * rh + 1][lane]) +
-                                                  (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]))
-                                : mk(0, 0);
+        if (w == 0) {
+            const cd a = rok ? yv[r0 + row] - sum8(part, lane) : mk(0, 0);
+            accs[row] = a;
+            ll_put(acc_ll, r0 + row, a, epoch);
+        } else if (w == 1) {
+            accs[NBM * (1 - h) + lane] =
+                ll_wait(acc_ll, r0 + NBM * (1 - h) + lane, epoch, status);
+        }
         __syncthreads();
-        cd z = diag128_apply(mv, acc, part, w, lane);
-        if (next) {
+        cd t = mk(0, 0);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) t = t + up(mv[i]) * accs[16 * w + i];
+        if (blk + 1 < nblk2) {
             // columns of a partial next block have zero coupling; their
             // index is clamped into x
-            const cd xl =
-                ll_wait(yll, min((blk + 1) * NB2 + 32 * q + (lane & 31), n - 1), epoch, status);
-            const cd t = dot32(wv, xl);
-            part[w][lane] = t;
-            __syncthreads();
-            if (q == 0)
-                z = z - ((part[4 * rh][lane] + part[4 * rh + 1][lane]) +
-                         (part[4 * rh + 2][lane] + part[4 * rh + 3][lane]));
+            cf wv[16];
+            const cf* Wb = Wc + (size_t)blk * NB2 * NB2 + (size_t)(16 * w) * NB2 + row;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) wv[i] = Wb[(size_t)i * NB2];
+            t = t - dot16(wv, ll_wait(yll, min((blk + 1) * NB2 + 16 * w + (lane & 15), n - 1),
+                                      epoch, status));
         }
-        if (q == 0 && row < nb) {
+        part[w][lane] = t;
+        __syncthreads();
+        if (w == 0 && rok) {
+            const cd z = sum8(part, lane);
             ll_put(yll, r0 + row, z, epoch);
             x[r0 + row] = z;
         }
         __syncthreads();
-        if (tid == 0) publish(&flags[blk], epoch);
+        if (tid == 0) publish(&flags[2 * blk + h], epoch);
     }
 }
 
@@ -721,9 +708,11 @@ __global__ __launch_bounds__(512) void k_trsv_coop(const cf* __restrict__ A, siz
                                                    int* __restrict__ flags, int e1, int e2,
                                                    int* __restrict__ status) {
     __shared__ cd part[8][NBM];
-    __shared__ cd acc[NB2];
-    trsv_lower_body(A, lda, n, nblk2, Ml, Wl, v, tmp, yll, flags, e1, status, part, acc);
-    trsv_upper_body(A, lda, n, nblk2, Mu, Wu, tmp, z, yll, flags, e2, e1, status, part, acc);
+    __shared__ cd accs[NB2];
+    u64* acc_ll = yll + 4 * (size_t)nblk2 * NB2;  // the right-hand-side halves' hand-off
+    trsv_lower_body(A, lda, n, nblk2, Ml, Wl, v, tmp, yll, acc_ll, flags, e1, status, part, accs);
+    trsv_upper_body(A, lda, n, nblk2, Mu, Wu, tmp, z, yll, acc_ll, flags, e2, e1, status, part,
+                    accs);
 }
 
 // ---------------------------------------------------------------------------
@@ -1627,11 +1616,14 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     F.wc = (cf*)w.get(M_WC, sizeof(cf) * (size_t)F.npair * NB2 * NB2 * 2, &e);
     MX_CHK(e);
     const size_t cap_before = w.cap[M_FLAGS];
-    F.flags = (int*)w.get(M_FLAGS, sizeof(int) * (size_t)nblk, &e);
+    // block-solve flags: one per 64-row half of a 128-row block
+    const int nflags = 2 * F.npair;
+    F.flags = (int*)w.get(M_FLAGS, sizeof(int) * (size_t)nflags, &e);
     MX_CHK(e);
     if (w.flags_init != (void*)F.flags || w.cap[M_FLAGS] != cap_before) {
         // fresh flag array: clear it once
-        hipLaunchKernelGGL(k_fill_i, dim3((nblk + 255) / 256), dim3(256), 0, st, F.flags, 0, nblk);
+        hipLaunchKernelGGL(k_fill_i, dim3((nflags + 255) / 256), dim3(256), 0, st, F.flags, 0,
+                           nflags);
         w.flags_init = F.flags;
     }
     F.epoch = &w.epoch;
@@ -1639,11 +1631,12 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
         // tags are launch epochs (increasing from 1): only fresh memory needs
         // clearing, old tags never match again
         const size_t cap_ll = w.cap[M_LL];
-        F.yll = (u64*)w.get(M_LL, sizeof(u64) * 4 * (size_t)n, &e);
+        const int nll = F.npair * NB2;  // solution entries, then as many right-hand-side entries
+        F.yll = (u64*)w.get(M_LL, sizeof(u64) * 4 * 2 * (size_t)nll, &e);
         MX_CHK(e);
         if (w.cap[M_LL] != cap_ll)
-            hipLaunchKernelGGL(k_fill_i, dim3((8 * n + 255) / 256), dim3(256), 0, st, (int*)F.yll, 0,
-                               8 * n);
+            hipLaunchKernelGGL(k_fill_i, dim3((16 * nll + 255) / 256), dim3(256), 0, st, (int*)F.yll,
+                               0, 16 * nll);
     }
     F.status = (int*)w.get(M_STATUS, sizeof(int), &e);
     MX_CHK(e);
@@ -1651,7 +1644,8 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     MX_CHK(hipGetDevice(&dev));
     MX_CHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trsv_coop, 512, 0));
-    F.coop = std::max(1, std::min(F.npair, ncu * per));
+    // two workgroups per 128-row block, as many block pairs as are resident
+    F.coop = 2 * std::max(1, std::min(F.npair, ncu * per / 2));
     MX_CHK(w.ensure_side());
     // bf16 planes of L21 and U12: one slot (8 planes) per block of a group,
     // double-buffered by group parity (group g+1 writes one set while the bulk
