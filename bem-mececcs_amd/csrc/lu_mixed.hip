@@ -1274,17 +1274,10 @@ static void launch_update(hipStream_t s, const UpdRegion& a, const UpdRegion& b,
     // larger one is the bottleneck itself and gets the whole device
     if (cap > 0 && tiles <= 16 * cap) g = std::min(g, cap);
     // 4 column tiles per panel of the tile order (2, 8 and 16 measured the
-    // same on the 25920-DOF solve)
-    static const int nbuf = [] {
-        const char* v = getenv("BEMX_UPD_LDSBUF");
-        return (v && atoi(v) == 1) ? 1 : 2;
-    }();
-    if (nbuf == 2)
-        hipLaunchKernelGGL(k_cgemm_update2<2>, dim3(g), dim3(512), 0, s, a, b, base, pstride,
-                           sstride, nks, ldc, 4);
-    else
-        hipLaunchKernelGGL(k_cgemm_update2<1>, dim3(g), dim3(512), 0, s, a, b, base, pstride,
-                           sstride, nks, ldc, 4);
+    // same on the 25920-DOF solve); two LDS stage buffers (one buffer with
+    // two barriers per stage measured the same or 1% slower)
+    hipLaunchKernelGGL(k_cgemm_update2<2>, dim3(g), dim3(512), 0, s, a, b, base, pstride, sstride,
+                       nks, ldc, 4);
 }
 
 // k_cgemm_update_inverse for regions a (non-empty; block k+1 of nb1 columns
@@ -1644,8 +1637,10 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     MX_CHK(hipGetDevice(&dev));
     MX_CHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trsv_coop, 512, 0));
-    // two workgroups per 128-row block, as many block pairs as are resident
-    F.coop = 2 * std::max(1, std::min(F.npair, ncu * per / 2));
+    // two workgroups per 128-row block, at most one workgroup per CU (two
+    // sharing a CU slow each other's products down more than looping over
+    // blocks costs)
+    F.coop = 2 * std::max(1, std::min(F.npair, std::min(ncu, ncu * per) / 2));
     MX_CHK(w.ensure_side());
     // bf16 planes of L21 and U12: one slot (8 planes) per block of a group,
     // double-buffered by group parity (group g+1 writes one set while the bulk
