@@ -1678,6 +1678,10 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     int hs = 0;
     MX_CHK(hipMemcpyAsync(&hs, F.status, sizeof(int), hipMemcpyDeviceToHost, st));
     MX_CHK(hipStreamSynchronize(st));
+    if (hs & ST_SYNC_TIMEOUT) {
+        err = "mixed solve: a device-side wait of the factorisation timed out";
+        return BEMX_ERR_TIMEOUT;
+    }
     if (hs & ST_PIVOT_ZERO) {
         err = "mixed solve: zero pivot in a diagonal block";
         return BEMX_ERR_SINGULAR;
