@@ -889,6 +889,18 @@ __global__ __launch_bounds__(256) void k_panel_inv(cf* __restrict__ A, size_t ld
         for (int e = 0; e < 4; ++e)
             X[(16 * y + li) * XS + 16 * w + 4 * lq + e] = mkf(accR[y][e], accI[y][e]);
     __syncthreads();
+    // this block's L21 first: the stores drain while workgroup 0 applies
+    // the step to the next diagonal block
+    const bool ok = rb + lane < c0 + nr;
+#pragma unroll 4
+    for (int c = w; c < NBM; c += 4)
+        if (ok) A[(size_t)(k0 + c) * lda + rb + lane] = X[c * XS + lane];
+    // planes: t -> (row rr, k), k fastest, so the 2-byte stores are coalesced
+#pragma unroll 4
+    for (int t = tid; t < TRR_ROWS * NBM; t += 256) {
+        const int rr = t >> 6, k = t & 63;
+        if (rb + rr < c0 + nr) store_planes(la, pstride, (size_t)(rb + rr) * NBM + k, X[k * XS + rr]);
+    }
     if (cn > 0) {
         // corner -= L21 A12: A12 into the planes (Bp is free), L21 rows of
         // this workgroup from X
@@ -919,16 +931,6 @@ __global__ __launch_bounds__(256) void k_panel_inv(cf* __restrict__ A, size_t ld
                     A[(size_t)(c0 + c) * lda + c0 + r] =
                         mkf(cc[y][e].x - accR[y][e], cc[y][e].y - accI[y][e]);
             }
-    }
-    const bool ok = rb + lane < c0 + nr;
-#pragma unroll 4
-    for (int c = w; c < NBM; c += 4)
-        if (ok) A[(size_t)(k0 + c) * lda + rb + lane] = X[c * XS + lane];
-    // planes: t -> (row rr, k), k fastest, so the 2-byte stores are coalesced
-#pragma unroll 4
-    for (int t = tid; t < TRR_ROWS * NBM; t += 256) {
-        const int rr = t >> 6, k = t & 63;
-        if (rb + rr < c0 + nr) store_planes(la, pstride, (size_t)(rb + rr) * NBM + k, X[k * XS + rr]);
     }
 }
 
