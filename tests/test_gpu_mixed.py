@@ -50,6 +50,23 @@ def test_mixed_random(ctx, n):
     assert st["gmres_iters"] <= 8, st
 
 
+def test_mixed_many_workgroups_matches_fp64(ctx):
+    """8320 unknowns: the block-column kernel of every step then runs more
+    workgroups (2 x 130) than the device has CUs, so some start only after
+    others have finished.  The mixed solve must still converge in a few GMRES
+    iterations to the FP64 path's solution (the GPU FP64 LU is the reference
+    here: a CPU solve of this size would dominate the test)."""
+    n = 8320
+    a, b = _rand_system(n, 4242)
+    x = ctx.linsolve(a.copy(order="F"), b, api.MODE_MIXED)
+    st = ctx.last_stats()
+    xf = ctx.linsolve(a.copy(order="F"), b, api.MODE_FP64)
+    rel = np.max(np.abs(x - xf)) / np.max(np.abs(xf))
+    assert rel < TOL, (rel, st)
+    assert not st["fell_back"], st
+    assert st["gmres_iters"] <= 8, st
+
+
 def test_mixed_needs_in_block_pivoting(ctx):
     """Rows permuted inside each 64-row diagonal block: diagonal-block
     pivoting must find them."""
