@@ -18,6 +18,7 @@
 //   k_interior       INTEREC displacements and SIGMAEC stresses
 //                    (Interec.for:86-135, 188-283, Sigmaec.for)
 #include <algorithm>
+#include <cmath>
 
 #include "bemx_common.hpp"
 
@@ -253,11 +254,10 @@ __global__ __launch_bounds__(64) void k_gest_diag(int n, int nbe, const int* __r
 //   C = (c1-2)(A + B/2 - 3 chi/R) - 2 chi/R.
 // 1/z is formed as (1/k)(1/R).
 // ---------------------------------------------------------------------------
+// Es = exp(ks R) / R and Epc = c2 exp(kp R) / R are given.
 template <bool WithPsi>
-__device__ inline void dyn_scalars(const DynConst& K, double R, double iR, cd& psi, cd& chi,
-                                   cd& A, cd& B, cd& C) {
-    cd Es = exp_over_r(K.ks, R, iR);
-    cd Ep = exp_over_r(K.kp, R, iR);
+__device__ inline void dyn_scalars_e(const DynConst& K, double R, double iR, cd Es, cd Epc,
+                                     cd& psi, cd& chi, cd& A, cd& B, cd& C) {
     double iR2 = iR * iR;
     cd as = K.as * iR, as2 = K.as2 * iR2;
     cd ap = K.ap * iR, ap2 = K.ap2 * iR2;
@@ -265,7 +265,6 @@ __device__ inline void dyn_scalars(const DynConst& K, double R, double iR, cd& p
     // chi = (1 + 3/zs^2 - 3/zs) Es - c2 (1 + 3/zp^2 - 3/zp) Ep
     cd fs = mk(1.0 + 3.0 * (as2.x - as.x), 3.0 * (as2.y - as.y));
     cd fp = mk(1.0 + 3.0 * (ap2.x - ap.x), 3.0 * (ap2.y - ap.y));
-    cd Epc = K.c2 * Ep;
     chi = fs * Es - fp * Epc;
     if (WithPsi) {
         cd gs = mk(1.0 + as2.x - as.x, as2.y - as.y);
@@ -284,6 +283,85 @@ __device__ inline void dyn_scalars(const DynConst& K, double R, double iR, cd& p
     A = psid - chr;
     B = 4.0 * chr - 2.0 * chid;
     C = K.c1m2 * (A + 0.5 * B - 3.0 * chr) - 2.0 * chr;
+}
+
+template <bool WithPsi>
+__device__ inline void dyn_scalars(const DynConst& K, double R, double iR, cd& psi, cd& chi,
+                                   cd& A, cd& B, cd& C) {
+    dyn_scalars_e<WithPsi>(K, R, iR, exp_over_r(K.ks, R, iR), K.c2 * exp_over_r(K.kp, R, iR),
+                           psi, chi, A, B, C);
+}
+
+// ---------------------------------------------------------------------------
+// exp(k R) for the two wave numbers without an exp and a sincos per Gauss
+// point: exp(k R) = exp(k x_i) exp(k d), x_i = i h the nearest point of a
+// grid whose spacing h is a power of two (so i = rint(R / h), x_i and
+// d = R - x_i are exact), exp(k x_i) from a table built once per frequency,
+// and exp(k d) = exp(u) (cos v + i sin v), u = Re(k) d, v = Im(k) d, from
+// Taylor polynomials with the constant coefficients 1/n! (exp to degree 7,
+// cos to 6, sin to 7).  h is at most 1 / (16 max|k|), so |u|, |v| <= 1/32
+// and each truncation error is below (1/32)^8 / 8! ~ 2e-17.
+// ---------------------------------------------------------------------------
+constexpr int EXP_TAB_MAX = 1 << 18;  // entries per wave number
+
+struct ExpTab {
+    const cd* ts;  // exp(ks x_i)
+    const cd* tp;  // c2 exp(kp x_i)
+    double h, ih;
+    int nt;
+};
+
+// exp(k x) with the rounding error of each product k x carried to first
+// order, so a table entry is as accurate as exp and sincos themselves.
+__device__ inline cd exp_tab_entry(cd k, double x) {
+    const double ax = k.x * x, ex = fma(k.x, x, -ax);
+    const double ay = k.y * x, ey = fma(k.y, x, -ay);
+    const double e = exp(ax);
+    const double m = fma(e, ex, e);
+    double s, c;
+    sincos(ay, &s, &c);
+    return cd{m * fma(-ey, s, c), m * fma(ey, c, s)};
+}
+
+__global__ void k_exp_table(int nt, double h, cd ks, cd kp, cd c2, cd* __restrict__ ts,
+                            cd* __restrict__ tp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const double x = (double)i * h;
+    ts[i] = exp_tab_entry(ks, x);
+    tp[i] = c2 * exp_tab_entry(kp, x);
+}
+
+// exp(k d) for |Re(k) d|, |Im(k) d| <= 1/32.
+__device__ inline cd exp_small(cd k, double d) {
+    const double u = k.x * d, v = k.y * d, v2 = v * v;
+    double e = fma(u, 1.0 / 5040, 1.0 / 720);
+    e = fma(e, u, 1.0 / 120);
+    e = fma(e, u, 1.0 / 24);
+    e = fma(e, u, 1.0 / 6);
+    e = fma(e, u, 0.5);
+    e = fma(e, u, 1.0);
+    e = fma(e, u, 1.0);
+    double c = fma(v2, -1.0 / 720, 1.0 / 24);
+    c = fma(c, v2, -0.5);
+    c = fma(c, v2, 1.0);
+    double sn = fma(v2, -1.0 / 5040, 1.0 / 120);
+    sn = fma(sn, v2, -1.0 / 6);
+    sn = fma(sn * v2, v, v);
+    return cd{e * c, e * sn};
+}
+
+// Es = exp(ks R) / R and Epc = c2 exp(kp R) / R from the tables.
+__device__ inline void exp_pair_tab(const DynConst& K, const ExpTab& T, double R, double iR,
+                                    cd& Es, cd& Epc) {
+    const double fi = __builtin_rint(R * T.ih);
+    const double d = fma(-fi, T.h, R);
+    const int i = min(max((int)fi, 0), T.nt - 1);
+    const cd ts = T.ts[i], tp = T.tp[i];
+    const cd es = exp_small(K.ks, d), ep = exp_small(K.kp, d);
+    const double ar = ts.x * iR, ai = ts.y * iR, br = tp.x * iR, bi = tp.y * iR;
+    Es = cd{fma(ar, es.x, -ai * es.y), fma(ar, es.y, ai * es.x)};
+    Epc = cd{fma(br, ep.x, -bi * ep.y), fma(br, ep.y, bi * ep.x)};
 }
 
 // KODE column swap of one 3x3 block (Ghmatecd.for:316-333 as called at
@@ -390,10 +468,11 @@ __device__ inline double rsqrt_refined(double x) {
 // the Gauss sum of Solfund.for:58-59 with n fixed per element.  With NeedG
 // it also integrates U (psi, chi) into G.
 // ---------------------------------------------------------------------------
-template <bool NeedG>
-__device__ inline void offdiag_element(const DynConst& K, const GaussPt* __restrict__ sg, int npg2,
-                                       double xc, double yc, double zc, const double nv[3],
-                                       cd H[9], cd G[9]) {
+template <bool NeedG, bool TAB>
+__device__ inline void offdiag_element(const DynConst& K, const ExpTab& T,
+                                       const GaussPt* __restrict__ sg, int npg2, double xc,
+                                       double yc, double zc, const double nv[3], cd H[9],
+                                       cd G[9]) {
     cd SD = mk(0, 0), SP[3], SRQ[9], Gpsi = mk(0, 0), Gchi[6];
     for (int k = 0; k < 3; ++k) SP[k] = mk(0, 0);
     for (int k = 0; k < 9; ++k) SRQ[k] = mk(0, 0);
@@ -406,7 +485,13 @@ __device__ inline void offdiag_element(const DynConst& K, const GaussPt* __restr
         double rd[3] = {r1 * iR, r2 * iR, r3 * iR};
         double drn = rd[0] * nv[0] + rd[1] * nv[1] + rd[2] * nv[2];
         cd psi, chi, A, B, C;
-        dyn_scalars<NeedG>(K, R, iR, psi, chi, A, B, C);
+        if constexpr (TAB) {
+            cd Es, Epc;
+            exp_pair_tab(K, T, R, iR, Es, Epc);
+            dyn_scalars_e<NeedG>(K, R, iR, Es, Epc, psi, chi, A, B, C);
+        } else {
+            dyn_scalars<NeedG>(K, R, iR, psi, chi, A, B, C);
+        }
         double wq = p.w * K.q4pi;
         cd Ap = A * wq, Bp = B * (wq * drn), Cp = C * wq;
         SD = SD + Ap * drn;
@@ -444,13 +529,13 @@ __device__ inline void offdiag_element(const DynConst& K, const GaussPt* __restr
 
 // H of every off-diagonal block; columns with KODE = 0 are left to
 // k_dyn_offdiag_g, which writes their swapped content.
-template <int WAVES, int JT>
+template <int WAVES, int JT, bool TAB>
 __global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag_h(int nbe, int npg2,
                                                               const ElemData* __restrict__ elem,
                                                               const GaussPt* __restrict__ gp,
                                                               const int* __restrict__ colflag,
-                                                              DynConst K, cd* __restrict__ zh,
-                                                              size_t ldzh) {
+                                                              DynConst K, ExpTab T,
+                                                              cd* __restrict__ zh, size_t ldzh) {
     extern __shared__ GaussPt sgp[];
     const int j0 = blockIdx.y * JT;
     const int jn = min(JT, nbe - j0);
@@ -465,7 +550,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag_h(int nbe, int npg2,
         const int flag = colflag[J];
         const double nv[3] = {elem[J].nx, elem[J].ny, elem[J].nz};
         cd H[9], G[9];
-        offdiag_element<false>(K, sgp + jj * npg2, npg2, xc, yc, zc, nv, H, G);
+        offdiag_element<false, TAB>(K, T, sgp + jj * npg2, npg2, xc, yc, zc, nv, H, G);
         if (rowok && I != J) {
             for (int k = 0; k < 3; ++k) {
                 if ((flag >> (1 + k)) & 1) continue;
@@ -518,7 +603,8 @@ __global__ __launch_bounds__(64 * JTG) void k_dyn_offdiag_g(
             const int flag = colflag[J];
             const double nv[3] = {elem[J].nx, elem[J].ny, elem[J].nz};
             cd H[9], G[9], Ho[9], Gn[9];
-            offdiag_element<true>(K, sgp + jj * npg2, npg2, xc, yc, zc, nv, H, G);
+            offdiag_element<true, false>(K, ExpTab{}, sgp + jj * npg2, npg2, xc, yc, zc, nv, H,
+                                          G);
             swap_cols(H, G, flag, K.zge, Ho, Gn);
             if (rowok && I != J) {
                 for (int k = 0; k < 3; ++k) {
@@ -804,15 +890,42 @@ constexpr int OG_JT = 4;
 
 int offdiag_tiles(int nbe) { return (nbe + OG_JT - 1) / OG_JT; }
 
+// Entries of each exp table of the H kernel for the wave numbers of K and
+// distances up to rmax, with the grid spacing in *h; 0 when more than
+// EXP_TAB_MAX entries would be needed (the H kernel then evaluates exp and
+// sincos at every Gauss point).
+int exp_table_entries(const DynConst& K, double rmax, double* h) {
+    const double kmax = std::max(std::hypot(K.ks.x, K.ks.y), std::hypot(K.kp.x, K.kp.y));
+    if (!(kmax > 0.0) || !(rmax > 0.0) || !std::isfinite(kmax * rmax)) return 0;
+    const double hh = std::exp2(std::floor(std::log2(1.0 / (16.0 * kmax))));
+    const double nt = std::floor(rmax / hh) + 2.0;
+    if (!(nt <= (double)EXP_TAB_MAX)) return 0;
+    *h = hh;
+    return (int)nt;
+}
+
 // Dynamic assembly of ZH and of ZFI for nrhs load cases (zdfi, rhs_diag,
 // zfi: nrhs columns of leading dimension ldr = 3 NBE).  flist/fcount: NBE + 1
 // ints of scratch; rhs_part: offdiag_tiles(nbe) * nrhs * ldr complex.
 hipError_t launch_dyn_assembly(hipStream_t st, int nbe, int npg2, const ElemData* elem,
                                const GaussPt* gp, const double* hdiag, const double* gdiag,
                                const int* kode, const cd* zdfi, int nrhs, int* colflag,
-                               int* flist, const DynConst& K, cd* zh, size_t ldzh, cd* rhs_part,
-                               cd* rhs_diag, cd* zfi, hipEvent_t ev_od0, hipEvent_t ev_od1) {
+                               int* flist, const DynConst& K, cd* etab, int nt, double h,
+                               cd* zh, size_t ldzh, cd* rhs_part, cd* rhs_diag, cd* zfi,
+                               hipEvent_t ev_od0, hipEvent_t ev_od1) {
     const size_t ldr = 3 * (size_t)nbe;
+    // exp tables of the H kernel (etab: 2 nt entries)
+    ExpTab T{};
+    const bool tab = etab && nt > 0;
+    if (tab) {
+        T.ts = etab;
+        T.tp = etab + nt;
+        T.h = h;
+        T.ih = 1.0 / h;
+        T.nt = nt;
+        hipLaunchKernelGGL(k_exp_table, dim3((nt + 255) / 256), dim3(256), 0, st, nt, h, K.ks,
+                           K.kp, K.c2, etab, etab + nt);
+    }
     int* fcount = flist + nbe;
     hipLaunchKernelGGL(k_colflags, dim3(1), dim3(256), 0, st, nbe, kode, zdfi, nrhs, ldr, colflag,
                        flist, fcount);
@@ -820,9 +933,14 @@ hipError_t launch_dyn_assembly(hipStream_t st, int nbe, int npg2, const ElemData
                        colflag, zdfi, nrhs, ldr, K, zh, ldzh, rhs_diag);
     const int rowblk = (nbe + 64 * OD_WAVES - 1) / (64 * OD_WAVES);
     if (ev_od0) (void)hipEventRecord(ev_od0, st);
-    hipLaunchKernelGGL((k_dyn_offdiag_h<OD_WAVES, OD_JT>), dim3(rowblk, (nbe + OD_JT - 1) / OD_JT),
-                       dim3(64 * OD_WAVES), (size_t)OD_JT * npg2 * sizeof(GaussPt), st, nbe, npg2,
-                       elem, gp, colflag, K, zh, ldzh);
+    const dim3 odgrid(rowblk, (nbe + OD_JT - 1) / OD_JT), odblock(64 * OD_WAVES);
+    const size_t odlds = (size_t)OD_JT * npg2 * sizeof(GaussPt);
+    if (tab)
+        hipLaunchKernelGGL((k_dyn_offdiag_h<OD_WAVES, OD_JT, true>), odgrid, odblock, odlds, st,
+                           nbe, npg2, elem, gp, colflag, K, T, zh, ldzh);
+    else
+        hipLaunchKernelGGL((k_dyn_offdiag_h<OD_WAVES, OD_JT, false>), odgrid, odblock, odlds, st,
+                           nbe, npg2, elem, gp, colflag, K, T, zh, ldzh);
     if (ev_od1) (void)hipEventRecord(ev_od1, st);
     hipLaunchKernelGGL((k_dyn_offdiag_g<OG_JT>),
                        dim3((nbe + 63) / 64, std::min(offdiag_tiles(nbe), 64)),

@@ -3,6 +3,7 @@
 // kernels.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -26,9 +27,10 @@ hipError_t launch_gest_diag(hipStream_t, int, int, const int*, const double*, co
                             double, double, double*, int*);
 hipError_t launch_dyn_assembly(hipStream_t, int, int, const ElemData*, const GaussPt*,
                                const double*, const double*, const int*, const cd*, int, int*,
-                               int*, const DynConst&, cd*, size_t, cd*, cd*, cd*, hipEvent_t,
-                               hipEvent_t);
+                               int*, const DynConst&, cd*, int, double, cd*, size_t, cd*, cd*,
+                               cd*, hipEvent_t, hipEvent_t);
 int offdiag_tiles(int nbe);
+int exp_table_entries(const DynConst& K, double rmax, double* h);
 hipError_t launch_unswap(hipStream_t, int, const int*, cd*, cd*, cd);
 int interior_blocks(int nbe);
 hipError_t launch_interior(hipStream_t, int, int, int, int, const ElemData*, const GaussPt*,
@@ -68,13 +70,14 @@ struct bemx_ctx {
     int n = 0, nbe = 0, np = 0, npg = 0;
     bool mesh_ok = false;
     DevBuf cone, cx, cy, cz, gi, ome, elem, gp, gpT, status;
+    double rmax = 0.0;  // diagonal of the box holding every node and collocation point
     // static pass
     bool static_ok = false;
     double consts[5] = {0, 0, 0, 0, 0};  // pi, c1..c4
     DevBuf hdiag, gdiag, hpart;
     // dynamic data
     bool dyn_ok = false;
-    DevBuf kode, zdfi, colflag, flist, rhs_part, rhs_diag, zfi, zh;
+    DevBuf kode, zdfi, colflag, flist, rhs_part, rhs_diag, zfi, zh, etab;
     int nrhs = 1;              // load cases (columns of zdfi / zfi)
     bool zh_valid = false;     // zh holds the current assembled matrix
     const void* zh_host = nullptr;  // host array the last GHMATECD copied ZH to
@@ -174,7 +177,7 @@ int bemx_destroy(bemx_ctx* c) {
     DevBuf* bufs[] = {&c->cone, &c->cx, &c->cy, &c->cz, &c->gi, &c->ome, &c->elem,
                       &c->gp, &c->gpT, &c->status, &c->hdiag, &c->gdiag, &c->hpart,
                       &c->kode, &c->zdfi, &c->colflag, &c->flist, &c->rhs_part, &c->rhs_diag,
-                      &c->zfi, &c->zh, &c->ipts, &c->ipart, &c->zdsol, &c->zssol,
+                      &c->zfi, &c->zh, &c->etab, &c->ipts, &c->ipart, &c->zdsol, &c->zssol,
                       &c->ifi, &c->idfi, &c->ikode};
     for (DevBuf* b : bufs) b->release();
     c->lu.release();
@@ -233,6 +236,26 @@ int bemx_send_shared_data(bemx_ctx* c, const double* cx, const double* cy, const
     c->nbe = nbe;
     c->np = np;
     c->npg = npg;
+    {
+        double lo[3], hi[3];
+        const double* xyz[3] = {cx, cy, cz};
+        for (int d = 0; d < 3; ++d) lo[d] = hi[d] = xyz[d][0];
+        for (int d = 0; d < 3; ++d)
+            for (int k = 0; k < np; ++k) {
+                lo[d] = std::min(lo[d], xyz[d][k]);
+                hi[d] = std::max(hi[d], xyz[d][k]);
+            }
+        if (given) {
+            const double* m[3] = {cxm, cym, czm};
+            for (int d = 0; d < 3; ++d)
+                for (int k = 0; k < n; ++k) {
+                    lo[d] = std::min(lo[d], m[d][k]);
+                    hi[d] = std::max(hi[d], m[d][k]);
+                }
+        }
+        c->rmax = std::sqrt((hi[0] - lo[0]) * (hi[0] - lo[0]) + (hi[1] - lo[1]) * (hi[1] - lo[1]) +
+                            (hi[2] - lo[2]) * (hi[2] - lo[2]));
+    }
     int npg2 = npg * npg;
     std::vector<double> g(npg), w(npg);
     if (gi && ome) {
@@ -370,11 +393,15 @@ static int assemble_dev(bemx_ctx* c, const DynConst& K) {
     HIPCHK(c, c->rhs_diag.ensure(sizeof(cd) * nn * nr));
     HIPCHK(c, c->zfi.ensure(sizeof(cd) * nn * nr));
     HIPCHK(c, c->zh.ensure(sizeof(cd) * nn * nn));
+    double h = 0.0;
+    const int nt = exp_table_entries(K, c->rmax, &h);
+    if (nt) HIPCHK(c, c->etab.ensure(sizeof(cd) * 2 * (size_t)nt));
     c->zh_valid = false;
     HIPCHK(c, launch_dyn_assembly(c->st, nbe, c->npg * c->npg, c->elem.as<ElemData>(),
                                   c->gp.as<GaussPt>(), c->hdiag.as<double>(),
                                   c->gdiag.as<double>(), c->kode.as<int>(), c->zdfi.as<cd>(), nr,
-                                  c->colflag.as<int>(), c->flist.as<int>(), K, c->zh.as<cd>(), nn,
+                                  c->colflag.as<int>(), c->flist.as<int>(), K,
+                                  nt ? c->etab.as<cd>() : nullptr, nt, h, c->zh.as<cd>(), nn,
                                   c->rhs_part.as<cd>(), c->rhs_diag.as<cd>(), c->zfi.as<cd>(),
                                   c->ev2, c->ev3));
     HIPCHK(c, hipEventRecord(c->ev4, c->st));

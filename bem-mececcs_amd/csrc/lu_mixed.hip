@@ -1298,7 +1298,7 @@ static void launch_update_inverse(hipStream_t s, const UpdRegion& a, const UpdRe
 // Workgroups of the bulk update on the side stream (BEMX_BULK_WGS; default
 // 128, 0 = one per tile).  The bulk update then loops over its tiles on
 // half the CUs and leaves the rest to the critical chain of the next group
-// (getrf, triangular solves, in-group updates), which otherwise queues
+// (panel kernels, eliminations, in-group updates), which otherwise queues
 // behind thousands of bulk workgroups.  Updates of more than 16 tiles per
 // capped workgroup (the early bulk updates of large systems, which bound
 // the factorisation) are not capped.  Measured on the 6480-DOF solve:

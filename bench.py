@@ -42,17 +42,18 @@ import numpy as np  # noqa: E402
 METRIC = ("BEM solves/s + p50/p99 latency at ≤1e-10 err vs FP64, mixed vs plain FP64")
 
 # FP64 operations per (collocation point, element, Gauss point) triple of the
-# off-diagonal assembly kernel k_dyn_offdiag_h, counted from its source:
-# 234 add/mul (an fma counts 2) + 8 special functions (sqrt, 1/R, 2 exp,
-# 2 sin, 2 cos) counted as one each; the Newton steps that refine 1/R are an
-# implementation detail and not counted.
+# off-diagonal assembly kernel k_dyn_offdiag_h, counted from the factored
+# formula it evaluates: 234 add/mul (an fma counts 2) + 8 special functions
+# (sqrt, 1/R, 2 exp, 2 sin, 2 cos) counted as one each, however they are
+# evaluated (the Newton steps that refine 1/R and the table-and-polynomial
+# evaluation of the exponentials are implementation details, not counted).
 FLOPS_PER_TRIPLE = 242
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 # HBM bytes per launch of k_dyn_offdiag_h on the default workload, from the
 # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a plain streaming copy of
-# known size) recorded in profiles/pmc_traffic_c2.md: 49.2 MB read + 671.6 MB
+# known size) recorded in profiles/pmc_traffic_c2.md: 48.6 MB read + 671.6 MB
 # written (the ZH store itself is 671.8 MB).
-OFFDIAG_TRAFFIC_BYTES_C2 = 720.8e6
+OFFDIAG_TRAFFIC_BYTES_C2 = 720.2e6
 HBM_PEAK_GBS = 8000.0
 DTYPE = "f64 assembly + f32/bf16x3 factor + f64 refine"
 

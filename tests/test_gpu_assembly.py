@@ -34,6 +34,11 @@ def _problem(case):
     if case == "npg6":
         return decks.flat_patch_problem(12, 11, half=4.0, npg=6, ge=1.7, rnu=0.31, dam=0.03,
                                         freqs=(3.5,), n_interior=9, seed=3)
+    if case == "hifreq":
+        # about 40 shear wavelengths across the patch: distances up to 22.6 at
+        # |ks| = 12 (the H kernel's exp table then holds ~5800 entries per wave)
+        return decks.flat_patch_problem(16, 15, half=8.0, dam=0.05, freqs=(12.0,),
+                                        n_interior=4, seed=5)
     raise KeyError(case)
 
 
@@ -41,7 +46,7 @@ def _vec_close(a, b, rtol=RTOL):
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(b)))
 
 
-@pytest.mark.parametrize("case", ["c1", "esolo240", "npg6"])
+@pytest.mark.parametrize("case", ["c1", "esolo240", "npg6", "hifreq"])
 def test_static_and_dynamic_assembly(ctx, orc, case):
     mesh, dyn = _problem(case)
     print("case", case, "nbe", mesh.nbe, "n", mesh.n, "npg", mesh.npg, flush=True)
