@@ -351,17 +351,16 @@ __device__ inline cd exp_small(cd k, double d) {
     return cd{e * c, e * sn};
 }
 
-// Es = exp(ks R) / R and Epc = c2 exp(kp R) / R from the tables.
-__device__ inline void exp_pair_tab(const DynConst& K, const ExpTab& T, double R, double iR,
-                                    cd& Es, cd& Epc) {
+// Es = exp(ks R) and Epc = c2 exp(kp R) from the tables.
+__device__ inline void exp_pair_tab(const DynConst& K, const ExpTab& T, double R, cd& Es,
+                                    cd& Epc) {
     const double fi = __builtin_rint(R * T.ih);
     const double d = fma(-fi, T.h, R);
     const int i = min(max((int)fi, 0), T.nt - 1);
     const cd ts = T.ts[i], tp = T.tp[i];
     const cd es = exp_small(K.ks, d), ep = exp_small(K.kp, d);
-    const double ar = ts.x * iR, ai = ts.y * iR, br = tp.x * iR, bi = tp.y * iR;
-    Es = cd{fma(ar, es.x, -ai * es.y), fma(ar, es.y, ai * es.x)};
-    Epc = cd{fma(br, ep.x, -bi * ep.y), fma(br, ep.y, bi * ep.x)};
+    Es = cd{fma(ts.x, es.x, -ts.y * es.y), fma(ts.x, es.y, ts.y * es.x)};
+    Epc = cd{fma(tp.x, ep.x, -tp.y * ep.y), fma(tp.x, ep.y, tp.y * ep.x)};
 }
 
 // KODE column swap of one 3x3 block (Ghmatecd.for:316-333 as called at
@@ -468,11 +467,10 @@ __device__ inline double rsqrt_refined(double x) {
 // the Gauss sum of Solfund.for:58-59 with n fixed per element.  With NeedG
 // it also integrates U (psi, chi) into G.
 // ---------------------------------------------------------------------------
-template <bool NeedG, bool TAB>
-__device__ inline void offdiag_element(const DynConst& K, const ExpTab& T,
-                                       const GaussPt* __restrict__ sg, int npg2, double xc,
-                                       double yc, double zc, const double nv[3], cd H[9],
-                                       cd G[9]) {
+template <bool NeedG>
+__device__ inline void offdiag_element(const DynConst& K, const GaussPt* __restrict__ sg, int npg2,
+                                       double xc, double yc, double zc, const double nv[3],
+                                       cd H[9], cd G[9]) {
     cd SD = mk(0, 0), SP[3], SRQ[9], Gpsi = mk(0, 0), Gchi[6];
     for (int k = 0; k < 3; ++k) SP[k] = mk(0, 0);
     for (int k = 0; k < 9; ++k) SRQ[k] = mk(0, 0);
@@ -485,13 +483,7 @@ __device__ inline void offdiag_element(const DynConst& K, const ExpTab& T,
         double rd[3] = {r1 * iR, r2 * iR, r3 * iR};
         double drn = rd[0] * nv[0] + rd[1] * nv[1] + rd[2] * nv[2];
         cd psi, chi, A, B, C;
-        if constexpr (TAB) {
-            cd Es, Epc;
-            exp_pair_tab(K, T, R, iR, Es, Epc);
-            dyn_scalars_e<NeedG>(K, R, iR, Es, Epc, psi, chi, A, B, C);
-        } else {
-            dyn_scalars<NeedG>(K, R, iR, psi, chi, A, B, C);
-        }
+        dyn_scalars<NeedG>(K, R, iR, psi, chi, A, B, C);
         double wq = p.w * K.q4pi;
         cd Ap = A * wq, Bp = B * (wq * drn), Cp = C * wq;
         SD = SD + Ap * drn;
@@ -527,6 +519,67 @@ __device__ inline void offdiag_element(const DynConst& K, const ExpTab& T,
         }
 }
 
+// H of one element with the exp tables.  The exponentials come without the
+// factor 1/R, so the scalars below are chi R, psi' R^2, chi' R^2 and
+// A R^2, B R^2, C R^2 (the relations of dyn_scalars_e multiplied through),
+// and 1/R^2 joins the quadrature weight.  SRQ_ij = sum B' drn r_i r_j +
+// (sum C' r_i) n_j: the first sum is symmetric (6 entries), the second a
+// vector.
+__device__ inline void offdiag_element_h_tab(const DynConst& K, const ExpTab& T,
+                                             const GaussPt* __restrict__ sg, int npg2, double xc,
+                                             double yc, double zc, const double nv[3], cd H[9]) {
+    cd SD = mk(0, 0), SP[3], SC[3], SB[6];
+    for (int k = 0; k < 3; ++k) SP[k] = SC[k] = mk(0, 0);
+    for (int k = 0; k < 6; ++k) SB[k] = mk(0, 0);
+    for (int q = 0; q < npg2; ++q) {
+        GaussPt p = sg[q];
+        double r1 = p.x - xc, r2 = p.y - yc, r3 = p.z - zc;
+        double iR = rsqrt_refined(r1 * r1 + r2 * r2 + r3 * r3);
+        double R = (r1 * r1 + r2 * r2 + r3 * r3) * iR;
+        double rd[3] = {r1 * iR, r2 * iR, r3 * iR};
+        double drn = rd[0] * nv[0] + rd[1] * nv[1] + rd[2] * nv[2];
+        cd Es, Epc;
+        exp_pair_tab(K, T, R, Es, Epc);
+        double iR2 = iR * iR;
+        cd as = K.as * iR, as2 = K.as2 * iR2;
+        cd ap = K.ap * iR, ap2 = K.ap2 * iR2;
+        cd zs = K.ks * R;
+        cd fs = mk(1.0 + 3.0 * (as2.x - as.x), 3.0 * (as2.y - as.y));
+        cd fp = mk(1.0 + 3.0 * (ap2.x - ap.x), 3.0 * (ap2.y - ap.y));
+        cd chiR = fs * Es - fp * Epc;
+        cd hs = mk(-2.0 + zs.x + 3.0 * (as.x - as2.x), zs.y + 3.0 * (as.y - as2.y));
+        cd hp = mk(-1.0 + 3.0 * (ap.x - ap2.x), 3.0 * (ap.y - ap2.y));
+        cd psidR2 = hs * Es - hp * Epc;
+        cd zp = K.kp * R;
+        cd ks_ = mk(zs.x - 4.0 + 9.0 * (as.x - as2.x), zs.y + 9.0 * (as.y - as2.y));
+        cd kp_ = mk(zp.x - 4.0 + 9.0 * (ap.x - ap2.x), zp.y + 9.0 * (ap.y - ap2.y));
+        cd chidR2 = ks_ * Es - kp_ * Epc;
+        cd A = psidR2 - chiR;
+        cd B = 4.0 * chiR - 2.0 * chidR2;
+        cd C = K.c1m2 * (A + 0.5 * B - 3.0 * chiR) - 2.0 * chiR;
+        double wq = p.w * K.q4pi * iR2;
+        cd Ap = A * wq, Bp = B * (wq * drn), Cp = C * wq;
+        SD = SD + Ap * drn;
+        for (int j = 0; j < 3; ++j) {
+            SP[j] = SP[j] + Ap * rd[j];
+            SC[j] = SC[j] + Cp * rd[j];
+        }
+        SB[0] = SB[0] + Bp * (rd[0] * rd[0]);
+        SB[1] = SB[1] + Bp * (rd[1] * rd[0]);
+        SB[2] = SB[2] + Bp * (rd[2] * rd[0]);
+        SB[3] = SB[3] + Bp * (rd[1] * rd[1]);
+        SB[4] = SB[4] + Bp * (rd[2] * rd[1]);
+        SB[5] = SB[5] + Bp * (rd[2] * rd[2]);
+    }
+    const int sidx[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) {
+            cd h = SB[sidx[i + 3 * j]] + SC[i] * nv[j] + SP[j] * nv[i];
+            if (i == j) h = h + SD;
+            H[i + 3 * j] = h;
+        }
+}
+
 // H of every off-diagonal block; columns with KODE = 0 are left to
 // k_dyn_offdiag_g, which writes their swapped content.
 template <int WAVES, int JT, bool TAB>
@@ -550,7 +603,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag_h(int nbe, int npg2,
         const int flag = colflag[J];
         const double nv[3] = {elem[J].nx, elem[J].ny, elem[J].nz};
         cd H[9], G[9];
-        offdiag_element<false, TAB>(K, T, sgp + jj * npg2, npg2, xc, yc, zc, nv, H, G);
+        if constexpr (TAB)
+            offdiag_element_h_tab(K, T, sgp + jj * npg2, npg2, xc, yc, zc, nv, H);
+        else
+            offdiag_element<false>(K, sgp + jj * npg2, npg2, xc, yc, zc, nv, H, G);
         if (rowok && I != J) {
             for (int k = 0; k < 3; ++k) {
                 if ((flag >> (1 + k)) & 1) continue;
@@ -603,8 +659,7 @@ __global__ __launch_bounds__(64 * JTG) void k_dyn_offdiag_g(
             const int flag = colflag[J];
             const double nv[3] = {elem[J].nx, elem[J].ny, elem[J].nz};
             cd H[9], G[9], Ho[9], Gn[9];
-            offdiag_element<true, false>(K, ExpTab{}, sgp + jj * npg2, npg2, xc, yc, zc, nv, H,
-                                          G);
+            offdiag_element<true>(K, sgp + jj * npg2, npg2, xc, yc, zc, nv, H, G);
             swap_cols(H, G, flag, K.zge, Ho, Gn);
             if (rowok && I != J) {
                 for (int k = 0; k < 3; ++k) {
