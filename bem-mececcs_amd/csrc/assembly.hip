@@ -524,11 +524,12 @@ __device__ inline void offdiag_element(const DynConst& K, const GaussPt* __restr
 // A R^2, B R^2, C R^2 (the relations of dyn_scalars_e multiplied through),
 // and 1/R^2 joins the quadrature weight.  SRQ_ij = sum B' drn r_i r_j +
 // (sum C' r_i) n_j: the first sum is symmetric (6 entries), the second a
-// vector.
+// vector.  The diagonal term sum A' drn is SP . n (n is fixed per element).
+// The staged weights already hold the factor 1/(4 pi).
 __device__ inline void offdiag_element_h_tab(const DynConst& K, const ExpTab& T,
                                              const GaussPt* __restrict__ sg, int npg2, double xc,
                                              double yc, double zc, const double nv[3], cd H[9]) {
-    cd SD = mk(0, 0), SP[3], SC[3], SB[6];
+    cd SP[3], SC[3], SB[6];
     for (int k = 0; k < 3; ++k) SP[k] = SC[k] = mk(0, 0);
     for (int k = 0; k < 6; ++k) SB[k] = mk(0, 0);
     for (int q = 0; q < npg2; ++q) {
@@ -557,9 +558,8 @@ __device__ inline void offdiag_element_h_tab(const DynConst& K, const ExpTab& T,
         cd A = psidR2 - chiR;
         cd B = 4.0 * chiR - 2.0 * chidR2;
         cd C = K.c1m2 * (A + 0.5 * B - 3.0 * chiR) - 2.0 * chiR;
-        double wq = p.w * K.q4pi * iR2;
+        double wq = p.w * iR2;
         cd Ap = A * wq, Bp = B * (wq * drn), Cp = C * wq;
-        SD = SD + Ap * drn;
         for (int j = 0; j < 3; ++j) {
             SP[j] = SP[j] + Ap * rd[j];
             SC[j] = SC[j] + Cp * rd[j];
@@ -571,6 +571,7 @@ __device__ inline void offdiag_element_h_tab(const DynConst& K, const ExpTab& T,
         SB[4] = SB[4] + Bp * (rd[2] * rd[1]);
         SB[5] = SB[5] + Bp * (rd[2] * rd[2]);
     }
+    const cd SD = SP[0] * nv[0] + SP[1] * nv[1] + SP[2] * nv[2];
     const int sidx[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
     for (int j = 0; j < 3; ++j)
         for (int i = 0; i < 3; ++i) {
@@ -592,7 +593,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag_h(int nbe, int npg2,
     extern __shared__ GaussPt sgp[];
     const int j0 = blockIdx.y * JT;
     const int jn = min(JT, nbe - j0);
-    for (int t = threadIdx.x; t < jn * npg2; t += 64 * WAVES) sgp[t] = gp[(size_t)j0 * npg2 + t];
+    for (int t = threadIdx.x; t < jn * npg2; t += 64 * WAVES) {
+        GaussPt g = gp[(size_t)j0 * npg2 + t];
+        if (TAB) g.w *= K.q4pi;
+        sgp[t] = g;
+    }
     __syncthreads();
     const int I = blockIdx.x * (64 * WAVES) + threadIdx.x;
     const bool rowok = I < nbe;

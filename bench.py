@@ -51,9 +51,9 @@ FLOPS_PER_TRIPLE = 242
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 # HBM bytes per launch of k_dyn_offdiag_h on the default workload, from the
 # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a plain streaming copy of
-# known size) recorded in profiles/pmc_traffic_c2.md: 48.6 MB read + 671.6 MB
+# known size) recorded in profiles/pmc_traffic_c2.md: 48.4 MB read + 671.6 MB
 # written (the ZH store itself is 671.8 MB).
-OFFDIAG_TRAFFIC_BYTES_C2 = 720.2e6
+OFFDIAG_TRAFFIC_BYTES_C2 = 720.0e6
 HBM_PEAK_GBS = 8000.0
 DTYPE = "f64 assembly + f32/bf16x3 factor + f64 refine"
 
@@ -81,12 +81,17 @@ def base_record(args, workload, value, ms_per_step, steps):
     }
 
 
-def stream_through_server(api, mode, slots, items, label):
+def stream_through_server(api, mode, slots, items, label, warm_items=()):
     """Submit every item at once to a request loop of `slots` slots; items
-    are (mesh, kode, zdfi, (ge, rnu, dam, rho), omega).  Returns (x list,
-    info list, wall seconds)."""
+    are (mesh, kode, zdfi, (ge, rnu, dam, rho), omega).  warm_items run
+    first on the same loop, untimed, so each slot has its workspace, its
+    mesh and static pass and its captured factorisation before the timed
+    items.  Returns (x list, info list, wall seconds) of the timed items."""
     srv = api.Server(0, nslots=slots, mode=mode)
     try:
+        if warm_items:
+            for i in [srv.submit(*it) for it in warm_items]:
+                srv.wait(i)
         t0 = time.perf_counter()
         ids = [srv.submit(*it) for it in items]
         outs = [srv.wait(i) for i in ids]
@@ -168,13 +173,12 @@ def workload_solve(args, api, decks):
     item = (mesh, dyn.kode, dyn.bc.astype(np.complex128), (mesh.ge, mesh.rnu, dyn.dam, dyn.rho),
             fr)
     n_stream = max(args.steps, 2 * args.slots)
-    stream_through_server(api, api.MODE_MIXED, args.slots, [item] * args.slots, "warm mixed")
+    warm = [item] * (max(1, args.warmup) * args.slots)
     xs, infos, wall = stream_through_server(api, api.MODE_MIXED, args.slots, [item] * n_stream,
-                                            "mixed stream")
+                                            "mixed stream", warm_items=warm)
     n64 = max(2, n_stream // 2)
-    stream_through_server(api, api.MODE_FP64, args.slots, [item] * args.slots, "warm fp64")
     x64s, _, wall64 = stream_through_server(api, api.MODE_FP64, args.slots, [item] * n64,
-                                            "fp64 stream")
+                                            "fp64 stream", warm_items=[item] * args.slots)
     errs.extend(rel_err(x, x64s[0]) for x in xs)
     value = n_stream / wall
     fp64_value = n64 / wall64
@@ -242,10 +246,10 @@ def workload_sweep(args, api, decks):
         xs, infos, wall = one_context_sweep(api, mesh, dyn, freqs, api.MODE_MIXED, "mixed sweep")
         x64, _, wall64 = one_context_sweep(api, mesh, dyn, freqs, api.MODE_FP64, "fp64 sweep")
     else:
-        stream_through_server(api, api.MODE_MIXED, args.slots, items[: args.slots], "warm mixed")
         xs, infos, wall = stream_through_server(api, api.MODE_MIXED, args.slots, items,
-                                                "mixed sweep")
-        x64, _, wall64 = stream_through_server(api, api.MODE_FP64, args.slots, items, "fp64 sweep")
+                                                "mixed sweep", warm_items=items[: args.slots])
+        x64, _, wall64 = stream_through_server(api, api.MODE_FP64, args.slots, items, "fp64 sweep",
+                                               warm_items=items[: args.slots])
     errs = [rel_err(a, b) for a, b in zip(xs, x64)]
     svc = [i["service_ms"] for i in infos]
     value = len(items) / wall
