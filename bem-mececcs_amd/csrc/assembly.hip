@@ -19,6 +19,7 @@
 //                    (Interec.for:86-135, 188-283, Sigmaec.for)
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "bemx_common.hpp"
 
@@ -952,9 +953,11 @@ int offdiag_tiles(int nbe) { return (nbe + OG_JT - 1) / OG_JT; }
 
 // Entries of each exp table of the H kernel for the wave numbers of K and
 // distances up to rmax, with the grid spacing in *h; 0 when more than
-// EXP_TAB_MAX entries would be needed (the H kernel then evaluates exp and
-// sincos at every Gauss point).
+// EXP_TAB_MAX entries would be needed or BEMX_EXP_TABLE=0 is set (the H
+// kernel then evaluates exp and sincos at every Gauss point).
 int exp_table_entries(const DynConst& K, double rmax, double* h) {
+    const char* env = getenv("BEMX_EXP_TABLE");
+    if (env && env[0] == '0') return 0;
     const double kmax = std::max(std::hypot(K.ks.x, K.ks.y), std::hypot(K.kp.x, K.kp.y));
     if (!(kmax > 0.0) || !(rmax > 0.0) || !std::isfinite(kmax * rmax)) return 0;
     const double hh = std::exp2(std::floor(std::log2(1.0 / (16.0 * kmax))));

@@ -70,6 +70,27 @@ def test_static_and_dynamic_assembly(ctx, orc, case):
     assert _vec_close(zfi, ozfi) < RTOL
 
 
+@pytest.mark.parametrize("case", ["c1", "hifreq"])
+def test_dynamic_assembly_without_exp_table(ctx, orc, case, monkeypatch):
+    # BEMX_EXP_TABLE=0: the H kernel evaluates exp and sincos at every Gauss
+    # point, the route it takes when the table would exceed its maximum size
+    monkeypatch.setenv("BEMX_EXP_TABLE", "0")
+    mesh, dyn = _problem(case)
+    print("case", case, "nbe", mesh.nbe, "without exp table", flush=True)
+    ctx.send_shared_data(mesh)
+    hd, gd, consts = ctx.ghmatece(mesh.ge, mesh.rnu)
+    ohd, ogd, oconsts = orc.ghmatece(mesh)
+    zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
+    fr = float(dyn.freqs[0])
+    zdfi = dyn.bc.astype(np.complex128)
+    ctx.send_dyn_data(zdfi, dyn.kode)
+    zh, zfi = ctx.ghmatecd(zmat, fr)
+    ozh, ozfi = orc.ghmatecd(mesh, ohd, ogd, oconsts, dyn.kode, zdfi, fr, zmat)
+    ok, worst = close_blocks(zh, ozh, RTOL)
+    assert ok, ("ZH", worst)
+    assert _vec_close(zfi, ozfi) < RTOL
+
+
 @pytest.mark.parametrize("case", ["c1", "esolo240", "npg6"])
 def test_interior_points(ctx, orc, case):
     mesh, dyn = _problem(case)
