@@ -8,8 +8,10 @@ a missing or unloadable library raises at import time.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 from typing import Optional, Tuple
 
 import numpy as np
@@ -30,7 +32,7 @@ EXPORTS = [
     "bemx_deallocate_shared_data", "bemx_ghmatece", "bemx_send_dyn_data", "bemx_ghmatecd",
     "bemx_linsolve", "bemx_interec", "bemx_solve", "bemx_device_buffers", "bemx_last_stats",
     "bemx_send_load_cases", "bemx_material", "bemx_server_create", "bemx_server_submit",
-    "bemx_server_wait", "bemx_server_destroy", "bemx_request_footprint",
+    "bemx_server_wait", "bemx_server_destroy", "bemx_request_footprint", "bemx_server_stats",
 ]
 
 
@@ -63,7 +65,7 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
                                  C.c_double, C.c_double, _dp, _dp]
     lib.bemx_solve.argtypes = [vp, _dp, _dp, _dp, C.c_double, C.c_int, _dp, _dp]
     lib.bemx_device_buffers.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
-    lib.bemx_last_stats.argtypes = [vp, _dp]
+    lib.bemx_last_stats.argtypes = [vp, _dp, C.c_int]
     lib.bemx_material.argtypes = [C.c_double] * 4 + [_dp] * 3
     lib.bemx_server_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(vp)]
     lib.bemx_server_submit.argtypes = ([vp, _dp, _dp, _dp, _ip] + [C.c_int] * 4 +
@@ -71,12 +73,28 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
                                                            C.POINTER(C.c_longlong)])
     lib.bemx_server_wait.argtypes = [vp, C.c_longlong, _dp, _dp, C.c_char_p, C.c_int]
     lib.bemx_server_destroy.argtypes = [vp]
-    lib.bemx_request_footprint.argtypes = [C.c_int, C.c_int, C.c_int]
+    lib.bemx_request_footprint.argtypes = [C.c_int] * 6
+    lib.bemx_server_stats.argtypes = [vp, _dp, C.c_int]
     lib.bemx_request_footprint.restype = C.c_double
     return lib
 
 
 _LIB: Optional[C.CDLL] = None
+
+# Every live Context and Server.  They are closed at interpreter exit (before
+# the interpreter's own finalisation and before the HIP runtime's exit
+# handlers run), so device objects are released in a fixed order and never
+# from a finaliser late in the shutdown.
+_LIVE: "weakref.WeakSet" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all() -> None:
+    for obj in list(_LIVE):
+        try:
+            obj.close()
+        except Exception:
+            pass
 
 
 def lib() -> C.CDLL:
@@ -113,7 +131,7 @@ def gauleg(npg: int) -> Tuple[np.ndarray, np.ndarray]:
 
 def material(ge: float, rnu: float, dam: float, rho: float) -> np.ndarray:
     """mu* = G(1 + 2i DAM), cs = sqrt(mu*/rho), cp = cs sqrt((2-2nu)/(1-2nu))
-    (Inputecd.for:166-168)."""
+    (Inputecd.for:58-60)."""
     zge = complex(ge, ge * 2.0 * dam)
     zcs = np.sqrt(zge / rho)
     zcp = zcs * np.sqrt((2.0 - 2.0 * rnu) / (1.0 - 2.0 * rnu))
@@ -132,11 +150,13 @@ class Context:
         self.h = h
         self.mesh = None
         self.consts = None
+        _LIVE.add(self)
 
     def close(self):
-        if self.h:
-            self._lib.bemx_destroy(self.h)
+        h = getattr(self, "h", None)
+        if h:
             self.h = None
+            self._lib.bemx_destroy(h)
 
     def __del__(self):
         try:
@@ -262,10 +282,12 @@ class Context:
         return zfi, zdfi, zdsol, zssol
 
     def last_stats(self) -> dict:
-        out = np.zeros(8)
-        self._check(self._lib.bemx_last_stats(self.h, _p(out)), "last_stats")
+        out = np.zeros(11)
+        self._check(self._lib.bemx_last_stats(self.h, _p(out), 11), "last_stats")
         return dict(gmres_iters=int(out[0]), relres=out[1], fell_back=bool(out[2]),
-                    solve_ms=out[3], offdiag_ms=out[4], assembly_ms=out[5], factor_ms=out[6])
+                    solve_ms=out[3], offdiag_ms=out[4], assembly_ms=out[5], lu_ms=out[6],
+                    graph_captures=int(out[7]), lu_factor_ms=out[8], lu_refine_ms=out[9],
+                    graph_replays=int(out[10]))
 
     # device-resident solve of one frequency
     def solve(self, zmat, fr: float, mode: int = MODE_FP64, want_x: bool = True):
@@ -291,20 +313,26 @@ class Server:
     def __init__(self, device: int = 0, nslots: int = 2, hbm_budget: float = 0.0,
                  mode: int = MODE_MIXED):
         self._lib = lib()
+        self.h = None
         h = C.c_void_p()
         code = self._lib.bemx_server_create(device, nslots, float(hbm_budget), mode, C.byref(h))
         if code:
             raise BemxError(code, "bemx_server_create(nslots=%d)" % nslots)
         self.h = h
         self._sizes = {}
+        _LIVE.add(self)
 
     def close(self):
-        if self.h:
-            self._lib.bemx_server_destroy(self.h)
+        h = getattr(self, "h", None)
+        if h:
             self.h = None
+            self._lib.bemx_server_destroy(h)
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def submit(self, mesh, kode, zdfi, zmat_params, fr: float) -> int:
         """zdfi: (3NBE,) or (3NBE, nrhs) complex load cases; zmat_params =
@@ -324,6 +352,15 @@ class Server:
         self._sizes[rid.value] = zdfi.shape
         return rid.value
 
+    def stats(self) -> dict:
+        out = np.zeros(8)
+        code = self._lib.bemx_server_stats(self.h, _p(out), 8)
+        if code:
+            raise BemxError(code, "bemx_server_stats")
+        keys = ("completed", "graph_captures", "graph_replays", "charged_bytes",
+                "peak_charged_bytes", "budget_bytes", "workspace_releases", "over_charge")
+        return {k: (int(v) if k != "budget_bytes" else float(v)) for k, v in zip(keys, out)}
+
     def wait(self, rid: int):
         """-> (x (3NBE,) or (3NBE, nrhs), info dict)."""
         shape = self._sizes.pop(rid)
@@ -340,9 +377,10 @@ class Server:
         return x, dict(zip(keys, info.tolist()))
 
 
-def request_footprint(nbe: int, nrhs: int = 1, mode: int = MODE_MIXED) -> int:
-    """Bytes the request loop charges a request against its HBM budget."""
-    return int(lib().bemx_request_footprint(nbe, nrhs, mode))
+def request_footprint(mesh, nrhs: int = 1, mode: int = MODE_MIXED) -> int:
+    """Bytes the request loop charges a slot that has served a request on
+    ``mesh`` with ``nrhs`` load cases (bound of its context's workspace)."""
+    return int(lib().bemx_request_footprint(mesh.nbe, mesh.n, mesh.np_, mesh.npg, nrhs, mode))
 
 
 def material_c(ge: float, rnu: float, dam: float, rho: float) -> np.ndarray:

@@ -5,7 +5,7 @@ Reads and writes the reference's input decks:
     NP lines ``x y z``; N lines of 4 node ids written as reals (truncated
     with INT); the first NBE elements are boundary elements, the rest are
     enclosing elements.
-  * D deck (dynamic): Inputecd.for:136-242 -- title; ``L NFR DAM RHO``; NFR
+  * D deck (dynamic): Inputecd.for:26-134 -- title; ``L NFR DAM RHO``; NFR
     frequencies; NBE lines ``KODE_x BC_x KODE_y BC_y KODE_z BC_z``; L lines of
     interior point coordinates.
 Every READ statement of the reference is a Fortran list-directed read: it
@@ -150,7 +150,7 @@ def read_e_deck(path_or_text: str, is_text: bool = False) -> Mesh:
 
 
 def read_d_deck(path_or_text: str, nbe: int, is_text: bool = False) -> Dynamic:
-    """INPUTECD (Inputecd.for:115-247)."""
+    """INPUTECD (Inputecd.for:7-139)."""
     text = path_or_text if is_text else open(path_or_text, "rb").read().decode("latin-1")
     rd = ListDirectedReader(text)
     title = rd.read_line()[:75]

@@ -3,7 +3,7 @@
 // FP64 throughout.  Kernels and the reference routines whose results they
 // reproduce:
 //   k_element_setup  NORMVEC (Normvec.for:4-39), centroids (Inputece.for:100-102)
-//                    and the per-element Gauss geometry of Nonsingd.for:108-176
+//                    and the per-element Gauss geometry of Nonsingd.for:45-106
 //   k_static_hdiag   GHMATECE off-diagonal loop + RIGID_BODY
 //                    (Ghmatece.for:92-150, 258-282): the static H row sums,
 //                    without ever storing the static H matrix
@@ -105,7 +105,7 @@ __global__ void k_element_setup(int n, const int* __restrict__ cone, const doubl
 // elements J of this split: T^st summed over J != I and its Gauss points, in
 // the factored form  T_ij = C3 [C4 d_ij S0 + 3 S2_ij + C4 (A_ij - A_ji)]  with
 // S0 = sum w drn/R^2, S2 = sum w drn r_i r_j/R^2 and A_ij = sum_J n_i(J) S1_j(J),
-// S1 = sum w r_j/R^2  (Solfune.for:139-152).  part[s][I][9] gets this split's
+// S1 = sum w r_j/R^2  (Solfune.for:27-31).  part[s][I][9] gets this split's
 // sum; k_static_hdiag_reduce negates and sums the splits in a fixed order.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_static_hdiag(int n, int nbe, int npg2, int jper,
@@ -286,9 +286,67 @@ __device__ inline void dyn_scalars_e(const DynConst& K, double R, double iR, cd 
     C = K.c1m2 * (A + 0.5 * B - 3.0 * chr) - 2.0 * chr;
 }
 
+// t z + c for complex t, z and real c.
+__device__ inline cd mad_re(cd t, cd z, double c) {
+    return cd{fma(t.x, z.x, fma(-t.y, z.y, c)), fma(t.x, z.y, t.y * z.x)};
+}
+
+// A R^2, B R^2, C R^2 from their series in z = ks R (|z| < SER_TAU), Horner
+// from the highest term; M0 = 1 leaves out the m = 0 terms (the static
+// limits).
+template <int M0>
+__device__ inline void series_abc(const DynConst& K, cd z, cd& AR2, cd& BR2, cd& CR2) {
+    cd ta = mk(K.ser[SER_A][NSER - 1], 0.0), tb = mk(K.ser[SER_B][NSER - 1], 0.0),
+       tc = mk(K.ser[SER_C][NSER - 1], 0.0);
+#pragma unroll
+    for (int m = NSER - 2; m >= M0; --m) {
+        ta = mad_re(ta, z, K.ser[SER_A][m]);
+        tb = mad_re(tb, z, K.ser[SER_B][m]);
+        tc = mad_re(tc, z, K.ser[SER_C][m]);
+    }
+    if (M0 == 1) {
+        ta = ta * z;
+        tb = tb * z;
+        tc = tc * z;
+    }
+    AR2 = ta;
+    BR2 = tb;
+    CR2 = tc;
+}
+
+// psi R and chi R from their series, as series_abc.
+template <int M0>
+__device__ inline void series_psichi(const DynConst& K, cd z, cd& psiR, cd& chiR) {
+    cd tp = mk(K.ser[SER_PSI][NSER - 1], 0.0), tc = mk(K.ser[SER_CHI][NSER - 1], 0.0);
+#pragma unroll
+    for (int m = NSER - 2; m >= M0; --m) {
+        tp = mad_re(tp, z, K.ser[SER_PSI][m]);
+        tc = mad_re(tc, z, K.ser[SER_CHI][m]);
+    }
+    if (M0 == 1) {
+        tp = tp * z;
+        tc = tc * z;
+    }
+    psiR = tp;
+    chiR = tc;
+}
+
 template <bool WithPsi>
 __device__ inline void dyn_scalars(const DynConst& K, double R, double iR, cd& psi, cd& chi,
                                    cd& A, cd& B, cd& C) {
+    if (K.ks_abs2 * (R * R) < SER_TAU * SER_TAU) {
+        const cd z = K.ks * R;
+        const double iR2 = iR * iR;
+        cd psiR, chiR, AR2, BR2, CR2;
+        series_psichi<0>(K, z, psiR, chiR);
+        series_abc<0>(K, z, AR2, BR2, CR2);
+        psi = psiR * iR;
+        chi = chiR * iR;
+        A = AR2 * iR2;
+        B = BR2 * iR2;
+        C = CR2 * iR2;
+        return;
+    }
     dyn_scalars_e<WithPsi>(K, R, iR, exp_over_r(K.ks, R, iR), K.c2 * exp_over_r(K.kp, R, iR),
                            psi, chi, A, B, C);
 }
@@ -407,20 +465,42 @@ __global__ __launch_bounds__(64) void k_dyn_diag(int nbe, int npg2, const ElemDa
         double iR = 1.0 / R;
         double rd[3] = {r[0] * iR, r[1] * iR, r[2] * iR};
         double drn = rd[0] * nv[0] + rd[1] * nv[1] + rd[2] * nv[2];
-        cd psi, chi, A, B, C;
-        dyn_scalars<true>(K, R, iR, psi, chi, A, B, C);
+        // dynamic minus static kernels (Solfundif.for:56-69) as
+        //   U = ud d_ij - ur r_i r_j,
+        //   T = ta (drn d_ij + r_j n_i) + tb drn r_i r_j + tc r_i n_j:
+        // below SER_TAU from the series without its static limits plus the
+        // residuals of those limits, above it from the closed form minus
+        // the static kernels
+        cd ud, ur, ta, tb, tc;
+        const double iR2 = iR * iR;
+        if (K.ks_abs2 * (R * R) < SER_TAU * SER_TAU) {
+            const cd z = K.ks * R;
+            cd psiR, chiR, AR2, BR2, CR2;
+            series_psichi<1>(K, z, psiR, chiR);
+            series_abc<1>(K, z, AR2, BR2, CR2);
+            ud = (K.u_d0 + K.c0 * psiR) * iR;
+            ur = (K.u_r0 + K.c0 * chiR) * iR;
+            ta = (mk(K.t_a0, 0.0) + AR2 * K.q4pi) * iR2;
+            tb = (mk(K.t_b0, 0.0) + BR2 * K.q4pi) * iR2;
+            tc = (mk(K.t_c0, 0.0) + CR2 * K.q4pi) * iR2;
+        } else {
+            cd psi, chi, A, B, C;
+            dyn_scalars_e<true>(K, R, iR, exp_over_r(K.ks, R, iR),
+                                K.c2 * exp_over_r(K.kp, R, iR), psi, chi, A, B, C);
+            const double ust = K.c1s * iR, tst = K.c3s * iR2;
+            ud = K.c0 * psi - mk(ust * K.c2s, 0.0);
+            ur = K.c0 * chi + mk(ust, 0.0);
+            ta = A * K.q4pi - mk(tst * K.c4s, 0.0);
+            tb = B * K.q4pi - mk(3.0 * tst, 0.0);
+            tc = C * K.q4pi + mk(tst * K.c4s, 0.0);
+        }
         double w = p.w;
-        double ust = K.c1s * iR, tst = K.c3s * iR * iR;
         for (int j = 0; j < 3; ++j)
             for (int i = 0; i < 3; ++i) {
                 double dij = (i == j) ? 1.0 : 0.0;
-                // Solfundif.for:373-383
-                cd u = K.c0 * (psi * dij - chi * (rd[j] * rd[i]));
-                u.x -= ust * (K.c2s * dij + rd[i] * rd[j]);
-                cd t = K.q4pi * (A * (drn * dij + rd[j] * nv[i]) + B * (rd[i] * rd[j] * drn) +
-                                 C * (rd[i] * nv[j]));
-                t.x -= tst * (drn * (K.c4s * dij + 3.0 * rd[i] * rd[j]) +
-                              K.c4s * (rd[j] * nv[i] - rd[i] * nv[j]));
+                cd u = ud * dij - ur * (rd[i] * rd[j]);
+                cd t = ta * (drn * dij + rd[j] * nv[i]) + tb * (rd[i] * rd[j] * drn) +
+                       tc * (rd[i] * nv[j]);
                 int k = i + 3 * j;
                 accH[2 * k] += t.x * w;
                 accH[2 * k + 1] += t.y * w;
@@ -536,29 +616,35 @@ __device__ inline void offdiag_element_h_tab(const DynConst& K, const ExpTab& T,
     for (int q = 0; q < npg2; ++q) {
         GaussPt p = sg[q];
         double r1 = p.x - xc, r2 = p.y - yc, r3 = p.z - zc;
-        double iR = rsqrt_refined(r1 * r1 + r2 * r2 + r3 * r3);
-        double R = (r1 * r1 + r2 * r2 + r3 * r3) * iR;
+        const double RR = r1 * r1 + r2 * r2 + r3 * r3;
+        double iR = rsqrt_refined(RR);
+        double R = RR * iR;
         double rd[3] = {r1 * iR, r2 * iR, r3 * iR};
         double drn = rd[0] * nv[0] + rd[1] * nv[1] + rd[2] * nv[2];
-        cd Es, Epc;
-        exp_pair_tab(K, T, R, Es, Epc);
         double iR2 = iR * iR;
-        cd as = K.as * iR, as2 = K.as2 * iR2;
-        cd ap = K.ap * iR, ap2 = K.ap2 * iR2;
-        cd zs = K.ks * R;
-        cd fs = mk(1.0 + 3.0 * (as2.x - as.x), 3.0 * (as2.y - as.y));
-        cd fp = mk(1.0 + 3.0 * (ap2.x - ap.x), 3.0 * (ap2.y - ap.y));
-        cd chiR = fs * Es - fp * Epc;
-        cd hs = mk(-2.0 + zs.x + 3.0 * (as.x - as2.x), zs.y + 3.0 * (as.y - as2.y));
-        cd hp = mk(-1.0 + 3.0 * (ap.x - ap2.x), 3.0 * (ap.y - ap2.y));
-        cd psidR2 = hs * Es - hp * Epc;
-        cd zp = K.kp * R;
-        cd ks_ = mk(zs.x - 4.0 + 9.0 * (as.x - as2.x), zs.y + 9.0 * (as.y - as2.y));
-        cd kp_ = mk(zp.x - 4.0 + 9.0 * (ap.x - ap2.x), zp.y + 9.0 * (ap.y - ap2.y));
-        cd chidR2 = ks_ * Es - kp_ * Epc;
-        cd A = psidR2 - chiR;
-        cd B = 4.0 * chiR - 2.0 * chidR2;
-        cd C = K.c1m2 * (A + 0.5 * B - 3.0 * chiR) - 2.0 * chiR;
+        cd A, B, C;
+        if (K.ks_abs2 * RR < SER_TAU * SER_TAU) {
+            series_abc<0>(K, K.ks * R, A, B, C);
+        } else {
+            cd Es, Epc;
+            exp_pair_tab(K, T, R, Es, Epc);
+            cd as = K.as * iR, as2 = K.as2 * iR2;
+            cd ap = K.ap * iR, ap2 = K.ap2 * iR2;
+            cd zs = K.ks * R;
+            cd fs = mk(1.0 + 3.0 * (as2.x - as.x), 3.0 * (as2.y - as.y));
+            cd fp = mk(1.0 + 3.0 * (ap2.x - ap.x), 3.0 * (ap2.y - ap.y));
+            cd chiR = fs * Es - fp * Epc;
+            cd hs = mk(-2.0 + zs.x + 3.0 * (as.x - as2.x), zs.y + 3.0 * (as.y - as2.y));
+            cd hp = mk(-1.0 + 3.0 * (ap.x - ap2.x), 3.0 * (ap.y - ap2.y));
+            cd psidR2 = hs * Es - hp * Epc;
+            cd zp = K.kp * R;
+            cd ks_ = mk(zs.x - 4.0 + 9.0 * (as.x - as2.x), zs.y + 9.0 * (as.y - as2.y));
+            cd kp_ = mk(zp.x - 4.0 + 9.0 * (ap.x - ap2.x), zp.y + 9.0 * (ap.y - ap2.y));
+            cd chidR2 = ks_ * Es - kp_ * Epc;
+            A = psidR2 - chiR;
+            B = 4.0 * chiR - 2.0 * chidR2;
+            C = K.c1m2 * (A + 0.5 * B - 3.0 * chiR) - 2.0 * chiR;
+        }
         double wq = p.w * iR2;
         cd Ap = A * wq, Bp = B * (wq * drn), Cp = C * wq;
         for (int j = 0; j < 3; ++j) {
@@ -1032,6 +1118,13 @@ hipError_t launch_interior(hipStream_t st, int n, int nbe, int npg2, int l, cons
     hipLaunchKernelGGL(k_interior_reduce, dim3((l + 63) / 64), dim3(64), 0, st, l, nblk, part,
                        zdsol, zssol);
     return hipGetLastError();
+}
+
+size_t exp_table_max_bytes() { return sizeof(cd) * 2 * (size_t)EXP_TAB_MAX; }
+
+hipError_t preload_assembly_code() {
+    hipFuncAttributes a;
+    return hipFuncGetAttributes(&a, (const void*)k_exp_table);
 }
 
 }  // namespace bemx

@@ -30,6 +30,7 @@ hipError_t launch_dyn_assembly(hipStream_t, int, int, const ElemData*, const Gau
                                int*, const DynConst&, cd*, int, double, cd*, size_t, cd*, cd*,
                                cd*, hipEvent_t, hipEvent_t);
 int offdiag_tiles(int nbe);
+size_t exp_table_max_bytes();
 int exp_table_entries(const DynConst& K, double rmax, double* h);
 hipError_t launch_unswap(hipStream_t, int, const int*, cd*, cd*, cd);
 int interior_blocks(int nbe);
@@ -37,6 +38,8 @@ hipError_t launch_interior(hipStream_t, int, int, int, int, const ElemData*, con
                            const double*, const double*, const double*, const cd*, const cd*,
                            const DynConst&, cd*, cd*, cd*);
 }  // namespace bemx
+
+constexpr int NLAST = 11;  // doubles of bemx_last_stats
 
 // A grow-only device buffer.
 struct DevBuf {
@@ -88,7 +91,7 @@ struct bemx_ctx {
     // ev0..ev1: whole solve; ev2..ev3: off-diagonal assembly kernel;
     // ev1 of assembly end is ev4
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr;
-    double last[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double last[NLAST] = {0};
 };
 
 namespace {
@@ -119,6 +122,47 @@ int check_status(bemx_ctx* c) {
     return BEMX_OK;
 }
 
+// Series coefficients of the small-|z| kernel scalars (see DynConst), in
+// long double on the host: f_m = sum_{j=-2..min(1,m)} (p_j - q_j r^(m+2)) /
+// (m-j)! for psi R, chi R, psi' R^2, chi' R^2 (P, Q of Solfund.for:42-49),
+// then A R^2 = psi' R^2 - chi R, B R^2 = 4 chi R - 2 chi' R^2 and
+// C R^2 = (1/r^2 - 2)(A R^2 + B R^2 / 2 - 3 chi R) - 2 chi R (Solfund.for:51-53).
+void fill_series(DynConst& K) {
+    static const int P[4][4] = {{1, -1, 1, 0}, {3, -3, 1, 0}, {-3, 3, -2, 1}, {-9, 9, -4, 1}};
+    static const int Q[4][4] = {{1, -1, 0, 0}, {3, -3, 1, 0}, {-3, 3, -1, 0}, {-9, 9, -4, 1}};
+    const long double r2 = K.c2.x, r = std::sqrt(r2), c1m2 = 1.0L / r2 - 2.0L;
+    long double f[4][NSER], fact[NSER + 2];
+    fact[0] = 1.0L;
+    for (int i = 1; i < NSER + 2; ++i) fact[i] = fact[i - 1] * i;
+    for (int k = 0; k < 4; ++k)
+        for (int m = 0; m < NSER; ++m) {
+            long double acc = 0.0L, rp = 1.0L;
+            for (int i = 0; i < m + 2; ++i) rp *= r;
+            for (int j = -2; j <= 1 && j <= m; ++j)
+                acc += ((long double)P[k][j + 2] - (long double)Q[k][j + 2] * rp) / fact[m - j];
+            f[k][m] = acc;
+        }
+    long double fa[NSER], fb[NSER], fc[NSER];
+    for (int m = 0; m < NSER; ++m) {
+        fa[m] = f[2][m] - f[1][m];
+        fb[m] = 4.0L * f[1][m] - 2.0L * f[3][m];
+        fc[m] = c1m2 * (fa[m] + 0.5L * fb[m] - 3.0L * f[1][m]) - 2.0L * f[1][m];
+        K.ser[SER_PSI][m] = (double)f[0][m];
+        K.ser[SER_CHI][m] = (double)f[1][m];
+        K.ser[SER_A][m] = (double)fa[m];
+        K.ser[SER_B][m] = (double)fb[m];
+        K.ser[SER_C][m] = (double)fc[m];
+    }
+    K.ks_abs2 = K.ks.x * K.ks.x + K.ks.y * K.ks.y;
+    const long double q4pi = K.q4pi, c3 = K.c3s, c4 = K.c4s;
+    K.t_a0 = (double)(q4pi * fa[0] - c3 * c4);
+    K.t_b0 = (double)(q4pi * fb[0] - 3.0L * c3);
+    K.t_c0 = (double)(q4pi * fc[0] + c3 * c4);
+    const cd fpsi = mk((double)f[0][0], 0.0), fchi = mk((double)f[1][0], 0.0);
+    K.u_d0 = K.c0 * fpsi - mk(K.c1s * K.c2s, 0.0);
+    K.u_r0 = K.c0 * fchi + mk(K.c1s, 0.0);
+}
+
 DynConst make_dyn_const(const bemx_ctx* c, cd zge, cd zcs, cd zcp, double fr, double rho) {
     DynConst K;
     const double pi = c->consts[0];
@@ -141,12 +185,80 @@ DynConst make_dyn_const(const bemx_ctx* c, cd zge, cd zcs, cd zcp, double fr, do
     K.c2s = c->consts[2];
     K.c3s = c->consts[3];
     K.c4s = c->consts[4];
+    fill_series(K);
     return K;
 }
 
 cd ld_c(const double* p) { return mk(p[0], p[1]); }
 
 }  // namespace
+
+namespace bemx {
+
+// Every device buffer of a context, the status word excepted.
+static void ctx_buffers(bemx_ctx* c, std::vector<DevBuf*>& out) {
+    out = {&c->cone, &c->cx, &c->cy, &c->cz, &c->gi, &c->ome, &c->elem, &c->gp, &c->gpT,
+           &c->hdiag, &c->gdiag, &c->hpart, &c->kode, &c->zdfi, &c->colflag, &c->flist,
+           &c->rhs_part, &c->rhs_diag, &c->zfi, &c->zh, &c->etab, &c->ipts, &c->ipart,
+           &c->zdsol, &c->zssol, &c->ifi, &c->idfi, &c->ikode};
+}
+
+size_t ctx_device_bytes(bemx_ctx* c) {
+    std::vector<DevBuf*> bufs;
+    ctx_buffers(c, bufs);
+    size_t b = c->status.p ? 256 : 0;
+    for (DevBuf* d : bufs) b += d->p ? std::max<size_t>(d->cap, 256) : 0;
+    return b + c->lu.bytes();
+}
+
+void ctx_release_buffers(bemx_ctx* c) {
+    hipSetDevice(c->device);
+    if (c->st) hipStreamSynchronize(c->st);
+    std::vector<DevBuf*> bufs;
+    ctx_buffers(c, bufs);
+    for (DevBuf* d : bufs) d->release();
+    c->status.release();
+    c->lu.release();
+    c->mesh_ok = c->static_ok = c->dyn_ok = c->zh_valid = false;
+    c->zh_host = nullptr;
+}
+
+void ctx_release_workspace(bemx_ctx* c) {
+    ctx_release_buffers(c);
+    // keep the status word: every entry point expects it
+    (void)c->status.ensure(sizeof(int));
+}
+
+// Upper bound of the device bytes a context holds after serving meshes of at
+// most these sizes through bemx_send_shared_data, bemx_ghmatece,
+// bemx_send_load_cases and bemx_solve: every buffer those calls grow (they
+// only grow), at its largest.  The FP64 solver workspace is counted in both
+// modes, since the mixed path falls back to it.  Monotone in each argument,
+// so the bound over several requests is the bound at their maxima.
+size_t ctx_solve_bytes_bound(int n, int nbe, int np, int npg, int nrhs, int mode) {
+    auto r = [](size_t b) { return std::max<size_t>(b, 256); };
+    const size_t nn = 3 * (size_t)nbe, npg2 = (size_t)npg * npg, rows = (nbe + 63) / 64;
+    size_t b = r(sizeof(int));  // status word
+    b += r(sizeof(int) * 4 * (size_t)n) + 3 * r(sizeof(double) * np) + 2 * r(sizeof(double) * npg) +
+         r(sizeof(ElemData) * (size_t)n) + 2 * r(sizeof(GaussPt) * (size_t)n * npg2);
+    // static diagonals; the split partials of bemx_ghmatece hold 9 nbe nsplit
+    // doubles with nsplit <= 2048 / rows + 1 and nbe <= 64 rows
+    b += 2 * r(sizeof(double) * 9 * nbe) + r(sizeof(double) * 9 * 64 * (2048 + rows));
+    b += r(sizeof(cd) * nn * nrhs) + r(sizeof(int) * nn);
+    b += r(sizeof(int) * nbe) + r(sizeof(int) * (nbe + 1)) +
+         r(sizeof(cd) * nn * offdiag_tiles(nbe) * nrhs) + 2 * r(sizeof(cd) * nn * nrhs) +
+         r(sizeof(cd) * nn * nn) + r(exp_table_max_bytes());
+    b += fp64_factor_bytes((int)nn);
+    if (mode == BEMX_MODE_MIXED) b += mixed_workspace_bytes((int)nn);
+    return b;
+}
+
+void ctx_graph_counts(const bemx_ctx* c, long long* captures, long long* replays) {
+    *captures = c->lu.captures;
+    *replays = c->lu.replays;
+}
+
+}  // namespace bemx
 
 extern "C" {
 
@@ -159,11 +271,16 @@ int bemx_create(int device, bemx_ctx** out) {
     if (hipSetDevice(device) != hipSuccess) return BEMX_ERR_HIP;
     bemx_ctx* c = new bemx_ctx();
     c->device = device;
+    // The creating thread also loads the device code of every kernel file,
+    // so no other thread (a request-loop worker) makes the process's first
+    // load of it.
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess ||
-        hipEventCreate(&c->ev4) != hipSuccess || c->status.ensure(sizeof(int)) != hipSuccess) {
-        delete c;
+        hipEventCreate(&c->ev4) != hipSuccess || c->status.ensure(sizeof(int)) != hipSuccess ||
+        preload_assembly_code() != hipSuccess || preload_lu_code() != hipSuccess ||
+        preload_mixed_code() != hipSuccess) {
+        bemx_destroy(c);
         return BEMX_ERR_HIP;
     }
     *out = c;
@@ -173,18 +290,12 @@ int bemx_create(int device, bemx_ctx** out) {
 int bemx_destroy(bemx_ctx* c) {
     if (!c) return BEMX_ERR_ARG;
     hipSetDevice(c->device);
-    hipStreamSynchronize(c->st);
-    DevBuf* bufs[] = {&c->cone, &c->cx, &c->cy, &c->cz, &c->gi, &c->ome, &c->elem,
-                      &c->gp, &c->gpT, &c->status, &c->hdiag, &c->gdiag, &c->hpart,
-                      &c->kode, &c->zdfi, &c->colflag, &c->flist, &c->rhs_part, &c->rhs_diag,
-                      &c->zfi, &c->zh, &c->etab, &c->ipts, &c->ipart, &c->zdsol, &c->zssol,
-                      &c->ifi, &c->idfi, &c->ikode};
-    for (DevBuf* b : bufs) b->release();
-    c->lu.release();
+    if (c->st) hipStreamSynchronize(c->st);
+    bemx::ctx_release_buffers(c);
     hipEvent_t evs[] = {c->ev0, c->ev1, c->ev2, c->ev3, c->ev4};
     for (hipEvent_t e : evs)
         if (e) hipEventDestroy(e);
-    hipStreamDestroy(c->st);
+    if (c->st) hipStreamDestroy(c->st);
     delete c;
     return BEMX_OK;
 }
@@ -476,7 +587,8 @@ int bemx_linsolve(bemx_ctx* c, int nn, double* zh, double* zfi, int mode, long l
     int s = bemx::lu_solve(c->lu, c->st, (int)n, c->zh.as<cd>(), n, c->zfi.as<cd>(), 1, mode,
                            &inf, st, c->err);
     c->zh_valid = false;  // the factorisation overwrote the matrix
-    double last[8] = {st[0], st[1], st[2], 0, 0, 0, 0, 0};
+    double last[NLAST] = {st[0], st[1], st[2], 0, 0, 0, 0, (double)c->lu.captures, 0, 0,
+                          (double)c->lu.replays};
     memcpy(c->last, last, sizeof last);
     if (info) *info = inf;
     if (s) return s;
@@ -563,11 +675,14 @@ int bemx_solve(bemx_ctx* c, const double* zge, const double* zcs, const double* 
         HIPCHK(c, hipMemcpyAsync(zfi_out, c->zfi.p, sizeof(cd) * nn * c->nrhs,
                                  hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    float ms = 0.f, ms_od = 0.f, ms_as = 0.f;
+    float ms = 0.f, ms_od = 0.f, ms_as = 0.f, ms_f = 0.f;
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
     hipEventElapsedTime(&ms_od, c->ev2, c->ev3);
     hipEventElapsedTime(&ms_as, c->ev0, c->ev4);
-    double last[8] = {st[0], st[1], st[2], ms, ms_od, ms_as, ms - ms_as, 0.0};
+    if (c->lu.factor_timed) hipEventElapsedTime(&ms_f, c->lu.ev_f0, c->lu.ev_f1);
+    double last[NLAST] = {st[0], st[1], st[2], ms, ms_od, ms_as, ms - ms_as,
+                          (double)c->lu.captures, ms_f, ms - ms_as - ms_f,
+                          (double)c->lu.replays};
     memcpy(c->last, last, sizeof last);
     if (stats) {
         stats[0] = st[0];
@@ -578,9 +693,9 @@ int bemx_solve(bemx_ctx* c, const double* zge, const double* zcs, const double* 
     return BEMX_OK;
 }
 
-int bemx_last_stats(const bemx_ctx* c, double* out) {
-    if (!c || !out) return BEMX_ERR_ARG;
-    memcpy(out, c->last, sizeof c->last);
+int bemx_last_stats(const bemx_ctx* c, double* out, int nout) {
+    if (!c || !out || nout < 1) return BEMX_ERR_ARG;
+    for (int i = 0; i < nout && i < NLAST; ++i) out[i] = c->last[i];
     return BEMX_OK;
 }
 

@@ -26,6 +26,20 @@ __host__ __device__ inline cd cinv(cd a) {
 }
 __host__ __device__ inline cd operator/(cd a, cd b) { return a * cinv(b); }
 
+// Small-|z| form of the dynamic kernel scalars, z = ks R.  Each of psi R,
+// chi R, psi' R^2, chi' R^2 (Solfund.for:42-49) is P(z) e^z - r^2 Q(r z)
+// e^(r z) with r = cs/cp (real: the damping is the same for both waves) and
+// P, Q polynomials in 1/z of degree 2, whose 1/z^2 and 1/z terms cancel
+// between the two waves; evaluated as written, FP64 rounding is amplified
+// by ~1/|z|^2.  Their power series sum_m f_m z^m (m >= 0) has real
+// coefficients f_m = sum_j (p_j - q_j r^(m+2)) / (m - j)! (p_j, q_j the
+// coefficients of z^j in P and Q), and so do A R^2, B R^2, C R^2.  Below
+// |z| = SER_TAU the kernels use the series: NSER terms truncate below 1e-19
+// relative, and above SER_TAU the closed form loses at most ~6e-14.
+constexpr int NSER = 18;
+constexpr double SER_TAU = 0.5;
+enum { SER_PSI = 0, SER_CHI, SER_A, SER_B, SER_C, SER_N };
+
 // Per-frequency constants of the dynamic Kelvin/Stokes kernel
 // (Solfund.for:29-40): k = -i w / c for the S and P waves, their inverses
 // (1/z = (1/k)(1/R) replaces a complex division per point), c2 = (cs/cp)^2,
@@ -41,6 +55,14 @@ struct DynConst {
     cd lm;         // lambda*/mu*  (stress kernels)
     double q4pi;   // 1/(4 pi)
     double c1s, c2s, c3s, c4s;  // static constants C1..C4 (Ghmatece.for:60-63)
+    double ks_abs2;             // |ks|^2: the series is used where |ks|^2 R^2 < SER_TAU^2
+    double ser[SER_N][NSER];    // series coefficients of psi R, chi R, A R^2, B R^2, C R^2
+    // Diagonal blocks (dynamic minus static kernels, Solfundif.for:56-69)
+    // with the series: the m = 0 terms are the static limits, which cancel
+    // the static kernels up to these residuals (zero in exact arithmetic for
+    // T; O(damping) for U, whose static kernel has the real G):
+    cd u_d0, u_r0;              // c0 f_psi,0 - C1 C2, c0 f_chi,0 + C1
+    double t_a0, t_b0, t_c0;    // f_A,0 / (4 pi) - C3 C4, f_B,0 / (4 pi) - 3 C3, f_C,0 / (4 pi) + C3 C4
 };
 
 // Gauss point of an element: position and weight OME(ig) OME(jg) DET.
@@ -55,7 +77,7 @@ struct ElemData {
 
 // Status bits written by kernels (OR-ed into a device word).
 enum : int {
-    ST_DEGENERATE = 1,  // DET < 1e-5 at some Gauss point (Nonsingd.for:152)
+    ST_DEGENERATE = 1,  // DET < 1e-5 at some Gauss point (Nonsingd.for:89-95)
     ST_PIVOT_ZERO = 2,  // exactly singular pivot in the LU
     ST_SYNC_TIMEOUT = 4 // a bounded inter-workgroup wait gave up
 };

@@ -13,21 +13,43 @@ struct LuWork {
     static constexpr int SLOTS = 32;  // slots 0..8: FP64 path, 9..: mixed path
     void* buf[SLOTS] = {nullptr};
     size_t cap[SLOTS] = {0};
+    // Bumped whenever a buffer is (re)allocated: a captured graph holds the
+    // addresses of every buffer it touches, so one captured before a
+    // reallocation must never be replayed after it.
+    unsigned long long gen = 0;
     // Flags of the cooperative triangular solves hold the epoch of the
-    // launch that set them; epochs only grow, so the array is cleared once,
-    // when it is (re)allocated, and never inside a solve.
+    // launch that set them; epochs only grow, so the array is cleared once
+    // per workspace generation, and never inside a solve.
     int epoch = 0;
-    void* flags_init = nullptr;
+    unsigned long long cleared_gen = ~0ull;
     // second stream and events of the look-ahead factorisation (lazy)
     hipStream_t side = nullptr;
     hipEvent_t ev_panel = nullptr, ev_bulk = nullptr;
+    // events around the last factorisation on the solve's stream (its device
+    // time; factor_timed says whether they bracket one)
+    hipEvent_t ev_f0 = nullptr, ev_f1 = nullptr;
+    bool factor_timed = false;
     hipError_t ensure_side();
-    // graph of the mixed factorisation and the key it was captured for
+    // Captured factorisation graphs, one per size class: keyed by the order,
+    // the matrix address, its leading dimension and the workspace generation;
+    // least recently used entries are evicted beyond GRAPHS.
+    static constexpr int GRAPHS = 8;
+    struct GraphEntry {
+        const void* key[4] = {nullptr, nullptr, nullptr, nullptr};
+        hipGraphExec_t exec = nullptr;
+        unsigned long long used = 0;
+    };
+    GraphEntry graphs[GRAPHS];
+    unsigned long long tick = 0;
+    long long captures = 0, replays = 0;  // graph statistics since creation
     bool use_graphs = true;
-    hipGraphExec_t gexec = nullptr;
-    const void* gkey[8] = {nullptr};
+    hipGraphExec_t find_graph(const void* const key[4]);
+    // takes ownership of exec; evicts the least recently used entry if full
+    void add_graph(const void* const key[4], hipGraphExec_t exec);
+    void drop_graphs();
     LuWork();
     void* get(int slot, size_t bytes, hipError_t* e);
+    size_t bytes() const;  // device bytes held by the workspace
     void release();
 };
 
@@ -46,8 +68,19 @@ int gmres_ir(LuWork& w, hipStream_t st, int n, const cd* A, size_t lda, cd* b, i
 int lu_solve(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, cd* b, int nrhs, int mode,
              long long* info, double stats[4], std::string& err);
 
-// Device bytes of the mixed factorisation's workspace for order n beyond the
-// FP32 copy of the matrix (bf16 planes, diagonal-block inverses).
-size_t mixed_factor_extra_bytes(int n);
+// Device bytes of every workspace buffer of the mixed path (factorisation
+// and refinement) for order n.
+size_t mixed_workspace_bytes(int n);
+
+// Device bytes of the FP64 factorisation's workspace for order n (pivots,
+// panel candidates, permutations, solve vector).
+size_t fp64_factor_bytes(int n);
+
+// Query one kernel of each kernel file so that its device code is loaded by
+// the calling thread (a context's creator) rather than by whichever thread
+// makes the first launch.
+hipError_t preload_lu_code();
+hipError_t preload_mixed_code();
+hipError_t preload_assembly_code();
 
 }  // namespace bemx

@@ -3,7 +3,7 @@
 // Continuous batching: requests (a mesh, a material, a frequency and 1..4
 // load cases) are queued as they arrive.  NSLOTS worker threads each own a
 // device context (its own streams and grow-only workspace); a free worker
-// takes the oldest queued request as soon as its HBM footprint fits in what
+// takes the oldest queued request as soon as the HBM it needs fits in what
 // the budget has left, runs assembly + factorisation + refinement for it
 // with bemx_solve and hands the solution back.  Requests that arrive while
 // others are on the GPU are admitted as slots and memory free up, and the
@@ -11,10 +11,19 @@
 // the latency-bound chain of its factorisation, another's bulk update fills
 // the CUs.  A slot keeps the mesh and static pass (GHMATECE) of the last
 // mesh it served, so a stream of requests on one mesh (a frequency sweep)
-// pays the static pass once per slot.  Each request's arithmetic is the
+// pays the static pass once per slot, and it keeps one captured
+// factorisation graph per size class.  Each request's arithmetic is the
 // same whichever slot runs it and whatever runs beside it.
+//
+// Admission charges each slot what its workspace can hold: the bound of
+// ctx_solve_bytes_bound at the largest sizes the slot has served (its
+// buffers only grow), not the footprint of the request it runs.  The sum of
+// the charges never exceeds the budget.  When the oldest request does not
+// fit, idle slots give their workspace back (the charge drops to zero) until
+// it does.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -31,6 +40,13 @@
 
 #include "../../include/bemx.h"
 #include "bemx_lu.hpp"
+
+namespace bemx {
+size_t ctx_solve_bytes_bound(int n, int nbe, int np, int npg, int nrhs, int mode);
+size_t ctx_device_bytes(bemx_ctx* c);
+void ctx_release_workspace(bemx_ctx* c);
+void ctx_graph_counts(const bemx_ctx* c, long long* captures, long long* replays);
+}  // namespace bemx
 
 namespace {
 
@@ -69,13 +85,40 @@ struct Request {
     int slot = -1;
 };
 
+// Largest sizes a slot has served: its buffers hold (at most) the bound at
+// these sizes.
+struct SlotSizes {
+    int n = 0, nbe = 0, np = 0, npg = 0, nrhs = 0;
+    SlotSizes grown(const Request& r) const {
+        SlotSizes g;
+        g.n = std::max(n, r.n);
+        g.nbe = std::max(nbe, r.nbe);
+        g.np = std::max(np, r.np);
+        g.npg = std::max(npg, r.npg);
+        g.nrhs = std::max(nrhs, r.nrhs);
+        return g;
+    }
+};
+
+size_t bound_of(const SlotSizes& z, int mode) {
+    if (!z.nbe) return 0;
+    return bemx::ctx_solve_bytes_bound(z.n, z.nbe, z.np, z.npg, z.nrhs, mode);
+}
+
+struct Slot {
+    bemx_ctx* ctx = nullptr;
+    SlotSizes sizes;
+    size_t charge = 0;  // bytes charged against the budget
+    bool busy = false;
+};
+
 }  // namespace
 
 struct bemx_server {
     int device = 0;
     clk::time_point t0 = clk::now();
     int mode = BEMX_MODE_MIXED;
-    size_t budget = 0, in_use = 0;
+    size_t budget = 0, charged = 0, peak_charged = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> queue;
@@ -83,29 +126,18 @@ struct bemx_server {
     long long next_id = 1;
     bool stop = false;
     std::vector<std::thread> workers;
-    std::vector<bemx_ctx*> ctxs;
+    std::vector<Slot> slots;
+    // statistics
+    long long completed = 0, releases = 0, over_charge = 0;
+    long long captures = 0, replays = 0;
+    std::vector<long long> slot_captures, slot_replays;
     std::string err;
 };
 
 namespace {
 
-// Device bytes one request keeps live while it runs: ZH (complex128), the
-// FP32 copy, bf16 panels and the refinement vectors of the mixed path, and
-// the per-chunk partial right-hand sides of the assembly.
-size_t request_footprint(int nbe, int nrhs, int mode) {
-    const size_t nn = 3 * (size_t)nbe;
-    const size_t ld32 = (nn + 31) & ~(size_t)31;
-    size_t b = nn * nn * 16;                                    // ZH
-    b += (size_t)((nbe + 3) / 4) * nn * nrhs * 16;               // assembly partial RHS
-    b += nn * nrhs * 16 * 4;                                     // ZDFI, ZFI, diagonal RHS
-    if (mode == BEMX_MODE_MIXED) {
-        b += ld32 * nn * 8;                                      // FP32 factors
-        b += bemx::mixed_factor_extra_bytes((int)nn);                  // bf16 planes, inverses
-        b += nn * 16 * (5 + 13) + ((nn + 127) / 128) * nn * 16;  // GMRES vectors, GEMV partials
-    } else {
-        b += nn * 16 * 2 + nn * 64 * 16;                         // solve vector, panel candidates
-    }
-    return b;
+size_t request_footprint(const Request& r, int mode) {
+    return bemx::ctx_solve_bytes_bound(r.n, r.nbe, r.np, r.npg, r.nrhs, mode);
 }
 
 void run_request(bemx_server* s, bemx_ctx* c, uint64_t* slot_key, Request* r) {
@@ -132,35 +164,69 @@ void run_request(bemx_server* s, bemx_ctx* c, uint64_t* slot_key, Request* r) {
 }
 
 void worker_main(bemx_server* s, int slot) {
-    bemx_ctx* c = s->ctxs[slot];
+    bemx_ctx* c = s->slots[slot].ctx;
     uint64_t key = 0;
+    std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
+        Slot& me = s->slots[slot];
+        // FIFO admission: the oldest request runs as soon as a slot is free
+        // and the slot's charge, grown to that request, fits in the budget
         Request* r = nullptr;
-        {
-            std::unique_lock<std::mutex> lk(s->mu);
-            // FIFO admission: the oldest request runs as soon as a slot is
-            // free and its footprint fits in the budget left
-            s->cv_work.wait(lk, [&] {
-                return s->stop ||
-                       (!s->queue.empty() && s->in_use + s->queue.front()->footprint <= s->budget);
-            });
-            if (s->stop) return;
-            r = s->queue.front();
-            s->queue.pop_front();
-            s->in_use += r->footprint;
-            r->slot = slot;
-            r->t_start = clk::now();
+        bool give_back = false;
+        s->cv_work.wait(lk, [&] {
+            if (s->stop) return true;
+            if (s->queue.empty()) return false;
+            const size_t need = bound_of(me.sizes.grown(*s->queue.front()), s->mode);
+            if (s->charged - me.charge + need <= s->budget) return true;
+            give_back = me.charge > 0;  // the request waits for memory: return ours
+            return give_back;
+        });
+        if (s->stop) break;
+        if (give_back) {
+            lk.unlock();
+            bemx::ctx_release_workspace(c);
+            key = 0;
+            lk.lock();
+            s->charged -= me.charge;
+            me.charge = 0;
+            me.sizes = SlotSizes();
+            ++s->releases;
+            s->cv_work.notify_all();
+            continue;
         }
+        r = s->queue.front();
+        s->queue.pop_front();
+        me.sizes = me.sizes.grown(*r);
+        const size_t need = bound_of(me.sizes, s->mode);
+        s->charged = s->charged - me.charge + need;
+        me.charge = need;
+        s->peak_charged = std::max(s->peak_charged, s->charged);
+        me.busy = true;
+        r->slot = slot;
+        r->t_start = clk::now();
+        lk.unlock();
         run_request(s, c, &key, r);
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            s->in_use -= r->footprint;
-            r->t_done = clk::now();
-            r->done = true;
-        }
+        const size_t held = bemx::ctx_device_bytes(c);
+        long long cap = 0, rep = 0;
+        bemx::ctx_graph_counts(c, &cap, &rep);
+        lk.lock();
+        if (held > me.charge) ++s->over_charge;  // the bound was too small
+        s->captures += cap - s->slot_captures[slot];
+        s->replays += rep - s->slot_replays[slot];
+        s->slot_captures[slot] = cap;
+        s->slot_replays[slot] = rep;
+        me.busy = false;
+        r->t_done = clk::now();
+        r->done = true;
+        ++s->completed;
         s->cv_done.notify_all();
         s->cv_work.notify_all();
     }
+    lk.unlock();
+    // The objects this thread created for the context's solves (its second
+    // stream, events and captured graphs) and the workspace are released by
+    // this thread, after its last work has finished.
+    bemx::ctx_release_workspace(c);
 }
 
 }  // namespace
@@ -183,9 +249,10 @@ int bemx_material(double ge, double rnu, double dam, double rho, double* zge, do
     return BEMX_OK;
 }
 
-double bemx_request_footprint(int nbe, int nrhs, int mode) {
-    if (nbe <= 0 || nrhs < 1) return 0.0;
-    return (double)request_footprint(nbe, nrhs, mode);
+double bemx_request_footprint(int nbe, int n, int np, int npg, int nrhs, int mode) {
+    if (nbe <= 0 || n < nbe || np <= 0 || npg <= 0 || nrhs < 1) return 0.0;
+    if (mode != BEMX_MODE_FP64 && mode != BEMX_MODE_MIXED) return 0.0;
+    return (double)bemx::ctx_solve_bytes_bound(n, nbe, np, npg, nrhs, mode);
 }
 
 int bemx_server_create(int device, int nslots, double hbm_budget_bytes, int mode,
@@ -200,16 +267,22 @@ int bemx_server_create(int device, int nslots, double hbm_budget_bytes, int mode
     s->device = device;
     s->mode = mode;
     s->budget = hbm_budget_bytes > 0 ? (size_t)hbm_budget_bytes : (size_t)(0.8 * (double)freeb);
+    // the contexts (and with them the device code of the library) are made
+    // on this thread, before any worker starts
     for (int i = 0; i < nslots; ++i) {
         bemx_ctx* c = nullptr;
         int st = bemx_create(device, &c);
         if (st) {
-            for (bemx_ctx* d : s->ctxs) bemx_destroy(d);
+            for (Slot& sl : s->slots) bemx_destroy(sl.ctx);
             delete s;
             return st;
         }
-        s->ctxs.push_back(c);
+        Slot sl;
+        sl.ctx = c;
+        s->slots.push_back(sl);
     }
+    s->slot_captures.assign(nslots, 0);
+    s->slot_replays.assign(nslots, 0);
     for (int i = 0; i < nslots; ++i) s->workers.emplace_back(worker_main, s, i);
     *out = s;
     return BEMX_OK;
@@ -250,7 +323,7 @@ int bemx_server_submit(bemx_server* s, const double* cx, const double* cy, const
     h = fnv1a(h, cz, sizeof(double) * np);
     h = fnv1a(h, cone, sizeof(int) * 4 * (size_t)n);
     r->mesh_key = h ? h : 1;
-    r->footprint = request_footprint(nbe, nrhs, s->mode);
+    r->footprint = request_footprint(*r, s->mode);
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return BEMX_ERR_STATE;
     if (r->footprint > s->budget) {
@@ -300,6 +373,24 @@ int bemx_server_wait(bemx_server* s, long long id, double* x, double* info, char
     return r->status;
 }
 
+int bemx_server_stats(bemx_server* s, double* out, int nout) {
+    if (!s || !out || nout < 1) return BEMX_ERR_ARG;
+    double v[8];
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        v[0] = (double)s->completed;
+        v[1] = (double)s->captures;
+        v[2] = (double)s->replays;
+        v[3] = (double)s->charged;
+        v[4] = (double)s->peak_charged;
+        v[5] = (double)s->budget;
+        v[6] = (double)s->releases;
+        v[7] = (double)s->over_charge;
+    }
+    for (int i = 0; i < nout && i < 8; ++i) out[i] = v[i];
+    return BEMX_OK;
+}
+
 int bemx_server_destroy(bemx_server* s) {
     if (!s) return BEMX_ERR_ARG;
     {
@@ -308,8 +399,10 @@ int bemx_server_destroy(bemx_server* s) {
     }
     s->cv_work.notify_all();
     s->cv_done.notify_all();
+    // each worker releases its context's workspace, streams and graphs
+    // before it returns
     for (std::thread& t : s->workers) t.join();
-    for (bemx_ctx* c : s->ctxs) bemx_destroy(c);
+    for (Slot& sl : s->slots) bemx_destroy(sl.ctx);
     delete s;
     return BEMX_OK;
 }

@@ -32,6 +32,9 @@ namespace bemx {
 void* LuWork::get(int slot, size_t bytes, hipError_t* e) {
     *e = hipSuccess;
     if (bytes <= cap[slot] && buf[slot]) return buf[slot];
+    // a captured graph may hold the old address of this buffer
+    drop_graphs();
+    ++gen;
     if (buf[slot]) hipFree(buf[slot]);
     buf[slot] = nullptr;
     cap[slot] = 0;
@@ -40,20 +43,61 @@ void* LuWork::get(int slot, size_t bytes, hipError_t* e) {
     return buf[slot];
 }
 
+size_t LuWork::bytes() const {
+    size_t b = 0;
+    for (int i = 0; i < SLOTS; ++i) b += buf[i] ? (cap[i] ? cap[i] : 256) : 0;
+    return b;
+}
+
+hipGraphExec_t LuWork::find_graph(const void* const key[4]) {
+    for (GraphEntry& g : graphs)
+        if (g.exec && g.key[0] == key[0] && g.key[1] == key[1] && g.key[2] == key[2] &&
+            g.key[3] == key[3]) {
+            g.used = ++tick;
+            return g.exec;
+        }
+    return nullptr;
+}
+
+void LuWork::add_graph(const void* const key[4], hipGraphExec_t exec) {
+    GraphEntry* slot = &graphs[0];
+    for (GraphEntry& g : graphs) {
+        if (!g.exec) {
+            slot = &g;
+            break;
+        }
+        if (g.used < slot->used) slot = &g;
+    }
+    if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
+    for (int i = 0; i < 4; ++i) slot->key[i] = key[i];
+    slot->exec = exec;
+    slot->used = ++tick;
+}
+
+void LuWork::drop_graphs() {
+    for (GraphEntry& g : graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g = GraphEntry();
+    }
+}
+
 void LuWork::release() {
+    if (side) (void)hipStreamSynchronize(side);
+    drop_graphs();
     for (int i = 0; i < SLOTS; ++i) {
         if (buf[i]) hipFree(buf[i]);
         buf[i] = nullptr;
         cap[i] = 0;
     }
-    if (gexec) hipGraphExecDestroy(gexec);
-    gexec = nullptr;
+    ++gen;
     if (side) hipStreamDestroy(side);
     if (ev_panel) hipEventDestroy(ev_panel);
     if (ev_bulk) hipEventDestroy(ev_bulk);
+    if (ev_f0) hipEventDestroy(ev_f0);
+    if (ev_f1) hipEventDestroy(ev_f1);
     side = nullptr;
-    ev_panel = ev_bulk = nullptr;
-    flags_init = nullptr;
+    ev_panel = ev_bulk = ev_f0 = ev_f1 = nullptr;
+    factor_timed = false;
 }
 
 LuWork::LuWork() {
@@ -68,6 +112,8 @@ hipError_t LuWork::ensure_side() {
     if (!side) e = hipStreamCreateWithFlags(&side, hipStreamNonBlocking);
     if (e == hipSuccess && !ev_panel) e = hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming);
     if (e == hipSuccess && !ev_bulk) e = hipEventCreateWithFlags(&ev_bulk, hipEventDisableTiming);
+    if (e == hipSuccess && !ev_f0) e = hipEventCreate(&ev_f0);
+    if (e == hipSuccess && !ev_f1) e = hipEventCreate(&ev_f1);
     return e;
 }
 
@@ -548,6 +594,18 @@ __global__ void k_copy_cd(const cd* __restrict__ s, cd* __restrict__ d, int n) {
 
 enum { W_IPIV, W_CNT, W_SLOTS, W_CAND, W_DROW, W_PERM, W_INFO, W_STATUS, W_BTMP };
 
+size_t fp64_factor_bytes(int n) {
+    const size_t npanel = (size_t)(n + NB - 1) / NB, pmax = (size_t)(n + PANEL_ROWS - 1) / PANEL_ROWS;
+    return sizeof(int) * n + sizeof(int) * npanel * NB + sizeof(Cand) * NB * pmax +
+           sizeof(cd) * NB * pmax * NB + sizeof(cd) * NB * NB + sizeof(int) * npanel * (1 + 4 * NB) +
+           sizeof(long long) + sizeof(int) + sizeof(cd) * n + 9 * 256;
+}
+
+hipError_t preload_lu_code() {
+    hipFuncAttributes a;
+    return hipFuncGetAttributes(&a, (const void*)k_panel_fp64);
+}
+
 static int factor_fp64(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, std::string& err) {
     hipError_t e;
     const int npanel = (n + NB - 1) / NB;
@@ -576,6 +634,8 @@ static int factor_fp64(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, std:
     LU_CHK(hipGetDevice(&dev));
     LU_CHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     LU_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_panel_fp64, 256, 0));
+    LU_CHK(w.ensure_side());
+    LU_CHK(hipEventRecord(w.ev_f0, st));
     for (int pk = 0; pk < npanel; ++pk) {
         const int k0 = pk * NB;
         const int nb = std::min(NB, n - k0);
@@ -604,6 +664,8 @@ static int factor_fp64(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, std:
                            A + (size_t)c0 * lda + k0, lda, A + (size_t)c0 * lda + c0, lda);
         LU_CHK(hipGetLastError());
     }
+    LU_CHK(hipEventRecord(w.ev_f1, st));
+    w.factor_timed = true;
     return BEMX_OK;
 }
 
@@ -640,6 +702,7 @@ int lu_solve(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, cd* b, int nrh
         return BEMX_ERR_ARG;
     }
     if (stats) stats[0] = stats[1] = stats[2] = stats[3] = 0.0;
+    w.factor_timed = false;
     int first = 0;  // first column the FP64 path still has to solve
     if (mode == BEMX_MODE_MIXED) {
         int sm = gmres_ir(w, st, n, A, lda, b, nrhs, stats, err, &first);

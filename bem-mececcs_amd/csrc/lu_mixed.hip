@@ -1470,11 +1470,42 @@ static int group_blocks() {
     return g;
 }
 
-size_t mixed_factor_extra_bytes(int n) {
-    const size_t nb = (size_t)(n + NBM - 1) / NBM, np = (nb + 1) / 2;
-    const size_t planes = sizeof(__bf16) * (size_t)n * NBM * 8 * 2 * group_blocks();
-    const size_t inv = sizeof(cf) * nb * NBM * NBM + sizeof(cf) * np * NB2 * NB2 * 4 + 2 * REC_BYTES;
-    return planes + inv;
+constexpr int GM_RESTART = 12;  // GMRES restart length of the refinement
+
+// Bytes of each workspace buffer of the mixed path (factorisation and
+// refinement) for order n, by LuWork slot (0 for slots it does not use).
+static void mixed_sizes(int n, size_t* sz) {
+    const size_t ld = ((size_t)n + 31) & ~(size_t)31, nblk = (size_t)(n + NBM - 1) / NBM;
+    const size_t npair = (nblk + 1) / 2, nchunk = (size_t)(n + MV_COLS - 1) / MV_COLS;
+    const size_t nb256 = (size_t)(n + 255) / 256, MR = GM_RESTART;
+    for (int i = 0; i < LuWork::SLOTS; ++i) sz[i] = 0;
+    sz[M_A32] = sizeof(cf) * ld * n;                    // FP32 copy of the matrix
+    sz[M_INV] = sizeof(cf) * nblk * NBM * NBM;          // diagonal-block inverses
+    sz[M_REC] = 2 * REC_BYTES;                          // two elimination records
+    sz[M_GJF] = sizeof(int) * nblk * GJ_PANELS;         // record panel flags
+    sz[M_MINV] = sizeof(cf) * npair * NB2 * NB2 * 2;    // 128-row inverses of L and U
+    sz[M_WC] = sizeof(cf) * npair * NB2 * NB2 * 2;      // coupling blocks
+    sz[M_FLAGS] = sizeof(int) * 2 * npair;              // block-solve flags
+    sz[M_LL] = sizeof(u64) * 4 * 2 * npair * NB2;       // tagged hand-off words
+    sz[M_STATUS] = sizeof(int);
+    sz[M_PLANES] = sizeof(__bf16) * (size_t)n * NBM * 8 * 2 * group_blocks();
+    sz[M_VEC] = sizeof(cd) * n * 5;                     // x, r, w, tmp, b
+    sz[M_V] = sizeof(cd) * n * (MR + 1);                // GMRES basis
+    sz[M_PART] = sizeof(cd) * std::max(nchunk * n, nb256 * (MR + 2));
+    sz[M_HOST] = sizeof(cd) * (MR + 2) * 3;             // Arnoldi coefficients
+}
+
+size_t mixed_workspace_bytes(int n) {
+    size_t sz[LuWork::SLOTS], b = 0;
+    mixed_sizes(n, sz);
+    for (int i = 0; i < LuWork::SLOTS; ++i)
+        if (sz[i]) b += std::max<size_t>(sz[i], 256);
+    return b;
+}
+
+hipError_t preload_mixed_code() {
+    hipFuncAttributes a;
+    return hipFuncGetAttributes(&a, (const void*)k_trsv_coop);
 }
 
 // The launch sequence of the factorisation (no host synchronisation, no
@@ -1597,44 +1628,39 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     F.n = n;
     F.nblk = nblk;
     F.ld = ((size_t)n + 31) & ~(size_t)31;  // 16-byte aligned 4-row groups
-    F.A = (cf*)w.get(M_A32, sizeof(cf) * F.ld * n, &e);
-    MX_CHK(e);
-    F.inv = (cf*)w.get(M_INV, sizeof(cf) * (size_t)nblk * NBM * NBM, &e);
-    MX_CHK(e);
-    F.rec = (char*)w.get(M_REC, 2 * REC_BYTES, &e);
-    MX_CHK(e);
-    F.gjflag = (int*)w.get(M_GJF, sizeof(int) * (size_t)nblk * GJ_PANELS, &e);
-    MX_CHK(e);
+    // Every buffer of the mixed path, the refinement's too, before anything
+    // is captured: a captured graph holds the addresses of the buffers it
+    // touches, and any later reallocation would drop it.
+    size_t sz[LuWork::SLOTS];
+    mixed_sizes(n, sz);
+    void* buf[LuWork::SLOTS] = {nullptr};
+    for (int i = 0; i < LuWork::SLOTS; ++i)
+        if (sz[i]) {
+            buf[i] = w.get(i, sz[i], &e);
+            MX_CHK(e);
+        }
+    F.A = (cf*)buf[M_A32];
+    F.inv = (cf*)buf[M_INV];
+    F.rec = (char*)buf[M_REC];
+    F.gjflag = (int*)buf[M_GJF];
     F.npair = (nblk + 1) / 2;
-    F.minv = (cf*)w.get(M_MINV, sizeof(cf) * (size_t)F.npair * NB2 * NB2 * 2, &e);
-    MX_CHK(e);
-    F.wc = (cf*)w.get(M_WC, sizeof(cf) * (size_t)F.npair * NB2 * NB2 * 2, &e);
-    MX_CHK(e);
-    const size_t cap_before = w.cap[M_FLAGS];
-    // block-solve flags: one per 64-row half of a 128-row block
-    const int nflags = 2 * F.npair;
-    F.flags = (int*)w.get(M_FLAGS, sizeof(int) * (size_t)nflags, &e);
-    MX_CHK(e);
-    if (w.flags_init != (void*)F.flags || w.cap[M_FLAGS] != cap_before) {
-        // fresh flag array: clear it once
+    F.minv = (cf*)buf[M_MINV];
+    F.wc = (cf*)buf[M_WC];
+    F.flags = (int*)buf[M_FLAGS];  // block-solve flags: one per 64-row half of a 128-row block
+    F.yll = (u64*)buf[M_LL];
+    F.epoch = &w.epoch;
+    if (w.cleared_gen != w.gen) {
+        // Flags and hand-off tags hold launch epochs, which only grow, so
+        // they are cleared once, when the workspace is (re)allocated, and
+        // never inside a solve.
+        const int nflags = 2 * F.npair, nll = F.npair * NB2;
         hipLaunchKernelGGL(k_fill_i, dim3((nflags + 255) / 256), dim3(256), 0, st, F.flags, 0,
                            nflags);
-        w.flags_init = F.flags;
+        hipLaunchKernelGGL(k_fill_i, dim3((16 * nll + 255) / 256), dim3(256), 0, st, (int*)F.yll,
+                           0, 16 * nll);
+        w.cleared_gen = w.gen;
     }
-    F.epoch = &w.epoch;
-    {
-        // tags are launch epochs (increasing from 1): only fresh memory needs
-        // clearing, old tags never match again
-        const size_t cap_ll = w.cap[M_LL];
-        const int nll = F.npair * NB2;  // solution entries, then as many right-hand-side entries
-        F.yll = (u64*)w.get(M_LL, sizeof(u64) * 4 * 2 * (size_t)nll, &e);
-        MX_CHK(e);
-        if (w.cap[M_LL] != cap_ll)
-            hipLaunchKernelGGL(k_fill_i, dim3((16 * nll + 255) / 256), dim3(256), 0, st, (int*)F.yll,
-                               0, 16 * nll);
-    }
-    F.status = (int*)w.get(M_STATUS, sizeof(int), &e);
-    MX_CHK(e);
+    F.status = (int*)buf[M_STATUS];
     int dev = 0, ncu = 0, per = 0;
     MX_CHK(hipGetDevice(&dev));
     MX_CHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1648,17 +1674,17 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     // double-buffered by group parity (group g+1 writes one set while the bulk
     // update of group g still reads the other)
     const size_t pstride = (size_t)n * NBM;
-    __bf16* planes = (__bf16*)w.get(
-        M_PLANES, sizeof(__bf16) * pstride * 8 * 2 * (size_t)group_blocks(), &e);
-    MX_CHK(e);
-    // The launch sequence depends only on n and the buffer addresses: it is
-    // captured once into a graph and replayed while those stay the same.
-    const void* key[8] = {(const void*)(size_t)n, A64, (const void*)lda, F.A, planes, F.inv,
-                          F.status, w.side};
+    __bf16* planes = (__bf16*)buf[M_PLANES];
+    // The launch sequence depends only on n, the matrix and the workspace
+    // buffers: it is captured once per size class into a graph and replayed
+    // while the workspace keeps its buffers (every reallocation bumps w.gen
+    // and drops the cached graphs).
+    MX_CHK(hipEventRecord(w.ev_f0, st));
     if (w.use_graphs) {
-        if (!w.gexec || memcmp(key, w.gkey, sizeof key) != 0) {
-            if (w.gexec) MX_CHK(hipGraphExecDestroy(w.gexec));
-            w.gexec = nullptr;
+        const void* key[4] = {(const void*)(size_t)n, A64, (const void*)lda,
+                              (const void*)(size_t)w.gen};
+        hipGraphExec_t ex = w.find_graph(key);
+        if (!ex) {
             hipGraph_t g = nullptr;
             MX_CHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             const hipError_t e1 = enqueue_factor(w, st, n, A64, lda, F, planes, pstride);
@@ -1667,15 +1693,20 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
                 if (g) (void)hipGraphDestroy(g);
                 MX_CHK(e1 != hipSuccess ? e1 : e2);
             }
-            const hipError_t e3 = hipGraphInstantiate(&w.gexec, g, nullptr, nullptr, 0);
+            const hipError_t e3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             MX_CHK(e3);
-            memcpy(w.gkey, key, sizeof key);
+            w.add_graph(key, ex);
+            ++w.captures;
+        } else {
+            ++w.replays;
         }
-        MX_CHK(hipGraphLaunch(w.gexec, st));
+        MX_CHK(hipGraphLaunch(ex, st));
     } else {
         MX_CHK(enqueue_factor(w, st, n, A64, lda, F, planes, pstride));
     }
+    MX_CHK(hipEventRecord(w.ev_f1, st));
+    w.factor_timed = true;
     MX_CHK(hipGetLastError());
     int hs = 0;
     MX_CHK(hipMemcpyAsync(&hs, F.status, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1725,24 +1756,25 @@ typedef std::complex<double> zx;
 static int refine_one(LuWork& w, hipStream_t st, const MixedFactor& F, int n, const cd* A64,
                       size_t lda, cd* b, int* iters, double* relres_out, std::string& err) {
     hipError_t e;
-    const int MR = 12;        // GMRES restart length
+    const int MR = GM_RESTART;
     const int MAXOUT = 12;    // refinement steps
     const double TOL = 2e-15; // normwise backward error target
     const int nb256 = (n + 255) / 256;
-    cd* vec = (cd*)w.get(M_VEC, sizeof(cd) * (size_t)n * 5, &e);
+    size_t sz[LuWork::SLOTS];  // reserved by factor_mixed: no reallocation here
+    mixed_sizes(n, sz);
+    cd* vec = (cd*)w.get(M_VEC, sz[M_VEC], &e);
     MX_CHK(e);
     cd *x = vec, *r = vec + n, *wv = vec + 2 * (size_t)n, *tmp = vec + 3 * (size_t)n,
        *bsave = vec + 4 * (size_t)n;
-    cd* V = (cd*)w.get(M_V, sizeof(cd) * (size_t)n * (MR + 1), &e);
+    cd* V = (cd*)w.get(M_V, sz[M_V], &e);
     MX_CHK(e);
     const int nchunk = (n + MV_COLS - 1) / MV_COLS;
-    cd* part = (cd*)w.get(M_PART, sizeof(cd) * std::max((size_t)nchunk * n,
-                                                         (size_t)nb256 * (MR + 2)), &e);
+    cd* part = (cd*)w.get(M_PART, sz[M_PART], &e);
     MX_CHK(e);
     // device scratch of the Arnoldi step: projections of the two
     // Gram-Schmidt passes and the norm (slots of MR + 2 entries each)
     const int HS = MR + 2;
-    cd* hdev = (cd*)w.get(M_HOST, sizeof(cd) * HS * 3, &e);
+    cd* hdev = (cd*)w.get(M_HOST, sz[M_HOST], &e);
     MX_CHK(e);
     cd *h1 = hdev + HS, *h2 = hdev + 2 * HS;
     std::vector<cd> hstep((size_t)HS * 3);

@@ -5,7 +5,7 @@
 ! kernels/shared.cu:6-31; here one context handle per process plays that
 ! role.  Errors come back as status codes; bemx_check prints the library's
 ! message and stops, as the reference's Fortran stops on its own errors
-! (e.g. Nonsingd.for:152-158).
+! (e.g. Nonsingd.for:89-95).
 module bemx_iface
   use iso_c_binding
   implicit none

@@ -1,6 +1,6 @@
 ! bemx_main.f90 -- driver with the reference's command line
 !   bemx_main <E-deck> <D-deck> <E-out> <D-out>       (Main.for:85-101)
-! It reads the decks (formats of Inputece.for:34-105 and Inputecd.for:136-242),
+! It reads the decks (formats of Inputece.for:34-105 and Inputecd.for:26-134),
 ! runs the static pass once and, per frequency, GHMATECD -> LINSOLVE ->
 ! INTEREC through libbemx.so (the drop-in subroutines of bemx_surface.f90),
 ! and writes the results in the reference's output layout (Outputec.for:9-68:

@@ -118,7 +118,7 @@ end subroutine
 
 ! ZH x = ZFI; x returned in ZFI, ZH destroyed (Linsolve.for:1-66).  A zero
 ! pivot is reported and the run continues, as LINSOLVE_CPU does
-! (Linsolve.for:314-318).
+! (Linsolve.for:102-106).
 subroutine LINSOLVE(NN, N, ZH, ZFI)
   use bemx_iface
   implicit none

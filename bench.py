@@ -1,31 +1,32 @@
 """Benchmark: BEM solves/s and p50/p99 latency, mixed precision vs plain FP64.
 
-Workloads (--workload):
-  solve  (default) the 48 x 45 flat free-surface patch (2160 boundary
-         elements + 5 enclosing, 6480 complex DOF), mu = 1, nu = 0.25,
-         rho = 1, damping 0.01, omega = 2.0, 8 x 8 Gauss points, unit
-         vertical traction on the central 2 x 2 elements.  A step is one
-         solve at one frequency with everything resident in HBM before the
-         timed region: dynamic assembly of ZH and the RHS (FP64), factor,
-         refine, solve.  Single-solve latency is measured on one context
-         (mixed and FP64 interleaved, every mixed result compared with the
-         FP64 result of the same step); `value` is the throughput of the
-         request loop (bemx_server) running the same solves on --slots
-         concurrent slots.
-  sweep  the same mesh, 256 frequencies log-spaced in [0.1, 10] rad/s, each
-         a full assembly + factor + refine, through the request loop.
-  serve  2000 requests with Poisson arrivals: meshes from {16x15, 24x22,
-         32x30, 48x45, 64x60}, omega ~ U[0.1, 10], damping ~ U[0.005, 0.05],
-         1..4 load cases; p50/p99 of submit-to-done latency.
-  large  96 x 90 patch over [-10, 10]^2 (8640 elements, 25920 complex DOF),
-         omega = 0.5, 6 x 6 Gauss, nu and mu drawn by seed; mixed path with
-         the plain FP64 path as reference.
-The mixed path (FP32 LU with split-BF16x3 MFMA trailing update + FP64
-GMRES-IR) is the headline; the plain FP64 partial-pivot path is measured
-beside it and the accuracy contract is <= 1e-10 relative to FP64.
-
-The CPU baseline (default workload) is the oracle (C, OpenMP, FP64, LAPACK
-zgesv via numpy) on the same problem, one solve, on the host's cores.
+Default run (`python bench.py`): the frequency sweep on the 48 x 45 flat
+free-surface patch over [-5, 5]^2 (2160 boundary elements + 5 enclosing,
+6480 complex DOF), mu = 1, nu = 0.25, rho = 1, damping 0.01, 8 x 8 Gauss
+points, unit vertical traction on the central 2 x 2 elements, 256
+frequencies log-spaced in [0.1, 10] rad/s.  A step is one frequency: the
+dynamic assembly of ZH and ZFI (FP64), the factorisation and the
+refinement, with the mesh, its static pass and the boundary data resident
+in HBM before the timed region.  The frequencies go through the request
+loop (bemx_server) on --slots slots; `value` is the mixed path's solves/s
+over the sweep and `fp64` the plain FP64 path's over the same sweep, every
+mixed result is compared with the FP64 result of its frequency.  Beside it:
+  single_solve  the same patch at omega = 2 on one context, mixed and FP64
+                interleaved: latency of one solve alone and the device times
+                of assembly, factorisation and refinement (HIP events)
+  serve         a slice of the serving stream (--requests Poisson arrivals at
+                --rate/s, meshes 16x15 .. 64x60, omega ~ U[0.1, 10], damping
+                ~ U[0.005, 0.05], 1..4 load cases): submit-to-done p50/p99,
+                graphs captured after warm-up, peak HBM charge
+  roofline      the mixed factorisation (split-BF16x3 MFMA trailing update):
+                8 n^3 BF16 flops per factorisation over its device time
+  roofline_assembly  k_dyn_offdiag_h (FP64 vector ALU): 242 flops per triple
+  cpu_baseline  the C oracle (OpenMP FP64 GHMATECD + LAPACK zgesv) solving
+                one frequency of the sweep on the host cores: the same unit
+                of work as a GPU step
+Other workloads (--workload): `solve` (the omega = 2 patch alone), `serve`
+(the full serving stream, 2000 requests by default), `large` (96 x 90 patch
+over [-10, 10]^2, 8640 BE, 25920 DOF, omega = 0.5, 6 x 6 Gauss).
 """
 import argparse
 import json
@@ -40,6 +41,7 @@ sys.path.insert(0, os.path.join(ROOT, "bem-mececcs_amd"))
 import numpy as np  # noqa: E402
 
 METRIC = ("BEM solves/s + p50/p99 latency at ≤1e-10 err vs FP64, mixed vs plain FP64")
+DTYPE = "f64 assembly + f32/bf16x3 factor + f64 refine"
 
 # FP64 operations per (collocation point, element, Gauss point) triple of the
 # off-diagonal assembly kernel k_dyn_offdiag_h, counted from the factored
@@ -48,14 +50,20 @@ METRIC = ("BEM solves/s + p50/p99 latency at ≤1e-10 err vs FP64, mixed vs plai
 # evaluated (the Newton steps that refine 1/R and the table-and-polynomial
 # evaluation of the exponentials are implementation details, not counted).
 FLOPS_PER_TRIPLE = 242
-FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
-# HBM bytes per launch of k_dyn_offdiag_h on the default workload, from the
-# rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a plain streaming copy of
-# known size) recorded in profiles/pmc_traffic_c2.md: 48.4 MB read + 671.6 MB
-# written (the ZH store itself is 671.8 MB).
-OFFDIAG_TRAFFIC_BYTES_C2 = 720.0e6
+FP64_PEAK_TFLOPS = 78.6    # MI355X dense FP64 (vector ALU and matrix cores alike)
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix cores
 HBM_PEAK_GBS = 8000.0
-DTYPE = "f64 assembly + f32/bf16x3 factor + f64 refine"
+# HBM bytes per launch on the 48 x 45 patch from the rocprofv3 FETCH_SIZE /
+# WRITE_SIZE runs (each scaled by the counter's reading for a streaming copy
+# of known size), recorded under profiles/.
+OFFDIAG_TRAFFIC_BYTES = 720.0e6
+FACTOR_TRAFFIC_BYTES = None
+# Speed of the reference's own CPU GHMATECD (its NONSINGD/SING_DE, built with
+# its Makefile's -Ofast -march=native -fopenmp) relative to the C oracle,
+# measured once on identical hardware (oracle/calibrate.py; profiles/
+# cpu_calibration.md): time(reference) / time(oracle) on the same mesh and
+# thread count.
+REF_OVER_ORACLE = 0.743
 
 
 def percentile(v, q):
@@ -67,7 +75,12 @@ def log(msg):
 
 
 def rel_err(a, b):
-    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(b)))
+
+
+def host_threads():
+    v = os.environ.get("OMP_NUM_THREADS")
+    return int(v) if v and v.isdigit() else (os.cpu_count() or 1)
 
 
 def base_record(args, workload, value, ms_per_step, steps):
@@ -76,8 +89,7 @@ def base_record(args, workload, value, ms_per_step, steps):
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": DTYPE,
         "data": "synthetic (generated flat-patch meshes)",
-        "config": dict(workload, parallelism="single GPU, %s" % (
-            "%d request-loop slots" % args.slots if args.slots else "one context")),
+        "config": dict(workload, parallelism="single GPU, %d request-loop slots" % args.slots),
     }
 
 
@@ -86,253 +98,316 @@ def stream_through_server(api, mode, slots, items, label, warm_items=()):
     are (mesh, kode, zdfi, (ge, rnu, dam, rho), omega).  warm_items run
     first on the same loop, untimed, so each slot has its workspace, its
     mesh and static pass and its captured factorisation before the timed
-    items.  Returns (x list, info list, wall seconds) of the timed items."""
+    items.  Returns (x list, info list, wall seconds, stats after warm-up,
+    stats at the end)."""
     srv = api.Server(0, nslots=slots, mode=mode)
     try:
         if warm_items:
             for i in [srv.submit(*it) for it in warm_items]:
                 srv.wait(i)
+        warm = srv.stats()
         t0 = time.perf_counter()
         ids = [srv.submit(*it) for it in items]
         outs = [srv.wait(i) for i in ids]
         wall = time.perf_counter() - t0
+        end = srv.stats()
     finally:
         srv.close()
     svc = [o[1]["service_ms"] for o in outs]
     log("%s: %d solves in %.3f s on %d slots (%.2f solves/s), service p50 %.2f ms" %
         (label, len(items), wall, slots, len(items) / wall, percentile(svc, 50)))
-    return [o[0] for o in outs], [o[1] for o in outs], wall
+    return [o[0] for o in outs], [o[1] for o in outs], wall, warm, end
 
 
-def one_context_sweep(api, mesh, dyn, freqs, mode, label):
-    """The frequencies of a sweep solved back to back on one context (no
-    request loop); returns (x list, info list, wall seconds)."""
+def sweep_frequencies(dyn, k):
+    """k frequencies of the 256-point sweep, evenly spaced over its indices
+    (all 256 for k = 256)."""
+    f = np.asarray(dyn.freqs, dtype=np.float64)
+    idx = np.round(np.linspace(0, len(f) - 1, k)).astype(int) if k <= len(f) else \
+        np.arange(k) % len(f)
+    return [float(f[i]) for i in idx]
+
+
+# ---------------------------------------------------------------------------
+def single_solve(args, api, decks, steps):
+    """The 48 x 45 patch at omega = 2 on one context: mixed and FP64
+    interleaved, latency of one solve alone and the device-time breakdown."""
+    mesh, dyn = decks.config_problem("c2")
+    fr = float(dyn.freqs[0])
+    zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
     ctx = api.Context(0)
     try:
         ctx.send_shared_data(mesh)
         ctx.ghmatece(mesh.ge, mesh.rnu)
         ctx.send_dyn_data(dyn.bc.astype(np.complex128), dyn.kode)
-        zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
-        ctx.solve(zmat, freqs[0], mode)  # warm
-        xs, infos = [], []
-        t0 = time.perf_counter()
-        for f in freqs:
-            t1 = time.perf_counter()
-            x, _ = ctx.solve(zmat, f, mode)
-            st = ctx.last_stats()
-            xs.append(x)
-            infos.append({"service_ms": (time.perf_counter() - t1) * 1e3,
-                          "gmres_iters": st["gmres_iters"], "fell_back": float(st["fell_back"])})
-        wall = time.perf_counter() - t0
+        for _ in range(max(1, args.warmup)):
+            ctx.solve(zmat, fr, api.MODE_FP64)
+            ctx.solve(zmat, fr, api.MODE_MIXED)
+        lat = {"mixed": [], "fp64": []}
+        dev = {"mixed": [], "fp64": []}
+        errs, iters = [], []
+        for step in range(steps):
+            out = {}
+            for name, mode in (("fp64", api.MODE_FP64), ("mixed", api.MODE_MIXED)):
+                t0 = time.perf_counter()
+                x, _ = ctx.solve(zmat, fr, mode)
+                lat[name].append((time.perf_counter() - t0) * 1e3)
+                dev[name].append(ctx.last_stats())
+                out[name] = x
+            errs.append(rel_err(out["mixed"], out["fp64"]))
+            iters.append(dev["mixed"][-1]["gmres_iters"])
+            sm, s64 = dev["mixed"][-1], dev["fp64"][-1]
+            log("omega=2 step %d: mixed %.2f ms (assembly %.2f, factor %.2f, refine %.2f, %d it), "
+                "fp64 %.2f ms (factor %.2f), rel err %.2e" %
+                (step, lat["mixed"][-1], sm["assembly_ms"], sm["lu_factor_ms"],
+                 sm["lu_refine_ms"], sm["gmres_iters"], lat["fp64"][-1], s64["lu_factor_ms"],
+                 errs[-1]))
     finally:
         ctx.close()
-    log("%s: %d solves in %.3f s on one context (%.2f solves/s)" %
-        (label, len(freqs), wall, len(freqs) / wall))
-    return xs, infos, wall
+
+    def mean(name, key):
+        return float(np.mean([d[key] for d in dev[name]]))
+
+    n = 3 * mesh.nbe
+    t_fac = mean("mixed", "lu_factor_ms")
+    t_fac64 = mean("fp64", "lu_factor_ms")
+    t_od = float(np.mean([d["offdiag_ms"] for d in dev["mixed"] + dev["fp64"]]))
+    triples = mesh.nbe * mesh.nbe * mesh.npg ** 2
+    bf16_flops = 8.0 * n ** 3          # 3 BF16 products x 8 n^3 / 3 complex-LU flops
+    fp64_flops = 8.0 * n ** 3 / 3.0
+    rec = {
+        "workload": "flat patch 48x45 (2160 BE + 5 EE, 6480 complex DOF), omega=2.0, 8x8 Gauss",
+        "mixed": {"p50_ms": percentile(lat["mixed"], 50), "p99_ms": percentile(lat["mixed"], 99),
+                  "device_ms": mean("mixed", "solve_ms"), "assembly_ms": mean("mixed", "assembly_ms"),
+                  "factor_ms": t_fac, "refine_ms": mean("mixed", "lu_refine_ms"),
+                  "gmres_iters_max": int(max(iters))},
+        "fp64": {"p50_ms": percentile(lat["fp64"], 50), "p99_ms": percentile(lat["fp64"], 99),
+                 "device_ms": mean("fp64", "solve_ms"), "assembly_ms": mean("fp64", "assembly_ms"),
+                 "factor_ms": t_fac64, "solve_ms": mean("fp64", "lu_refine_ms")},
+        "max_rel_err_mixed_vs_fp64": max(errs),
+    }
+    roof_lu = {"kernel": "mixed factorisation (graph: k_panel_inv, k_cgemm_update_inverse, "
+                         "k_cgemm_update2, ...; split-BF16x3 trailing update on MFMA)",
+               "bound": "mfma", "achieved": bf16_flops / (t_fac * 1e-3) / 1e12,
+               "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+               "frac": bf16_flops / (t_fac * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
+               "traffic": FACTOR_TRAFFIC_BYTES, "avg_ms": t_fac,
+               "algorithmic_flops": bf16_flops,
+               "note": "device time of the whole factorisation from HIP events around its graph "
+                       "launch; flops = 3 BF16 products x 8n^3/3, n = 6480"}
+    roof_fp64 = {"kernel": "FP64 factorisation (k_panel_fp64, k_zgemm_mfma, ...)",
+                 "bound": "mfma", "achieved": fp64_flops / (t_fac64 * 1e-3) / 1e12,
+                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": fp64_flops / (t_fac64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                 "avg_ms": t_fac64, "algorithmic_flops": fp64_flops}
+    ach = FLOPS_PER_TRIPLE * triples / (t_od * 1e-3) / 1e12
+    roof_asm = {"kernel": "k_dyn_offdiag_h (dynamic H assembly)", "bound": "valu",
+                "pipe": "FP64 vector ALU", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+                "traffic": OFFDIAG_TRAFFIC_BYTES, "avg_ms": t_od,
+                "flops_per_triple": FLOPS_PER_TRIPLE, "triples": triples}
+    return rec, roof_lu, roof_fp64, roof_asm
+
+
+def serve_slice(args, api, decks, n_requests, rate, check):
+    """Poisson arrivals of the serving mix through the request loop: every
+    slot warmed on the largest class first (its workspace then never grows)
+    and on every class, then the timed stream."""
+    meshes, kodes, reqs = decks.serving_stream(n_requests, rate, seed=0)
+    srv = api.Server(0, nslots=args.slots, mode=api.MODE_MIXED)
+    try:
+        big = len(meshes) - 1
+        order = [big] + list(range(big))
+        for c in order:
+            m = meshes[c]
+            z = np.zeros((3 * m.nbe, 4), np.complex128)
+            z[np.arange(4), np.arange(4)] = 1.0
+            for w in [srv.submit(m, kodes[c], z, (m.ge, m.rnu, 0.01, 1.0), 1.0)
+                      for _ in range(args.slots)]:
+                srv.wait(w)
+        warm = srv.stats()
+        log("serve: warm-up done (%d graphs captured)" % warm["graph_captures"])
+        ids = [None] * len(reqs)
+
+        def feeder():
+            t0 = time.perf_counter()
+            for k, r in enumerate(reqs):
+                dt = r["t_arrival"] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                m = meshes[r["cls"]]
+                ids[k] = srv.submit(m, kodes[r["cls"]], r["zdfi"], (m.ge, m.rnu, r["dam"], 1.0),
+                                    r["omega"])
+
+        t0 = time.perf_counter()
+        th = threading.Thread(target=feeder)
+        th.start()
+        outs = []
+        for k in range(len(reqs)):
+            while ids[k] is None:
+                time.sleep(0.0005)
+            outs.append(srv.wait(ids[k]))
+            if (k + 1) % 250 == 0:
+                log("serve: %d/%d requests done" % (k + 1, len(reqs)))
+        th.join()
+        wall = time.perf_counter() - t0
+        end = srv.stats()
+    finally:
+        srv.close()
+    lat = [o[1]["latency_ms"] for o in outs]
+    # accuracy: a sample of requests re-solved alone on the plain FP64 path
+    errs = []
+    ctx = api.Context(0)
+    try:
+        for k in range(0, len(reqs), max(1, len(reqs) // check)):
+            r = reqs[k]
+            m = meshes[r["cls"]]
+            ctx.send_shared_data(m)
+            ctx.ghmatece(m.ge, m.rnu)
+            ctx.send_load_cases(r["zdfi"], kodes[r["cls"]])
+            x64, _ = ctx.solve(api.material(m.ge, m.rnu, r["dam"], 1.0), r["omega"],
+                               api.MODE_FP64)
+            errs.append(rel_err(outs[k][0].reshape(x64.shape, order="F"), x64))
+    finally:
+        ctx.close()
+    solves = sum(r["zdfi"].shape[1] for r in reqs)
+    log("serve: %d requests in %.2f s, p50 %.1f ms, p99 %.1f ms, captures after warm-up %d" %
+        (len(reqs), wall, percentile(lat, 50), percentile(lat, 99),
+         end["graph_captures"] - warm["graph_captures"]))
+    return {
+        "workload": "serving stream: %d requests, Poisson arrivals at %.1f/s, meshes 16x15..64x60 "
+                    "(240-3840 BE), omega U[0.1,10], damping U[0.005,0.05], 1-4 load cases" %
+                    (len(reqs), rate),
+        "requests": len(reqs), "requests_per_s": len(reqs) / wall,
+        "load_cases_per_s": solves / wall,
+        "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+        "latency_note": "submit-to-done latency per request (queueing included)",
+        "service_p50_ms": percentile([o[1]["service_ms"] for o in outs], 50),
+        "service_p99_ms": percentile([o[1]["service_ms"] for o in outs], 99),
+        "graph_captures_after_warmup": end["graph_captures"] - warm["graph_captures"],
+        "graph_replays": end["graph_replays"] - warm["graph_replays"],
+        "peak_charged_bytes": end["peak_charged_bytes"], "budget_bytes": end["budget_bytes"],
+        "over_charge": end["over_charge"],
+        "max_rel_err_mixed_vs_fp64": max(errs), "checked_requests": len(errs),
+        "fallbacks_to_fp64": int(sum(o[1]["fell_back"] for o in outs)),
+    }
+
+
+def cpu_baseline(decks, mesh, dyn, fr):
+    """The C oracle solving one frequency of the sweep (GHMATECD + LAPACK
+    zgesv, the unit a GPU step does) on the host cores.  Returns the record
+    and the reference solution for the accuracy check: LAPACK on the
+    oracle's matrix with its diagonal blocks from the extended-precision
+    build (the FP64 difference kernels of those blocks lose digits at small
+    k R; not timed)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    orc = oracle.Oracle("c")
+    threads = host_threads()
+    hd, gd, consts = orc.ghmatece(mesh)  # resident before the timed GPU region too
+    zmat = orc.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
+    zdfi = dyn.bc.astype(np.complex128)
+    t0 = time.perf_counter()
+    zh, zfi = orc.ghmatecd(mesh, hd, gd, consts, dyn.kode, zdfi, fr, zmat)
+    t1 = time.perf_counter()
+    x = orc.linsolve(zh, zfi)
+    t2 = time.perf_counter()
+    del zh
+    tcpu = t2 - t0
+    rec = {"value": 1.0 / tcpu, "unit": "solves/s", "cores": threads, "kind": "port",
+           "sample": "one frequency (omega=%.4g) of the sweep: GHMATECD %.1f s + LAPACK zgesv "
+                     "%.1f s by the C oracle (OpenMP) and numpy on %d threads" %
+                     (fr, t1 - t0, t2 - t1, threads),
+           "reference_over_port_time": REF_OVER_ORACLE,
+           "reference_estimate_solves_per_s": 1.0 / (tcpu * REF_OVER_ORACLE)}
+    log("cpu oracle: %.1f s per solve on %d threads" % (tcpu, threads))
+    zh, zfi, _ = orc.ghmatecd_accurate(mesh, hd, gd, consts, dyn.kode, zdfi, fr, zmat)
+    x_acc = orc.linsolve(zh, zfi)
+    return rec, x_acc, x
 
 
 # ---------------------------------------------------------------------------
-def workload_solve(args, api, decks):
-    mesh, dyn = decks.config_problem("c2")
-    fr = float(dyn.freqs[0])
-    zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
-    ctx = api.Context(0)
-    ctx.send_shared_data(mesh)
-    ctx.ghmatece(mesh.ge, mesh.rnu)
-    ctx.send_dyn_data(dyn.bc.astype(np.complex128), dyn.kode)
-    log("workload c2: nbe=%d n=%d dof=%d npg=%d omega=%g" %
-        (mesh.nbe, mesh.n, 3 * mesh.nbe, mesh.npg, fr))
-    for _ in range(args.warmup):
-        ctx.solve(zmat, fr, api.MODE_FP64)
-        ctx.solve(zmat, fr, api.MODE_MIXED)
-    log("warmup done")
+def workload_default(args, api, decks):
+    mesh, dyn = decks.config_problem("c4")
+    log("frequency sweep: nbe=%d dof=%d npg=%d, %d frequencies" %
+        (mesh.nbe, 3 * mesh.nbe, mesh.npg, args.steps))
+    single, roof_lu, roof_fp64, roof_asm = single_solve(args, api, decks, args.single_steps)
 
-    # ---- single-solve latency: mixed and plain FP64, interleaved
-    lat_mixed, lat_fp64, errs, iters, fallbacks, offdiag_ms = [], [], [], [], 0, []
-    for step in range(args.steps):
-        t0 = time.perf_counter()
-        x64, _ = ctx.solve(zmat, fr, api.MODE_FP64)
-        t1 = time.perf_counter()
-        st64 = ctx.last_stats()
-        xm, _ = ctx.solve(zmat, fr, api.MODE_MIXED)
-        t2 = time.perf_counter()
-        stm = ctx.last_stats()
-        lat_fp64.append((t1 - t0) * 1e3)
-        lat_mixed.append((t2 - t1) * 1e3)
-        offdiag_ms.extend([st64["offdiag_ms"], stm["offdiag_ms"]])
-        iters.append(stm["gmres_iters"])
-        fallbacks += int(stm["fell_back"])
-        errs.append(rel_err(xm, x64))
-        log("step %d: fp64 %.2f ms (assembly %.2f, factor+solve %.2f), mixed %.2f ms "
-            "(assembly %.2f, factor+refine %.2f, %d GMRES it), rel err %.2e" %
-            (step, lat_fp64[-1], st64["assembly_ms"], st64["factor_ms"], lat_mixed[-1],
-             stm["assembly_ms"], stm["factor_ms"], stm["gmres_iters"], errs[-1]))
-    ctx.close()
+    freqs = sweep_frequencies(dyn, args.steps)
+    p = (mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
+    bc = dyn.bc.astype(np.complex128)
+    items = [(mesh, dyn.kode, bc, p, f) for f in freqs]
+    warm = [(mesh, dyn.kode, bc, p, freqs[(k * 37) % len(freqs)])
+            for k in range(max(1, args.warmup) * args.slots)]
+    xs, infos, wall, w0, w1 = stream_through_server(api, api.MODE_MIXED, args.slots, items,
+                                                    "mixed sweep", warm_items=warm)
+    x64, infos64, wall64, _, _ = stream_through_server(api, api.MODE_FP64, args.slots, items,
+                                                       "fp64 sweep", warm_items=warm)
+    errs = [rel_err(a, b) for a, b in zip(xs, x64)]
+    value = len(items) / wall
+    fp64_value = len(items) / wall64
+    svc = [i["service_ms"] for i in infos]
 
-    # ---- throughput: the same solves through the request loop
-    item = (mesh, dyn.kode, dyn.bc.astype(np.complex128), (mesh.ge, mesh.rnu, dyn.dam, dyn.rho),
-            fr)
-    n_stream = max(args.steps, 2 * args.slots)
-    warm = [item] * (max(1, args.warmup) * args.slots)
-    xs, infos, wall = stream_through_server(api, api.MODE_MIXED, args.slots, [item] * n_stream,
-                                            "mixed stream", warm_items=warm)
-    n64 = max(2, n_stream // 2)
-    x64s, _, wall64 = stream_through_server(api, api.MODE_FP64, args.slots, [item] * n64,
-                                            "fp64 stream", warm_items=[item] * args.slots)
-    errs.extend(rel_err(x, x64s[0]) for x in xs)
-    value = n_stream / wall
-    fp64_value = n64 / wall64
+    serve = serve_slice(args, api, decks, args.requests, args.rate, args.check) \
+        if args.requests > 0 else None
 
-    triples = mesh.nbe * mesh.nbe * mesh.npg ** 2
-    od = float(np.mean(offdiag_ms))
-    achieved = FLOPS_PER_TRIPLE * triples / (od * 1e-3) / 1e12
-
-    cpu = None
-    err_oracle = None
+    cpu, err_oracle, err_fp64_oracle = None, None, None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle
-        orc = oracle.Oracle("c")
-        threads = os.cpu_count() or 1
-        t0 = time.perf_counter()
-        ref = orc.solve_problem(mesh, dyn)
-        tcpu = time.perf_counter() - t0
-        err_oracle = rel_err(xm, ref["x"])
-        cpu = {"value": 1.0 / tcpu, "unit": "solves/s", "cores": threads, "kind": "port",
-               "sample": "one full solve of the same workload (GHMATECE, GHMATECD, LAPACK "
-                         "zgesv, INTEREC at 64 points) by the C oracle with OpenMP on %d "
-                         "threads: %.1f s" % (threads, tcpu)}
-        log("cpu oracle solve %.1f s; mixed vs oracle rel err %.2e" % (tcpu, err_oracle))
+        k = int(np.argmin(np.abs(np.log(np.asarray(freqs) / 2.0))))
+        cpu, xo, xo64 = cpu_baseline(decks, mesh, dyn, freqs[k])
+        err_oracle = rel_err(xs[k], xo)
+        err_fp64_oracle = rel_err(xo64, xo)
+        log("mixed vs oracle at omega=%.4g: %.2e (FP64 oracle vs accurate oracle %.2e)" %
+            (freqs[k], err_oracle, err_fp64_oracle))
 
-    out = base_record(args, {"workload": "flat patch 48x45 (2160 BE + 5 EE, 6480 complex DOF), "
-                                         "omega=2.0, 8x8 Gauss, one frequency per solve"},
-                      value, 1e3 / value, n_stream)
+    out = base_record(args, {"workload": "frequency sweep: flat patch 48x45 over [-5,5]^2 "
+                                         "(2160 BE + 5 EE, 6480 complex DOF), 8x8 Gauss, %d "
+                                         "frequencies log-spaced in [0.1, 10], one full assembly "
+                                         "+ factor + refine per frequency" % len(items)},
+                      value, 1e3 / value, len(items))
     out.update({
-        "p50_ms": percentile(lat_mixed, 50),
-        "p99_ms": percentile(lat_mixed, 99),
-        "latency_note": "p50/p99: one solve alone on one context; value: request-loop "
-                        "throughput on %d slots" % args.slots,
-        "stream_service_p50_ms": percentile([i["service_ms"] for i in infos], 50),
-        "stream_service_p99_ms": percentile([i["service_ms"] for i in infos], 99),
-        "single_context_solves_per_s": 1e3 / float(np.mean(lat_mixed)),
-        "fp64": {"value": fp64_value, "unit": "solves/s", "p50_ms": percentile(lat_fp64, 50),
-                 "p99_ms": percentile(lat_fp64, 99),
-                 "single_context_solves_per_s": 1e3 / float(np.mean(lat_fp64))},
+        "p50_ms": percentile(svc, 50), "p99_ms": percentile(svc, 99),
+        "latency_note": "p50/p99: per-frequency service time in the request loop (assembly + "
+                        "factor + refine); the serving stream's submit-to-done p50/p99 are under "
+                        "'serve'",
+        "fp64": {"value": fp64_value, "unit": "solves/s",
+                 "p50_ms": percentile([i["service_ms"] for i in infos64], 50),
+                 "p99_ms": percentile([i["service_ms"] for i in infos64], 99)},
         "speedup_mixed_vs_fp64": value / fp64_value,
         "max_rel_err_mixed_vs_fp64": max(errs),
         "rel_err_mixed_vs_oracle": err_oracle,
-        "gmres_iters": int(np.max(iters)),
-        "fallbacks_to_fp64": fallbacks,
-        "roofline": {"kernel": "k_dyn_offdiag_h (dynamic H assembly)",
-                     "pipe": "FP64 vector ALU", "bound": "mfma", "achieved": achieved,
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": OFFDIAG_TRAFFIC_BYTES_C2,
-                     "traffic_unit": "bytes per launch (rocprofv3 PMC)",
-                     "avg_ms": od, "flops_per_triple": FLOPS_PER_TRIPLE,
-                     "triples": triples},
+        "rel_err_fp64_oracle_vs_accurate_oracle": err_fp64_oracle,
+        "gmres_iters_max": int(max(i["gmres_iters"] for i in infos)),
+        "fallbacks_to_fp64": int(sum(i["fell_back"] for i in infos)),
+        "graph_captures_after_warmup": w1["graph_captures"] - w0["graph_captures"],
+        "single_solve": single,
+        "serve": serve,
+        "roofline": roof_lu,
+        "roofline_assembly": roof_asm,
+        "roofline_fp64_factor": roof_fp64,
         "cpu_baseline": cpu,
     })
     return out
 
 
-def workload_sweep(args, api, decks):
-    mesh, dyn = decks.config_problem("c4")
-    freqs = [float(f) for f in dyn.freqs][: args.nfreq]
-    p = (mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
-    bc = dyn.bc.astype(np.complex128)
-    items = [(mesh, dyn.kode, bc, p, f) for f in freqs]
-    if args.slots == 0:
-        # no request loop: one context in this thread, frequencies back to back
-        xs, infos, wall = one_context_sweep(api, mesh, dyn, freqs, api.MODE_MIXED, "mixed sweep")
-        x64, _, wall64 = one_context_sweep(api, mesh, dyn, freqs, api.MODE_FP64, "fp64 sweep")
-    else:
-        xs, infos, wall = stream_through_server(api, api.MODE_MIXED, args.slots, items,
-                                                "mixed sweep", warm_items=items[: args.slots])
-        x64, _, wall64 = stream_through_server(api, api.MODE_FP64, args.slots, items, "fp64 sweep",
-                                               warm_items=items[: args.slots])
-    errs = [rel_err(a, b) for a, b in zip(xs, x64)]
-    svc = [i["service_ms"] for i in infos]
-    value = len(items) / wall
-    out = base_record(args, {"workload": "frequency sweep: flat patch 48x45 (6480 complex DOF), "
-                                         "%d frequencies log-spaced in [0.1, 10]" % len(items)},
-                      value, 1e3 / value, len(items))
-    out.update({
-        "p50_ms": percentile(svc, 50), "p99_ms": percentile(svc, 99),
-        "latency_note": "per-frequency service time (assembly + factor + refine) in the loop",
-        "fp64": {"value": len(items) / wall64, "unit": "solves/s"},
-        "speedup_mixed_vs_fp64": value / (len(items) / wall64),
-        "max_rel_err_mixed_vs_fp64": max(errs),
-        "gmres_iters_max": int(max(i["gmres_iters"] for i in infos)),
-        "fallbacks_to_fp64": int(sum(i["fell_back"] for i in infos)),
-    })
+def workload_solve(args, api, decks):
+    single, roof_lu, roof_fp64, roof_asm = single_solve(args, api, decks, args.steps)
+    value = 1e3 / single["mixed"]["p50_ms"]
+    out = base_record(args, {"workload": single["workload"]}, value, single["mixed"]["p50_ms"],
+                      args.steps)
+    out["config"]["parallelism"] = "single GPU, one context"
+    out.update({"p50_ms": single["mixed"]["p50_ms"], "p99_ms": single["mixed"]["p99_ms"],
+                "fp64": single["fp64"], "single_solve": single, "roofline": roof_lu,
+                "roofline_assembly": roof_asm, "roofline_fp64_factor": roof_fp64})
     return out
 
 
 def workload_serve(args, api, decks):
-    meshes, kodes, reqs = decks.serving_stream(args.requests, args.rate, seed=0)
-    srv = api.Server(0, nslots=args.slots, mode=api.MODE_MIXED)
-    # warm every size class on every slot's workspace (4 load cases: the largest)
-    warm = []
-    for c, m in enumerate(meshes):
-        z = np.zeros((3 * m.nbe, 4), np.complex128)
-        z[np.arange(4), np.arange(4)] = 1.0
-        for _ in range(args.slots):
-            warm.append(srv.submit(m, kodes[c], z, (m.ge, m.rnu, 0.01, 1.0), 1.0))
-    for w in warm:
-        srv.wait(w)
-    log("warm done")
-    ids = [None] * len(reqs)
-
-    def feeder():
-        t0 = time.perf_counter()
-        for k, r in enumerate(reqs):
-            dt = r["t_arrival"] - (time.perf_counter() - t0)
-            if dt > 0:
-                time.sleep(dt)
-            m = meshes[r["cls"]]
-            ids[k] = srv.submit(m, kodes[r["cls"]], r["zdfi"], (m.ge, m.rnu, r["dam"], 1.0),
-                                r["omega"])
-
-    t0 = time.perf_counter()
-    th = threading.Thread(target=feeder)
-    th.start()
-    outs = []
-    for k in range(len(reqs)):
-        while ids[k] is None:
-            time.sleep(0.0005)
-        outs.append(srv.wait(ids[k]))
-        if (k + 1) % 250 == 0:
-            log("%d/%d requests done" % (k + 1, len(reqs)))
-    th.join()
-    wall = time.perf_counter() - t0
-    srv.close()
-    lat = [o[1]["latency_ms"] for o in outs]
-    # accuracy: a sample of requests re-solved alone on the plain FP64 path
-    errs = []
-    ctx = api.Context(0)
-    for k in range(0, len(reqs), max(1, len(reqs) // args.check)):
-        r = reqs[k]
-        m = meshes[r["cls"]]
-        ctx.send_shared_data(m)
-        ctx.ghmatece(m.ge, m.rnu)
-        ctx.send_load_cases(r["zdfi"], kodes[r["cls"]])
-        x64, _ = ctx.solve(api.material(m.ge, m.rnu, r["dam"], 1.0), r["omega"], api.MODE_FP64)
-        xm = outs[k][0]
-        errs.append(rel_err(xm.reshape(x64.shape, order="F"), x64))
-    ctx.close()
-    value = len(reqs) / wall
-    out = base_record(args, {"workload": "serving stream: %d requests, Poisson arrivals at %.1f/s, "
-                                         "meshes 16x15..64x60 (240-3840 BE), omega U[0.1,10], "
-                                         "damping U[0.005,0.05], 1-4 load cases" %
-                                         (len(reqs), args.rate)},
-                      value, 1e3 / value, len(reqs))
-    out.update({
-        "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
-        "latency_note": "submit-to-done latency per request (queueing included)",
-        "max_rel_err_mixed_vs_fp64": max(errs), "checked_requests": len(errs),
-        "fallbacks_to_fp64": int(sum(o[1]["fell_back"] for o in outs)),
-        "offered_rate": args.rate,
-    })
+    serve = serve_slice(args, api, decks, args.requests, args.rate, args.check)
+    out = base_record(args, {"workload": serve["workload"]}, serve["load_cases_per_s"],
+                      1e3 / serve["load_cases_per_s"], serve["requests"])
+    out.update({"p50_ms": serve["p50_ms"], "p99_ms": serve["p99_ms"], "serve": serve,
+                "value_note": "load cases solved per second (1-4 per request)"})
     return out
 
 
@@ -341,46 +416,60 @@ def workload_large(args, api, decks):
     fr = float(dyn.freqs[0])
     zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
     ctx = api.Context(0)
-    ctx.send_shared_data(mesh)
-    ctx.ghmatece(mesh.ge, mesh.rnu)
-    ctx.send_dyn_data(dyn.bc.astype(np.complex128), dyn.kode)
-    log("workload c3: nbe=%d dof=%d nu=%.3f mu=%.3f" % (mesh.nbe, 3 * mesh.nbe, mesh.rnu, mesh.ge))
-    ctx.solve(zmat, fr, api.MODE_MIXED)
-    lat = []
-    for _ in range(args.steps):
+    try:
+        ctx.send_shared_data(mesh)
+        ctx.ghmatece(mesh.ge, mesh.rnu)
+        ctx.send_dyn_data(dyn.bc.astype(np.complex128), dyn.kode)
+        log("workload large: nbe=%d dof=%d nu=%.3f mu=%.3f" %
+            (mesh.nbe, 3 * mesh.nbe, mesh.rnu, mesh.ge))
+        ctx.solve(zmat, fr, api.MODE_MIXED)
+        lat, fac = [], []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            xm, _ = ctx.solve(zmat, fr, api.MODE_MIXED)
+            lat.append((time.perf_counter() - t0) * 1e3)
+            stm = ctx.last_stats()
+            fac.append(stm["lu_factor_ms"])
+            log("mixed %.1f ms (assembly %.1f, factor %.1f, refine %.1f, %d GMRES it)" %
+                (lat[-1], stm["assembly_ms"], stm["lu_factor_ms"], stm["lu_refine_ms"],
+                 stm["gmres_iters"]))
         t0 = time.perf_counter()
-        xm, _ = ctx.solve(zmat, fr, api.MODE_MIXED)
-        lat.append((time.perf_counter() - t0) * 1e3)
-        stm = ctx.last_stats()
-        log("mixed %.1f ms (assembly %.1f, factor+refine %.1f, %d GMRES it)" %
-            (lat[-1], stm["assembly_ms"], stm["factor_ms"], stm["gmres_iters"]))
-    t0 = time.perf_counter()
-    x64, _ = ctx.solve(zmat, fr, api.MODE_FP64)
-    t64 = time.perf_counter() - t0
-    ctx.close()
+        x64, _ = ctx.solve(zmat, fr, api.MODE_FP64)
+        t64 = time.perf_counter() - t0
+    finally:
+        ctx.close()
+    n = 3 * mesh.nbe
+    t_fac = float(np.mean(fac))
     value = 1e3 / float(np.mean(lat))
     out = base_record(args, {"workload": "large single solve: flat patch 96x90 over [-10,10]^2 "
                                          "(8640 BE, 25920 complex DOF), omega=0.5, 6x6 Gauss"},
                       value, float(np.mean(lat)), args.steps)
+    out["config"]["parallelism"] = "single GPU, one context"
+    ach = 8.0 * n ** 3 / (t_fac * 1e-3) / 1e12
     out.update({"p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
                 "fp64": {"value": 1.0 / t64, "unit": "solves/s"},
                 "speedup_mixed_vs_fp64": value * t64,
-                "max_rel_err_mixed_vs_fp64": rel_err(xm, x64)})
+                "max_rel_err_mixed_vs_fp64": rel_err(xm, x64),
+                "roofline": {"kernel": "mixed factorisation (split-BF16x3 MFMA)", "bound": "mfma",
+                             "achieved": ach, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": ach / BF16_PEAK_TFLOPS, "traffic": None, "avg_ms": t_fac}})
     return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="solve", choices=["solve", "sweep", "serve", "large"])
-    ap.add_argument("--slots", type=int, default=2,
-                    help="request-loop slots (sweep: 0 = one context, no request loop)")
-    ap.add_argument("--nfreq", type=int, default=256, help="sweep: number of frequencies")
-    ap.add_argument("--requests", type=int, default=2000, help="serve: number of requests")
-    ap.add_argument("--rate", type=float, default=20.0, help="serve: arrivals per second")
-    ap.add_argument("--check", type=int, default=20, help="serve: requests re-solved in FP64")
+    ap.add_argument("--steps", type=int, default=256,
+                    help="sweep frequencies timed (default: all 256); other workloads: solves")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed requests per slot")
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "solve", "serve", "large"])
+    ap.add_argument("--slots", type=int, default=2, help="request-loop slots")
+    ap.add_argument("--single-steps", type=int, default=10,
+                    help="single-solve latency steps beside the sweep")
+    ap.add_argument("--requests", type=int, default=None,
+                    help="serving stream requests (default 300 beside the sweep, 2000 alone)")
+    ap.add_argument("--rate", type=float, default=20.0, help="serving arrivals per second")
+    ap.add_argument("--check", type=int, default=10, help="serving requests re-solved in FP64")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
@@ -388,10 +477,14 @@ def main():
         print("bench.py runs on one GPU: the solve is not sharded across devices "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.requests is None:
+        args.requests = 2000 if args.workload == "serve" else 300
+    if args.workload in ("solve", "large") and args.steps == 256:
+        args.steps = 10 if args.workload == "solve" else 5
     from bemx import api, decks
-    fn = {"solve": workload_solve, "sweep": workload_sweep, "serve": workload_serve,
+    fn = {"sweep": workload_default, "solve": workload_solve, "serve": workload_serve,
           "large": workload_large}[args.workload]
-    print(json.dumps(fn(args, api, decks)))
+    print(json.dumps(fn(args, api, decks)), flush=True)
 
 
 if __name__ == "__main__":

@@ -26,7 +26,7 @@ typedef struct bemx_ctx bemx_ctx;
 
 typedef enum {
     BEMX_OK = 0,
-    BEMX_ERR_DEGENERATE = 1, /* Jacobian DET < 1e-5 (Nonsingd.for:152-158); the
+    BEMX_ERR_DEGENERATE = 1, /* Jacobian DET < 1e-5 (Nonsingd.for:89-95); the
                                 reference's GPU status "Matriz Singular" */
     BEMX_ERR_SINGULAR = 2,   /* exactly zero pivot (LAPACK INFO > 0) */
     BEMX_ERR_ARG = 3,        /* invalid argument / size */
@@ -120,20 +120,23 @@ int bemx_interec(bemx_ctx *ctx, double *zfi, double *zdfi, const int *kode, int 
 int bemx_solve(bemx_ctx *ctx, const double *zge, const double *zcs, const double *zcp,
                double fr, int mode, double *zfi_out, double *stats);
 
-/* Statistics of the last bemx_linsolve / bemx_solve (8 doubles): GMRES
- * iterations, final ||r||/||b||, 1 if the mixed path fell back to FP64,
- * solve device time (ms), off-diagonal assembly kernel time (ms), whole
- * assembly time (ms), factor+refine+solve time (ms), 0.  Times are filled by
- * bemx_solve only.  No reference counterpart (the reference prints
- * per-routine wall times, Ghmatecd.for:183-184). */
-int bemx_last_stats(const bemx_ctx *ctx, double *out);
+/* Statistics of the last bemx_linsolve / bemx_solve (the first nout of 11
+ * doubles): GMRES iterations, final ||r||/||b||, 1 if the mixed path fell
+ * back to FP64, solve device time (ms), off-diagonal assembly kernel time
+ * (ms), whole assembly time (ms), factor+refine+solve time (ms),
+ * factorisation graphs captured by the context so far, factorisation time
+ * (ms), refinement/solve time after it (ms), graph replays so far.  Times
+ * are device times from HIP events, filled by bemx_solve only.  No
+ * reference counterpart (the reference prints per-routine wall times,
+ * Ghmatecd.for:183-184). */
+int bemx_last_stats(const bemx_ctx *ctx, double *out, int nout);
 
 /* Device buffer addresses of the last solve (for zero-copy consumers). */
 int bemx_device_buffers(bemx_ctx *ctx, void **zh, void **zfi, void **hip_stream);
 
 /* Complex material constants: mu* = G(1 + 2i DAM), cs = sqrt(mu* / rho),
  * cp = cs sqrt((2-2nu)/(1-2nu)).  Replaces the INPUTECD lines that form
- * ZGE/ZCS/ZCP (Inputecd.for:166-168).  Outputs are complex (re, im). */
+ * ZGE/ZCS/ZCP (Inputecd.for:58-60).  Outputs are complex (re, im). */
 int bemx_material(double ge, double rnu, double dam, double rho, double *zge, double *zcs,
                   double *zcp);
 
@@ -141,17 +144,21 @@ int bemx_material(double ge, double rnu, double dam, double rho, double *zge, do
  * No reference counterpart: the reference solves one deck per process
  * (Main.for).  nslots worker slots (1..16), each a context with its own
  * streams, take queued requests oldest first as soon as the request's HBM
- * footprint (ZH + factor workspace + assembly partials) fits in
- * hbm_budget_bytes (<= 0: 80% of the free HBM at creation); their work
- * overlaps on the device.  mode: BEMX_MODE_FP64 or BEMX_MODE_MIXED for
+ * workspace fits in hbm_budget_bytes (<= 0: 80% of the free HBM at
+ * creation); their work overlaps on the device.  A slot is charged the
+ * workspace bound (bemx_request_footprint) at the largest sizes it has
+ * served, since its buffers only grow; idle slots give their workspace back
+ * when the oldest request does not fit.  mode: BEMX_MODE_FP64 or BEMX_MODE_MIXED for
  * every request.  Mesh arrays as bemx_send_shared_data, material as
  * bemx_material, ZDFI(3NBE, nrhs) / KODE(3NBE) as bemx_send_load_cases;
  * everything is copied at submit.  A request larger than the whole budget
  * is refused with BEMX_ERR_NOMEM. */
 typedef struct bemx_server bemx_server;
-/* Device bytes a request of nbe elements and nrhs load cases is admitted
- * with (what the budget is charged while it runs). */
-double bemx_request_footprint(int nbe, int nrhs, int mode);
+/* Device bytes a slot is charged once it has served a request of these
+ * sizes (nbe boundary elements, n elements, np nodes, npg Gauss points per
+ * direction, nrhs load cases): an upper bound of every buffer its context
+ * holds after the request.  0 for invalid sizes. */
+double bemx_request_footprint(int nbe, int n, int np, int npg, int nrhs, int mode);
 int bemx_server_create(int device, int nslots, double hbm_budget_bytes, int mode,
                        bemx_server **srv);
 int bemx_server_submit(bemx_server *srv, const double *cx, const double *cy, const double *cz,
@@ -166,8 +173,15 @@ int bemx_server_submit(bemx_server *srv, const double *cx, const double *cy, con
  * (may be NULL) receives the error text.  Each id can be waited once. */
 int bemx_server_wait(bemx_server *srv, long long id, double *x, double *info, char *msg,
                      int msglen);
+/* Statistics of the request loop (nout <= 8 doubles): requests completed,
+ * factorisation graphs captured, graph replays, bytes charged now, peak
+ * bytes charged, the budget, workspaces given back to make room, and the
+ * number of requests after which a slot held more device bytes than it was
+ * charged (0 unless the bound is wrong). */
+int bemx_server_stats(bemx_server *srv, double *out, int nout);
 /* Stops the workers after their current request; requests still queued
- * are dropped.  Must not run concurrently with bemx_server_wait. */
+ * are dropped.  Each worker releases its slot's workspace before it
+ * returns.  Must not run concurrently with bemx_server_wait. */
 int bemx_server_destroy(bemx_server *srv);
 
 #ifdef __cplusplus

@@ -5,7 +5,8 @@
  * path is compared against; it is linked by tests/, __graft_entry__.smoke()
  * and bench.py's cpu_baseline leg, never by the product library.
  *
- * It restates, in plain C99 double / double complex arithmetic, the
+ * It restates, in plain C99 real / real complex arithmetic (real = double
+ * in the default build), the
  * algorithm of the reference (giulianobelinassi/BEM-MECECCS, FPREC=double
  * CPU build).  Each routine cites the reference file:line it follows.
  * Layout conventions are the reference's Fortran ones:
@@ -23,12 +24,24 @@
  * against closed forms (Legendre rules, Kelvin static limit, Hooke's law on
  * finite-difference gradients, LAPACK zgesv).
  */
-#include <complex.h>
-#include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <tgmath.h>
 
+/* The working precision.  The default build is the FP64 restatement of the
+ * reference; the build with -DORC_LONG_DOUBLE evaluates the same formulas in
+ * x87 extended precision (64-bit significand) and serves as the reference
+ * value where the FP64 formulas lose digits (the dynamic kernels at small
+ * k R), through the orcld_* entry points at the end of this file. */
+#ifdef ORC_LONG_DOUBLE
+typedef long double real;
+typedef long double complex zc;
+#define ORC_CMPLX(a, b) CMPLXL((a), (b))
+#else
+typedef double real;
 typedef double complex zc;
+#define ORC_CMPLX(a, b) CMPLX((a), (b))
+#endif
 
 #define ORC_OK 0
 #define ORC_DEGENERATE 1
@@ -38,14 +51,14 @@ typedef double complex zc;
 /* Gauss-Legendre rule on [-1,1].  Gauleg.for:9-36 (Newton on P_n,    */
 /* EPS = 3e-14, initial guess cos(3.141592654 (i-1/4)/(n+1/2))).       */
 /* ------------------------------------------------------------------ */
-void orc_gauleg(int n, double *x, double *w)
+void orc_gauleg(int n, real *x, real *w)
 {
-    const double x1 = -1.0, x2 = 1.0;
-    const double xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1);
+    const real x1 = -1.0, x2 = 1.0;
+    const real xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1);
     int m = (n + 1) / 2;
     for (int i = 1; i <= m; ++i) {
-        double z = cos(3.141592654 * (i - 0.25) / (n + 0.5));
-        double z1, pp, p1, p2, p3;
+        real z = cos(3.141592654 * (i - 0.25) / (n + 0.5));
+        real z1, pp, p1, p2, p3;
         do {
             p1 = 1.0;
             p2 = 0.0;
@@ -66,19 +79,19 @@ void orc_gauleg(int n, double *x, double *w)
 }
 
 /* Unit normal (P2-P1)x(P3-P1)/|.|.  Normvec.for:4-39 */
-void orc_normvec(int n, int np, const int *cone, const double *cx,
-                 const double *cy, const double *cz, double *etas)
+void orc_normvec(int n, int np, const int *cone, const real *cx,
+                 const real *cy, const real *cz, real *etas)
 {
     (void)np;
     for (int j = 0; j < n; ++j) {
         int n1 = cone[j] - 1, n2 = cone[n + j] - 1, n3 = cone[2 * n + j] - 1;
-        double a = (cy[n2] - cy[n1]) * (cz[n3] - cz[n1]) -
+        real a = (cy[n2] - cy[n1]) * (cz[n3] - cz[n1]) -
                    (cz[n2] - cz[n1]) * (cy[n3] - cy[n1]);
-        double b = (cz[n2] - cz[n1]) * (cx[n3] - cx[n1]) -
+        real b = (cz[n2] - cz[n1]) * (cx[n3] - cx[n1]) -
                    (cx[n2] - cx[n1]) * (cz[n3] - cz[n1]);
-        double c = (cx[n2] - cx[n1]) * (cy[n3] - cy[n1]) -
+        real c = (cx[n2] - cx[n1]) * (cy[n3] - cy[n1]) -
                    (cy[n2] - cy[n1]) * (cx[n3] - cx[n1]);
-        double r = sqrt(a * a + b * b + c * c);
+        real r = sqrt(a * a + b * b + c * c);
         etas[3 * j + 0] = a / r;
         etas[3 * j + 1] = b / r;
         etas[3 * j + 2] = c / r;
@@ -86,8 +99,8 @@ void orc_normvec(int n, int np, const int *cone, const double *cx,
 }
 
 /* Collocation points: centroid of the 4 nodes.  Inputece.for:93-102 */
-void orc_centroids(int n, const int *cone, const double *cx, const double *cy,
-                   const double *cz, double *cxm, double *cym, double *czm)
+void orc_centroids(int n, const int *cone, const real *cx, const real *cy,
+                   const real *cz, real *cxm, real *cym, real *czm)
 {
     for (int j = 0; j < n; ++j) {
         int a = cone[j] - 1, b = cone[n + j] - 1, c = cone[2 * n + j] - 1,
@@ -100,9 +113,9 @@ void orc_centroids(int n, const int *cone, const double *cx, const double *cy,
 
 /* Element corner coordinates CO(4,3) of element j (0-based).
  * Ghmatecd.for:116-134 */
-static void element_corners(int n, const int *cone, const double *cx,
-                            const double *cy, const double *cz, int j,
-                            double co[4][3])
+static void element_corners(int n, const int *cone, const real *cx,
+                            const real *cy, const real *cz, int j,
+                            real co[4][3])
 {
     for (int k = 0; k < 4; ++k) {
         int p = cone[k * n + j] - 1;
@@ -113,13 +126,13 @@ static void element_corners(int n, const int *cone, const double *cx,
 }
 
 /* One Gauss point of the bilinear quad map: position xg and the weight
- * OME(ig)*OME(jg)*DET.  Nonsingd.for:108-176 (shape functions F, their
+ * OME(ig)*OME(jg)*DET.  Nonsingd.for:45-106 (shape functions F, their
  * derivatives P, Jacobian XJ, DET = |XJ1 x XJ2|, DET < 1e-5 fatal). */
-static int quad_point(const double co[4][3], double xi, double wxi, double eta,
-                      double weta, double xg[3], double *wdet)
+static int quad_point(const real co[4][3], real xi, real wxi, real eta,
+                      real weta, real xg[3], real *wdet)
 {
-    double sp = 1.0 + eta, sm = 1.0 - eta, rp = 1.0 + xi, rm = 1.0 - xi;
-    double p[2][4], f[4], xj[2][3];
+    real sp = 1.0 + eta, sm = 1.0 - eta, rp = 1.0 + xi, rm = 1.0 - xi;
+    real p[2][4], f[4], xj[2][3];
     p[0][0] = -0.25 * sm;
     p[0][1] = 0.25 * sm;
     p[0][2] = 0.25 * sp;
@@ -134,17 +147,17 @@ static int quad_point(const double co[4][3], double xi, double wxi, double eta,
     p[1][3] = 0.25 * rm;
     for (int i = 0; i < 2; ++i)
         for (int c = 0; c < 3; ++c) {
-            double t = 0.0;
+            real t = 0.0;
             for (int k = 0; k < 4; ++k) t = t + p[i][k] * co[k][c];
             xj[i][c] = t;
         }
-    double a = xj[0][1] * xj[1][2] - xj[1][1] * xj[0][2];
-    double b = xj[1][0] * xj[0][2] - xj[0][0] * xj[1][2];
-    double c = xj[0][0] * xj[1][1] - xj[1][0] * xj[0][1];
-    double det = sqrt(a * a + b * b + c * c);
+    real a = xj[0][1] * xj[1][2] - xj[1][1] * xj[0][2];
+    real b = xj[1][0] * xj[0][2] - xj[0][0] * xj[1][2];
+    real c = xj[0][0] * xj[1][1] - xj[1][0] * xj[0][1];
+    real det = sqrt(a * a + b * b + c * c);
     if (det < 1.0e-5) return ORC_DEGENERATE;
     for (int d = 0; d < 3; ++d) {
-        double s = 0.0;
+        real s = 0.0;
         for (int k = 0; k < 4; ++k) s = s + co[k][d] * f[k];
         xg[d] = s;
     }
@@ -157,16 +170,16 @@ static int quad_point(const double co[4][3], double xi, double wxi, double eta,
 /* ------------------------------------------------------------------ */
 
 /* Static traction kernel T(3,3).  Solfune.for:7-34 */
-void orc_solfune(const double xp[3], const double xg[3], const double eta[3],
-                 double c3, double c4, double *t)
+void orc_solfune(const real xp[3], const real xg[3], const real eta[3],
+                 real c3, real c4, real *t)
 {
-    double r1 = xg[0] - xp[0], r2 = xg[1] - xp[1], r3 = xg[2] - xp[2];
-    double r = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
-    double rdn = (r1 * eta[0] + r2 * eta[1] + r3 * eta[2]) / r;
-    double rd[3] = {r1 / r, r2 / r, r3 / r};
+    real r1 = xg[0] - xp[0], r2 = xg[1] - xp[1], r3 = xg[2] - xp[2];
+    real r = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
+    real rdn = (r1 * eta[0] + r2 * eta[1] + r3 * eta[2]) / r;
+    real rd[3] = {r1 / r, r2 / r, r3 / r};
     for (int j = 0; j < 3; ++j)
         for (int i = 0; i < 3; ++i) {
-            double dij = (i == j) ? 1.0 : 0.0;
+            real dij = (i == j) ? 1.0 : 0.0;
             t[i + 3 * j] = (c3 / (r * r)) *
                            (rdn * (c4 * dij + 3.0 * rd[i] * rd[j]) +
                             c4 * (rd[j] * eta[i] - rd[i] * eta[j]));
@@ -174,41 +187,41 @@ void orc_solfune(const double xp[3], const double xg[3], const double eta[3],
 }
 
 /* Static displacement kernel U(3,3).  Singge.for:175-202 */
-void orc_solfune_g(const double xp[3], const double xg[3], double c1,
-                   double c2, double *u)
+void orc_solfune_g(const real xp[3], const real xg[3], real c1,
+                   real c2, real *u)
 {
-    double r1 = xg[0] - xp[0], r2 = xg[1] - xp[1], r3 = xg[2] - xp[2];
-    double r = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
-    double rd[3] = {r1 / r, r2 / r, r3 / r};
+    real r1 = xg[0] - xp[0], r2 = xg[1] - xp[1], r3 = xg[2] - xp[2];
+    real r = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
+    real rd[3] = {r1 / r, r2 / r, r3 / r};
     for (int j = 0; j < 3; ++j)
         for (int i = 0; i < 3; ++i) {
-            double dij = (i == j) ? 1.0 : 0.0;
+            real dij = (i == j) ? 1.0 : 0.0;
             u[i + 3 * j] = (c1 / r) * (c2 * dij + rd[i] * rd[j]);
         }
 }
 
 /* Dynamic scalar functions psi, chi, psi', chi' and A, B, C.
- * Solfund.for:29-53 (identical in Solfundif.for:344-368). */
+ * Solfund.for:29-53 (identical in Solfundif.for:29-53). */
 struct dyn_scalars {
     zc c0, fhi, cappa, fhidr, cappadr, aa, bb, cc;
 };
 
-static void dyn_scalars(double r, zc zge, zc zcs, zc zcp, double pi,
-                        double fr, struct dyn_scalars *s)
+static void dyn_scalars(real r, zc zge, zc zcs, zc zcp, real pi,
+                        real fr, struct dyn_scalars *s)
 {
-    zc zwi = CMPLX(0.0, fr);
-    zc zc0 = 1.0 / (CMPLX(4.0 * pi, 0.0) * zge);
+    zc zwi = ORC_CMPLX(0.0, fr);
+    zc zc0 = 1.0 / (ORC_CMPLX(4.0 * pi, 0.0) * zge);
     zc zc1 = (zcp / zcs) * (zcp / zcs);
     zc zc2 = (zcs / zcp) * (zcs / zcp);
     zc zkp = -zwi / zcp;
     zc zks = -zwi / zcs;
     zc zzp = zkp * r;
     zc zzs = zks * r;
-    zc zezp = cexp(zzp);
-    zc zezs = cexp(zzs);
+    zc zezp = exp(zzp);
+    zc zezs = exp(zzs);
     zc zp2 = zzp * zzp;
     zc zs2 = zzs * zzs;
-    double r2 = r * r;
+    real r2 = r * r;
     s->c0 = zc0;
     s->fhi = (1.0 + 1.0 / zs2 - 1.0 / zzs) * zezs / r -
              zc2 * (1.0 / zp2 - 1.0 / zzp) * zezp / r;
@@ -226,18 +239,18 @@ static void dyn_scalars(double r, zc zge, zc zcs, zc zcp, double pi,
 
 /* Dynamic displacement ZU(3,3) and traction ZT(3,3) kernels.
  * Solfund.for:7-63 */
-void orc_solfund(const double xp[3], const double xg[3], const double rn[3],
-                 zc zge, zc zcs, zc zcp, double pi, double fr, zc *zu, zc *zt)
+void orc_solfund(const real xp[3], const real xg[3], const real rn[3],
+                 zc zge, zc zcs, zc zcp, real pi, real fr, zc *zu, zc *zt)
 {
-    double r1 = xg[0] - xp[0], r2 = xg[1] - xp[1], r3 = xg[2] - xp[2];
-    double r = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
-    double drn = (r1 * rn[0] + r2 * rn[1] + r3 * rn[2]) / r;
-    double rd[3] = {r1 / r, r2 / r, r3 / r};
+    real r1 = xg[0] - xp[0], r2 = xg[1] - xp[1], r3 = xg[2] - xp[2];
+    real r = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
+    real drn = (r1 * rn[0] + r2 * rn[1] + r3 * rn[2]) / r;
+    real rd[3] = {r1 / r, r2 / r, r3 / r};
     struct dyn_scalars s;
     dyn_scalars(r, zge, zcs, zcp, pi, fr, &s);
     for (int j = 0; j < 3; ++j)
         for (int i = 0; i < 3; ++i) {
-            double dij = (i == j) ? 1.0 : 0.0;
+            real dij = (i == j) ? 1.0 : 0.0;
             zu[i + 3 * j] = s.c0 * (s.fhi * dij - s.cappa * rd[j] * rd[i]);
             zt[i + 3 * j] = (1.0 / (4.0 * pi)) *
                             ((s.aa * (drn * dij + rd[j] * rn[i])) +
@@ -246,26 +259,26 @@ void orc_solfund(const double xp[3], const double xg[3], const double rn[3],
 }
 
 /* Regularised difference (dynamic - static) kernels.  Solfundif.for:7-72 */
-void orc_solfundif(const double xp[3], const double xg[3], const double rn[3],
-                   zc zge, zc zcs, zc zcp, double c1, double c2, double c3,
-                   double c4, double pi, double fr, zc *zudif, zc *ztdif)
+void orc_solfundif(const real xp[3], const real xg[3], const real rn[3],
+                   zc zge, zc zcs, zc zcp, real c1, real c2, real c3,
+                   real c4, real pi, real fr, zc *zudif, zc *ztdif)
 {
-    double r1 = xg[0] - xp[0], r2 = xg[1] - xp[1], r3 = xg[2] - xp[2];
-    double r = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
-    double drn = (r1 * rn[0] + r2 * rn[1] + r3 * rn[2]) / r;
-    double rd[3] = {r1 / r, r2 / r, r3 / r};
+    real r1 = xg[0] - xp[0], r2 = xg[1] - xp[1], r3 = xg[2] - xp[2];
+    real r = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
+    real drn = (r1 * rn[0] + r2 * rn[1] + r3 * rn[2]) / r;
+    real rd[3] = {r1 / r, r2 / r, r3 / r};
     struct dyn_scalars s;
     dyn_scalars(r, zge, zcs, zcp, pi, fr, &s);
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
-            double dij = (i == j) ? 1.0 : 0.0;
+            real dij = (i == j) ? 1.0 : 0.0;
             zudif[i + 3 * j] = s.c0 * (s.fhi * dij - s.cappa * rd[j] * rd[i]) -
-                               CMPLX((c1 / r) * (c2 * dij + rd[i] * rd[j]), 0.0);
+                               ORC_CMPLX((c1 / r) * (c2 * dij + rd[i] * rd[j]), 0.0);
             ztdif[i + 3 * j] =
                 (1.0 / (4.0 * pi)) * ((s.aa * (drn * dij + rd[j] * rn[i])) +
                                       rd[i] * rd[j] * drn * s.bb +
                                       rd[i] * rn[j] * s.cc) -
-                CMPLX((c3 / (r * r)) * (drn * (c4 * dij + 3.0 * rd[i] * rd[j]) +
+                ORC_CMPLX((c3 / (r * r)) * (drn * (c4 * dij + 3.0 * rd[i] * rd[j]) +
                                         c4 * (rd[j] * rn[i] - rd[i] * rn[j])),
                       0.0);
         }
@@ -276,11 +289,11 @@ void orc_solfundif(const double xp[3], const double xg[3], const double rn[3],
 /* ------------------------------------------------------------------ */
 
 /* NPG x NPG Gauss of the static traction kernel.  Nonsinge.for:12-121 */
-int orc_nonsinge(const double co[4][3], const double xp[3], const double eta[3],
-                 double c3, double c4, const double *gi, const double *ome,
-                 int npg, double *helem)
+int orc_nonsinge(const real co[4][3], const real xp[3], const real eta[3],
+                 real c3, real c4, const real *gi, const real *ome,
+                 int npg, real *helem)
 {
-    double t[9], xg[3], w;
+    real t[9], xg[3], w;
     for (int k = 0; k < 9; ++k) helem[k] = 0.0;
     for (int jg = 0; jg < npg; ++jg)
         for (int ig = 0; ig < npg; ++ig) {
@@ -293,11 +306,11 @@ int orc_nonsinge(const double co[4][3], const double xp[3], const double eta[3],
 }
 
 /* NPG x NPG Gauss of U over a quad.  Singge.for:83-173 (NONSINGE_G) */
-static int nonsinge_g(const double co[4][3], const double xp[3], double c1,
-                      double c2, const double *gi, const double *ome, int npg,
-                      double *gelem)
+static int nonsinge_g(const real co[4][3], const real xp[3], real c1,
+                      real c2, const real *gi, const real *ome, int npg,
+                      real *gelem)
 {
-    double u[9], xg[3], w;
+    real u[9], xg[3], w;
     for (int k = 0; k < 9; ++k) gelem[k] = 0.0;
     for (int jg = 0; jg < npg; ++jg)
         for (int ig = 0; ig < npg; ++ig) {
@@ -311,11 +324,11 @@ static int nonsinge_g(const double co[4][3], const double xp[3], double c1,
 
 /* Weakly singular static G diagonal block: four quads collapsed onto the
  * centroid, edges (N1,N2),(N2,N3),(N3,N4),(N4,N1).  Singge.for:11-81 */
-int orc_singge(const double corners[4][3], const double xm[3], double c1,
-               double c2, const double *gi, const double *ome, int npg,
-               double *gelem)
+int orc_singge(const real corners[4][3], const real xm[3], real c1,
+               real c2, const real *gi, const real *ome, int npg,
+               real *gelem)
 {
-    double co[4][3], gp[9];
+    real co[4][3], gp[9];
     for (int k = 0; k < 9; ++k) gelem[k] = 0.0;
     for (int d = 0; d < 3; ++d) co[2][d] = co[3][d] = xm[d];
     for (int it = 0; it < 4; ++it) {
@@ -331,14 +344,14 @@ int orc_singge(const double corners[4][3], const double xm[3], double c1,
     return ORC_OK;
 }
 
-/* NPG x NPG Gauss of the dynamic kernels.  Nonsingd.for:74-185 */
-int orc_nonsingd(const double co[4][3], const double xp[3], const double eta[3],
-                 zc zge, zc zcs, zc zcp, double pi, double fr,
-                 const double *gi, const double *ome, int npg, zc *zhelem,
+/* NPG x NPG Gauss of the dynamic kernels.  Nonsingd.for:45-118 */
+int orc_nonsingd(const real co[4][3], const real xp[3], const real eta[3],
+                 zc zge, zc zcs, zc zcp, real pi, real fr,
+                 const real *gi, const real *ome, int npg, zc *zhelem,
                  zc *zgelem)
 {
     zc zu[9], zt[9];
-    double xg[3], w;
+    real xg[3], w;
     for (int k = 0; k < 9; ++k) zhelem[k] = zgelem[k] = 0.0;
     for (int jg = 0; jg < npg; ++jg)
         for (int ig = 0; ig < npg; ++ig) {
@@ -353,14 +366,14 @@ int orc_nonsingd(const double co[4][3], const double xp[3], const double eta[3],
     return ORC_OK;
 }
 
-/* Diagonal block by plain Gauss of the difference kernel.  Sing_de.for:215-315 */
-int orc_sing_de(const double co[4][3], const double xp[3], const double eta[3],
-                zc zge, zc zcs, zc zcp, double c1, double c2, double c3,
-                double c4, double pi, double fr, const double *gi,
-                const double *ome, int npg, zc *zhdif, zc *zgdif)
+/* Diagonal block by plain Gauss of the difference kernel.  Sing_de.for:13-113 */
+int orc_sing_de(const real co[4][3], const real xp[3], const real eta[3],
+                zc zge, zc zcs, zc zcp, real c1, real c2, real c3,
+                real c4, real pi, real fr, const real *gi,
+                const real *ome, int npg, zc *zhdif, zc *zgdif)
 {
     zc zu[9], zt[9];
-    double xg[3], w;
+    real xg[3], w;
     for (int k = 0; k < 9; ++k) zhdif[k] = zgdif[k] = 0.0;
     for (int jg = 0; jg < npg; ++jg)
         for (int ig = 0; ig < npg; ++ig) {
@@ -377,32 +390,32 @@ int orc_sing_de(const double co[4][3], const double xp[3], const double eta[3],
 }
 
 /* Interior stress kernels ZD(3,3,3), ZS(3,3,3).  Sigmaec.for:9-212 */
-int orc_sigmaec(const double co[4][3], const double xp[3], const double rn[3],
-                double pi, double fr, zc zge, double rho, zc zcs, zc zcp,
-                const double *gi, const double *ome, int npg, zc *zd, zc *zs)
+int orc_sigmaec(const real co[4][3], const real xp[3], const real rn[3],
+                real pi, real fr, zc zge, real rho, zc zcs, zc zcp,
+                const real *gi, const real *ome, int npg, zc *zd, zc *zs)
 {
-    double xg[3], w;
+    real xg[3], w;
     for (int k = 0; k < 27; ++k) zd[k] = zs[k] = 0.0;
     for (int jg = 0; jg < npg; ++jg)
         for (int ig = 0; ig < npg; ++ig) {
             if (quad_point(co, gi[ig], ome[ig], gi[jg], ome[jg], xg, &w))
                 return ORC_DEGENERATE;
-            double r1 = xg[0] - xp[0], r2 = xg[1] - xp[1], r3 = xg[2] - xp[2];
-            double r = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
-            double drn = (r1 * rn[0] + r2 * rn[1] + r3 * rn[2]) / r;
-            double rd[3] = {r1 / r, r2 / r, r3 / r};
+            real r1 = xg[0] - xp[0], r2 = xg[1] - xp[1], r3 = xg[2] - xp[2];
+            real r = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
+            real drn = (r1 * rn[0] + r2 * rn[1] + r3 * rn[2]) / r;
+            real rd[3] = {r1 / r, r2 / r, r3 / r};
             struct dyn_scalars s;
             dyn_scalars(r, zge, zcs, zcp, pi, fr, &s);
             /* Fedelinski form, Sigmaec.for:139-176 */
-            zc zwi = CMPLX(0.0, fr);
+            zc zwi = ORC_CMPLX(0.0, fr);
             zc zk1 = zwi / zcp, zk2 = zwi / zcs;
-            zc zek1 = cexp(-zk1 * r), zek2 = cexp(-zk2 * r);
+            zc zek1 = exp(-zk1 * r), zek2 = exp(-zk2 * r);
             zc zmu = zge;
             zc zlamb = (zcp * zcp * rho) - (2.0 * zmu);
             zc q1 = zk1 * r, q2 = zk2 * r;
             zc q1_2 = q1 * q1, q2_2 = q2 * q2;
             zc csp2 = (zcs / zcp) * (zcs / zcp);
-            double r3c = r * r * r;
+            real r3c = r * r * r;
             zc fhidrr = -csp2 * zek1 * (12.0 + 12.0 * q1 + 5.0 * q1_2 + q1_2 * q1) *
                             (1.0 / (q1_2 * r3c)) +
                         zek2 *
@@ -419,7 +432,7 @@ int orc_sigmaec(const double co[4][3], const double xp[3], const double rn[3],
                      q2_2 * q2_2) *
                     (1.0 / (q2_2 * r3c));
             zc cappa = s.cappa, fhidr = s.fhidr, cappadr = s.cappadr;
-            double rr = r * r;
+            real rr = r * r;
             zc lm = zlamb / zmu;
             zc cte1 = 2.0 * (cappa / r);
             zc cte2 = lm * (fhidr - cappadr - cte1);
@@ -436,11 +449,11 @@ int orc_sigmaec(const double co[4][3], const double xp[3], const double rn[3],
             zc cte11 = (lm * lm) * (cappadrr + 4.0 * (cappadr / r) +
                                     2.0 * (cappa / rr) - fhidrr - 2.0 * (fhidr / r));
             zc cte12 = 2.0 * ((fhidr / r) - (cappa / rr));
-            double q4pi = 1.0 / (4.0 * pi);
+            real q4pi = 1.0 / (4.0 * pi);
             for (int k = 0; k < 3; ++k)
                 for (int i = 0; i < 3; ++i)
                     for (int j = 0; j < 3; ++j) {
-                        double dji = (j == i), djk = (j == k), dik = (i == k);
+                        real dji = (j == i), djk = (j == k), dik = (i == k);
                         zc dp = q4pi * ((cte1 - cte2) * dji * rd[k] -
                                         cte3 * (djk * rd[i] + dik * rd[j]) +
                                         cte4 * rd[j] * rd[i] * rd[k]);
@@ -468,13 +481,13 @@ int orc_sigmaec(const double co[4][3], const double xp[3], const double rn[3],
 /* Mesh-wide geometry (centroids, normals, Gauss rule) for one call. */
 struct orc_mesh {
     int n, nbe, np, npg;
-    const double *cx, *cy, *cz;
+    const real *cx, *cy, *cz;
     const int *cone;
-    double *cxm, *cym, *czm, *etas, *gi, *ome;
+    real *cxm, *cym, *czm, *etas, *gi, *ome;
 };
 
 static int mesh_setup(struct orc_mesh *m, int n, int nbe, int np, int npg,
-                      const double *cx, const double *cy, const double *cz,
+                      const real *cx, const real *cy, const real *cz,
                       const int *cone)
 {
     if (n <= 0 || nbe <= 0 || nbe > n || np <= 0 || npg <= 0) return ORC_BADARG;
@@ -486,12 +499,12 @@ static int mesh_setup(struct orc_mesh *m, int n, int nbe, int np, int npg,
     m->cy = cy;
     m->cz = cz;
     m->cone = cone;
-    m->cxm = malloc(sizeof(double) * n);
-    m->cym = malloc(sizeof(double) * n);
-    m->czm = malloc(sizeof(double) * n);
-    m->etas = malloc(sizeof(double) * 3 * n);
-    m->gi = malloc(sizeof(double) * npg);
-    m->ome = malloc(sizeof(double) * npg);
+    m->cxm = malloc(sizeof(real) * n);
+    m->cym = malloc(sizeof(real) * n);
+    m->czm = malloc(sizeof(real) * n);
+    m->etas = malloc(sizeof(real) * 3 * n);
+    m->gi = malloc(sizeof(real) * npg);
+    m->ome = malloc(sizeof(real) * npg);
     orc_gauleg(npg, m->gi, m->ome);
     orc_normvec(n, np, cone, cx, cy, cz, m->etas);
     orc_centroids(n, cone, cx, cy, cz, m->cxm, m->cym, m->czm);
@@ -509,10 +522,10 @@ static void mesh_free(struct orc_mesh *m)
 }
 
 /* Static constants.  Ghmatece.for:55-72 */
-void orc_static_constants(double ge, double rnu, double *consts)
+void orc_static_constants(real ge, real rnu, real *consts)
 {
-    double pi = 4.0 * atan(1.0);
-    double rmu = ge; /* Inputece.for:49 */
+    real pi = 4.0 * atan(1.0);
+    real rmu = ge; /* Inputece.for:49 */
     consts[0] = pi;
     consts[1] = 1.0 / (16.0 * pi * rmu * (1.0 - rnu));
     consts[2] = 3.0 - (4.0 * rnu);
@@ -521,11 +534,11 @@ void orc_static_constants(double ge, double rnu, double *consts)
 }
 
 /* Material: zge = G(1+2i DAM), zcs = sqrt(zge/rho),
- * zcp = zcs sqrt((2-2nu)/(1-2nu)).  Inputecd.for:166-168 */
-void orc_material(double ge, double rnu, double dam, double rho, zc *out3)
+ * zcp = zcs sqrt((2-2nu)/(1-2nu)).  Inputecd.for:58-60 */
+void orc_material(real ge, real rnu, real dam, real rho, zc *out3)
 {
-    zc zge = CMPLX(ge, ge * 2.0 * dam);
-    zc zcs = csqrt(zge / rho);
+    zc zge = ORC_CMPLX(ge, ge * 2.0 * dam);
+    zc zcs = sqrt(zge / rho);
     zc zcp = zcs * sqrt((2.0 - 2.0 * rnu) / (1.0 - 2.0 * rnu));
     out3[0] = zge;
     out3[1] = zcs;
@@ -536,26 +549,26 @@ void orc_material(double ge, double rnu, double dam, double rho, zc *out3)
  * Ghmatece.for:9-256 (off-diagonal loop :92-150, RIGID_BODY :258-282,
  * GEST_SINGULAR :288-323).  consts out: pi, c1, c2, c3, c4.
  * If hest != NULL it receives the full static H (3NBE x 3N, col-major). */
-int orc_ghmatece(int n, int nbe, int np, int npg, const double *cx,
-                 const double *cy, const double *cz, const int *cone,
-                 double ge, double rnu, double *hestdiag, double *gestdiag,
-                 double *consts, double *hest_out)
+int orc_ghmatece(int n, int nbe, int np, int npg, const real *cx,
+                 const real *cy, const real *cz, const int *cone,
+                 real ge, real rnu, real *hestdiag, real *gestdiag,
+                 real *consts, real *hest_out)
 {
     struct orc_mesh m;
     int st = mesh_setup(&m, n, nbe, np, npg, cx, cy, cz, cone);
     if (st) return st;
     orc_static_constants(ge, rnu, consts);
-    double c1 = consts[1], c2 = consts[2], c3 = consts[3], c4 = consts[4];
+    real c1 = consts[1], c2 = consts[2], c3 = consts[3], c4 = consts[4];
     size_t ld = (size_t)3 * nbe;
-    double *hest = calloc(ld * 3 * (size_t)n, sizeof(double));
+    real *hest = calloc(ld * 3 * (size_t)n, sizeof(real));
     int err = 0;
 #pragma omp parallel for schedule(dynamic, 4) reduction(| : err)
     for (int j = 0; j < n; ++j) {
-        double co[4][3], helem[9];
+        real co[4][3], helem[9];
         element_corners(n, cone, cx, cy, cz, j, co);
         for (int i = 0; i < nbe; ++i) {
             if (i == j) continue;
-            double xp[3] = {m.cxm[i], m.cym[i], m.czm[i]};
+            real xp[3] = {m.cxm[i], m.cym[i], m.czm[i]};
             err |= orc_nonsinge(co, xp, &m.etas[3 * j], c3, c4, m.gi, m.ome, npg,
                                 helem);
             for (int b = 0; b < 3; ++b)
@@ -567,7 +580,7 @@ int orc_ghmatece(int n, int nbe, int np, int npg, const double *cx,
         /* RIGID_BODY: HESTdiag(:,:,I) = -sum_{J=1..N} HEST(I,J), J ascending */
 #pragma omp parallel for
         for (int i = 0; i < nbe; ++i) {
-            double acc[9] = {0};
+            real acc[9] = {0};
             for (int j = 0; j < n; ++j)
                 for (int b = 0; b < 3; ++b)
                     for (int a = 0; a < 3; ++a)
@@ -578,13 +591,13 @@ int orc_ghmatece(int n, int nbe, int np, int npg, const double *cx,
         /* GEST_SINGULAR */
 #pragma omp parallel for reduction(| : err)
         for (int j = 0; j < nbe; ++j) {
-            double co[4][3];
+            real co[4][3];
             element_corners(n, cone, cx, cy, cz, j, co);
-            double xm[3] = {m.cxm[j], m.cym[j], m.czm[j]};
+            real xm[3] = {m.cxm[j], m.cym[j], m.czm[j]};
             err |= orc_singge(co, xm, c1, c2, m.gi, m.ome, npg, &gestdiag[9 * j]);
         }
     }
-    if (hest_out && !err) memcpy(hest_out, hest, sizeof(double) * ld * 3 * n);
+    if (hest_out && !err) memcpy(hest_out, hest, sizeof(real) * ld * 3 * n);
     free(hest);
     mesh_free(&m);
     return err ? ORC_DEGENERATE : ORC_OK;
@@ -594,27 +607,27 @@ int orc_ghmatece(int n, int nbe, int np, int npg, const double *cx,
  * Ghmatecd.for:11-239 (assembly :113-166, BOUNDARY_CONDITIONS call :168
  * with the actual arguments swapped relative to :316, ZGEMV :172-181).
  * zg_out (optional) receives ZG after the swap. */
-int orc_ghmatecd(int n, int nbe, int np, int npg, const double *cx,
-                 const double *cy, const double *cz, const int *cone,
-                 const double *hestdiag, const double *gestdiag, const int *kode,
-                 const zc *zdfi, double fr, zc zge, zc zcs, zc zcp,
-                 const double *consts, zc *zh, zc *zfi, zc *zg_out)
+int orc_ghmatecd(int n, int nbe, int np, int npg, const real *cx,
+                 const real *cy, const real *cz, const int *cone,
+                 const real *hestdiag, const real *gestdiag, const int *kode,
+                 const zc *zdfi, real fr, zc zge, zc zcs, zc zcp,
+                 const real *consts, zc *zh, zc *zfi, zc *zg_out)
 {
     struct orc_mesh m;
     int st = mesh_setup(&m, n, nbe, np, npg, cx, cy, cz, cone);
     if (st) return st;
-    double pi = consts[0], c1 = consts[1], c2 = consts[2], c3 = consts[3],
+    real pi = consts[0], c1 = consts[1], c2 = consts[2], c3 = consts[3],
            c4 = consts[4];
     size_t nn = (size_t)3 * nbe;
     zc *zg = zg_out ? zg_out : malloc(sizeof(zc) * nn * nn);
     int err = 0;
 #pragma omp parallel for schedule(dynamic, 2) reduction(| : err)
     for (int j = 0; j < nbe; ++j) {
-        double co[4][3];
+        real co[4][3];
         zc hb[9], gb[9];
         element_corners(n, cone, cx, cy, cz, j, co);
         for (int i = 0; i < nbe; ++i) {
-            double xp[3] = {m.cxm[i], m.cym[i], m.czm[i]};
+            real xp[3] = {m.cxm[i], m.cym[i], m.czm[i]};
             if (i == j) {
                 err |= orc_sing_de(co, xp, &m.etas[3 * j], zge, zcs, zcp, c1, c2,
                                    c3, c4, pi, fr, m.gi, m.ome, npg, hb, gb);
@@ -657,15 +670,58 @@ int orc_ghmatecd(int n, int nbe, int np, int npg, const double *cx,
     return err ? ORC_DEGENERATE : ORC_OK;
 }
 
+/* Selected 3x3 blocks (bi[k], bj[k]) of ZH and ZG before the KODE swap,
+ * each formed exactly as orc_ghmatecd forms it (Ghmatecd.for:113-166):
+ * NONSINGD off the diagonal, SING_DE + HESTdiag/GESTdiag on it.  Blocks come
+ * back column-major (a + 3 b), 9 per block. */
+int orc_ghmatecd_blocks(int n, int nbe, int np, int npg, const real *cx,
+                        const real *cy, const real *cz, const int *cone,
+                        const real *hestdiag, const real *gestdiag, real fr,
+                        zc zge, zc zcs, zc zcp, const real *consts, int nblk,
+                        const int *bi, const int *bj, zc *hblk, zc *gblk)
+{
+    struct orc_mesh m;
+    int st = mesh_setup(&m, n, nbe, np, npg, cx, cy, cz, cone);
+    if (st) return st;
+    real pi = consts[0], c1 = consts[1], c2 = consts[2], c3 = consts[3],
+           c4 = consts[4];
+    int err = 0;
+#pragma omp parallel for schedule(dynamic, 8) reduction(| : err)
+    for (int k = 0; k < nblk; ++k) {
+        const int i = bi[k], j = bj[k];
+        if (i < 0 || i >= nbe || j < 0 || j >= nbe) {
+            err |= ORC_DEGENERATE;
+            continue;
+        }
+        real co[4][3];
+        element_corners(n, cone, cx, cy, cz, j, co);
+        real xp[3] = {m.cxm[i], m.cym[i], m.czm[i]};
+        zc *hb = hblk + 9 * (size_t)k, *gb = gblk + 9 * (size_t)k;
+        if (i == j) {
+            err |= orc_sing_de(co, xp, &m.etas[3 * j], zge, zcs, zcp, c1, c2, c3, c4,
+                               pi, fr, m.gi, m.ome, npg, hb, gb);
+            for (int q = 0; q < 9; ++q) {
+                gb[q] = gb[q] + gestdiag[9 * i + q];
+                hb[q] = hb[q] + hestdiag[9 * i + q];
+            }
+        } else {
+            err |= orc_nonsingd(co, xp, &m.etas[3 * j], zge, zcs, zcp, pi, fr, m.gi,
+                                m.ome, npg, hb, gb);
+        }
+    }
+    mesh_free(&m);
+    return err ? ORC_DEGENERATE : ORC_OK;
+}
+
 /* INTEREC: KODE un-swap, interior displacements ZDSOL(3L) and stresses
  * ZSSOL(9L).  Interec.for:59-67 (un-swap), :86-135 (ZDSOL), :188-283
  * (ZSSOL, entry e = I + 3(J-1) of ZD(K,I,J)).  zfi and zdfi are updated in
  * place.  Points are summed over elements J in ascending order. */
-int orc_interec(int n, int nbe, int np, int npg, const double *cx,
-                const double *cy, const double *cz, const int *cone, zc *zfi,
-                zc *zdfi, const int *kode, int l, const double *cxi,
-                const double *cyi, const double *czi, zc zge, zc zcs, zc zcp,
-                double fr, double rho, double pi, zc *zdsol, zc *zssol)
+int orc_interec(int n, int nbe, int np, int npg, const real *cx,
+                const real *cy, const real *cz, const int *cone, zc *zfi,
+                zc *zdfi, const int *kode, int l, const real *cxi,
+                const real *cyi, const real *czi, zc zge, zc zcs, zc zcp,
+                real fr, real rho, real pi, zc *zdsol, zc *zssol)
 {
     struct orc_mesh m;
     int st = mesh_setup(&m, n, nbe, np, npg, cx, cy, cz, cone);
@@ -681,11 +737,11 @@ int orc_interec(int n, int nbe, int np, int npg, const double *cx,
     int err = 0;
 #pragma omp parallel for schedule(dynamic, 1) reduction(| : err)
     for (int k = 0; k < l; ++k) {
-        double xp[3] = {cxi[k], cyi[k], czi[k]};
+        real xp[3] = {cxi[k], cyi[k], czi[k]};
         zc du[3] = {0, 0, 0}, sg[9] = {0};
         zc hb[9], gb[9], zd[27], zs[27];
         for (int j = 0; j < nbe; ++j) {
-            double co[4][3];
+            real co[4][3];
             element_corners(n, cone, cx, cy, cz, j, co);
             err |= orc_nonsingd(co, xp, &m.etas[3 * j], zge, zcs, zcp, pi, fr,
                                 m.gi, m.ome, npg, hb, gb);
@@ -720,9 +776,9 @@ int orc_zgesv(int nn, zc *a, zc *b)
 {
     for (int k = 0; k < nn; ++k) {
         int p = k;
-        double best = cabs(a[(size_t)k * nn + k]);
+        real best = fabs(a[(size_t)k * nn + k]);
         for (int r = k + 1; r < nn; ++r) {
-            double v = cabs(a[(size_t)k * nn + r]);
+            real v = fabs(a[(size_t)k * nn + r]);
             if (v > best) {
                 best = v;
                 p = r;
@@ -759,8 +815,8 @@ int orc_zgesv(int nn, zc *a, zc *b)
 
 /* The NPG x NPG Gauss points and weights of one element, in the loop order
  * of Nonsingd.for:108-119 (JG outer, IG inner).  Exposed for tests. */
-int orc_quad_rule(const double co[4][3], const double *gi, const double *ome,
-                  int npg, double *xg, double *w)
+int orc_quad_rule(const real co[4][3], const real *gi, const real *ome,
+                  int npg, real *xg, real *w)
 {
     for (int jg = 0; jg < npg; ++jg)
         for (int ig = 0; ig < npg; ++ig) {
@@ -770,3 +826,103 @@ int orc_quad_rule(const double co[4][3], const double *gi, const double *ome,
         }
     return ORC_OK;
 }
+
+#ifdef ORC_LONG_DOUBLE
+/* ------------------------------------------------------------------ */
+/* Entry points of the extended-precision build: the same inputs as    */
+/* the FP64 build (double arrays; complex scalars as (re, im) pairs),  */
+/* every operation in long double, results rounded to double.          */
+/* ------------------------------------------------------------------ */
+static real *ld_copy(const double *a, size_t n)
+{
+    real *r = malloc(sizeof(real) * (n ? n : 1));
+    for (size_t i = 0; i < n; ++i) r[i] = a[i];
+    return r;
+}
+
+static zc ld_z(const double *p) { return ORC_CMPLX(p[0], p[1]); }
+
+int orcld_ghmatece(int n, int nbe, int np, int npg, const double *cx,
+                   const double *cy, const double *cz, const int *cone, double ge,
+                   double rnu, double *hestdiag, double *gestdiag, double *consts)
+{
+    real *x = ld_copy(cx, np), *y = ld_copy(cy, np), *z = ld_copy(cz, np);
+    real *hd = malloc(sizeof(real) * 9 * nbe), *gd = malloc(sizeof(real) * 9 * nbe);
+    real c[5];
+    int st = orc_ghmatece(n, nbe, np, npg, x, y, z, cone, ge, rnu, hd, gd, c, NULL);
+    for (int i = 0; i < 9 * nbe; ++i) {
+        hestdiag[i] = (double)hd[i];
+        gestdiag[i] = (double)gd[i];
+    }
+    for (int i = 0; i < 5; ++i) consts[i] = (double)c[i];
+    free(x); free(y); free(z); free(hd); free(gd);
+    return st;
+}
+
+/* zmat: zge, zcs, zcp as 6 doubles; zdfi, zh, zfi interleaved complex. */
+int orcld_ghmatecd(int n, int nbe, int np, int npg, const double *cx,
+                   const double *cy, const double *cz, const int *cone,
+                   const double *hestdiag, const double *gestdiag, const int *kode,
+                   const double *zdfi, double fr, const double *zmat,
+                   const double *consts, double *zh, double *zfi)
+{
+    size_t nn = (size_t)3 * nbe;
+    real *x = ld_copy(cx, np), *y = ld_copy(cy, np), *z = ld_copy(cz, np);
+    real *hd = ld_copy(hestdiag, 9 * (size_t)nbe), *gd = ld_copy(gestdiag, 9 * (size_t)nbe);
+    real *c = ld_copy(consts, 5);
+    zc *zd = malloc(sizeof(zc) * nn), *h = malloc(sizeof(zc) * nn * nn),
+       *f = malloc(sizeof(zc) * nn);
+    for (size_t i = 0; i < nn; ++i) zd[i] = ld_z(zdfi + 2 * i);
+    int st = orc_ghmatecd(n, nbe, np, npg, x, y, z, cone, hd, gd, kode, zd, fr,
+                          ld_z(zmat), ld_z(zmat + 2), ld_z(zmat + 4), c, h, f, NULL);
+    for (size_t i = 0; i < nn * nn; ++i) {
+        zh[2 * i] = (double)creal(h[i]);
+        zh[2 * i + 1] = (double)cimag(h[i]);
+    }
+    for (size_t i = 0; i < nn; ++i) {
+        zfi[2 * i] = (double)creal(f[i]);
+        zfi[2 * i + 1] = (double)cimag(f[i]);
+    }
+    free(x); free(y); free(z); free(hd); free(gd); free(c); free(zd); free(h); free(f);
+    return st;
+}
+
+int orcld_ghmatecd_blocks(int n, int nbe, int np, int npg, const double *cx,
+                          const double *cy, const double *cz, const int *cone,
+                          const double *hestdiag, const double *gestdiag, double fr,
+                          const double *zmat, const double *consts, int nblk,
+                          const int *bi, const int *bj, double *hblk, double *gblk)
+{
+    real *x = ld_copy(cx, np), *y = ld_copy(cy, np), *z = ld_copy(cz, np);
+    real *hd = ld_copy(hestdiag, 9 * (size_t)nbe), *gd = ld_copy(gestdiag, 9 * (size_t)nbe);
+    real *c = ld_copy(consts, 5);
+    zc *hb = malloc(sizeof(zc) * 9 * (size_t)(nblk ? nblk : 1)),
+       *gb = malloc(sizeof(zc) * 9 * (size_t)(nblk ? nblk : 1));
+    int st = orc_ghmatecd_blocks(n, nbe, np, npg, x, y, z, cone, hd, gd, fr, ld_z(zmat),
+                                 ld_z(zmat + 2), ld_z(zmat + 4), c, nblk, bi, bj, hb, gb);
+    for (size_t i = 0; i < 9 * (size_t)nblk; ++i) {
+        hblk[2 * i] = (double)creal(hb[i]);
+        hblk[2 * i + 1] = (double)cimag(hb[i]);
+        gblk[2 * i] = (double)creal(gb[i]);
+        gblk[2 * i + 1] = (double)cimag(gb[i]);
+    }
+    free(x); free(y); free(z); free(hd); free(gd); free(c); free(hb); free(gb);
+    return st;
+}
+
+/* Point kernels ZU, ZT (Solfund.for:7-63) in extended precision. */
+void orcld_solfund(const double xp[3], const double xg[3], const double rn[3],
+                   const double *zmat, double pi, double fr, double *zu, double *zt)
+{
+    real p[3] = {xp[0], xp[1], xp[2]}, g[3] = {xg[0], xg[1], xg[2]},
+         e[3] = {rn[0], rn[1], rn[2]};
+    zc u[9], t[9];
+    orc_solfund(p, g, e, ld_z(zmat), ld_z(zmat + 2), ld_z(zmat + 4), pi, fr, u, t);
+    for (int k = 0; k < 9; ++k) {
+        zu[2 * k] = (double)creal(u[k]);
+        zu[2 * k + 1] = (double)cimag(u[k]);
+        zt[2 * k] = (double)creal(t[k]);
+        zt[2 * k + 1] = (double)cimag(t[k]);
+    }
+}
+#endif /* ORC_LONG_DOUBLE */

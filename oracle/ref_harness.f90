@@ -80,7 +80,7 @@ contains
     etas = NORMVEC(cone, cx, cy, cz, n, np)
   end subroutine
 
-  ! Material constants exactly as INPUTECD computes them (Inputecd.for:166-168).
+  ! Material constants exactly as INPUTECD computes them (Inputecd.for:58-60).
   subroutine ref_material(ge, rnu, dam, rho, z) bind(C, name='ref_material')
     real(c_double), value :: ge, rnu, dam, rho
     complex(c_double_complex) :: z(3)

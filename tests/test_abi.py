@@ -53,21 +53,35 @@ def test_material_matches_reference_formula(orc):
 
 def test_library_material_matches_reference_formula(orc):
     """bemx_material (host code of the library) against the oracle's
-    restatement of Inputecd.for:166-168."""
+    restatement of Inputecd.for:58-60."""
     for args in ((1.0, 0.25, 0.01, 1.0), (1.7, 0.31, 0.03, 2.2), (0.5, 0.2, 0.005, 1.3)):
         np.testing.assert_allclose(api.material_c(*args), orc.material(*args), rtol=1e-15)
 
 
 def test_request_footprint_and_server_without_device():
-    """The admission footprint is at least ZH (16 nn^2 bytes) and grows with
-    the size; a request loop cannot be created without a device."""
+    """The admission bound covers at least ZH (16 nn^2 bytes) and, on the
+    mixed path, its FP32 copy (8 nn^2), grows with every size, and is 0 for
+    invalid sizes; a request loop cannot be created without a device."""
     import pytest
-    for nbe in (240, 2160):
-        nn = 3 * nbe
-        fp = api.request_footprint(nbe, 1, api.MODE_MIXED)
+    lib = api.lib()
+    for nx, ny in ((16, 15), (48, 45)):
+        mesh = decks.flat_patch(nx, ny)
+        nn = 3 * mesh.nbe
+        fp = api.request_footprint(mesh, 1, api.MODE_MIXED)
         assert fp >= 16 * nn * nn + 8 * nn * nn
-        assert api.request_footprint(nbe, 4, api.MODE_MIXED) > fp
-    assert api.request_footprint(2160) > api.request_footprint(240)
+        assert api.request_footprint(mesh, 4, api.MODE_MIXED) > fp
+        assert api.request_footprint(mesh, 1, api.MODE_FP64) < fp
+        base = (mesh.nbe, mesh.n, mesh.np_, mesh.npg, 1, api.MODE_MIXED)
+        for k in range(5):  # monotone in n, nbe, np, npg, nrhs
+            grown = list(base)
+            grown[k] += 1 if k != 1 else 0
+            if k == 1:
+                grown[1] += 7
+            assert lib.bemx_request_footprint(*grown) >= lib.bemx_request_footprint(*base)
+        assert lib.bemx_request_footprint(0, 5, 9, 8, 1, api.MODE_MIXED) == 0.0
+        assert lib.bemx_request_footprint(mesh.nbe, mesh.n, mesh.np_, mesh.npg, 1, 7) == 0.0
+    assert (api.request_footprint(decks.flat_patch(48, 45)) >
+            api.request_footprint(decks.flat_patch(16, 15)))
     with pytest.raises(api.BemxError):
         api.Server(99, 2)
 

@@ -108,7 +108,7 @@ def test_mixed_bem_solve_matches_fp64_oracle(ctx, orc, case):
         mesh, dyn = decks.config_problem("c1")
     else:
         mesh, dyn = decks.flat_patch_problem(24, 22, n_interior=0, freqs=(7.5,), dam=0.03)
-    ref = orc.solve_problem(mesh, dyn)
+    ref = orc.solve_problem(mesh, dyn, accurate=True)
     ctx.send_shared_data(mesh)
     ctx.ghmatece(mesh.ge, mesh.rnu)
     zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)

@@ -78,7 +78,7 @@ def test_mixed_request_matches_fp64_oracle(orc):
         x, info = srv.wait(rid)
     finally:
         srv.close()
-    ref = orc.solve_problem(mesh, dyn)["x"]
+    ref = orc.solve_problem(mesh, dyn, accurate=True)["x"]
     assert np.max(np.abs(x - ref)) / np.max(np.abs(ref)) <= TOL
     assert info["fell_back"] == 0.0
 
@@ -87,7 +87,7 @@ def test_admission_holds_requests_to_the_budget():
     allr = _requests()
     # three requests on the 12 x 11 mesh (1 or 2 load cases, other frequencies)
     reqs = [allr[1], allr[3], allr[1][:4] + (0.7,)]
-    fps = [api.request_footprint(r[0].nbe, r[2].shape[1], api.MODE_MIXED) for r in reqs]
+    fps = [api.request_footprint(r[0], r[2].shape[1], api.MODE_MIXED) for r in reqs]
     assert 2 * min(fps) > 1.5 * max(fps)
     srv = api.Server(0, nslots=3, hbm_budget=1.5 * max(fps), mode=api.MODE_MIXED)
     try:
@@ -99,7 +99,7 @@ def test_admission_holds_requests_to_the_budget():
         for (s0, e0), (s1, e1) in zip(spans, spans[1:]):
             assert s1 >= e0, (spans,)
         mesh, kode, zdfi, p, f = reqs[0]
-        small = api.Server(0, nslots=1, hbm_budget=0.5 * api.request_footprint(mesh.nbe, 1),
+        small = api.Server(0, nslots=1, hbm_budget=0.5 * api.request_footprint(mesh, 1),
                            mode=api.MODE_MIXED)
         try:
             with pytest.raises(api.BemxError):

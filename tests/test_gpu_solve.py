@@ -93,7 +93,7 @@ def test_bem_chain_fp64(ctx, orc, case):
         mesh = decks.read_e_deck(os.path.join(GOLDEN, "ESOLO240E_-5+5.DAT"))
         dyn = decks.read_d_deck(os.path.join(GOLDEN, "ESOLO240D_-5+5.DAT"), mesh.nbe)
     res = api.solve_problem(ctx, mesh, dyn)
-    ref = orc.solve_problem(mesh, dyn)
+    ref = orc.solve_problem(mesh, dyn, accurate=True)
     for key in ("x", "zdsol", "zssol"):
         rel = np.max(np.abs(res[key] - ref[key])) / np.max(np.abs(ref[key]))
         assert rel < 1e-12, (key, rel)
@@ -106,7 +106,7 @@ def test_device_resident_solve_matches(ctx, orc):
     zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
     ctx.send_dyn_data(dyn.bc.astype(np.complex128), dyn.kode)
     x, stats = ctx.solve(zmat, 2.0, api.MODE_FP64)
-    ref = orc.solve_problem(mesh, dyn)
+    ref = orc.solve_problem(mesh, dyn, accurate=True)
     rel = np.max(np.abs(x - ref["x"])) / np.max(np.abs(ref["x"]))
     assert rel < 1e-12, rel
     assert stats[3] > 0.0

@@ -359,3 +359,108 @@ def test_kode_swap_round_trip(orc):
     zfi2, zdfi2, zdsol, _ = orc.interec(mesh, dyn2, x, zdfi, o["fr"], o["zmat"], consts[0])
     np.testing.assert_allclose(zfi2[sel], u[sel] / zge, atol=1e-12)
     np.testing.assert_allclose(zdfi2[sel], t[sel] * zge, atol=1e-12)
+
+
+def test_sampled_blocks_equal_the_full_ghmatecd(orc):
+    """orc_ghmatecd_blocks forms each block with the arithmetic of the full
+    GHMATECD restatement: the sampled blocks, diagonal ones included, are
+    bit-identical to the full matrices (before the KODE swap)."""
+    mesh, dyn = decks.flat_patch_problem(9, 8, n_interior=0, freqs=(0.7,), dam=0.02)
+    hd, gd, consts = orc.ghmatece(mesh)
+    zmat = orc.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
+    zh, zfi, zg = orc.ghmatecd(mesh, hd, gd, consts, dyn.kode, dyn.bc.astype(np.complex128),
+                               0.7, zmat, return_zg=True)
+    rng = np.random.default_rng(3)
+    bi = np.concatenate([np.arange(mesh.nbe), rng.integers(0, mesh.nbe, 60)])
+    bj = np.concatenate([np.arange(mesh.nbe), rng.integers(0, mesh.nbe, 60)])
+    hb, gb = orc.ghmatecd_blocks(mesh, hd, gd, consts, 0.7, zmat, bi, bj)
+    for k, (i, j) in enumerate(zip(bi, bj)):
+        assert np.array_equal(hb[k], zh[3 * i:3 * i + 3, 3 * j:3 * j + 3])
+        assert np.array_equal(gb[k], zg[3 * i:3 * i + 3, 3 * j:3 * j + 3])
+
+
+def _solfund_mp(xp, xg, rn, zge, zcs, zcp, pi, fr):
+    """Solfund.for:29-60 in 40-digit arithmetic (mpmath), for pinning the
+    extended-precision oracle where the FP64 formula loses digits."""
+    import mpmath as mp
+    mp.mp.dps = 40
+    x = [mp.mpf(float(v)) for v in xg]
+    p = [mp.mpf(float(v)) for v in xp]
+    n = [mp.mpf(float(v)) for v in rn]
+    zge, zcs, zcp = (mp.mpc(complex(v).real, complex(v).imag) for v in (zge, zcs, zcp))
+    r = [x[i] - p[i] for i in range(3)]
+    R = mp.sqrt(sum(v * v for v in r))
+    rd = [v / R for v in r]
+    drn = sum(rd[i] * n[i] for i in range(3))
+    zwi = mp.mpc(0, fr)
+    c0 = 1 / (4 * mp.pi * zge)
+    c1, c2 = (zcp / zcs) ** 2, (zcs / zcp) ** 2
+    zp, zs = -zwi / zcp * R, -zwi / zcs * R
+    ep, es = mp.exp(zp), mp.exp(zs)
+    fhi = (1 + 1 / zs ** 2 - 1 / zs) * es / R - c2 * (1 / zp ** 2 - 1 / zp) * ep / R
+    cap = (1 + 3 / zs ** 2 - 3 / zs) * es / R - c2 * (1 + 3 / zp ** 2 - 3 / zp) * ep / R
+    fhid = (-2 + zs + 3 / zs - 3 / zs ** 2) * es / R ** 2 - c2 * (-1 + 3 / zp - 3 / zp ** 2) * ep / R ** 2
+    capd = (zs - 4 + 9 / zs - 9 / zs ** 2) * es / R ** 2 - c2 * (zp - 4 + 9 / zp - 9 / zp ** 2) * ep / R ** 2
+    aa = fhid - cap / R
+    bb = 4 * cap / R - 2 * capd
+    cc = (c1 - 2) * (aa + bb / 2 - 3 * cap / R) - 2 * cap / R
+    zu = np.zeros((3, 3), complex)
+    zt = np.zeros((3, 3), complex)
+    for i in range(3):
+        for j in range(3):
+            d = 1 if i == j else 0
+            zu[i, j] = complex(c0 * (fhi * d - cap * rd[j] * rd[i]))
+            zt[i, j] = complex((aa * (drn * d + rd[j] * n[i]) + rd[i] * rd[j] * drn * bb +
+                                rd[i] * n[j] * cc) / (4 * mp.pi))
+    return zu, zt
+
+
+@pytest.mark.parametrize("kr", [1e-3, 1e-2, 0.1, 0.5, 2.0, 10.0])
+def test_long_double_oracle_against_mpmath(orc, kr):
+    """Both builds of the oracle lose digits like eps / (kR)^2 in Solfund
+    (its 1/z^2 terms cancel between the waves), the x87 build with an eps
+    2048 times smaller: it stays within 5e-18 / (kR)^2 relative of a
+    40-digit evaluation (7e-13 at kR = 2.7e-3, the smallest kR of the
+    benchmark meshes: the innermost Gauss point of a diagonal element at
+    omega = 0.1), the FP64 build within 2e-14 / (kR)^2."""
+    ld = oracle_mod.OracleLD()
+    zmat = orc.material(1.3, 0.3, 0.02, 1.1)
+    ks = abs(1.0 / zmat[1])  # |k_s| per unit frequency
+    xp = np.array([0.3, -0.1, 0.05])
+    rn = np.array([0.36, 0.48, 0.8])
+    for fr in (0.1, 2.0):
+        R = kr / (ks * fr)
+        xg = xp + R * np.array([0.6, -0.64, 0.48])
+        u_mp, t_mp = _solfund_mp(xp, xg, rn, *zmat, np.pi, fr)
+        u_ld, t_ld = ld.solfund(xp, xg, rn, *zmat, np.pi, fr)
+        u_64, t_64 = orc.solfund(xp, xg, rn, *zmat, np.pi, fr)
+        for a, b in ((u_ld, u_mp), (t_ld, t_mp)):
+            err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+            assert err <= max(2e-15, 5e-18 / kr ** 2), (kr, fr, err)
+        err64 = np.max(np.abs(t_64 - t_mp)) / np.max(np.abs(t_mp))
+        assert err64 <= max(2e-15, 2e-14 / kr ** 2), (kr, err64)
+
+
+def test_long_double_oracle_matches_fp64_at_moderate_kr(orc):
+    """Where the FP64 formulas are accurate (kR >~ 1 over most of the mesh),
+    the extended-precision build agrees with the FP64 one to 1e-12 per
+    entry: it is the same restatement."""
+    ld = oracle_mod.OracleLD()
+    mesh, dyn = decks.flat_patch_problem(7, 6, n_interior=0, freqs=(6.0,), dam=0.02)
+    hd, gd, consts = orc.ghmatece(mesh)
+    lhd, lgd, lconsts = ld.ghmatece(mesh)
+    assert close_blocks(lhd.reshape(-1, 9), hd.reshape(-1, 9), 1e-13)[0]
+    assert close_blocks(lgd.reshape(-1, 9), gd.reshape(-1, 9), 1e-13)[0]
+    np.testing.assert_allclose(lconsts, consts, rtol=1e-15)
+    zmat = orc.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
+    zdfi = dyn.bc.astype(np.complex128)
+    zh, zfi = orc.ghmatecd(mesh, hd, gd, consts, dyn.kode, zdfi, 6.0, zmat)
+    lzh, lzfi = ld.ghmatecd(mesh, hd, gd, consts, dyn.kode, zdfi, 6.0, zmat)
+    ok, worst = close_blocks(zh, lzh, 1e-12)
+    assert ok, worst
+    rng = np.random.default_rng(2)
+    bi = np.concatenate([np.arange(mesh.nbe), rng.integers(0, mesh.nbe, 40)])
+    bj = np.concatenate([np.arange(mesh.nbe), rng.integers(0, mesh.nbe, 40)])
+    hb, gb = ld.ghmatecd_blocks(mesh, hd, gd, consts, 6.0, zmat, bi, bj)
+    for k, (i, j) in enumerate(zip(bi, bj)):
+        assert np.array_equal(hb[k], lzh[3 * i:3 * i + 3, 3 * j:3 * j + 3])
