@@ -307,9 +307,16 @@ def test_serving_stream_under_a_tight_budget(stream, stream_alone):
     it: idle slots give their workspace back, the charge never exceeds the
     budget, no slot holds more than it is charged, and the results are
     unchanged."""
-    meshes, _, _, items = stream
+    meshes, _, reqs, items = stream
     budget = 1.2 * api.request_footprint(meshes[-1], 4, api.MODE_MIXED)
-    sel = list(range(16))
+    # a 48 x 45 and a 32 x 30 request, then two 64 x 60 ones: whichever of
+    # the first two finishes first, a 64 x 60 request cannot start beside
+    # the other slots' charges until an idle slot gives its workspace back
+    first = {}
+    for k, r in enumerate(reqs):
+        first.setdefault(r["cls"], []).append(k)
+    sel = [first[3][0], first[2][0], first[4][0], first[4][1]]
+    sel += [k for k in range(N_STREAM) if k not in sel][:12]
     srv = api.Server(0, nslots=3, hbm_budget=budget, mode=api.MODE_MIXED)
     try:
         outs = [srv.wait(i) for i in [srv.submit(*items[k]) for k in sel]]
