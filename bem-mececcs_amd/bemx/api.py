@@ -31,7 +31,7 @@ EXPORTS = [
     "bemx_create", "bemx_destroy", "bemx_last_error", "bemx_gauleg", "bemx_send_shared_data",
     "bemx_deallocate_shared_data", "bemx_ghmatece", "bemx_send_dyn_data", "bemx_ghmatecd",
     "bemx_linsolve", "bemx_interec", "bemx_solve", "bemx_device_buffers", "bemx_last_stats",
-    "bemx_send_load_cases", "bemx_material", "bemx_server_create", "bemx_server_submit",
+    "bemx_send_load_cases", "bemx_send_load_cases_kode", "bemx_material", "bemx_server_create", "bemx_server_submit",
     "bemx_server_wait", "bemx_server_destroy", "bemx_request_footprint", "bemx_server_stats",
 ]
 
@@ -59,6 +59,7 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
     lib.bemx_ghmatece.argtypes = [vp, C.c_double, C.c_double] + [_dp] * 8
     lib.bemx_send_dyn_data.argtypes = [vp, _dp, _ip]
     lib.bemx_send_load_cases.argtypes = [vp, C.c_int, _dp, _ip]
+    lib.bemx_send_load_cases_kode.argtypes = [vp, C.c_int, _dp, _ip]
     lib.bemx_ghmatecd.argtypes = [vp, _dp, _dp, _dp] + [C.c_double] * 5 + [_dp] * 4
     lib.bemx_linsolve.argtypes = [vp, C.c_int, _dp, _dp, C.c_int, C.POINTER(C.c_longlong)]
     lib.bemx_interec.argtypes = [vp, _dp, _dp, _ip, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
@@ -69,7 +70,7 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
     lib.bemx_material.argtypes = [C.c_double] * 4 + [_dp] * 3
     lib.bemx_server_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(vp)]
     lib.bemx_server_submit.argtypes = ([vp, _dp, _dp, _dp, _ip] + [C.c_int] * 4 +
-                                       [C.c_double] * 5 + [C.c_int, _dp, _ip,
+                                       [C.c_double] * 5 + [C.c_int, _dp, _ip, C.c_int,
                                                            C.POINTER(C.c_longlong)])
     lib.bemx_server_wait.argtypes = [vp, C.c_longlong, _dp, _dp, C.c_char_p, C.c_int]
     lib.bemx_server_destroy.argtypes = [vp]
@@ -212,13 +213,20 @@ class Context:
         self.nrhs = 1
 
     def send_load_cases(self, zdfi, kode):
-        """zdfi: (3NBE, nrhs) complex, one load case per column (1..4)."""
-        zdfi = np.asfortranarray(np.asarray(zdfi, dtype=np.complex128).reshape(len(kode), -1))
-        kode = np.ascontiguousarray(kode, dtype=np.int32)
+        """zdfi: (3NBE, nrhs) complex, one load case per column (1..4); kode:
+        (3NBE,) for every case, or (3NBE, nrhs) with a KODE column per case
+        (bemx_send_load_cases_kode)."""
+        kode = np.asarray(kode, dtype=np.int32)
+        nn = kode.shape[0]
+        zdfi = np.asfortranarray(np.asarray(zdfi, dtype=np.complex128).reshape(nn, -1))
         nrhs = zdfi.shape[1]
-        code = self._lib.bemx_send_load_cases(self.h, nrhs,
-                                              _p(zdfi.reshape(-1, order="F").view(np.float64)),
-                                              _pi(kode))
+        zp = _p(zdfi.reshape(-1, order="F").view(np.float64))
+        if kode.ndim == 2:
+            kf = np.asfortranarray(kode).reshape(-1, order="F")
+            code = self._lib.bemx_send_load_cases_kode(self.h, nrhs, zp, _pi(kf))
+        else:
+            code = self._lib.bemx_send_load_cases(self.h, nrhs, zp,
+                                                  _pi(np.ascontiguousarray(kode)))
         self._check(code, "send_load_cases")
         self.nrhs = nrhs
 
@@ -335,10 +343,13 @@ class Server:
             pass
 
     def submit(self, mesh, kode, zdfi, zmat_params, fr: float) -> int:
-        """zdfi: (3NBE,) or (3NBE, nrhs) complex load cases; zmat_params =
-        (ge, rnu, dam, rho).  Returns the request id."""
-        zdfi = np.asfortranarray(np.asarray(zdfi, dtype=np.complex128).reshape(len(kode), -1))
-        kode = np.ascontiguousarray(kode, dtype=np.int32)
+        """zdfi: (3NBE,) or (3NBE, nrhs) complex load cases; kode: (3NBE,) for
+        every case or (3NBE, nrhs), a column per case; zmat_params = (ge, rnu,
+        dam, rho).  Returns the request id."""
+        kode = np.asarray(kode, dtype=np.int32)
+        zdfi = np.asfortranarray(np.asarray(zdfi, dtype=np.complex128).reshape(kode.shape[0], -1))
+        kode_cols = kode.shape[1] if kode.ndim == 2 else 1
+        kode = np.ascontiguousarray(np.asfortranarray(kode).reshape(-1, order="F"))
         cx, cy, cz = (np.ascontiguousarray(v, dtype=np.float64) for v in (mesh.cx, mesh.cy, mesh.cz))
         cone = mesh.cone_f()  # column-major CONE(N,4) memory
         ge, rnu, dam, rho = (float(v) for v in zmat_params)
@@ -346,7 +357,7 @@ class Server:
         code = self._lib.bemx_server_submit(
             self.h, _p(cx), _p(cy), _p(cz), _pi(cone), len(cx), mesh.n, mesh.nbe, mesh.npg,
             ge, rnu, dam, rho, float(fr), zdfi.shape[1],
-            _p(zdfi.reshape(-1, order="F").view(np.float64)), _pi(kode), C.byref(rid))
+            _p(zdfi.reshape(-1, order="F").view(np.float64)), _pi(kode), kode_cols, C.byref(rid))
         if code:
             raise BemxError(code, "bemx_server_submit")
         self._sizes[rid.value] = zdfi.shape

@@ -82,6 +82,12 @@ struct bemx_ctx {
     bool dyn_ok = false;
     DevBuf kode, zdfi, colflag, flist, rhs_part, rhs_diag, zfi, zh, etab;
     int nrhs = 1;              // load cases (columns of zdfi / zfi)
+    // load cases with KODE columns of their own (bemx_send_load_cases_kode):
+    // host copies, grouped by identical KODE column; empty when every case
+    // shares one KODE (the device buffers then hold the cases directly)
+    std::vector<double> case_zdfi;            // 6 NBE doubles per case
+    std::vector<int> case_kode;               // 3 NBE per case
+    std::vector<std::vector<int>> case_groups;  // case indices per KODE set
     bool zh_valid = false;     // zh holds the current assembled matrix
     const void* zh_host = nullptr;  // host array the last GHMATECD copied ZH to
     // interior
@@ -470,12 +476,7 @@ int bemx_ghmatece(bemx_ctx* c, double ge, double rnu, double* hestdiag, double* 
     return BEMX_OK;
 }
 
-int bemx_send_load_cases(bemx_ctx* c, int nrhs, const double* zdfi, const int* kode) {
-    if (!c) return BEMX_ERR_ARG;
-    if (!c->mesh_ok) return fail(c, BEMX_ERR_STATE, "%sdynamic data before the mesh", "", 0);
-    if (!zdfi || !kode) return fail(c, BEMX_ERR_ARG, "%snull ZDFI/KODE", "", 0);
-    if (nrhs < 1 || nrhs > 4) return fail(c, BEMX_ERR_ARG, "%s1..4 load cases, got %ld", "", nrhs);
-    HIPCHK(c, hipSetDevice(c->device));
+static int upload_cases(bemx_ctx* c, int nrhs, const double* zdfi, const int* kode) {
     size_t nn = 3 * (size_t)c->nbe;
     c->nrhs = nrhs;
     HIPCHK(c, c->zdfi.ensure(sizeof(cd) * nn * nrhs));
@@ -484,6 +485,45 @@ int bemx_send_load_cases(bemx_ctx* c, int nrhs, const double* zdfi, const int* k
                              c->st));
     HIPCHK(c, hipMemcpyAsync(c->kode.p, kode, sizeof(int) * nn, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    return BEMX_OK;
+}
+
+int bemx_send_load_cases(bemx_ctx* c, int nrhs, const double* zdfi, const int* kode) {
+    if (!c) return BEMX_ERR_ARG;
+    if (!c->mesh_ok) return fail(c, BEMX_ERR_STATE, "%sdynamic data before the mesh", "", 0);
+    if (!zdfi || !kode) return fail(c, BEMX_ERR_ARG, "%snull ZDFI/KODE", "", 0);
+    if (nrhs < 1 || nrhs > 4) return fail(c, BEMX_ERR_ARG, "%s1..4 load cases, got %ld", "", nrhs);
+    HIPCHK(c, hipSetDevice(c->device));
+    c->case_groups.clear();
+    int s = upload_cases(c, nrhs, zdfi, kode);
+    if (s) return s;
+    c->dyn_ok = true;
+    return BEMX_OK;
+}
+
+int bemx_send_load_cases_kode(bemx_ctx* c, int nrhs, const double* zdfi, const int* kode) {
+    if (!c) return BEMX_ERR_ARG;
+    if (!c->mesh_ok) return fail(c, BEMX_ERR_STATE, "%sdynamic data before the mesh", "", 0);
+    if (!zdfi || !kode) return fail(c, BEMX_ERR_ARG, "%snull ZDFI/KODE", "", 0);
+    if (nrhs < 1 || nrhs > 4) return fail(c, BEMX_ERR_ARG, "%s1..4 load cases, got %ld", "", nrhs);
+    const size_t nn = 3 * (size_t)c->nbe;
+    std::vector<std::vector<int>> groups;
+    for (int j = 0; j < nrhs; ++j) {
+        bool found = false;
+        for (auto& g : groups)
+            if (!memcmp(kode + nn * g[0], kode + nn * j, sizeof(int) * nn)) {
+                g.push_back(j);
+                found = true;
+                break;
+            }
+        if (!found) groups.push_back({j});
+    }
+    if (groups.size() == 1) return bemx_send_load_cases(c, nrhs, zdfi, kode);
+    HIPCHK(c, hipSetDevice(c->device));
+    c->case_zdfi.assign(zdfi, zdfi + 2 * nn * nrhs);
+    c->case_kode.assign(kode, kode + nn * nrhs);
+    c->case_groups = groups;
+    c->nrhs = nrhs;
     c->dyn_ok = true;
     return BEMX_OK;
 }
@@ -528,6 +568,10 @@ int bemx_ghmatecd(bemx_ctx* c, const double* zge, const double* zcs, const doubl
         return fail(c, BEMX_ERR_STATE, "%sGHMATECD needs the mesh and the dynamic data", "", 0);
     if (!zge || !zcs || !zcp || !(fr != 0.0))
         return fail(c, BEMX_ERR_ARG, "%sGHMATECD needs the material and a nonzero frequency", "", 0);
+    if (!c->case_groups.empty())
+        return fail(c, BEMX_ERR_ARG,
+                    "%sGHMATECD assembles for one KODE: the load cases have %ld KODE sets", "",
+                    (long)c->case_groups.size());
     HIPCHK(c, hipSetDevice(c->device));
     const int nbe = c->nbe;
     const size_t nn = 3 * (size_t)nbe;
@@ -649,6 +693,47 @@ int bemx_interec(bemx_ctx* c, double* zfi, double* zdfi, const int* kode, int l,
     return BEMX_OK;
 }
 
+}  // extern "C"
+
+// Assembly + factorisation (+ refinement) + solve of the load cases now in
+// the device buffers; the solution columns go to out (host, may be NULL),
+// statistics are accumulated into acc (GMRES iterations summed, residual
+// and fallback as worst, device times summed).
+static int solve_pass(bemx_ctx* c, const DynConst& K, int mode, double* out, double acc[NLAST]) {
+    const size_t nn = 3 * (size_t)c->nbe;
+    HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    int s = assemble_dev(c, K);
+    if (s) return s;
+    long long inf = 0;
+    double st[4] = {0, 0, 0, 0};
+    s = bemx::lu_solve(c->lu, c->st, (int)nn, c->zh.as<cd>(), nn, c->zfi.as<cd>(), c->nrhs, mode,
+                       &inf, st, c->err);
+    c->zh_valid = false;
+    if (s) return s;
+    HIPCHK(c, hipEventRecord(c->ev1, c->st));
+    if (out)
+        HIPCHK(c, hipMemcpyAsync(out, c->zfi.p, sizeof(cd) * nn * c->nrhs, hipMemcpyDeviceToHost,
+                                 c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    float ms = 0.f, ms_od = 0.f, ms_as = 0.f, ms_f = 0.f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    hipEventElapsedTime(&ms_od, c->ev2, c->ev3);
+    hipEventElapsedTime(&ms_as, c->ev0, c->ev4);
+    if (c->lu.factor_timed) hipEventElapsedTime(&ms_f, c->lu.ev_f0, c->lu.ev_f1);
+    acc[0] += st[0];
+    acc[1] = std::max(acc[1], st[1]);
+    acc[2] = std::max(acc[2], st[2]);
+    acc[3] += ms;
+    acc[4] += ms_od;
+    acc[5] += ms_as;
+    acc[6] += ms - ms_as;
+    acc[8] += ms_f;
+    acc[9] += ms - ms_as - ms_f;
+    return BEMX_OK;
+}
+
+extern "C" {
+
 int bemx_solve(bemx_ctx* c, const double* zge, const double* zcs, const double* zcp, double fr,
                int mode, double* zfi_out, double* stats) {
     if (!c) return BEMX_ERR_ARG;
@@ -661,34 +746,41 @@ int bemx_solve(bemx_ctx* c, const double* zge, const double* zcs, const double* 
     HIPCHK(c, hipSetDevice(c->device));
     const size_t nn = 3 * (size_t)c->nbe;
     DynConst K = make_dyn_const(c, ld_c(zge), ld_c(zcs), ld_c(zcp), fr, 1.0);
-    HIPCHK(c, hipEventRecord(c->ev0, c->st));
-    int s = assemble_dev(c, K);
+    double acc[NLAST] = {0};
+    int s = BEMX_OK;
+    if (c->case_groups.empty()) {
+        s = solve_pass(c, K, mode, zfi_out, acc);
+    } else {
+        // one KODE column swap, assembly and factorisation per KODE set
+        // (Ghmatecd.for:316-333 applies one KODE to the whole matrix)
+        const int ncase = c->nrhs;
+        std::vector<double> z, x;
+        for (const std::vector<int>& g : c->case_groups) {
+            const int ng = (int)g.size();
+            z.resize(2 * nn * ng);
+            for (int q = 0; q < ng; ++q)
+                memcpy(&z[2 * nn * q], &c->case_zdfi[2 * nn * g[q]], sizeof(double) * 2 * nn);
+            s = upload_cases(c, ng, z.data(), &c->case_kode[nn * g[0]]);
+            if (!s) {
+                x.assign(2 * nn * ng, 0.0);
+                s = solve_pass(c, K, mode, zfi_out ? x.data() : nullptr, acc);
+            }
+            if (s) break;
+            if (zfi_out)
+                for (int q = 0; q < ng; ++q)
+                    memcpy(zfi_out + 2 * nn * g[q], &x[2 * nn * q], sizeof(double) * 2 * nn);
+        }
+        c->nrhs = ncase;
+    }
     if (s) return s;
-    long long inf = 0;
-    double st[4] = {0, 0, 0, 0};
-    s = bemx::lu_solve(c->lu, c->st, (int)nn, c->zh.as<cd>(), nn, c->zfi.as<cd>(), c->nrhs, mode,
-                       &inf, st, c->err);
-    c->zh_valid = false;
-    if (s) return s;
-    HIPCHK(c, hipEventRecord(c->ev1, c->st));
-    if (zfi_out)
-        HIPCHK(c, hipMemcpyAsync(zfi_out, c->zfi.p, sizeof(cd) * nn * c->nrhs,
-                                 hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    float ms = 0.f, ms_od = 0.f, ms_as = 0.f, ms_f = 0.f;
-    hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    hipEventElapsedTime(&ms_od, c->ev2, c->ev3);
-    hipEventElapsedTime(&ms_as, c->ev0, c->ev4);
-    if (c->lu.factor_timed) hipEventElapsedTime(&ms_f, c->lu.ev_f0, c->lu.ev_f1);
-    double last[NLAST] = {st[0], st[1], st[2], ms, ms_od, ms_as, ms - ms_as,
-                          (double)c->lu.captures, ms_f, ms - ms_as - ms_f,
-                          (double)c->lu.replays};
-    memcpy(c->last, last, sizeof last);
+    acc[7] = (double)c->lu.captures;
+    acc[10] = (double)c->lu.replays;
+    memcpy(c->last, acc, sizeof acc);
     if (stats) {
-        stats[0] = st[0];
-        stats[1] = st[1];
-        stats[2] = st[2];
-        stats[3] = ms;
+        stats[0] = acc[0];
+        stats[1] = acc[1];
+        stats[2] = acc[2];
+        stats[3] = acc[3];
     }
     return BEMX_OK;
 }

@@ -73,7 +73,8 @@ struct Request {
     double ge = 0, rnu = 0, dam = 0, rho = 0, fr = 0;
     int nrhs = 1;
     std::vector<double> zdfi;  // interleaved complex, 3NBE x nrhs
-    std::vector<int> kode;
+    std::vector<int> kode;     // 3NBE x kode_cols
+    int kode_cols = 1;         // 1: one KODE for every case; nrhs: one per case
     uint64_t mesh_key = 0;
     size_t footprint = 0;
     clk::time_point t_submit, t_start, t_done;
@@ -154,7 +155,10 @@ void run_request(bemx_server* s, bemx_ctx* c, uint64_t* slot_key, Request* r) {
     }
     double zge[2], zcs[2], zcp[2];
     if (!st) st = bemx_material(r->ge, r->rnu, r->dam, r->rho, zge, zcs, zcp);
-    if (!st) st = bemx_send_load_cases(c, r->nrhs, r->zdfi.data(), r->kode.data());
+    if (!st)
+        st = r->kode_cols == 1
+                 ? bemx_send_load_cases(c, r->nrhs, r->zdfi.data(), r->kode.data())
+                 : bemx_send_load_cases_kode(c, r->nrhs, r->zdfi.data(), r->kode.data());
     if (!st) {
         r->x.assign((size_t)6 * r->nbe * r->nrhs, 0.0);
         st = bemx_solve(c, zge, zcs, zcp, r->fr, s->mode, r->x.data(), r->stats);
@@ -291,10 +295,10 @@ int bemx_server_create(int device, int nslots, double hbm_budget_bytes, int mode
 int bemx_server_submit(bemx_server* s, const double* cx, const double* cy, const double* cz,
                        const int* cone, int np, int n, int nbe, int npg, double ge, double rnu,
                        double dam, double rho, double fr, int nrhs, const double* zdfi,
-                       const int* kode, long long* id) {
+                       const int* kode, int kode_cols, long long* id) {
     if (!s || !cx || !cy || !cz || !cone || !zdfi || !kode || !id) return BEMX_ERR_ARG;
     if (np <= 0 || n <= 0 || nbe <= 0 || nbe > n || npg <= 0 || nrhs < 1 || nrhs > 4 ||
-        !(fr != 0.0))
+        !(fr != 0.0) || (kode_cols != 1 && kode_cols != nrhs))
         return BEMX_ERR_ARG;
     auto r = std::make_unique<Request>();
     r->cx.assign(cx, cx + np);
@@ -312,7 +316,8 @@ int bemx_server_submit(bemx_server* s, const double* cx, const double* cy, const
     r->fr = fr;
     r->nrhs = nrhs;
     r->zdfi.assign(zdfi, zdfi + (size_t)6 * nbe * nrhs);
-    r->kode.assign(kode, kode + (size_t)3 * nbe);
+    r->kode.assign(kode, kode + (size_t)3 * nbe * kode_cols);
+    r->kode_cols = kode_cols;
     uint64_t h = 1469598103934665603ull;
     int dims[4] = {np, n, nbe, npg};
     double mat[2] = {ge, rnu};

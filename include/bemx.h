@@ -81,6 +81,15 @@ int bemx_send_dyn_data(bemx_ctx *ctx, const double *zdfi, const int *kode);
  * ZFI(3NBE, nrhs) and bemx_solve solves all columns with one factorisation. */
 int bemx_send_load_cases(bemx_ctx *ctx, int nrhs, const double *zdfi, const int *kode);
 
+/* Load cases with a KODE column each: ZDFI(3NBE, nrhs) and KODE(3NBE, nrhs),
+ * column-major, nrhs 1..4.  A KODE set is one column swap of the matrix
+ * (BOUNDARY_CONDITIONS, Ghmatecd.for:316-333, called at :168 with one KODE
+ * per run), so bemx_solve groups the cases by identical KODE columns and
+ * assembles, factors and solves once per group, returning every case in its
+ * own column.  With a single KODE set this is bemx_send_load_cases;
+ * bemx_ghmatecd refuses several sets (BEMX_ERR_ARG). */
+int bemx_send_load_cases_kode(bemx_ctx *ctx, int nrhs, const double *zdfi, const int *kode);
+
 /* Dynamic assembly at one frequency.  Replaces GHMATECD (Ghmatecd.for:11-239)
  * = cuda_ghmatecd_ (kernels/Ghmatecd_cu.cu:305-483): ZH (3NBE x 3NBE) after
  * the KODE column swap, and ZFI = ZG ZDFI.  zge/zcs/zcp point to complex
@@ -164,7 +173,9 @@ int bemx_server_create(int device, int nslots, double hbm_budget_bytes, int mode
 int bemx_server_submit(bemx_server *srv, const double *cx, const double *cy, const double *cz,
                        const int *cone, int np, int n, int nbe, int npg, double ge, double rnu,
                        double dam, double rho, double fr, int nrhs, const double *zdfi,
-                       const int *kode, long long *id);
+                       const int *kode, int kode_cols, long long *id);
+/* kode_cols: 1 when KODE(3NBE) holds for every load case, nrhs for
+ * KODE(3NBE, nrhs) with a column per case (as bemx_send_load_cases_kode). */
 /* Blocks until request id is done; returns its status.  x (may be NULL):
  * 3NBE x nrhs complex solution.  info (may be NULL, 10 doubles): latency
  * ms (submit to done), queue ms, service ms, GMRES iterations, worst

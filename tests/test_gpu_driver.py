@@ -74,3 +74,32 @@ def test_driver_matches_oracle_with_displacement_bcs(tmp_path, orc):
                                            ref["zssol"], mesh.nbe, dyn.l), mesh.nbe, dyn.l)
         for key in ("u", "t", "du", "sigma"):
             np.testing.assert_array_equal(mine[key], back[key])
+
+
+def _echo_part(data: bytes) -> bytes:
+    """A D output up to its first OUTPUTEC block (the INPUTECD echo)."""
+    mark = b"\n\n " + b"*" * 79 + b"\n\n" + b" " * 30 + b"FREQ\xdc\xcaNCIA"
+    k = data.find(mark)
+    assert k > 0
+    return data[:k]
+
+
+@pytest.mark.parametrize("size", [240, 2160])
+def test_output_files_in_the_reference_layout(tmp_path, size):
+    """src/tests.sh:42-55: the E output is byte-identical to the reference's
+    SSOLO*E file (INPUTECE's echo), the D output's INPUTECD echo is
+    byte-identical to the SSOLO*D file's, and the D output carries the
+    labels src/dtester.c:41-70,86-170 looks for."""
+    e, d = str(tmp_path / "e.dat"), str(tmp_path / "d.dat")
+    _lf_copy(os.path.join(GOLDEN, "ESOLO%dE_-5+5.DAT" % size), e)
+    _lf_copy(os.path.join(GOLDEN, "ESOLO%dD_-5+5.DAT" % size), d)
+    _run_driver(tmp_path, e, d)
+    eo = open(str(tmp_path / "out_e.txt"), "rb").read()
+    assert eo == open(os.path.join(GOLDEN, "SSOLO%dE_-5+5.SOL" % size), "rb").read()
+    do = open(str(tmp_path / "out_d.txt"), "rb").read()
+    gold = open(os.path.join(GOLDEN, "SSOLO%dD_-5+5.SOL" % size), "rb").read()
+    assert _echo_part(do) == _echo_part(gold.replace(b"\r\n", b"\n"))
+    for label in (b" N\xdaMERO DE ELEMENTOS DA MALHA=", b" N\xdaMERO DE ELEMENTOS DE CONTORNO=",
+                  b" N\xdaMERO DE PONTOS INTERNOS=", b" N\xd3S DO CONTORNO", b"TRACTION X",
+                  b"  PONTOS INTERNOS", b"SIGMAXX"):
+        assert label in do, label

@@ -67,9 +67,11 @@ def test_general_mesh_assembly_solve_interior(ctx, orc, name):
     print("ZH worst", worst, flush=True)
     assert ok, ("ZH", worst)
     assert _rel(zfi, ozfi) <= RTOL
-    # the trench has element pairs off the flat surface: dr/dn != 0 there
-    rows = np.abs(ozh).reshape(mesh.nbe, 3, mesh.nbe, 3)
-    assert np.count_nonzero(rows[:, 2, :, 2] > 1e-3 * rows.max()) > mesh.nbe
+    if name == "E510":
+        # the trench has element pairs off the flat surface: dr/dn != 0, so
+        # the normal-normal entries are nonzero off the diagonal
+        rows = np.abs(ozh).reshape(mesh.nbe, 3, mesh.nbe, 3)
+        assert np.count_nonzero(rows[:, 2, :, 2] > 1e-3 * rows.max()) > mesh.nbe
     x_ref = np.linalg.solve(ozh, ozfi)
     x64, _ = ctx.solve(zmat, fr, api.MODE_FP64)
     xm, st = ctx.solve(zmat, fr, api.MODE_MIXED)

@@ -138,3 +138,45 @@ def test_load_cases_share_one_factorisation(ctx, mode):
         x, _ = ctx.solve(zmat, 1.5, mode)
         rel = np.max(np.abs(x_all[:, j] - x)) / np.max(np.abs(x))
         assert rel < (1e-13 if mode == api.MODE_FP64 else 1e-10), (j, rel)
+
+
+@pytest.mark.parametrize("mode", [api.MODE_FP64, api.MODE_MIXED])
+def test_load_cases_with_their_own_kode(ctx, orc, mode):
+    """Three load cases, two KODE sets (cases 0 and 2 all traction, case 1
+    with prescribed displacements on 15 DOFs): one column swap and
+    factorisation per set, every case as if solved alone, and case 1 against
+    the oracle's chain with its KODE (Ghmatecd.for:316-333)."""
+    mesh, dyn = decks.flat_patch_problem(8, 7, n_interior=0, freqs=(1.7,), load_cases=3, seed=4)
+    nn = 3 * mesh.nbe
+    rng = np.random.default_rng(8)
+    kode = np.ones((nn, 3), dtype=np.int32)
+    sel = rng.choice(nn, 15, replace=False)
+    kode[sel, 1] = 0
+    zdfi = dyn.load_cases.T.astype(np.complex128).copy()
+    zdfi[sel, 1] = rng.normal(size=15) + 1j * rng.normal(size=15)
+    zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
+    ctx.send_shared_data(mesh)
+    ctx.ghmatece(mesh.ge, mesh.rnu)
+    ctx.send_load_cases(zdfi, kode)
+    with pytest.raises(api.BemxError) as ei:
+        ctx.ghmatecd(zmat, 1.7)
+    assert ei.value.code == 3
+    x, st = ctx.solve(zmat, 1.7, mode)
+    tol = 1e-12 if mode == api.MODE_FP64 else 1e-10
+    for j in range(3):
+        ctx.send_load_cases(zdfi[:, j:j + 1], kode[:, j])
+        xj, _ = ctx.solve(zmat, 1.7, mode)
+        rel = np.max(np.abs(x[:, j] - xj)) / np.max(np.abs(xj))
+        assert rel <= tol, (j, rel)
+    hd, gd, consts = orc.ghmatece(mesh)
+    ozh, ozfi, _ = orc.ghmatecd_accurate(mesh, hd, gd, consts, kode[:, 1], zdfi[:, 1], 1.7, zmat)
+    xr = np.linalg.solve(ozh, ozfi)
+    assert np.max(np.abs(x[:, 1] - xr)) / np.max(np.abs(xr)) <= tol
+    # the same request through the request loop
+    srv = api.Server(0, nslots=2, mode=mode)
+    try:
+        xs, info = srv.wait(srv.submit(mesh, kode, zdfi, (mesh.ge, mesh.rnu, dyn.dam, dyn.rho),
+                                       1.7))
+    finally:
+        srv.close()
+    assert np.max(np.abs(xs - x)) / np.max(np.abs(x)) <= tol
