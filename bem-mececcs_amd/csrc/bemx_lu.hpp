@@ -76,6 +76,10 @@ size_t mixed_workspace_bytes(int n);
 // panel candidates, permutations, solve vector).
 size_t fp64_factor_bytes(int n);
 
+// Error text for a cooperative launch the runtime refuses because another
+// process on the GPU holds its cooperative-launch resource.
+std::string coop_refused_message(const char* what);
+
 // Query one kernel of each kernel file so that its device code is loaded by
 // the calling thread (a context's creator) rather than by whichever thread
 // makes the first launch.

@@ -601,6 +601,12 @@ size_t fp64_factor_bytes(int n) {
            sizeof(long long) + sizeof(int) + sizeof(cd) * n + 9 * 256;
 }
 
+std::string coop_refused_message(const char* what) {
+    return std::string(what) +
+           ": cooperative launch refused (hipErrorIllegalState): another process on this GPU "
+           "holds the device's cooperative-launch resource; run one solver process per GPU";
+}
+
 hipError_t preload_lu_code() {
     hipFuncAttributes a;
     return hipFuncGetAttributes(&a, (const void*)k_panel_fp64);
@@ -649,8 +655,13 @@ static int factor_fp64(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, std:
         int* permk = perm + (size_t)pk * (1 + 4 * NB);
         void* args[] = {&A, &lda, (void*)&n, (void*)&k0, (void*)&nb, &ipiv, &cntk, &slots, &cand,
                         &drow, &permk, &info, &status};
-        LU_CHK(hipLaunchCooperativeKernel((const void*)k_panel_fp64, dim3(P), dim3(256), args, 0,
-                                          st));
+        const hipError_t ec = hipLaunchCooperativeKernel((const void*)k_panel_fp64, dim3(P),
+                                                         dim3(256), args, 0, st);
+        if (ec == hipErrorIllegalState) {
+            err = coop_refused_message("LU panel");
+            return BEMX_ERR_HIP;
+        }
+        LU_CHK(ec);
         const int c0 = k0 + nb, nc = n - c0;
         if (nc <= 0) continue;
         hipLaunchKernelGGL(k_rowperm, dim3((nc + 3) / 4), dim3(256), 0, st, A, lda, c0, nc, permk);

@@ -1803,7 +1803,14 @@ static int refine_one(LuWork& w, hipStream_t st, const MixedFactor& F, int n, co
     // norms for the backward error: ||A||_inf estimate via |A| row sums is
     // costly; use ||b|| and the residual relative to ||A x|| instead.
     hipLaunchKernelGGL(k_copy_z, dim3(nb256), dim3(256), 0, st, b, bsave, n);
-    MX_CHK(precond(st, F, bsave, x, tmp));  // x0 = M^-1 b
+    {
+        const hipError_t ep = precond(st, F, bsave, x, tmp);  // x0 = M^-1 b
+        if (ep == hipErrorIllegalState) {
+            err = coop_refused_message("mixed solve, block solves");
+            return BEMX_ERR_HIP;
+        }
+        MX_CHK(ep);
+    }
     std::vector<zx> h;
     double nrm2 = 0, bnorm = 0, relres = 1.0;
     int it_total = 0;

@@ -2,7 +2,13 @@
 drop-in Fortran subroutine surface and its driver: the D output of each
 reference deck must match the reference's golden result file under
 dtester.c's rule (row 1-norms <= 1e-3, no NaN).  Also the KODE = 0 path of
-the driver and the mixed-precision LINSOLVE through the same surface."""
+the driver, the mixed-precision LINSOLVE through the same surface, and the
+trench deck of malhas.zip.
+
+The driver runs in a process of its own while this one holds the GPU: the
+solver's cooperative launches need the GPU's cooperative-launch resource,
+which the first process to use it keeps, so these tests come before any
+test that solves in this process (the file order of the GPU suite)."""
 import os
 import subprocess
 
@@ -103,3 +109,29 @@ def test_output_files_in_the_reference_layout(tmp_path, size):
                   b" N\xdaMERO DE PONTOS INTERNOS=", b" N\xd3S DO CONTORNO", b"TRACTION X",
                   b"  PONTOS INTERNOS", b"SIGMAXX"):
         assert label in do, label
+
+
+@pytest.mark.parametrize("solver", ["fp64", "mixed"])
+def test_driver_on_the_trench(tmp_path, orc, solver):
+    """The Fortran driver (GHMATECE, GHMATECD, LINSOLVE, INTEREC through the
+    drop-in surface) on E510: its D output against the oracle chain's
+    results printed in the same format, under dtester's rule."""
+    e, d = str(tmp_path / "e.dat"), str(tmp_path / "d.dat")
+    _lf_copy(os.path.join(GOLDEN, "E510e.dat"), e)
+    _lf_copy(os.path.join(GOLDEN, "E510d.dat"), d)
+    text = _run_driver(tmp_path, e, d, {"BEMX_SOLVER": solver})
+    mesh = decks.read_e_deck(e)
+    dyn = decks.read_d_deck(d, mesh.nbe)
+    mine = outputs.parse_dynamic_results(text, mesh.nbe, dyn.l)
+    ref = orc.solve_problem(mesh, dyn, accurate=True)
+    want = outputs.parse_dynamic_results(
+        outputs.format_frequency_block(ref["fr"], ref["u"], ref["t"], ref["zdsol"], ref["zssol"],
+                                       mesh.nbe, dyn.l), mesh.nbe, dyn.l)
+    ok, err = outputs.compare_results(mine, want)
+    assert ok, err
+    # tighter than dtester: every printed value (4 significant digits) within
+    # one unit of its last digit, relative to the largest value of its row
+    for key in ("u", "t", "du", "sigma"):
+        a, b = mine[key], want[key]
+        scale = np.max(np.abs(b), axis=1, keepdims=True)
+        assert np.all(np.abs(a - b) <= 1.01e-3 * scale), key

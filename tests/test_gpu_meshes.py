@@ -11,17 +11,16 @@ elements.  The decks are the reference's own files (tests/golden/).
 Bars as for the configs: assembly 1e-12 per entry (block norm floor), the
 FP64 solution 1e-12 and the mixed one 1e-10 relative against LAPACK on the
 oracle's matrix (Oracle.ghmatecd_accurate), interior values 1e-12 from the
-same boundary solution and 1e-10 along each path's own chain; the driver's
-printed results within dtester's rule (src/dtester.c) of the oracle's.
+same boundary solution and 1e-10 along each path's own chain.  The driver
+run on the trench is in test_gpu_driver.py.
 """
 import os
-import subprocess
 
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, PKG, close_blocks
-from bemx import api, decks, outputs
+from conftest import GOLDEN, close_blocks
+from bemx import api, decks
 
 pytestmark = pytest.mark.gpu
 
@@ -87,40 +86,3 @@ def test_general_mesh_assembly_solve_interior(ctx, orc, name):
     assert _rel(zd, ozd) <= RTOL and _rel(zs, ozs) <= RTOL
     u, t, zd, zs = ctx.interec(xm, zdfi, dyn.kode, dyn.cxi, dyn.cyi, dyn.czi, zmat, fr, dyn.rho)
     assert _rel(zd, ozd) <= TOL_MIXED and _rel(zs, ozs) <= TOL_MIXED
-
-
-def _lf_copy(src, dst):
-    with open(src, "rb") as f:
-        data = f.read().replace(b"\r\n", b"\n")
-    with open(dst, "wb") as f:
-        f.write(data)
-
-
-@pytest.mark.parametrize("solver", ["fp64", "mixed"])
-def test_driver_on_the_trench(tmp_path, orc, solver):
-    """The Fortran driver (GHMATECE, GHMATECD, LINSOLVE, INTEREC through the
-    drop-in surface) on E510: its D output against the oracle chain's
-    results printed in the same format, under dtester's rule."""
-    e, d = str(tmp_path / "e.dat"), str(tmp_path / "d.dat")
-    _lf_copy(os.path.join(GOLDEN, "E510e.dat"), e)
-    _lf_copy(os.path.join(GOLDEN, "E510d.dat"), d)
-    env = dict(os.environ, BEMX_SOLVER=solver)
-    eo, do = str(tmp_path / "out_e.txt"), str(tmp_path / "out_d.txt")
-    res = subprocess.run([os.path.join(PKG, "bemx", "bemx_main"), e, d, eo, do],
-                         cwd=str(tmp_path), capture_output=True, text=True, env=env, timeout=300)
-    assert res.returncode == 0, res.stdout + res.stderr
-    mesh, dyn = _deck("E510")
-    mine = outputs.parse_dynamic_results(open(do, "rb").read().decode("latin-1"), mesh.nbe,
-                                         dyn.l)
-    ref = orc.solve_problem(mesh, dyn, accurate=True)
-    want = outputs.parse_dynamic_results(
-        outputs.format_frequency_block(ref["fr"], ref["u"], ref["t"], ref["zdsol"], ref["zssol"],
-                                       mesh.nbe, dyn.l), mesh.nbe, dyn.l)
-    ok, err = outputs.compare_results(mine, want)
-    assert ok, err
-    # tighter than dtester: every printed value (4 significant digits) within
-    # one unit of its last digit, relative to the largest value of its row
-    for key in ("u", "t", "du", "sigma"):
-        a, b = mine[key], want[key]
-        scale = np.max(np.abs(b), axis=1, keepdims=True)
-        assert np.all(np.abs(a - b) <= 1.01e-3 * scale), key
