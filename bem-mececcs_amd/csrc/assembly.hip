@@ -67,6 +67,8 @@ __global__ void k_element_setup(int n, const int* __restrict__ cone, const doubl
     }
     int npg2 = npg * npg;
     int bad = 0;
+    const double ecx = elem[j].cx, ecy = elem[j].cy, ecz = elem[j].cz;
+    double rad = 0.0;
     for (int jg = 0; jg < npg; ++jg) {
         double eta = gi[jg], sp = 1.0 + eta, sm = 1.0 - eta;
         for (int ig = 0; ig < npg; ++ig) {
@@ -92,11 +94,14 @@ __global__ void k_element_setup(int n, const int* __restrict__ cone, const doubl
             double det = sqrt(a * a + b * b + c * c);
             if (det < 1.0e-5) bad = 1;
             GaussPt g{xg[0], xg[1], xg[2], ome[ig] * ome[jg] * det};
+            rad = fmax(rad, sqrt((xg[0] - ecx) * (xg[0] - ecx) + (xg[1] - ecy) * (xg[1] - ecy) +
+                                 (xg[2] - ecz) * (xg[2] - ecz)));
             int q = jg * npg + ig;
             gp[(size_t)j * npg2 + q] = g;
             gpT[(size_t)q * n + j] = g;
         }
     }
+    elem[j].pad0 = rad;  // largest distance of a Gauss point from the collocation point
     if (bad) atomicOr(status, ST_DEGENERATE);
 }
 
@@ -291,55 +296,64 @@ __device__ inline cd mad_re(cd t, cd z, double c) {
     return cd{fma(t.x, z.x, fma(-t.y, z.y, c)), fma(t.x, z.y, t.y * z.x)};
 }
 
-// A R^2, B R^2, C R^2 from their series in z = ks R (|z| < SER_TAU), Horner
-// from the highest term; M0 = 1 leaves out the m = 0 terms (the static
-// limits).
-template <int M0>
-__device__ inline void series_abc(const DynConst& K, cd z, cd& AR2, cd& BR2, cd& CR2) {
-    cd ta = mk(K.ser[SER_A][NSER - 1], 0.0), tb = mk(K.ser[SER_B][NSER - 1], 0.0),
-       tc = mk(K.ser[SER_C][NSER - 1], 0.0);
-#pragma unroll
-    for (int m = NSER - 2; m >= M0; --m) {
-        ta = mad_re(ta, z, K.ser[SER_A][m]);
-        tb = mad_re(tb, z, K.ser[SER_B][m]);
-        tc = mad_re(tc, z, K.ser[SER_C][m]);
+// Horner steps M, M-1, .., M0 of the series of chi R, psi' R^2, chi' R^2
+// (and psi R with Psi): t = t z + f_m, f_m = a_m - b_m rho, rho = r^(m+2)
+// on entry and divided by r between steps.  a_m, b_m are compile-time
+// constants.
+template <int M, int M0, bool Psi>
+__device__ inline void ser_steps(cd z, double rho, double rinv, cd& tc, cd& tpd, cd& tcd,
+                                 cd& tp) {
+    constexpr double ac = ser_ab(SER_CHI, M, false), bc = ser_ab(SER_CHI, M, true);
+    constexpr double apd = ser_ab(SER_PSID, M, false), bpd = ser_ab(SER_PSID, M, true);
+    constexpr double acd = ser_ab(SER_CHID, M, false), bcd = ser_ab(SER_CHID, M, true);
+    tc = mad_re(tc, z, fma(-bc, rho, ac));
+    tpd = mad_re(tpd, z, fma(-bpd, rho, apd));
+    tcd = mad_re(tcd, z, fma(-bcd, rho, acd));
+    if constexpr (Psi) {
+        constexpr double ap = ser_ab(SER_PSI, M, false), bp = ser_ab(SER_PSI, M, true);
+        tp = mad_re(tp, z, fma(-bp, rho, ap));
     }
-    if (M0 == 1) {
-        ta = ta * z;
-        tb = tb * z;
-        tc = tc * z;
-    }
-    AR2 = ta;
-    BR2 = tb;
-    CR2 = tc;
+    if constexpr (M > M0) ser_steps<M - 1, M0, Psi>(z, rho * rinv, rinv, tc, tpd, tcd, tp);
 }
 
-// psi R and chi R from their series, as series_abc.
-template <int M0>
-__device__ inline void series_psichi(const DynConst& K, cd z, cd& psiR, cd& chiR) {
-    cd tp = mk(K.ser[SER_PSI][NSER - 1], 0.0), tc = mk(K.ser[SER_CHI][NSER - 1], 0.0);
-#pragma unroll
-    for (int m = NSER - 2; m >= M0; --m) {
-        tp = mad_re(tp, z, K.ser[SER_PSI][m]);
-        tc = mad_re(tc, z, K.ser[SER_CHI][m]);
-    }
-    if (M0 == 1) {
-        tp = tp * z;
+// chi R, psi' R^2, chi' R^2 (and psi R) summed over m = M0..N-1 of their
+// series in z = ks R; rtop = r^(N+1).  With M0 = 1 the static limits (the
+// m = 0 terms) are left out.
+template <int N, int M0, bool Psi>
+__device__ inline void ser_sums(const DynConst& K, cd z, double rtop, cd& chiR, cd& psidR2,
+                                cd& chidR2, cd& psiR) {
+    cd tc = mk(0.0, 0.0), tpd = tc, tcd = tc, tp = tc;
+    ser_steps<N - 1, M0, Psi>(z, rtop, K.ser_rinv, tc, tpd, tcd, tp);
+    if constexpr (M0 == 1) {
         tc = tc * z;
+        tpd = tpd * z;
+        tcd = tcd * z;
+        if constexpr (Psi) tp = tp * z;
     }
-    psiR = tp;
     chiR = tc;
+    psidR2 = tpd;
+    chidR2 = tcd;
+    psiR = tp;
+}
+
+// A R^2, B R^2, C R^2 from chi R, psi' R^2, chi' R^2 (Solfund.for:51-53
+// multiplied through by R^2; A + B/2 - 3 chi = psi' - chi' - 2 chi).
+__device__ inline void abc_from(const DynConst& K, cd chiR, cd psidR2, cd chidR2, cd& AR2,
+                               cd& BR2, cd& CR2) {
+    AR2 = psidR2 - chiR;
+    BR2 = 4.0 * chiR - 2.0 * chidR2;
+    CR2 = K.ser_c1m2 * (psidR2 - chidR2 - 2.0 * chiR) - 2.0 * chiR;
 }
 
 template <bool WithPsi>
 __device__ inline void dyn_scalars(const DynConst& K, double R, double iR, cd& psi, cd& chi,
                                    cd& A, cd& B, cd& C) {
-    if (K.ks_abs2 * (R * R) < SER_TAU * SER_TAU) {
+    if (K.ks_abs2 * (R * R) < TAU_OFF * TAU_OFF) {
         const cd z = K.ks * R;
         const double iR2 = iR * iR;
-        cd psiR, chiR, AR2, BR2, CR2;
-        series_psichi<0>(K, z, psiR, chiR);
-        series_abc<0>(K, z, AR2, BR2, CR2);
+        cd psiR, chiR, psidR2, chidR2, AR2, BR2, CR2;
+        ser_sums<NSER_OFF, 0, WithPsi>(K, z, K.ser_rtop_off, chiR, psidR2, chidR2, psiR);
+        abc_from(K, chiR, psidR2, chidR2, AR2, BR2, CR2);
         psi = psiR * iR;
         chi = chiR * iR;
         A = AR2 * iR2;
@@ -349,6 +363,17 @@ __device__ inline void dyn_scalars(const DynConst& K, double R, double iR, cd& p
     }
     dyn_scalars_e<WithPsi>(K, R, iR, exp_over_r(K.ks, R, iR), K.c2 * exp_over_r(K.kp, R, iR),
                            psi, chi, A, B, C);
+}
+
+// Squared distance with explicit FMAs: the near-field split of the H
+// assembly (series below |ks| R = TAU_OFF, closed form above) must classify
+// every Gauss point the same way in both kernels.
+__device__ inline double dist2(double r1, double r2, double r3) {
+    return fma(r3, r3, fma(r2, r2, r1 * r1));
+}
+
+__device__ inline bool near_field(const DynConst& K, double RR) {
+    return K.ks_abs2 * RR < TAU_OFF * TAU_OFF;
 }
 
 // ---------------------------------------------------------------------------
@@ -473,11 +498,11 @@ __global__ __launch_bounds__(64) void k_dyn_diag(int nbe, int npg2, const ElemDa
         // the static kernels
         cd ud, ur, ta, tb, tc;
         const double iR2 = iR * iR;
-        if (K.ks_abs2 * (R * R) < SER_TAU * SER_TAU) {
+        if (K.ks_abs2 * (R * R) < TAU_DIAG * TAU_DIAG) {
             const cd z = K.ks * R;
-            cd psiR, chiR, AR2, BR2, CR2;
-            series_psichi<1>(K, z, psiR, chiR);
-            series_abc<1>(K, z, AR2, BR2, CR2);
+            cd psiR, chiR, psidR2, chidR2, AR2, BR2, CR2;
+            ser_sums<NSER_DIAG, 1, true>(K, z, K.ser_rtop_diag, chiR, psidR2, chidR2, psiR);
+            abc_from(K, chiR, psidR2, chidR2, AR2, BR2, CR2);
             ud = (K.u_d0 + K.c0 * psiR) * iR;
             ur = (K.u_r0 + K.c0 * chiR) * iR;
             ta = (mk(K.t_a0, 0.0) + AR2 * K.q4pi) * iR2;
@@ -616,16 +641,15 @@ __device__ inline void offdiag_element_h_tab(const DynConst& K, const ExpTab& T,
     for (int q = 0; q < npg2; ++q) {
         GaussPt p = sg[q];
         double r1 = p.x - xc, r2 = p.y - yc, r3 = p.z - zc;
-        const double RR = r1 * r1 + r2 * r2 + r3 * r3;
+        const double RR = dist2(r1, r2, r3);
+        if (near_field(K, RR)) continue;  // k_dyn_offdiag_near adds these points
         double iR = rsqrt_refined(RR);
         double R = RR * iR;
         double rd[3] = {r1 * iR, r2 * iR, r3 * iR};
         double drn = rd[0] * nv[0] + rd[1] * nv[1] + rd[2] * nv[2];
         double iR2 = iR * iR;
         cd A, B, C;
-        if (K.ks_abs2 * RR < SER_TAU * SER_TAU) {
-            series_abc<0>(K, K.ks * R, A, B, C);
-        } else {
+        {
             cd Es, Epc;
             exp_pair_tab(K, T, R, Es, Epc);
             cd as = K.as * iR, as2 = K.as2 * iR2;
@@ -706,6 +730,96 @@ __global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag_h(int nbe, int npg2,
                 col[0] = H[3 * k];
                 col[1] = H[3 * k + 1];
                 col[2] = H[3 * k + 2];
+            }
+        }
+    }
+}
+
+// Near-field part of the H blocks that k_dyn_offdiag_h leaves out: the
+// Gauss points with |ks| R < TAU_OFF, where the closed form of the kernel
+// scalars loses digits, summed from their series and added to ZH (same
+// factored accumulation, same columns).  A workgroup covers the rows and
+// the tile of column elements of one k_dyn_offdiag_h workgroup; it returns
+// at once when no row is within TAU_OFF / |ks| of an element of the tile
+// (every tile but the diagonal neighbourhood once |ks| h >~ 2), and a wave
+// skips an element none of its rows can reach.
+template <int WAVES, int JT>
+__global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag_near(int nbe, int npg2,
+                                                                 const ElemData* __restrict__ elem,
+                                                                 const GaussPt* __restrict__ gp,
+                                                                 const int* __restrict__ colflag,
+                                                                 DynConst K, cd* __restrict__ zh,
+                                                                 size_t ldzh) {
+    extern __shared__ GaussPt sgp[];
+    const int j0 = blockIdx.y * JT;
+    const int jn = min(JT, nbe - j0);
+    const int I = blockIdx.x * (64 * WAVES) + threadIdx.x;
+    const bool rowok = I < nbe;
+    const int Ic = rowok ? I : nbe - 1;
+    const double xc = elem[Ic].cx, yc = elem[Ic].cy, zc = elem[Ic].cz;
+    // reach^2 with a margin: a Gauss point of J lies within rad_J of its
+    // collocation point
+    const double reach = TAU_OFF / sqrt(K.ks_abs2);
+    int nearmask = 0;
+    for (int jj = 0; jj < jn; ++jj) {
+        const ElemData e = elem[j0 + jj];
+        const double d = sqrt(dist2(e.cx - xc, e.cy - yc, e.cz - zc));
+        if (rowok && I != j0 + jj && d < reach + e.pad0 + 1e-9 * (reach + d)) nearmask |= 1 << jj;
+    }
+    if (!__syncthreads_or(nearmask)) return;
+    for (int t = threadIdx.x; t < jn * npg2; t += 64 * WAVES) {
+        GaussPt g = gp[(size_t)j0 * npg2 + t];
+        g.w *= K.q4pi;
+        sgp[t] = g;
+    }
+    __syncthreads();
+    for (int jj = 0; jj < jn; ++jj) {
+        if (!__any((nearmask >> jj) & 1)) continue;
+        const int J = j0 + jj;
+        const int flag = colflag[J];
+        const double nv[3] = {elem[J].nx, elem[J].ny, elem[J].nz};
+        const GaussPt* sg = sgp + jj * npg2;
+        cd SP[3], SC[3], SB[6];
+        for (int k = 0; k < 3; ++k) SP[k] = SC[k] = mk(0, 0);
+        for (int k = 0; k < 6; ++k) SB[k] = mk(0, 0);
+        bool any = false;
+        for (int q = 0; q < npg2; ++q) {
+            GaussPt p = sg[q];
+            double r1 = p.x - xc, r2 = p.y - yc, r3 = p.z - zc;
+            const double RR = dist2(r1, r2, r3);
+            if (!near_field(K, RR)) continue;
+            any = true;
+            const double iR = rsqrt_refined(RR);
+            const double R = RR * iR;
+            double rd[3] = {r1 * iR, r2 * iR, r3 * iR};
+            double drn = rd[0] * nv[0] + rd[1] * nv[1] + rd[2] * nv[2];
+            cd chiR, psidR2, chidR2, unused, A, B, C;
+            ser_sums<NSER_OFF, 0, false>(K, K.ks * R, K.ser_rtop_off, chiR, psidR2, chidR2,
+                                         unused);
+            abc_from(K, chiR, psidR2, chidR2, A, B, C);
+            double wq = p.w * (iR * iR);
+            cd Ap = A * wq, Bp = B * (wq * drn), Cp = C * wq;
+            for (int j = 0; j < 3; ++j) {
+                SP[j] = SP[j] + Ap * rd[j];
+                SC[j] = SC[j] + Cp * rd[j];
+            }
+            SB[0] = SB[0] + Bp * (rd[0] * rd[0]);
+            SB[1] = SB[1] + Bp * (rd[1] * rd[0]);
+            SB[2] = SB[2] + Bp * (rd[2] * rd[0]);
+            SB[3] = SB[3] + Bp * (rd[1] * rd[1]);
+            SB[4] = SB[4] + Bp * (rd[2] * rd[1]);
+            SB[5] = SB[5] + Bp * (rd[2] * rd[2]);
+        }
+        if (!(any && rowok && I != J)) continue;
+        const cd SD = SP[0] * nv[0] + SP[1] * nv[1] + SP[2] * nv[2];
+        const int sidx[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+        for (int k = 0; k < 3; ++k) {
+            if ((flag >> (1 + k)) & 1) continue;
+            cd* col = zh + (size_t)(3 * J + k) * ldzh + 3 * I;
+            for (int i = 0; i < 3; ++i) {
+                cd h = SB[sidx[i + 3 * k]] + SC[i] * nv[k] + SP[k] * nv[i];
+                if (i == k) h = h + SD;
+                col[i] = col[i] + h;
             }
         }
     }
@@ -1084,10 +1198,12 @@ hipError_t launch_dyn_assembly(hipStream_t st, int nbe, int npg2, const ElemData
     if (ev_od0) (void)hipEventRecord(ev_od0, st);
     const dim3 odgrid(rowblk, (nbe + OD_JT - 1) / OD_JT), odblock(64 * OD_WAVES);
     const size_t odlds = (size_t)OD_JT * npg2 * sizeof(GaussPt);
-    if (tab)
+    if (tab) {
         hipLaunchKernelGGL((k_dyn_offdiag_h<OD_WAVES, OD_JT, true>), odgrid, odblock, odlds, st,
                            nbe, npg2, elem, gp, colflag, K, T, zh, ldzh);
-    else
+        hipLaunchKernelGGL((k_dyn_offdiag_near<OD_WAVES, OD_JT>), odgrid, odblock, odlds, st,
+                           nbe, npg2, elem, gp, colflag, K, zh, ldzh);
+    } else
         hipLaunchKernelGGL((k_dyn_offdiag_h<OD_WAVES, OD_JT, false>), odgrid, odblock, odlds, st,
                            nbe, npg2, elem, gp, colflag, K, T, zh, ldzh);
     if (ev_od1) (void)hipEventRecord(ev_od1, st);

@@ -128,43 +128,31 @@ int check_status(bemx_ctx* c) {
     return BEMX_OK;
 }
 
-// Series coefficients of the small-|z| kernel scalars (see DynConst), in
-// long double on the host: f_m = sum_{j=-2..min(1,m)} (p_j - q_j r^(m+2)) /
-// (m-j)! for psi R, chi R, psi' R^2, chi' R^2 (P, Q of Solfund.for:42-49),
-// then A R^2 = psi' R^2 - chi R, B R^2 = 4 chi R - 2 chi' R^2 and
-// C R^2 = (1/r^2 - 2)(A R^2 + B R^2 / 2 - 3 chi R) - 2 chi R (Solfund.for:51-53).
+// Run-time constants of the small-|z| series (see DynConst): r = cs/cp and
+// its powers, and the residuals of the static limits (the m = 0 terms) that
+// the diagonal blocks' difference kernels subtract, in long double on the
+// host.  A R^2 = psi' R^2 - chi R, B R^2 = 4 chi R - 2 chi' R^2 and
+// C R^2 = (1/r^2 - 2)(psi' R^2 - chi' R^2 - 2 chi R) - 2 chi R
+// (Solfund.for:51-53 multiplied through by R^2).
 void fill_series(DynConst& K) {
-    static const int P[4][4] = {{1, -1, 1, 0}, {3, -3, 1, 0}, {-3, 3, -2, 1}, {-9, 9, -4, 1}};
-    static const int Q[4][4] = {{1, -1, 0, 0}, {3, -3, 1, 0}, {-3, 3, -1, 0}, {-9, 9, -4, 1}};
     const long double r2 = K.c2.x, r = std::sqrt(r2), c1m2 = 1.0L / r2 - 2.0L;
-    long double f[4][NSER], fact[NSER + 2];
-    fact[0] = 1.0L;
-    for (int i = 1; i < NSER + 2; ++i) fact[i] = fact[i - 1] * i;
-    for (int k = 0; k < 4; ++k)
-        for (int m = 0; m < NSER; ++m) {
-            long double acc = 0.0L, rp = 1.0L;
-            for (int i = 0; i < m + 2; ++i) rp *= r;
-            for (int j = -2; j <= 1 && j <= m; ++j)
-                acc += ((long double)P[k][j + 2] - (long double)Q[k][j + 2] * rp) / fact[m - j];
-            f[k][m] = acc;
-        }
-    long double fa[NSER], fb[NSER], fc[NSER];
-    for (int m = 0; m < NSER; ++m) {
-        fa[m] = f[2][m] - f[1][m];
-        fb[m] = 4.0L * f[1][m] - 2.0L * f[3][m];
-        fc[m] = c1m2 * (fa[m] + 0.5L * fb[m] - 3.0L * f[1][m]) - 2.0L * f[1][m];
-        K.ser[SER_PSI][m] = (double)f[0][m];
-        K.ser[SER_CHI][m] = (double)f[1][m];
-        K.ser[SER_A][m] = (double)fa[m];
-        K.ser[SER_B][m] = (double)fb[m];
-        K.ser[SER_C][m] = (double)fc[m];
-    }
+    auto f0 = [&](int f) { return (long double)ser_ab(f, 0, false) - ser_ab(f, 0, true) * r2; };
+    const long double chi = f0(SER_CHI), psid = f0(SER_PSID), chid = f0(SER_CHID);
+    const long double fa = psid - chi, fb = 4.0L * chi - 2.0L * chid;
+    const long double fc = c1m2 * (psid - chid - 2.0L * chi) - 2.0L * chi;
+    long double rt = r;
+    for (int i = 1; i < NSER_OFF + 1; ++i) rt *= r;
+    K.ser_rtop_off = (double)rt;
+    for (int i = NSER_OFF + 1; i < NSER_DIAG + 1; ++i) rt *= r;
+    K.ser_rtop_diag = (double)rt;
+    K.ser_rinv = (double)(1.0L / r);
+    K.ser_c1m2 = (double)c1m2;
     K.ks_abs2 = K.ks.x * K.ks.x + K.ks.y * K.ks.y;
     const long double q4pi = K.q4pi, c3 = K.c3s, c4 = K.c4s;
-    K.t_a0 = (double)(q4pi * fa[0] - c3 * c4);
-    K.t_b0 = (double)(q4pi * fb[0] - 3.0L * c3);
-    K.t_c0 = (double)(q4pi * fc[0] + c3 * c4);
-    const cd fpsi = mk((double)f[0][0], 0.0), fchi = mk((double)f[1][0], 0.0);
+    K.t_a0 = (double)(q4pi * fa - c3 * c4);
+    K.t_b0 = (double)(q4pi * fb - 3.0L * c3);
+    K.t_c0 = (double)(q4pi * fc + c3 * c4);
+    const cd fpsi = mk((double)f0(SER_PSI), 0.0), fchi = mk((double)chi, 0.0);
     K.u_d0 = K.c0 * fpsi - mk(K.c1s * K.c2s, 0.0);
     K.u_r0 = K.c0 * fchi + mk(K.c1s, 0.0);
 }

@@ -31,14 +31,31 @@ __host__ __device__ inline cd operator/(cd a, cd b) { return a * cinv(b); }
 // e^(r z) with r = cs/cp (real: the damping is the same for both waves) and
 // P, Q polynomials in 1/z of degree 2, whose 1/z^2 and 1/z terms cancel
 // between the two waves; evaluated as written, FP64 rounding is amplified
-// by ~1/|z|^2.  Their power series sum_m f_m z^m (m >= 0) has real
-// coefficients f_m = sum_j (p_j - q_j r^(m+2)) / (m - j)! (p_j, q_j the
-// coefficients of z^j in P and Q), and so do A R^2, B R^2, C R^2.  Below
-// |z| = SER_TAU the kernels use the series: NSER terms truncate below 1e-19
-// relative, and above SER_TAU the closed form loses at most ~6e-14.
-constexpr int NSER = 18;
-constexpr double SER_TAU = 0.5;
-enum { SER_PSI = 0, SER_CHI, SER_A, SER_B, SER_C, SER_N };
+// by ~1/|z|^2.  Their power series sum_m f_m z^m (m >= 0) has coefficients
+// f_m = a_m - b_m r^(m+2), a_m = sum_j p_j / (m-j)!, b_m = sum_j q_j / (m-j)!
+// (p_j, q_j the coefficients of z^j in P and Q, j = -2..1): a_m and b_m are
+// pure numbers, so the series need only r at run time.  Below |z| = tau the
+// kernels sum the series: off the diagonal tau = 0.35 with 16 terms (the
+// closed form loses at most ~1.2e-13 above it, the series truncates below
+// 1e-17), in the diagonal blocks tau = 0.5 with 18 terms (3e-20).
+constexpr int NSER_OFF = 16, NSER_DIAG = 18;
+constexpr double TAU_OFF = 0.35, TAU_DIAG = 0.5;
+enum { SER_PSI = 0, SER_CHI, SER_PSID, SER_CHID };
+constexpr int SER_P[4][4] = {{1, -1, 1, 0}, {3, -3, 1, 0}, {-3, 3, -2, 1}, {-9, 9, -4, 1}};
+constexpr int SER_Q[4][4] = {{1, -1, 0, 0}, {3, -3, 1, 0}, {-3, 3, -1, 0}, {-9, 9, -4, 1}};
+
+constexpr double cfact(int k) {
+    double f = 1.0;
+    for (int i = 2; i <= k; ++i) f *= i;
+    return f;
+}
+
+// a_m (q = false) or b_m (q = true) of function f
+constexpr double ser_ab(int f, int m, bool q) {
+    double s = 0.0;
+    for (int j = -2; j <= 1 && j <= m; ++j) s += (q ? SER_Q[f][j + 2] : SER_P[f][j + 2]) / cfact(m - j);
+    return s;
+}
 
 // Per-frequency constants of the dynamic Kelvin/Stokes kernel
 // (Solfund.for:29-40): k = -i w / c for the S and P waves, their inverses
@@ -55,8 +72,11 @@ struct DynConst {
     cd lm;         // lambda*/mu*  (stress kernels)
     double q4pi;   // 1/(4 pi)
     double c1s, c2s, c3s, c4s;  // static constants C1..C4 (Ghmatece.for:60-63)
-    double ks_abs2;             // |ks|^2: the series is used where |ks|^2 R^2 < SER_TAU^2
-    double ser[SER_N][NSER];    // series coefficients of psi R, chi R, A R^2, B R^2, C R^2
+    double ks_abs2;             // |ks|^2: the series is used where |ks|^2 R^2 < tau^2
+    double ser_rinv;            // 1/r, r = cs/cp
+    double ser_rtop_off;        // r^(NSER_OFF + 1)
+    double ser_rtop_diag;       // r^(NSER_DIAG + 1)
+    double ser_c1m2;            // 1/r^2 - 2
     // Diagonal blocks (dynamic minus static kernels, Solfundif.for:56-69)
     // with the series: the m = 0 terms are the static limits, which cancel
     // the static kernels up to these residuals (zero in exact arithmetic for
