@@ -33,6 +33,7 @@ EXPORTS = [
     "bemx_linsolve", "bemx_interec", "bemx_solve", "bemx_device_buffers", "bemx_last_stats",
     "bemx_send_load_cases", "bemx_send_load_cases_kode", "bemx_material", "bemx_server_create", "bemx_server_submit",
     "bemx_server_wait", "bemx_server_destroy", "bemx_request_footprint", "bemx_server_stats",
+    "bemx_linsolve_device_noalloc",
 ]
 
 
@@ -62,6 +63,8 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
     lib.bemx_send_load_cases_kode.argtypes = [vp, C.c_int, _dp, _ip]
     lib.bemx_ghmatecd.argtypes = [vp, _dp, _dp, _dp] + [C.c_double] * 5 + [_dp] * 4
     lib.bemx_linsolve.argtypes = [vp, C.c_int, _dp, _dp, C.c_int, C.POINTER(C.c_longlong)]
+    lib.bemx_linsolve_device_noalloc.argtypes = [vp, C.c_int, vp, C.c_longlong, vp,
+                                                 C.POINTER(C.c_longlong)]
     lib.bemx_interec.argtypes = [vp, _dp, _dp, _ip, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
                                  C.c_double, C.c_double, _dp, _dp]
     lib.bemx_solve.argtypes = [vp, _dp, _dp, _dp, C.c_double, C.c_int, _dp, _dp]

@@ -273,7 +273,7 @@ int bemx_create(int device, bemx_ctx** out) {
         hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess ||
         hipEventCreate(&c->ev4) != hipSuccess || c->status.ensure(sizeof(int)) != hipSuccess ||
         preload_assembly_code() != hipSuccess || preload_lu_code() != hipSuccess ||
-        preload_mixed_code() != hipSuccess) {
+        preload_mixed_code() != hipSuccess || scratch_setup(device) != hipSuccess) {
         bemx_destroy(c);
         return BEMX_ERR_HIP;
     }
@@ -627,6 +627,23 @@ int bemx_linsolve(bemx_ctx* c, int nn, double* zh, double* zfi, int mode, long l
     HIPCHK(c, hipMemcpyAsync(zfi, c->zfi.p, sizeof(cd) * n, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return BEMX_OK;
+}
+
+int bemx_linsolve_device_noalloc(bemx_ctx* c, int nn, void* a, long long lda, void* b,
+                                 long long* info) {
+    if (!c) return BEMX_ERR_ARG;
+    if (info) *info = 0;
+    if (!a || !b || nn < 1 || lda < nn)
+        return fail(c, BEMX_ERR_ARG, "%sbad device LINSOLVE arguments", "", 0);
+    if (nn > bemx::noalloc_max_n())
+        return fail(c, BEMX_ERR_ARG, "%sorder above the static workspace's maximum %ld", "",
+                    (long)bemx::noalloc_max_n());
+    HIPCHK(c, hipSetDevice(c->device));
+    long long inf = 0;
+    const int s = bemx::lu_solve_fp64_noalloc(c->device, c->st, nn, (cd*)a, (size_t)lda, (cd*)b,
+                                              &inf, c->err);
+    if (info) *info = inf;
+    return s;
 }
 
 int bemx_interec(bemx_ctx* c, double* zfi, double* zdfi, const int* kode, int l,

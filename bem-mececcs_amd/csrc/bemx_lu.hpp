@@ -76,6 +76,14 @@ size_t mixed_workspace_bytes(int n);
 // panel candidates, permutations, solve vector).
 size_t fp64_factor_bytes(int n);
 
+// FP64 solve of one right-hand side with its workspace in the static
+// device array of the device (no allocation); n <= noalloc_max_n().
+int lu_solve_fp64_noalloc(int device, hipStream_t st, int n, cd* A, size_t lda, cd* b,
+                          long long* info, std::string& err);
+int noalloc_max_n();
+// Looks up the static array's device address on `device` (current).
+hipError_t scratch_setup(int device);
+
 // Error text for a cooperative launch the runtime refuses because another
 // process on the GPU holds its cooperative-launch resource.
 std::string coop_refused_message(const char* what);

@@ -19,8 +19,10 @@
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "../../include/bemx.h"
@@ -592,14 +594,112 @@ __global__ void k_copy_cd(const cd* __restrict__ s, cd* __restrict__ d, int n) {
         }                                                                   \
     } while (0)
 
-enum { W_IPIV, W_CNT, W_SLOTS, W_CAND, W_DROW, W_PERM, W_INFO, W_STATUS, W_BTMP };
+enum { W_IPIV, W_CNT, W_SLOTS, W_CAND, W_DROW, W_PERM, W_INFO, W_STATUS, W_BTMP, W_N };
+
+// Bytes of buffer i of the FP64 factorisation and solve for order n:
+// pivots, per-column arrival counters, per-workgroup pivot candidates and
+// their rows, the pivot row block, the panels' row permutations, INFO, the
+// status word and the solve's work vector.
+constexpr size_t fp64_size(int n, int i) {
+    return i == W_IPIV     ? sizeof(int) * (size_t)n
+           : i == W_CNT    ? sizeof(int) * (size_t)((n + NB - 1) / NB) * NB
+           : i == W_SLOTS  ? sizeof(Cand) * NB * (size_t)((n + PANEL_ROWS - 1) / PANEL_ROWS)
+           : i == W_CAND   ? sizeof(cd) * NB * (size_t)((n + PANEL_ROWS - 1) / PANEL_ROWS) * NB
+           : i == W_DROW   ? sizeof(cd) * NB * NB
+           : i == W_PERM   ? sizeof(int) * (size_t)((n + NB - 1) / NB) * (1 + 4 * NB)
+           : i == W_INFO   ? sizeof(long long)
+           : i == W_STATUS ? sizeof(int)
+                           : sizeof(cd) * (size_t)n;
+}
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// The buffers of fp64_size laid out back to back, 256-byte aligned.
+constexpr size_t fp64_carved_bytes(int n) {
+    size_t t = 0;
+    for (int i = 0; i < W_N; ++i) t += align256(fp64_size(n, i));
+    return t;
+}
 
 size_t fp64_factor_bytes(int n) {
-    const size_t npanel = (size_t)(n + NB - 1) / NB, pmax = (size_t)(n + PANEL_ROWS - 1) / PANEL_ROWS;
-    return sizeof(int) * n + sizeof(int) * npanel * NB + sizeof(Cand) * NB * pmax +
-           sizeof(cd) * NB * pmax * NB + sizeof(cd) * NB * NB + sizeof(int) * npanel * (1 + 4 * NB) +
-           sizeof(long long) + sizeof(int) + sizeof(cd) * n + 9 * 256;
+    size_t t = 0;
+    for (int i = 0; i < W_N; ++i) t += std::max<size_t>(fp64_size(n, i), 256);
+    return t;
 }
+
+// Device pointers of one FP64 workspace.
+struct Fp64Ws {
+    int* ipiv = nullptr;
+    int* cnt = nullptr;
+    Cand* slots = nullptr;
+    cd* cand = nullptr;
+    cd* drow = nullptr;
+    int* perm = nullptr;
+    long long* info = nullptr;
+    int* status = nullptr;
+    cd* btmp = nullptr;
+    void set(void* const p[W_N]) {
+        ipiv = (int*)p[W_IPIV];
+        cnt = (int*)p[W_CNT];
+        slots = (Cand*)p[W_SLOTS];
+        cand = (cd*)p[W_CAND];
+        drow = (cd*)p[W_DROW];
+        perm = (int*)p[W_PERM];
+        info = (long long*)p[W_INFO];
+        status = (int*)p[W_STATUS];
+        btmp = (cd*)p[W_BTMP];
+    }
+};
+
+// The workspace in LuWork's grow-only buffers (allocates on growth).
+static int fp64_ws_alloc(LuWork& w, int n, Fp64Ws& ws, std::string& err) {
+    void* p[W_N];
+    hipError_t e;
+    for (int i = 0; i < W_N; ++i) {
+        p[i] = w.get(i, fp64_size(n, i), &e);
+        LU_CHK(e);
+    }
+    ws.set(p);
+    return BEMX_OK;
+}
+
+// The workspace carved out of one block of fp64_carved_bytes(n) bytes.
+static void fp64_ws_carve(char* base, int n, Fp64Ws& ws) {
+    void* p[W_N];
+    size_t off = 0;
+    for (int i = 0; i < W_N; ++i) {
+        p[i] = base + off;
+        off += align256(fp64_size(n, i));
+    }
+    ws.set(p);
+}
+
+// Static device workspace of bemx_linsolve_device_noalloc: the FP64
+// factorisation's buffers for orders up to NOALLOC_MAX_N (18.4 MB).  Each
+// GPU the library runs on holds its own copy until the process ends; every
+// call on that GPU shares it.  Host code takes its address from
+// hipGetSymbolAddress (scratch_setup, once per device when a context is
+// made on it) and never uses the array's name, which on the host is only a
+// stand-in for the symbol.
+constexpr int NOALLOC_MAX_N = 32768;
+constexpr size_t NOALLOC_BYTES = fp64_carved_bytes(NOALLOC_MAX_N);
+static __device__ __align__(256) char scratch[NOALLOC_BYTES];
+constexpr int MAX_DEVICES = 64;
+static std::atomic<void*> scratch_addr[MAX_DEVICES];
+static std::mutex scratch_mu;
+
+hipError_t scratch_setup(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (scratch_addr[device].load()) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    scratch_addr[device].store(p);
+    return hipSuccess;
+}
+
+int noalloc_max_n() { return NOALLOC_MAX_N; }
 
 std::string coop_refused_message(const char* what) {
     return std::string(what) +
@@ -612,26 +712,18 @@ hipError_t preload_lu_code() {
     return hipFuncGetAttributes(&a, (const void*)k_panel_fp64);
 }
 
-static int factor_fp64(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, std::string& err) {
-    hipError_t e;
+// tw (may be NULL): the workspace whose events time the factorisation.
+static int factor_fp64(const Fp64Ws& ws, hipStream_t st, int n, cd* A, size_t lda, LuWork* tw,
+                       std::string& err) {
     const int npanel = (n + NB - 1) / NB;
-    const int pmax = (n + PANEL_ROWS - 1) / PANEL_ROWS;
-    int* ipiv = (int*)w.get(W_IPIV, sizeof(int) * n, &e);
-    LU_CHK(e);
-    int* cnt = (int*)w.get(W_CNT, sizeof(int) * (size_t)npanel * NB, &e);
-    LU_CHK(e);
-    Cand* slots = (Cand*)w.get(W_SLOTS, sizeof(Cand) * (size_t)NB * pmax, &e);
-    LU_CHK(e);
-    cd* cand = (cd*)w.get(W_CAND, sizeof(cd) * (size_t)NB * pmax * NB, &e);
-    LU_CHK(e);
-    cd* drow = (cd*)w.get(W_DROW, sizeof(cd) * NB * NB, &e);
-    LU_CHK(e);
-    int* perm = (int*)w.get(W_PERM, sizeof(int) * (size_t)npanel * (1 + 4 * NB), &e);
-    LU_CHK(e);
-    long long* info = (long long*)w.get(W_INFO, sizeof(long long), &e);
-    LU_CHK(e);
-    int* status = (int*)w.get(W_STATUS, sizeof(int), &e);
-    LU_CHK(e);
+    int* ipiv = ws.ipiv;
+    int* cnt = ws.cnt;
+    Cand* slots = ws.slots;
+    cd* cand = ws.cand;
+    cd* drow = ws.drow;
+    int* perm = ws.perm;
+    long long* info = ws.info;
+    int* status = ws.status;
     size_t ncnt = (size_t)npanel * NB;
     hipLaunchKernelGGL(k_fill_int, dim3((ncnt + 255) / 256), dim3(256), 0, st, cnt, 0, ncnt);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, (int*)info, 0, 2);
@@ -640,8 +732,10 @@ static int factor_fp64(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, std:
     LU_CHK(hipGetDevice(&dev));
     LU_CHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     LU_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_panel_fp64, 256, 0));
-    LU_CHK(w.ensure_side());
-    LU_CHK(hipEventRecord(w.ev_f0, st));
+    if (tw) {
+        LU_CHK(tw->ensure_side());
+        LU_CHK(hipEventRecord(tw->ev_f0, st));
+    }
     for (int pk = 0; pk < npanel; ++pk) {
         const int k0 = pk * NB;
         const int nb = std::min(NB, n - k0);
@@ -675,19 +769,18 @@ static int factor_fp64(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, std:
                            A + (size_t)c0 * lda + k0, lda, A + (size_t)c0 * lda + c0, lda);
         LU_CHK(hipGetLastError());
     }
-    LU_CHK(hipEventRecord(w.ev_f1, st));
-    w.factor_timed = true;
+    if (tw) {
+        LU_CHK(hipEventRecord(tw->ev_f1, st));
+        tw->factor_timed = true;
+    }
     return BEMX_OK;
 }
 
-static int solve_fp64(LuWork& w, hipStream_t st, int n, const cd* A, size_t lda, cd* b,
+static int solve_fp64(const Fp64Ws& ws, hipStream_t st, int n, const cd* A, size_t lda, cd* b,
                       std::string& err) {
-    hipError_t e;
     const int npanel = (n + NB - 1) / NB;
-    int* perm = (int*)w.get(W_PERM, sizeof(int) * (size_t)npanel * (1 + 4 * NB), &e);
-    LU_CHK(e);
-    cd* btmp = (cd*)w.get(W_BTMP, sizeof(cd) * n, &e);
-    LU_CHK(e);
+    const int* perm = ws.perm;
+    cd* btmp = ws.btmp;
     for (int pk = 0; pk < npanel; ++pk) {
         const int k0 = pk * NB, nb = std::min(NB, n - k0);
         hipLaunchKernelGGL(k_copy_cd, dim3((n + 255) / 256), dim3(256), 0, st, b, btmp, n);
@@ -703,6 +796,25 @@ static int solve_fp64(LuWork& w, hipStream_t st, int n, const cd* A, size_t lda,
                            b, btmp);
     }
     LU_CHK(hipGetLastError());
+    return BEMX_OK;
+}
+
+// INFO and the status word of the factorisation just enqueued (waits for it).
+static int factor_status(const Fp64Ws& ws, hipStream_t st, long long* info, std::string& err) {
+    long long hinfo = 0;
+    int hstatus = 0;
+    LU_CHK(hipMemcpyAsync(&hinfo, ws.info, sizeof(long long), hipMemcpyDeviceToHost, st));
+    LU_CHK(hipMemcpyAsync(&hstatus, ws.status, sizeof(int), hipMemcpyDeviceToHost, st));
+    LU_CHK(hipStreamSynchronize(st));
+    if (hstatus & ST_SYNC_TIMEOUT) {
+        err = "LU panel: inter-workgroup wait timed out";
+        return BEMX_ERR_TIMEOUT;
+    }
+    if (info) *info = hinfo;
+    if (hinfo > 0) {
+        err = "LU: exactly singular pivot (INFO > 0)";
+        return BEMX_ERR_SINGULAR;
+    }
     return BEMX_OK;
 }
 
@@ -722,29 +834,36 @@ int lu_solve(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, cd* b, int nrh
         // b are still intact
         if (stats) stats[2] = 1.0;
     }
-    int s = factor_fp64(w, st, n, A, lda, err);
+    Fp64Ws ws;
+    int s = fp64_ws_alloc(w, n, ws, err);
+    if (!s) s = factor_fp64(ws, st, n, A, lda, &w, err);
+    if (!s) s = factor_status(ws, st, info, err);
     if (s) return s;
-    long long hinfo = 0;
-    int hstatus = 0;
-    hipError_t e;
-    LU_CHK(hipMemcpyAsync(&hinfo, w.get(W_INFO, sizeof(long long), &e), sizeof(long long),
-                          hipMemcpyDeviceToHost, st));
-    LU_CHK(hipMemcpyAsync(&hstatus, w.get(W_STATUS, sizeof(int), &e), sizeof(int),
-                          hipMemcpyDeviceToHost, st));
-    LU_CHK(hipStreamSynchronize(st));
-    if (hstatus & ST_SYNC_TIMEOUT) {
-        err = "LU panel: inter-workgroup wait timed out";
-        return BEMX_ERR_TIMEOUT;
-    }
-    if (info) *info = hinfo;
-    if (hinfo > 0) {
-        err = "LU: exactly singular pivot (INFO > 0)";
-        return BEMX_ERR_SINGULAR;
-    }
     for (int j = first; j < nrhs; ++j) {
-        s = solve_fp64(w, st, n, A, lda, b + (size_t)j * n, err);
+        s = solve_fp64(ws, st, n, A, lda, b + (size_t)j * n, err);
         if (s) return s;
     }
+    return BEMX_OK;
+}
+
+int lu_solve_fp64_noalloc(int device, hipStream_t st, int n, cd* A, size_t lda, cd* b,
+                          long long* info, std::string& err) {
+    if (n < 1 || n > NOALLOC_MAX_N) {
+        err = "LU: order outside 1..32768, the static workspace's range";
+        return BEMX_ERR_ARG;
+    }
+    void* base = (device >= 0 && device < MAX_DEVICES) ? scratch_addr[device].load() : nullptr;
+    if (!base) {
+        err = "LU: the static workspace of this device is not set up (no context made on it)";
+        return BEMX_ERR_STATE;
+    }
+    Fp64Ws ws;
+    fp64_ws_carve((char*)base, n, ws);
+    int s = factor_fp64(ws, st, n, A, lda, nullptr, err);
+    if (!s) s = factor_status(ws, st, info, err);
+    if (!s) s = solve_fp64(ws, st, n, A, lda, b, err);
+    if (s) return s;
+    LU_CHK(hipStreamSynchronize(st));
     return BEMX_OK;
 }
 

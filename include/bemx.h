@@ -109,6 +109,21 @@ int bemx_ghmatecd(bemx_ctx *ctx, const double *zge, const double *zcs, const dou
  * info (may be NULL): LAPACK-style (0, or k > 0 for a zero pivot at k). */
 int bemx_linsolve(bemx_ctx *ctx, int nn, double *zh, double *zfi, int mode, long long *info);
 
+/* Dense FP64 solve of a device-resident system that allocates no device
+ * memory.  No counterpart in the reference: for callers that cannot call
+ * hipMalloc (tight serving loops, harnesses that forbid allocation).  It
+ * reads INFO back and synchronises the stream, so it is not for use under
+ * graph capture.  a: nn x nn complex, column-major with leading
+ * dimension lda >= nn, overwritten by its LU factors; b: nn complex,
+ * overwritten by the solution; both device pointers on the context's
+ * device.  ZGESV semantics and kernels of bemx_linsolve's FP64 path, on the
+ * context's stream; returns when the solve is done.  The pivoting
+ * workspace is a static device array of 18.4 MB per GPU, so nn <= 32768
+ * (larger orders: BEMX_ERR_ARG); calls that overlap on one GPU share that
+ * array and race.  info as bemx_linsolve. */
+int bemx_linsolve_device_noalloc(bemx_ctx *ctx, int nn, void *a, long long lda, void *b,
+                                 long long *info);
+
 /* Interior points.  Replaces INTEREC (Interec.for:9-288) = the KODE
  * un-swap of ZFI/ZDFI (in place, as the reference), cuda_interec1_
  * (kernels/Interec1_cu.cu:44-237) for ZDSOL(3L) and the SIGMAEC stress loop
