@@ -738,67 +738,89 @@ __global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag_h(int nbe, int npg2,
 // Near-field part of the H blocks that k_dyn_offdiag_h leaves out: the
 // Gauss points with |ks| R < TAU_OFF, where the closed form of the kernel
 // scalars loses digits, summed from their series and added to ZH (same
-// factored accumulation, same columns).  A workgroup covers the rows and
-// the tile of column elements of one k_dyn_offdiag_h workgroup; it returns
-// at once when no row is within TAU_OFF / |ks| of an element of the tile
-// (every tile but the diagonal neighbourhood once |ks| h >~ 2), and a wave
-// skips an element none of its rows can reach.
-template <int WAVES, int JT>
-__global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag_near(int nbe, int npg2,
-                                                                 const ElemData* __restrict__ elem,
-                                                                 const GaussPt* __restrict__ gp,
-                                                                 const int* __restrict__ colflag,
-                                                                 DynConst K, cd* __restrict__ zh,
-                                                                 size_t ldzh) {
-    extern __shared__ GaussPt sgp[];
-    const int j0 = blockIdx.y * JT;
-    const int jn = min(JT, nbe - j0);
-    const int I = blockIdx.x * (64 * WAVES) + threadIdx.x;
-    const bool rowok = I < nbe;
-    const int Ic = rowok ? I : nbe - 1;
-    const double xc = elem[Ic].cx, yc = elem[Ic].cy, zc = elem[Ic].cz;
-    // reach^2 with a margin: a Gauss point of J lies within rad_J of its
-    // collocation point
-    const double reach = TAU_OFF / sqrt(K.ks_abs2);
-    int nearmask = 0;
-    for (int jj = 0; jj < jn; ++jj) {
-        const ElemData e = elem[j0 + jj];
-        const double d = sqrt(dist2(e.cx - xc, e.cy - yc, e.cz - zc));
-        if (rowok && I != j0 + jj && d < reach + e.pad0 + 1e-9 * (reach + d)) nearmask |= 1 << jj;
+// factored accumulation, same columns).  A workgroup takes a tile of
+// NF_T rows x NF_T column elements, keeps the pairs (I, J) whose element J
+// has a Gauss point within TAU_OFF / |ks| of row I's collocation point
+// (centroid distance < reach + the element's Gauss-point radius) in an LDS
+// list, and gives each listed pair to one lane, which loops over J's Gauss
+// points.  Tiles far from the diagonal band keep no pair and end after the
+// distance tests; the pair order in the list does not affect results (each
+// pair's sum is one lane's, in Gauss-point order).
+constexpr int NF_T = 32;
+__global__ __launch_bounds__(256) void k_dyn_offdiag_near(int nbe, int npg2,
+                                                          const ElemData* __restrict__ elem,
+                                                          const GaussPt* __restrict__ gp,
+                                                          const int* __restrict__ colflag,
+                                                          DynConst K, cd* __restrict__ zh,
+                                                          size_t ldzh) {
+    __shared__ double rx[NF_T], ry[NF_T], rz[NF_T];
+    __shared__ double jx[NF_T], jy[NF_T], jz[NF_T], jr[NF_T];
+    __shared__ unsigned short list[NF_T * NF_T];
+    __shared__ int count;
+    const int i0 = blockIdx.x * NF_T, j0 = blockIdx.y * NF_T;
+    const int t = threadIdx.x;
+    if (t < NF_T) {
+        const ElemData e = elem[min(i0 + t, nbe - 1)];
+        rx[t] = e.cx;
+        ry[t] = e.cy;
+        rz[t] = e.cz;
+    } else if (t < 2 * NF_T) {
+        const ElemData e = elem[min(j0 + t - NF_T, nbe - 1)];
+        jx[t - NF_T] = e.cx;
+        jy[t - NF_T] = e.cy;
+        jz[t - NF_T] = e.cz;
+        jr[t - NF_T] = e.pad0;
     }
-    if (!__syncthreads_or(nearmask)) return;
-    for (int t = threadIdx.x; t < jn * npg2; t += 64 * WAVES) {
-        GaussPt g = gp[(size_t)j0 * npg2 + t];
-        g.w *= K.q4pi;
-        sgp[t] = g;
+    if (t == 0) count = 0;
+    __syncthreads();
+    const double reach = TAU_OFF / sqrt(K.ks_abs2);
+    // NF_T^2 / 256 rounds with every lane active (the ballot needs them all)
+    for (int c = t; c < NF_T * NF_T; c += 256) {
+        const int il = c % NF_T, jl = c / NF_T;
+        const int I = i0 + il, J = j0 + jl;
+        bool nf = false;
+        if (I < nbe && J < nbe && I != J && (colflag[J] & 14) != 14) {
+            const double d = sqrt(dist2(jx[jl] - rx[il], jy[jl] - ry[il], jz[jl] - rz[il]));
+            nf = d < reach + jr[jl] + 1e-9 * (reach + d);
+        }
+        const unsigned long long m = __ballot(nf);
+        if (m) {
+            int base = 0;
+            if ((t & 63) == 0) base = atomicAdd(&count, __popcll(m));
+            base = __shfl(base, 0);
+            const int below = __builtin_amdgcn_mbcnt_hi(
+                (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            if (nf) list[base + below] = (unsigned short)c;
+        }
     }
     __syncthreads();
-    for (int jj = 0; jj < jn; ++jj) {
-        if (!__any((nearmask >> jj) & 1)) continue;
-        const int J = j0 + jj;
+    const int npairs = count;
+    for (int k = t; k < npairs; k += 256) {
+        const int c = list[k];
+        const int il = c % NF_T, jl = c / NF_T;
+        const int I = i0 + il, J = j0 + jl;
+        const double xc = rx[il], yc = ry[il], zc = rz[il];
         const int flag = colflag[J];
         const double nv[3] = {elem[J].nx, elem[J].ny, elem[J].nz};
-        const GaussPt* sg = sgp + jj * npg2;
+        const GaussPt* g = gp + (size_t)J * npg2;
         cd SP[3], SC[3], SB[6];
-        for (int k = 0; k < 3; ++k) SP[k] = SC[k] = mk(0, 0);
-        for (int k = 0; k < 6; ++k) SB[k] = mk(0, 0);
-        bool any = false;
+        for (int q = 0; q < 3; ++q) SP[q] = SC[q] = mk(0, 0);
+        for (int q = 0; q < 6; ++q) SB[q] = mk(0, 0);
         for (int q = 0; q < npg2; ++q) {
-            GaussPt p = sg[q];
-            double r1 = p.x - xc, r2 = p.y - yc, r3 = p.z - zc;
+            const GaussPt p = g[q];
+            const double r1 = p.x - xc, r2 = p.y - yc, r3 = p.z - zc;
             const double RR = dist2(r1, r2, r3);
             if (!near_field(K, RR)) continue;
-            any = true;
             const double iR = rsqrt_refined(RR);
             const double R = RR * iR;
-            double rd[3] = {r1 * iR, r2 * iR, r3 * iR};
-            double drn = rd[0] * nv[0] + rd[1] * nv[1] + rd[2] * nv[2];
+            const double rd[3] = {r1 * iR, r2 * iR, r3 * iR};
+            const double drn = rd[0] * nv[0] + rd[1] * nv[1] + rd[2] * nv[2];
             cd chiR, psidR2, chidR2, unused, A, B, C;
             ser_sums<NSER_OFF, 0, false>(K, K.ks * R, K.ser_rtop_off, chiR, psidR2, chidR2,
                                          unused);
             abc_from(K, chiR, psidR2, chidR2, A, B, C);
-            double wq = p.w * (iR * iR);
-            cd Ap = A * wq, Bp = B * (wq * drn), Cp = C * wq;
+            const double wq = (p.w * K.q4pi) * (iR * iR);
+            const cd Ap = A * wq, Bp = B * (wq * drn), Cp = C * wq;
             for (int j = 0; j < 3; ++j) {
                 SP[j] = SP[j] + Ap * rd[j];
                 SC[j] = SC[j] + Cp * rd[j];
@@ -810,15 +832,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag_near(int nbe, int np
             SB[4] = SB[4] + Bp * (rd[2] * rd[1]);
             SB[5] = SB[5] + Bp * (rd[2] * rd[2]);
         }
-        if (!(any && rowok && I != J)) continue;
         const cd SD = SP[0] * nv[0] + SP[1] * nv[1] + SP[2] * nv[2];
         const int sidx[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
-        for (int k = 0; k < 3; ++k) {
-            if ((flag >> (1 + k)) & 1) continue;
-            cd* col = zh + (size_t)(3 * J + k) * ldzh + 3 * I;
+        for (int kk = 0; kk < 3; ++kk) {
+            if ((flag >> (1 + kk)) & 1) continue;
+            cd* col = zh + (size_t)(3 * J + kk) * ldzh + 3 * I;
             for (int i = 0; i < 3; ++i) {
-                cd h = SB[sidx[i + 3 * k]] + SC[i] * nv[k] + SP[k] * nv[i];
-                if (i == k) h = h + SD;
+                cd h = SB[sidx[i + 3 * kk]] + SC[i] * nv[kk] + SP[kk] * nv[i];
+                if (i == kk) h = h + SD;
                 col[i] = col[i] + h;
             }
         }
@@ -1201,8 +1222,9 @@ hipError_t launch_dyn_assembly(hipStream_t st, int nbe, int npg2, const ElemData
     if (tab) {
         hipLaunchKernelGGL((k_dyn_offdiag_h<OD_WAVES, OD_JT, true>), odgrid, odblock, odlds, st,
                            nbe, npg2, elem, gp, colflag, K, T, zh, ldzh);
-        hipLaunchKernelGGL((k_dyn_offdiag_near<OD_WAVES, OD_JT>), odgrid, odblock, odlds, st,
-                           nbe, npg2, elem, gp, colflag, K, zh, ldzh);
+        const int nft = (nbe + NF_T - 1) / NF_T;
+        hipLaunchKernelGGL(k_dyn_offdiag_near, dim3(nft, nft), dim3(256), 0, st, nbe, npg2, elem,
+                           gp, colflag, K, zh, ldzh);
     } else
         hipLaunchKernelGGL((k_dyn_offdiag_h<OD_WAVES, OD_JT, false>), odgrid, odblock, odlds, st,
                            nbe, npg2, elem, gp, colflag, K, T, zh, ldzh);
