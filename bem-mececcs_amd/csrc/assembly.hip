@@ -1222,13 +1222,15 @@ hipError_t launch_dyn_assembly(hipStream_t st, int nbe, int npg2, const ElemData
     if (tab) {
         hipLaunchKernelGGL((k_dyn_offdiag_h<OD_WAVES, OD_JT, true>), odgrid, odblock, odlds, st,
                            nbe, npg2, elem, gp, colflag, K, T, zh, ldzh);
+        if (ev_od1) (void)hipEventRecord(ev_od1, st);
         const int nft = (nbe + NF_T - 1) / NF_T;
         hipLaunchKernelGGL(k_dyn_offdiag_near, dim3(nft, nft), dim3(256), 0, st, nbe, npg2, elem,
                            gp, colflag, K, zh, ldzh);
-    } else
+    } else {
         hipLaunchKernelGGL((k_dyn_offdiag_h<OD_WAVES, OD_JT, false>), odgrid, odblock, odlds, st,
                            nbe, npg2, elem, gp, colflag, K, T, zh, ldzh);
-    if (ev_od1) (void)hipEventRecord(ev_od1, st);
+        if (ev_od1) (void)hipEventRecord(ev_od1, st);
+    }
     hipLaunchKernelGGL((k_dyn_offdiag_g<OG_JT>),
                        dim3((nbe + 63) / 64, std::min(offdiag_tiles(nbe), 64)),
                        dim3(64 * OG_JT), (size_t)OG_JT * npg2 * sizeof(GaussPt), st, nbe, npg2,

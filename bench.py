@@ -53,11 +53,24 @@ FLOPS_PER_TRIPLE = 242
 FP64_PEAK_TFLOPS = 78.6    # MI355X dense FP64 (vector ALU and matrix cores alike)
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix cores
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch on the 48 x 45 patch from the rocprofv3 FETCH_SIZE /
-# WRITE_SIZE runs (each scaled by the counter's reading for a streaming copy
-# of known size), recorded under profiles/.
-OFFDIAG_TRAFFIC_BYTES = 720.0e6
-FACTOR_TRAFFIC_BYTES = None
+# HBM bytes on the 48 x 45 patch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+# runs of `bench.py --workload solve` (FETCH_SIZE x 2 and WRITE_SIZE x 1: the
+# factors that a streaming copy of known size, k_cvt_z2c, reads back), per
+# k_dyn_offdiag_h launch and per factorisation (every kernel of it), recorded
+# under profiles/.
+OFFDIAG_TRAFFIC_BYTES = 723.4e6
+FACTOR_TRAFFIC_BYTES = 10.22e9
+FP64_FACTOR_TRAFFIC_BYTES = 65.23e9
+# Counter figures of the same runs: FP64 VALU flops counted by
+# SQ_INSTS_VALU_FLOPS_FP64 (per wave instruction, x 64 lanes) per
+# k_dyn_offdiag_h launch, and MFMA flops (SQ_INSTS_VALU_MFMA_MOPS_* x 512)
+# and MFMA-busy SIMD cycles (SQ_VALU_MFMA_BUSY_CYCLES) per factorisation.
+OFFDIAG_COUNTED_FLOPS = 92.8e9
+FACTOR_COUNTED_FLOPS = 2261.3e9
+FACTOR_MFMA_BUSY_CYCLES = 2208.3e6
+FP64_FACTOR_COUNTED_FLOPS = 731.0e9
+FP64_FACTOR_MFMA_BUSY_CYCLES = 22842.6e6
+SIMDS, CLOCK_HZ = 1024, 2.4e9  # MI355X: 256 CUs x 4 SIMDs, peak engine clock
 # Speed of the reference's own CPU GHMATECD (its NONSINGD/SING_DE, built with
 # its Makefile's -Ofast -march=native -fopenmp) relative to the C oracle,
 # measured once on identical hardware (oracle/calibrate.py; profiles/
@@ -193,19 +206,27 @@ def single_solve(args, api, decks, steps):
                "frac": bf16_flops / (t_fac * 1e-3) / 1e12 / BF16_PEAK_TFLOPS,
                "traffic": FACTOR_TRAFFIC_BYTES, "avg_ms": t_fac,
                "algorithmic_flops": bf16_flops,
+               "counted_mfma_flops": FACTOR_COUNTED_FLOPS,
+               "mfma_busy": FACTOR_MFMA_BUSY_CYCLES / (SIMDS * CLOCK_HZ * t_fac * 1e-3),
                "note": "device time of the whole factorisation from HIP events around its graph "
-                       "launch; flops = 3 BF16 products x 8n^3/3, n = 6480"}
+                       "launch; flops = 3 BF16 products x 8n^3/3, n = 6480; traffic, counted "
+                       "flops and MFMA-busy cycles (over 1024 SIMDs at 2.4 GHz) from the "
+                       "rocprofv3 counters"}
     roof_fp64 = {"kernel": "FP64 factorisation (k_panel_fp64, k_zgemm_mfma, ...)",
                  "bound": "mfma", "achieved": fp64_flops / (t_fac64 * 1e-3) / 1e12,
                  "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                  "frac": fp64_flops / (t_fac64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
-                 "avg_ms": t_fac64, "algorithmic_flops": fp64_flops}
+                 "avg_ms": t_fac64, "algorithmic_flops": fp64_flops,
+                 "traffic": FP64_FACTOR_TRAFFIC_BYTES, "counted_mfma_flops": FP64_FACTOR_COUNTED_FLOPS,
+                 "mfma_busy": FP64_FACTOR_MFMA_BUSY_CYCLES / (SIMDS * CLOCK_HZ * t_fac64 * 1e-3)}
     ach = FLOPS_PER_TRIPLE * triples / (t_od * 1e-3) / 1e12
     roof_asm = {"kernel": "k_dyn_offdiag_h (dynamic H assembly)", "bound": "valu",
                 "pipe": "FP64 vector ALU", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
                 "traffic": OFFDIAG_TRAFFIC_BYTES, "avg_ms": t_od,
-                "flops_per_triple": FLOPS_PER_TRIPLE, "triples": triples}
+                "flops_per_triple": FLOPS_PER_TRIPLE, "triples": triples,
+                "counted_valu_flops": OFFDIAG_COUNTED_FLOPS,
+                "counted_frac": OFFDIAG_COUNTED_FLOPS / (t_od * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
     return rec, roof_lu, roof_fp64, roof_asm
 
 
