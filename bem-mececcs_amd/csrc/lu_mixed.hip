@@ -499,6 +499,25 @@ __device__ inline cd ll_wait(u64* __restrict__ ll, int i, int tag, int* status) 
               __longlong_as_double((long long)(((w3 & 0xffffffffull) << 32) | (w2 & 0xffffffffull))));
 }
 
+// The first poll of entry i's hand-off words, issued ahead of other loads
+// (a wave's loads complete in issue order, so a poll issued behind a factor
+// tile would wait for the tile); ll_finish takes it if every lane's words
+// carry the tag and polls on (ll_wait) otherwise.
+__device__ inline void ll_issue(u64* __restrict__ ll, int i, u64 (&q)[4]) {
+    u64* p = ll + 4 * (size_t)i;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) q[j] = __hip_atomic_load(p + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline cd ll_finish(u64* __restrict__ ll, int i, int tag, int* status, const u64 (&q)[4]) {
+    const unsigned t = (unsigned)tag;
+    const bool ok = (unsigned)(q[0] >> 32) == t && (unsigned)(q[1] >> 32) == t &&
+                    (unsigned)(q[2] >> 32) == t && (unsigned)(q[3] >> 32) == t;
+    if (!__all(ok)) return ll_wait(ll, i, tag, status);
+    return mk(__longlong_as_double((long long)(((q[1] & 0xffffffffull) << 32) | (q[0] & 0xffffffffull))),
+              __longlong_as_double((long long)(((q[3] & 0xffffffffull) << 32) | (q[2] & 0xffffffffull))));
+}
+
 // sum_i f[i] * y_i over 16 columns (FP32 factor entries, FP64 vector), y_i
 // held by lane i (i < 16) and broadcast with v_readlane.
 __device__ inline cd dot16(const cf (&f)[16], cd yl) {
@@ -523,16 +542,20 @@ __device__ inline cd sum8(const cd (*part)[NBM], int lane) {
 // of 64 rows (one lane per row; wave w takes the 16 columns [16 w, 16 w + 16)
 // of every block it multiplies), so the product that waits on the previous
 // block is 64 x 128 per workgroup.  Workgroup 2 g + h owns row half h of
-// blocks g, g + G, ...  Per block: the contributions of the finished blocks
-// as their flags come up (the block two back through its hand-off words),
-// the halves of the accumulated right-hand side exchanged with the partner
-// workgroup through hand-off words (acc_ll), the product with the block's
-// 128 x 128 inverse, and the coupling to the previous block as its hand-off
-// words arrive; one reduction, then this half's solution goes out as
-// hand-off words and plain stores, and flags[2 blk + h] is raised.  Blocks
-// only wait on lower-numbered blocks and on the partner half, which
-// publishes before it waits, and every workgroup is resident (cooperative
-// launch), so the chain always drains.
+// blocks g, g + G, ...  Per block: the block's inverse rows, its coupling
+// rows and its right-hand side are loaded first (none depends on the
+// chain); then the contributions of the finished blocks, each solution
+// block read from its tagged hand-off words (one memory round trip, no
+// flag, fence and second load), polled before the next factor tile is
+// requested and with that tile loaded one block ahead; the halves of the
+// accumulated right-hand side exchanged with the partner workgroup through
+// hand-off words (acc_ll); the product with the block's 128 x 128 inverse,
+// and the coupling to the previous block as its hand-off words arrive; one
+// reduction, then this half's solution goes out as hand-off words and plain
+// stores, and flags[2 blk + h] is raised (the upper solve waits on it before
+// reading this block of y).  Blocks only wait on lower-numbered blocks and on
+// the partner half, which publishes before it waits, and every workgroup is
+// resident (cooperative launch), so the chain always drains.
 __device__ __forceinline__ void trsv_lower_body(const cf* __restrict__ A, size_t lda, int n,
                                                 int nblk2, const cf* __restrict__ Minv,
                                                 const cf* __restrict__ Wc,
@@ -541,37 +564,49 @@ __device__ __forceinline__ void trsv_lower_body(const cf* __restrict__ A, size_t
                                                 int* __restrict__ flags, int epoch,
                                                 int* __restrict__ status, cd (*part)[NBM],
                                                 cd* accs) {
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, hw = w >> 2;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int h = blockIdx.x & 1, G = gridDim.x >> 1;
     for (int blk = blockIdx.x >> 1; blk < nblk2; blk += G) {
         const int r0 = blk * NB2, nb = min(NB2, n - r0);
         const int row = NBM * h + lane;  // row within the block
         const bool rok = row < nb;
         const int rowc = r0 + min(row, nb - 1);
-        cf mv[16];
+        cf mv[16], wv[16];
         const cf* Mb = Minv + (size_t)blk * NB2 * NB2 + (size_t)(16 * w) * NB2 + row;
 #pragma unroll
         for (int i = 0; i < 16; ++i) mv[i] = Mb[(size_t)i * NB2];
+        if (blk > 0) {
+            const cf* Wb = Wc + (size_t)blk * NB2 * NB2 + (size_t)(16 * w) * NB2 + row;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) wv[i] = Wb[(size_t)i * NB2];
+        }
+        const cd bv = (w == 0 && rok) ? b[r0 + row] : mk(0, 0);
         cd s = mk(0, 0);
-        for (int k = 0; k < blk - 1; ++k) {
-            const int c0 = k * NB2 + 16 * w;
-            const cf* Lc = A + (size_t)c0 * lda + rowc;
-            cf l[16];  // the factor tile does not depend on the flag: load it first
+        const int nk = blk - 1;  // blocks 0 .. blk - 2
+        auto tile = [&](int k, cf (&l)[16]) {
+            const cf* Lc = A + (size_t)(k * NB2 + 16 * w) * lda + rowc;
 #pragma unroll
             for (int i = 0; i < 16; ++i) l[i] = Lc[(size_t)i * lda];
-            cd yk;
-            if (k == blk - 2) {
-                yk = ll_wait(yll, c0 + (lane & 15), epoch, status);
-            } else {
-                wave_wait_ge(&flags[2 * k + hw], epoch, status);
-                yk = y[c0 + (lane & 15)];
-            }
-            s = s + dot16(l, yk);
+        };
+        auto step = [&](int k, const cf (&cur)[16], cf (&nxt)[16]) {
+            const int i = k * NB2 + 16 * w + (lane & 15);
+            u64 q[4];
+            ll_issue(yll, i, q);
+            if (k + 1 < nk) tile(k + 1, nxt);
+            s = s + dot16(cur, ll_finish(yll, i, epoch, status, q));
+        };
+        cf la[16], lb[16];
+        if (nk > 0) tile(0, la);
+        int k = 0;
+        for (; k + 1 < nk; k += 2) {
+            step(k, la, lb);
+            step(k + 1, lb, la);
         }
+        if (k < nk) step(k, la, lb);
         part[w][lane] = s;
         __syncthreads();
         if (w == 0) {
-            const cd a = rok ? b[r0 + row] - sum8(part, lane) : mk(0, 0);
+            const cd a = rok ? bv - sum8(part, lane) : mk(0, 0);
             accs[row] = a;
             ll_put(acc_ll, r0 + row, a, epoch);
         } else if (w == 1) {
@@ -582,13 +617,8 @@ __device__ __forceinline__ void trsv_lower_body(const cf* __restrict__ A, size_t
         cd t = mk(0, 0);
 #pragma unroll
         for (int i = 0; i < 16; ++i) t = t + up(mv[i]) * accs[16 * w + i];
-        if (blk > 0) {
-            cf wv[16];  // coupling to the previous block
-            const cf* Wb = Wc + (size_t)blk * NB2 * NB2 + (size_t)(16 * w) * NB2 + row;
-#pragma unroll
-            for (int i = 0; i < 16; ++i) wv[i] = Wb[(size_t)i * NB2];
+        if (blk > 0)  // coupling to the previous block
             t = t - dot16(wv, ll_wait(yll, (blk - 1) * NB2 + 16 * w + (lane & 15), epoch, status));
-        }
         part[w][lane] = t;
         __syncthreads();
         if (w == 0 && rok) {
@@ -603,11 +633,16 @@ __device__ __forceinline__ void trsv_lower_body(const cf* __restrict__ A, size_t
     }
 }
 
-// x = U^-1 y over 128-row blocks (in reverse order), same scheme.
-// ylow: epoch of the lower solve that produced yv; this half's entries of
-// yv are read after its lower flag reached ylow (x may alias the lower
-// solve's right-hand side: those entries were read before that flag was
-// set).
+// x = U^-1 y over 128-row blocks (in reverse order), same scheme; the
+// solution blocks to the right come from their hand-off words (a partial
+// last block, n not a multiple of 128, is taken first, outside the
+// pipelined loop).  ylow: epoch of the lower solve that produced yv; this
+// half's entries of yv are read after its lower flag reached ylow (x may
+// alias the lower solve's right-hand side: those entries were read before
+// that flag was set).  The hand-off words of block blk hold y (tag ylow)
+// until this block writes x into them: every lower block that reads them
+// has finished by then (each upper block starts after its own lower block,
+// and block blk needs every upper block to its right).
 __device__ __forceinline__ void trsv_upper_body(const cf* __restrict__ A, size_t lda, int n,
                                                 int nblk2, const cf* __restrict__ Minv,
                                                 const cf* __restrict__ Wc,
@@ -616,48 +651,58 @@ __device__ __forceinline__ void trsv_upper_body(const cf* __restrict__ A, size_t
                                                 int* __restrict__ flags, int epoch, int ylow,
                                                 int* __restrict__ status, cd (*part)[NBM],
                                                 cd* accs) {
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, hw = w >> 2;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int h = blockIdx.x & 1, G = gridDim.x >> 1;
+    const int klast = nblk2 - 1;
+    const int kblast = n - klast * NB2;  // columns of the last block
     for (int bi = blockIdx.x >> 1; bi < nblk2; bi += G) {
         const int blk = nblk2 - 1 - bi;
         const int r0 = blk * NB2, nb = min(NB2, n - r0);
         const int row = NBM * h + lane;
         const bool rok = row < nb;
         const int rowc = r0 + min(row, nb - 1);
-        cf mv[16];
+        cf mv[16], wv[16];
         const cf* Mb = Minv + (size_t)blk * NB2 * NB2 + (size_t)(16 * w) * NB2 + row;
 #pragma unroll
         for (int i = 0; i < 16; ++i) mv[i] = Mb[(size_t)i * NB2];
-        cd s = mk(0, 0);
-        for (int k = nblk2 - 1; k > blk + 1; --k) {
-            const int kb = min(NB2, n - k * NB2);
-            const int c0 = k * NB2 + 16 * w;
-            const int cnt = min(16, kb - 16 * w);  // columns of this wave in block k
-            if (cnt <= 0) continue;
-            const cf* Uc = A + (size_t)c0 * lda + rowc;
-            if (cnt == 16) {
-                cf u[16];  // the factor tile does not depend on the flag: load it first
+        if (blk + 1 < nblk2) {
+            const cf* Wb = Wc + (size_t)blk * NB2 * NB2 + (size_t)(16 * w) * NB2 + row;
 #pragma unroll
-                for (int i = 0; i < 16; ++i) u[i] = Uc[(size_t)i * lda];
-                cd xk;  // the block two ahead through its hand-off words
-                if (k == blk + 2) {
-                    xk = ll_wait(yll, c0 + (lane & 15), epoch, status);
-                } else {
-                    wave_wait_ge(&flags[2 * k + hw], epoch, status);
-                    xk = x[c0 + (lane & 15)];
-                }
-                s = s + dot16(u, xk);
-            } else {
-                cd xk;
-                if (k == blk + 2) {
-                    xk = ll_wait(yll, c0 + min(lane & 15, cnt - 1), epoch, status);
-                } else {
-                    wave_wait_ge(&flags[2 * k + hw], epoch, status);
-                    xk = x[c0 + min(lane & 15, cnt - 1)];
-                }
+            for (int i = 0; i < 16; ++i) wv[i] = Wb[(size_t)i * NB2];
+        }
+        cd s = mk(0, 0);
+        int khi = klast;  // first block of the pipelined loop (full blocks)
+        if (kblast < NB2) {
+            khi = klast - 1;
+            const int c0 = klast * NB2 + 16 * w;
+            const int cnt = min(16, kblast - 16 * w);  // columns of this wave
+            if (klast > blk + 1 && cnt > 0) {
+                const cd xk = ll_wait(yll, c0 + min(lane & 15, cnt - 1), epoch, status);
+                const cf* Uc = A + (size_t)c0 * lda + rowc;
                 for (int i = 0; i < cnt; ++i) s = s + up(Uc[(size_t)i * lda]) * readlane_c(xk, i);
             }
         }
+        const int nk = khi - (blk + 1);  // full blocks khi .. blk + 2
+        auto tile = [&](int j, cf (&u)[16]) {
+            const cf* Uc = A + (size_t)((khi - j) * NB2 + 16 * w) * lda + rowc;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) u[i] = Uc[(size_t)i * lda];
+        };
+        auto step = [&](int j, const cf (&cur)[16], cf (&nxt)[16]) {
+            const int i = (khi - j) * NB2 + 16 * w + (lane & 15);
+            u64 q[4];
+            ll_issue(yll, i, q);
+            if (j + 1 < nk) tile(j + 1, nxt);
+            s = s + dot16(cur, ll_finish(yll, i, epoch, status, q));
+        };
+        cf ua[16], ub[16];
+        if (nk > 0) tile(0, ua);
+        int j = 0;
+        for (; j + 1 < nk; j += 2) {
+            step(j, ua, ub);
+            step(j + 1, ub, ua);
+        }
+        if (j < nk) step(j, ua, ub);
         wave_wait_ge(&flags[2 * blk + h], ylow, status);
         part[w][lane] = s;
         __syncthreads();
@@ -676,10 +721,6 @@ __device__ __forceinline__ void trsv_upper_body(const cf* __restrict__ A, size_t
         if (blk + 1 < nblk2) {
             // columns of a partial next block have zero coupling; their
             // index is clamped into x
-            cf wv[16];
-            const cf* Wb = Wc + (size_t)blk * NB2 * NB2 + (size_t)(16 * w) * NB2 + row;
-#pragma unroll
-            for (int i = 0; i < 16; ++i) wv[i] = Wb[(size_t)i * NB2];
             t = t - dot16(wv, ll_wait(yll, min((blk + 1) * NB2 + 16 * w + (lane & 15), n - 1),
                                       epoch, status));
         }
