@@ -123,12 +123,6 @@ constexpr int NB = 64;          // panel width (FP64 path)
 constexpr int PANEL_ROWS = 128;  // rows of the panel per workgroup (128 x 64 x 16 B = 128 KB LDS)
 constexpr int SPIN_LIMIT = 1 << 22;
 
-struct Cand {
-    double val;
-    int row;
-    int pad;
-};
-
 __device__ inline double cabs1(cd a) { return fabs(a.x) + fabs(a.y); }
 
 // (val, row) ordering: larger val wins; equal val -> smaller row.
@@ -136,18 +130,54 @@ __device__ inline bool better(double v, int r, double bv, int br) {
     return v > bv || (v == bv && r < br);
 }
 
-// Bounded wait until *cnt >= target.  Gives up (and flags ST_SYNC_TIMEOUT)
-// after SPIN_LIMIT polls, or at once if an earlier wait already gave up, so
-// every wave of the grid always runs to its end.
-__device__ inline void wait_count(int* cnt, int target, int* status) {
+// Tagged exchange words of the panel: every 64-bit word carries the panel
+// launch's tag in its high half and 32 bits of data in its low half, and is
+// written and read as one relaxed agent-scope atomic, so a reader that sees
+// the tag in every word it needs has the data: no counter, fence or second
+// read.  A column's candidate slot is three words (the two halves of
+// |re|+|im|, the row); a row entry is four (the halves of re and im).  Tags
+// are unique in the process (tag_block), so words left from an earlier
+// panel or in reused memory never carry the tag being waited for.
+typedef unsigned long long u64;
+__device__ inline void tput(u64* p, unsigned tag, unsigned v) {
+    __hip_atomic_store(p, ((u64)tag << 32) | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void tput_d(u64* p, unsigned tag, double v) {
+    const u64 b = (u64)__double_as_longlong(v);
+    tput(p, tag, (unsigned)b);
+    tput(p + 1, tag, (unsigned)(b >> 32));
+}
+__device__ inline u64 tget(const u64* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline bool tok(u64 w, unsigned tag) { return (unsigned)(w >> 32) == tag; }
+__device__ inline double tval_d(u64 lo, u64 hi) {
+    return __longlong_as_double((long long)(((hi & 0xffffffffull) << 32) | (lo & 0xffffffffull)));
+}
+
+// Bounded poll of n consecutive tagged words at p (n <= 4) until all carry
+// the tag; gives up (and flags ST_SYNC_TIMEOUT) after SPIN_LIMIT polls, or
+// at once if an earlier wait already gave up, so every wave of the grid
+// always runs to its end.  Lanes poll their own words; the loop ends when
+// every lane of the wave has its words.
+template <int N>
+__device__ inline void tpoll(const u64* p, unsigned tag, u64 (&w)[N], bool active, int* status) {
     int it = 0;
-    // relaxed polls; the caller issues one acquire fence after the wait
-    while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        if (++it > SPIN_LIMIT ||
-            (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
-             ST_SYNC_TIMEOUT)) {
+    for (;;) {
+        bool ok = true;
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) w[i] = tget(p + i);
+#pragma unroll
+            for (int i = 0; i < N; ++i) ok = ok && tok(w[i], tag);
+        }
+        if (__all(ok)) return;
+        if ((++it & 63) == 0 &&
+            (it > SPIN_LIMIT ||
+             (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+              ST_SYNC_TIMEOUT))) {
             atomicOr(status, ST_SYNC_TIMEOUT);
-            break;
+            return;
         }
         __builtin_amdgcn_s_sleep(1);
     }
@@ -156,12 +186,13 @@ __device__ inline void wait_count(int* cnt, int target, int* status) {
 // ---------------------------------------------------------------------------
 // Panel factorisation.  Workgroup p owns panel rows [r0, r1) (global row
 // indices, r0 = k0 + p*PANEL_ROWS) held column-major in LDS.  Column j
-// (diagonal row d = k0 + j) costs one inter-workgroup exchange:
-//   wait   all P workgroups have published their candidate for column j
-//          (best |re|+|im| among owned rows >= d, lowest row on ties, its
-//          complete row) and the owner of row d has published row d;
-//   pick   every workgroup reads the P candidates, picks the same winner and
-//          reads the winner's row and row d (one round of global reads);
+// (diagonal row d = k0 + j) costs one inter-workgroup exchange through
+// tagged words:
+//   pick   wave 0 polls the P candidate slots of column j (best |re|+|im|
+//          among each workgroup's owned rows >= d, lowest row on ties),
+//          picks the winner, the same in every workgroup, and polls the
+//          winner's row; wave 1 meanwhile polls row d, which its owner
+//          publishes (two memory round trips);
 //   apply  swap rows d and piv where owned, scale column j, update column
 //          j+1 (only), find the local candidate for column j+1, complete
 //          that row and row d+1 for the columns after j+1 and publish them;
@@ -173,17 +204,16 @@ __device__ inline void wait_count(int* cnt, int target, int* status) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t lda, int n, int k0,
                                                     int nb, int* __restrict__ ipiv,
-                                                    int* __restrict__ counters,
-                                                    Cand* __restrict__ slots,
-                                                    cd* __restrict__ cand, cd* __restrict__ drow,
+                                                    u64* __restrict__ slots,
+                                                    u64* __restrict__ cand, u64* __restrict__ drow,
                                                     int* __restrict__ perm,
                                                     long long* __restrict__ info,
-                                                    int* __restrict__ status) {
+                                                    int* __restrict__ status, unsigned tag) {
     __shared__ cd sA[NB * PANEL_ROWS];
     __shared__ cd urow[NB], dbuf[NB];
     __shared__ double rv[4];
     __shared__ int rr[4];
-    __shared__ int s_piv, s_wp, s_cand;
+    __shared__ int s_piv, s_cand;
     __shared__ int s_pivs[NB];  // pivot row of each column
     __shared__ double s_pval, s_cval;
     const int P = gridDim.x, p = blockIdx.x, tid = threadIdx.x;
@@ -229,21 +259,28 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
         }
         __syncthreads();
     };
-    // publish the candidate row and row k0+jj (complete for column jj on) of step jj
+    // publish the candidate row and row k0+jj (complete for column jj on) of
+    // step jj, and the candidate slot; each word stands alone, so no barrier
+    // or fence orders them
     auto publish = [&](int jj) {
         const int myrow = s_cand;
-        if (myrow != 0x7fffffff && tid < nb)
-            cand[((size_t)jj * P + p) * NB + tid] = sA[tid * PANEL_ROWS + (myrow - r0)];
+        if (myrow != 0x7fffffff && tid < nb) {
+            const cd v = sA[tid * PANEL_ROWS + (myrow - r0)];
+            u64* q = cand + (((size_t)jj * P + p) * NB + tid) * 4;
+            tput_d(q, tag, v.x);
+            tput_d(q + 2, tag, v.y);
+        }
         const int d = k0 + jj;
-        if (d >= r0 && d < r1 && tid >= 64 && tid < 64 + nb)
-            drow[(size_t)jj * NB + tid - 64] = sA[(tid - 64) * PANEL_ROWS + (d - r0)];
-        // the barrier completes this workgroup's stores; the release add of
-        // the count then makes them (and the slot) visible at device scope
-        // with a single L2 write-back
-        __syncthreads();
+        if (d >= r0 && d < r1 && tid >= 64 && tid < 64 + nb) {
+            const cd v = sA[(tid - 64) * PANEL_ROWS + (d - r0)];
+            u64* q = drow + ((size_t)jj * NB + tid - 64) * 4;
+            tput_d(q, tag, v.x);
+            tput_d(q + 2, tag, v.y);
+        }
         if (tid == 0) {
-            slots[(size_t)jj * P + p] = Cand{s_cval, myrow, 0};
-            __hip_atomic_fetch_add(&counters[jj], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            u64* q = slots + ((size_t)jj * P + p) * 3;
+            tput_d(q, tag, s_cval);
+            tput(q + 2, tag, (unsigned)myrow);
         }
     };
 
@@ -252,18 +289,22 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
     const int r = tid & (PANEL_ROWS - 1), half = tid >> 7;  // update mapping: row, column parity
     for (int j = 0; j < nb; ++j) {
         const int d = k0 + j;
-        if (tid == 0) wait_count(&counters[j], P, status);
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        // pick: the winner (wave 0) and row d (wave 1) in one round of reads
         if (wave == 0) {
+            // the P slots (lane q: slots q, q + 64, ...; P <= 4 * 64), then
+            // the winner's row
             double wv = -1.0;
             int wr = 0x7fffffff, wp = 0;
-            for (int q = lane; q < P; q += 64) {
-                Cand c = slots[(size_t)j * P + q];
-                if (better(c.val, c.row, wv, wr)) {
-                    wv = c.val;
-                    wr = c.row;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int q = lane + 64 * u;
+                if (64 * u >= P) break;  // wave-uniform
+                u64 w3[3];
+                tpoll<3>(slots + ((size_t)j * P + min(q, P - 1)) * 3, tag, w3, q < P, status);
+                const double cv = tval_d(w3[0], w3[1]);
+                const int cr = (int)(unsigned)w3[2];
+                if (q < P && better(cv, cr, wv, wr)) {
+                    wv = cv;
+                    wr = cr;
                     wp = q;
                 }
             }
@@ -277,18 +318,22 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
                     wp = op;
                 }
             }
+            u64 w4[4];
+            tpoll<4>(cand + (((size_t)j * P + wp) * NB + min(lane, nb - 1)) * 4, tag, w4, lane < nb,
+                     status);
+            if (lane < nb) urow[lane] = mk(tval_d(w4[0], w4[1]), tval_d(w4[2], w4[3]));
             if (lane == 0) {
                 s_piv = wr;
                 s_pval = wv;
-                s_wp = wp;
             }
-        } else if (wave == 1 && lane < nb) {
-            dbuf[lane] = drow[(size_t)j * NB + lane];
+        } else if (wave == 1) {
+            u64 w4[4];
+            tpoll<4>(drow + ((size_t)j * NB + min(lane, nb - 1)) * 4, tag, w4, lane < nb, status);
+            if (lane < nb) dbuf[lane] = mk(tval_d(w4[0], w4[1]), tval_d(w4[2], w4[3]));
         }
         __syncthreads();
         const int piv = s_piv;
         const double pval = s_pval;
-        if (tid < nb) urow[tid] = cand[((size_t)j * P + s_wp) * NB + tid];
         if (tid == 0) s_pivs[j] = piv;
         if (p == 0 && tid == 0) {
             ipiv[d] = piv;
@@ -594,18 +639,18 @@ __global__ void k_copy_cd(const cd* __restrict__ s, cd* __restrict__ d, int n) {
         }                                                                   \
     } while (0)
 
-enum { W_IPIV, W_CNT, W_SLOTS, W_CAND, W_DROW, W_PERM, W_INFO, W_STATUS, W_BTMP, W_N };
+enum { W_IPIV, W_SLOTS, W_CAND, W_DROW, W_PERM, W_INFO, W_STATUS, W_BTMP, W_N };
 
 // Bytes of buffer i of the FP64 factorisation and solve for order n:
-// pivots, per-column arrival counters, per-workgroup pivot candidates and
-// their rows, the pivot row block, the panels' row permutations, INFO, the
+// pivots, the panel's tagged exchange words (per column and workgroup a
+// 3-word candidate slot and a 64-entry candidate row of 4 words per entry;
+// per column the 64-entry row d), the panels' row permutations, INFO, the
 // status word and the solve's work vector.
 constexpr size_t fp64_size(int n, int i) {
     return i == W_IPIV     ? sizeof(int) * (size_t)n
-           : i == W_CNT    ? sizeof(int) * (size_t)((n + NB - 1) / NB) * NB
-           : i == W_SLOTS  ? sizeof(Cand) * NB * (size_t)((n + PANEL_ROWS - 1) / PANEL_ROWS)
-           : i == W_CAND   ? sizeof(cd) * NB * (size_t)((n + PANEL_ROWS - 1) / PANEL_ROWS) * NB
-           : i == W_DROW   ? sizeof(cd) * NB * NB
+           : i == W_SLOTS  ? 3 * sizeof(u64) * NB * (size_t)((n + PANEL_ROWS - 1) / PANEL_ROWS)
+           : i == W_CAND   ? 4 * sizeof(u64) * NB * (size_t)((n + PANEL_ROWS - 1) / PANEL_ROWS) * NB
+           : i == W_DROW   ? 4 * sizeof(u64) * NB * NB
            : i == W_PERM   ? sizeof(int) * (size_t)((n + NB - 1) / NB) * (1 + 4 * NB)
            : i == W_INFO   ? sizeof(long long)
            : i == W_STATUS ? sizeof(int)
@@ -630,20 +675,18 @@ size_t fp64_factor_bytes(int n) {
 // Device pointers of one FP64 workspace.
 struct Fp64Ws {
     int* ipiv = nullptr;
-    int* cnt = nullptr;
-    Cand* slots = nullptr;
-    cd* cand = nullptr;
-    cd* drow = nullptr;
+    u64* slots = nullptr;
+    u64* cand = nullptr;
+    u64* drow = nullptr;
     int* perm = nullptr;
     long long* info = nullptr;
     int* status = nullptr;
     cd* btmp = nullptr;
     void set(void* const p[W_N]) {
         ipiv = (int*)p[W_IPIV];
-        cnt = (int*)p[W_CNT];
-        slots = (Cand*)p[W_SLOTS];
-        cand = (cd*)p[W_CAND];
-        drow = (cd*)p[W_DROW];
+        slots = (u64*)p[W_SLOTS];
+        cand = (u64*)p[W_CAND];
+        drow = (u64*)p[W_DROW];
         perm = (int*)p[W_PERM];
         info = (long long*)p[W_INFO];
         status = (int*)p[W_STATUS];
@@ -675,7 +718,7 @@ static void fp64_ws_carve(char* base, int n, Fp64Ws& ws) {
 }
 
 // Static device workspace of bemx_linsolve_device_noalloc: the FP64
-// factorisation's buffers for orders up to NOALLOC_MAX_N (18.4 MB).  Each
+// factorisation's buffers for orders up to NOALLOC_MAX_N (35.3 MB).  Each
 // GPU the library runs on holds its own copy until the process ends; every
 // call on that GPU shares it.  Host code takes its address from
 // hipGetSymbolAddress (scratch_setup, once per device when a context is
@@ -712,20 +755,28 @@ hipError_t preload_lu_code() {
     return hipFuncGetAttributes(&a, (const void*)k_panel_fp64);
 }
 
+// Tags of the panel exchange words: n consecutive tags, never 0, unique in
+// the process until the 32-bit counter wraps (4e9 panel launches).
+static std::atomic<unsigned> g_panel_tag{0};
+static unsigned tag_block(int n) {
+    for (;;) {
+        const unsigned t = g_panel_tag.fetch_add((unsigned)n) + 1;
+        if (t != 0 && t + (unsigned)n > t) return t;  // skip a block that wraps through 0
+    }
+}
+
 // tw (may be NULL): the workspace whose events time the factorisation.
 static int factor_fp64(const Fp64Ws& ws, hipStream_t st, int n, cd* A, size_t lda, LuWork* tw,
                        std::string& err) {
     const int npanel = (n + NB - 1) / NB;
     int* ipiv = ws.ipiv;
-    int* cnt = ws.cnt;
-    Cand* slots = ws.slots;
-    cd* cand = ws.cand;
-    cd* drow = ws.drow;
+    u64* slots = ws.slots;
+    u64* cand = ws.cand;
+    u64* drow = ws.drow;
     int* perm = ws.perm;
     long long* info = ws.info;
     int* status = ws.status;
-    size_t ncnt = (size_t)npanel * NB;
-    hipLaunchKernelGGL(k_fill_int, dim3((ncnt + 255) / 256), dim3(256), 0, st, cnt, 0, ncnt);
+    const unsigned tag0 = tag_block(npanel);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, (int*)info, 0, 2);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, status, 0, 1);
     int dev = 0, ncu = 0, perCU = 0;
@@ -745,10 +796,10 @@ static int factor_fp64(const Fp64Ws& ws, hipStream_t st, int n, cd* A, size_t ld
             err = "LU panel needs more co-resident workgroups than the device holds";
             return BEMX_ERR_ARG;
         }
-        int* cntk = cnt + (size_t)pk * NB;
         int* permk = perm + (size_t)pk * (1 + 4 * NB);
-        void* args[] = {&A, &lda, (void*)&n, (void*)&k0, (void*)&nb, &ipiv, &cntk, &slots, &cand,
-                        &drow, &permk, &info, &status};
+        unsigned tag = tag0 + (unsigned)pk;
+        void* args[] = {&A, &lda, (void*)&n, (void*)&k0, (void*)&nb, &ipiv, &slots, &cand,
+                        &drow, &permk, &info, &status, &tag};
         const hipError_t ec = hipLaunchCooperativeKernel((const void*)k_panel_fp64, dim3(P),
                                                          dim3(256), args, 0, st);
         if (ec == hipErrorIllegalState) {

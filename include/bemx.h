@@ -118,7 +118,7 @@ int bemx_linsolve(bemx_ctx *ctx, int nn, double *zh, double *zfi, int mode, long
  * overwritten by the solution; both device pointers on the context's
  * device.  ZGESV semantics and kernels of bemx_linsolve's FP64 path, on the
  * context's stream; returns when the solve is done.  The pivoting
- * workspace is a static device array of 18.4 MB per GPU, so nn <= 32768
+ * workspace is a static device array of 35.3 MB per GPU, so nn <= 32768
  * (larger orders: BEMX_ERR_ARG); calls that overlap on one GPU share that
  * array and race.  info as bemx_linsolve. */
 int bemx_linsolve_device_noalloc(bemx_ctx *ctx, int nn, void *a, long long lda, void *b,
