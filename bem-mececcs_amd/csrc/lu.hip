@@ -1,21 +1,27 @@
 // lu.hip -- dense complex solve of ZH x = ZFI on MI355X (LINSOLVE).
 //
 // Plain FP64 path (ZGESV semantics, Linsolve.for:68-109): right-looking
-// blocked LU with partial pivoting, panel width NB = 64.
+// blocked LU with partial pivoting, panel width NB = 64, panels in groups of
+// FP64_GROUP.
 //   k_panel_fp64   the m x NB panel held in LDS across P co-resident
 //                  workgroups; per column one inter-workgroup exchange of the
 //                  local pivot candidates (largest |re|+|im|, lowest row on
-//                  ties, as LAPACK's IZAMAX), then the row swap and the rank-1
-//                  update in LDS.  Also emits the panel's row permutation.
-//   k_rowperm      applies that permutation to the trailing columns (the
-//                  factored L columns are left in place: the forward solve
-//                  applies each panel's permutation before using its L block).
+//                  ties, as LAPACK's IZAMAX) through tagged words, then the
+//                  row swap and the rank-1 update in LDS.  Also emits the
+//                  panel's row permutation.
+//   k_rowperm      applies that permutation to other columns: inside the
+//                  group to its earlier L columns and its later columns,
+//                  after the group to the columns right of it (L columns of
+//                  earlier groups are left in place: the forward solve
+//                  applies a group's permutations before its L blocks).
 //   k_trsm_lower_cols  U12 = L11^-1 A12, one lane per column.
-//   k_zgemm_mfma   A22 -= L21 U12 as four real FP64 MFMA products
-//                  (v_mfma_f64_16x16x4f64): Cr -= Ar Br - Ai Bi,
-//                  Ci -= Ar Bi + Ai Br.
-// The solve walks the panels forward (permute, L11 solve, L21 update) and
-// backward (U11 solve, U update) with one fused kernel per step.
+//   k_zgemm_mfma, k_zgemm_mfma2  A22 -= L21 U12 as four real FP64 MFMA
+//                  products (v_mfma_f64_16x16x4f64): Cr -= Ar Br - Ai Bi,
+//                  Ci -= Ar Bi + Ai Br; K = 64 inside a group, K = 256 for
+//                  the rest of the matrix after it.
+// The solve walks the groups forward (permute, then per panel L11 solve and
+// L21 update) and the panels backward (U11 solve, U update) with one fused
+// kernel per step.
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
@@ -538,6 +544,126 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(int m, int nc, int K, const 
         }
 }
 
+// Trailing update, second form: C[m x nc] -= A[m x K] B[K x nc] (K <= NB)
+// with 128 x 128 workgroup tiles.  512 threads = 8 waves of 64 x 32 (4 x 2
+// MFMA blocks of 16 x 16), the product accumulated from zero and C read and
+// updated once at the end.  K runs in chunks of 16 through two LDS buffers
+// of real and imaginary planes (rows padded by 2 doubles); the next chunk's
+// operands are loaded into registers while the MFMAs of the current one run.
+// Tiles are numbered so that each XCD takes a contiguous row-major range
+// (workgroup id runs on XCD id % 8), sharing A tiles in its L2.
+constexpr int ZB = 128, ZK = 16, ZLD = ZB + 2, ZXCD = 8;
+__global__ __launch_bounds__(512) void k_zgemm_mfma2(int m, int nc, int K,
+                                                     const cd* __restrict__ A, size_t lda,
+                                                     const cd* __restrict__ B, size_t ldb,
+                                                     cd* __restrict__ C, size_t ldc) {
+    __shared__ double sA[2][2][ZK][ZLD];
+    __shared__ double sB[2][2][ZK][ZLD];
+    const int gx = (m + ZB - 1) / ZB, gy = (nc + ZB - 1) / ZB, tiles = gx * gy;
+    const int per = (tiles + ZXCD - 1) / ZXCD;
+    const int t = (blockIdx.x % ZXCD) * per + blockIdx.x / ZXCD;
+    if (t >= tiles) return;
+    const int bm = (t / gy) * ZB, bn = (t % gy) * ZB;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = 64 * (wave & 1), wn = 32 * (wave >> 1);
+    const int li = lane & 15, lk = lane >> 4;
+    // staging assignment: A row ai, k = 4 aq .. 4 aq + 3; B column bj, same k
+    const int ai = tid & 127, aq = tid >> 7;
+    const int bj = tid >> 2, bq = tid & 3;
+    const bool arow = bm + ai < m, bcol = bn + bj < nc;
+    cd ra[4], rb[4];
+    auto gload = [&](int kc) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = kc + 4 * aq + q;
+            ra[q] = (arow && k < K) ? A[(size_t)k * lda + bm + ai] : mk(0.0, 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = kc + 4 * bq + q;
+            rb[q] = (bcol && k < K) ? B[(size_t)(bn + bj) * ldb + k] : mk(0.0, 0.0);
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            sA[buf][0][4 * aq + q][ai] = ra[q].x;
+            sA[buf][1][4 * aq + q][ai] = ra[q].y;
+            sB[buf][0][4 * bq + q][bj] = rb[q].x;
+            sB[buf][1][4 * bq + q][bj] = rb[q].y;
+        }
+    };
+    v4d accR[4][2], accI[4][2];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            accR[x][y] = v4d{0.0, 0.0, 0.0, 0.0};
+            accI[x][y] = v4d{0.0, 0.0, 0.0, 0.0};
+        }
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < ZK / 4; ++s) {
+            const int k = 4 * s + lk;
+            double ar[4], an[4], br[2], bi[2];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                ar[x] = sA[buf][0][k][wm + 16 * x + li];
+                an[x] = -sA[buf][1][k][wm + 16 * x + li];
+            }
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                br[y] = sB[buf][0][k][wn + 16 * y + li];
+                bi[y] = sB[buf][1][k][wn + 16 * y + li];
+            }
+            // re += ar br - ai bi, im += ar bi + ai br: the 16 first products,
+            // then the 16 second ones (16 independent MFMAs between two that
+            // update the same accumulator)
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    accR[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], accR[x][y], 0, 0, 0);
+                    accI[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], bi[y], accI[x][y], 0, 0, 0);
+                }
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    accR[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(an[x], bi[y], accR[x][y], 0, 0, 0);
+                    accI[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-an[x], br[y], accI[x][y], 0, 0, 0);
+                }
+        }
+    };
+    const int nch = (K + ZK - 1) / ZK;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+        if (c + 1 < nch) gload(ZK * (c + 1));
+        compute(c & 1);
+        if (c + 1 < nch) lstore((c + 1) & 1);
+        __syncthreads();
+    }
+    // D[i][j] of a 16 x 16 block: lane j + 16 (i % 4), register i / 4
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const int col = bn + wn + 16 * y + li;
+            if (col >= nc) continue;
+            cd* cp = C + (size_t)col * ldc;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = bm + wm + 16 * x + 4 * r + lk;
+                if (row < m) {
+                    const cd v = cp[row];
+                    cp[row] = mk(v.x - accR[x][y][r], v.y - accI[x][y][r]);
+                }
+            }
+        }
+}
+
 // ---------------------------------------------------------------------------
 // Solve with the factors.  Forward step for panel k: apply the panel's row
 // permutation to b, y = L11^-1 b[k0:k0+nb] (every workgroup recomputes the
@@ -557,9 +683,10 @@ __global__ __launch_bounds__(256) void k_fwd_step(const cd* __restrict__ A, size
         L[c * NB + r] = A[(size_t)(k0 + c) * lda + k0 + r];
     }
     // the permuted block of b (rows k0..k0+nb), read from the pre-step copy
+    // (perm NULL: the rows are already in place)
     if (tid < nb) {
         int row = k0 + tid;
-        int m = perm[0], src = row;
+        int m = perm ? perm[0] : 0, src = row;
         for (int i = 0; i < m; ++i)
             if (perm[1 + i] == row) src = perm[1 + 2 * NB + i];
         y[tid] = btmp[src];
@@ -577,7 +704,7 @@ __global__ __launch_bounds__(256) void k_fwd_step(const cd* __restrict__ A, size
     const int row = k0 + nb + blockIdx.x * 256 + tid;
     if (blockIdx.x == 0 && tid < nb) b[k0 + tid] = y[tid];
     if (row < n) {
-        int m = perm[0], src = row;
+        int m = perm ? perm[0] : 0, src = row;
         for (int i = 0; i < m; ++i)
             if (perm[1 + i] == row) src = perm[1 + 2 * NB + i];
         cd s = btmp[src];
@@ -625,6 +752,17 @@ __global__ void k_fill_int(int* p, int v, size_t cnt) {
 __global__ void k_copy_cd(const cd* __restrict__ s, cd* __restrict__ d, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) d[i] = s[i];
+}
+
+// b <- one panel's row changes (perm = {m, dst[0..2NB), src[0..2NB)}), in
+// place: every source is read before any destination is written.
+__global__ __launch_bounds__(2 * NB) void k_permute_vec(cd* __restrict__ b,
+                                                        const int* __restrict__ perm) {
+    const int m = perm[0], t = threadIdx.x;
+    cd v = mk(0.0, 0.0);
+    if (t < m) v = b[perm[1 + 2 * NB + t]];
+    __syncthreads();
+    if (t < m) b[perm[1 + t]] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -755,6 +893,33 @@ hipError_t preload_lu_code() {
     return hipFuncGetAttributes(&a, (const void*)k_panel_fp64);
 }
 
+// Panels per group of the FP64 factorisation: inside a group each panel
+// updates only the group's columns (K = 64); the rest of the matrix gets one
+// update per group with K = 64 x FP64_GROUP, which reads and writes it a
+// quarter as often as an update per panel.  Row interchanges follow LAPACK's
+// convention inside a group (a panel's changes also reach the group's earlier
+// L columns, so the group's L rows share one order) and the per-panel one
+// across groups; the forward solve applies a group's changes before its
+// blocks.
+constexpr int FP64_GROUP = 4;
+
+// C -= A B for the trailing update: 128 x 128 tiles from 1024 x 1024 on
+// (on the 6480-DOF solve 32 against 27 TFLOP/s at m = n = 3216 .. 6416), the
+// 64 x 64 tiles below, where 128 x 128 tiles leave most CUs idle (m = n =
+// 416: 16 tiles, 49 against 23 us).
+static void launch_zgemm(hipStream_t st, int m, int nc, int K, const cd* A, size_t lda,
+                         const cd* B, size_t ldb, cd* C, size_t ldc) {
+    if (m >= 1024 && nc >= 1024) {
+        const int tiles = ((m + ZB - 1) / ZB) * ((nc + ZB - 1) / ZB);
+        const int per = (tiles + ZXCD - 1) / ZXCD;
+        hipLaunchKernelGGL(k_zgemm_mfma2, dim3(per * ZXCD), dim3(512), 0, st, m, nc, K, A, lda, B,
+                           ldb, C, ldc);
+    } else {
+        hipLaunchKernelGGL(k_zgemm_mfma, dim3((m + GB - 1) / GB, (nc + GB - 1) / GB), dim3(256), 0,
+                           st, m, nc, K, A, lda, B, ldb, C, ldc);
+    }
+}
+
 // Tags of the panel exchange words: n consecutive tags, never 0, unique in
 // the process until the 32-bit counter wraps (4e9 panel launches).
 static std::atomic<unsigned> g_panel_tag{0};
@@ -787,37 +952,69 @@ static int factor_fp64(const Fp64Ws& ws, hipStream_t st, int n, cd* A, size_t ld
         LU_CHK(tw->ensure_side());
         LU_CHK(hipEventRecord(tw->ev_f0, st));
     }
-    for (int pk = 0; pk < npanel; ++pk) {
-        const int k0 = pk * NB;
-        const int nb = std::min(NB, n - k0);
-        const int m = n - k0;
-        const int P = (m + PANEL_ROWS - 1) / PANEL_ROWS;
-        if (P > ncu * perCU) {
-            err = "LU panel needs more co-resident workgroups than the device holds";
-            return BEMX_ERR_ARG;
-        }
-        int* permk = perm + (size_t)pk * (1 + 4 * NB);
-        unsigned tag = tag0 + (unsigned)pk;
-        void* args[] = {&A, &lda, (void*)&n, (void*)&k0, (void*)&nb, &ipiv, &slots, &cand,
-                        &drow, &permk, &info, &status, &tag};
-        const hipError_t ec = hipLaunchCooperativeKernel((const void*)k_panel_fp64, dim3(P),
-                                                         dim3(256), args, 0, st);
-        if (ec == hipErrorIllegalState) {
-            err = coop_refused_message("LU panel");
-            return BEMX_ERR_HIP;
-        }
-        LU_CHK(ec);
-        const int c0 = k0 + nb, nc = n - c0;
-        if (nc <= 0) continue;
-        hipLaunchKernelGGL(k_rowperm, dim3((nc + 3) / 4), dim3(256), 0, st, A, lda, c0, nc, permk);
+    auto permk_of = [&](int pk) { return perm + (size_t)pk * (1 + 4 * NB); };
+    auto rowperm = [&](int c0, int nc, const int* pm) {
+        if (nc > 0)
+            hipLaunchKernelGGL(k_rowperm, dim3((nc + 3) / 4), dim3(256), 0, st, A, lda, c0, nc, pm);
+    };
+    auto trsm = [&](int k0, int nb, int c0, int nc) {
         hipLaunchKernelGGL((k_trsm_lower_cols<cd, false>),
                            dim3((nc + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG), dim3(256), 0, st,
                            A, lda, k0, nb, c0, nc, (const int*)nullptr, (__bf16*)nullptr,
                            (size_t)0);
-        const int mr = n - c0;
-        hipLaunchKernelGGL(k_zgemm_mfma, dim3((mr + GB - 1) / GB, (nc + GB - 1) / GB), dim3(256), 0,
-                           st, mr, nc, nb, A + (size_t)k0 * lda + c0, lda,
-                           A + (size_t)c0 * lda + k0, lda, A + (size_t)c0 * lda + c0, lda);
+    };
+    for (int g0 = 0; g0 < npanel; g0 += FP64_GROUP) {
+        const int pend = std::min(npanel, g0 + FP64_GROUP);
+        const int G0 = g0 * NB, Ge = std::min(n, pend * NB);
+        for (int pk = g0; pk < pend; ++pk) {
+            const int k0 = pk * NB;
+            const int nb = std::min(NB, n - k0);
+            const int m = n - k0;
+            const int P = (m + PANEL_ROWS - 1) / PANEL_ROWS;
+            if (P > ncu * perCU) {
+                err = "LU panel needs more co-resident workgroups than the device holds";
+                return BEMX_ERR_ARG;
+            }
+            int* permk = permk_of(pk);
+            unsigned tag = tag0 + (unsigned)pk;
+            void* args[] = {&A, &lda, (void*)&n, (void*)&k0, (void*)&nb, &ipiv, &slots, &cand,
+                            &drow, &permk, &info, &status, &tag};
+            const hipError_t ec = hipLaunchCooperativeKernel((const void*)k_panel_fp64, dim3(P),
+                                                             dim3(256), args, 0, st);
+            if (ec == hipErrorIllegalState) {
+                err = coop_refused_message("LU panel");
+                return BEMX_ERR_HIP;
+            }
+            LU_CHK(ec);
+            // this panel's row changes on the group's earlier L columns (the
+            // group's L rows stay in one order) and on its later columns,
+            // which are then solved for and updated with K = 64
+            rowperm(G0, k0 - G0, permk);
+            const int c0 = k0 + nb, ncg = Ge - c0;
+            if (ncg > 0) {
+                rowperm(c0, ncg, permk);
+                trsm(k0, nb, c0, ncg);
+                launch_zgemm(st, n - c0, ncg, nb, A + (size_t)k0 * lda + c0, lda,
+                             A + (size_t)c0 * lda + k0, lda, A + (size_t)c0 * lda + c0, lda);
+            }
+            LU_CHK(hipGetLastError());
+        }
+        // the columns right of the group: the group's row changes in panel
+        // order, its 64-row blocks of U solved for in turn (each updating the
+        // group's later rows), then one update with K = the group's width
+        const int cr = Ge, nr = n - Ge;
+        if (nr <= 0) continue;
+        for (int pk = g0; pk < pend; ++pk) rowperm(cr, nr, permk_of(pk));
+        for (int pk = g0; pk < pend; ++pk) {
+            const int k0 = pk * NB, nb = std::min(NB, n - k0);
+            trsm(k0, nb, cr, nr);
+            const int mrow = Ge - (k0 + nb);
+            if (mrow > 0)
+                launch_zgemm(st, mrow, nr, nb, A + (size_t)k0 * lda + k0 + nb, lda,
+                             A + (size_t)cr * lda + k0, lda, A + (size_t)cr * lda + k0 + nb, lda);
+        }
+        launch_zgemm(st, n - Ge, nr, Ge - G0, A + (size_t)G0 * lda + Ge, lda,
+                     A + (size_t)cr * lda + G0, lda, A + (size_t)cr * lda + Ge, lda);
         LU_CHK(hipGetLastError());
     }
     if (tw) {
@@ -832,12 +1029,18 @@ static int solve_fp64(const Fp64Ws& ws, hipStream_t st, int n, const cd* A, size
     const int npanel = (n + NB - 1) / NB;
     const int* perm = ws.perm;
     cd* btmp = ws.btmp;
-    for (int pk = 0; pk < npanel; ++pk) {
-        const int k0 = pk * NB, nb = std::min(NB, n - k0);
-        hipLaunchKernelGGL(k_copy_cd, dim3((n + 255) / 256), dim3(256), 0, st, b, btmp, n);
-        const int rows = std::max(1, n - k0 - nb);
-        hipLaunchKernelGGL(k_fwd_step, dim3((rows + 255) / 256), dim3(256), 0, st, A, lda, n, k0,
-                           nb, perm + (size_t)pk * (1 + 4 * NB), b, btmp);
+    for (int g0 = 0; g0 < npanel; g0 += FP64_GROUP) {
+        const int pend = std::min(npanel, g0 + FP64_GROUP);
+        for (int pk = g0; pk < pend; ++pk)
+            hipLaunchKernelGGL(k_permute_vec, dim3(1), dim3(2 * NB), 0, st, b,
+                               perm + (size_t)pk * (1 + 4 * NB));
+        for (int pk = g0; pk < pend; ++pk) {
+            const int k0 = pk * NB, nb = std::min(NB, n - k0);
+            hipLaunchKernelGGL(k_copy_cd, dim3((n + 255) / 256), dim3(256), 0, st, b, btmp, n);
+            const int rows = std::max(1, n - k0 - nb);
+            hipLaunchKernelGGL(k_fwd_step, dim3((rows + 255) / 256), dim3(256), 0, st, A, lda, n,
+                               k0, nb, (const int*)nullptr, b, btmp);
+        }
     }
     for (int pk = npanel - 1; pk >= 0; --pk) {
         const int k0 = pk * NB, nb = std::min(NB, n - k0);
