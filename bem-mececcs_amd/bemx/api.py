@@ -30,7 +30,8 @@ STATUS = {
 EXPORTS = [
     "bemx_create", "bemx_destroy", "bemx_last_error", "bemx_gauleg", "bemx_send_shared_data",
     "bemx_deallocate_shared_data", "bemx_ghmatece", "bemx_send_dyn_data", "bemx_ghmatecd",
-    "bemx_linsolve", "bemx_interec", "bemx_solve", "bemx_device_buffers", "bemx_last_stats",
+    "bemx_linsolve", "bemx_interec", "bemx_solve", "bemx_solve_sweep", "bemx_device_buffers",
+    "bemx_last_stats",
     "bemx_send_load_cases", "bemx_send_load_cases_kode", "bemx_material", "bemx_server_create", "bemx_server_submit",
     "bemx_server_wait", "bemx_server_destroy", "bemx_request_footprint", "bemx_server_stats",
     "bemx_linsolve_device_noalloc",
@@ -68,6 +69,7 @@ def load_library(path: str = LIBPATH) -> C.CDLL:
     lib.bemx_interec.argtypes = [vp, _dp, _dp, _ip, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
                                  C.c_double, C.c_double, _dp, _dp]
     lib.bemx_solve.argtypes = [vp, _dp, _dp, _dp, C.c_double, C.c_int, _dp, _dp]
+    lib.bemx_solve_sweep.argtypes = [vp, _dp, _dp, _dp, C.c_int, _dp, C.c_int, _dp, _dp]
     lib.bemx_device_buffers.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
     lib.bemx_last_stats.argtypes = [vp, _dp, C.c_int]
     lib.bemx_material.argtypes = [C.c_double] * 4 + [_dp] * 3
@@ -315,6 +317,25 @@ class Context:
         if want_x and nr > 1:
             x = x.reshape(nn, nr, order="F")
         return x, stats
+
+
+    def solve_sweep(self, zmat, freqs, mode: int = MODE_FP64, want_x: bool = True):
+        """bemx_solve_sweep: every frequency of ``freqs`` on the resident mesh
+        and load cases -> (x, stats); x is (nfreq, 3NBE) for one load case,
+        (nfreq, 3NBE, nrhs) for several; stats (nfreq, 4)."""
+        nn = 3 * self.mesh.nbe
+        nr = getattr(self, "nrhs", 1)
+        f = np.ascontiguousarray(freqs, dtype=np.float64)
+        zge, zcs, zcp = (_cz(v) for v in zmat)
+        x = np.zeros(len(f) * nn * nr, dtype=np.complex128) if want_x else None
+        stats = np.zeros(4 * len(f))
+        code = self._lib.bemx_solve_sweep(self.h, _p(zge), _p(zcs), _p(zcp), len(f), _p(f), mode,
+                                          _p(x.view(np.float64)) if want_x else None,
+                                          _p(stats))
+        self._check(code, "solve_sweep")
+        if want_x:
+            x = x.reshape(len(f), nr, nn).transpose(0, 2, 1) if nr > 1 else x.reshape(len(f), nn)
+        return x, stats.reshape(len(f), 4)
 
 
 class Server:

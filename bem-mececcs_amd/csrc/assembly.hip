@@ -632,64 +632,97 @@ __device__ inline void offdiag_element(const DynConst& K, const GaussPt* __restr
 // (sum C' r_i) n_j: the first sum is symmetric (6 entries), the second a
 // vector.  The diagonal term sum A' drn is SP . n (n is fixed per element).
 // The staged weights already hold the factor 1/(4 pi).
-__device__ inline void offdiag_element_h_tab(const DynConst& K, const ExpTab& T,
-                                             const GaussPt* __restrict__ sg, int npg2, double xc,
-                                             double yc, double zc, const double nv[3], cd H[9]) {
+// Factored H sums of one element: SP = sum A' r, SC = sum C' r, SB = the
+// symmetric sum B' drn r r^T (6 entries).
+struct HSums {
     cd SP[3], SC[3], SB[6];
-    for (int k = 0; k < 3; ++k) SP[k] = SC[k] = mk(0, 0);
-    for (int k = 0; k < 6; ++k) SB[k] = mk(0, 0);
+    __device__ void zero() {
+        for (int k = 0; k < 3; ++k) SP[k] = SC[k] = mk(0, 0);
+        for (int k = 0; k < 6; ++k) SB[k] = mk(0, 0);
+    }
+    // H_ij = delta_ij sum A' drn + n_i sum A' r_j + sum r_i (B' drn r_j + C' n_j),
+    // with sum A' drn = SP . n
+    __device__ void to_block(const double nv[3], cd H[9]) const {
+        const cd SD = SP[0] * nv[0] + SP[1] * nv[1] + SP[2] * nv[2];
+        const int sidx[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) {
+                cd h = SB[sidx[i + 3 * j]] + SC[i] * nv[j] + SP[j] * nv[i];
+                if (i == j) h = h + SD;
+                H[i + 3 * j] = h;
+            }
+    }
+};
+
+// One Gauss point's contribution to the H sums of one frequency (table
+// route), from the point's frequency-independent geometry: R, 1/R, 1/R^2,
+// the unit vector rd, drn and the weight w (with 1/(4 pi)).  The one- and
+// two-frequency kernels both call it, so each frequency's arithmetic is the
+// same in both.
+__device__ __forceinline__ void h_point_tab(const DynConst& K, const ExpTab& T, double R,
+                                            double iR, double iR2, const double rd[3], double drn,
+                                            double w, HSums& S) {
+    cd A, B, C;
+    {
+        cd Es, Epc;
+        exp_pair_tab(K, T, R, Es, Epc);
+        cd as = K.as * iR, as2 = K.as2 * iR2;
+        cd ap = K.ap * iR, ap2 = K.ap2 * iR2;
+        cd zs = K.ks * R;
+        cd fs = mk(1.0 + 3.0 * (as2.x - as.x), 3.0 * (as2.y - as.y));
+        cd fp = mk(1.0 + 3.0 * (ap2.x - ap.x), 3.0 * (ap2.y - ap.y));
+        cd chiR = fs * Es - fp * Epc;
+        cd hs = mk(-2.0 + zs.x + 3.0 * (as.x - as2.x), zs.y + 3.0 * (as.y - as2.y));
+        cd hp = mk(-1.0 + 3.0 * (ap.x - ap2.x), 3.0 * (ap.y - ap2.y));
+        cd psidR2 = hs * Es - hp * Epc;
+        cd zp = K.kp * R;
+        cd ks_ = mk(zs.x - 4.0 + 9.0 * (as.x - as2.x), zs.y + 9.0 * (as.y - as2.y));
+        cd kp_ = mk(zp.x - 4.0 + 9.0 * (ap.x - ap2.x), zp.y + 9.0 * (ap.y - ap2.y));
+        cd chidR2 = ks_ * Es - kp_ * Epc;
+        A = psidR2 - chiR;
+        B = 4.0 * chiR - 2.0 * chidR2;
+        C = K.c1m2 * (A + 0.5 * B - 3.0 * chiR) - 2.0 * chiR;
+    }
+    double wq = w * iR2;
+    cd Ap = A * wq, Bp = B * (wq * drn), Cp = C * wq;
+    for (int j = 0; j < 3; ++j) {
+        S.SP[j] = S.SP[j] + Ap * rd[j];
+        S.SC[j] = S.SC[j] + Cp * rd[j];
+    }
+    S.SB[0] = S.SB[0] + Bp * (rd[0] * rd[0]);
+    S.SB[1] = S.SB[1] + Bp * (rd[1] * rd[0]);
+    S.SB[2] = S.SB[2] + Bp * (rd[2] * rd[0]);
+    S.SB[3] = S.SB[3] + Bp * (rd[1] * rd[1]);
+    S.SB[4] = S.SB[4] + Bp * (rd[2] * rd[1]);
+    S.SB[5] = S.SB[5] + Bp * (rd[2] * rd[2]);
+}
+
+// H blocks of NF frequencies for one element (table route): the geometry of
+// each Gauss point once, then each frequency's kernel scalars.  Points with
+// |ks| R < TAU_OFF are left to k_dyn_offdiag_near (decided per frequency).
+template <int NF>
+__device__ inline void offdiag_element_h_tab(const DynConst* K, const ExpTab* T,
+                                             const GaussPt* __restrict__ sg, int npg2, double xc,
+                                             double yc, double zc, const double nv[3],
+                                             cd (*H)[9]) {
+    HSums S[NF];
+    for (int f = 0; f < NF; ++f) S[f].zero();
     for (int q = 0; q < npg2; ++q) {
         GaussPt p = sg[q];
         double r1 = p.x - xc, r2 = p.y - yc, r3 = p.z - zc;
         const double RR = dist2(r1, r2, r3);
-        if (near_field(K, RR)) continue;  // k_dyn_offdiag_near adds these points
+        bool any = false;
+        for (int f = 0; f < NF; ++f) any = any || !near_field(K[f], RR);
+        if (!any) continue;
         double iR = rsqrt_refined(RR);
         double R = RR * iR;
         double rd[3] = {r1 * iR, r2 * iR, r3 * iR};
         double drn = rd[0] * nv[0] + rd[1] * nv[1] + rd[2] * nv[2];
         double iR2 = iR * iR;
-        cd A, B, C;
-        {
-            cd Es, Epc;
-            exp_pair_tab(K, T, R, Es, Epc);
-            cd as = K.as * iR, as2 = K.as2 * iR2;
-            cd ap = K.ap * iR, ap2 = K.ap2 * iR2;
-            cd zs = K.ks * R;
-            cd fs = mk(1.0 + 3.0 * (as2.x - as.x), 3.0 * (as2.y - as.y));
-            cd fp = mk(1.0 + 3.0 * (ap2.x - ap.x), 3.0 * (ap2.y - ap.y));
-            cd chiR = fs * Es - fp * Epc;
-            cd hs = mk(-2.0 + zs.x + 3.0 * (as.x - as2.x), zs.y + 3.0 * (as.y - as2.y));
-            cd hp = mk(-1.0 + 3.0 * (ap.x - ap2.x), 3.0 * (ap.y - ap2.y));
-            cd psidR2 = hs * Es - hp * Epc;
-            cd zp = K.kp * R;
-            cd ks_ = mk(zs.x - 4.0 + 9.0 * (as.x - as2.x), zs.y + 9.0 * (as.y - as2.y));
-            cd kp_ = mk(zp.x - 4.0 + 9.0 * (ap.x - ap2.x), zp.y + 9.0 * (ap.y - ap2.y));
-            cd chidR2 = ks_ * Es - kp_ * Epc;
-            A = psidR2 - chiR;
-            B = 4.0 * chiR - 2.0 * chidR2;
-            C = K.c1m2 * (A + 0.5 * B - 3.0 * chiR) - 2.0 * chiR;
-        }
-        double wq = p.w * iR2;
-        cd Ap = A * wq, Bp = B * (wq * drn), Cp = C * wq;
-        for (int j = 0; j < 3; ++j) {
-            SP[j] = SP[j] + Ap * rd[j];
-            SC[j] = SC[j] + Cp * rd[j];
-        }
-        SB[0] = SB[0] + Bp * (rd[0] * rd[0]);
-        SB[1] = SB[1] + Bp * (rd[1] * rd[0]);
-        SB[2] = SB[2] + Bp * (rd[2] * rd[0]);
-        SB[3] = SB[3] + Bp * (rd[1] * rd[1]);
-        SB[4] = SB[4] + Bp * (rd[2] * rd[1]);
-        SB[5] = SB[5] + Bp * (rd[2] * rd[2]);
+        for (int f = 0; f < NF; ++f)
+            if (!near_field(K[f], RR)) h_point_tab(K[f], T[f], R, iR, iR2, rd, drn, p.w, S[f]);
     }
-    const cd SD = SP[0] * nv[0] + SP[1] * nv[1] + SP[2] * nv[2];
-    const int sidx[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
-    for (int j = 0; j < 3; ++j)
-        for (int i = 0; i < 3; ++i) {
-            cd h = SB[sidx[i + 3 * j]] + SC[i] * nv[j] + SP[j] * nv[i];
-            if (i == j) h = h + SD;
-            H[i + 3 * j] = h;
-        }
+    for (int f = 0; f < NF; ++f) S[f].to_block(nv, H[f]);
 }
 
 // H of every off-diagonal block; columns with KODE = 0 are left to
@@ -718,19 +751,64 @@ __global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag_h(int nbe, int npg2,
         const int J = j0 + jj;
         const int flag = colflag[J];
         const double nv[3] = {elem[J].nx, elem[J].ny, elem[J].nz};
-        cd H[9], G[9];
+        cd H[1][9], G[9];
         if constexpr (TAB)
-            offdiag_element_h_tab(K, T, sgp + jj * npg2, npg2, xc, yc, zc, nv, H);
+            offdiag_element_h_tab<1>(&K, &T, sgp + jj * npg2, npg2, xc, yc, zc, nv, H);
         else
-            offdiag_element<false>(K, sgp + jj * npg2, npg2, xc, yc, zc, nv, H, G);
+            offdiag_element<false>(K, sgp + jj * npg2, npg2, xc, yc, zc, nv, H[0], G);
         if (rowok && I != J) {
             for (int k = 0; k < 3; ++k) {
                 if ((flag >> (1 + k)) & 1) continue;
                 cd* col = zh + (size_t)(3 * J + k) * ldzh + 3 * I;
-                col[0] = H[3 * k];
-                col[1] = H[3 * k + 1];
-                col[2] = H[3 * k + 2];
+                col[0] = H[0][3 * k];
+                col[1] = H[0][3 * k + 1];
+                col[2] = H[0][3 * k + 2];
             }
+        }
+    }
+}
+
+// Two frequencies of a sweep in one launch (table route): the same tiles,
+// the Gauss-point geometry shared, each frequency's H into its own ZH.
+struct DynPair {
+    DynConst K[2];
+    ExpTab T[2];
+    cd* zh[2];
+};
+template <int WAVES, int JT>
+__global__ __launch_bounds__(64 * WAVES) void k_dyn_offdiag_h2(int nbe, int npg2,
+                                                               const ElemData* __restrict__ elem,
+                                                               const GaussPt* __restrict__ gp,
+                                                               const int* __restrict__ colflag,
+                                                               DynPair P, size_t ldzh) {
+    extern __shared__ GaussPt sgp[];
+    const int j0 = blockIdx.y * JT;
+    const int jn = min(JT, nbe - j0);
+    for (int t = threadIdx.x; t < jn * npg2; t += 64 * WAVES) {
+        GaussPt g = gp[(size_t)j0 * npg2 + t];
+        g.w *= P.K[0].q4pi;
+        sgp[t] = g;
+    }
+    __syncthreads();
+    const int I = blockIdx.x * (64 * WAVES) + threadIdx.x;
+    const bool rowok = I < nbe;
+    const int Ic = rowok ? I : nbe - 1;
+    const double xc = elem[Ic].cx, yc = elem[Ic].cy, zc = elem[Ic].cz;
+    for (int jj = 0; jj < jn; ++jj) {
+        const int J = j0 + jj;
+        const int flag = colflag[J];
+        const double nv[3] = {elem[J].nx, elem[J].ny, elem[J].nz};
+        cd H[2][9];
+        offdiag_element_h_tab<2>(P.K, P.T, sgp + jj * npg2, npg2, xc, yc, zc, nv, H);
+        if (rowok && I != J) {
+            for (int f = 0; f < 2; ++f)
+                for (int k = 0; k < 3; ++k) {
+                    if ((flag >> (1 + k)) & 1) continue;
+                    cd* col = P.zh[f] + (size_t)(3 * J + k) * ldzh + 3 * I;
+                    col[0] = H[f][3 * k];
+                    col[1] = H[f][3 * k + 1];
+                    col[2] = H[f][3 * k + 2];
+                }
         }
     }
 }
@@ -1237,6 +1315,59 @@ hipError_t launch_dyn_assembly(hipStream_t st, int nbe, int npg2, const ElemData
                        elem, gp, colflag, flist, fcount, zdfi, nrhs, ldr, K, zh, ldzh, rhs_part);
     hipLaunchKernelGGL(k_rhs_reduce, dim3((3 * nbe + 255) / 256, nrhs), dim3(256), 0, st, 3 * nbe,
                        nrhs, ldr, OG_JT, fcount, rhs_part, rhs_diag, zfi);
+    return hipGetLastError();
+}
+
+// Dynamic assembly of two frequencies of a sweep (same mesh, boundary data
+// and load cases): per frequency the exp table, the diagonal blocks, the
+// near field, the flagged columns and the right-hand sides, as
+// launch_dyn_assembly; the column flags once; the H blocks of both
+// frequencies in one k_dyn_offdiag_h2 launch.  Each frequency's ZH and ZFI
+// are bit-identical to launch_dyn_assembly's.  Both frequencies use the
+// table route (nt > 0).
+hipError_t launch_dyn_assembly_pair(hipStream_t st, int nbe, int npg2, const ElemData* elem,
+                                    const GaussPt* gp, const double* hdiag, const double* gdiag,
+                                    const int* kode, const cd* zdfi, int nrhs, int* colflag,
+                                    int* flist, const DynFreq (&F)[2], size_t ldzh,
+                                    hipEvent_t ev_od0, hipEvent_t ev_od1) {
+    const size_t ldr = 3 * (size_t)nbe;
+    DynPair P{};
+    for (int f = 0; f < 2; ++f) {
+        if (!F[f].etab || F[f].nt <= 0) return hipErrorInvalidValue;
+        P.K[f] = F[f].K;
+        P.T[f] = ExpTab{F[f].etab, F[f].etab + F[f].nt, F[f].h, 1.0 / F[f].h, F[f].nt};
+        P.zh[f] = F[f].zh;
+        hipLaunchKernelGGL(k_exp_table, dim3((F[f].nt + 255) / 256), dim3(256), 0, st, F[f].nt,
+                           F[f].h, F[f].K.ks, F[f].K.kp, F[f].K.c2, F[f].etab,
+                           F[f].etab + F[f].nt);
+    }
+    int* fcount = flist + nbe;
+    hipLaunchKernelGGL(k_colflags, dim3(1), dim3(256), 0, st, nbe, kode, zdfi, nrhs, ldr, colflag,
+                       flist, fcount);
+    for (int f = 0; f < 2; ++f)
+        hipLaunchKernelGGL(k_dyn_diag, dim3(nbe), dim3(64), 0, st, nbe, npg2, elem, gp, hdiag,
+                           gdiag, colflag, zdfi, nrhs, ldr, F[f].K, F[f].zh, ldzh,
+                           F[f].rhs_diag);
+    const int rowblk = (nbe + 64 * OD_WAVES - 1) / (64 * OD_WAVES);
+    if (ev_od0) (void)hipEventRecord(ev_od0, st);
+    hipLaunchKernelGGL((k_dyn_offdiag_h2<OD_WAVES, OD_JT>),
+                       dim3(rowblk, (nbe + OD_JT - 1) / OD_JT), dim3(64 * OD_WAVES),
+                       (size_t)OD_JT * npg2 * sizeof(GaussPt), st, nbe, npg2, elem, gp, colflag, P,
+                       ldzh);
+    if (ev_od1) (void)hipEventRecord(ev_od1, st);
+    const int nft = (nbe + NF_T - 1) / NF_T;
+    for (int f = 0; f < 2; ++f) {
+        hipLaunchKernelGGL(k_dyn_offdiag_near, dim3(nft, nft), dim3(256), 0, st, nbe, npg2, elem,
+                           gp, colflag, F[f].K, F[f].zh, ldzh);
+        hipLaunchKernelGGL((k_dyn_offdiag_g<OG_JT>),
+                           dim3((nbe + 63) / 64, std::min(offdiag_tiles(nbe), 64)),
+                           dim3(64 * OG_JT), (size_t)OG_JT * npg2 * sizeof(GaussPt), st, nbe,
+                           npg2, elem, gp, colflag, flist, fcount, zdfi, nrhs, ldr, F[f].K,
+                           F[f].zh, ldzh, F[f].rhs_part);
+        hipLaunchKernelGGL(k_rhs_reduce, dim3((3 * nbe + 255) / 256, nrhs), dim3(256), 0, st,
+                           3 * nbe, nrhs, ldr, OG_JT, fcount, F[f].rhs_part, F[f].rhs_diag,
+                           F[f].zfi);
+    }
     return hipGetLastError();
 }
 

@@ -29,6 +29,10 @@ hipError_t launch_dyn_assembly(hipStream_t, int, int, const ElemData*, const Gau
                                const double*, const double*, const int*, const cd*, int, int*,
                                int*, const DynConst&, cd*, int, double, cd*, size_t, cd*, cd*,
                                cd*, hipEvent_t, hipEvent_t);
+hipError_t launch_dyn_assembly_pair(hipStream_t, int, int, const ElemData*, const GaussPt*,
+                                    const double*, const double*, const int*, const cd*, int,
+                                    int*, int*, const DynFreq (&)[2], size_t, hipEvent_t,
+                                    hipEvent_t);
 int offdiag_tiles(int nbe);
 size_t exp_table_max_bytes();
 int exp_table_entries(const DynConst& K, double rmax, double* h);
@@ -92,6 +96,12 @@ struct bemx_ctx {
     const void* zh_host = nullptr;  // host array the last GHMATECD copied ZH to
     // interior
     DevBuf ipts, ipart, zdsol, zssol, ifi, idfi, ikode;
+    // frequency sweep (bemx_solve_sweep): four frequency slots (two batches of
+    // two), the assembly's stream and the batch events
+    static constexpr int NSW = 4;
+    DevBuf sw_zh[NSW], sw_zfi[NSW], sw_rhs_part[NSW], sw_rhs_diag[NSW], sw_etab[NSW];
+    hipStream_t st_asm = nullptr;
+    hipEvent_t sw_asm[2] = {nullptr, nullptr}, sw_done[2] = {nullptr, nullptr};
     // solver
     bemx::LuWork lu;
     // ev0..ev1: whole solve; ev2..ev3: off-diagonal assembly kernel;
@@ -195,6 +205,10 @@ static void ctx_buffers(bemx_ctx* c, std::vector<DevBuf*>& out) {
            &c->hdiag, &c->gdiag, &c->hpart, &c->kode, &c->zdfi, &c->colflag, &c->flist,
            &c->rhs_part, &c->rhs_diag, &c->zfi, &c->zh, &c->etab, &c->ipts, &c->ipart,
            &c->zdsol, &c->zssol, &c->ifi, &c->idfi, &c->ikode};
+    for (int i = 0; i < bemx_ctx::NSW; ++i)
+        for (DevBuf* d : {&c->sw_zh[i], &c->sw_zfi[i], &c->sw_rhs_part[i], &c->sw_rhs_diag[i],
+                          &c->sw_etab[i]})
+            out.push_back(d);
 }
 
 size_t ctx_device_bytes(bemx_ctx* c) {
@@ -208,6 +222,7 @@ size_t ctx_device_bytes(bemx_ctx* c) {
 void ctx_release_buffers(bemx_ctx* c) {
     hipSetDevice(c->device);
     if (c->st) hipStreamSynchronize(c->st);
+    if (c->st_asm) hipStreamSynchronize(c->st_asm);
     std::vector<DevBuf*> bufs;
     ctx_buffers(c, bufs);
     for (DevBuf* d : bufs) d->release();
@@ -286,9 +301,11 @@ int bemx_destroy(bemx_ctx* c) {
     hipSetDevice(c->device);
     if (c->st) hipStreamSynchronize(c->st);
     bemx::ctx_release_buffers(c);
-    hipEvent_t evs[] = {c->ev0, c->ev1, c->ev2, c->ev3, c->ev4};
+    hipEvent_t evs[] = {c->ev0,     c->ev1,     c->ev2,      c->ev3,     c->ev4,
+                        c->sw_asm[0], c->sw_asm[1], c->sw_done[0], c->sw_done[1]};
     for (hipEvent_t e : evs)
         if (e) hipEventDestroy(e);
+    if (c->st_asm) hipStreamDestroy(c->st_asm);
     if (c->st) hipStreamDestroy(c->st);
     delete c;
     return BEMX_OK;
@@ -787,6 +804,131 @@ int bemx_solve(bemx_ctx* c, const double* zge, const double* zcs, const double* 
         stats[2] = acc[2];
         stats[3] = acc[3];
     }
+    return BEMX_OK;
+}
+
+int bemx_solve_sweep(bemx_ctx* c, const double* zge, const double* zcs, const double* zcp,
+                     int nfreq, const double* freqs, int mode, double* zfi_out, double* stats) {
+    if (!c) return BEMX_ERR_ARG;
+    if (!c->mesh_ok || !c->dyn_ok || !c->static_ok)
+        return fail(c, BEMX_ERR_STATE, "%ssweep needs mesh, static pass and dynamic data", "", 0);
+    if (!zge || !zcs || !zcp || !freqs || nfreq < 1)
+        return fail(c, BEMX_ERR_ARG, "%sbad material or frequency list", "", 0);
+    for (int f = 0; f < nfreq; ++f)
+        if (!(freqs[f] != 0.0)) return fail(c, BEMX_ERR_ARG, "%szero frequency at %ld", "", f);
+    if (mode != BEMX_MODE_FP64 && mode != BEMX_MODE_MIXED)
+        return fail(c, BEMX_ERR_ARG, "%sunknown solver mode %ld", "", mode);
+    if (!c->case_groups.empty())
+        return fail(c, BEMX_ERR_ARG,
+                    "%sload cases with KODE sets of their own are solved with bemx_solve", "", 0);
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nbe = c->nbe, npg2 = c->npg * c->npg, nr = c->nrhs;
+    const size_t nn = 3 * (size_t)nbe;
+    std::vector<DynConst> K(nfreq);
+    std::vector<int> nt(nfreq);
+    std::vector<double> h(nfreq);
+    int ntmax = 0;
+    for (int f = 0; f < nfreq; ++f) {
+        K[f] = make_dyn_const(c, ld_c(zge), ld_c(zcs), ld_c(zcp), freqs[f], 1.0);
+        nt[f] = exp_table_entries(K[f], c->rmax, &h[f]);
+        ntmax = std::max(ntmax, nt[f]);
+    }
+    HIPCHK(c, c->colflag.ensure(sizeof(int) * nbe));
+    HIPCHK(c, c->flist.ensure(sizeof(int) * (nbe + 1)));
+    for (int i = 0; i < bemx_ctx::NSW; ++i) {
+        HIPCHK(c, c->sw_zh[i].ensure(sizeof(cd) * nn * nn));
+        HIPCHK(c, c->sw_zfi[i].ensure(sizeof(cd) * nn * nr));
+        HIPCHK(c, c->sw_rhs_part[i].ensure(sizeof(cd) * nn * offdiag_tiles(nbe) * nr));
+        HIPCHK(c, c->sw_rhs_diag[i].ensure(sizeof(cd) * nn * nr));
+        if (ntmax) HIPCHK(c, c->sw_etab[i].ensure(sizeof(cd) * 2 * (size_t)ntmax));
+    }
+    if (!c->st_asm) HIPCHK(c, hipStreamCreateWithFlags(&c->st_asm, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) {
+        if (!c->sw_asm[i]) HIPCHK(c, hipEventCreateWithFlags(&c->sw_asm[i], hipEventDisableTiming));
+        if (!c->sw_done[i])
+            HIPCHK(c, hipEventCreateWithFlags(&c->sw_done[i], hipEventDisableTiming));
+    }
+    c->zh_valid = false;
+    const ElemData* elem = c->elem.as<ElemData>();
+    const GaussPt* gp = c->gp.as<GaussPt>();
+    // batch b: frequencies 2b, 2b + 1 into slots 2 (b & 1), 2 (b & 1) + 1
+    auto assemble = [&](int b) -> int {
+        const int f0 = 2 * b, nf = std::min(2, nfreq - f0), s0 = 2 * (b & 1);
+        DynFreq F[2];
+        for (int i = 0; i < nf; ++i) {
+            const int f = f0 + i, s = s0 + i;
+            F[i] = DynFreq{K[f], nt[f] ? c->sw_etab[s].as<cd>() : nullptr, nt[f], h[f],
+                           c->sw_zh[s].as<cd>(), c->sw_rhs_part[s].as<cd>(),
+                           c->sw_rhs_diag[s].as<cd>(), c->sw_zfi[s].as<cd>()};
+        }
+        if (nf == 2 && nt[f0] > 0 && nt[f0 + 1] > 0) {
+            HIPCHK(c, launch_dyn_assembly_pair(c->st_asm, nbe, npg2, elem, gp,
+                                               c->hdiag.as<double>(), c->gdiag.as<double>(),
+                                               c->kode.as<int>(), c->zdfi.as<cd>(), nr,
+                                               c->colflag.as<int>(), c->flist.as<int>(), F, nn,
+                                               nullptr, nullptr));
+        } else {
+            for (int i = 0; i < nf; ++i)
+                HIPCHK(c, launch_dyn_assembly(c->st_asm, nbe, npg2, elem, gp,
+                                              c->hdiag.as<double>(), c->gdiag.as<double>(),
+                                              c->kode.as<int>(), c->zdfi.as<cd>(), nr,
+                                              c->colflag.as<int>(), c->flist.as<int>(), F[i].K,
+                                              F[i].etab, F[i].nt, F[i].h, F[i].zh, nn,
+                                              F[i].rhs_part, F[i].rhs_diag, F[i].zfi, nullptr,
+                                              nullptr));
+        }
+        HIPCHK(c, hipEventRecord(c->sw_asm[b & 1], c->st_asm));
+        return BEMX_OK;
+    };
+    const int nb = (nfreq + 1) / 2;
+    double acc[NLAST] = {0};
+    HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    HIPCHK(c, hipStreamWaitEvent(c->st_asm, c->ev0, 0));  // after the context's earlier work
+    int s = assemble(0);
+    for (int b = 0; b < nb && !s; ++b) {
+        // the next batch's assembly beside this batch's solves, once the
+        // solves that last used its slots are done
+        if (b + 1 < nb) {
+            if (b >= 1) HIPCHK(c, hipStreamWaitEvent(c->st_asm, c->sw_done[(b + 1) & 1], 0));
+            s = assemble(b + 1);
+            if (s) break;
+        }
+        HIPCHK(c, hipStreamWaitEvent(c->st, c->sw_asm[b & 1], 0));
+        for (int i = 0; i < std::min(2, nfreq - 2 * b) && !s; ++i) {
+            const int f = 2 * b + i, sl = 2 * (b & 1) + i;
+            long long inf = 0;
+            double st4[4] = {0, 0, 0, 0};
+            s = bemx::lu_solve(c->lu, c->st, (int)nn, c->sw_zh[sl].as<cd>(), nn,
+                               c->sw_zfi[sl].as<cd>(), nr, mode, &inf, st4, c->err);
+            if (s) break;
+            if (zfi_out)
+                HIPCHK(c, hipMemcpyAsync(zfi_out + 2 * nn * nr * f, c->sw_zfi[sl].p,
+                                         sizeof(cd) * nn * nr, hipMemcpyDeviceToHost, c->st));
+            if (stats) {
+                stats[4 * f] = st4[0];
+                stats[4 * f + 1] = st4[1];
+                stats[4 * f + 2] = st4[2];
+                stats[4 * f + 3] = 0.0;
+            }
+            acc[0] += st4[0];
+            acc[1] = std::max(acc[1], st4[1]);
+            acc[2] = std::max(acc[2], st4[2]);
+        }
+        if (!s) HIPCHK(c, hipEventRecord(c->sw_done[b & 1], c->st));
+    }
+    hipStreamSynchronize(c->st_asm);
+    if (s) {
+        hipStreamSynchronize(c->st);
+        return s;
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    acc[3] = ms;
+    acc[7] = (double)c->lu.captures;
+    acc[10] = (double)c->lu.replays;
+    memcpy(c->last, acc, sizeof acc);
     return BEMX_OK;
 }
 

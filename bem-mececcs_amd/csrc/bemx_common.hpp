@@ -85,6 +85,16 @@ struct DynConst {
     double t_a0, t_b0, t_c0;    // f_A,0 / (4 pi) - C3 C4, f_B,0 / (4 pi) - 3 C3, f_C,0 / (4 pi) + C3 C4
 };
 
+// One frequency of a batched sweep assembly: its constants, its exp table
+// (2 nt entries, spacing h) and its outputs.
+struct DynFreq {
+    DynConst K;
+    cd* etab;
+    int nt;
+    double h;
+    cd *zh, *rhs_part, *rhs_diag, *zfi;
+};
+
 // Gauss point of an element: position and weight OME(ig) OME(jg) DET.
 struct GaussPt {
     double x, y, z, w;

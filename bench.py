@@ -37,6 +37,11 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "bem-mececcs_amd"))
+# Hardware queues of this process (read by the HIP runtime at its first call):
+# HIP maps every stream onto one of them, so with 4 request slots x 2
+# streams the default of 4 queues serialises unrelated slots.  Measured on
+# the sweep: 4 slots reach 108 solves/s with 4 queues and 123-124 with 8 or 16.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 import numpy as np  # noqa: E402
 
@@ -368,6 +373,7 @@ def workload_default(args, api, decks):
     value = len(items) / wall
     fp64_value = len(items) / wall64
     svc = [i["service_ms"] for i in infos]
+    engine = sweep_engine(api, mesh, dyn, freqs, x64)
 
     serve = serve_slice(args, api, decks, args.requests, args.rate, args.check) \
         if args.requests > 0 else None
@@ -402,6 +408,7 @@ def workload_default(args, api, decks):
         "fallbacks_to_fp64": int(sum(i["fell_back"] for i in infos)),
         "graph_captures_after_warmup": w1["graph_captures"] - w0["graph_captures"],
         "single_solve": single,
+        "sweep_engine": engine,
         "serve": serve,
         "roofline": roof_lu,
         "roofline_assembly": roof_asm,
@@ -423,7 +430,54 @@ def workload_solve(args, api, decks):
     return out
 
 
+def sweep_engine(api, mesh, dyn, freqs, x64):
+    """The same sweep through bemx_solve_sweep on one context: frequencies
+    assembled two per launch on a second stream, beside the previous pair's
+    solves; warmed by a 4-frequency sweep (workspace, captured factorisation
+    graphs).  Every result is compared with the request loop's FP64 one."""
+    zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
+    ctx = api.Context(0)
+    try:
+        ctx.send_shared_data(mesh)
+        ctx.ghmatece(mesh.ge, mesh.rnu)
+        ctx.send_dyn_data(dyn.bc.astype(np.complex128), dyn.kode)
+        ctx.solve_sweep(zmat, freqs[:4], api.MODE_MIXED)
+        t0 = time.perf_counter()
+        xe, st = ctx.solve_sweep(zmat, freqs, api.MODE_MIXED)
+        wall = time.perf_counter() - t0
+    finally:
+        ctx.close()
+    err = max(rel_err(a, b) for a, b in zip(xe, x64))
+    log("sweep engine: %d solves in %.3f s on one context (%.2f solves/s), max err vs fp64 %.2e" %
+        (len(freqs), wall, len(freqs) / wall, err))
+    return {"value": len(freqs) / wall, "unit": "solves/s", "wall_s": wall,
+            "max_rel_err_vs_fp64": err, "gmres_iters_max": int(st[:, 0].max()),
+            "fallbacks_to_fp64": int(st[:, 2].sum()),
+            "note": "bemx_solve_sweep on one context: two frequencies per assembly launch "
+                    "(k_dyn_offdiag_h2) on a second stream beside the previous pair's "
+                    "factorisations and refinements"}
+
+
+def warm_main_thread(api, decks):
+    """One solve of the smallest serving class on the main thread, so the
+    process's first HIP work is not made by the request loop's workers."""
+    meshes, kodes, _ = decks.serving_stream(1, 1.0, seed=0)
+    m = meshes[0]
+    ctx = api.Context(0)
+    try:
+        ctx.send_shared_data(m)
+        ctx.ghmatece(m.ge, m.rnu)
+        z = np.zeros((3 * m.nbe, 1), np.complex128)
+        z[0, 0] = 1.0
+        ctx.send_load_cases(z, kodes[0])
+        for mode in (api.MODE_MIXED, api.MODE_FP64):
+            ctx.solve(api.material(m.ge, m.rnu, 0.01, 1.0), 1.0, mode)
+    finally:
+        ctx.close()
+
+
 def workload_serve(args, api, decks):
+    warm_main_thread(api, decks)
     serve = serve_slice(args, api, decks, args.requests, args.rate, args.check)
     out = base_record(args, {"workload": serve["workload"]}, serve["load_cases_per_s"],
                       1e3 / serve["load_cases_per_s"], serve["requests"])
@@ -484,7 +538,7 @@ def main():
                     help="sweep frequencies timed (default: all 256); other workloads: solves")
     ap.add_argument("--warmup", type=int, default=2, help="untimed requests per slot")
     ap.add_argument("--workload", default="sweep", choices=["sweep", "solve", "serve", "large"])
-    ap.add_argument("--slots", type=int, default=2, help="request-loop slots")
+    ap.add_argument("--slots", type=int, default=4, help="request-loop slots")
     ap.add_argument("--single-steps", type=int, default=10,
                     help="single-solve latency steps beside the sweep")
     ap.add_argument("--requests", type=int, default=None,

@@ -144,6 +144,21 @@ int bemx_interec(bemx_ctx *ctx, double *zfi, double *zdfi, const int *kode, int 
 int bemx_solve(bemx_ctx *ctx, const double *zge, const double *zcs, const double *zcp,
                double fr, int mode, double *zfi_out, double *stats);
 
+/* Frequency sweep on the resident mesh, static pass and load cases (one
+ * KODE for every case): for each of nfreq frequencies the dynamic assembly,
+ * factorisation (+ refinement in BEMX_MODE_MIXED) and solve of bemx_solve.
+ * Replaces the frequency loop of the reference's main program
+ * (Main.for:167-200: GHMATECD + LINSOLVE per frequency) for one material.
+ * Frequencies are assembled two at a time, the H blocks of both in one
+ * kernel launch (each frequency's matrix bit-identical to bemx_solve's), on a
+ * second stream that overlaps the previous pair's solves.  zfi_out (may be
+ * NULL): nfreq blocks of 3 NBE x nrhs complex solutions in frequency order;
+ * stats (may be NULL): 4 doubles per frequency (GMRES iterations, final
+ * relative residual, 1 if the FP64 fallback ran, 0).  Device memory: four
+ * frequency slots (ZH, ZFI and assembly partials each). */
+int bemx_solve_sweep(bemx_ctx *ctx, const double *zge, const double *zcs, const double *zcp,
+                     int nfreq, const double *freqs, int mode, double *zfi_out, double *stats);
+
 /* Statistics of the last bemx_linsolve / bemx_solve (the first nout of 11
  * doubles): GMRES iterations, final ||r||/||b||, 1 if the mixed path fell
  * back to FP64, solve device time (ms), off-diagonal assembly kernel time

@@ -1,0 +1,60 @@
+"""bemx_solve_sweep, the frequency loop of the reference's main program
+(Main.for:167-200: GHMATECD + LINSOLVE per frequency) on the device (GPU).
+
+Frequencies are assembled two at a time, the H blocks of both in one launch
+(k_dyn_offdiag_h2, the Gauss-point geometry shared), on a second stream that
+overlaps the previous pair's solves.  Each frequency's arithmetic is that of
+bemx_solve, so every solution must be bit-identical to solving that
+frequency alone: five frequencies (two pairs and a single one at the end),
+down to omega = 0.1 where the near-field series kernel carries most of the
+element pairs near the diagonal, with two load cases, in both modes.
+"""
+import numpy as np
+import pytest
+
+from bemx import api, decks
+
+pytestmark = pytest.mark.gpu
+
+FREQS = (0.1, 0.7, 2.0, 5.5, 9.0)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    mesh, dyn = decks.flat_patch_problem(24, 22, freqs=FREQS, n_interior=0, load_cases=2)
+    c = api.Context(0)
+    c.send_shared_data(mesh)
+    c.ghmatece(mesh.ge, mesh.rnu)
+    c.send_load_cases(dyn.load_cases.T.astype(np.complex128), dyn.kode)
+    yield c, mesh, dyn
+    c.close()
+
+
+@pytest.mark.parametrize("mode", [api.MODE_MIXED, api.MODE_FP64])
+def test_sweep_equals_each_frequency_alone(ctx, mode):
+    c, mesh, dyn = ctx
+    zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
+    print("sweep of", len(FREQS), "frequencies, mode", mode, flush=True)
+    xs, st = c.solve_sweep(zmat, FREQS, mode)
+    assert xs.shape == (len(FREQS), 3 * mesh.nbe, 2)
+    assert np.all(st[:, 2] == 0.0)  # no fallback to FP64
+    for k, fr in enumerate(FREQS):
+        x, s1 = c.solve(zmat, fr, mode)
+        print("omega", fr, "max |sweep - alone|", float(np.max(np.abs(xs[k] - x))), flush=True)
+        np.testing.assert_array_equal(xs[k], x)
+        assert st[k, 0] == s1[0]  # same GMRES iterations
+
+
+def test_sweep_refuses_per_case_kode(ctx):
+    c, mesh, dyn = ctx
+    zdfi = dyn.load_cases.T.astype(np.complex128)
+    kode = np.repeat(np.asarray(dyn.kode, dtype=np.int32)[:, None], 2, axis=1)
+    kode[0, 1] = 0  # the second case prescribes a displacement
+    c.send_load_cases(zdfi, kode)
+    try:
+        zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
+        with pytest.raises(api.BemxError) as ei:
+            c.solve_sweep(zmat, FREQS[:2], api.MODE_MIXED)
+        assert ei.value.code == 3  # BEMX_ERR_ARG
+    finally:
+        c.send_load_cases(zdfi, dyn.kode)
