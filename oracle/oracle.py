@@ -60,8 +60,9 @@ def _pi(a: np.ndarray):
 
 
 def build(ref: bool = False) -> None:
-    """Compile the oracle (and, with ref=True, the reference build)."""
-    targets = ["all"] + (["ref"] if ref else [])
+    """Compile the oracle (and, with ref=True, the reference build and the
+    reference's result checker, dtester)."""
+    targets = ["all"] + (["ref", "dtester"] if ref else [])
     subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
 
 

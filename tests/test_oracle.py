@@ -7,11 +7,12 @@ Kelvin static limit, Hooke's law on finite-difference gradients, the wave
 equation, a closed-form singular integral, LAPACK).
 """
 import os
+import subprocess
 
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, close_blocks
+from conftest import GOLDEN, ROOT, close_blocks
 from bemx import decks, outputs
 import oracle as oracle_mod
 
@@ -142,6 +143,84 @@ def test_oracle_reproduces_reference_goldens(orc, size):
             "du": o["zdsol"].reshape(-1, 3), "sigma": o["zssol"].reshape(-1, 9)}
     ok, err = outputs.compare_results(mine, gold)
     assert ok, err
+
+
+# ------------------------------------------------------------------ dtester's rule
+DTESTER = os.path.join(ROOT, "oracle", "_ref", "dtester")
+_DT_LABELS = {"Error in boundary nodes": "u", "Errors in tractions": "t",
+              "Errors in Internal Displacements": "du", "Errors in Internal Sigmas": "sigma"}
+
+
+def _run_dtester(path, path_sol):
+    r = subprocess.run([DTESTER, path, path_sol], capture_output=True, timeout=60)
+    err = {}
+    for line in r.stdout.decode("latin-1").splitlines():
+        label, _, val = line.partition(":")
+        if label in _DT_LABELS:
+            err[_DT_LABELS[label]] = float(val)
+    return r.returncode, err
+
+
+@pytest.mark.skipif(not os.path.exists(DTESTER),
+                    reason="oracle/_ref/dtester not built (make -C oracle dtester)")
+def test_compare_results_is_dtesters_rule(tmp_path):
+    """outputs.compare_results against the reference's own checker
+    (src/dtester.c, built from source by the oracle recipe) on copies of the
+    SSOLO240D golden with differences planted in its results block: equal
+    errors per section and the same verdict.  The plants separate a row
+    maximum from a sum (two rows of u differ), and the stresses' per-point
+    largest row from a sum over all nine entries."""
+    mesh, dyn = _deck(240)
+    nbe, l = mesh.nbe, dyn.l
+    gold_path = os.path.join(GOLDEN, "SSOLO240D_-5+5.SOL")
+    text = open(gold_path, "rb").read().decode("latin-1").replace("\r\n", "\n")
+    gold = outputs.parse_dynamic_results(text, nbe, l)
+    lines = text.split("\n")
+    banner = next(i for i, s in enumerate(lines) if "NCIA =" in s)
+    head = "\n".join(lines[:banner - 3]) + "\n"
+    fr = float(lines[banner].split("=")[1].replace("D", "E"))
+
+    def write(res, name):
+        body = outputs.format_frequency_block(fr, res["u"].ravel(), res["t"].ravel(), res["du"].ravel(),
+                                              res["sigma"].ravel(), nbe, l)
+        p = str(tmp_path / name)
+        open(p, "wb").write((head + body).encode("latin-1"))
+        return p
+
+    def plant(res, key, row, col, dz):
+        a = res[key]
+        a[row, col] += dz * max(1.0, abs(a[row, col]))
+
+    cases = []
+    same = {k: v.copy() for k, v in gold.items()}
+    cases.append(("unchanged", same))
+    small = {k: v.copy() for k, v in gold.items()}
+    plant(small, "u", 4, 1, 3e-4)
+    plant(small, "u", 8, 0, 2e-4j)
+    plant(small, "du", 1, 2, 4e-4)
+    plant(small, "sigma", 0, 0, 1e-4)
+    plant(small, "sigma", 0, 1, 2e-4)
+    plant(small, "sigma", 0, 8, 2.5e-4)
+    plant(small, "sigma", 3, 4, 1e-4)
+    cases.append(("within", small))
+    big = {k: v.copy() for k, v in small.items()}
+    plant(big, "t", 2, 0, 8e-4)
+    plant(big, "t", 2, 2, -8e-4j)
+    plant(big, "sigma", 5, 3, 7e-4)
+    cases.append(("beyond", big))
+
+    verdicts = []
+    for name, res in cases:
+        p = write(res, name + ".sol")
+        rc, dt = _run_dtester(p, gold_path)
+        mine = outputs.parse_dynamic_results(open(p, "rb").read().decode("latin-1"), nbe, l)
+        ok, err = outputs.compare_results(mine, gold)
+        assert set(dt) == set(err), (name, dt)
+        for k in err:
+            assert abs(err[k] - dt[k]) <= 1e-6 * max(dt[k], 1e-12), (name, k, err[k], dt[k])
+        assert ok == (rc == 0), (name, rc, err)
+        verdicts.append(ok)
+    assert verdicts == [True, True, False]
 
 
 # ------------------------------------------------------------------ mathematics
