@@ -8,7 +8,11 @@
 //                  local pivot candidates (largest |re|+|im|, lowest row on
 //                  ties, as LAPACK's IZAMAX) through tagged words, then the
 //                  row swap and the rank-1 update in LDS.  Also emits the
-//                  panel's row permutation.
+//                  panel's row permutation.  Orders whose panel needs more
+//                  workgroups of 128 rows than the device holds at once
+//                  (above 32768 on 256 CUs) use the tall form, whose
+//                  workgroups own 256 or more rows: the first 128 in LDS,
+//                  the rest updated in place in the matrix.
 //   k_rowperm      applies that permutation to other columns: inside the
 //                  group to its earlier L columns and its later columns,
 //                  after the group to the columns right of it (L columns of
@@ -191,7 +195,9 @@ __device__ inline void tpoll(const u64* p, unsigned tag, u64 (&w)[N], bool activ
 
 // ---------------------------------------------------------------------------
 // Panel factorisation.  Workgroup p owns panel rows [r0, r1) (global row
-// indices, r0 = k0 + p*PANEL_ROWS) held column-major in LDS.  Column j
+// indices, r0 = k0 + p*rpw) held column-major in LDS (TALL: the first
+// PANEL_ROWS of them; rows from PANEL_ROWS on stay in the matrix, where the
+// same thread updates them, one thread per row).  Column j
 // (diagonal row d = k0 + j) costs one inter-workgroup exchange through
 // tagged words:
 //   pick   wave 0 polls the P candidate slots of column j (best |re|+|im|
@@ -208,8 +214,9 @@ __device__ inline void tpoll(const u64* p, unsigned tag, u64 (&w)[N], bool activ
 // At the end workgroup 0 turns the pivot sequence into the list of rows
 // whose content changed (dst) and where it came from (src).
 // ---------------------------------------------------------------------------
+template <bool TALL>
 __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t lda, int n, int k0,
-                                                    int nb, int* __restrict__ ipiv,
+                                                    int nb, int rpw, int* __restrict__ ipiv,
                                                     u64* __restrict__ slots,
                                                     u64* __restrict__ cand, u64* __restrict__ drow,
                                                     int* __restrict__ perm,
@@ -224,11 +231,17 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
     __shared__ double s_pval, s_cval;
     const int P = gridDim.x, p = blockIdx.x, tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int r0 = k0 + p * PANEL_ROWS;
-    const int r1 = min(n, r0 + PANEL_ROWS);
+    const int r0 = k0 + p * rpw;
+    const int r1 = min(n, r0 + rpw);
     const int nr = r1 - r0;
-    for (int t = tid; t < nr * nb; t += 256) {
-        int c = t / nr, r = t - c * nr;
+    const int nl = min(nr, PANEL_ROWS);  // rows held in LDS
+    // entry (column c, owned row r) of the panel
+    auto el = [&](int c, int r) -> cd& {
+        if (!TALL || r < PANEL_ROWS) return sA[c * PANEL_ROWS + r];
+        return A[(size_t)(k0 + c) * lda + r0 + r];
+    };
+    for (int t = tid; t < nl * nb; t += 256) {
+        int c = t / nl, r = t - c * nl;
         sA[c * PANEL_ROWS + r] = A[(size_t)(k0 + c) * lda + r0 + r];
     }
     __syncthreads();
@@ -237,7 +250,16 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
     auto local_cand = [&](int jj, int dmin) {
         double bv = -1.0;
         int br = 0x7fffffff;
-        if (tid < nr && r0 + tid >= dmin) {
+        if (TALL) {
+            for (int r = tid; r < nr; r += 256)
+                if (r0 + r >= dmin) {
+                    const double v = cabs1(el(jj, r));
+                    if (better(v, r0 + r, bv, br)) {
+                        bv = v;
+                        br = r0 + r;
+                    }
+                }
+        } else if (tid < nr && r0 + tid >= dmin) {
             bv = cabs1(sA[jj * PANEL_ROWS + tid]);
             br = r0 + tid;
         }
@@ -271,14 +293,14 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
     auto publish = [&](int jj) {
         const int myrow = s_cand;
         if (myrow != 0x7fffffff && tid < nb) {
-            const cd v = sA[tid * PANEL_ROWS + (myrow - r0)];
+            const cd v = el(tid, myrow - r0);
             u64* q = cand + (((size_t)jj * P + p) * NB + tid) * 4;
             tput_d(q, tag, v.x);
             tput_d(q + 2, tag, v.y);
         }
         const int d = k0 + jj;
         if (d >= r0 && d < r1 && tid >= 64 && tid < 64 + nb) {
-            const cd v = sA[(tid - 64) * PANEL_ROWS + (d - r0)];
+            const cd v = el(tid - 64, d - r0);
             u64* q = drow + ((size_t)jj * NB + tid - 64) * 4;
             tput_d(q, tag, v.x);
             tput_d(q + 2, tag, v.y);
@@ -348,16 +370,17 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
         __syncthreads();
         // apply: swap, scale column j, update column j+1
         if (piv != d) {
-            if (d >= r0 && d < r1 && tid < nb) sA[tid * PANEL_ROWS + (d - r0)] = urow[tid];
-            if (piv >= r0 && piv < r1 && tid < nb) sA[tid * PANEL_ROWS + (piv - r0)] = dbuf[tid];
+            if (d >= r0 && d < r1 && tid < nb) el(tid, d - r0) = urow[tid];
+            if (piv >= r0 && piv < r1 && tid < nb) el(tid, piv - r0) = dbuf[tid];
         }
         __syncthreads();
         const cd inv = (pval != 0.0) ? cinv(urow[j]) : mk(0.0, 0.0);
-        const bool live = tid < nr && r0 + tid > d && pval != 0.0;
-        if (live) {
-            const cd l = sA[j * PANEL_ROWS + tid] * inv;
-            sA[j * PANEL_ROWS + tid] = l;
-            if (j + 1 < nb) sA[(j + 1) * PANEL_ROWS + tid] = cfms(sA[(j + 1) * PANEL_ROWS + tid], l, urow[j + 1]);
+        for (int ri = tid; ri < (TALL ? nr : min(nr, 256)); ri += 256) {
+            if (r0 + ri > d && pval != 0.0) {
+                const cd l = el(j, ri) * inv;
+                el(j, ri) = l;
+                if (j + 1 < nb) el(j + 1, ri) = cfms(el(j + 1, ri), l, urow[j + 1]);
+            }
         }
         __syncthreads();
         int sp1 = -1, sp2 = -1;  // local rows already completed for columns j+2..
@@ -369,21 +392,41 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
             // complete the candidate row (threads 0..63) and row d+1 (64..127)
             const int sr = tid < 64 ? sp1 : (tid < 128 ? sp2 : -1);
             const int c = j + 2 + lane;
-            if (sr >= 0 && c < nb && pval != 0.0)
-                sA[c * PANEL_ROWS + sr] = cfms(sA[c * PANEL_ROWS + sr], sA[j * PANEL_ROWS + sr], urow[c]);
+            if (sr >= 0 && c < nb && pval != 0.0) el(c, sr) = cfms(el(c, sr), el(j, sr), urow[c]);
             __syncthreads();
             publish(j + 1);
         }
         // the rest of the rank-1 update (columns j+2..), overlapping the exchange
-        if (r < nr && r0 + r > d && r != sp1 && r != sp2 && pval != 0.0) {
+        if (TALL) {
+            // one thread per row; a row's entries in the matrix are loaded
+            // eight at a time so their latencies overlap
+            for (int rr = tid; rr < nr; rr += 256) {
+                if (r0 + rr > d && rr != sp1 && rr != sp2 && pval != 0.0) {
+                    const cd l = el(j, rr);
+                    int c = j + 2;
+                    if (rr >= PANEL_ROWS) {
+                        cd* col = &A[(size_t)k0 * lda + r0 + rr];
+                        for (; c + 8 <= nb; c += 8) {
+                            cd v[8];
+#pragma unroll
+                            for (int u = 0; u < 8; ++u) v[u] = col[(size_t)(c + u) * lda];
+#pragma unroll
+                            for (int u = 0; u < 8; ++u)
+                                col[(size_t)(c + u) * lda] = cfms(v[u], l, urow[c + u]);
+                        }
+                    }
+                    for (; c < nb; ++c) el(c, rr) = cfms(el(c, rr), l, urow[c]);
+                }
+            }
+        } else if (r < nr && r0 + r > d && r != sp1 && r != sp2 && pval != 0.0) {
             const cd l = sA[j * PANEL_ROWS + r];
             for (int c = j + 2 + half; c < nb; c += 2)
                 sA[c * PANEL_ROWS + r] = cfms(sA[c * PANEL_ROWS + r], l, urow[c]);
         }
         __syncthreads();
     }
-    for (int t = tid; t < nr * nb; t += 256) {
-        int c = t / nr, r2 = t - c * nr;
+    for (int t = tid; t < nl * nb; t += 256) {
+        int c = t / nl, r2 = t - c * nl;
         A[(size_t)(k0 + c) * lda + r0 + r2] = sA[c * PANEL_ROWS + r2];
     }
     if (p == 0 && wave == 0) {
@@ -890,7 +933,9 @@ std::string coop_refused_message(const char* what) {
 
 hipError_t preload_lu_code() {
     hipFuncAttributes a;
-    return hipFuncGetAttributes(&a, (const void*)k_panel_fp64);
+    hipError_t e = hipFuncGetAttributes(&a, (const void*)k_panel_fp64<false>);
+    if (e == hipSuccess) e = hipFuncGetAttributes(&a, (const void*)k_panel_fp64<true>);
+    return e;
 }
 
 // Panels per group of the FP64 factorisation: inside a group each panel
@@ -930,6 +975,23 @@ static unsigned tag_block(int n) {
     }
 }
 
+// Rows per panel workgroup for a panel of m rows when at most cap
+// workgroups can be co-resident: PANEL_ROWS while that suffices, else the
+// smallest multiple of PANEL_ROWS that fits (the tall form).  The environment
+// variable BEMX_PANEL_ROWS (a multiple of PANEL_ROWS, read at every
+// factorisation) raises it, so the tall form can be compared with the plain
+// one at orders both handle.
+static int forced_panel_rows() {
+    const char* v = std::getenv("BEMX_PANEL_ROWS");
+    const int r = v ? std::atoi(v) : 0;
+    return r > PANEL_ROWS && r % PANEL_ROWS == 0 ? r : PANEL_ROWS;
+}
+static int panel_rows(int m, int cap, int forced) {
+    const int need = (m + PANEL_ROWS - 1) / PANEL_ROWS;
+    const int rpw = cap > 0 && need > cap ? PANEL_ROWS * ((need + cap - 1) / cap) : PANEL_ROWS;
+    return std::max(rpw, forced);
+}
+
 // tw (may be NULL): the workspace whose events time the factorisation.
 static int factor_fp64(const Fp64Ws& ws, hipStream_t st, int n, cd* A, size_t lda, LuWork* tw,
                        std::string& err) {
@@ -944,10 +1006,15 @@ static int factor_fp64(const Fp64Ws& ws, hipStream_t st, int n, cd* A, size_t ld
     const unsigned tag0 = tag_block(npanel);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, (int*)info, 0, 2);
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, status, 0, 1);
-    int dev = 0, ncu = 0, perCU = 0;
+    int dev = 0, ncu = 0, perCU = 0, perCUt = 0;
     LU_CHK(hipGetDevice(&dev));
     LU_CHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    LU_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_panel_fp64, 256, 0));
+    LU_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_panel_fp64<false>, 256, 0));
+    LU_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCUt, k_panel_fp64<true>, 256, 0));
+    // co-resident panel workgroups: the device's limit, and at most 256
+    // (the candidate slots are polled by one wave, four per lane)
+    const int cap = std::min(256, ncu * std::min(perCU, perCUt));
+    const int forced = forced_panel_rows();
     if (tw) {
         LU_CHK(tw->ensure_side());
         LU_CHK(hipEventRecord(tw->ev_f0, st));
@@ -970,17 +1037,20 @@ static int factor_fp64(const Fp64Ws& ws, hipStream_t st, int n, cd* A, size_t ld
             const int k0 = pk * NB;
             const int nb = std::min(NB, n - k0);
             const int m = n - k0;
-            const int P = (m + PANEL_ROWS - 1) / PANEL_ROWS;
-            if (P > ncu * perCU) {
+            int rpw = panel_rows(m, cap, forced);
+            const int P = (m + rpw - 1) / rpw;
+            if (cap < 1 || P > cap) {
                 err = "LU panel needs more co-resident workgroups than the device holds";
                 return BEMX_ERR_ARG;
             }
             int* permk = permk_of(pk);
             unsigned tag = tag0 + (unsigned)pk;
-            void* args[] = {&A, &lda, (void*)&n, (void*)&k0, (void*)&nb, &ipiv, &slots, &cand,
-                            &drow, &permk, &info, &status, &tag};
-            const hipError_t ec = hipLaunchCooperativeKernel((const void*)k_panel_fp64, dim3(P),
-                                                             dim3(256), args, 0, st);
+            void* args[] = {&A, &lda, (void*)&n, (void*)&k0, (void*)&nb, &rpw, &ipiv, &slots,
+                            &cand, &drow, &permk, &info, &status, &tag};
+            const void* kfn = rpw > PANEL_ROWS ? (const void*)k_panel_fp64<true>
+                                               : (const void*)k_panel_fp64<false>;
+            const hipError_t ec =
+                hipLaunchCooperativeKernel(kfn, dim3(P), dim3(256), args, 0, st);
             if (ec == hipErrorIllegalState) {
                 err = coop_refused_message("LU panel");
                 return BEMX_ERR_HIP;

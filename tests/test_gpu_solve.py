@@ -58,6 +58,30 @@ def test_pivoting_across_workgroups(ctx, n):
     _check(a, b, x)
 
 
+@pytest.mark.parametrize("rows", [256, 384])
+def test_tall_panel_equals_plain_panel(ctx, rows, monkeypatch):
+    """The FP64 panel's tall form (workgroups of 256 or 384 rows: the first
+    128 in LDS, the rest updated in place in the matrix), which orders above
+    32768 need, forced at an order the plain form also handles: every column
+    pivots to a row far below it, in another workgroup or among a
+    workgroup's rows kept in the matrix.  Both forms pick the same pivots and
+    apply the same operations to every entry, so the solutions are
+    bit-identical."""
+    n = 3000
+    rng = np.random.default_rng(21)
+    a = 1e-3 * (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)))
+    a[rng.permutation(n), np.arange(n)] += 10.0 + 5j
+    a = np.asfortranarray(a)
+    b = rng.normal(size=n) + 1j * rng.normal(size=n)
+    print("plain panel, n", n, flush=True)
+    x0 = ctx.linsolve(a, b)
+    monkeypatch.setenv("BEMX_PANEL_ROWS", str(rows))
+    print("tall panel, rows per workgroup", rows, flush=True)
+    x1 = ctx.linsolve(a, b)
+    np.testing.assert_array_equal(x1, x0)
+    _check(a, b, x1)
+
+
 def test_tied_pivot_candidates(ctx):
     """Equal |re|+|im| candidates: the lowest row wins, as IZAMAX."""
     n = 96

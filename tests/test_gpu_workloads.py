@@ -7,6 +7,8 @@
     mu drawn by seed): HESTdiag/GESTdiag against the oracle's GHMATECE,
     sampled ZH blocks (random, diagonal, and adjacent-element pairs) against
     the oracle's blocks, and the mixed solution against the FP64 one.
+  * 120 x 120 patch (14400 BE, 43200 DOF): the FP64 solution, its panel in
+    the tall form that orders above 32768 need, against the mixed one.
   * frequency sweep on the 48 x 45 patch (256 omega log-spaced in [0.1, 10]):
     at omega in {0.1, 0.3, 1, 3.2, 10} sampled ZH blocks and all of ZFI
     against the oracle (the exp-table route of the H kernel at small and
@@ -151,6 +153,25 @@ def test_patch_96x90_large_solve(ctx, orc):
     xm, st = ctx.solve(zmat, fr, api.MODE_MIXED)
     x64, _ = ctx.solve(zmat, fr, api.MODE_FP64)
     _say("mixed vs fp64", _rel(xm, x64), "gmres iters", st[0])
+    assert _rel(xm, x64) <= TOL_MIXED and st[2] == 0.0
+
+
+def test_patch_120x120_beyond_32768_unknowns(ctx):
+    """43200 unknowns (a 120 x 120 patch, the order of the reference's
+    ESOLO14400 deck): the FP64 panel needs more 128-row workgroups than the
+    device holds at once and runs in its tall form.  Its solution against the
+    mixed path's, whose GMRES-IR residuals come from the same FP64 matrix."""
+    mesh, dyn = decks.flat_patch_problem(120, 120, half=10.0, npg=6, freqs=(0.5,), n_interior=0)
+    assert 3 * mesh.nbe > 32768
+    _say("120x120 patch: unknowns", 3 * mesh.nbe)
+    ctx.send_shared_data(mesh)
+    ctx.ghmatece(mesh.ge, mesh.rnu)
+    zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
+    ctx.send_dyn_data(dyn.bc.astype(np.complex128), dyn.kode)
+    xm, st = ctx.solve(zmat, 0.5, api.MODE_MIXED)
+    _say("mixed done, gmres iters", st[0])
+    x64, _ = ctx.solve(zmat, 0.5, api.MODE_FP64)
+    _say("120x120 mixed vs fp64", _rel(xm, x64))
     assert _rel(xm, x64) <= TOL_MIXED and st[2] == 0.0
 
 
