@@ -38,7 +38,6 @@
 // caller falls back to the FP64 partial-pivot LU.
 #include <algorithm>
 #include <cmath>
-#include <complex>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -1447,13 +1446,106 @@ __global__ void k_scale_by_norm(int n, const cd* __restrict__ w, cd* __restrict_
     out[row] = hn > 0.0 ? w[row] * (1.0 / hn) : mk(0, 0);
 }
 
-// x += sum_i y_i V_i
-__global__ void k_combine(int n, const cd* __restrict__ V, size_t ldv, int nv,
-                          const cd* __restrict__ y, cd* __restrict__ x) {
-    int row = blockIdx.x * blockDim.x + threadIdx.x;
+constexpr int GM_RESTART = 12;  // longest GMRES cycle of the refinement
+
+// One GMRES cycle's state on the device: the norms the host reads back
+// (head), the Hessenberg matrix (column-major, GM_RESTART + 1 rows), the
+// Givens rotations, the rotated right-hand side g and the least-squares
+// solution y.
+struct GmHead {
+    cd bn2, rn2, beta2;  // ||b||^2, ||r||^2, ||M^-1 r||^2 (real parts)
+    double beta, gres;   // beta and |g[kk]|, the cycle's residual estimate
+    int kk, stop;        // Arnoldi steps in the solution; converged / broken down
+};
+struct GmState {
+    GmHead head;
+    cd H[(GM_RESTART + 1) * GM_RESTART];
+    cd cs[GM_RESTART], sn[GM_RESTART], g[GM_RESTART + 1], y[GM_RESTART];
+};
+
+__device__ inline cd conjz(cd a) { return mk(a.x, -a.y); }
+__device__ inline double absz(cd a) { return hypot(a.x, a.y); }
+// a / b by Smith's method (no overflow of |b|^2)
+__device__ inline cd divz(cd a, cd b) {
+    if (fabs(b.x) >= fabs(b.y)) {
+        const double t = b.y / b.x, d = b.x + b.y * t;
+        return mk((a.x + a.y * t) / d, (a.y - a.x * t) / d);
+    }
+    const double t = b.x / b.y, d = b.x * t + b.y;
+    return mk((a.x * t + a.y) / d, (a.y * t - a.x) / d);
+}
+
+// Start of a cycle: beta = ||M^-1 r||, g = beta e1.
+__global__ void k_gm_start(GmState* gs) {
+    if (threadIdx.x != 0) return;
+    GmHead& h = gs->head;
+    h.beta = sqrt(h.beta2.x);
+    for (int i = 0; i <= GM_RESTART; ++i) gs->g[i] = mk(0.0, 0.0);
+    gs->g[0] = mk(h.beta, 0.0);
+    h.gres = h.beta;
+    h.kk = 0;
+    h.stop = h.beta == 0.0;
+}
+
+// Arnoldi step k into the cycle: column k of H from the two Gram-Schmidt
+// passes' projections (hp, hq) and the new vector's norm (hn2), the earlier
+// rotations applied to it, a new rotation that zeroes H[k+1][k], and g
+// rotated with it.  After convergence (|g[k+1]| <= tol beta) or a breakdown
+// (zero norm) later steps leave the state alone.
+__global__ void k_gm_step(GmState* gs, int k, const cd* __restrict__ hp,
+                          const cd* __restrict__ hq, const cd* __restrict__ hn2, double tol) {
+    if (threadIdx.x != 0) return;
+    GmHead& h = gs->head;
+    if (h.stop) return;
+    constexpr int LD = GM_RESTART + 1;
+    cd* col = gs->H + (size_t)k * LD;
+    for (int i = 0; i <= k; ++i) col[i] = hp[i] + hq[i];
+    const double hn = sqrt(hn2[0].x);
+    col[k + 1] = mk(hn, 0.0);
+    cd *cs = gs->cs, *sn = gs->sn, *g = gs->g;
+    for (int i = 0; i < k; ++i) {
+        const cd t = conjz(cs[i]) * col[i] + conjz(sn[i]) * col[i + 1];
+        col[i + 1] = -(sn[i] * col[i]) + cs[i] * col[i + 1];
+        col[i] = t;
+    }
+    const double a = absz(col[k]), bb = absz(col[k + 1]);
+    const double den = sqrt(a * a + bb * bb);
+    if (den == 0.0) {
+        cs[k] = mk(1.0, 0.0);
+        sn[k] = mk(0.0, 0.0);
+    } else {
+        cs[k] = col[k] * (1.0 / den);
+        sn[k] = col[k + 1] * (1.0 / den);
+    }
+    col[k] = conjz(cs[k]) * col[k] + conjz(sn[k]) * col[k + 1];
+    col[k + 1] = mk(0.0, 0.0);
+    g[k + 1] = -(sn[k] * g[k]);
+    g[k] = conjz(cs[k]) * g[k];
+    h.kk = k + 1;
+    h.gres = absz(g[k + 1]);
+    if (h.gres <= tol * h.beta || hn == 0.0) h.stop = 1;
+}
+
+// y = H^-1 g over the cycle's kk steps (back substitution).
+__global__ void k_gm_solve(GmState* gs) {
+    if (threadIdx.x != 0) return;
+    constexpr int LD = GM_RESTART + 1;
+    const int kk = gs->head.kk;
+    for (int i = kk - 1; i >= 0; --i) {
+        cd t = gs->g[i];
+        for (int j = i + 1; j < kk; ++j) t = t - gs->H[(size_t)j * LD + i] * gs->y[j];
+        gs->y[i] = divz(t, gs->H[(size_t)i * LD + i]);
+    }
+}
+
+// x += sum_{i < kk} y_i V_i
+__global__ void k_combine_gm(int n, const cd* __restrict__ V, size_t ldv,
+                             const GmState* __restrict__ gs, cd* __restrict__ x) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= n) return;
+    const int kk = gs->head.kk;
     cd v = x[row];
-    for (int i = 0; i < nv; ++i) v = v + y[i] * V[(size_t)i * ldv + row];
+    for (int i = 0; i < kk; ++i) v = v + gs->y[i] * V[(size_t)i * ldv + row];
     x[row] = v;
 }
 
@@ -1511,7 +1603,6 @@ static int group_blocks() {
     return g;
 }
 
-constexpr int GM_RESTART = 12;  // GMRES restart length of the refinement
 
 // Bytes of each workspace buffer of the mixed path (factorisation and
 // refinement) for order n, by LuWork slot (0 for slots it does not use).
@@ -1533,7 +1624,7 @@ static void mixed_sizes(int n, size_t* sz) {
     sz[M_VEC] = sizeof(cd) * n * 5;                     // x, r, w, tmp, b
     sz[M_V] = sizeof(cd) * n * (MR + 1);                // GMRES basis
     sz[M_PART] = sizeof(cd) * std::max(nchunk * n, nb256 * (MR + 2));
-    sz[M_HOST] = sizeof(cd) * (MR + 2) * 3;             // Arnoldi coefficients
+    sz[M_HOST] = sizeof(cd) * (MR + 2) * 3 + sizeof(GmState);  // Arnoldi scratch, cycle state
 }
 
 size_t mixed_workspace_bytes(int n) {
@@ -1791,15 +1882,22 @@ static void gemv(hipStream_t st, int n, const cd* A, size_t lda, const cd* x, co
                        sub);
 }
 
-typedef std::complex<double> zx;
-
-// GMRES-IR of one right-hand side against the FP32 factors F.
+// GMRES-IR of one right-hand side against the FP32 factors F.  The outer
+// iteration (residual b - A x in FP64, then a GMRES cycle on M^-1 A d =
+// M^-1 r) runs on the device from start to end: the Arnoldi steps, the
+// Givens rotations (k_gm_step), the least-squares solution and x += V y
+// (k_gm_solve, k_combine_gm), then the next residual and its norm.  The host
+// reads the norms and the cycle's state once per outer iteration to decide
+// whether to stop.  A cycle takes `ch` Arnoldi steps (2 at first; doubled up
+// to GM_RESTART while a cycle ends short of its inner tolerance) and stops
+// early on the device once |g| <= 1e-8 beta or on breakdown.
 static int refine_one(LuWork& w, hipStream_t st, const MixedFactor& F, int n, const cd* A64,
                       size_t lda, cd* b, int* iters, double* relres_out, std::string& err) {
     hipError_t e;
     const int MR = GM_RESTART;
     const int MAXOUT = 12;    // refinement steps
     const double TOL = 2e-15; // normwise backward error target
+    const double TOL_IN = 1e-8;  // inner (GMRES cycle) reduction
     const int nb256 = (n + 255) / 256;
     size_t sz[LuWork::SLOTS];  // reserved by factor_mixed: no reallocation here
     mixed_sizes(n, sz);
@@ -1809,16 +1907,16 @@ static int refine_one(LuWork& w, hipStream_t st, const MixedFactor& F, int n, co
        *bsave = vec + 4 * (size_t)n;
     cd* V = (cd*)w.get(M_V, sz[M_V], &e);
     MX_CHK(e);
-    const int nchunk = (n + MV_COLS - 1) / MV_COLS;
     cd* part = (cd*)w.get(M_PART, sz[M_PART], &e);
     MX_CHK(e);
-    // device scratch of the Arnoldi step: projections of the two
-    // Gram-Schmidt passes and the norm (slots of MR + 2 entries each)
+    // device scratch of the Arnoldi step (projections of the two
+    // Gram-Schmidt passes and the norm, slots of MR + 2 entries each), then
+    // the cycle's state
     const int HS = MR + 2;
     cd* hdev = (cd*)w.get(M_HOST, sz[M_HOST], &e);
     MX_CHK(e);
     cd *h1 = hdev + HS, *h2 = hdev + 2 * HS;
-    std::vector<cd> hstep((size_t)HS * 3);
+    GmState* gs = (GmState*)(hdev + 3 * HS);
     // k_dots over nv vectors + reduction into out (nv + 1 entries, the last
     // one ||wvec||^2), all on the device
     auto dots_dev = [&](const cd* Vb, int nv, const cd* wvec, cd* out) {
@@ -1826,44 +1924,40 @@ static int refine_one(LuWork& w, hipStream_t st, const MixedFactor& F, int n, co
         hipLaunchKernelGGL(k_dots_reduce, dim3(1), dim3(64), 0, st, nb256, nv + 1,
                            (const cd*)part, out);
     };
-    std::vector<cd> hpart((size_t)nb256 * (MR + 2));
-    auto dots = [&](const cd* Vb, int nv, const cd* wvec, std::vector<zx>& out, double& nrm2) -> int {
-        hipLaunchKernelGGL(k_dots, dim3(nb256), dim3(256), 0, st, n, Vb, (size_t)n, nv, wvec, part);
-        MX_CHK(hipMemcpyAsync(hpart.data(), part, sizeof(cd) * nb256 * (nv + 1),
-                              hipMemcpyDeviceToHost, st));
-        MX_CHK(hipStreamSynchronize(st));
-        out.assign(nv, zx(0, 0));
-        nrm2 = 0;
-        for (int bk = 0; bk < nb256; ++bk) {
-            for (int i = 0; i < nv; ++i)
-                out[i] += zx(hpart[(size_t)bk * (nv + 1) + i].x, hpart[(size_t)bk * (nv + 1) + i].y);
-            nrm2 += hpart[(size_t)bk * (nv + 1) + nv].x;
-        }
-        return 0;
-    };
-    // norms for the backward error: ||A||_inf estimate via |A| row sums is
-    // costly; use ||b|| and the residual relative to ||A x|| instead.
-    hipLaunchKernelGGL(k_copy_z, dim3(nb256), dim3(256), 0, st, b, bsave, n);
-    {
-        const hipError_t ep = precond(st, F, bsave, x, tmp);  // x0 = M^-1 b
+    auto pre = [&](const cd* v, cd* z) -> int {
+        const hipError_t ep = precond(st, F, v, z, tmp);
         if (ep == hipErrorIllegalState) {
             err = coop_refused_message("mixed solve, block solves");
             return BEMX_ERR_HIP;
         }
         MX_CHK(ep);
-    }
-    std::vector<zx> h;
-    double nrm2 = 0, bnorm = 0, relres = 1.0;
-    int it_total = 0;
-    dots(bsave, 0, bsave, h, nrm2);
-    bnorm = std::sqrt(nrm2);
+        return BEMX_OK;
+    };
+    // the cycle's norms and state, and the status word, in one round trip
+    GmHead hd;
+    int hst = 0;
+    auto readback = [&]() -> int {
+        MX_CHK(hipMemcpyAsync(&hd, gs, sizeof(GmHead), hipMemcpyDeviceToHost, st));
+        MX_CHK(hipMemcpyAsync(&hst, F.status, sizeof(int), hipMemcpyDeviceToHost, st));
+        MX_CHK(hipStreamSynchronize(st));
+        if (hst & ST_SYNC_TIMEOUT) {
+            err = "mixed solve: a device-side wait of the triangular solves timed out";
+            return BEMX_ERR_TIMEOUT;
+        }
+        return BEMX_OK;
+    };
+    hipLaunchKernelGGL(k_copy_z, dim3(nb256), dim3(256), 0, st, b, bsave, n);
+    int s = pre(bsave, x);  // x0 = M^-1 b
+    if (s) return s;
+    dots_dev(bsave, 0, bsave, &gs->head.bn2);
+    gemv(st, n, A64, lda, x, bsave, r, part, 1);  // r = b - A x (FP64)
+    dots_dev(r, 0, r, &gs->head.rn2);
+    if ((s = readback())) return s;
+    const double bnorm = std::sqrt(hd.bn2.x);
+    double relres = std::sqrt(hd.rn2.x) / std::max(bnorm, 1e-300), prev = relres;
+    int it_total = 0, ch = 2;
     bool converged = false;
-    for (int outer = 0; outer < MAXOUT; ++outer) {
-        gemv(st, n, A64, lda, x, bsave, r, part, 1);  // r = b - A x (FP64)
-        double rn2 = 0;
-        dots(r, 0, r, h, rn2);
-        double prev = relres;
-        relres = std::sqrt(rn2) / std::max(bnorm, 1e-300);
+    for (int outer = 0;; ++outer) {
         // ||A x|| ~ ||b||, so ||r|| <= TOL ||b|| bounds the normwise backward
         // error ||r|| / (||A|| ||x|| + ||b||) by TOL; a residual that stops
         // shrinking at a level near the FP64 rounding of A x is accepted too.
@@ -1871,27 +1965,17 @@ static int refine_one(LuWork& w, hipStream_t st, const MixedFactor& F, int n, co
             converged = true;
             break;
         }
-        // GMRES on M^-1 A d = M^-1 r
-        MX_CHK(precond(st, F, r, V, tmp));
-        double beta2 = 0;
-        dots(V, 0, V, h, beta2);
-        double beta = std::sqrt(beta2);
-        if (beta == 0.0) {
-            converged = true;
-            break;
-        }
-        hipLaunchKernelGGL(k_orth_update, dim3(nb256), dim3(256), 0, st, n, V, (size_t)n, 0, hdev,
-                           V, V, 1.0 / beta);
-        std::vector<std::vector<zx>> H(MR + 1, std::vector<zx>(MR, zx(0, 0)));
-        std::vector<zx> cs(MR), sn(MR), g(MR + 1, zx(0, 0));
-        g[0] = beta;
-        int k = 0;
-        for (; k < MR; ++k) {
-            ++it_total;
+        if (outer == MAXOUT) break;
+        // GMRES cycle on M^-1 A d = M^-1 r: v0 = M^-1 r / beta
+        if ((s = pre(r, V))) return s;
+        dots_dev(V, 0, V, &gs->head.beta2);
+        hipLaunchKernelGGL(k_scale_by_norm, dim3(nb256), dim3(256), 0, st, n, V, V,
+                           &gs->head.beta2);
+        hipLaunchKernelGGL(k_gm_start, dim3(1), dim3(64), 0, st, gs);
+        for (int k = 0; k < ch; ++k) {
             gemv(st, n, A64, lda, V + (size_t)k * n, nullptr, wv, part, 0);
-            MX_CHK(precond(st, F, wv, wv, tmp));
-            // classical Gram-Schmidt, two passes, and the normalisation, all
-            // on the device; one copy of the coefficients to the host
+            if ((s = pre(wv, wv))) return s;
+            // classical Gram-Schmidt, two passes, and the normalisation
             dots_dev(V, k + 1, wv, hdev);
             hipLaunchKernelGGL(k_orth_update, dim3(nb256), dim3(256), 0, st, n, V, (size_t)n,
                                k + 1, hdev, wv, wv, 1.0);
@@ -1901,67 +1985,32 @@ static int refine_one(LuWork& w, hipStream_t st, const MixedFactor& F, int n, co
             dots_dev(V, 0, wv, h2);
             hipLaunchKernelGGL(k_scale_by_norm, dim3(nb256), dim3(256), 0, st, n, wv,
                                V + (size_t)(k + 1) * n, h2);
-            MX_CHK(hipMemcpyAsync(hstep.data(), hdev, sizeof(cd) * HS * 3, hipMemcpyDeviceToHost,
-                                  st));
-            MX_CHK(hipStreamSynchronize(st));
-            const double hn = std::sqrt(hstep[2 * HS].x);
-            for (int i = 0; i <= k; ++i)
-                H[i][k] = (zx(0, 0) + zx(hstep[i].x, hstep[i].y)) +
-                          zx(hstep[HS + i].x, hstep[HS + i].y);
-            H[k + 1][k] = hn;
-            // Givens
-            for (int i = 0; i < k; ++i) {
-                zx t = std::conj(cs[i]) * H[i][k] + std::conj(sn[i]) * H[i + 1][k];
-                H[i + 1][k] = -sn[i] * H[i][k] + cs[i] * H[i + 1][k];
-                H[i][k] = t;
-            }
-            double a = std::abs(H[k][k]), bb = std::abs(H[k + 1][k]);
-            double den = std::sqrt(a * a + bb * bb);
-            if (den == 0) {
-                cs[k] = 1;
-                sn[k] = 0;
-            } else {
-                cs[k] = H[k][k] / den;
-                sn[k] = H[k + 1][k] / den;
-            }
-            H[k][k] = std::conj(cs[k]) * H[k][k] + std::conj(sn[k]) * H[k + 1][k];
-            H[k + 1][k] = 0;
-            g[k + 1] = -sn[k] * g[k];
-            g[k] = std::conj(cs[k]) * g[k];
-            if (std::abs(g[k + 1]) <= 1e-8 * beta || hn == 0.0) {
-                ++k;
-                break;
-            }
+            hipLaunchKernelGGL(k_gm_step, dim3(1), dim3(64), 0, st, gs, k, (const cd*)hdev,
+                               (const cd*)h1, (const cd*)h2, TOL_IN);
         }
-        // back substitution H y = g, then x += V y
-        std::vector<zx> y(k);
-        for (int i = k - 1; i >= 0; --i) {
-            zx t = g[i];
-            for (int j = i + 1; j < k; ++j) t -= H[i][j] * y[j];
-            y[i] = t / H[i][i];
-        }
-        std::vector<cd> yc(k);
-        for (int i = 0; i < k; ++i) yc[i] = mk(y[i].real(), y[i].imag());
-        MX_CHK(hipMemcpyAsync(hdev, yc.data(), sizeof(cd) * k, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_combine, dim3(nb256), dim3(256), 0, st, n, V, (size_t)n, k, hdev, x);
+        // y = H^-1 g, x += V y, then the next residual
+        hipLaunchKernelGGL(k_gm_solve, dim3(1), dim3(64), 0, st, gs);
+        hipLaunchKernelGGL(k_combine_gm, dim3(nb256), dim3(256), 0, st, n, (const cd*)V,
+                           (size_t)n, (const GmState*)gs, x);
+        gemv(st, n, A64, lda, x, bsave, r, part, 1);
+        dots_dev(r, 0, r, &gs->head.rn2);
+        if ((s = readback())) return s;
+        it_total += hd.kk;
+        prev = relres;
+        relres = std::sqrt(hd.rn2.x) / std::max(bnorm, 1e-300);
+        // a cycle that used all its steps short of the inner tolerance: a
+        // longer one next time
+        if (!hd.stop && hd.kk == ch) ch = std::min(2 * ch, MR);
     }
     MX_CHK(hipGetLastError());
-    int hst = 0;
-    MX_CHK(hipMemcpyAsync(&hst, F.status, sizeof(int), hipMemcpyDeviceToHost, st));
-    MX_CHK(hipStreamSynchronize(st));
-    if (hst & ST_SYNC_TIMEOUT) {
-        err = "mixed solve: a device-side wait of the triangular solves timed out";
-        return BEMX_ERR_TIMEOUT;
-    }
     *iters = it_total;
     *relres_out = relres;
     if (!converged) {
-        MX_CHK(hipStreamSynchronize(st));
         err = "mixed solve: GMRES-IR did not reach the FP64 residual level";
         return BEMX_ERR_NOCONV;
     }
     hipLaunchKernelGGL(k_copy_z, dim3(nb256), dim3(256), 0, st, x, b, n);
-    MX_CHK(hipStreamSynchronize(st));
+    MX_CHK(hipGetLastError());
     return BEMX_OK;
 }
 
