@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -96,10 +97,13 @@ struct bemx_ctx {
     const void* zh_host = nullptr;  // host array the last GHMATECD copied ZH to
     // interior
     DevBuf ipts, ipart, zdsol, zssol, ifi, idfi, ikode;
-    // frequency sweep (bemx_solve_sweep): four frequency slots (two batches of
-    // two), the assembly's stream and the batch events
-    static constexpr int NSW = 4;
-    DevBuf sw_zh[NSW], sw_zfi[NSW], sw_rhs_part[NSW], sw_rhs_diag[NSW], sw_etab[NSW];
+    // frequency sweep (bemx_solve_sweep): two halves of up to SWB frequency
+    // slots each (one batch of frequencies is assembled into one half while
+    // the other half's batch is solved); the slots' matrices and right-hand
+    // sides are contiguous (a batch's solve takes them at one stride), the
+    // assembly's stream and the halves' events
+    static constexpr int SWB = bemx::MIXED_MAX_BATCH, NSW = 2 * SWB;
+    DevBuf sw_zh, sw_zfi, sw_rhs_part[NSW], sw_rhs_diag[NSW], sw_etab[NSW];
     hipStream_t st_asm = nullptr;
     hipEvent_t sw_asm[2] = {nullptr, nullptr}, sw_done[2] = {nullptr, nullptr};
     // solver
@@ -205,9 +209,10 @@ static void ctx_buffers(bemx_ctx* c, std::vector<DevBuf*>& out) {
            &c->hdiag, &c->gdiag, &c->hpart, &c->kode, &c->zdfi, &c->colflag, &c->flist,
            &c->rhs_part, &c->rhs_diag, &c->zfi, &c->zh, &c->etab, &c->ipts, &c->ipart,
            &c->zdsol, &c->zssol, &c->ifi, &c->idfi, &c->ikode};
+    out.push_back(&c->sw_zh);
+    out.push_back(&c->sw_zfi);
     for (int i = 0; i < bemx_ctx::NSW; ++i)
-        for (DevBuf* d : {&c->sw_zh[i], &c->sw_zfi[i], &c->sw_rhs_part[i], &c->sw_rhs_diag[i],
-                          &c->sw_etab[i]})
+        for (DevBuf* d : {&c->sw_rhs_part[i], &c->sw_rhs_diag[i], &c->sw_etab[i]})
             out.push_back(d);
 }
 
@@ -807,6 +812,16 @@ int bemx_solve(bemx_ctx* c, const double* zge, const double* zcs, const double* 
     return BEMX_OK;
 }
 
+// Frequencies per batch of the sweep (BEMX_SWEEP_BATCH, 1..8; default 4).
+static int sweep_batch() {
+    static const int b = [] {
+        const char* v = getenv("BEMX_SWEEP_BATCH");
+        const int x = v ? atoi(v) : 4;
+        return std::min(bemx::MIXED_MAX_BATCH, std::max(1, x));
+    }();
+    return b;
+}
+
 int bemx_solve_sweep(bemx_ctx* c, const double* zge, const double* zcs, const double* zcp,
                      int nfreq, const double* freqs, int mode, double* zfi_out, double* stats) {
     if (!c) return BEMX_ERR_ARG;
@@ -833,11 +848,23 @@ int bemx_solve_sweep(bemx_ctx* c, const double* zge, const double* zcs, const do
         nt[f] = exp_table_entries(K[f], c->rmax, &h[f]);
         ntmax = std::max(ntmax, nt[f]);
     }
+    // frequencies per batch: at most sweep_batch(), and the two halves' matrices
+    // and mixed workspaces within half of the device memory now free
+    const size_t zh_bytes = sizeof(cd) * nn * nn, zfi_bytes = sizeof(cd) * nn * nr;
+    int nbat = std::min(sweep_batch(), nfreq);
+    {
+        size_t fr = 0, tot = 0;
+        HIPCHK(c, hipMemGetInfo(&fr, &tot));
+        fr += c->sw_zh.cap + c->lu.bytes();
+        while (nbat > 1 && 2 * nbat * (zh_bytes + zfi_bytes) +
+                                   bemx::mixed_workspace_bytes((int)nn, nbat) > fr / 2)
+            --nbat;
+    }
     HIPCHK(c, c->colflag.ensure(sizeof(int) * nbe));
     HIPCHK(c, c->flist.ensure(sizeof(int) * (nbe + 1)));
-    for (int i = 0; i < bemx_ctx::NSW; ++i) {
-        HIPCHK(c, c->sw_zh[i].ensure(sizeof(cd) * nn * nn));
-        HIPCHK(c, c->sw_zfi[i].ensure(sizeof(cd) * nn * nr));
+    HIPCHK(c, c->sw_zh.ensure(zh_bytes * 2 * nbat));
+    HIPCHK(c, c->sw_zfi.ensure(zfi_bytes * 2 * nbat));
+    for (int i = 0; i < 2 * nbat; ++i) {
         HIPCHK(c, c->sw_rhs_part[i].ensure(sizeof(cd) * nn * offdiag_tiles(nbe) * nr));
         HIPCHK(c, c->sw_rhs_diag[i].ensure(sizeof(cd) * nn * nr));
         if (ntmax) HIPCHK(c, c->sw_etab[i].ensure(sizeof(cd) * 2 * (size_t)ntmax));
@@ -851,37 +878,44 @@ int bemx_solve_sweep(bemx_ctx* c, const double* zge, const double* zcs, const do
     c->zh_valid = false;
     const ElemData* elem = c->elem.as<ElemData>();
     const GaussPt* gp = c->gp.as<GaussPt>();
-    // batch b: frequencies 2b, 2b + 1 into slots 2 (b & 1), 2 (b & 1) + 1
+    auto zh_of = [&](int sl) { return (cd*)((char*)c->sw_zh.p + zh_bytes * sl); };
+    auto zfi_of = [&](int sl) { return (cd*)((char*)c->sw_zfi.p + zfi_bytes * sl); };
+    // batch b: frequencies nbat b .. into the slots of half b & 1, two
+    // frequencies per assembly launch
     auto assemble = [&](int b) -> int {
-        const int f0 = 2 * b, nf = std::min(2, nfreq - f0), s0 = 2 * (b & 1);
-        DynFreq F[2];
-        for (int i = 0; i < nf; ++i) {
-            const int f = f0 + i, s = s0 + i;
-            F[i] = DynFreq{K[f], nt[f] ? c->sw_etab[s].as<cd>() : nullptr, nt[f], h[f],
-                           c->sw_zh[s].as<cd>(), c->sw_rhs_part[s].as<cd>(),
-                           c->sw_rhs_diag[s].as<cd>(), c->sw_zfi[s].as<cd>()};
-        }
-        if (nf == 2 && nt[f0] > 0 && nt[f0 + 1] > 0) {
-            HIPCHK(c, launch_dyn_assembly_pair(c->st_asm, nbe, npg2, elem, gp,
-                                               c->hdiag.as<double>(), c->gdiag.as<double>(),
-                                               c->kode.as<int>(), c->zdfi.as<cd>(), nr,
-                                               c->colflag.as<int>(), c->flist.as<int>(), F, nn,
-                                               nullptr, nullptr));
-        } else {
-            for (int i = 0; i < nf; ++i)
-                HIPCHK(c, launch_dyn_assembly(c->st_asm, nbe, npg2, elem, gp,
-                                              c->hdiag.as<double>(), c->gdiag.as<double>(),
-                                              c->kode.as<int>(), c->zdfi.as<cd>(), nr,
-                                              c->colflag.as<int>(), c->flist.as<int>(), F[i].K,
-                                              F[i].etab, F[i].nt, F[i].h, F[i].zh, nn,
-                                              F[i].rhs_part, F[i].rhs_diag, F[i].zfi, nullptr,
-                                              nullptr));
+        const int f0 = nbat * b, nf = std::min(nbat, nfreq - f0), s0 = nbat * (b & 1);
+        for (int i = 0; i < nf; i += 2) {
+            const int np = std::min(2, nf - i);
+            DynFreq F[2];
+            for (int q = 0; q < np; ++q) {
+                const int f = f0 + i + q, s = s0 + i + q;
+                F[q] = DynFreq{K[f], nt[f] ? c->sw_etab[s].as<cd>() : nullptr, nt[f], h[f],
+                               zh_of(s), c->sw_rhs_part[s].as<cd>(), c->sw_rhs_diag[s].as<cd>(),
+                               zfi_of(s)};
+            }
+            if (np == 2 && nt[f0 + i] > 0 && nt[f0 + i + 1] > 0) {
+                HIPCHK(c, launch_dyn_assembly_pair(c->st_asm, nbe, npg2, elem, gp,
+                                                   c->hdiag.as<double>(), c->gdiag.as<double>(),
+                                                   c->kode.as<int>(), c->zdfi.as<cd>(), nr,
+                                                   c->colflag.as<int>(), c->flist.as<int>(), F,
+                                                   nn, nullptr, nullptr));
+            } else {
+                for (int q = 0; q < np; ++q)
+                    HIPCHK(c, launch_dyn_assembly(c->st_asm, nbe, npg2, elem, gp,
+                                                  c->hdiag.as<double>(), c->gdiag.as<double>(),
+                                                  c->kode.as<int>(), c->zdfi.as<cd>(), nr,
+                                                  c->colflag.as<int>(), c->flist.as<int>(),
+                                                  F[q].K, F[q].etab, F[q].nt, F[q].h, F[q].zh,
+                                                  nn, F[q].rhs_part, F[q].rhs_diag, F[q].zfi,
+                                                  nullptr, nullptr));
+            }
         }
         HIPCHK(c, hipEventRecord(c->sw_asm[b & 1], c->st_asm));
         return BEMX_OK;
     };
-    const int nb = (nfreq + 1) / 2;
+    const int nb = (nfreq + nbat - 1) / nbat;
     double acc[NLAST] = {0};
+    std::vector<double> st4(4 * nbat);
     HIPCHK(c, hipEventRecord(c->ev0, c->st));
     HIPCHK(c, hipStreamWaitEvent(c->st_asm, c->ev0, 0));  // after the context's earlier work
     int s = assemble(0);
@@ -894,27 +928,26 @@ int bemx_solve_sweep(bemx_ctx* c, const double* zge, const double* zcs, const do
             if (s) break;
         }
         HIPCHK(c, hipStreamWaitEvent(c->st, c->sw_asm[b & 1], 0));
-        for (int i = 0; i < std::min(2, nfreq - 2 * b) && !s; ++i) {
-            const int f = 2 * b + i, sl = 2 * (b & 1) + i;
-            long long inf = 0;
-            double st4[4] = {0, 0, 0, 0};
-            s = bemx::lu_solve(c->lu, c->st, (int)nn, c->sw_zh[sl].as<cd>(), nn,
-                               c->sw_zfi[sl].as<cd>(), nr, mode, &inf, st4, c->err);
-            if (s) break;
-            if (zfi_out)
-                HIPCHK(c, hipMemcpyAsync(zfi_out + 2 * nn * nr * f, c->sw_zfi[sl].p,
-                                         sizeof(cd) * nn * nr, hipMemcpyDeviceToHost, c->st));
+        const int f0 = nbat * b, nf = std::min(nbat, nfreq - f0), s0 = nbat * (b & 1);
+        s = bemx::lu_solve_batch(c->lu, c->st, (int)nn, nf, zh_of(s0), nn, zh_bytes, zfi_of(s0),
+                                 zfi_bytes, nr, mode, st4.data(), c->err);
+        if (s) break;
+        if (zfi_out)
+            HIPCHK(c, hipMemcpyAsync(zfi_out + 2 * nn * nr * f0, zfi_of(s0), zfi_bytes * nf,
+                                     hipMemcpyDeviceToHost, c->st));
+        for (int i = 0; i < nf; ++i) {
+            const int f = f0 + i;
             if (stats) {
-                stats[4 * f] = st4[0];
-                stats[4 * f + 1] = st4[1];
-                stats[4 * f + 2] = st4[2];
+                stats[4 * f] = st4[4 * i];
+                stats[4 * f + 1] = st4[4 * i + 1];
+                stats[4 * f + 2] = st4[4 * i + 2];
                 stats[4 * f + 3] = 0.0;
             }
-            acc[0] += st4[0];
-            acc[1] = std::max(acc[1], st4[1]);
-            acc[2] = std::max(acc[2], st4[2]);
+            acc[0] += st4[4 * i];
+            acc[1] = std::max(acc[1], st4[4 * i + 1]);
+            acc[2] = std::max(acc[2], st4[4 * i + 2]);
         }
-        if (!s) HIPCHK(c, hipEventRecord(c->sw_done[b & 1], c->st));
+        HIPCHK(c, hipEventRecord(c->sw_done[b & 1], c->st));
     }
     hipStreamSynchronize(c->st_asm);
     if (s) {

@@ -18,10 +18,14 @@ struct LuWork {
     // reallocation must never be replayed after it.
     unsigned long long gen = 0;
     // Flags of the cooperative triangular solves hold the epoch of the
-    // launch that set them; epochs only grow, so the array is cleared once
-    // per workspace generation, and never inside a solve.
+    // launch that set them; epochs only grow, so they are cleared only when
+    // the mixed path's arenas change (a new allocation or another order n,
+    // which moves every buffer inside an arena), and never inside a solve:
+    // arenas 0 .. arena_clear - 1 hold cleared flags of the layout of order
+    // arena_n in workspace generation arena_gen.
     int epoch = 0;
-    unsigned long long cleared_gen = ~0ull;
+    unsigned long long arena_gen = ~0ull;
+    int arena_n = -1, arena_clear = 0;
     // second stream and events of the look-ahead factorisation (lazy)
     hipStream_t side = nullptr;
     hipEvent_t ev_panel = nullptr, ev_bulk = nullptr;
@@ -31,11 +35,12 @@ struct LuWork {
     bool factor_timed = false;
     hipError_t ensure_side();
     // Captured factorisation graphs, one per size class: keyed by the order,
-    // the matrix address, its leading dimension and the workspace generation;
-    // least recently used entries are evicted beyond GRAPHS.
-    static constexpr int GRAPHS = 8;
+    // the matrix address, its leading dimension, the workspace generation,
+    // the batch size and the batch's matrix stride; least recently used
+    // entries are evicted beyond GRAPHS.
+    static constexpr int GRAPHS = 8, KEYS = 6;
     struct GraphEntry {
-        const void* key[4] = {nullptr, nullptr, nullptr, nullptr};
+        const void* key[KEYS] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
         hipGraphExec_t exec = nullptr;
         unsigned long long used = 0;
     };
@@ -43,9 +48,9 @@ struct LuWork {
     unsigned long long tick = 0;
     long long captures = 0, replays = 0;  // graph statistics since creation
     bool use_graphs = true;
-    hipGraphExec_t find_graph(const void* const key[4]);
+    hipGraphExec_t find_graph(const void* const key[KEYS]);
     // takes ownership of exec; evicts the least recently used entry if full
-    void add_graph(const void* const key[4], hipGraphExec_t exec);
+    void add_graph(const void* const key[KEYS], hipGraphExec_t exec);
     void drop_graphs();
     LuWork();
     void* get(int slot, size_t bytes, hipError_t* e);
@@ -65,12 +70,34 @@ struct LuWork {
 int gmres_ir(LuWork& w, hipStream_t st, int n, const cd* A, size_t lda, cd* b, int nrhs,
              double stats[4], std::string& err, int* nsolved);
 
+// Most problems in one batch of the mixed path.
+constexpr int MIXED_MAX_BATCH = 8;
+
+// The mixed solve of nz (1 .. MIXED_MAX_BATCH) problems of order n in one
+// batch: problem z's matrix at A + z sa bytes (read only) and its nrhs
+// right-hand sides at b + z sb bytes (leading dimension n; overwritten with
+// the solutions that converge).  Each problem's arithmetic is that of
+// gmres_ir alone.  Per problem: code[z] (BEMX_OK, BEMX_ERR_SINGULAR for a
+// zero pivot in a diagonal block, BEMX_ERR_NOCONV when the refinement
+// stalls), nsolved[z] leading columns solved, stats[4 z ..] = [GMRES
+// iterations, worst final ||r||/||b||, 0, 0].  The return value reports a
+// failure of the whole batch (HIP error, timed-out device wait).
+int gmres_ir_batch(LuWork& w, hipStream_t st, int n, int nz, const cd* A, size_t lda, size_t sa,
+                   cd* b, size_t sb, int nrhs, int* code, int* nsolved, double* stats,
+                   std::string& err);
+
+// lu_solve for a batch: the mixed solve of every problem (gmres_ir_batch),
+// then the plain FP64 LU for each problem whose mixed solve failed
+// (stats[4 z + 2] = 1).  mode BEMX_MODE_FP64 solves the problems one by one.
+int lu_solve_batch(LuWork& w, hipStream_t st, int n, int nz, cd* A, size_t lda, size_t sa, cd* b,
+                   size_t sb, int nrhs, int mode, double* stats, std::string& err);
+
 int lu_solve(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, cd* b, int nrhs, int mode,
              long long* info, double stats[4], std::string& err);
 
 // Device bytes of every workspace buffer of the mixed path (factorisation
 // and refinement) for order n.
-size_t mixed_workspace_bytes(int n);
+size_t mixed_workspace_bytes(int n, int nz = 1);
 
 // Device bytes of the FP64 factorisation's workspace for order n (pivots,
 // panel candidates, permutations, solve vector).

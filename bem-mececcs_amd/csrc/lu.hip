@@ -61,17 +61,20 @@ size_t LuWork::bytes() const {
     return b;
 }
 
-hipGraphExec_t LuWork::find_graph(const void* const key[4]) {
-    for (GraphEntry& g : graphs)
-        if (g.exec && g.key[0] == key[0] && g.key[1] == key[1] && g.key[2] == key[2] &&
-            g.key[3] == key[3]) {
+hipGraphExec_t LuWork::find_graph(const void* const key[KEYS]) {
+    for (GraphEntry& g : graphs) {
+        if (!g.exec) continue;
+        bool same = true;
+        for (int i = 0; i < KEYS; ++i) same = same && g.key[i] == key[i];
+        if (same) {
             g.used = ++tick;
             return g.exec;
         }
+    }
     return nullptr;
 }
 
-void LuWork::add_graph(const void* const key[4], hipGraphExec_t exec) {
+void LuWork::add_graph(const void* const key[KEYS], hipGraphExec_t exec) {
     GraphEntry* slot = &graphs[0];
     for (GraphEntry& g : graphs) {
         if (!g.exec) {
@@ -81,7 +84,7 @@ void LuWork::add_graph(const void* const key[4], hipGraphExec_t exec) {
         if (g.used < slot->used) slot = &g;
     }
     if (slot->exec) (void)hipGraphExecDestroy(slot->exec);
-    for (int i = 0; i < 4; ++i) slot->key[i] = key[i];
+    for (int i = 0; i < KEYS; ++i) slot->key[i] = key[i];
     slot->exec = exec;
     slot->used = ++tick;
 }
@@ -1165,6 +1168,46 @@ int lu_solve(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, cd* b, int nrh
     if (s) return s;
     for (int j = first; j < nrhs; ++j) {
         s = solve_fp64(ws, st, n, A, lda, b + (size_t)j * n, err);
+        if (s) return s;
+    }
+    return BEMX_OK;
+}
+
+int lu_solve_batch(LuWork& w, hipStream_t st, int n, int nz, cd* A, size_t lda, size_t sa, cd* b,
+                   size_t sb, int nrhs, int mode, double* stats, std::string& err) {
+    if (nz < 1 || nz > MIXED_MAX_BATCH || nrhs < 1) {
+        err = "LU: batch of 1..8 problems with at least one right-hand side";
+        return BEMX_ERR_ARG;
+    }
+    for (int i = 0; i < 4 * nz; ++i) stats[i] = 0.0;
+    auto mat = [&](int z) { return (cd*)((char*)A + (size_t)z * sa); };
+    auto rhs = [&](int z) { return (cd*)((char*)b + (size_t)z * sb); };
+    if (mode != BEMX_MODE_MIXED) {
+        for (int z = 0; z < nz; ++z) {
+            long long info = 0;
+            const int s = lu_solve(w, st, n, mat(z), lda, rhs(z), nrhs, mode, &info, stats + 4 * z,
+                                   err);
+            if (s) return s;
+        }
+        return BEMX_OK;
+    }
+    w.factor_timed = false;
+    int code[MIXED_MAX_BATCH], nsolved[MIXED_MAX_BATCH];
+    int s = gmres_ir_batch(w, st, n, nz, A, lda, sa, b, sb, nrhs, code, nsolved, stats, err);
+    if (s) return s;
+    for (int z = 0; z < nz; ++z) {
+        if (code[z] == BEMX_OK) continue;
+        if (code[z] != BEMX_ERR_NOCONV && code[z] != BEMX_ERR_SINGULAR) return code[z];
+        // the plain FP64 path is the fallback; the matrix and the unsolved
+        // columns of this problem are still intact
+        stats[4 * z + 2] = 1.0;
+        Fp64Ws ws;
+        long long info = 0;
+        s = fp64_ws_alloc(w, n, ws, err);
+        if (!s) s = factor_fp64(ws, st, n, mat(z), lda, &w, err);
+        if (!s) s = factor_status(ws, st, &info, err);
+        for (int j = nsolved[z]; !s && j < nrhs; ++j)
+            s = solve_fp64(ws, st, n, mat(z), lda, rhs(z) + (size_t)j * n, err);
         if (s) return s;
     }
     return BEMX_OK;

@@ -52,8 +52,21 @@ constexpr int NBM = 64;
 static_assert(NBM == TB, "the mixed LU block is the triangular-kernel block");
 constexpr int FLAG_SPIN_LIMIT = 1 << 22;
 
+// Batches.  B problems of one order share every launch of the factorisation
+// and the refinement; problem z is blockIdx.z (k_pair_coupling: z / 4).  Each
+// problem's workspace is one arena, the arenas bs bytes apart, so a kernel
+// moves every workspace pointer it is given by z * bs; the caller's FP64
+// matrices and right-hand sides have strides of their own.  A problem's
+// arithmetic does not depend on B or on its place in the batch.
+template <class T>
+__device__ __forceinline__ T* zp(T* p, size_t bs, int z) {
+    return p ? (T*)((const char*)p + (size_t)z * bs) : p;
+}
+
 __global__ void k_cvt_z2c(const cd* __restrict__ a, size_t lda, cf* __restrict__ o, size_t ldo,
-                          int n) {
+                          int n, size_t sa, size_t bs) {
+    a = zp(a, sa, blockIdx.z);
+    o = zp(o, bs, blockIdx.z);
     size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     size_t tot = (size_t)n * n;
     if (t >= tot) return;
@@ -282,12 +295,14 @@ __global__ __launch_bounds__(64 * (NBM / PW)) void k_diag_gj(const cf* __restric
                                                              int nb, char* __restrict__ rec,
                                                              cf* __restrict__ Xinv,
                                                              int with_inverse,
-                                                             int* __restrict__ status) {
+                                                             int* __restrict__ status, size_t bs) {
     __shared__ cf lmul[NBM * NBM];
     __shared__ cf s_inv[NBM];
     __shared__ int s_piv[NBM];
     __shared__ int s_done;
-    gj_block<PW>(A, lda, nb, rec, Xinv, with_inverse != 0, status, lmul, s_inv, s_piv, &s_done);
+    const int z = blockIdx.z;
+    gj_block<PW>(zp(A, bs, z), lda, nb, zp(rec, bs, z), zp(Xinv, bs, z), with_inverse != 0,
+                 zp(status, bs, z), lmul, s_inv, s_piv, &s_done);
 }
 
 // Wave-uniform copy of lane src's complex double (v_readlane, no LDS).
@@ -316,8 +331,13 @@ constexpr int NB2 = 2 * NBM;
 // Grid (2, pairs): x = 0 lower, 1 upper; y = pair.
 __global__ __launch_bounds__(256) void k_pair_inverses(const cf* __restrict__ A, size_t lda, int n,
                                                        const cf* __restrict__ inv, int nblk,
-                                                       cf* __restrict__ Ml, cf* __restrict__ Mu) {
+                                                       cf* __restrict__ Ml, cf* __restrict__ Mu,
+                                                       size_t bs) {
     __shared__ cf S0[NBM * NBM], S1[NBM * NBM], T[NBM * NBM];
+    A = zp(A, bs, blockIdx.z);
+    inv = zp(inv, bs, blockIdx.z);
+    Ml = zp(Ml, bs, blockIdx.z);
+    Mu = zp(Mu, bs, blockIdx.z);
     const int tid = threadIdx.x;
     const int p = blockIdx.y;
     const bool lower = blockIdx.x == 0;
@@ -398,16 +418,24 @@ __global__ __launch_bounds__(256) void k_pair_inverses(const cf* __restrict__ A,
 // (upper) are zero).  With them a block's solution is
 //   y_p = Ml_p (b_p - sum_{k < p-1} L_{p,k} y_k) - Wl_p y_{p-1}
 // so only the last product waits for the previous block.  Grid (2, npair,
-// 4): z = 32-column quarter of W; 256 threads of 4 rows x 4 columns.
+// 4 B): z = 4 problem + 32-column quarter of W; 256 threads of 4 rows x 4
+// columns.
 __global__ __launch_bounds__(256) void k_pair_coupling(const cf* __restrict__ A, size_t lda,
                                                        int n, int npair,
                                                        const cf* __restrict__ Ml,
                                                        const cf* __restrict__ Mu,
-                                                       cf* __restrict__ Wl, cf* __restrict__ Wu) {
+                                                       cf* __restrict__ Wl, cf* __restrict__ Wu,
+                                                       size_t bs) {
     __shared__ cf Ms[16][NB2];
     __shared__ cf Bs[16][32];
+    const int z = blockIdx.z >> 2;
+    A = zp(A, bs, z);
+    Ml = zp(Ml, bs, z);
+    Mu = zp(Mu, bs, z);
+    Wl = zp(Wl, bs, z);
+    Wu = zp(Wu, bs, z);
     const bool lower = blockIdx.x == 0;
-    const int p = blockIdx.y, cq = blockIdx.z;
+    const int p = blockIdx.y, cq = blockIdx.z & 3;
     if (lower ? p == 0 : p == npair - 1) return;
     const int tid = threadIdx.x, tr = tid & 31, tc = tid >> 5;
     const cf* M = (lower ? Ml : Mu) + (size_t)p * NB2 * NB2;
@@ -746,9 +774,25 @@ __global__ __launch_bounds__(512) void k_trsv_coop(const cf* __restrict__ A, siz
                                                    const cd* __restrict__ v, cd* __restrict__ tmp,
                                                    cd* __restrict__ z, u64* __restrict__ yll,
                                                    int* __restrict__ flags, int e1, int e2,
-                                                   int* __restrict__ status) {
+                                                   int* __restrict__ status, size_t bs,
+                                                   unsigned act) {
     __shared__ cd part[8][NBM];
     __shared__ cd accs[NB2];
+    if (!((act >> blockIdx.z) & 1u)) return;  // a converged problem of the batch
+    {
+        const int q = blockIdx.z;
+        A = zp(A, bs, q);
+        Ml = zp(Ml, bs, q);
+        Wl = zp(Wl, bs, q);
+        Mu = zp(Mu, bs, q);
+        Wu = zp(Wu, bs, q);
+        v = zp(v, bs, q);
+        tmp = zp(tmp, bs, q);
+        z = zp(z, bs, q);
+        yll = zp(yll, bs, q);
+        flags = zp(flags, bs, q);
+        status = zp(status, bs, q);
+    }
     u64* acc_ll = yll + 4 * (size_t)nblk2 * NB2;  // the right-hand-side halves' hand-off
     trsv_lower_body(A, lda, n, nblk2, Ml, Wl, v, tmp, yll, acc_ll, flags, e1, status, part, accs);
     trsv_upper_body(A, lda, n, nblk2, Mu, Wu, tmp, z, yll, acc_ll, flags, e2, e1, status, part,
@@ -856,10 +900,14 @@ __global__ __launch_bounds__(256) void k_panel_inv(cf* __restrict__ A, size_t ld
                                                    int nr, int nc, const cf* __restrict__ Xinv,
                                                    __bf16* __restrict__ la,
                                                    __bf16* __restrict__ ub, size_t pstride,
-                                                   int corner) {
+                                                   int corner, size_t bs) {
     constexpr int XS = NBM + 1;
     __shared__ __attribute__((aligned(16))) __bf16 Bp[4 * NBM * PI_BK];  // [pl][col BK + k]
     __shared__ cf X[NBM * XS];                                           // X[col XS + row]
+    A = zp(A, bs, blockIdx.z);
+    Xinv = zp(Xinv, bs, blockIdx.z);
+    la = zp(la, bs, blockIdx.z);
+    ub = zp(ub, bs, blockIdx.z);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int nrb = (nr + TRR_ROWS - 1) / TRR_ROWS;
     if ((int)blockIdx.x >= nrb) {
@@ -1200,9 +1248,12 @@ template <int NBUF>
 __global__ __launch_bounds__(512) void k_cgemm_update2(UpdRegion r0, UpdRegion r1,
                                                        const __bf16* __restrict__ base,
                                                        size_t pstride, size_t sstride, int nks,
-                                                       size_t ldc, int panel) {
+                                                       size_t ldc, int panel, size_t bs) {
     __shared__ __attribute__((aligned(16))) __bf16 lds_a[NBUF * 4 * PLN];
     __shared__ __attribute__((aligned(16))) __bf16 lds_b[NBUF * 4 * PLN];
+    r0.C = zp(r0.C, bs, blockIdx.z);
+    r1.C = zp(r1.C, bs, blockIdx.z);
+    base = zp(base, bs, blockIdx.z);
     const int t0 = upd2_tiles(r0), tiles = t0 + upd2_tiles(r1);
     const int per = (tiles + NXCD - 1) / NXCD;
     for (int id = blockIdx.x; id < per * NXCD; id += gridDim.x) {
@@ -1239,11 +1290,22 @@ __global__ __launch_bounds__(512) void k_cgemm_update_inverse(UpdRegion r0, UpdR
                                                               char* __restrict__ rec1,
                                                               int* __restrict__ gpan,
                                                               cf* __restrict__ Xinv1,
-                                                              int* __restrict__ status) {
+                                                              int* __restrict__ status,
+                                                              size_t bs) {
     __shared__ __attribute__((aligned(16))) __bf16 lds_a[2 * 4 * PLN];
     __shared__ __attribute__((aligned(16))) __bf16 lds_b[2 * 4 * PLN];
     static_assert(sizeof(lds_a) >= sizeof(cf) * (NBM + 1) * NBM + sizeof(int) * (NBM + 1),
                   "elimination scratch fits the tile buffer");
+    {
+        const int z = blockIdx.z;
+        r0.C = zp(r0.C, bs, z);
+        r1.C = zp(r1.C, bs, z);
+        base = zp(base, bs, z);
+        rec1 = zp(rec1, bs, z);
+        gpan = zp(gpan, bs, z);
+        Xinv1 = zp(Xinv1, bs, z);
+        status = zp(status, bs, z);
+    }
     const int bid = blockIdx.x;
     if (bid == 0) {
         if (do_corner) {
@@ -1304,35 +1366,37 @@ static int upd2_tiles_host(const UpdRegion& R) {
 }
 
 
-// Launch of k_cgemm_update2 for regions a and b (b may be empty); nothing
-// when both are empty.
+// Launch of k_cgemm_update2 for regions a and b (b may be empty) of each of
+// the nz problems of a batch; nothing when both are empty.  cap: workgroups
+// over the whole batch (0: one per tile).
 static void launch_update(hipStream_t s, const UpdRegion& a, const UpdRegion& b,
                           const __bf16* base, size_t pstride, size_t sstride, int nks,
-                          size_t ldc, int cap = 0) {
+                          size_t ldc, int nz, size_t bs, int cap = 0) {
     const int tiles = upd2_tiles_host(a) + upd2_tiles_host(b);
     if (tiles == 0) return;
     int g = ((tiles + NXCD - 1) / NXCD) * NXCD;
     // a cap only for updates of up to 16 tiles per capped workgroup: a
     // larger one is the bottleneck itself and gets the whole device
-    if (cap > 0 && tiles <= 16 * cap) g = std::min(g, cap);
+    const int capz = cap > 0 ? std::max(NXCD, ((cap / nz + NXCD - 1) / NXCD) * NXCD) : 0;
+    if (capz > 0 && tiles <= 16 * capz) g = std::min(g, capz);
     // 4 column tiles per panel of the tile order (2, 8 and 16 measured the
     // same on the 25920-DOF solve); two LDS stage buffers (one buffer with
     // two barriers per stage measured the same or 1% slower)
-    hipLaunchKernelGGL(k_cgemm_update2<2>, dim3(g), dim3(512), 0, s, a, b, base, pstride, sstride,
-                       nks, ldc, 4);
+    hipLaunchKernelGGL(k_cgemm_update2<2>, dim3(g, 1, nz), dim3(512), 0, s, a, b, base, pstride,
+                       sstride, nks, ldc, 4, bs);
 }
 
 // k_cgemm_update_inverse for regions a (non-empty; block k+1 of nb1 columns
-// at its top-left corner) and b.
+// at its top-left corner) and b, for each of the nz problems of a batch.
 static void launch_update_inverse(hipStream_t s, const UpdRegion& a, const UpdRegion& b,
                                   const __bf16* base, size_t pstride, size_t sstride, int nks,
                                   size_t ldc, int nb1, bool do_corner, char* rec1, int* gpan,
-                                  cf* Xinv1, int* status) {
+                                  cf* Xinv1, int* status, int nz, size_t bs) {
     const int tiles = upd2_tiles_host(a) + upd2_tiles_host(b);
     const int g = ((tiles + NXCD - 1) / NXCD) * NXCD;
-    hipLaunchKernelGGL(k_cgemm_update_inverse, dim3(1 + g + XINV_WGS), dim3(512), 0, s, a, b,
-                       base, pstride, sstride, nks, ldc, 4, g, nb1, (int)do_corner, rec1, gpan,
-                       Xinv1, status);
+    hipLaunchKernelGGL(k_cgemm_update_inverse, dim3(1 + g + XINV_WGS, 1, nz), dim3(512), 0, s, a,
+                       b, base, pstride, sstride, nks, ldc, 4, g, nb1, (int)do_corner, rec1, gpan,
+                       Xinv1, status, bs);
 }
 
 // Workgroups of the bulk update on the side stream (BEMX_BULK_WGS; default
@@ -1358,10 +1422,19 @@ static int bulk_wg_cap() {
 // ---------------------------------------------------------------------------
 constexpr int MV_COLS = 128;  // columns per GEMV chunk
 
+// Whether problem blockIdx.z takes part in a launch of the refinement (bit
+// z of act; converged problems drop out and keep their solution).
+__device__ __forceinline__ bool z_active(unsigned act) { return (act >> blockIdx.z) & 1u; }
+
 // part[chunk][row] = sum over the chunk's columns of A[row, col] x[col]
 __global__ __launch_bounds__(256) void k_zgemv_part(const cd* __restrict__ A, size_t lda, int n,
                                                     const cd* __restrict__ x,
-                                                    cd* __restrict__ part) {
+                                                    cd* __restrict__ part, size_t sa, size_t bs,
+                                                    unsigned act) {
+    if (!z_active(act)) return;
+    A = zp(A, sa, blockIdx.z);
+    x = zp(x, bs, blockIdx.z);
+    part = zp(part, bs, blockIdx.z);
     const int row = blockIdx.x * 256 + threadIdx.x;
     const int c0 = blockIdx.y * MV_COLS;
     const int c1 = min(n, c0 + MV_COLS);
@@ -1377,7 +1450,12 @@ __global__ __launch_bounds__(256) void k_zgemv_part(const cd* __restrict__ A, si
 
 // r = b - sum(part)   (sub = 1)   or   r = sum(part)   (sub = 0)
 __global__ void k_gemv_reduce(int n, int nchunk, const cd* __restrict__ part,
-                              const cd* __restrict__ b, cd* __restrict__ r, int sub) {
+                              const cd* __restrict__ b, cd* __restrict__ r, int sub, size_t bs,
+                              unsigned act) {
+    if (!z_active(act)) return;
+    part = zp(part, bs, blockIdx.z);
+    b = zp(b, bs, blockIdx.z);
+    r = zp(r, bs, blockIdx.z);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     cd s = mk(0, 0);
@@ -1388,8 +1466,13 @@ __global__ void k_gemv_reduce(int n, int nchunk, const cd* __restrict__ part,
 // partial[blk][i] = sum over the block's rows of conj(V_i) w   (i < nv), and
 // partial[blk][nv] = sum |w|^2 (real part).
 __global__ __launch_bounds__(256) void k_dots(int n, const cd* __restrict__ V, size_t ldv, int nv,
-                                              const cd* __restrict__ w, cd* __restrict__ partial) {
+                                              const cd* __restrict__ w, cd* __restrict__ partial,
+                                              size_t bs, unsigned act) {
     __shared__ double red[2][4];
+    if (!z_active(act)) return;
+    V = zp(V, bs, blockIdx.z);
+    w = zp(w, bs, blockIdx.z);
+    partial = zp(partial, bs, blockIdx.z);
     const int row = blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     cd wv = row < n ? w[row] : mk(0, 0);
@@ -1418,7 +1501,12 @@ __global__ __launch_bounds__(256) void k_dots(int n, const cd* __restrict__ V, s
 // w = (w - sum_i h_i V_i) * s
 __global__ void k_orth_update(int n, const cd* __restrict__ V, size_t ldv, int nv,
                               const cd* __restrict__ h, cd* __restrict__ w, cd* __restrict__ out,
-                              double s) {
+                              double s, size_t bs, unsigned act) {
+    if (!z_active(act)) return;
+    V = zp(V, bs, blockIdx.z);
+    h = zp(h, bs, blockIdx.z);
+    w = zp(w, bs, blockIdx.z);
+    out = zp(out, bs, blockIdx.z);
     int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= n) return;
     cd v = w[row];
@@ -1429,7 +1517,10 @@ __global__ void k_orth_update(int n, const cd* __restrict__ V, size_t ldv, int n
 // out[i] = sum over the nparts partial rows of entry i (i < nv1), in a fixed
 // order: the reduction of k_dots on the device, same result on every run.
 __global__ void k_dots_reduce(int nparts, int nv1, const cd* __restrict__ part,
-                              cd* __restrict__ out) {
+                              cd* __restrict__ out, size_t bs, unsigned act) {
+    if (!z_active(act)) return;
+    part = zp(part, bs, blockIdx.z);
+    out = zp(out, bs, blockIdx.z);
     const int i = threadIdx.x;
     if (i >= nv1) return;
     cd s = mk(0, 0);
@@ -1439,7 +1530,11 @@ __global__ void k_dots_reduce(int nparts, int nv1, const cd* __restrict__ part,
 
 // out = w / sqrt(nrm2->x) (zero when the norm is zero)
 __global__ void k_scale_by_norm(int n, const cd* __restrict__ w, cd* __restrict__ out,
-                                const cd* __restrict__ nrm2) {
+                                const cd* __restrict__ nrm2, size_t bs, unsigned act) {
+    if (!z_active(act)) return;
+    w = zp(w, bs, blockIdx.z);
+    out = zp(out, bs, blockIdx.z);
+    nrm2 = zp(nrm2, bs, blockIdx.z);
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= n) return;
     const double hn = sqrt(nrm2->x);
@@ -1476,8 +1571,9 @@ __device__ inline cd divz(cd a, cd b) {
 }
 
 // Start of a cycle: beta = ||M^-1 r||, g = beta e1.
-__global__ void k_gm_start(GmState* gs) {
-    if (threadIdx.x != 0) return;
+__global__ void k_gm_start(GmState* gs, size_t bs, unsigned act) {
+    if (threadIdx.x != 0 || !z_active(act)) return;
+    gs = zp(gs, bs, blockIdx.z);
     GmHead& h = gs->head;
     h.beta = sqrt(h.beta2.x);
     for (int i = 0; i <= GM_RESTART; ++i) gs->g[i] = mk(0.0, 0.0);
@@ -1493,8 +1589,13 @@ __global__ void k_gm_start(GmState* gs) {
 // rotated with it.  After convergence (|g[k+1]| <= tol beta) or a breakdown
 // (zero norm) later steps leave the state alone.
 __global__ void k_gm_step(GmState* gs, int k, const cd* __restrict__ hp,
-                          const cd* __restrict__ hq, const cd* __restrict__ hn2, double tol) {
-    if (threadIdx.x != 0) return;
+                          const cd* __restrict__ hq, const cd* __restrict__ hn2, double tol,
+                          size_t bs, unsigned act) {
+    if (threadIdx.x != 0 || !z_active(act)) return;
+    gs = zp(gs, bs, blockIdx.z);
+    hp = zp(hp, bs, blockIdx.z);
+    hq = zp(hq, bs, blockIdx.z);
+    hn2 = zp(hn2, bs, blockIdx.z);
     GmHead& h = gs->head;
     if (h.stop) return;
     constexpr int LD = GM_RESTART + 1;
@@ -1527,8 +1628,9 @@ __global__ void k_gm_step(GmState* gs, int k, const cd* __restrict__ hp,
 }
 
 // y = H^-1 g over the cycle's kk steps (back substitution).
-__global__ void k_gm_solve(GmState* gs) {
-    if (threadIdx.x != 0) return;
+__global__ void k_gm_solve(GmState* gs, size_t bs, unsigned act) {
+    if (threadIdx.x != 0 || !z_active(act)) return;
+    gs = zp(gs, bs, blockIdx.z);
     constexpr int LD = GM_RESTART + 1;
     const int kk = gs->head.kk;
     for (int i = kk - 1; i >= 0; --i) {
@@ -1540,7 +1642,12 @@ __global__ void k_gm_solve(GmState* gs) {
 
 // x += sum_{i < kk} y_i V_i
 __global__ void k_combine_gm(int n, const cd* __restrict__ V, size_t ldv,
-                             const GmState* __restrict__ gs, cd* __restrict__ x) {
+                             const GmState* __restrict__ gs, cd* __restrict__ x, size_t bs,
+                             unsigned act) {
+    if (!z_active(act)) return;
+    V = zp(V, bs, blockIdx.z);
+    gs = zp(gs, bs, blockIdx.z);
+    x = zp(x, bs, blockIdx.z);
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= n) return;
     const int kk = gs->head.kk;
@@ -1549,14 +1656,36 @@ __global__ void k_combine_gm(int n, const cd* __restrict__ V, size_t ldv,
     x[row] = v;
 }
 
-__global__ void k_copy_z(const cd* __restrict__ s, cd* __restrict__ d, int n) {
+// d = s for each active problem (s and d with strides of their own)
+__global__ void k_copy_z(const cd* __restrict__ s, cd* __restrict__ d, int n, size_t ss,
+                         size_t ds, unsigned act) {
+    if (!z_active(act)) return;
+    s = zp(s, ss, blockIdx.z);
+    d = zp(d, ds, blockIdx.z);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) d[i] = s[i];
 }
 
-__global__ void k_fill_i(int* p, int v, int cnt) {
+__global__ void k_fill_i(int* p, int v, int cnt, size_t bs) {
+    p = zp(p, bs, blockIdx.z);
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < cnt) p[i] = v;
+}
+
+// What the host reads back per problem after a factorisation or a GMRES
+// cycle: the cycle's head and the status word, gathered from the arenas into
+// one contiguous array so that one copy serves the whole batch.
+struct ReadRec {
+    GmHead head;
+    int status;
+};
+
+__global__ void k_gather_heads(const GmState* gs, const int* status, size_t bs,
+                               ReadRec* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    const int z = blockIdx.z;
+    if (gs) out[z].head = zp(gs, bs, z)->head;
+    out[z].status = *zp(status, bs, z);
 }
 
 // ---------------------------------------------------------------------------
@@ -1571,9 +1700,13 @@ __global__ void k_fill_i(int* p, int v, int cnt) {
         }                                                                     \
     } while (0)
 
-// Workspace slots 9.. of LuWork (0..8 belong to the FP64 path).
-enum { M_A32 = 9, M_REC, M_FLAGS, M_STATUS, M_VEC, M_V, M_PART, M_HOST, M_PTMP, M_PLANES, M_INV,
-       M_MINV, M_WC, M_GJF, M_LL };
+// Workspace of the mixed path: LuWork slot M_ARENA holds one arena per
+// problem of a batch (every buffer of the factorisation and the refinement,
+// at the offsets of arena_layout), slot M_RREC the host-read records.
+// Slots 0..8 belong to the FP64 path.
+enum { M_ARENA = 9, M_RREC = 10 };
+enum { A_A32, A_REC, A_FLAGS, A_STATUS, A_VEC, A_V, A_PART, A_HOST, A_PLANES, A_INV, A_MINV,
+       A_WC, A_GJF, A_LL, A_PARTS };
 
 struct MixedFactor {
     cf* A = nullptr;
@@ -1588,6 +1721,12 @@ struct MixedFactor {
     cf* wc = nullptr;       // per pair: lower coupling Wl, then (after npair) upper Wu
     int n = 0, nblk = 0, npair = 0, coop = 1;
     size_t ld = 0;  // leading dimension of A
+    // the batch: nz problems, problem z's arena at arena + z bs (the
+    // pointers above are problem 0's)
+    int nz = 1;
+    size_t bs = 0;
+    char* arena = nullptr;
+    ReadRec* rrec = nullptr;  // nz host-read records
 };
 
 // Blocks per group of the factorisation (BEMX_GROUP, 1..8; default 8): the
@@ -1603,36 +1742,46 @@ static int group_blocks() {
     return g;
 }
 
+// Offsets of the buffers in one problem's arena for order n (each 256-byte
+// aligned) and the arena's size (a multiple of 64 KB).
+struct ArenaLayout {
+    size_t off[A_PARTS];
+    size_t bytes;
+};
 
-// Bytes of each workspace buffer of the mixed path (factorisation and
-// refinement) for order n, by LuWork slot (0 for slots it does not use).
-static void mixed_sizes(int n, size_t* sz) {
+static ArenaLayout arena_layout(int n) {
     const size_t ld = ((size_t)n + 31) & ~(size_t)31, nblk = (size_t)(n + NBM - 1) / NBM;
     const size_t npair = (nblk + 1) / 2, nchunk = (size_t)(n + MV_COLS - 1) / MV_COLS;
     const size_t nb256 = (size_t)(n + 255) / 256, MR = GM_RESTART;
-    for (int i = 0; i < LuWork::SLOTS; ++i) sz[i] = 0;
-    sz[M_A32] = sizeof(cf) * ld * n;                    // FP32 copy of the matrix
-    sz[M_INV] = sizeof(cf) * nblk * NBM * NBM;          // diagonal-block inverses
-    sz[M_REC] = 2 * REC_BYTES;                          // two elimination records
-    sz[M_GJF] = sizeof(int) * nblk * GJ_PANELS;         // record panel flags
-    sz[M_MINV] = sizeof(cf) * npair * NB2 * NB2 * 2;    // 128-row inverses of L and U
-    sz[M_WC] = sizeof(cf) * npair * NB2 * NB2 * 2;      // coupling blocks
-    sz[M_FLAGS] = sizeof(int) * 2 * npair;              // block-solve flags
-    sz[M_LL] = sizeof(u64) * 4 * 2 * npair * NB2;       // tagged hand-off words
-    sz[M_STATUS] = sizeof(int);
-    sz[M_PLANES] = sizeof(__bf16) * (size_t)n * NBM * 8 * 2 * group_blocks();
-    sz[M_VEC] = sizeof(cd) * n * 5;                     // x, r, w, tmp, b
-    sz[M_V] = sizeof(cd) * n * (MR + 1);                // GMRES basis
-    sz[M_PART] = sizeof(cd) * std::max(nchunk * n, nb256 * (MR + 2));
-    sz[M_HOST] = sizeof(cd) * (MR + 2) * 3 + sizeof(GmState);  // Arnoldi scratch, cycle state
+    size_t sz[A_PARTS];
+    sz[A_A32] = sizeof(cf) * ld * n;                    // FP32 copy of the matrix
+    sz[A_INV] = sizeof(cf) * nblk * NBM * NBM;          // diagonal-block inverses
+    sz[A_REC] = 2 * REC_BYTES;                          // two elimination records
+    sz[A_GJF] = sizeof(int) * nblk * GJ_PANELS;         // record panel flags
+    sz[A_MINV] = sizeof(cf) * npair * NB2 * NB2 * 2;    // 128-row inverses of L and U
+    sz[A_WC] = sizeof(cf) * npair * NB2 * NB2 * 2;      // coupling blocks
+    sz[A_FLAGS] = sizeof(int) * 2 * npair;              // block-solve flags
+    sz[A_LL] = sizeof(u64) * 4 * 2 * npair * NB2;       // tagged hand-off words
+    sz[A_STATUS] = sizeof(int);
+    sz[A_PLANES] = sizeof(__bf16) * (size_t)n * NBM * 8 * 2 * group_blocks();
+    sz[A_VEC] = sizeof(cd) * n * 5;                     // x, r, w, tmp, b
+    sz[A_V] = sizeof(cd) * n * (MR + 1);                // GMRES basis
+    sz[A_PART] = sizeof(cd) * std::max(nchunk * n, nb256 * (MR + 2));
+    sz[A_HOST] = sizeof(cd) * (MR + 2) * 3 + sizeof(GmState);  // Arnoldi scratch, cycle state
+    ArenaLayout L;
+    size_t o = 0;
+    for (int i = 0; i < A_PARTS; ++i) {
+        L.off[i] = o;
+        o += (sz[i] + 255) & ~(size_t)255;
+    }
+    L.bytes = (o + 65535) & ~(size_t)65535;
+    return L;
 }
 
-size_t mixed_workspace_bytes(int n) {
-    size_t sz[LuWork::SLOTS], b = 0;
-    mixed_sizes(n, sz);
-    for (int i = 0; i < LuWork::SLOTS; ++i)
-        if (sz[i]) b += std::max<size_t>(sz[i], 256);
-    return b;
+static size_t rrec_bytes(int nz) { return std::max<size_t>(sizeof(ReadRec) * nz, 256); }
+
+size_t mixed_workspace_bytes(int n, int nz) {
+    return (size_t)nz * arena_layout(n).bytes + rrec_bytes(nz);
 }
 
 hipError_t preload_mixed_code() {
@@ -1641,7 +1790,8 @@ hipError_t preload_mixed_code() {
 }
 
 // The launch sequence of the factorisation (no host synchronisation, no
-// allocation), so it can be captured into a graph.
+// allocation), so it can be captured into a graph; every launch covers the
+// F.nz problems of the batch (matrix z at A64 + z sa bytes).
 //
 // Block LU in the form A = L U with unit block-lower L (L21 = A21 A11^-1 per
 // step) and block-upper U whose diagonal blocks are the Schur-complement
@@ -1669,23 +1819,25 @@ hipError_t preload_mixed_code() {
 // are double-buffered by group parity: group g+1 writes the slots Z(g-1) read,
 // after the wait for Z(g-1).
 static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda,
-                                 const MixedFactor& F, __bf16* planes, size_t pstride) {
+                                 size_t sa, const MixedFactor& F, __bf16* planes,
+                                 size_t pstride) {
     hipError_t first = hipSuccess;
     auto chk = [&](hipError_t e) {
         if (first == hipSuccess && e != hipSuccess) first = e;
     };
-    const int nblk = F.nblk;
+    const int nblk = F.nblk, nz = F.nz;
+    const size_t bs = F.bs;
     const int sg = group_blocks();
     const size_t sstride = 8 * pstride;  // one slot: 4 L21 planes, 4 U12 planes
-    hipLaunchKernelGGL(k_fill_i, dim3(1), dim3(64), 0, st, F.status, 0, 1);
+    hipLaunchKernelGGL(k_fill_i, dim3(1, 1, nz), dim3(64), 0, st, F.status, 0, 1, bs);
     // elimination-record flags (per block and panel), raised by
     // k_cgemm_update_inverse
-    hipLaunchKernelGGL(k_fill_i, dim3((nblk * GJ_PANELS + 255) / 256), dim3(256), 0, st, F.gjflag,
-                       0, nblk * GJ_PANELS);
+    hipLaunchKernelGGL(k_fill_i, dim3((nblk * GJ_PANELS + 255) / 256, 1, nz), dim3(256), 0, st,
+                       F.gjflag, 0, nblk * GJ_PANELS, bs);
     size_t tot = (size_t)n * n;
     const size_t ld = F.ld;
-    hipLaunchKernelGGL(k_cvt_z2c, dim3((tot + 255) / 256), dim3(256), 0, st, A64, lda, F.A,
-                       ld, n);
+    hipLaunchKernelGGL(k_cvt_z2c, dim3((tot + 255) / 256, 1, nz), dim3(256), 0, st, A64, lda, F.A,
+                       ld, n, sa, bs);
     auto at = [&](int r, int c) { return F.A + (size_t)c * ld + r; };
     const UpdRegion none{0, 0, 0, 0, nullptr};
     const hipStream_t sd = w.side;
@@ -1703,16 +1855,16 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
         // block 0 alone; block k+1 is eliminated and inverted inside step
         // k's update
         if (k == 0)
-            hipLaunchKernelGGL(k_diag_gj<4>, dim3(1), dim3(1024), 0, st, (const cf*)F.A, ld, nb,
-                               F.rec, xinv, 1, F.status);
+            hipLaunchKernelGGL(k_diag_gj<4>, dim3(1, 1, nz), dim3(1024), 0, st, (const cf*)F.A, ld,
+                               nb, F.rec, xinv, 1, F.status, bs);
         if (rest <= 0) break;
         const int nrb = (rest + TRR_ROWS - 1) / TRR_ROWS;
         const int ncb = (rest + NBM - 1) / NBM;
         const int nb1 = std::min(NBM, rest);
         // inside a group the next diagonal block is updated by k_panel_inv
         const bool in_group = c0 < gend;
-        hipLaunchKernelGGL(k_panel_inv, dim3(nrb + ncb), dim3(256), 0, st, F.A, ld, k0, c0, rest,
-                           rest, (const cf*)xinv, la, ub, pstride, in_group ? nb1 : 0);
+        hipLaunchKernelGGL(k_panel_inv, dim3(nrb + ncb, 1, nz), dim3(256), 0, st, F.A, ld, k0, c0,
+                           rest, rest, (const cf*)xinv, la, ub, pstride, in_group ? nb1 : 0, bs);
         cf* xinv1 = xinv + NBM * NBM;
         int* gjf1 = F.gjflag + (size_t)(k + 1) * GJ_PANELS;
         if (in_group) {
@@ -1720,7 +1872,7 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
             const UpdRegion cs{rest, gend - c0, c0, c0, at(c0, c0)};
             const UpdRegion rs{gend - c0, n - gend, c0, gend, at(c0, gend)};
             launch_update_inverse(st, cs, n > gend ? rs : none, la, pstride, sstride, 1, ld, nb1,
-                                  false, rec1, gjf1, xinv1, F.status);
+                                  false, rec1, gjf1, xinv1, F.status, nz, bs);
             continue;
         }
         // last block of group g: apply the group to the rest, K = 64 * sg
@@ -1730,96 +1882,106 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
         const UpdRegion X{n - gend, nend - gend, gend, gend, at(gend, gend)};
         const UpdRegion Y{nend - gend, n - nend, gend, nend, at(gend, nend)};
         launch_update_inverse(st, X, n > nend ? Y : none, gbase, pstride, sstride, sg, ld, nb1,
-                              true, rec1, gjf1, xinv1, F.status);
+                              true, rec1, gjf1, xinv1, F.status, nz, bs);
         if (n > nend) {
             const UpdRegion Z{n - nend, n - nend, nend, nend, at(nend, nend)};
             chk(hipEventRecord(w.ev_panel, st));
             chk(hipStreamWaitEvent(sd, w.ev_panel, 0));
-            launch_update(sd, Z, none, gbase, pstride, sstride, sg, ld, bulk_wg_cap());
+            launch_update(sd, Z, none, gbase, pstride, sstride, sg, ld, nz, bs, bulk_wg_cap());
             chk(hipEventRecord(w.ev_bulk, sd));
             bulk_pending = true;
         }
     }
     if (bulk_pending) chk(hipStreamWaitEvent(st, w.ev_bulk, 0));
     // 128-row diagonal-block inverses of both factors for the block solves
-    hipLaunchKernelGGL(k_pair_inverses, dim3(2, F.npair), dim3(256), 0, st, (const cf*)F.A, ld,
-                       n, (const cf*)F.inv, nblk, F.minv, F.minv + (size_t)F.npair * NB2 * NB2);
+    hipLaunchKernelGGL(k_pair_inverses, dim3(2, F.npair, nz), dim3(256), 0, st, (const cf*)F.A,
+                       ld, n, (const cf*)F.inv, nblk, F.minv,
+                       F.minv + (size_t)F.npair * NB2 * NB2, bs);
     // coupling blocks of consecutive pairs for the block solves
-    hipLaunchKernelGGL(k_pair_coupling, dim3(2, F.npair, 4), dim3(256), 0, st, (const cf*)F.A, ld,
-                       n, F.npair, (const cf*)F.minv,
+    hipLaunchKernelGGL(k_pair_coupling, dim3(2, F.npair, 4 * nz), dim3(256), 0, st,
+                       (const cf*)F.A, ld, n, F.npair, (const cf*)F.minv,
                        (const cf*)(F.minv + (size_t)F.npair * NB2 * NB2), F.wc,
-                       F.wc + (size_t)F.npair * NB2 * NB2);
+                       F.wc + (size_t)F.npair * NB2 * NB2, bs);
     chk(hipGetLastError());
     return first;
 }
 
-static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda,
-                        MixedFactor& F, std::string& err) {
+// The factorisation of the nz problems of a batch.  fcode[z]: BEMX_OK or
+// BEMX_ERR_SINGULAR (a zero pivot in one of problem z's diagonal blocks).
+static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda, size_t sa,
+                        int nz, MixedFactor& F, int* fcode, std::string& err) {
     hipError_t e;
     const int nblk = (n + NBM - 1) / NBM;
     F.n = n;
     F.nblk = nblk;
     F.ld = ((size_t)n + 31) & ~(size_t)31;  // 16-byte aligned 4-row groups
+    F.nz = nz;
     // Every buffer of the mixed path, the refinement's too, before anything
     // is captured: a captured graph holds the addresses of the buffers it
     // touches, and any later reallocation would drop it.
-    size_t sz[LuWork::SLOTS];
-    mixed_sizes(n, sz);
-    void* buf[LuWork::SLOTS] = {nullptr};
-    for (int i = 0; i < LuWork::SLOTS; ++i)
-        if (sz[i]) {
-            buf[i] = w.get(i, sz[i], &e);
-            MX_CHK(e);
-        }
-    F.A = (cf*)buf[M_A32];
-    F.inv = (cf*)buf[M_INV];
-    F.rec = (char*)buf[M_REC];
-    F.gjflag = (int*)buf[M_GJF];
+    const ArenaLayout L = arena_layout(n);
+    F.bs = L.bytes;
+    char* ar = (char*)w.get(M_ARENA, (size_t)nz * L.bytes, &e);
+    MX_CHK(e);
+    F.rrec = (ReadRec*)w.get(M_RREC, rrec_bytes(nz), &e);
+    MX_CHK(e);
+    F.arena = ar;
+    F.A = (cf*)(ar + L.off[A_A32]);
+    F.inv = (cf*)(ar + L.off[A_INV]);
+    F.rec = ar + L.off[A_REC];
+    F.gjflag = (int*)(ar + L.off[A_GJF]);
     F.npair = (nblk + 1) / 2;
-    F.minv = (cf*)buf[M_MINV];
-    F.wc = (cf*)buf[M_WC];
-    F.flags = (int*)buf[M_FLAGS];  // block-solve flags: one per 64-row half of a 128-row block
-    F.yll = (u64*)buf[M_LL];
+    F.minv = (cf*)(ar + L.off[A_MINV]);
+    F.wc = (cf*)(ar + L.off[A_WC]);
+    F.flags = (int*)(ar + L.off[A_FLAGS]);  // block-solve flags: one per 64-row half of a 128-row block
+    F.yll = (u64*)(ar + L.off[A_LL]);
+    F.status = (int*)(ar + L.off[A_STATUS]);
     F.epoch = &w.epoch;
-    if (w.cleared_gen != w.gen) {
-        // Flags and hand-off tags hold launch epochs, which only grow, so
-        // they are cleared once, when the workspace is (re)allocated, and
-        // never inside a solve.
-        const int nflags = 2 * F.npair, nll = F.npair * NB2;
-        hipLaunchKernelGGL(k_fill_i, dim3((nflags + 255) / 256), dim3(256), 0, st, F.flags, 0,
-                           nflags);
-        hipLaunchKernelGGL(k_fill_i, dim3((16 * nll + 255) / 256), dim3(256), 0, st, (int*)F.yll,
-                           0, 16 * nll);
-        w.cleared_gen = w.gen;
+    if (w.arena_gen != w.gen || w.arena_n != n) {
+        w.arena_gen = w.gen;
+        w.arena_n = n;
+        w.arena_clear = 0;
     }
-    F.status = (int*)buf[M_STATUS];
+    if (w.arena_clear < nz) {
+        // Flags and hand-off tags hold launch epochs, which only grow, so
+        // an arena's are cleared once for its layout, before its first use,
+        // and never inside a solve.
+        const int z0 = w.arena_clear, cnt = nz - z0;
+        const int nflags = 2 * F.npair, nll = F.npair * NB2;
+        hipLaunchKernelGGL(k_fill_i, dim3((nflags + 255) / 256, 1, cnt), dim3(256), 0, st,
+                           (int*)((char*)F.flags + (size_t)z0 * F.bs), 0, nflags, F.bs);
+        hipLaunchKernelGGL(k_fill_i, dim3((16 * nll + 255) / 256, 1, cnt), dim3(256), 0, st,
+                           (int*)((char*)F.yll + (size_t)z0 * F.bs), 0, 16 * nll, F.bs);
+        w.arena_clear = nz;
+    }
     int dev = 0, ncu = 0, per = 0;
     MX_CHK(hipGetDevice(&dev));
     MX_CHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     MX_CHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trsv_coop, 512, 0));
-    // two workgroups per 128-row block, at most one workgroup per CU (two
-    // sharing a CU slow each other's products down more than looping over
-    // blocks costs)
-    F.coop = 2 * std::max(1, std::min(F.npair, std::min(ncu, ncu * per) / 2));
+    // two workgroups per 128-row block, at most one workgroup per CU over
+    // the batch (two sharing a CU slow each other's products down more than
+    // looping over blocks costs); every workgroup of the batch resident
+    F.coop = 2 * std::max(1, std::min(F.npair, std::min(ncu, ncu * per) / (2 * nz)));
     MX_CHK(w.ensure_side());
     // bf16 planes of L21 and U12: one slot (8 planes) per block of a group,
     // double-buffered by group parity (group g+1 writes one set while the bulk
     // update of group g still reads the other)
     const size_t pstride = (size_t)n * NBM;
-    __bf16* planes = (__bf16*)buf[M_PLANES];
-    // The launch sequence depends only on n, the matrix and the workspace
-    // buffers: it is captured once per size class into a graph and replayed
-    // while the workspace keeps its buffers (every reallocation bumps w.gen
-    // and drops the cached graphs).
+    __bf16* planes = (__bf16*)(ar + L.off[A_PLANES]);
+    // The launch sequence depends only on n, the matrices, the batch and the
+    // workspace buffers: it is captured once per size class into a graph and
+    // replayed while the workspace keeps its buffers (every reallocation
+    // bumps w.gen and drops the cached graphs).
     MX_CHK(hipEventRecord(w.ev_f0, st));
     if (w.use_graphs) {
-        const void* key[4] = {(const void*)(size_t)n, A64, (const void*)lda,
-                              (const void*)(size_t)w.gen};
+        const void* key[LuWork::KEYS] = {(const void*)(size_t)n, A64, (const void*)lda,
+                                         (const void*)(size_t)w.gen, (const void*)(size_t)nz,
+                                         (const void*)sa};
         hipGraphExec_t ex = w.find_graph(key);
         if (!ex) {
             hipGraph_t g = nullptr;
             MX_CHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            const hipError_t e1 = enqueue_factor(w, st, n, A64, lda, F, planes, pstride);
+            const hipError_t e1 = enqueue_factor(w, st, n, A64, lda, sa, F, planes, pstride);
             const hipError_t e2 = hipStreamEndCapture(st, &g);
             if (e1 != hipSuccess || e2 != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
@@ -1835,29 +1997,32 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
         }
         MX_CHK(hipGraphLaunch(ex, st));
     } else {
-        MX_CHK(enqueue_factor(w, st, n, A64, lda, F, planes, pstride));
+        MX_CHK(enqueue_factor(w, st, n, A64, lda, sa, F, planes, pstride));
     }
     MX_CHK(hipEventRecord(w.ev_f1, st));
     w.factor_timed = true;
+    hipLaunchKernelGGL(k_gather_heads, dim3(1, 1, nz), dim3(64), 0, st, (const GmState*)nullptr,
+                       (const int*)F.status, F.bs, F.rrec);
     MX_CHK(hipGetLastError());
-    int hs = 0;
-    MX_CHK(hipMemcpyAsync(&hs, F.status, sizeof(int), hipMemcpyDeviceToHost, st));
+    ReadRec hr[MIXED_MAX_BATCH];
+    MX_CHK(hipMemcpyAsync(hr, F.rrec, sizeof(ReadRec) * nz, hipMemcpyDeviceToHost, st));
     MX_CHK(hipStreamSynchronize(st));
-    if (hs & ST_SYNC_TIMEOUT) {
-        err = "mixed solve: a device-side wait of the factorisation timed out";
-        return BEMX_ERR_TIMEOUT;
-    }
-    if (hs & ST_PIVOT_ZERO) {
-        err = "mixed solve: zero pivot in a diagonal block";
-        return BEMX_ERR_SINGULAR;
+    for (int z = 0; z < nz; ++z) {
+        if (hr[z].status & ST_SYNC_TIMEOUT) {
+            err = "mixed solve: a device-side wait of the factorisation timed out";
+            return BEMX_ERR_TIMEOUT;
+        }
+        fcode[z] = (hr[z].status & ST_PIVOT_ZERO) ? BEMX_ERR_SINGULAR : BEMX_OK;
     }
     return BEMX_OK;
 }
 
-// z = M^-1 v = U^-1 L^-1 P v   (tmp: work vector of n; z may alias v)
-static hipError_t precond(hipStream_t st, const MixedFactor& F, const cd* v, cd* z, cd* tmp) {
+// z = M^-1 v = U^-1 L^-1 P v for the problems of the batch in act (tmp:
+// work vector of n; z may alias v; every pointer is problem 0's)
+static hipError_t precond(hipStream_t st, const MixedFactor& F, const cd* v, cd* z, cd* tmp,
+                          unsigned act) {
     int n = F.n, nblk = F.npair;
-    size_t lda = F.ld;
+    size_t lda = F.ld, bs = F.bs;
     const cf* A = F.A;
     const cf* li = F.minv;
     const cf* ui = F.minv + (size_t)F.npair * NB2 * NB2;
@@ -1869,63 +2034,69 @@ static hipError_t precond(hipStream_t st, const MixedFactor& F, const cd* v, cd*
     int e2 = ++*F.epoch;
     u64* yll = F.yll;
     void* a[] = {(void*)&A, &lda, &n, &nblk, (void*)&li, (void*)&wl, (void*)&ui, (void*)&wu,
-                 (void*)&v, &tmp, &z, &yll, &flags, &e1, &e2, &status};
-    return hipLaunchCooperativeKernel((const void*)k_trsv_coop, dim3(F.coop), dim3(512), a, 0, st);
+                 (void*)&v, &tmp, &z, &yll, &flags, &e1, &e2, &status, &bs, &act};
+    return hipLaunchCooperativeKernel((const void*)k_trsv_coop, dim3(F.coop, 1, F.nz), dim3(512),
+                                      a, 0, st);
 }
 
-static void gemv(hipStream_t st, int n, const cd* A, size_t lda, const cd* x, const cd* b, cd* r,
-                 cd* part, int sub) {
+// r = b - A x (sub = 1) or r = A x (sub = 0) for the problems in act
+// (matrix z at A + z sa bytes; the vectors in the arenas).
+static void gemv(hipStream_t st, const MixedFactor& F, int n, const cd* A, size_t lda, size_t sa,
+                 const cd* x, const cd* b, cd* r, cd* part, int sub, unsigned act) {
     const int nchunk = (n + MV_COLS - 1) / MV_COLS;
-    hipLaunchKernelGGL(k_zgemv_part, dim3((n + 255) / 256, nchunk), dim3(256), 0, st, A, lda, n, x,
-                       part);
-    hipLaunchKernelGGL(k_gemv_reduce, dim3((n + 255) / 256), dim3(256), 0, st, n, nchunk, part, b, r,
-                       sub);
+    hipLaunchKernelGGL(k_zgemv_part, dim3((n + 255) / 256, nchunk, F.nz), dim3(256), 0, st, A, lda,
+                       n, x, part, sa, F.bs, act);
+    hipLaunchKernelGGL(k_gemv_reduce, dim3((n + 255) / 256, 1, F.nz), dim3(256), 0, st, n, nchunk,
+                       (const cd*)part, b, r, sub, F.bs, act);
 }
 
-// GMRES-IR of one right-hand side against the FP32 factors F.  The outer
-// iteration (residual b - A x in FP64, then a GMRES cycle on M^-1 A d =
-// M^-1 r) runs on the device from start to end: the Arnoldi steps, the
-// Givens rotations (k_gm_step), the least-squares solution and x += V y
-// (k_gm_solve, k_combine_gm), then the next residual and its norm.  The host
-// reads the norms and the cycle's state once per outer iteration to decide
-// whether to stop.  A cycle takes `ch` Arnoldi steps (2 at first; doubled up
-// to GM_RESTART while a cycle ends short of its inner tolerance) and stops
-// early on the device once |g| <= 1e-8 beta or on breakdown.
-static int refine_one(LuWork& w, hipStream_t st, const MixedFactor& F, int n, const cd* A64,
-                      size_t lda, cd* b, int* iters, double* relres_out, std::string& err) {
-    hipError_t e;
+// GMRES-IR of one right-hand side per problem against the FP32 factors F:
+// problem z's right-hand side at b + z sb bytes, for the problems in act0.
+// The outer iteration (residual b - A x in FP64, then a GMRES cycle on
+// M^-1 A d = M^-1 r) runs on the device from start to end: the Arnoldi
+// steps, the Givens rotations (k_gm_step), the least-squares solution and
+// x += V y (k_gm_solve, k_combine_gm), then the next residual and its norm.
+// The host reads the norms and the cycle's state of every problem once per
+// outer iteration to decide which problems go on.  A problem's cycle takes
+// `ch` Arnoldi steps (2 at first; doubled up to GM_RESTART while a cycle ends
+// short of its inner tolerance) and stops early on the device once
+// |g| <= 1e-8 beta or on breakdown; a launch of step k covers the problems
+// whose cycle has more than k steps, so each problem follows its own
+// sequence, that of a batch of one.  Per problem: code[z] (BEMX_OK, b
+// overwritten with x; BEMX_ERR_NOCONV, b intact), iters[z], relres[z].
+static int refine_batch(LuWork& w, hipStream_t st, const MixedFactor& F, int n, const cd* A64,
+                        size_t lda, size_t sa, cd* b, size_t sb, unsigned act0, int* code,
+                        int* iters, double* relres_out, std::string& err) {
+    (void)w;
     const int MR = GM_RESTART;
     const int MAXOUT = 12;    // refinement steps
     const double TOL = 2e-15; // normwise backward error target
     const double TOL_IN = 1e-8;  // inner (GMRES cycle) reduction
-    const int nb256 = (n + 255) / 256;
-    size_t sz[LuWork::SLOTS];  // reserved by factor_mixed: no reallocation here
-    mixed_sizes(n, sz);
-    cd* vec = (cd*)w.get(M_VEC, sz[M_VEC], &e);
-    MX_CHK(e);
+    const int nb256 = (n + 255) / 256, nz = F.nz;
+    const size_t bs = F.bs;
+    const ArenaLayout L = arena_layout(n);
+    cd* vec = (cd*)(F.arena + L.off[A_VEC]);
     cd *x = vec, *r = vec + n, *wv = vec + 2 * (size_t)n, *tmp = vec + 3 * (size_t)n,
        *bsave = vec + 4 * (size_t)n;
-    cd* V = (cd*)w.get(M_V, sz[M_V], &e);
-    MX_CHK(e);
-    cd* part = (cd*)w.get(M_PART, sz[M_PART], &e);
-    MX_CHK(e);
+    cd* V = (cd*)(F.arena + L.off[A_V]);
+    cd* part = (cd*)(F.arena + L.off[A_PART]);
     // device scratch of the Arnoldi step (projections of the two
     // Gram-Schmidt passes and the norm, slots of MR + 2 entries each), then
     // the cycle's state
     const int HS = MR + 2;
-    cd* hdev = (cd*)w.get(M_HOST, sz[M_HOST], &e);
-    MX_CHK(e);
+    cd* hdev = (cd*)(F.arena + L.off[A_HOST]);
     cd *h1 = hdev + HS, *h2 = hdev + 2 * HS;
     GmState* gs = (GmState*)(hdev + 3 * HS);
+    const dim3 gv(nb256, 1, nz);
     // k_dots over nv vectors + reduction into out (nv + 1 entries, the last
     // one ||wvec||^2), all on the device
-    auto dots_dev = [&](const cd* Vb, int nv, const cd* wvec, cd* out) {
-        hipLaunchKernelGGL(k_dots, dim3(nb256), dim3(256), 0, st, n, Vb, (size_t)n, nv, wvec, part);
-        hipLaunchKernelGGL(k_dots_reduce, dim3(1), dim3(64), 0, st, nb256, nv + 1,
-                           (const cd*)part, out);
+    auto dots_dev = [&](const cd* Vb, int nv, const cd* wvec, cd* out, unsigned m) {
+        hipLaunchKernelGGL(k_dots, gv, dim3(256), 0, st, n, Vb, (size_t)n, nv, wvec, part, bs, m);
+        hipLaunchKernelGGL(k_dots_reduce, dim3(1, 1, nz), dim3(64), 0, st, nb256, nv + 1,
+                           (const cd*)part, out, bs, m);
     };
-    auto pre = [&](const cd* v, cd* z) -> int {
-        const hipError_t ep = precond(st, F, v, z, tmp);
+    auto pre = [&](const cd* v, cd* z, unsigned m) -> int {
+        const hipError_t ep = precond(st, F, v, z, tmp, m);
         if (ep == hipErrorIllegalState) {
             err = coop_refused_message("mixed solve, block solves");
             return BEMX_ERR_HIP;
@@ -1933,109 +2104,161 @@ static int refine_one(LuWork& w, hipStream_t st, const MixedFactor& F, int n, co
         MX_CHK(ep);
         return BEMX_OK;
     };
-    // the cycle's norms and state, and the status word, in one round trip
-    GmHead hd;
-    int hst = 0;
+    // every problem's norms, cycle state and status word in one round trip
+    ReadRec hr[MIXED_MAX_BATCH];
     auto readback = [&]() -> int {
-        MX_CHK(hipMemcpyAsync(&hd, gs, sizeof(GmHead), hipMemcpyDeviceToHost, st));
-        MX_CHK(hipMemcpyAsync(&hst, F.status, sizeof(int), hipMemcpyDeviceToHost, st));
+        hipLaunchKernelGGL(k_gather_heads, dim3(1, 1, nz), dim3(64), 0, st, (const GmState*)gs,
+                           (const int*)F.status, bs, F.rrec);
+        MX_CHK(hipMemcpyAsync(hr, F.rrec, sizeof(ReadRec) * nz, hipMemcpyDeviceToHost, st));
         MX_CHK(hipStreamSynchronize(st));
-        if (hst & ST_SYNC_TIMEOUT) {
-            err = "mixed solve: a device-side wait of the triangular solves timed out";
-            return BEMX_ERR_TIMEOUT;
-        }
+        for (int z = 0; z < nz; ++z)
+            if (hr[z].status & ST_SYNC_TIMEOUT) {
+                err = "mixed solve: a device-side wait of the triangular solves timed out";
+                return BEMX_ERR_TIMEOUT;
+            }
         return BEMX_OK;
     };
-    hipLaunchKernelGGL(k_copy_z, dim3(nb256), dim3(256), 0, st, b, bsave, n);
-    int s = pre(bsave, x);  // x0 = M^-1 b
+    hipLaunchKernelGGL(k_copy_z, gv, dim3(256), 0, st, (const cd*)b, bsave, n, sb, bs, act0);
+    int s = pre(bsave, x, act0);  // x0 = M^-1 b
     if (s) return s;
-    dots_dev(bsave, 0, bsave, &gs->head.bn2);
-    gemv(st, n, A64, lda, x, bsave, r, part, 1);  // r = b - A x (FP64)
-    dots_dev(r, 0, r, &gs->head.rn2);
+    dots_dev(bsave, 0, bsave, &gs->head.bn2, act0);
+    gemv(st, F, n, A64, lda, sa, x, bsave, r, part, 1, act0);  // r = b - A x (FP64)
+    dots_dev(r, 0, r, &gs->head.rn2, act0);
     if ((s = readback())) return s;
-    const double bnorm = std::sqrt(hd.bn2.x);
-    double relres = std::sqrt(hd.rn2.x) / std::max(bnorm, 1e-300), prev = relres;
-    int it_total = 0, ch = 2;
-    bool converged = false;
+    double bnorm[MIXED_MAX_BATCH], relres[MIXED_MAX_BATCH], prev[MIXED_MAX_BATCH];
+    int ch[MIXED_MAX_BATCH], it_total[MIXED_MAX_BATCH];
+    for (int z = 0; z < nz; ++z) {
+        bnorm[z] = std::sqrt(hr[z].head.bn2.x);
+        relres[z] = prev[z] = std::sqrt(hr[z].head.rn2.x) / std::max(bnorm[z], 1e-300);
+        ch[z] = 2;
+        it_total[z] = 0;
+    }
+    unsigned act = act0, conv = 0;
     for (int outer = 0;; ++outer) {
         // ||A x|| ~ ||b||, so ||r|| <= TOL ||b|| bounds the normwise backward
         // error ||r|| / (||A|| ||x|| + ||b||) by TOL; a residual that stops
         // shrinking at a level near the FP64 rounding of A x is accepted too.
-        if (relres <= TOL || (outer >= 2 && relres > 0.5 * prev && relres <= 1e-13)) {
-            converged = true;
-            break;
+        for (int z = 0; z < nz; ++z) {
+            if (!((act >> z) & 1u)) continue;
+            if (relres[z] <= TOL || (outer >= 2 && relres[z] > 0.5 * prev[z] && relres[z] <= 1e-13)) {
+                conv |= 1u << z;
+                act &= ~(1u << z);
+            } else if (outer == MAXOUT) {
+                act &= ~(1u << z);
+            }
         }
-        if (outer == MAXOUT) break;
+        if (!act) break;
+        int chmax = 0;
+        for (int z = 0; z < nz; ++z)
+            if ((act >> z) & 1u) chmax = std::max(chmax, ch[z]);
         // GMRES cycle on M^-1 A d = M^-1 r: v0 = M^-1 r / beta
-        if ((s = pre(r, V))) return s;
-        dots_dev(V, 0, V, &gs->head.beta2);
-        hipLaunchKernelGGL(k_scale_by_norm, dim3(nb256), dim3(256), 0, st, n, V, V,
-                           &gs->head.beta2);
-        hipLaunchKernelGGL(k_gm_start, dim3(1), dim3(64), 0, st, gs);
-        for (int k = 0; k < ch; ++k) {
-            gemv(st, n, A64, lda, V + (size_t)k * n, nullptr, wv, part, 0);
-            if ((s = pre(wv, wv))) return s;
+        if ((s = pre(r, V, act))) return s;
+        dots_dev(V, 0, V, &gs->head.beta2, act);
+        hipLaunchKernelGGL(k_scale_by_norm, gv, dim3(256), 0, st, n, (const cd*)V, V,
+                           (const cd*)&gs->head.beta2, bs, act);
+        hipLaunchKernelGGL(k_gm_start, dim3(1, 1, nz), dim3(64), 0, st, gs, bs, act);
+        for (int k = 0; k < chmax; ++k) {
+            unsigned mk = 0;  // the problems whose cycle has a step k
+            for (int z = 0; z < nz; ++z)
+                if (((act >> z) & 1u) && ch[z] > k) mk |= 1u << z;
+            gemv(st, F, n, A64, lda, sa, V + (size_t)k * n, nullptr, wv, part, 0, mk);
+            if ((s = pre(wv, wv, mk))) return s;
             // classical Gram-Schmidt, two passes, and the normalisation
-            dots_dev(V, k + 1, wv, hdev);
-            hipLaunchKernelGGL(k_orth_update, dim3(nb256), dim3(256), 0, st, n, V, (size_t)n,
-                               k + 1, hdev, wv, wv, 1.0);
-            dots_dev(V, k + 1, wv, h1);
-            hipLaunchKernelGGL(k_orth_update, dim3(nb256), dim3(256), 0, st, n, V, (size_t)n,
-                               k + 1, h1, wv, wv, 1.0);
-            dots_dev(V, 0, wv, h2);
-            hipLaunchKernelGGL(k_scale_by_norm, dim3(nb256), dim3(256), 0, st, n, wv,
-                               V + (size_t)(k + 1) * n, h2);
-            hipLaunchKernelGGL(k_gm_step, dim3(1), dim3(64), 0, st, gs, k, (const cd*)hdev,
-                               (const cd*)h1, (const cd*)h2, TOL_IN);
+            dots_dev(V, k + 1, wv, hdev, mk);
+            hipLaunchKernelGGL(k_orth_update, gv, dim3(256), 0, st, n, (const cd*)V, (size_t)n,
+                               k + 1, (const cd*)hdev, wv, wv, 1.0, bs, mk);
+            dots_dev(V, k + 1, wv, h1, mk);
+            hipLaunchKernelGGL(k_orth_update, gv, dim3(256), 0, st, n, (const cd*)V, (size_t)n,
+                               k + 1, (const cd*)h1, wv, wv, 1.0, bs, mk);
+            dots_dev(V, 0, wv, h2, mk);
+            hipLaunchKernelGGL(k_scale_by_norm, gv, dim3(256), 0, st, n, (const cd*)wv,
+                               V + (size_t)(k + 1) * n, (const cd*)h2, bs, mk);
+            hipLaunchKernelGGL(k_gm_step, dim3(1, 1, nz), dim3(64), 0, st, gs, k, (const cd*)hdev,
+                               (const cd*)h1, (const cd*)h2, TOL_IN, bs, mk);
         }
         // y = H^-1 g, x += V y, then the next residual
-        hipLaunchKernelGGL(k_gm_solve, dim3(1), dim3(64), 0, st, gs);
-        hipLaunchKernelGGL(k_combine_gm, dim3(nb256), dim3(256), 0, st, n, (const cd*)V,
-                           (size_t)n, (const GmState*)gs, x);
-        gemv(st, n, A64, lda, x, bsave, r, part, 1);
-        dots_dev(r, 0, r, &gs->head.rn2);
+        hipLaunchKernelGGL(k_gm_solve, dim3(1, 1, nz), dim3(64), 0, st, gs, bs, act);
+        hipLaunchKernelGGL(k_combine_gm, gv, dim3(256), 0, st, n, (const cd*)V, (size_t)n,
+                           (const GmState*)gs, x, bs, act);
+        gemv(st, F, n, A64, lda, sa, x, bsave, r, part, 1, act);
+        dots_dev(r, 0, r, &gs->head.rn2, act);
         if ((s = readback())) return s;
-        it_total += hd.kk;
-        prev = relres;
-        relres = std::sqrt(hd.rn2.x) / std::max(bnorm, 1e-300);
-        // a cycle that used all its steps short of the inner tolerance: a
-        // longer one next time
-        if (!hd.stop && hd.kk == ch) ch = std::min(2 * ch, MR);
+        for (int z = 0; z < nz; ++z) {
+            if (!((act >> z) & 1u)) continue;
+            const GmHead& hd = hr[z].head;
+            it_total[z] += hd.kk;
+            prev[z] = relres[z];
+            relres[z] = std::sqrt(hd.rn2.x) / std::max(bnorm[z], 1e-300);
+            // a cycle that used all its steps short of the inner tolerance: a
+            // longer one next time
+            if (!hd.stop && hd.kk == ch[z]) ch[z] = std::min(2 * ch[z], MR);
+        }
     }
     MX_CHK(hipGetLastError());
-    *iters = it_total;
-    *relres_out = relres;
-    if (!converged) {
-        err = "mixed solve: GMRES-IR did not reach the FP64 residual level";
-        return BEMX_ERR_NOCONV;
+    for (int z = 0; z < nz; ++z) {
+        if (!((act0 >> z) & 1u)) continue;
+        iters[z] = it_total[z];
+        relres_out[z] = relres[z];
+        code[z] = ((conv >> z) & 1u) ? BEMX_OK : BEMX_ERR_NOCONV;
     }
-    hipLaunchKernelGGL(k_copy_z, dim3(nb256), dim3(256), 0, st, x, b, n);
+    if (conv)
+        hipLaunchKernelGGL(k_copy_z, gv, dim3(256), 0, st, (const cd*)x, b, n, bs, sb, conv);
     MX_CHK(hipGetLastError());
+    return BEMX_OK;
+}
+
+int gmres_ir_batch(LuWork& w, hipStream_t st, int n, int nz, const cd* A64, size_t lda, size_t sa,
+                   cd* b, size_t sb, int nrhs, int* code, int* nsolved, double* stats,
+                   std::string& err) {
+    if (nz < 1 || nz > MIXED_MAX_BATCH || n < 1 || nrhs < 1) {
+        err = "mixed solve: batch of 1..8 problems, order and right-hand sides at least 1";
+        return BEMX_ERR_ARG;
+    }
+    MixedFactor F;
+    int fc[MIXED_MAX_BATCH];
+    int s = factor_mixed(w, st, n, A64, lda, sa, nz, F, fc, err);
+    if (s) return s;
+    unsigned act = 0;
+    for (int z = 0; z < nz; ++z) {
+        nsolved[z] = 0;
+        for (int i = 0; i < 4; ++i) stats[4 * z + i] = 0.0;
+        code[z] = fc[z];
+        if (fc[z] == BEMX_OK) act |= 1u << z;
+    }
+    for (int j = 0; j < nrhs && act; ++j) {
+        int c[MIXED_MAX_BATCH] = {0}, it[MIXED_MAX_BATCH] = {0};
+        double rr[MIXED_MAX_BATCH] = {0};
+        s = refine_batch(w, st, F, n, A64, lda, sa, b + (size_t)j * n, sb, act, c, it, rr, err);
+        if (s) return s;
+        for (int z = 0; z < nz; ++z) {
+            if (!((act >> z) & 1u)) continue;
+            stats[4 * z] += it[z];
+            stats[4 * z + 1] = std::max(stats[4 * z + 1], rr[z]);
+            if (c[z] != BEMX_OK) {
+                code[z] = c[z];
+                act &= ~(1u << z);
+            } else {
+                nsolved[z] = j + 1;
+            }
+        }
+    }
     return BEMX_OK;
 }
 
 int gmres_ir(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda, cd* b, int nrhs,
              double stats[4], std::string& err, int* nsolved) {
+    int code = BEMX_OK;
+    double st4[4] = {0, 0, 0, 0};
     *nsolved = 0;
-    MixedFactor F;
-    int s = factor_mixed(w, st, n, A64, lda, F, err);
-    if (s) return s;
-    int iters = 0;
-    double worst = 0.0;
-    for (int j = 0; j < nrhs; ++j) {
-        int it = 0;
-        double rr = 0.0;
-        s = refine_one(w, st, F, n, A64, lda, b + (size_t)j * n, &it, &rr, err);
-        iters += it;
-        worst = std::max(worst, rr);
-        if (stats) {
-            stats[0] = iters;
-            stats[1] = worst;
-        }
-        if (s) return s;
-        *nsolved = j + 1;
+    const int s = gmres_ir_batch(w, st, n, 1, A64, lda, 0, b, 0, nrhs, &code, nsolved, st4, err);
+    if (stats) {
+        stats[0] = st4[0];
+        stats[1] = st4[1];
     }
-    return BEMX_OK;
+    if (s) return s;
+    if (code == BEMX_ERR_SINGULAR) err = "mixed solve: zero pivot in a diagonal block";
+    if (code == BEMX_ERR_NOCONV) err = "mixed solve: GMRES-IR did not reach the FP64 residual level";
+    return code;
 }
 
 }  // namespace bemx

@@ -1,13 +1,15 @@
 """bemx_solve_sweep, the frequency loop of the reference's main program
 (Main.for:167-200: GHMATECD + LINSOLVE per frequency) on the device (GPU).
 
-Frequencies are assembled two at a time, the H blocks of both in one launch
-(k_dyn_offdiag_h2, the Gauss-point geometry shared), on a second stream that
-overlaps the previous pair's solves.  Each frequency's arithmetic is that of
-bemx_solve, so every solution must be bit-identical to solving that
-frequency alone: five frequencies (two pairs and a single one at the end),
-down to omega = 0.1 where the near-field series kernel carries most of the
-element pairs near the diagonal, with two load cases, in both modes.
+Frequencies go in batches (4 by default): a batch is assembled two
+frequencies per launch (k_dyn_offdiag_h2, the Gauss-point geometry shared)
+on a second stream while the previous batch is solved, and the batch's
+mixed solves share every launch of the factorisation and the refinement
+(problem = blockIdx.z).  Each frequency's arithmetic is that of bemx_solve,
+so every solution must be bit-identical to solving that frequency alone:
+five frequencies (a batch of four and one of one) and seven (four and
+three), down to omega = 0.1 where the near-field series kernel carries most
+of the element pairs near the diagonal, with two load cases, in both modes.
 """
 import numpy as np
 import pytest
@@ -17,6 +19,7 @@ from bemx import api, decks
 pytestmark = pytest.mark.gpu
 
 FREQS = (0.1, 0.7, 2.0, 5.5, 9.0)
+FREQS7 = (0.1, 0.35, 0.7, 1.3, 2.0, 5.5, 9.0)
 
 
 @pytest.fixture(scope="module")
@@ -30,15 +33,16 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("mode", [api.MODE_MIXED, api.MODE_FP64])
-def test_sweep_equals_each_frequency_alone(ctx, mode):
+@pytest.mark.parametrize("mode,freqs", [(api.MODE_MIXED, FREQS), (api.MODE_FP64, FREQS),
+                                        (api.MODE_MIXED, FREQS7)])
+def test_sweep_equals_each_frequency_alone(ctx, mode, freqs):
     c, mesh, dyn = ctx
     zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
-    print("sweep of", len(FREQS), "frequencies, mode", mode, flush=True)
-    xs, st = c.solve_sweep(zmat, FREQS, mode)
-    assert xs.shape == (len(FREQS), 3 * mesh.nbe, 2)
+    print("sweep of", len(freqs), "frequencies, mode", mode, flush=True)
+    xs, st = c.solve_sweep(zmat, freqs, mode)
+    assert xs.shape == (len(freqs), 3 * mesh.nbe, 2)
     assert np.all(st[:, 2] == 0.0)  # no fallback to FP64
-    for k, fr in enumerate(FREQS):
+    for k, fr in enumerate(freqs):
         x, s1 = c.solve(zmat, fr, mode)
         print("omega", fr, "max |sweep - alone|", float(np.max(np.abs(xs[k] - x))), flush=True)
         np.testing.assert_array_equal(xs[k], x)
