@@ -388,19 +388,20 @@ class Server:
         return rid.value
 
     def stats(self) -> dict:
-        out = np.zeros(8)
-        code = self._lib.bemx_server_stats(self.h, _p(out), 8)
+        out = np.zeros(9)
+        code = self._lib.bemx_server_stats(self.h, _p(out), 9)
         if code:
             raise BemxError(code, "bemx_server_stats")
         keys = ("completed", "graph_captures", "graph_replays", "charged_bytes",
-                "peak_charged_bytes", "budget_bytes", "workspace_releases", "over_charge")
+                "peak_charged_bytes", "budget_bytes", "workspace_releases", "over_charge",
+                "batches")
         return {k: (int(v) if k != "budget_bytes" else float(v)) for k, v in zip(keys, out)}
 
     def wait(self, rid: int):
         """-> (x (3NBE,) or (3NBE, nrhs), info dict)."""
         shape = self._sizes.pop(rid)
         x = np.zeros(shape[0] * shape[1], dtype=np.complex128)
-        info = np.zeros(10)
+        info = np.zeros(11)
         msg = C.create_string_buffer(512)
         code = self._lib.bemx_server_wait(self.h, rid, _p(x.view(np.float64)), _p(info), msg, 512)
         if code:
@@ -408,7 +409,7 @@ class Server:
         if shape[1] > 1:
             x = x.reshape(shape, order="F")
         keys = ("latency_ms", "queue_ms", "service_ms", "gmres_iters", "relres", "fell_back",
-                "slot", "device_ms", "start_ms", "done_ms")
+                "slot", "device_ms", "start_ms", "done_ms", "batch")
         return x, dict(zip(keys, info.tolist()))
 
 

@@ -103,7 +103,7 @@ struct bemx_ctx {
     // sides are contiguous (a batch's solve takes them at one stride), the
     // assembly's stream and the halves' events
     static constexpr int SWB = bemx::MIXED_MAX_BATCH, NSW = 2 * SWB;
-    DevBuf sw_zh, sw_zfi, sw_rhs_part[NSW], sw_rhs_diag[NSW], sw_etab[NSW];
+    DevBuf sw_zh, sw_zfi, sw_rhs_part[NSW], sw_rhs_diag[NSW], sw_etab[NSW], sw_zdfi[NSW];
     hipStream_t st_asm = nullptr;
     hipEvent_t sw_asm[2] = {nullptr, nullptr}, sw_done[2] = {nullptr, nullptr};
     // solver
@@ -212,7 +212,7 @@ static void ctx_buffers(bemx_ctx* c, std::vector<DevBuf*>& out) {
     out.push_back(&c->sw_zh);
     out.push_back(&c->sw_zfi);
     for (int i = 0; i < bemx_ctx::NSW; ++i)
-        for (DevBuf* d : {&c->sw_rhs_part[i], &c->sw_rhs_diag[i], &c->sw_etab[i]})
+        for (DevBuf* d : {&c->sw_rhs_part[i], &c->sw_rhs_diag[i], &c->sw_etab[i], &c->sw_zdfi[i]})
             out.push_back(d);
 }
 
@@ -245,11 +245,12 @@ void ctx_release_workspace(bemx_ctx* c) {
 
 // Upper bound of the device bytes a context holds after serving meshes of at
 // most these sizes through bemx_send_shared_data, bemx_ghmatece,
-// bemx_send_load_cases and bemx_solve: every buffer those calls grow (they
-// only grow), at its largest.  The FP64 solver workspace is counted in both
-// modes, since the mixed path falls back to it.  Monotone in each argument,
-// so the bound over several requests is the bound at their maxima.
-size_t ctx_solve_bytes_bound(int n, int nbe, int np, int npg, int nrhs, int mode) {
+// bemx_send_load_cases, bemx_solve and batches of up to nb problems through
+// ctx_solve_items: every buffer those calls grow (they only grow), at its
+// largest.  The FP64 solver workspace is counted in both modes, since the
+// mixed path falls back to it.  Monotone in each argument, so the bound over
+// several requests is the bound at their maxima.
+size_t ctx_solve_bytes_bound(int n, int nbe, int np, int npg, int nrhs, int mode, int nb) {
     auto r = [](size_t b) { return std::max<size_t>(b, 256); };
     const size_t nn = 3 * (size_t)nbe, npg2 = (size_t)npg * npg, rows = (nbe + 63) / 64;
     size_t b = r(sizeof(int));  // status word
@@ -263,7 +264,15 @@ size_t ctx_solve_bytes_bound(int n, int nbe, int np, int npg, int nrhs, int mode
          r(sizeof(cd) * nn * offdiag_tiles(nbe) * nrhs) + 2 * r(sizeof(cd) * nn * nrhs) +
          r(sizeof(cd) * nn * nn) + r(exp_table_max_bytes());
     b += fp64_factor_bytes((int)nn);
-    if (mode == BEMX_MODE_MIXED) b += mixed_workspace_bytes((int)nn);
+    nb = std::max(1, nb);
+    if (nb > 1) {
+        // the batch's frequency slots: matrices and right-hand sides in one
+        // buffer each, partials, exponential tables and load cases per slot
+        b += r(sizeof(cd) * nn * nn * nb) + r(sizeof(cd) * nn * nrhs * nb);
+        b += nb * (r(sizeof(cd) * nn * offdiag_tiles(nbe) * nrhs) + r(sizeof(cd) * nn * nrhs) +
+                   r(exp_table_max_bytes()) + r(sizeof(cd) * nn * nrhs));
+    }
+    if (mode == BEMX_MODE_MIXED) b += mixed_workspace_bytes((int)nn, nb);
     return b;
 }
 
@@ -812,7 +821,10 @@ int bemx_solve(bemx_ctx* c, const double* zge, const double* zcs, const double* 
     return BEMX_OK;
 }
 
-// Frequencies per batch of the sweep (BEMX_SWEEP_BATCH, 1..8; default 4).
+}  // extern "C"
+
+// Frequencies per batch of the batched solves (BEMX_SWEEP_BATCH, 1..8;
+// default 4).
 static int sweep_batch() {
     static const int b = [] {
         const char* v = getenv("BEMX_SWEEP_BATCH");
@@ -821,6 +833,153 @@ static int sweep_batch() {
     }();
     return b;
 }
+
+namespace bemx {
+
+int batch_size() { return sweep_batch(); }
+
+// Batched solves on one context whose mesh, static pass and KODE are in
+// place: items go in batches of up to batch_size() problems.  A batch is
+// assembled (two problems per launch where they share their load cases) on
+// the context's second stream into one half of the frequency slots while
+// the previous batch is solved on the context's stream as one batch
+// (lu_solve_batch).  Each problem's arithmetic is that of bemx_solve.
+int ctx_solve_items(bemx_ctx* c, int nitem, SolveItem* items, int mode) {
+    if (nitem < 1) return BEMX_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nbe = c->nbe, npg2 = c->npg * c->npg, nr = c->nrhs;
+    const size_t nn = 3 * (size_t)nbe;
+    std::vector<DynConst> K(nitem);
+    std::vector<int> nt(nitem);
+    std::vector<double> h(nitem);
+    int ntmax = 0;
+    for (int f = 0; f < nitem; ++f) {
+        const SolveItem& it = items[f];
+        K[f] = make_dyn_const(c, ld_c(it.zge), ld_c(it.zcs), ld_c(it.zcp), it.fr, 1.0);
+        nt[f] = exp_table_entries(K[f], c->rmax, &h[f]);
+        ntmax = std::max(ntmax, nt[f]);
+    }
+    // problems per batch: at most batch_size(), and the matrices and mixed
+    // workspaces of the slots within half of the device memory now free
+    const size_t zh_bytes = sizeof(cd) * nn * nn, zfi_bytes = sizeof(cd) * nn * nr;
+    int nbat = std::min(sweep_batch(), nitem);
+    const int halves = nitem > nbat ? 2 : 1;
+    {
+        size_t fr = 0, tot = 0;
+        HIPCHK(c, hipMemGetInfo(&fr, &tot));
+        fr += c->sw_zh.cap + c->lu.bytes();
+        while (nbat > 1 && halves * nbat * (zh_bytes + zfi_bytes) +
+                                   mixed_workspace_bytes((int)nn, nbat) > fr / 2)
+            --nbat;
+    }
+    const int nslot = std::min(nitem, halves * nbat);
+    HIPCHK(c, c->colflag.ensure(sizeof(int) * nbe));
+    HIPCHK(c, c->flist.ensure(sizeof(int) * (nbe + 1)));
+    HIPCHK(c, c->sw_zh.ensure(zh_bytes * nslot));
+    HIPCHK(c, c->sw_zfi.ensure(zfi_bytes * nslot));
+    bool own_cases = false;
+    for (int f = 0; f < nitem; ++f) own_cases = own_cases || items[f].zdfi;
+    for (int i = 0; i < nslot; ++i) {
+        HIPCHK(c, c->sw_rhs_part[i].ensure(sizeof(cd) * nn * offdiag_tiles(nbe) * nr));
+        HIPCHK(c, c->sw_rhs_diag[i].ensure(sizeof(cd) * nn * nr));
+        if (ntmax) HIPCHK(c, c->sw_etab[i].ensure(sizeof(cd) * 2 * (size_t)ntmax));
+        if (own_cases) HIPCHK(c, c->sw_zdfi[i].ensure(zfi_bytes));
+    }
+    if (!c->st_asm) HIPCHK(c, hipStreamCreateWithFlags(&c->st_asm, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) {
+        if (!c->sw_asm[i]) HIPCHK(c, hipEventCreateWithFlags(&c->sw_asm[i], hipEventDisableTiming));
+        if (!c->sw_done[i])
+            HIPCHK(c, hipEventCreateWithFlags(&c->sw_done[i], hipEventDisableTiming));
+    }
+    c->zh_valid = false;
+    const ElemData* elem = c->elem.as<ElemData>();
+    const GaussPt* gp = c->gp.as<GaussPt>();
+    auto zh_of = [&](int sl) { return (cd*)((char*)c->sw_zh.p + zh_bytes * sl); };
+    auto zfi_of = [&](int sl) { return (cd*)((char*)c->sw_zfi.p + zfi_bytes * sl); };
+    // the load cases of problem f in slot sl on the device
+    auto cases_of = [&](int f, int sl) {
+        return items[f].zdfi ? c->sw_zdfi[sl].as<cd>() : c->zdfi.as<cd>();
+    };
+    // batch b: problems nbat b .. into the slots of half b & 1, two per
+    // assembly launch when they share their load cases
+    auto assemble = [&](int b) -> int {
+        const int f0 = nbat * b, nf = std::min(nbat, nitem - f0), s0 = nbat * (b & 1);
+        for (int i = 0; i < nf; ++i)
+            if (items[f0 + i].zdfi)
+                HIPCHK(c, hipMemcpyAsync(c->sw_zdfi[s0 + i].p, items[f0 + i].zdfi, zfi_bytes,
+                                         hipMemcpyHostToDevice, c->st_asm));
+        for (int i = 0; i < nf; i += 2) {
+            const int np = std::min(2, nf - i);
+            DynFreq F[2];
+            for (int q = 0; q < np; ++q) {
+                const int f = f0 + i + q, s = s0 + i + q;
+                F[q] = DynFreq{K[f], nt[f] ? c->sw_etab[s].as<cd>() : nullptr, nt[f], h[f],
+                               zh_of(s), c->sw_rhs_part[s].as<cd>(), c->sw_rhs_diag[s].as<cd>(),
+                               zfi_of(s)};
+            }
+            const bool pair = np == 2 && nt[f0 + i] > 0 && nt[f0 + i + 1] > 0 &&
+                              !items[f0 + i].zdfi && !items[f0 + i + 1].zdfi;
+            if (pair) {
+                HIPCHK(c, launch_dyn_assembly_pair(c->st_asm, nbe, npg2, elem, gp,
+                                                   c->hdiag.as<double>(), c->gdiag.as<double>(),
+                                                   c->kode.as<int>(), c->zdfi.as<cd>(), nr,
+                                                   c->colflag.as<int>(), c->flist.as<int>(), F,
+                                                   nn, nullptr, nullptr));
+            } else {
+                for (int q = 0; q < np; ++q)
+                    HIPCHK(c, launch_dyn_assembly(c->st_asm, nbe, npg2, elem, gp,
+                                                  c->hdiag.as<double>(), c->gdiag.as<double>(),
+                                                  c->kode.as<int>(), cases_of(f0 + i + q, s0 + i + q),
+                                                  nr, c->colflag.as<int>(), c->flist.as<int>(),
+                                                  F[q].K, F[q].etab, F[q].nt, F[q].h, F[q].zh,
+                                                  nn, F[q].rhs_part, F[q].rhs_diag, F[q].zfi,
+                                                  nullptr, nullptr));
+            }
+        }
+        HIPCHK(c, hipEventRecord(c->sw_asm[b & 1], c->st_asm));
+        return BEMX_OK;
+    };
+    const int nb = (nitem + nbat - 1) / nbat;
+    std::vector<double> st4(4 * nbat);
+    HIPCHK(c, hipEventRecord(c->ev0, c->st));
+    HIPCHK(c, hipStreamWaitEvent(c->st_asm, c->ev0, 0));  // after the context's earlier work
+    int s = assemble(0);
+    for (int b = 0; b < nb && !s; ++b) {
+        // the next batch's assembly beside this batch's solves, once the
+        // solves that last used its slots are done
+        if (b + 1 < nb) {
+            if (b >= 1) HIPCHK(c, hipStreamWaitEvent(c->st_asm, c->sw_done[(b + 1) & 1], 0));
+            s = assemble(b + 1);
+            if (s) break;
+        }
+        HIPCHK(c, hipStreamWaitEvent(c->st, c->sw_asm[b & 1], 0));
+        const int f0 = nbat * b, nf = std::min(nbat, nitem - f0), s0 = nbat * (b & 1);
+        s = lu_solve_batch(c->lu, c->st, (int)nn, nf, zh_of(s0), nn, zh_bytes, zfi_of(s0),
+                           zfi_bytes, nr, mode, st4.data(), c->err);
+        if (s) break;
+        for (int i = 0; i < nf; ++i) {
+            SolveItem& it = items[f0 + i];
+            if (it.x)
+                HIPCHK(c, hipMemcpyAsync(it.x, zfi_of(s0 + i), zfi_bytes, hipMemcpyDeviceToHost,
+                                         c->st));
+            for (int q = 0; q < 4; ++q) it.stats[q] = st4[4 * i + q];
+            it.stats[3] = 0.0;
+        }
+        HIPCHK(c, hipEventRecord(c->sw_done[b & 1], c->st));
+    }
+    hipStreamSynchronize(c->st_asm);
+    if (s) {
+        hipStreamSynchronize(c->st);
+        return s;
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return BEMX_OK;
+}
+
+}  // namespace bemx
+
+extern "C" {
 
 int bemx_solve_sweep(bemx_ctx* c, const double* zge, const double* zcs, const double* zcp,
                      int nfreq, const double* freqs, int mode, double* zfi_out, double* stats) {
@@ -836,126 +995,30 @@ int bemx_solve_sweep(bemx_ctx* c, const double* zge, const double* zcs, const do
     if (!c->case_groups.empty())
         return fail(c, BEMX_ERR_ARG,
                     "%sload cases with KODE sets of their own are solved with bemx_solve", "", 0);
-    HIPCHK(c, hipSetDevice(c->device));
-    const int nbe = c->nbe, npg2 = c->npg * c->npg, nr = c->nrhs;
-    const size_t nn = 3 * (size_t)nbe;
-    std::vector<DynConst> K(nfreq);
-    std::vector<int> nt(nfreq);
-    std::vector<double> h(nfreq);
-    int ntmax = 0;
+    const size_t nn = 3 * (size_t)c->nbe;
+    std::vector<bemx::SolveItem> items(nfreq);
     for (int f = 0; f < nfreq; ++f) {
-        K[f] = make_dyn_const(c, ld_c(zge), ld_c(zcs), ld_c(zcp), freqs[f], 1.0);
-        nt[f] = exp_table_entries(K[f], c->rmax, &h[f]);
-        ntmax = std::max(ntmax, nt[f]);
-    }
-    // frequencies per batch: at most sweep_batch(), and the two halves' matrices
-    // and mixed workspaces within half of the device memory now free
-    const size_t zh_bytes = sizeof(cd) * nn * nn, zfi_bytes = sizeof(cd) * nn * nr;
-    int nbat = std::min(sweep_batch(), nfreq);
-    {
-        size_t fr = 0, tot = 0;
-        HIPCHK(c, hipMemGetInfo(&fr, &tot));
-        fr += c->sw_zh.cap + c->lu.bytes();
-        while (nbat > 1 && 2 * nbat * (zh_bytes + zfi_bytes) +
-                                   bemx::mixed_workspace_bytes((int)nn, nbat) > fr / 2)
-            --nbat;
-    }
-    HIPCHK(c, c->colflag.ensure(sizeof(int) * nbe));
-    HIPCHK(c, c->flist.ensure(sizeof(int) * (nbe + 1)));
-    HIPCHK(c, c->sw_zh.ensure(zh_bytes * 2 * nbat));
-    HIPCHK(c, c->sw_zfi.ensure(zfi_bytes * 2 * nbat));
-    for (int i = 0; i < 2 * nbat; ++i) {
-        HIPCHK(c, c->sw_rhs_part[i].ensure(sizeof(cd) * nn * offdiag_tiles(nbe) * nr));
-        HIPCHK(c, c->sw_rhs_diag[i].ensure(sizeof(cd) * nn * nr));
-        if (ntmax) HIPCHK(c, c->sw_etab[i].ensure(sizeof(cd) * 2 * (size_t)ntmax));
-    }
-    if (!c->st_asm) HIPCHK(c, hipStreamCreateWithFlags(&c->st_asm, hipStreamNonBlocking));
-    for (int i = 0; i < 2; ++i) {
-        if (!c->sw_asm[i]) HIPCHK(c, hipEventCreateWithFlags(&c->sw_asm[i], hipEventDisableTiming));
-        if (!c->sw_done[i])
-            HIPCHK(c, hipEventCreateWithFlags(&c->sw_done[i], hipEventDisableTiming));
-    }
-    c->zh_valid = false;
-    const ElemData* elem = c->elem.as<ElemData>();
-    const GaussPt* gp = c->gp.as<GaussPt>();
-    auto zh_of = [&](int sl) { return (cd*)((char*)c->sw_zh.p + zh_bytes * sl); };
-    auto zfi_of = [&](int sl) { return (cd*)((char*)c->sw_zfi.p + zfi_bytes * sl); };
-    // batch b: frequencies nbat b .. into the slots of half b & 1, two
-    // frequencies per assembly launch
-    auto assemble = [&](int b) -> int {
-        const int f0 = nbat * b, nf = std::min(nbat, nfreq - f0), s0 = nbat * (b & 1);
-        for (int i = 0; i < nf; i += 2) {
-            const int np = std::min(2, nf - i);
-            DynFreq F[2];
-            for (int q = 0; q < np; ++q) {
-                const int f = f0 + i + q, s = s0 + i + q;
-                F[q] = DynFreq{K[f], nt[f] ? c->sw_etab[s].as<cd>() : nullptr, nt[f], h[f],
-                               zh_of(s), c->sw_rhs_part[s].as<cd>(), c->sw_rhs_diag[s].as<cd>(),
-                               zfi_of(s)};
-            }
-            if (np == 2 && nt[f0 + i] > 0 && nt[f0 + i + 1] > 0) {
-                HIPCHK(c, launch_dyn_assembly_pair(c->st_asm, nbe, npg2, elem, gp,
-                                                   c->hdiag.as<double>(), c->gdiag.as<double>(),
-                                                   c->kode.as<int>(), c->zdfi.as<cd>(), nr,
-                                                   c->colflag.as<int>(), c->flist.as<int>(), F,
-                                                   nn, nullptr, nullptr));
-            } else {
-                for (int q = 0; q < np; ++q)
-                    HIPCHK(c, launch_dyn_assembly(c->st_asm, nbe, npg2, elem, gp,
-                                                  c->hdiag.as<double>(), c->gdiag.as<double>(),
-                                                  c->kode.as<int>(), c->zdfi.as<cd>(), nr,
-                                                  c->colflag.as<int>(), c->flist.as<int>(),
-                                                  F[q].K, F[q].etab, F[q].nt, F[q].h, F[q].zh,
-                                                  nn, F[q].rhs_part, F[q].rhs_diag, F[q].zfi,
-                                                  nullptr, nullptr));
-            }
+        bemx::SolveItem& it = items[f];
+        for (int q = 0; q < 2; ++q) {
+            it.zge[q] = zge[q];
+            it.zcs[q] = zcs[q];
+            it.zcp[q] = zcp[q];
         }
-        HIPCHK(c, hipEventRecord(c->sw_asm[b & 1], c->st_asm));
-        return BEMX_OK;
-    };
-    const int nb = (nfreq + nbat - 1) / nbat;
+        it.fr = freqs[f];
+        it.zdfi = nullptr;
+        it.x = zfi_out ? zfi_out + 2 * nn * c->nrhs * f : nullptr;
+    }
+    const int s = bemx::ctx_solve_items(c, nfreq, items.data(), mode);
+    if (s) return s;
     double acc[NLAST] = {0};
-    std::vector<double> st4(4 * nbat);
-    HIPCHK(c, hipEventRecord(c->ev0, c->st));
-    HIPCHK(c, hipStreamWaitEvent(c->st_asm, c->ev0, 0));  // after the context's earlier work
-    int s = assemble(0);
-    for (int b = 0; b < nb && !s; ++b) {
-        // the next batch's assembly beside this batch's solves, once the
-        // solves that last used its slots are done
-        if (b + 1 < nb) {
-            if (b >= 1) HIPCHK(c, hipStreamWaitEvent(c->st_asm, c->sw_done[(b + 1) & 1], 0));
-            s = assemble(b + 1);
-            if (s) break;
-        }
-        HIPCHK(c, hipStreamWaitEvent(c->st, c->sw_asm[b & 1], 0));
-        const int f0 = nbat * b, nf = std::min(nbat, nfreq - f0), s0 = nbat * (b & 1);
-        s = bemx::lu_solve_batch(c->lu, c->st, (int)nn, nf, zh_of(s0), nn, zh_bytes, zfi_of(s0),
-                                 zfi_bytes, nr, mode, st4.data(), c->err);
-        if (s) break;
-        if (zfi_out)
-            HIPCHK(c, hipMemcpyAsync(zfi_out + 2 * nn * nr * f0, zfi_of(s0), zfi_bytes * nf,
-                                     hipMemcpyDeviceToHost, c->st));
-        for (int i = 0; i < nf; ++i) {
-            const int f = f0 + i;
-            if (stats) {
-                stats[4 * f] = st4[4 * i];
-                stats[4 * f + 1] = st4[4 * i + 1];
-                stats[4 * f + 2] = st4[4 * i + 2];
-                stats[4 * f + 3] = 0.0;
-            }
-            acc[0] += st4[4 * i];
-            acc[1] = std::max(acc[1], st4[4 * i + 1]);
-            acc[2] = std::max(acc[2], st4[4 * i + 2]);
-        }
-        HIPCHK(c, hipEventRecord(c->sw_done[b & 1], c->st));
+    for (int f = 0; f < nfreq; ++f) {
+        const double* st4 = items[f].stats;
+        if (stats)
+            for (int q = 0; q < 4; ++q) stats[4 * f + q] = st4[q];
+        acc[0] += st4[0];
+        acc[1] = std::max(acc[1], st4[1]);
+        acc[2] = std::max(acc[2], st4[2]);
     }
-    hipStreamSynchronize(c->st_asm);
-    if (s) {
-        hipStreamSynchronize(c->st);
-        return s;
-    }
-    HIPCHK(c, hipEventRecord(c->ev1, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
     float ms = 0.f;
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
     acc[3] = ms;

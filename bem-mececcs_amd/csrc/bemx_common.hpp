@@ -105,6 +105,19 @@ struct ElemData {
     double nx, ny, nz, cx, cy, cz, pad0, pad1;
 };
 
+// One problem of a batched solve on a context (ctx_solve_items): its
+// material (bemx_material's zge, zcs, zcp) and frequency, its load cases
+// (host, 3 NBE x nrhs interleaved complex; null: the context's), where its
+// solution goes (host, may be null) and, out, its statistics [GMRES
+// iterations, final ||r||/||b||, 1 if it fell back to FP64, 0].
+struct SolveItem {
+    double zge[2], zcs[2], zcp[2];
+    double fr;
+    const double* zdfi;
+    double* x;
+    double stats[4];
+};
+
 // Status bits written by kernels (OR-ed into a device word).
 enum : int {
     ST_DEGENERATE = 1,  // DET < 1e-5 at some Gauss point (Nonsingd.for:89-95)

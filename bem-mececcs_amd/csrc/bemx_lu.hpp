@@ -38,7 +38,7 @@ struct LuWork {
     // the matrix address, its leading dimension, the workspace generation,
     // the batch size and the batch's matrix stride; least recently used
     // entries are evicted beyond GRAPHS.
-    static constexpr int GRAPHS = 8, KEYS = 6;
+    static constexpr int GRAPHS = 32, KEYS = 6;
     struct GraphEntry {
         const void* key[KEYS] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
         hipGraphExec_t exec = nullptr;

@@ -4,8 +4,11 @@
 // load cases) are queued as they arrive.  NSLOTS worker threads each own a
 // device context (its own streams and grow-only workspace); a free worker
 // takes the oldest queued request as soon as the HBM it needs fits in what
-// the budget has left, runs assembly + factorisation + refinement for it
-// with bemx_solve and hands the solution back.  Requests that arrive while
+// the budget has left, together with up to batch - 1 more queued requests
+// that can share its launches (the same mesh and KODE and as many load
+// cases), runs assembly + factorisation + refinement for them as one batch
+// (ctx_solve_items; every launch of the mixed factorisation and refinement
+// covers the whole batch) and hands the solutions back.  Requests that arrive while
 // others are on the GPU are admitted as slots and memory free up, and the
 // work of different slots overlaps on the device: while one solve sits in
 // the latency-bound chain of its factorisation, another's bulk update fills
@@ -42,7 +45,9 @@
 #include "bemx_lu.hpp"
 
 namespace bemx {
-size_t ctx_solve_bytes_bound(int n, int nbe, int np, int npg, int nrhs, int mode);
+size_t ctx_solve_bytes_bound(int n, int nbe, int np, int npg, int nrhs, int mode, int nb);
+int ctx_solve_items(bemx_ctx* c, int nitem, SolveItem* items, int mode);
+int batch_size();
 size_t ctx_device_bytes(bemx_ctx* c);
 void ctx_release_workspace(bemx_ctx* c);
 void ctx_graph_counts(const bemx_ctx* c, long long* captures, long long* replays);
@@ -84,26 +89,36 @@ struct Request {
     std::vector<double> x;
     double stats[4] = {0, 0, 0, 0};
     int slot = -1;
+    int batch = 1;  // requests in the batch it was solved in
 };
 
 // Largest sizes a slot has served: its buffers hold (at most) the bound at
 // these sizes.
 struct SlotSizes {
-    int n = 0, nbe = 0, np = 0, npg = 0, nrhs = 0;
-    SlotSizes grown(const Request& r) const {
+    int n = 0, nbe = 0, np = 0, npg = 0, nrhs = 0, nb = 0;
+    SlotSizes grown(const Request& r, int batch) const {
         SlotSizes g;
         g.n = std::max(n, r.n);
         g.nbe = std::max(nbe, r.nbe);
         g.np = std::max(np, r.np);
         g.npg = std::max(npg, r.npg);
         g.nrhs = std::max(nrhs, r.nrhs);
+        g.nb = std::max(nb, batch);
         return g;
     }
 };
 
 size_t bound_of(const SlotSizes& z, int mode) {
     if (!z.nbe) return 0;
-    return bemx::ctx_solve_bytes_bound(z.n, z.nbe, z.np, z.npg, z.nrhs, mode);
+    return bemx::ctx_solve_bytes_bound(z.n, z.nbe, z.np, z.npg, z.nrhs, mode, z.nb);
+}
+
+// Whether request b can share the launches of a batch led by request a: the
+// same mesh and static pass, one KODE for every load case, the same KODE
+// and the same number of load cases (the load cases themselves may differ).
+bool batchable(const Request& a, const Request& b) {
+    return a.kode_cols == 1 && b.kode_cols == 1 && a.mesh_key == b.mesh_key &&
+           a.nrhs == b.nrhs && a.kode == b.kode;
 }
 
 struct Slot {
@@ -125,11 +140,12 @@ struct bemx_server {
     std::deque<Request*> queue;
     std::map<long long, std::unique_ptr<Request>> reqs;
     long long next_id = 1;
+    int batch = 1;  // most requests per batch
     bool stop = false;
     std::vector<std::thread> workers;
     std::vector<Slot> slots;
     // statistics
-    long long completed = 0, releases = 0, over_charge = 0;
+    long long completed = 0, releases = 0, over_charge = 0, batches = 0;
     long long captures = 0, replays = 0;
     std::vector<long long> slot_captures, slot_replays;
     std::string err;
@@ -138,10 +154,12 @@ struct bemx_server {
 namespace {
 
 size_t request_footprint(const Request& r, int mode) {
-    return bemx::ctx_solve_bytes_bound(r.n, r.nbe, r.np, r.npg, r.nrhs, mode);
+    return bemx::ctx_solve_bytes_bound(r.n, r.nbe, r.np, r.npg, r.nrhs, mode, 1);
 }
 
-void run_request(bemx_server* s, bemx_ctx* c, uint64_t* slot_key, Request* r) {
+// The mesh and static pass of request r on the slot's context (kept from
+// the last request when it is the same mesh).
+int ensure_mesh(bemx_ctx* c, uint64_t* slot_key, const Request* r) {
     int st = BEMX_OK;
     if (*slot_key != r->mesh_key) {
         *slot_key = 0;
@@ -153,6 +171,11 @@ void run_request(bemx_server* s, bemx_ctx* c, uint64_t* slot_key, Request* r) {
                                nullptr, nullptr, nullptr);
         if (!st) *slot_key = r->mesh_key;
     }
+    return st;
+}
+
+void run_request(bemx_server* s, bemx_ctx* c, uint64_t* slot_key, Request* r) {
+    int st = ensure_mesh(c, slot_key, r);
     double zge[2], zcs[2], zcp[2];
     if (!st) st = bemx_material(r->ge, r->rnu, r->dam, r->rho, zge, zcs, zcp);
     if (!st)
@@ -167,6 +190,34 @@ void run_request(bemx_server* s, bemx_ctx* c, uint64_t* slot_key, Request* r) {
     if (st) r->err = bemx_last_error(c);
 }
 
+// Requests that share one batch (batchable with the first): one upload of
+// the KODE, every request's load cases, and one batched solve.
+void run_batch(bemx_server* s, bemx_ctx* c, uint64_t* slot_key, std::vector<Request*>& rs) {
+    Request* r0 = rs[0];
+    int st = ensure_mesh(c, slot_key, r0);
+    if (!st) st = bemx_send_load_cases(c, r0->nrhs, r0->zdfi.data(), r0->kode.data());
+    std::vector<bemx::SolveItem> items(rs.size());
+    for (size_t i = 0; i < rs.size() && !st; ++i) {
+        Request* r = rs[i];
+        bemx::SolveItem& it = items[i];
+        st = bemx_material(r->ge, r->rnu, r->dam, r->rho, it.zge, it.zcs, it.zcp);
+        it.fr = r->fr;
+        it.zdfi = r->zdfi.data();
+        r->x.assign((size_t)6 * r->nbe * r->nrhs, 0.0);
+        it.x = r->x.data();
+    }
+    if (!st) st = bemx::ctx_solve_items(c, (int)rs.size(), items.data(), s->mode);
+    for (size_t i = 0; i < rs.size(); ++i) {
+        Request* r = rs[i];
+        r->status = st;
+        if (st) {
+            r->err = bemx_last_error(c);
+        } else {
+            for (int q = 0; q < 4; ++q) r->stats[q] = items[i].stats[q];
+        }
+    }
+}
+
 void worker_main(bemx_server* s, int slot) {
     bemx_ctx* c = s->slots[slot].ctx;
     uint64_t key = 0;
@@ -174,14 +225,21 @@ void worker_main(bemx_server* s, int slot) {
     for (;;) {
         Slot& me = s->slots[slot];
         // FIFO admission: the oldest request runs as soon as a slot is free
-        // and the slot's charge, grown to that request, fits in the budget
-        Request* r = nullptr;
+        // and the slot's charge, grown to that request (and to a batch when
+        // the request can lead one and the budget has room for it), fits in
+        // the budget
         bool give_back = false;
+        int nb = 1;
         s->cv_work.wait(lk, [&] {
             if (s->stop) return true;
             if (s->queue.empty()) return false;
-            const size_t need = bound_of(me.sizes.grown(*s->queue.front()), s->mode);
-            if (s->charged - me.charge + need <= s->budget) return true;
+            const Request& f = *s->queue.front();
+            nb = (s->batch > 1 && f.kode_cols == 1) ? s->batch : 1;
+            for (;; nb = 1) {
+                const size_t need = bound_of(me.sizes.grown(f, nb), s->mode);
+                if (s->charged - me.charge + need <= s->budget) return true;
+                if (nb == 1) break;
+            }
             give_back = me.charge > 0;  // the request waits for memory: return ours
             return give_back;
         });
@@ -198,18 +256,35 @@ void worker_main(bemx_server* s, int slot) {
             s->cv_work.notify_all();
             continue;
         }
-        r = s->queue.front();
+        std::vector<Request*> rs(1, s->queue.front());
         s->queue.pop_front();
-        me.sizes = me.sizes.grown(*r);
+        // the queued requests that can share its launches, oldest first
+        for (auto it = s->queue.begin(); it != s->queue.end() && (int)rs.size() < nb;) {
+            if (batchable(*rs[0], **it)) {
+                rs.push_back(*it);
+                it = s->queue.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        me.sizes = me.sizes.grown(*rs[0], (int)rs.size());
         const size_t need = bound_of(me.sizes, s->mode);
         s->charged = s->charged - me.charge + need;
         me.charge = need;
         s->peak_charged = std::max(s->peak_charged, s->charged);
         me.busy = true;
-        r->slot = slot;
-        r->t_start = clk::now();
+        const clk::time_point t_start = clk::now();
+        for (Request* r : rs) {
+            r->slot = slot;
+            r->t_start = t_start;
+            r->batch = (int)rs.size();
+        }
+        ++s->batches;
         lk.unlock();
-        run_request(s, c, &key, r);
+        if (rs.size() == 1)
+            run_request(s, c, &key, rs[0]);
+        else
+            run_batch(s, c, &key, rs);
         const size_t held = bemx::ctx_device_bytes(c);
         long long cap = 0, rep = 0;
         bemx::ctx_graph_counts(c, &cap, &rep);
@@ -220,9 +295,12 @@ void worker_main(bemx_server* s, int slot) {
         s->slot_captures[slot] = cap;
         s->slot_replays[slot] = rep;
         me.busy = false;
-        r->t_done = clk::now();
-        r->done = true;
-        ++s->completed;
+        const clk::time_point t_done = clk::now();
+        for (Request* r : rs) {
+            r->t_done = t_done;
+            r->done = true;
+            ++s->completed;
+        }
         s->cv_done.notify_all();
         s->cv_work.notify_all();
     }
@@ -256,7 +334,7 @@ int bemx_material(double ge, double rnu, double dam, double rho, double* zge, do
 double bemx_request_footprint(int nbe, int n, int np, int npg, int nrhs, int mode) {
     if (nbe <= 0 || n < nbe || np <= 0 || npg <= 0 || nrhs < 1) return 0.0;
     if (mode != BEMX_MODE_FP64 && mode != BEMX_MODE_MIXED) return 0.0;
-    return (double)bemx::ctx_solve_bytes_bound(n, nbe, np, npg, nrhs, mode);
+    return (double)bemx::ctx_solve_bytes_bound(n, nbe, np, npg, nrhs, mode, 1);
 }
 
 int bemx_server_create(int device, int nslots, double hbm_budget_bytes, int mode,
@@ -270,6 +348,7 @@ int bemx_server_create(int device, int nslots, double hbm_budget_bytes, int mode
     auto* s = new bemx_server();
     s->device = device;
     s->mode = mode;
+    s->batch = bemx::batch_size();
     s->budget = hbm_budget_bytes > 0 ? (size_t)hbm_budget_bytes : (size_t)(0.8 * (double)freeb);
     // the contexts (and with them the device code of the library) are made
     // on this thread, before any worker starts
@@ -370,6 +449,7 @@ int bemx_server_wait(bemx_server* s, long long id, double* x, double* info, char
         info[7] = r->stats[3];
         info[8] = ms_between(s->t0, r->t_start);
         info[9] = ms_between(s->t0, r->t_done);
+        info[10] = r->batch;
     }
     if (msg && msglen > 0) {
         strncpy(msg, r->err.c_str(), (size_t)msglen - 1);
@@ -380,9 +460,10 @@ int bemx_server_wait(bemx_server* s, long long id, double* x, double* info, char
 
 int bemx_server_stats(bemx_server* s, double* out, int nout) {
     if (!s || !out || nout < 1) return BEMX_ERR_ARG;
-    double v[8];
+    double v[9];
     {
         std::lock_guard<std::mutex> lk(s->mu);
+        v[8] = (double)s->batches;
         v[0] = (double)s->completed;
         v[1] = (double)s->captures;
         v[2] = (double)s->replays;
@@ -392,7 +473,7 @@ int bemx_server_stats(bemx_server* s, double* out, int nout) {
         v[6] = (double)s->releases;
         v[7] = (double)s->over_charge;
     }
-    for (int i = 0; i < nout && i < 8; ++i) out[i] = v[i];
+    for (int i = 0; i < nout && i < 9; ++i) out[i] = v[i];
     return BEMX_OK;
 }
 

@@ -8,9 +8,12 @@ frequencies log-spaced in [0.1, 10] rad/s.  A step is one frequency: the
 dynamic assembly of ZH and ZFI (FP64), the factorisation and the
 refinement, with the mesh, its static pass and the boundary data resident
 in HBM before the timed region.  The frequencies go through the request
-loop (bemx_server) on --slots slots; `value` is the mixed path's solves/s
-over the sweep and `fp64` the plain FP64 path's over the same sweep, every
-mixed result is compared with the FP64 result of its frequency.  Beside it:
+loop (bemx_server) on --slots slots, which takes the queued frequencies in
+batches (BEMX_SWEEP_BATCH, 4 by default) whose mixed factorisations and
+refinements share every launch; `value` is the mixed path's solves/s over
+the sweep and `fp64` the plain FP64 path's over the same sweep (the same
+loop and batched assembly, one FP64 LU per frequency), every mixed result
+is compared with the FP64 result of its frequency.  Beside it:
   single_solve  the same patch at omega = 2 on one context, mixed and FP64
                 interleaved: latency of one solve alone and the device times
                 of assembly, factorisation and refinement (HIP events)
@@ -38,9 +41,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "bem-mececcs_amd"))
 # Hardware queues of this process (read by the HIP runtime at its first call):
-# HIP maps every stream onto one of them, so with 4 request slots x 2
-# streams the default of 4 queues serialises unrelated slots.  Measured on
-# the sweep: 4 slots reach 108 solves/s with 4 queues and 123-124 with 8 or 16.
+# HIP maps every stream onto one of them, so with several request slots x 2
+# streams the default of 4 queues serialises unrelated slots.
 os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 import numpy as np  # noqa: E402
@@ -431,17 +433,18 @@ def workload_solve(args, api, decks):
 
 
 def sweep_engine(api, mesh, dyn, freqs, x64):
-    """The same sweep through bemx_solve_sweep on one context: frequencies
-    assembled two per launch on a second stream, beside the previous pair's
-    solves; warmed by a 4-frequency sweep (workspace, captured factorisation
-    graphs).  Every result is compared with the request loop's FP64 one."""
+    """The same sweep through bemx_solve_sweep on one context: batches of
+    frequencies assembled (two per launch) on a second stream beside the
+    previous batch's batched solve; warmed by an 8-frequency sweep
+    (workspace, captured factorisation graphs).  Every result is compared
+    with the request loop's FP64 one."""
     zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
     ctx = api.Context(0)
     try:
         ctx.send_shared_data(mesh)
         ctx.ghmatece(mesh.ge, mesh.rnu)
         ctx.send_dyn_data(dyn.bc.astype(np.complex128), dyn.kode)
-        ctx.solve_sweep(zmat, freqs[:4], api.MODE_MIXED)
+        ctx.solve_sweep(zmat, freqs[:8], api.MODE_MIXED)
         t0 = time.perf_counter()
         xe, st = ctx.solve_sweep(zmat, freqs, api.MODE_MIXED)
         wall = time.perf_counter() - t0
@@ -453,9 +456,9 @@ def sweep_engine(api, mesh, dyn, freqs, x64):
     return {"value": len(freqs) / wall, "unit": "solves/s", "wall_s": wall,
             "max_rel_err_vs_fp64": err, "gmres_iters_max": int(st[:, 0].max()),
             "fallbacks_to_fp64": int(st[:, 2].sum()),
-            "note": "bemx_solve_sweep on one context: two frequencies per assembly launch "
-                    "(k_dyn_offdiag_h2) on a second stream beside the previous pair's "
-                    "factorisations and refinements"}
+            "note": "bemx_solve_sweep on one context: batches of frequencies assembled two "
+                    "per launch (k_dyn_offdiag_h2) on a second stream beside the previous "
+                    "batch's factorisations and refinements, which share every launch"}
 
 
 def warm_main_thread(api, decks):
@@ -538,7 +541,7 @@ def main():
                     help="sweep frequencies timed (default: all 256); other workloads: solves")
     ap.add_argument("--warmup", type=int, default=2, help="untimed requests per slot")
     ap.add_argument("--workload", default="sweep", choices=["sweep", "solve", "serve", "large"])
-    ap.add_argument("--slots", type=int, default=4, help="request-loop slots")
+    ap.add_argument("--slots", type=int, default=2, help="request-loop slots")
     ap.add_argument("--single-steps", type=int, default=10,
                     help="single-solve latency steps beside the sweep")
     ap.add_argument("--requests", type=int, default=None,

@@ -149,13 +149,16 @@ int bemx_solve(bemx_ctx *ctx, const double *zge, const double *zcs, const double
  * factorisation (+ refinement in BEMX_MODE_MIXED) and solve of bemx_solve.
  * Replaces the frequency loop of the reference's main program
  * (Main.for:167-200: GHMATECD + LINSOLVE per frequency) for one material.
- * Frequencies are assembled two at a time, the H blocks of both in one
- * kernel launch (each frequency's matrix bit-identical to bemx_solve's), on a
- * second stream that overlaps the previous pair's solves.  zfi_out (may be
- * NULL): nfreq blocks of 3 NBE x nrhs complex solutions in frequency order;
- * stats (may be NULL): 4 doubles per frequency (GMRES iterations, final
- * relative residual, 1 if the FP64 fallback ran, 0).  Device memory: four
- * frequency slots (ZH, ZFI and assembly partials each). */
+ * Frequencies go in batches of up to BEMX_SWEEP_BATCH (environment, 1..8,
+ * default 4; fewer when device memory is short): a batch is assembled two
+ * frequencies per kernel launch on a second stream while the previous batch
+ * is solved, and in BEMX_MODE_MIXED the batch's factorisations and
+ * refinements share every launch.  Each frequency's solution is
+ * bit-identical to bemx_solve's.  zfi_out (may be NULL): nfreq blocks of
+ * 3 NBE x nrhs complex solutions in frequency order; stats (may be NULL): 4
+ * doubles per frequency (GMRES iterations, final relative residual, 1 if
+ * the FP64 fallback ran, 0).  Device memory: two batches of frequency slots
+ * (ZH, ZFI and assembly partials each) and the batch's mixed workspace. */
 int bemx_solve_sweep(bemx_ctx *ctx, const double *zge, const double *zcs, const double *zcp,
                      int nfreq, const double *freqs, int mode, double *zfi_out, double *stats);
 
@@ -184,7 +187,11 @@ int bemx_material(double ge, double rnu, double dam, double rho, double *zge, do
  * (Main.for).  nslots worker slots (1..16), each a context with its own
  * streams, take queued requests oldest first as soon as the request's HBM
  * workspace fits in hbm_budget_bytes (<= 0: 80% of the free HBM at
- * creation); their work overlaps on the device.  A slot is charged the
+ * creation); their work overlaps on the device.  A slot takes with the
+ * oldest request up to BEMX_SWEEP_BATCH - 1 more queued requests on the same
+ * mesh with the same KODE and number of load cases and solves them as one
+ * batch (as bemx_solve_sweep does; each result bit-identical to the request
+ * solved alone).  A slot is charged the
  * workspace bound (bemx_request_footprint) at the largest sizes it has
  * served, since its buffers only grow; idle slots give their workspace back
  * when the oldest request does not fit.  mode: BEMX_MODE_FP64 or BEMX_MODE_MIXED for
@@ -207,18 +214,19 @@ int bemx_server_submit(bemx_server *srv, const double *cx, const double *cy, con
 /* kode_cols: 1 when KODE(3NBE) holds for every load case, nrhs for
  * KODE(3NBE, nrhs) with a column per case (as bemx_send_load_cases_kode). */
 /* Blocks until request id is done; returns its status.  x (may be NULL):
- * 3NBE x nrhs complex solution.  info (may be NULL, 10 doubles): latency
- * ms (submit to done), queue ms, service ms, GMRES iterations, worst
- * relative residual, 1 if it fell back to FP64, slot, device ms of the
- * solve, start and done times in ms since the server was created.  msg
+ * 3NBE x nrhs complex solution.  info (may be NULL, 11 doubles): latency
+ * ms (submit to done), queue ms, service ms (of its batch), GMRES
+ * iterations, worst relative residual, 1 if it fell back to FP64, slot,
+ * device ms of the solve, start and done times in ms since the server was
+ * created, and the number of requests in its batch.  msg
  * (may be NULL) receives the error text.  Each id can be waited once. */
 int bemx_server_wait(bemx_server *srv, long long id, double *x, double *info, char *msg,
                      int msglen);
-/* Statistics of the request loop (nout <= 8 doubles): requests completed,
+/* Statistics of the request loop (nout <= 9 doubles): requests completed,
  * factorisation graphs captured, graph replays, bytes charged now, peak
- * bytes charged, the budget, workspaces given back to make room, and the
- * number of requests after which a slot held more device bytes than it was
- * charged (0 unless the bound is wrong). */
+ * bytes charged, the budget, workspaces given back to make room, the
+ * number of batches after which a slot held more device bytes than it was
+ * charged (0 unless the bound is wrong), and batches run. */
 int bemx_server_stats(bemx_server *srv, double *out, int nout);
 /* Stops the workers after their current request; requests still queued
  * are dropped.  Each worker releases its slot's workspace before it

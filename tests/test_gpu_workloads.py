@@ -241,8 +241,14 @@ def test_full_sweep_through_the_request_loop(sweep_setup):
     _say("sweep worst mixed vs fp64 %.2e" % max(errs), sm)
     assert max(errs) <= TOL_MIXED
     assert all(o[1]["fell_back"] == 0.0 for o in outm)
-    # one mesh, one size: one factorisation graph per slot
-    assert sm["graph_captures"] <= 2 and sm["graph_replays"] >= 254
+    # the loop batches the queued frequencies (one mesh, one KODE): every
+    # batch is one factorisation graph launch, and each slot captures one
+    # graph per batch size it meets (a full batch, and the smaller one that
+    # can end the sweep)
+    sizes = {int(o[1]["batch"]) for o in outm}
+    assert max(sizes) > 1, sizes
+    assert sm["graph_captures"] + sm["graph_replays"] == sm["batches"], sm
+    assert sm["graph_captures"] <= 2 * len(sizes), sm
     assert sm["over_charge"] == 0 and sm["peak_charged_bytes"] <= sm["budget_bytes"]
 
 
@@ -300,27 +306,31 @@ def test_serving_stream_alone_and_in_the_mix(stream, stream_alone):
 
 
 def test_serving_stream_one_graph_per_size_class(stream, stream_alone):
-    """One slot warmed on the largest class with 4 load cases: its workspace
-    then never grows, so the stream captures one factorisation graph per
-    other size class and replays it for every later request of that class."""
-    meshes, kodes, _, items = stream
+    """One slot warmed on the largest class with a full batch of 4 load
+    cases: its workspace then never grows, so the stream captures at most
+    one factorisation graph per (size class, batch size) and replays it for
+    every later batch of that kind; every batch is one graph launch."""
+    meshes, kodes, reqs, items = stream
     big = len(meshes) - 1
     z = np.zeros((3 * meshes[big].nbe, 4), np.complex128)
     z[np.arange(4), np.arange(4)] = 1.0
+    warm_item = (meshes[big], kodes[big], z, (meshes[big].ge, meshes[big].rnu, 0.01, 1.0), 1.0)
     srv = api.Server(0, nslots=1, mode=api.MODE_MIXED)
     try:
-        srv.wait(srv.submit(meshes[big], kodes[big], z,
-                            (meshes[big].ge, meshes[big].rnu, 0.01, 1.0), 1.0))
+        ids = [srv.submit(*warm_item) for _ in range(8)]
+        wb = {int(srv.wait(i)[1]["batch"]) for i in ids}
         warm = srv.stats()
-        assert warm["graph_captures"] == 1
         outs = [srv.wait(i) for i in [srv.submit(*it) for it in items]]
         st = srv.stats()
     finally:
         srv.close()
     for (x, _), ref in zip(outs, stream_alone):
         assert _rel(_shaped(x, ref), ref) <= TOL_MIXED
-    assert st["graph_captures"] - warm["graph_captures"] <= len(meshes) - 1, st
-    assert st["graph_replays"] >= N_STREAM - (len(meshes) - 1), st
+    kinds = {(r["cls"], int(o[1]["batch"])) for r, o in zip(reqs, outs)}
+    new_kinds = kinds - {(big, b) for b in wb}
+    _say("stream batches", sorted(kinds), "warm-up batch sizes", wb, st)
+    assert st["graph_captures"] + st["graph_replays"] == st["batches"], st
+    assert st["graph_captures"] - warm["graph_captures"] <= len(new_kinds), st
 
 
 def test_serving_stream_under_a_tight_budget(stream, stream_alone):
