@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -39,6 +40,7 @@ size_t exp_table_max_bytes();
 int exp_table_entries(const DynConst& K, double rmax, double* h);
 hipError_t launch_unswap(hipStream_t, int, const int*, cd*, cd*, cd);
 int interior_blocks(int nbe);
+int batch_size();
 hipError_t launch_interior(hipStream_t, int, int, int, int, const ElemData*, const GaussPt*,
                            const double*, const double*, const double*, const cd*, const cd*,
                            const DynConst&, cd*, cd*, cd*);
@@ -265,6 +267,7 @@ size_t ctx_solve_bytes_bound(int n, int nbe, int np, int npg, int nrhs, int mode
          r(sizeof(cd) * nn * nn) + r(exp_table_max_bytes());
     b += fp64_factor_bytes((int)nn);
     nb = std::max(1, nb);
+    const int nar = std::min(nb, batch_size());  // arenas of the mixed workspace
     if (nb > 1) {
         // the batch's frequency slots: matrices and right-hand sides in one
         // buffer each, partials, exponential tables and load cases per slot
@@ -272,7 +275,7 @@ size_t ctx_solve_bytes_bound(int n, int nbe, int np, int npg, int nrhs, int mode
         b += nb * (r(sizeof(cd) * nn * offdiag_tiles(nbe) * nrhs) + r(sizeof(cd) * nn * nrhs) +
                    r(exp_table_max_bytes()) + r(sizeof(cd) * nn * nrhs));
     }
-    if (mode == BEMX_MODE_MIXED) b += mixed_workspace_bytes((int)nn, nb);
+    if (mode == BEMX_MODE_MIXED) b += mixed_workspace_bytes((int)nn, nar);
     return b;
 }
 
@@ -843,8 +846,13 @@ int batch_size() { return sweep_batch(); }
 // assembled (two problems per launch where they share their load cases) on
 // the context's second stream into one half of the frequency slots while
 // the previous batch is solved on the context's stream as one batch
-// (lu_solve_batch).  Each problem's arithmetic is that of bemx_solve.
-int ctx_solve_items(bemx_ctx* c, int nitem, SolveItem* items, int mode) {
+// (lu_solve_batch); the next batch's assembly is enqueued right behind the
+// current batch's factorisation.  Each problem's arithmetic is that of
+// bemx_solve.
+// batch_done (may be null) is called with (first item, count) once a
+// batch's solutions are in their host arrays.
+int ctx_solve_items(bemx_ctx* c, int nitem, SolveItem* items, int mode,
+                    const std::function<void(int, int)>* batch_done) {
     if (nitem < 1) return BEMX_OK;
     HIPCHK(c, hipSetDevice(c->device));
     const int nbe = c->nbe, npg2 = c->npg * c->npg, nr = c->nrhs;
@@ -945,17 +953,17 @@ int ctx_solve_items(bemx_ctx* c, int nitem, SolveItem* items, int mode) {
     HIPCHK(c, hipStreamWaitEvent(c->st_asm, c->ev0, 0));  // after the context's earlier work
     int s = assemble(0);
     for (int b = 0; b < nb && !s; ++b) {
-        // the next batch's assembly beside this batch's solves, once the
-        // solves that last used its slots are done
-        if (b + 1 < nb) {
-            if (b >= 1) HIPCHK(c, hipStreamWaitEvent(c->st_asm, c->sw_done[(b + 1) & 1], 0));
-            s = assemble(b + 1);
-            if (s) break;
-        }
         HIPCHK(c, hipStreamWaitEvent(c->st, c->sw_asm[b & 1], 0));
+        // the next batch's assembly beside this batch's solves, after the
+        // solves that last used its slots
+        const std::function<int()> next = [&]() -> int {
+            if (b + 1 >= nb) return BEMX_OK;
+            if (b >= 1) HIPCHK(c, hipStreamWaitEvent(c->st_asm, c->sw_done[(b + 1) & 1], 0));
+            return assemble(b + 1);
+        };
         const int f0 = nbat * b, nf = std::min(nbat, nitem - f0), s0 = nbat * (b & 1);
         s = lu_solve_batch(c->lu, c->st, (int)nn, nf, zh_of(s0), nn, zh_bytes, zfi_of(s0),
-                           zfi_bytes, nr, mode, st4.data(), c->err);
+                           zfi_bytes, nr, mode, st4.data(), c->err, &next);
         if (s) break;
         for (int i = 0; i < nf; ++i) {
             SolveItem& it = items[f0 + i];
@@ -966,6 +974,10 @@ int ctx_solve_items(bemx_ctx* c, int nitem, SolveItem* items, int mode) {
             it.stats[3] = 0.0;
         }
         HIPCHK(c, hipEventRecord(c->sw_done[b & 1], c->st));
+        if (batch_done) {
+            HIPCHK(c, hipStreamSynchronize(c->st));
+            (*batch_done)(f0, nf);
+        }
     }
     hipStreamSynchronize(c->st_asm);
     if (s) {
@@ -1008,7 +1020,7 @@ int bemx_solve_sweep(bemx_ctx* c, const double* zge, const double* zcs, const do
         it.zdfi = nullptr;
         it.x = zfi_out ? zfi_out + 2 * nn * c->nrhs * f : nullptr;
     }
-    const int s = bemx::ctx_solve_items(c, nfreq, items.data(), mode);
+    const int s = bemx::ctx_solve_items(c, nfreq, items.data(), mode, nullptr);
     if (s) return s;
     double acc[NLAST] = {0};
     for (int f = 0; f < nfreq; ++f) {

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 
 #include "bemx_common.hpp"
@@ -34,10 +35,9 @@ struct LuWork {
     hipEvent_t ev_f0 = nullptr, ev_f1 = nullptr;
     bool factor_timed = false;
     hipError_t ensure_side();
-    // Captured factorisation graphs, one per size class: keyed by the order,
-    // the matrix address, its leading dimension, the workspace generation,
-    // the batch size and the batch's matrix stride; least recently used
-    // entries are evicted beyond GRAPHS.
+    // Captured factorisation graphs, one per size class and batch size:
+    // keyed by the order, the workspace generation and the batch size (and
+    // unused zeros); least recently used entries are evicted beyond GRAPHS.
     static constexpr int GRAPHS = 32, KEYS = 6;
     struct GraphEntry {
         const void* key[KEYS] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -82,15 +82,19 @@ constexpr int MIXED_MAX_BATCH = 8;
 // stalls), nsolved[z] leading columns solved, stats[4 z ..] = [GMRES
 // iterations, worst final ||r||/||b||, 0, 0].  The return value reports a
 // failure of the whole batch (HIP error, timed-out device wait).
+// after (may be null) runs on the host once the factorisation is enqueued,
+// before anything waits for it (a caller's work for other streams, such as
+// the next batch's assembly, goes in behind the factorisation's launches).
 int gmres_ir_batch(LuWork& w, hipStream_t st, int n, int nz, const cd* A, size_t lda, size_t sa,
                    cd* b, size_t sb, int nrhs, int* code, int* nsolved, double* stats,
-                   std::string& err);
+                   std::string& err, const std::function<int()>* after = nullptr);
 
 // lu_solve for a batch: the mixed solve of every problem (gmres_ir_batch),
 // then the plain FP64 LU for each problem whose mixed solve failed
 // (stats[4 z + 2] = 1).  mode BEMX_MODE_FP64 solves the problems one by one.
 int lu_solve_batch(LuWork& w, hipStream_t st, int n, int nz, cd* A, size_t lda, size_t sa, cd* b,
-                   size_t sb, int nrhs, int mode, double* stats, std::string& err);
+                   size_t sb, int nrhs, int mode, double* stats, std::string& err,
+                   const std::function<int()>* after = nullptr);
 
 int lu_solve(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, cd* b, int nrhs, int mode,
              long long* info, double stats[4], std::string& err);

@@ -32,8 +32,10 @@
 #include <complex>
 #include <condition_variable>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -46,7 +48,8 @@
 
 namespace bemx {
 size_t ctx_solve_bytes_bound(int n, int nbe, int np, int npg, int nrhs, int mode, int nb);
-int ctx_solve_items(bemx_ctx* c, int nitem, SolveItem* items, int mode);
+int ctx_solve_items(bemx_ctx* c, int nitem, SolveItem* items, int mode,
+                    const std::function<void(int, int)>* batch_done);
 int batch_size();
 size_t ctx_device_bytes(bemx_ctx* c);
 void ctx_release_workspace(bemx_ctx* c);
@@ -190,9 +193,15 @@ void run_request(bemx_server* s, bemx_ctx* c, uint64_t* slot_key, Request* r) {
     if (st) r->err = bemx_last_error(c);
 }
 
-// Requests that share one batch (batchable with the first): one upload of
-// the KODE, every request's load cases, and one batched solve.
-void run_batch(bemx_server* s, bemx_ctx* c, uint64_t* slot_key, std::vector<Request*>& rs) {
+void finish(bemx_server* s, Request* r);
+void count_graphs(bemx_server* s, bemx_ctx* c, int slot);
+
+// Requests batchable with the first: one upload of the KODE, every
+// request's load cases, and the batched solves (up to two batches, the
+// second one's assembly beside the first one's solve); each batch's
+// requests are handed back as soon as it is done.
+void run_batch(bemx_server* s, int slot, bemx_ctx* c, uint64_t* slot_key,
+               std::vector<Request*>& rs) {
     Request* r0 = rs[0];
     int st = ensure_mesh(c, slot_key, r0);
     if (!st) st = bemx_send_load_cases(c, r0->nrhs, r0->zdfi.data(), r0->kode.data());
@@ -206,16 +215,63 @@ void run_batch(bemx_server* s, bemx_ctx* c, uint64_t* slot_key, std::vector<Requ
         r->x.assign((size_t)6 * r->nbe * r->nrhs, 0.0);
         it.x = r->x.data();
     }
-    if (!st) st = bemx::ctx_solve_items(c, (int)rs.size(), items.data(), s->mode);
-    for (size_t i = 0; i < rs.size(); ++i) {
-        Request* r = rs[i];
-        r->status = st;
-        if (st) {
-            r->err = bemx_last_error(c);
-        } else {
-            for (int q = 0; q < 4; ++q) r->stats[q] = items[i].stats[q];
+    // a request handed back belongs to its waiter, who may free it: it is
+    // not touched again (finished[] is this thread's record)
+    std::vector<char> finished(rs.size(), 0);
+    const std::function<void(int, int)> done = [&](int f0, int nf) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        ++s->batches;
+        count_graphs(s, c, slot);  // before the batch is handed back
+        const clk::time_point now = clk::now();
+        for (int i = f0; i < f0 + nf; ++i) {
+            for (int q = 0; q < 4; ++q) rs[i]->stats[q] = items[i].stats[q];
+            rs[i]->status = BEMX_OK;
+            rs[i]->batch = nf;
+            finished[i] = 1;
+            finish(s, rs[i]);
         }
+        // the next batch's solve starts now: its service time runs from here
+        for (size_t i = f0 + nf; i < rs.size() && i < (size_t)(f0 + 2 * nf); ++i)
+            rs[i]->t_start = now;
+    };
+    if (!st) st = bemx::ctx_solve_items(c, (int)rs.size(), items.data(), s->mode, &done);
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (size_t i = 0; i < rs.size(); ++i) {
+        if (finished[i]) continue;
+        rs[i]->status = st ? st : BEMX_ERR_STATE;
+        rs[i]->err = st ? bemx_last_error(c) : "request left unsolved by its batch";
+        finish(s, rs[i]);
     }
+}
+
+// Adds the graph captures and replays of the slot's context since the last
+// count to the server's statistics (the server's lock held).
+void count_graphs(bemx_server* s, bemx_ctx* c, int slot) {
+    long long cap = 0, rep = 0;
+    bemx::ctx_graph_counts(c, &cap, &rep);
+    s->captures += cap - s->slot_captures[slot];
+    s->replays += rep - s->slot_replays[slot];
+    s->slot_captures[slot] = cap;
+    s->slot_replays[slot] = rep;
+}
+
+// Hands request r back to its waiter (the server's lock held).
+void finish(bemx_server* s, Request* r) {
+    r->t_done = clk::now();
+    r->done = true;
+    ++s->completed;
+    s->cv_done.notify_all();
+}
+
+// Most batches one worker takes at once of requests batchable with the
+// oldest one (BEMX_TAKE_BATCHES, 1..64; default 16).
+int take_batches() {
+    static const int t = [] {
+        const char* v = getenv("BEMX_TAKE_BATCHES");
+        const int x = v ? atoi(v) : 16;
+        return std::min(64, std::max(1, x));
+    }();
+    return t;
 }
 
 void worker_main(bemx_server* s, int slot) {
@@ -229,12 +285,12 @@ void worker_main(bemx_server* s, int slot) {
         // the request can lead one and the budget has room for it), fits in
         // the budget
         bool give_back = false;
-        int nb = 1;
+        int nb = 1;  // device slots for the requests taken (two batches)
         s->cv_work.wait(lk, [&] {
             if (s->stop) return true;
             if (s->queue.empty()) return false;
             const Request& f = *s->queue.front();
-            nb = (s->batch > 1 && f.kode_cols == 1) ? s->batch : 1;
+            nb = (s->batch > 1 && f.kode_cols == 1) ? 2 * s->batch : 1;
             for (;; nb = 1) {
                 const size_t need = bound_of(me.sizes.grown(f, nb), s->mode);
                 if (s->charged - me.charge + need <= s->budget) return true;
@@ -258,8 +314,12 @@ void worker_main(bemx_server* s, int slot) {
         }
         std::vector<Request*> rs(1, s->queue.front());
         s->queue.pop_front();
-        // the queued requests that can share its launches, oldest first
-        for (auto it = s->queue.begin(); it != s->queue.end() && (int)rs.size() < nb;) {
+        // the queued requests that can share its launches, oldest first: up
+        // to take_batches() batches, solved in turn through two batches of
+        // device slots (each batch's assembly beside the previous batch's
+        // solve) and handed back batch by batch
+        const int take = nb > 1 ? take_batches() * s->batch : 1;
+        for (auto it = s->queue.begin(); it != s->queue.end() && (int)rs.size() < take;) {
             if (batchable(*rs[0], **it)) {
                 rs.push_back(*it);
                 it = s->queue.erase(it);
@@ -267,7 +327,7 @@ void worker_main(bemx_server* s, int slot) {
                 ++it;
             }
         }
-        me.sizes = me.sizes.grown(*rs[0], (int)rs.size());
+        me.sizes = me.sizes.grown(*rs[0], std::min(nb, (int)rs.size()));
         const size_t need = bound_of(me.sizes, s->mode);
         s->charged = s->charged - me.charge + need;
         me.charge = need;
@@ -277,31 +337,19 @@ void worker_main(bemx_server* s, int slot) {
         for (Request* r : rs) {
             r->slot = slot;
             r->t_start = t_start;
-            r->batch = (int)rs.size();
         }
-        ++s->batches;
+        if (rs.size() == 1) ++s->batches;
         lk.unlock();
         if (rs.size() == 1)
             run_request(s, c, &key, rs[0]);
         else
-            run_batch(s, c, &key, rs);
+            run_batch(s, slot, c, &key, rs);
         const size_t held = bemx::ctx_device_bytes(c);
-        long long cap = 0, rep = 0;
-        bemx::ctx_graph_counts(c, &cap, &rep);
         lk.lock();
         if (held > me.charge) ++s->over_charge;  // the bound was too small
-        s->captures += cap - s->slot_captures[slot];
-        s->replays += rep - s->slot_replays[slot];
-        s->slot_captures[slot] = cap;
-        s->slot_replays[slot] = rep;
+        count_graphs(s, c, slot);
         me.busy = false;
-        const clk::time_point t_done = clk::now();
-        for (Request* r : rs) {
-            r->t_done = t_done;
-            r->done = true;
-            ++s->completed;
-        }
-        s->cv_done.notify_all();
+        if (rs.size() == 1) finish(s, rs[0]);  // run_batch hands back its own
         s->cv_work.notify_all();
     }
     lk.unlock();

@@ -1174,7 +1174,8 @@ int lu_solve(LuWork& w, hipStream_t st, int n, cd* A, size_t lda, cd* b, int nrh
 }
 
 int lu_solve_batch(LuWork& w, hipStream_t st, int n, int nz, cd* A, size_t lda, size_t sa, cd* b,
-                   size_t sb, int nrhs, int mode, double* stats, std::string& err) {
+                   size_t sb, int nrhs, int mode, double* stats, std::string& err,
+                   const std::function<int()>* after) {
     if (nz < 1 || nz > MIXED_MAX_BATCH || nrhs < 1) {
         err = "LU: batch of 1..8 problems with at least one right-hand side";
         return BEMX_ERR_ARG;
@@ -1183,6 +1184,10 @@ int lu_solve_batch(LuWork& w, hipStream_t st, int n, int nz, cd* A, size_t lda, 
     auto mat = [&](int z) { return (cd*)((char*)A + (size_t)z * sa); };
     auto rhs = [&](int z) { return (cd*)((char*)b + (size_t)z * sb); };
     if (mode != BEMX_MODE_MIXED) {
+        if (after) {
+            const int s = (*after)();
+            if (s) return s;
+        }
         for (int z = 0; z < nz; ++z) {
             long long info = 0;
             const int s = lu_solve(w, st, n, mat(z), lda, rhs(z), nrhs, mode, &info, stats + 4 * z,
@@ -1193,7 +1198,8 @@ int lu_solve_batch(LuWork& w, hipStream_t st, int n, int nz, cd* A, size_t lda, 
     }
     w.factor_timed = false;
     int code[MIXED_MAX_BATCH], nsolved[MIXED_MAX_BATCH];
-    int s = gmres_ir_batch(w, st, n, nz, A, lda, sa, b, sb, nrhs, code, nsolved, stats, err);
+    int s = gmres_ir_batch(w, st, n, nz, A, lda, sa, b, sb, nrhs, code, nsolved, stats, err,
+                           after);
     if (s) return s;
     for (int z = 0; z < nz; ++z) {
         if (code[z] == BEMX_OK) continue;

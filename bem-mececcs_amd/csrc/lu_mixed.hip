@@ -1742,6 +1742,7 @@ static int group_blocks() {
     return g;
 }
 
+
 // Offsets of the buffers in one problem's arena for order n (each 256-byte
 // aligned) and the arena's size (a multiple of 64 KB).
 struct ArenaLayout {
@@ -1789,9 +1790,19 @@ hipError_t preload_mixed_code() {
     return hipFuncGetAttributes(&a, (const void*)k_trsv_coop);
 }
 
+// The FP32 copies of the batch's matrices (matrix z at A64 + z sa bytes),
+// launched before the factorisation's graph, which then depends on the
+// workspace only.
+static void enqueue_convert(hipStream_t st, int n, const cd* A64, size_t lda, size_t sa,
+                            const MixedFactor& F) {
+    const size_t tot = (size_t)n * n;
+    hipLaunchKernelGGL(k_cvt_z2c, dim3((tot + 255) / 256, 1, F.nz), dim3(256), 0, st, A64, lda,
+                       F.A, F.ld, n, sa, F.bs);
+}
+
 // The launch sequence of the factorisation (no host synchronisation, no
 // allocation), so it can be captured into a graph; every launch covers the
-// F.nz problems of the batch (matrix z at A64 + z sa bytes).
+// F.nz problems of the batch, whose FP32 matrices are in place.
 //
 // Block LU in the form A = L U with unit block-lower L (L21 = A21 A11^-1 per
 // step) and block-upper U whose diagonal blocks are the Schur-complement
@@ -1818,9 +1829,8 @@ hipError_t preload_mixed_code() {
 // so group g+1 is factored while the bulk update of group g runs.  Planes
 // are double-buffered by group parity: group g+1 writes the slots Z(g-1) read,
 // after the wait for Z(g-1).
-static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda,
-                                 size_t sa, const MixedFactor& F, __bf16* planes,
-                                 size_t pstride) {
+static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const MixedFactor& F,
+                                 __bf16* planes, size_t pstride) {
     hipError_t first = hipSuccess;
     auto chk = [&](hipError_t e) {
         if (first == hipSuccess && e != hipSuccess) first = e;
@@ -1834,10 +1844,7 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
     // k_cgemm_update_inverse
     hipLaunchKernelGGL(k_fill_i, dim3((nblk * GJ_PANELS + 255) / 256, 1, nz), dim3(256), 0, st,
                        F.gjflag, 0, nblk * GJ_PANELS, bs);
-    size_t tot = (size_t)n * n;
     const size_t ld = F.ld;
-    hipLaunchKernelGGL(k_cvt_z2c, dim3((tot + 255) / 256, 1, nz), dim3(256), 0, st, A64, lda, F.A,
-                       ld, n, sa, bs);
     auto at = [&](int r, int c) { return F.A + (size_t)c * ld + r; };
     const UpdRegion none{0, 0, 0, 0, nullptr};
     const hipStream_t sd = w.side;
@@ -1909,7 +1916,8 @@ static hipError_t enqueue_factor(LuWork& w, hipStream_t st, int n, const cd* A64
 // The factorisation of the nz problems of a batch.  fcode[z]: BEMX_OK or
 // BEMX_ERR_SINGULAR (a zero pivot in one of problem z's diagonal blocks).
 static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t lda, size_t sa,
-                        int nz, MixedFactor& F, int* fcode, std::string& err) {
+                        int nz, MixedFactor& F, int* fcode, std::string& err,
+                        const std::function<int()>* after) {
     hipError_t e;
     const int nblk = (n + NBM - 1) / NBM;
     F.n = n;
@@ -1968,20 +1976,20 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     // update of group g still reads the other)
     const size_t pstride = (size_t)n * NBM;
     __bf16* planes = (__bf16*)(ar + L.off[A_PLANES]);
-    // The launch sequence depends only on n, the matrices, the batch and the
-    // workspace buffers: it is captured once per size class into a graph and
-    // replayed while the workspace keeps its buffers (every reallocation
-    // bumps w.gen and drops the cached graphs).
+    // The launch sequence after the FP32 copies depends only on n, the batch
+    // size and the workspace buffers: it is captured once per size class and
+    // batch size into a graph and replayed while the workspace keeps its
+    // buffers (every reallocation bumps w.gen and drops the cached graphs).
     MX_CHK(hipEventRecord(w.ev_f0, st));
+    enqueue_convert(st, n, A64, lda, sa, F);
     if (w.use_graphs) {
-        const void* key[LuWork::KEYS] = {(const void*)(size_t)n, A64, (const void*)lda,
-                                         (const void*)(size_t)w.gen, (const void*)(size_t)nz,
-                                         (const void*)sa};
+        const void* key[LuWork::KEYS] = {(const void*)(size_t)n, (const void*)(size_t)w.gen,
+                                         (const void*)(size_t)nz, nullptr, nullptr, nullptr};
         hipGraphExec_t ex = w.find_graph(key);
         if (!ex) {
             hipGraph_t g = nullptr;
             MX_CHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            const hipError_t e1 = enqueue_factor(w, st, n, A64, lda, sa, F, planes, pstride);
+            const hipError_t e1 = enqueue_factor(w, st, n, F, planes, pstride);
             const hipError_t e2 = hipStreamEndCapture(st, &g);
             if (e1 != hipSuccess || e2 != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
@@ -1997,13 +2005,17 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
         }
         MX_CHK(hipGraphLaunch(ex, st));
     } else {
-        MX_CHK(enqueue_factor(w, st, n, A64, lda, sa, F, planes, pstride));
+        MX_CHK(enqueue_factor(w, st, n, F, planes, pstride));
     }
     MX_CHK(hipEventRecord(w.ev_f1, st));
     w.factor_timed = true;
     hipLaunchKernelGGL(k_gather_heads, dim3(1, 1, nz), dim3(64), 0, st, (const GmState*)nullptr,
                        (const int*)F.status, F.bs, F.rrec);
     MX_CHK(hipGetLastError());
+    if (after) {
+        const int s = (*after)();
+        if (s) return s;
+    }
     ReadRec hr[MIXED_MAX_BATCH];
     MX_CHK(hipMemcpyAsync(hr, F.rrec, sizeof(ReadRec) * nz, hipMemcpyDeviceToHost, st));
     MX_CHK(hipStreamSynchronize(st));
@@ -2209,14 +2221,14 @@ static int refine_batch(LuWork& w, hipStream_t st, const MixedFactor& F, int n, 
 
 int gmres_ir_batch(LuWork& w, hipStream_t st, int n, int nz, const cd* A64, size_t lda, size_t sa,
                    cd* b, size_t sb, int nrhs, int* code, int* nsolved, double* stats,
-                   std::string& err) {
+                   std::string& err, const std::function<int()>* after) {
     if (nz < 1 || nz > MIXED_MAX_BATCH || n < 1 || nrhs < 1) {
         err = "mixed solve: batch of 1..8 problems, order and right-hand sides at least 1";
         return BEMX_ERR_ARG;
     }
     MixedFactor F;
     int fc[MIXED_MAX_BATCH];
-    int s = factor_mixed(w, st, n, A64, lda, sa, nz, F, fc, err);
+    int s = factor_mixed(w, st, n, A64, lda, sa, nz, F, fc, err, after);
     if (s) return s;
     unsigned act = 0;
     for (int z = 0; z < nz; ++z) {
