@@ -1280,7 +1280,11 @@ constexpr int GJ_PANELS = NBM / 8;  // panels of gj_factor_body<8>
 // step on), so the inverse is done shortly after the elimination.  The waits
 // cannot block workgroup 0: at most XINV_WGS workgroups wait, far fewer
 // than the CUs.  The next block's elimination and inversion thus run beside
-// the update instead of after it.
+// the update instead of after it.  Over a batch of nz problems the grid is
+// one-dimensional in role order -- the nz eliminations, then the nz
+// XINV_WGS inverse builders, then each problem's ntile tile workgroups --
+// since workgroups are dispatched in order: every problem's elimination
+// starts at once instead of behind the tiles of the problems before it.
 __global__ __launch_bounds__(512) void k_cgemm_update_inverse(UpdRegion r0, UpdRegion r1,
                                                               const __bf16* __restrict__ base,
                                                               size_t pstride, size_t sstride,
@@ -1291,13 +1295,27 @@ __global__ __launch_bounds__(512) void k_cgemm_update_inverse(UpdRegion r0, UpdR
                                                               int* __restrict__ gpan,
                                                               cf* __restrict__ Xinv1,
                                                               int* __restrict__ status,
-                                                              size_t bs) {
+                                                              size_t bs, int nz) {
     __shared__ __attribute__((aligned(16))) __bf16 lds_a[2 * 4 * PLN];
     __shared__ __attribute__((aligned(16))) __bf16 lds_b[2 * 4 * PLN];
     static_assert(sizeof(lds_a) >= sizeof(cf) * (NBM + 1) * NBM + sizeof(int) * (NBM + 1),
                   "elimination scratch fits the tile buffer");
+    // role order: problem z and its workgroup number bid (0: elimination,
+    // 1 .. ntile: tiles, ntile + 1 ..: inverse builders)
+    int z, bid;
     {
-        const int z = blockIdx.z;
+        const int L = blockIdx.x;
+        if (L < nz) {
+            z = L;
+            bid = 0;
+        } else if (L < nz * (1 + XINV_WGS)) {
+            z = (L - nz) / XINV_WGS;
+            bid = ntile + 1 + (L - nz) % XINV_WGS;
+        } else {
+            const int q = L - nz * (1 + XINV_WGS);
+            z = q / ntile;
+            bid = 1 + q % ntile;
+        }
         r0.C = zp(r0.C, bs, z);
         r1.C = zp(r1.C, bs, z);
         base = zp(base, bs, z);
@@ -1306,7 +1324,6 @@ __global__ __launch_bounds__(512) void k_cgemm_update_inverse(UpdRegion r0, UpdR
         Xinv1 = zp(Xinv1, bs, z);
         status = zp(status, bs, z);
     }
-    const int bid = blockIdx.x;
     if (bid == 0) {
         if (do_corner) {
             const UpdRegion corner{nb1, nb1, r0.r0, r0.c0, r0.C};
@@ -1375,8 +1392,12 @@ static void launch_update(hipStream_t s, const UpdRegion& a, const UpdRegion& b,
     const int tiles = upd2_tiles_host(a) + upd2_tiles_host(b);
     if (tiles == 0) return;
     int g = ((tiles + NXCD - 1) / NXCD) * NXCD;
-    // a cap only for updates of up to 16 tiles per capped workgroup: a
-    // larger one is the bottleneck itself and gets the whole device
+    // a cap only for updates of up to 16 tiles per capped workgroup (of a
+    // problem; a batch's problems share the cap's workgroups): a larger one
+    // is the bottleneck itself and gets the whole device.  In a batch of 4
+    // on the 6480-DOF patch the early bulk updates are over that bound;
+    // capping them anyway (192, 128, 64 workgroups) was measured 3%, 8% and
+    // 15% slower.
     const int capz = cap > 0 ? std::max(NXCD, ((cap / nz + NXCD - 1) / NXCD) * NXCD) : 0;
     if (capz > 0 && tiles <= 16 * capz) g = std::min(g, capz);
     // 4 column tiles per panel of the tile order (2, 8 and 16 measured the
@@ -1394,9 +1415,9 @@ static void launch_update_inverse(hipStream_t s, const UpdRegion& a, const UpdRe
                                   cf* Xinv1, int* status, int nz, size_t bs) {
     const int tiles = upd2_tiles_host(a) + upd2_tiles_host(b);
     const int g = ((tiles + NXCD - 1) / NXCD) * NXCD;
-    hipLaunchKernelGGL(k_cgemm_update_inverse, dim3(1 + g + XINV_WGS, 1, nz), dim3(512), 0, s, a,
+    hipLaunchKernelGGL(k_cgemm_update_inverse, dim3(nz * (1 + g + XINV_WGS)), dim3(512), 0, s, a,
                        b, base, pstride, sstride, nks, ldc, 4, g, nb1, (int)do_corner, rec1, gpan,
-                       Xinv1, status, bs);
+                       Xinv1, status, bs, nz);
 }
 
 // Workgroups of the bulk update on the side stream (BEMX_BULK_WGS; default
