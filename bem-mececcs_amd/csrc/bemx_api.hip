@@ -108,6 +108,7 @@ struct bemx_ctx {
     DevBuf sw_zh, sw_zfi, sw_rhs_part[NSW], sw_rhs_diag[NSW], sw_etab[NSW], sw_zdfi[NSW];
     hipStream_t st_asm = nullptr;
     hipEvent_t sw_asm[2] = {nullptr, nullptr}, sw_done[2] = {nullptr, nullptr};
+    double sw_factor_ms = 0.0;  // device time of the last batched solves' factorisations
     // solver
     bemx::LuWork lu;
     // ev0..ev1: whole solve; ev2..ev3: off-diagonal assembly kernel;
@@ -826,12 +827,13 @@ int bemx_solve(bemx_ctx* c, const double* zge, const double* zcs, const double* 
 
 }  // extern "C"
 
-// Frequencies per batch of the batched solves (BEMX_SWEEP_BATCH, 1..8;
-// default 4).
+// Problems per batch of the batched solves (BEMX_SWEEP_BATCH, 1..8; default
+// 8).  Measured on the 6480-DOF sweep, one context: 118, 141 and 152
+// solves/s with batches of 2, 4 and 8.
 static int sweep_batch() {
     static const int b = [] {
         const char* v = getenv("BEMX_SWEEP_BATCH");
-        const int x = v ? atoi(v) : 4;
+        const int x = v ? atoi(v) : 8;
         return std::min(bemx::MIXED_MAX_BATCH, std::max(1, x));
     }();
     return b;
@@ -949,6 +951,7 @@ int ctx_solve_items(bemx_ctx* c, int nitem, SolveItem* items, int mode,
     };
     const int nb = (nitem + nbat - 1) / nbat;
     std::vector<double> st4(4 * nbat);
+    c->sw_factor_ms = 0.0;
     HIPCHK(c, hipEventRecord(c->ev0, c->st));
     HIPCHK(c, hipStreamWaitEvent(c->st_asm, c->ev0, 0));  // after the context's earlier work
     int s = assemble(0);
@@ -962,9 +965,15 @@ int ctx_solve_items(bemx_ctx* c, int nitem, SolveItem* items, int mode,
             return assemble(b + 1);
         };
         const int f0 = nbat * b, nf = std::min(nbat, nitem - f0), s0 = nbat * (b & 1);
+        c->lu.factor_timed = false;
         s = lu_solve_batch(c->lu, c->st, (int)nn, nf, zh_of(s0), nn, zh_bytes, zfi_of(s0),
                            zfi_bytes, nr, mode, st4.data(), c->err, &next);
         if (s) break;
+        if (c->lu.factor_timed) {  // the batch's factorisation, waited for by its refinement
+            float ms_f = 0.f;
+            if (hipEventElapsedTime(&ms_f, c->lu.ev_f0, c->lu.ev_f1) == hipSuccess)
+                c->sw_factor_ms += ms_f;
+        }
         for (int i = 0; i < nf; ++i) {
             SolveItem& it = items[f0 + i];
             if (it.x)
@@ -1035,6 +1044,7 @@ int bemx_solve_sweep(bemx_ctx* c, const double* zge, const double* zcs, const do
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
     acc[3] = ms;
     acc[7] = (double)c->lu.captures;
+    acc[8] = c->sw_factor_ms;
     acc[10] = (double)c->lu.replays;
     memcpy(c->last, acc, sizeof acc);
     return BEMX_OK;

@@ -150,7 +150,7 @@ int bemx_solve(bemx_ctx *ctx, const double *zge, const double *zcs, const double
  * Replaces the frequency loop of the reference's main program
  * (Main.for:167-200: GHMATECD + LINSOLVE per frequency) for one material.
  * Frequencies go in batches of up to BEMX_SWEEP_BATCH (environment, 1..8,
- * default 4; fewer when device memory is short): a batch is assembled two
+ * default 8; fewer when device memory is short): a batch is assembled two
  * frequencies per kernel launch on a second stream while the previous batch
  * is solved, and in BEMX_MODE_MIXED the batch's factorisations and
  * refinements share every launch.  Each frequency's solution is
@@ -168,7 +168,10 @@ int bemx_solve_sweep(bemx_ctx *ctx, const double *zge, const double *zcs, const 
  * (ms), whole assembly time (ms), factor+refine+solve time (ms),
  * factorisation graphs captured by the context so far, factorisation time
  * (ms), refinement/solve time after it (ms), graph replays so far.  Times
- * are device times from HIP events, filled by bemx_solve only.  No
+ * are device times from HIP events, filled by bemx_solve; after
+ * bemx_solve_sweep: iterations summed, worst residual and fallback, the
+ * sweep's device time, graph counts and the factorisations' device time
+ * summed over its batches (mixed mode).  No
  * reference counterpart (the reference prints per-routine wall times,
  * Ghmatecd.for:183-184). */
 int bemx_last_stats(const bemx_ctx *ctx, double *out, int nout);

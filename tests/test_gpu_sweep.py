@@ -1,15 +1,15 @@
 """bemx_solve_sweep, the frequency loop of the reference's main program
 (Main.for:167-200: GHMATECD + LINSOLVE per frequency) on the device (GPU).
 
-Frequencies go in batches (4 by default): a batch is assembled two
+Frequencies go in batches (8 by default): a batch is assembled two
 frequencies per launch (k_dyn_offdiag_h2, the Gauss-point geometry shared)
 on a second stream while the previous batch is solved, and the batch's
-mixed solves share every launch of the factorisation and the refinement
-(problem = blockIdx.z).  Each frequency's arithmetic is that of bemx_solve,
-so every solution must be bit-identical to solving that frequency alone:
-five frequencies (a batch of four and one of one) and seven (four and
-three), down to omega = 0.1 where the near-field series kernel carries most
-of the element pairs near the diagonal, with two load cases, in both modes.
+mixed solves share every launch of the factorisation and the refinement.
+Each frequency's arithmetic is that of bemx_solve, so every solution must be
+bit-identical to solving that frequency alone: five frequencies (one batch)
+in both modes, and eleven (a batch of eight, then one of three assembled
+beside it), down to omega = 0.1 where the near-field series kernel carries
+most of the element pairs near the diagonal, with two load cases.
 """
 import numpy as np
 import pytest
@@ -19,7 +19,7 @@ from bemx import api, decks
 pytestmark = pytest.mark.gpu
 
 FREQS = (0.1, 0.7, 2.0, 5.5, 9.0)
-FREQS7 = (0.1, 0.35, 0.7, 1.3, 2.0, 5.5, 9.0)
+FREQS11 = (0.1, 0.2, 0.35, 0.5, 0.7, 1.0, 1.3, 2.0, 3.5, 5.5, 9.0)
 
 
 @pytest.fixture(scope="module")
@@ -34,7 +34,7 @@ def ctx():
 
 
 @pytest.mark.parametrize("mode,freqs", [(api.MODE_MIXED, FREQS), (api.MODE_FP64, FREQS),
-                                        (api.MODE_MIXED, FREQS7)])
+                                        (api.MODE_MIXED, FREQS11)])
 def test_sweep_equals_each_frequency_alone(ctx, mode, freqs):
     c, mesh, dyn = ctx
     zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
