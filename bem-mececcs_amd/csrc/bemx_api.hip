@@ -827,7 +827,7 @@ int bemx_solve(bemx_ctx* c, const double* zge, const double* zcs, const double* 
 
 }  // extern "C"
 
-// Problems per batch of the batched solves (BEMX_SWEEP_BATCH, 1..8; default
+// Problems per batch of the batched solves (BEMX_SWEEP_BATCH, 1..16; default
 // 8).  Measured on the 6480-DOF sweep, one context: 118, 141 and 152
 // solves/s with batches of 2, 4 and 8.
 static int sweep_batch() {

@@ -71,7 +71,7 @@ int gmres_ir(LuWork& w, hipStream_t st, int n, const cd* A, size_t lda, cd* b, i
              double stats[4], std::string& err, int* nsolved);
 
 // Most problems in one batch of the mixed path.
-constexpr int MIXED_MAX_BATCH = 8;
+constexpr int MIXED_MAX_BATCH = 16;
 
 // The mixed solve of nz (1 .. MIXED_MAX_BATCH) problems of order n in one
 // batch: problem z's matrix at A + z sa bytes (read only) and its nrhs

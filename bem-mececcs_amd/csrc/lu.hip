@@ -1177,7 +1177,7 @@ int lu_solve_batch(LuWork& w, hipStream_t st, int n, int nz, cd* A, size_t lda, 
                    size_t sb, int nrhs, int mode, double* stats, std::string& err,
                    const std::function<int()>* after) {
     if (nz < 1 || nz > MIXED_MAX_BATCH || nrhs < 1) {
-        err = "LU: batch of 1..8 problems with at least one right-hand side";
+        err = "LU: batch of 1..16 problems with at least one right-hand side";
         return BEMX_ERR_ARG;
     }
     for (int i = 0; i < 4 * nz; ++i) stats[i] = 0.0;

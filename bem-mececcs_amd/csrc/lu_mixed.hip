@@ -2244,7 +2244,7 @@ int gmres_ir_batch(LuWork& w, hipStream_t st, int n, int nz, const cd* A64, size
                    cd* b, size_t sb, int nrhs, int* code, int* nsolved, double* stats,
                    std::string& err, const std::function<int()>* after) {
     if (nz < 1 || nz > MIXED_MAX_BATCH || n < 1 || nrhs < 1) {
-        err = "mixed solve: batch of 1..8 problems, order and right-hand sides at least 1";
+        err = "mixed solve: batch of 1..16 problems, order and right-hand sides at least 1";
         return BEMX_ERR_ARG;
     }
     MixedFactor F;
