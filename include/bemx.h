@@ -149,7 +149,7 @@ int bemx_solve(bemx_ctx *ctx, const double *zge, const double *zcs, const double
  * factorisation (+ refinement in BEMX_MODE_MIXED) and solve of bemx_solve.
  * Replaces the frequency loop of the reference's main program
  * (Main.for:167-200: GHMATECD + LINSOLVE per frequency) for one material.
- * Frequencies go in batches of up to BEMX_SWEEP_BATCH (environment, 1..8,
+ * Frequencies go in batches of up to BEMX_SWEEP_BATCH (environment, 1..16,
  * default 8; fewer when device memory is short): a batch is assembled two
  * frequencies per kernel launch on a second stream while the previous batch
  * is solved, and in BEMX_MODE_MIXED the batch's factorisations and
