@@ -38,6 +38,15 @@ __device__ inline cd zero_of<cd>() { return mk(0.0, 0.0); }
 template <>
 __device__ inline cf zero_of<cf>() { return mkf(0.f, 0.f); }
 
+// Wave-uniform copy of lane src's complex double (v_readlane, no LDS).
+__device__ inline cd readlane_c(cd v, int src) {
+    long long bx = __double_as_longlong(v.x), by = __double_as_longlong(v.y);
+    int xl = __builtin_amdgcn_readlane((int)bx, src), xh = __builtin_amdgcn_readlane((int)(bx >> 32), src);
+    int yl = __builtin_amdgcn_readlane((int)by, src), yh = __builtin_amdgcn_readlane((int)(by >> 32), src);
+    return mk(__longlong_as_double(((long long)xh << 32) | (unsigned)xl),
+              __longlong_as_double(((long long)yh << 32) | (unsigned)yl));
+}
+
 __device__ inline cd shfl_c(cd v, int src) { return mk(__shfl(v.x, src, 64), __shfl(v.y, src, 64)); }
 __device__ inline cf shfl_c(cf v, int src) { return mkf(__shfl(v.x, src, 64), __shfl(v.y, src, 64)); }
 

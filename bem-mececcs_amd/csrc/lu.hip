@@ -23,9 +23,12 @@
 //                  products (v_mfma_f64_16x16x4f64): Cr -= Ar Br - Ai Bi,
 //                  Ci -= Ar Bi + Ai Br; K = 64 inside a group, K = 256 for
 //                  the rest of the matrix after it.
+// After a group, its update of the next group's columns runs on the
+// solve's stream and the update of everything beyond them on a second
+// stream (look-ahead), so the next group's panels overlap the bulk update.
 // The solve walks the groups forward (permute, then per panel L11 solve and
-// L21 update) and the panels backward (U11 solve, U update) with one fused
-// kernel per step.
+// L21 update) and the panels backward (U11 solve, U update) with one kernel
+// per step.
 #include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
@@ -491,23 +494,31 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
     }
 }
 
-// Row permutation of columns [c0, c0+nc): row dst[i] <- old row src[i].
-// perm = {m, dst[0..2NB), src[0..2NB)}.  One workgroup handles 4 columns.
+// Row changes of cnt consecutive panels on the columns [c0, c0 + nc) except
+// [s0, s1), in panel order: row dst[i] <- old row src[i], with panel p's
+// perm at perm + p (1 + 4 NB) = {m, dst[0..2NB), src[0..2NB)}.  A wave
+// takes one column (a workgroup four); every source of a panel is read
+// before any destination is written.
 __global__ __launch_bounds__(256) void k_rowperm(cd* __restrict__ A, size_t lda, int c0, int nc,
-                                                 const int* __restrict__ perm) {
-    const int m = perm[0];
+                                                 int s0, int s1, const int* __restrict__ perm,
+                                                 int cnt) {
     const int c = c0 + blockIdx.x * 4 + (threadIdx.x >> 6);
     const int s = threadIdx.x & 63;
-    cd v[2];
-    const bool colok = c < c0 + nc;
-    for (int h = 0; h < 2; ++h) {
-        int i = s + 64 * h;
-        if (colok && i < m) v[h] = A[(size_t)c * lda + perm[1 + 2 * NB + i]];
-    }
-    __syncthreads();
-    for (int h = 0; h < 2; ++h) {
-        int i = s + 64 * h;
-        if (colok && i < m) A[(size_t)c * lda + perm[1 + i]] = v[h];
+    const bool colok = c < c0 + nc && (c < s0 || c >= s1);
+    for (int p = 0; p < cnt; ++p) {
+        const int* __restrict__ pm = perm + (size_t)p * (1 + 4 * NB);
+        const int m = pm[0];
+        cd v[2];
+        for (int h = 0; h < 2; ++h) {
+            const int i = s + 64 * h;
+            if (colok && i < m) v[h] = A[(size_t)c * lda + pm[1 + 2 * NB + i]];
+        }
+        __syncthreads();
+        for (int h = 0; h < 2; ++h) {
+            const int i = s + 64 * h;
+            if (colok && i < m) A[(size_t)c * lda + pm[1 + i]] = v[h];
+        }
+        __syncthreads();
     }
 }
 
@@ -597,7 +608,9 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(int m, int nc, int K, const 
 // of real and imaginary planes (rows padded by 2 doubles); the next chunk's
 // operands are loaded into registers while the MFMAs of the current one run.
 // Tiles are numbered so that each XCD takes a contiguous row-major range
-// (workgroup id runs on XCD id % 8), sharing A tiles in its L2.
+// (workgroup id runs on XCD id % 8), sharing A tiles in its L2.  The grid is
+// a multiple of 8 workgroups; with fewer workgroups than tiles each one
+// takes every (grid / 8)-th tile of its XCD's range in turn.
 constexpr int ZB = 128, ZK = 16, ZLD = ZB + 2, ZXCD = 8;
 __global__ __launch_bounds__(512) void k_zgemm_mfma2(int m, int nc, int K,
                                                      const cd* __restrict__ A, size_t lda,
@@ -607,8 +620,10 @@ __global__ __launch_bounds__(512) void k_zgemm_mfma2(int m, int nc, int K,
     __shared__ double sB[2][2][ZK][ZLD];
     const int gx = (m + ZB - 1) / ZB, gy = (nc + ZB - 1) / ZB, tiles = gx * gy;
     const int per = (tiles + ZXCD - 1) / ZXCD;
-    const int t = (blockIdx.x % ZXCD) * per + blockIdx.x / ZXCD;
-    if (t >= tiles) return;
+    const int xcd = blockIdx.x % ZXCD, step = gridDim.x / ZXCD;
+    for (int i = blockIdx.x / ZXCD; i < per; i += step) {
+    const int t = xcd * per + i;
+    if (t >= tiles) break;
     const int bm = (t / gy) * ZB, bn = (t % gy) * ZB;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = 64 * (wave & 1), wn = 32 * (wave >> 1);
@@ -708,86 +723,118 @@ __global__ __launch_bounds__(512) void k_zgemm_mfma2(int m, int nc, int K,
                 }
             }
         }
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Solve with the factors.  Forward step for panel k: apply the panel's row
-// permutation to b, y = L11^-1 b[k0:k0+nb] (every workgroup recomputes the
-// small block solve), then b[r] -= L21[r,:] y for the workgroup's rows.
-// Workgroup 0 also writes y back.  Backward step for panel k (last first):
-// x = U11^-1 b[k0:k0+nb], b[r] -= U[r, k0:k0+nb] x for rows r < k0.
-// One thread per row; b holds nrhs = 1 column.
+// Solve with the factors, one right-hand side, with one work vector y.
+// Forward, group by group: the group's row changes on b (k_permute_group),
+// then per panel k_fwd_blk: y_k = L11^-1 b_k into y, and b_r -= L21[r,:] y_k
+// in place for the rows below the block.  Backward, last panel first,
+// k_bwd_blk: x_k = U11^-1 y_k into b, and y_r -= U[r, k] x_k in place for
+// the rows above the block.  A step reads its block's entries of the vector
+// it updates and writes its block's result to the other vector, so no
+// workgroup writes what another one of the same step reads.
+// Each workgroup takes 64 rows; thread (r, q) = (tid & 63, tid >> 6) loads
+// row r's entries of the block columns 16q .. 16q + 15 before the block
+// solve, so those loads overlap it, and the four partial sums are added in
+// a fixed order.  Every workgroup solves the 64 x 64 triangular block itself
+// (one wave, lane = row of the block, v_readlane broadcasts).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_fwd_step(const cd* __restrict__ A, size_t lda, int n,
-                                                  int k0, int nb, const int* __restrict__ perm,
-                                                  cd* __restrict__ b, cd* __restrict__ btmp) {
-    __shared__ cd y[NB];
-    __shared__ cd L[NB * NB];
+__device__ __forceinline__ void blk_row_loads(const cd* __restrict__ A, size_t lda, int k0, int nb,
+                                              int row, bool rok, int q, cd (&a)[16]) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int c = 16 * q + i;
+        a[i] = (rok && c < nb) ? A[(size_t)(k0 + c) * lda + row] : mk(0.0, 0.0);
+    }
+}
+
+// The nb x nb diagonal block at (k0, k0), column-major with stride NB, the
+// rest of the 64 x 64 tile the identity.
+__device__ __forceinline__ void blk_stage(const cd* __restrict__ A, size_t lda, int k0, int nb,
+                                          cd* __restrict__ T) {
     const int tid = threadIdx.x;
-    for (int t = tid; t < nb * nb; t += 256) {
-        int c = t / nb, r = t - c * nb;
-        L[c * NB + r] = A[(size_t)(k0 + c) * lda + k0 + r];
+    cd v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int t = tid + 256 * u, c = t >> 6, rr = t & 63;
+        v[u] = (c < nb && rr < nb) ? A[(size_t)(k0 + c) * lda + k0 + rr]
+                                   : mk(rr == c ? 1.0 : 0.0, 0.0);
     }
-    // the permuted block of b (rows k0..k0+nb), read from the pre-step copy
-    // (perm NULL: the rows are already in place)
-    if (tid < nb) {
-        int row = k0 + tid;
-        int m = perm ? perm[0] : 0, src = row;
-        for (int i = 0; i < m; ++i)
-            if (perm[1 + i] == row) src = perm[1 + 2 * NB + i];
-        y[tid] = btmp[src];
-    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) T[tid + 256 * u] = v[u];
+}
+
+// out[row] = base - (s0 + s1 + s2 + s3) with s_q thread (r, q)'s partial sum.
+__device__ __forceinline__ cd blk_reduce(const cd (&a)[16], const cd* __restrict__ xs, int q, int r,
+                                         cd (*part)[64], cd base) {
+    cd s = mk(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) s = s + a[i] * xs[16 * q + i];
+    if (q > 0) part[q - 1][r] = s;
     __syncthreads();
-    if (tid < 64) {  // unit-lower solve, lane r owns y[r]
-        cd yr = tid < nb ? y[tid] : mk(0, 0);
-        for (int i = 0; i < nb; ++i) {
-            cd yi = mk(__shfl(yr.x, i, 64), __shfl(yr.y, i, 64));
-            if (tid > i && tid < nb) yr = yr - L[i * NB + tid] * yi;
+    return base - (((s + part[0][r]) + part[1][r]) + part[2][r]);
+}
+
+__global__ __launch_bounds__(256) void k_fwd_blk(const cd* __restrict__ A, size_t lda, int n,
+                                                 int k0, int nb, cd* __restrict__ b,
+                                                 cd* __restrict__ y) {
+    __shared__ cd T[NB * NB];
+    __shared__ cd ys[NB];
+    __shared__ cd part[3][64];
+    const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
+    const int row = k0 + nb + blockIdx.x * 64 + r;
+    const bool rok = row < n;
+    cd a[16];
+    blk_row_loads(A, lda, k0, nb, row, rok, q, a);
+    const cd br = (q == 0 && rok) ? b[row] : mk(0.0, 0.0);
+    blk_stage(A, lda, k0, nb, T);
+    __syncthreads();
+    if (q == 0) {
+        // unit lower: lane r subtracts L[r][i] y_i for every i < r
+        cd yr = r < nb ? b[k0 + r] : mk(0.0, 0.0);
+        for (int i = 0; i + 1 < nb; ++i) {
+            const cd yi = readlane_c(yr, i);
+            if (r > i) yr = cfms(yr, T[i * NB + r], yi);
         }
-        if (tid < nb) y[tid] = yr;
+        ys[r] = yr;
+        if (blockIdx.x == 0 && r < nb) y[k0 + r] = yr;
     }
     __syncthreads();
-    const int row = k0 + nb + blockIdx.x * 256 + tid;
-    if (blockIdx.x == 0 && tid < nb) b[k0 + tid] = y[tid];
-    if (row < n) {
-        int m = perm ? perm[0] : 0, src = row;
-        for (int i = 0; i < m; ++i)
-            if (perm[1 + i] == row) src = perm[1 + 2 * NB + i];
-        cd s = btmp[src];
-        for (int i = 0; i < nb; ++i) s = s - A[(size_t)(k0 + i) * lda + row] * y[i];
-        b[row] = s;
-    }
+    const cd v = blk_reduce(a, ys, q, r, part, br);
+    if (q == 0 && rok) b[row] = v;
 }
 
-__global__ __launch_bounds__(256) void k_bwd_step(const cd* __restrict__ A, size_t lda, int k0,
-                                                  int nb, cd* __restrict__ b,
-                                                  const cd* __restrict__ btmp) {
-    __shared__ cd x[NB];
-    __shared__ cd U[NB * NB];
-    const int tid = threadIdx.x;
-    for (int t = tid; t < nb * nb; t += 256) {
-        int c = t / nb, r = t - c * nb;
-        U[c * NB + r] = A[(size_t)(k0 + c) * lda + k0 + r];
-    }
-    if (tid < nb) x[tid] = btmp[k0 + tid];
+__global__ __launch_bounds__(256) void k_bwd_blk(const cd* __restrict__ A, size_t lda, int k0,
+                                                 int nb, cd* __restrict__ y, cd* __restrict__ x) {
+    __shared__ cd T[NB * NB];
+    __shared__ cd xs[NB];
+    __shared__ cd part[3][64];
+    const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
+    const int row = blockIdx.x * 64 + r;
+    const bool rok = row < k0;
+    cd a[16];
+    blk_row_loads(A, lda, k0, nb, row, rok, q, a);
+    const cd yr0 = (q == 0 && rok) ? y[row] : mk(0.0, 0.0);
+    blk_stage(A, lda, k0, nb, T);
     __syncthreads();
-    if (tid < 64) {  // upper solve, lane r owns x[r]
-        cd xr = tid < nb ? x[tid] : mk(0, 0);
+    if (q == 0) {
+        // upper: lane r divides by U[r][r] at step r, then subtracts
+        // U[r][i] x_i for every i > r
+        const cd dinv = cinv(T[r * NB + r]);
+        cd xr = r < nb ? y[k0 + r] : mk(0.0, 0.0);
         for (int i = nb - 1; i >= 0; --i) {
-            if (tid == i) xr = xr * cinv(U[i * NB + i]);
-            cd xi = mk(__shfl(xr.x, i, 64), __shfl(xr.y, i, 64));
-            if (tid < i) xr = xr - U[i * NB + tid] * xi;
+            if (r == i) xr = xr * dinv;
+            const cd xi = readlane_c(xr, i);
+            if (r < i) xr = cfms(xr, T[i * NB + r], xi);
         }
-        if (tid < nb) x[tid] = xr;
+        xs[r] = xr;
+        if (blockIdx.x == 0 && r < nb) x[k0 + r] = xr;
     }
     __syncthreads();
-    if (blockIdx.x == 0 && tid < nb) b[k0 + tid] = x[tid];
-    const int row = blockIdx.x * 256 + tid;
-    if (row < k0) {
-        cd s = btmp[row];
-        for (int i = 0; i < nb; ++i) s = s - A[(size_t)(k0 + i) * lda + row] * x[i];
-        b[row] = s;
-    }
+    const cd v = blk_reduce(a, xs, q, r, part, yr0);
+    if (q == 0 && rok) y[row] = v;
 }
 
 __global__ void k_fill_int(int* p, int v, size_t cnt) {
@@ -795,20 +842,21 @@ __global__ void k_fill_int(int* p, int v, size_t cnt) {
     if (i < cnt) p[i] = v;
 }
 
-__global__ void k_copy_cd(const cd* __restrict__ s, cd* __restrict__ d, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) d[i] = s[i];
-}
-
-// b <- one panel's row changes (perm = {m, dst[0..2NB), src[0..2NB)}), in
-// place: every source is read before any destination is written.
-__global__ __launch_bounds__(2 * NB) void k_permute_vec(cd* __restrict__ b,
-                                                        const int* __restrict__ perm) {
-    const int m = perm[0], t = threadIdx.x;
-    cd v = mk(0.0, 0.0);
-    if (t < m) v = b[perm[1 + 2 * NB + t]];
-    __syncthreads();
-    if (t < m) b[perm[1 + t]] = v;
+// b <- the row changes of cnt consecutive panels, in panel order (perm of
+// panel i at perm + i (1 + 4 NB) = {m, dst[0..2NB), src[0..2NB)}), in
+// place: every source of a panel is read before any destination is written.
+__global__ __launch_bounds__(2 * NB) void k_permute_group(cd* __restrict__ b,
+                                                          const int* __restrict__ perm, int cnt) {
+    const int t = threadIdx.x;
+    for (int i = 0; i < cnt; ++i) {
+        const int* pm = perm + (size_t)i * (1 + 4 * NB);
+        const int m = pm[0];
+        cd v = mk(0.0, 0.0);
+        if (t < m) v = b[pm[1 + 2 * NB + t]];
+        __syncthreads();
+        if (t < m) b[pm[1 + t]] = v;
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -955,13 +1003,17 @@ constexpr int FP64_GROUP = 4;
 // (on the 6480-DOF solve 32 against 27 TFLOP/s at m = n = 3216 .. 6416), the
 // 64 x 64 tiles below, where 128 x 128 tiles leave most CUs idle (m = n =
 // 416: 16 tiles, 49 against 23 us).
+// cap > 0: at most cap workgroups of the 128 x 128 form (a multiple of 8),
+// each taking several tiles, so the rest of the device stays free for other
+// streams' work.
 static void launch_zgemm(hipStream_t st, int m, int nc, int K, const cd* A, size_t lda,
-                         const cd* B, size_t ldb, cd* C, size_t ldc) {
+                         const cd* B, size_t ldb, cd* C, size_t ldc, int cap = 0) {
     if (m >= 1024 && nc >= 1024) {
         const int tiles = ((m + ZB - 1) / ZB) * ((nc + ZB - 1) / ZB);
         const int per = (tiles + ZXCD - 1) / ZXCD;
-        hipLaunchKernelGGL(k_zgemm_mfma2, dim3(per * ZXCD), dim3(512), 0, st, m, nc, K, A, lda, B,
-                           ldb, C, ldc);
+        const int wgs = cap > 0 ? std::min(per * ZXCD, cap) : per * ZXCD;
+        hipLaunchKernelGGL(k_zgemm_mfma2, dim3(wgs), dim3(512), 0, st, m, nc, K, A, lda, B, ldb,
+                           C, ldc);
     } else {
         hipLaunchKernelGGL(k_zgemm_mfma, dim3((m + GB - 1) / GB, (nc + GB - 1) / GB), dim3(256), 0,
                            st, m, nc, K, A, lda, B, ldb, C, ldc);
@@ -1023,16 +1075,62 @@ static int factor_fp64(const Fp64Ws& ws, hipStream_t st, int n, cd* A, size_t ld
         LU_CHK(hipEventRecord(tw->ev_f0, st));
     }
     auto permk_of = [&](int pk) { return perm + (size_t)pk * (1 + 4 * NB); };
-    auto rowperm = [&](int c0, int nc, const int* pm) {
+    // the row changes of panels pk .. pk + cnt - 1 on the columns [c0, c0 + nc)
+    // except [s0, s1)
+    auto rowperm = [&](hipStream_t s, int c0, int nc, int s0, int s1, int pk, int cnt) {
         if (nc > 0)
-            hipLaunchKernelGGL(k_rowperm, dim3((nc + 3) / 4), dim3(256), 0, st, A, lda, c0, nc, pm);
+            hipLaunchKernelGGL(k_rowperm, dim3((nc + 3) / 4), dim3(256), 0, s, A, lda, c0, nc, s0,
+                               s1, (const int*)permk_of(pk), cnt);
     };
-    auto trsm = [&](int k0, int nb, int c0, int nc) {
+    auto trsm = [&](hipStream_t s, int k0, int nb, int c0, int nc) {
         hipLaunchKernelGGL((k_trsm_lower_cols<cd, false>),
-                           dim3((nc + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG), dim3(256), 0, st,
+                           dim3((nc + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG), dim3(256), 0, s,
                            A, lda, k0, nb, c0, nc, (const int*)nullptr, (__bf16*)nullptr,
                            (size_t)0);
     };
+    // Group [g0, pend) applied to the columns [cr, cr + nr) right of it: the
+    // group's row changes in panel order, its 64-row blocks of U solved for
+    // in turn (each updating the group's later rows), then one update with
+    // K = the group's width.
+    auto apply_group = [&](hipStream_t s, int g0, int pend, int cr, int nr, int cap) {
+        if (nr <= 0) return;
+        const int G0 = g0 * NB, Ge = std::min(n, pend * NB);
+        rowperm(s, cr, nr, 0, 0, g0, pend - g0);
+        for (int pk = g0; pk < pend; ++pk) {
+            const int k0 = pk * NB, nb = std::min(NB, n - k0);
+            trsm(s, k0, nb, cr, nr);
+            const int mrow = Ge - (k0 + nb);
+            if (mrow > 0)
+                launch_zgemm(s, mrow, nr, nb, A + (size_t)k0 * lda + k0 + nb, lda,
+                             A + (size_t)cr * lda + k0, lda, A + (size_t)cr * lda + k0 + nb, lda,
+                             cap);
+        }
+        launch_zgemm(s, n - Ge, nr, Ge - G0, A + (size_t)G0 * lda + Ge, lda,
+                     A + (size_t)cr * lda + G0, lda, A + (size_t)cr * lda + Ge, lda, cap);
+    };
+    // Look-ahead (with a workspace that has a second stream): after group g,
+    // the next group's columns get group g on st, and every column beyond
+    // them gets it on the side stream, which runs while st factors the next
+    // group's panels.  st waits for the side stream before it applies the
+    // next group to anything beyond its own columns (those columns still
+    // lack group g there), and once more at the end.
+    // BEMX_FP64_LOOKAHEAD=0 keeps every update on st (for A/B timing)
+    static const bool lookahead = [] {
+        const char* v = std::getenv("BEMX_FP64_LOOKAHEAD");
+        return !(v && v[0] == '0');
+    }();
+    // BEMX_FP64_SIDE_WGS: workgroups of the side stream's updates (a
+    // multiple of 8; 0 = one per tile).  192 of the 256 CUs leave room for
+    // the panels and the next group's updates on st (6480 unknowns: 47.8 ms
+    // against 51.1 with one workgroup per tile and 49.3 with 160).
+    static const int side_wgs = [] {
+        const char* v = std::getenv("BEMX_FP64_SIDE_WGS");
+        const int c = v ? std::atoi(v) : 192;
+        return c > 0 ? (c + ZXCD - 1) / ZXCD * ZXCD : 0;
+    }();
+    LuWork* const la = lookahead ? tw : nullptr;
+    const hipStream_t sd = la ? la->side : st;
+    bool side_pending = false;
     for (int g0 = 0; g0 < npanel; g0 += FP64_GROUP) {
         const int pend = std::min(npanel, g0 + FP64_GROUP);
         const int G0 = g0 * NB, Ge = std::min(n, pend * NB);
@@ -1062,34 +1160,35 @@ static int factor_fp64(const Fp64Ws& ws, hipStream_t st, int n, cd* A, size_t ld
             // this panel's row changes on the group's earlier L columns (the
             // group's L rows stay in one order) and on its later columns,
             // which are then solved for and updated with K = 64
-            rowperm(G0, k0 - G0, permk);
             const int c0 = k0 + nb, ncg = Ge - c0;
+            rowperm(st, G0, Ge - G0, k0, c0, pk, 1);
             if (ncg > 0) {
-                rowperm(c0, ncg, permk);
-                trsm(k0, nb, c0, ncg);
+                trsm(st, k0, nb, c0, ncg);
                 launch_zgemm(st, n - c0, ncg, nb, A + (size_t)k0 * lda + c0, lda,
                              A + (size_t)c0 * lda + k0, lda, A + (size_t)c0 * lda + c0, lda);
             }
             LU_CHK(hipGetLastError());
         }
-        // the columns right of the group: the group's row changes in panel
-        // order, its 64-row blocks of U solved for in turn (each updating the
-        // group's later rows), then one update with K = the group's width
-        const int cr = Ge, nr = n - Ge;
+        const int nr = n - Ge;
         if (nr <= 0) continue;
-        for (int pk = g0; pk < pend; ++pk) rowperm(cr, nr, permk_of(pk));
-        for (int pk = g0; pk < pend; ++pk) {
-            const int k0 = pk * NB, nb = std::min(NB, n - k0);
-            trsm(k0, nb, cr, nr);
-            const int mrow = Ge - (k0 + nb);
-            if (mrow > 0)
-                launch_zgemm(st, mrow, nr, nb, A + (size_t)k0 * lda + k0 + nb, lda,
-                             A + (size_t)cr * lda + k0, lda, A + (size_t)cr * lda + k0 + nb, lda);
+        if (side_pending) LU_CHK(hipStreamWaitEvent(st, la->ev_bulk, 0));
+        side_pending = false;
+        if (!la) {
+            apply_group(st, g0, pend, Ge, nr, 0);
+        } else {
+            const int nn = std::min(nr, FP64_GROUP * NB);  // the next group's columns
+            apply_group(st, g0, pend, Ge, nn, 0);
+            if (nr > nn) {
+                LU_CHK(hipEventRecord(la->ev_panel, st));
+                LU_CHK(hipStreamWaitEvent(sd, la->ev_panel, 0));
+                apply_group(sd, g0, pend, Ge + nn, nr - nn, side_wgs);
+                LU_CHK(hipEventRecord(la->ev_bulk, sd));
+                side_pending = true;
+            }
         }
-        launch_zgemm(st, n - Ge, nr, Ge - G0, A + (size_t)G0 * lda + Ge, lda,
-                     A + (size_t)cr * lda + G0, lda, A + (size_t)cr * lda + Ge, lda);
         LU_CHK(hipGetLastError());
     }
+    if (side_pending) LU_CHK(hipStreamWaitEvent(st, la->ev_bulk, 0));
     if (tw) {
         LU_CHK(hipEventRecord(tw->ev_f1, st));
         tw->factor_timed = true;
@@ -1100,27 +1199,22 @@ static int factor_fp64(const Fp64Ws& ws, hipStream_t st, int n, cd* A, size_t ld
 static int solve_fp64(const Fp64Ws& ws, hipStream_t st, int n, const cd* A, size_t lda, cd* b,
                       std::string& err) {
     const int npanel = (n + NB - 1) / NB;
-    const int* perm = ws.perm;
-    cd* btmp = ws.btmp;
+    cd* y = ws.btmp;
     for (int g0 = 0; g0 < npanel; g0 += FP64_GROUP) {
         const int pend = std::min(npanel, g0 + FP64_GROUP);
-        for (int pk = g0; pk < pend; ++pk)
-            hipLaunchKernelGGL(k_permute_vec, dim3(1), dim3(2 * NB), 0, st, b,
-                               perm + (size_t)pk * (1 + 4 * NB));
+        hipLaunchKernelGGL(k_permute_group, dim3(1), dim3(2 * NB), 0, st, b,
+                           ws.perm + (size_t)g0 * (1 + 4 * NB), pend - g0);
         for (int pk = g0; pk < pend; ++pk) {
             const int k0 = pk * NB, nb = std::min(NB, n - k0);
-            hipLaunchKernelGGL(k_copy_cd, dim3((n + 255) / 256), dim3(256), 0, st, b, btmp, n);
-            const int rows = std::max(1, n - k0 - nb);
-            hipLaunchKernelGGL(k_fwd_step, dim3((rows + 255) / 256), dim3(256), 0, st, A, lda, n,
-                               k0, nb, (const int*)nullptr, b, btmp);
+            const int rows = n - k0 - nb;
+            hipLaunchKernelGGL(k_fwd_blk, dim3(std::max(1, (rows + 63) / 64)), dim3(256), 0, st,
+                               A, lda, n, k0, nb, b, y);
         }
     }
     for (int pk = npanel - 1; pk >= 0; --pk) {
         const int k0 = pk * NB, nb = std::min(NB, n - k0);
-        hipLaunchKernelGGL(k_copy_cd, dim3((n + 255) / 256), dim3(256), 0, st, b, btmp, n);
-        const int rows = std::max(1, k0);
-        hipLaunchKernelGGL(k_bwd_step, dim3((rows + 255) / 256), dim3(256), 0, st, A, lda, k0, nb,
-                           b, btmp);
+        hipLaunchKernelGGL(k_bwd_blk, dim3(std::max(1, (k0 + 63) / 64)), dim3(256), 0, st, A,
+                           lda, k0, nb, y, b);
     }
     LU_CHK(hipGetLastError());
     return BEMX_OK;

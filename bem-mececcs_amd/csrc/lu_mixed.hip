@@ -305,15 +305,6 @@ __global__ __launch_bounds__(64 * (NBM / PW)) void k_diag_gj(const cf* __restric
                  zp(status, bs, z), lmul, s_inv, s_piv, &s_done);
 }
 
-// Wave-uniform copy of lane src's complex double (v_readlane, no LDS).
-__device__ inline cd readlane_c(cd v, int src) {
-    long long bx = __double_as_longlong(v.x), by = __double_as_longlong(v.y);
-    int xl = __builtin_amdgcn_readlane((int)bx, src), xh = __builtin_amdgcn_readlane((int)(bx >> 32), src);
-    int yl = __builtin_amdgcn_readlane((int)by, src), yh = __builtin_amdgcn_readlane((int)(by >> 32), src);
-    return mk(__longlong_as_double(((long long)xh << 32) | (unsigned)xl),
-              __longlong_as_double(((long long)yh << 32) | (unsigned)yl));
-}
-
 // ---------------------------------------------------------------------------
 // Preconditioner block solves over 128-row blocks (two factor blocks each),
 // which halves the number of block hand-offs of the dependency chain.
