@@ -141,6 +141,64 @@ constexpr int SPIN_LIMIT = 1 << 22;
 
 __device__ inline double cabs1(cd a) { return fabs(a.x) + fabs(a.y); }
 
+// Wave maximum of a 64-bit unsigned key, uniform: DPP row shifts by 1, 2,
+// 4, 8 within each row of 16 lanes, then row_bcast 15 into rows 1 and 3 and
+// row_bcast 31 into rows 2 and 3, so lane 63 ends with the maximum; lanes
+// whose DPP source lies outside the row take 0, the identity.  A few VALU
+// instructions per step instead of an LDS permute per 32 bits.
+__device__ inline unsigned long long wave_max_u64(unsigned long long k) {
+    unsigned hi = (unsigned)(k >> 32), lo = (unsigned)k;
+#define BEMX_MAX_STEP(ctrl, rmask)                                                       \
+    {                                                                                    \
+        const unsigned nh = (unsigned)__builtin_amdgcn_update_dpp(0, (int)hi, ctrl, rmask, \
+                                                                  0xf, false);           \
+        const unsigned nl = (unsigned)__builtin_amdgcn_update_dpp(0, (int)lo, ctrl, rmask, \
+                                                                  0xf, false);           \
+        const bool gt = nh > hi || (nh == hi && nl > lo);                                \
+        hi = gt ? nh : hi;                                                               \
+        lo = gt ? nl : lo;                                                               \
+    }
+    BEMX_MAX_STEP(0x111, 0xf)
+    BEMX_MAX_STEP(0x112, 0xf)
+    BEMX_MAX_STEP(0x114, 0xf)
+    BEMX_MAX_STEP(0x118, 0xf)
+    BEMX_MAX_STEP(0x142, 0xa)
+    BEMX_MAX_STEP(0x143, 0xc)
+#undef BEMX_MAX_STEP
+    return ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)hi, 63) << 32) |
+           (unsigned)__builtin_amdgcn_readlane((int)lo, 63);
+}
+
+// Wave minimum of a 32-bit unsigned value (the same DPP pattern, identity
+// 0xffffffff), uniform.
+__device__ inline unsigned wave_min_u32(unsigned v) {
+    v = min(v, (unsigned)__builtin_amdgcn_update_dpp(-1, (int)v, 0x111, 0xf, 0xf, false));
+    v = min(v, (unsigned)__builtin_amdgcn_update_dpp(-1, (int)v, 0x112, 0xf, 0xf, false));
+    v = min(v, (unsigned)__builtin_amdgcn_update_dpp(-1, (int)v, 0x114, 0xf, 0xf, false));
+    v = min(v, (unsigned)__builtin_amdgcn_update_dpp(-1, (int)v, 0x118, 0xf, 0xf, false));
+    v = min(v, (unsigned)__builtin_amdgcn_update_dpp(-1, (int)v, 0x142, 0xa, 0xf, false));
+    v = min(v, (unsigned)__builtin_amdgcn_update_dpp(-1, (int)v, 0x143, 0xc, 0xf, false));
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Order-preserving key of a candidate |re|+|im| (>= 0; the bits of a
+// non-negative double order like its value), 0 for no candidate (v < 0).
+__device__ inline unsigned long long cand_key(double v) {
+    return v < 0.0 ? 0ull : (unsigned long long)__double_as_longlong(v) + 1ull;
+}
+__device__ inline double key_val(unsigned long long k) {
+    return k ? __longlong_as_double((long long)(k - 1ull)) : -1.0;
+}
+
+// a / u for the panel's multipliers with every operation written out (the
+// fused multiply-adds explicit), so both forms of the panel round it alike
+// whatever the compiler contracts around them.
+__device__ inline cd panel_scale(cd a, cd u) {
+    const double d = 1.0 / fma(u.x, u.x, u.y * u.y);
+    const double ix = u.x * d, iy = -u.y * d;
+    return cd{fma(a.x, ix, -(a.y * iy)), fma(a.x, iy, a.y * ix)};
+}
+
 // (val, row) ordering: larger val wins; equal val -> smaller row.
 __device__ inline bool better(double v, int r, double bv, int br) {
     return v > bv || (v == bv && r < br);
@@ -199,6 +257,34 @@ __device__ inline void tpoll(const u64* p, unsigned tag, u64 (&w)[N], bool activ
     }
 }
 
+// tpoll for two groups of N consecutive words at once (their loads in flight
+// together), into wa and wb.
+template <int N>
+__device__ inline void tpoll_pair(const u64* pa, const u64* pb, unsigned tag, u64 (&wa)[N],
+                                  u64 (&wb)[N], bool active, int* status) {
+    int it = 0;
+    for (;;) {
+        bool ok = true;
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) wa[i] = tget(pa + i);
+#pragma unroll
+            for (int i = 0; i < N; ++i) wb[i] = tget(pb + i);
+#pragma unroll
+            for (int i = 0; i < N; ++i) ok = ok && tok(wa[i], tag) && tok(wb[i], tag);
+        }
+        if (__all(ok)) return;
+        if ((++it & 63) == 0 &&
+            (it > SPIN_LIMIT ||
+             (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+              ST_SYNC_TIMEOUT))) {
+            atomicOr(status, ST_SYNC_TIMEOUT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Panel factorisation.  Workgroup p owns panel rows [r0, r1) (global row
 // indices, r0 = k0 + p*rpw) held column-major in LDS (TALL: the first
@@ -230,6 +316,9 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
                                                     int* __restrict__ status, unsigned tag) {
     __shared__ cd sA[NB * PANEL_ROWS];
     __shared__ cd urow[NB], dbuf[NB];
+    __shared__ cd urow2[2][NB];  // the plain form's pivot rows, by step parity
+    __shared__ int s_piv2[2];
+    __shared__ double s_pval2[2];
     __shared__ double rv[4];
     __shared__ int rr[4];
     __shared__ int s_piv, s_cand;
@@ -321,6 +410,171 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
     local_cand(0, k0);
     publish(0);
     const int r = tid & (PANEL_ROWS - 1), half = tid >> 7;  // update mapping: row, column parity
+    if constexpr (!TALL) {
+    // Plain form (at most 128 rows, all in LDS), two barriers per column:
+    //   poll   wave 0: the slots, then the winner's row and row d together,
+    //          into the buffers of the step's parity (waves 1-3 may still run
+    //          the previous step's rank-1 update, which reads the other one);
+    //   [A]    swap: row d <- pivot row; row piv <- old row d except its
+    //          columns j, j+1, which the thread of that row takes from dbuf;
+    //          thread r (< 128) scales column j of its row and updates
+    //          column j+1, and its row's |re|+|im| there is its candidate
+    //          key; a wave maximum per wave;
+    //   [E]    every thread combines the two waves' candidates; thread c of
+    //          wave 0 (wave 1) completes entry c of the candidate row (row
+    //          d+1) for columns j+2.. and publishes it with no further
+    //          barrier; then waves 1-3 do the rest of the rank-1 update, in
+    //          batches of eight independent LDS reads, ordered before the
+    //          next column by the next step's [A].
+    for (int j = 0; j < nb; ++j) {
+        const int d = k0 + j, buf = j & 1;
+        cd* __restrict__ U = urow2[buf];
+        if (wave == 0) {
+            double wv = -1.0;
+            int wr = 0x7fffffff;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int q = lane + 64 * u;
+                if (64 * u >= P) break;  // wave-uniform
+                u64 w3[3];
+                tpoll<3>(slots + ((size_t)j * P + min(q, P - 1)) * 3, tag, w3, q < P, status);
+                const double cv = tval_d(w3[0], w3[1]);
+                const int cr = (int)(unsigned)w3[2];
+                if (q < P && better(cv, cr, wv, wr)) {
+                    wv = cv;
+                    wr = cr;
+                }
+            }
+            // the largest value, then the lowest row holding it; workgroup
+            // q owns the rows from k0 + q rpw on
+            const unsigned long long kmax = wave_max_u64(cand_key(wv));
+            wr = (int)wave_min_u32(cand_key(wv) == kmax ? (unsigned)wr : 0xffffffffu);
+            wv = key_val(kmax);
+            const int wp = (wr - k0) / rpw;
+            // the winner's row and row d together
+            u64 w4[4], wd[4];
+            tpoll_pair<4>(cand + (((size_t)j * P + wp) * NB + min(lane, nb - 1)) * 4,
+                          drow + ((size_t)j * NB + min(lane, nb - 1)) * 4, tag, w4, wd, lane < nb,
+                          status);
+            if (lane < nb) {
+                U[lane] = mk(tval_d(w4[0], w4[1]), tval_d(w4[2], w4[3]));
+                dbuf[lane] = mk(tval_d(wd[0], wd[1]), tval_d(wd[2], wd[3]));
+            }
+            if (lane == 0) {
+                s_piv2[buf] = wr;
+                s_pval2[buf] = wv;
+            }
+        }
+        __syncthreads();  // [A]
+        const int piv = s_piv2[buf];
+        const double pval = s_pval2[buf];
+        if (tid == 0) s_pivs[j] = piv;
+        if (p == 0 && tid == 0) {
+            ipiv[d] = piv;
+            if (pval == 0.0 && *info == 0) *info = d + 1;
+        }
+        if (piv != d && tid < nb) {
+            if (d >= r0 && d < r1) sA[tid * PANEL_ROWS + d - r0] = U[tid];
+            if (piv >= r0 && piv < r1 && tid != j && tid != j + 1)
+                sA[tid * PANEL_ROWS + piv - r0] = dbuf[tid];
+        }
+        const bool more = j + 1 < nb;
+        double bv = -1.0;
+        int br = 0x7fffffff;
+        if (tid < nr && r0 + tid > d) {
+            const bool moved = r0 + tid == piv && piv != d;
+            cd aj = moved ? dbuf[j] : sA[j * PANEL_ROWS + tid];
+            cd aj1 = more ? (moved ? dbuf[j + 1] : sA[(j + 1) * PANEL_ROWS + tid]) : mk(0.0, 0.0);
+            if (pval != 0.0) {
+                aj = panel_scale(aj, U[j]);
+                if (more) aj1 = cfms(aj1, aj, U[j + 1]);
+            }
+            sA[j * PANEL_ROWS + tid] = aj;
+            if (more) {
+                sA[(j + 1) * PANEL_ROWS + tid] = aj1;
+                bv = cabs1(aj1);
+                br = r0 + tid;
+            }
+        }
+        if (wave < 2) {
+            // lane order is row order: the lowest lane holding the largest
+            // value is the lowest row
+            const unsigned long long k = cand_key(bv), kmax = wave_max_u64(k);
+            const unsigned long long tie = __ballot(k == kmax);
+            if (lane == 0) {
+                rv[wave] = key_val(kmax);
+                rr[wave] = kmax ? r0 + 64 * wave + __ffsll((long long)tie) - 1 : 0x7fffffff;
+            }
+        }
+        __syncthreads();  // [E]
+        int sp1 = -1, sp2 = -1;  // local rows already completed for columns j+2..
+        if (more) {
+            double cv = rv[0];
+            int cr = rr[0];
+            if (better(rv[1], rr[1], cv, cr)) {
+                cv = rv[1];
+                cr = rr[1];
+            }
+            if (cr != 0x7fffffff) sp1 = cr - r0;
+            const bool own_d1 = d + 1 >= r0 && d + 1 < r1;
+            if (own_d1 && d + 1 != cr) sp2 = d + 1 - r0;
+            // wave 0: the candidate row (and row d+1 when it is the
+            // candidate); wave 1: row d+1 otherwise; thread c of the wave
+            // owns entry c
+            const int sr = wave == 0 ? sp1 : (wave == 1 ? sp2 : -1);
+            const int c = lane;
+            if (sr >= 0 && c < nb) {
+                cd v = sA[c * PANEL_ROWS + sr];
+                if (c >= j + 2 && pval != 0.0) {
+                    v = cfms(v, sA[j * PANEL_ROWS + sr], U[c]);
+                    sA[c * PANEL_ROWS + sr] = v;
+                }
+                if (wave == 0) {
+                    u64* q = cand + (((size_t)(j + 1) * P + p) * NB + c) * 4;
+                    tput_d(q, tag, v.x);
+                    tput_d(q + 2, tag, v.y);
+                }
+                if (wave == 1 || (own_d1 && d + 1 == cr)) {
+                    u64* q = drow + ((size_t)(j + 1) * NB + c) * 4;
+                    tput_d(q, tag, v.x);
+                    tput_d(q + 2, tag, v.y);
+                }
+            }
+            if (tid == 0) {
+                u64* q = slots + ((size_t)(j + 1) * P + p) * 3;
+                tput_d(q, tag, cv);
+                tput(q + 2, tag, (unsigned)cr);
+            }
+        }
+        // the rest of the rank-1 update (columns j+2..) by waves 1-3, while
+        // wave 0 goes on to the next column's polls: item (row, third) of
+        // the 128 x 3 items, the third q taking the columns j+2+q, step 3
+        if (wave > 0 && pval != 0.0) {
+#pragma unroll
+            for (int it = 0; it < 2; ++it) {
+                const int item = tid - 64 + 192 * it, rw = item & (PANEL_ROWS - 1);
+                if (rw < nr && r0 + rw > d && rw != sp1 && rw != sp2) {
+                    const cd l = sA[j * PANEL_ROWS + rw];
+                    int c = j + 2 + (item >> 7);
+                    for (; c + 21 < nb; c += 24) {
+                        cd v[8], u[8];
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            v[k] = sA[(c + 3 * k) * PANEL_ROWS + rw];
+                            u[k] = U[c + 3 * k];
+                        }
+#pragma unroll
+                        for (int k = 0; k < 8; ++k)
+                            sA[(c + 3 * k) * PANEL_ROWS + rw] = cfms(v[k], l, u[k]);
+                    }
+                    for (; c < nb; c += 3)
+                        sA[c * PANEL_ROWS + rw] = cfms(sA[c * PANEL_ROWS + rw], l, U[c]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    } else {
     for (int j = 0; j < nb; ++j) {
         const int d = k0 + j;
         if (wave == 0) {
@@ -380,10 +634,10 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
             if (piv >= r0 && piv < r1 && tid < nb) el(tid, piv - r0) = dbuf[tid];
         }
         __syncthreads();
-        const cd inv = (pval != 0.0) ? cinv(urow[j]) : mk(0.0, 0.0);
+
         for (int ri = tid; ri < (TALL ? nr : min(nr, 256)); ri += 256) {
             if (r0 + ri > d && pval != 0.0) {
-                const cd l = el(j, ri) * inv;
+                const cd l = panel_scale(el(j, ri), urow[j]);
                 el(j, ri) = l;
                 if (j + 1 < nb) el(j + 1, ri) = cfms(el(j + 1, ri), l, urow[j + 1]);
             }
@@ -430,6 +684,7 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
                 sA[c * PANEL_ROWS + r] = cfms(sA[c * PANEL_ROWS + r], l, urow[c]);
         }
         __syncthreads();
+    }
     }
     for (int t = tid; t < nl * nb; t += 256) {
         int c = t / nl, r2 = t - c * nl;
