@@ -1375,9 +1375,10 @@ static int factor_fp64(const Fp64Ws& ws, hipStream_t st, int n, cd* A, size_t ld
         return !(v && v[0] == '0');
     }();
     // BEMX_FP64_SIDE_WGS: workgroups of the side stream's updates (a
-    // multiple of 8; 0 = one per tile).  192 of the 256 CUs leave room for
-    // the panels and the next group's updates on st (6480 unknowns: 47.8 ms
-    // against 51.1 with one workgroup per tile and 49.3 with 160).
+    // multiple of 8; 0 = as many as the next panels leave free).  192 of the
+    // 256 CUs leave room for the panels and the next group's updates on st
+    // (6480 unknowns: 47.8 ms against 51.1 with one workgroup per tile and
+    // 49.3 with 160).
     static const int side_wgs = [] {
         const char* v = std::getenv("BEMX_FP64_SIDE_WGS");
         const int c = v ? std::atoi(v) : 192;
@@ -1428,18 +1429,28 @@ static int factor_fp64(const Fp64Ws& ws, hipStream_t st, int n, cd* A, size_t ld
         if (nr <= 0) continue;
         if (side_pending) LU_CHK(hipStreamWaitEvent(st, la->ev_bulk, 0));
         side_pending = false;
-        if (!la) {
+        // The second stream may hold only the CUs that the next group's
+        // panels leave free (one workgroup per CU each).  On C of the ncu
+        // CUs the bulk update takes ncu / C times as long as on all of them,
+        // so it goes beside the next group only when that stretch costs less
+        // than the group's panels take (about 1.2 ms for four panels, nearly
+        // independent of n); the bulk's own time is estimated at 47 TFLOP/s.
+        const int pnext = (nr + panel_rows(nr, cap, forced) - 1) / panel_rows(nr, cap, forced);
+        const int room = (ncu - pnext) / ZXCD * ZXCD;
+        const int nn = std::min(nr, FP64_GROUP * NB);  // the next group's columns
+        const int cside = side_wgs > 0 ? std::min(side_wgs, room) : room;
+        const double bulk_ms = 8.0 * nr * (double)(nr - nn) * (Ge - G0) / 47e9;
+        const bool ahead =
+            la && nr > nn && cside >= 64 && bulk_ms * ((double)ncu / cside - 1.0) < 1.2;
+        if (!ahead) {
             apply_group(st, g0, pend, Ge, nr, 0);
         } else {
-            const int nn = std::min(nr, FP64_GROUP * NB);  // the next group's columns
             apply_group(st, g0, pend, Ge, nn, 0);
-            if (nr > nn) {
-                LU_CHK(hipEventRecord(la->ev_panel, st));
-                LU_CHK(hipStreamWaitEvent(sd, la->ev_panel, 0));
-                apply_group(sd, g0, pend, Ge + nn, nr - nn, side_wgs);
-                LU_CHK(hipEventRecord(la->ev_bulk, sd));
-                side_pending = true;
-            }
+            LU_CHK(hipEventRecord(la->ev_panel, st));
+            LU_CHK(hipStreamWaitEvent(sd, la->ev_panel, 0));
+            apply_group(sd, g0, pend, Ge + nn, nr - nn, cside);
+            LU_CHK(hipEventRecord(la->ev_bulk, sd));
+            side_pending = true;
         }
         LU_CHK(hipGetLastError());
     }

@@ -69,7 +69,7 @@ HBM_PEAK_GBS = 8000.0
 # under profiles/.
 OFFDIAG_TRAFFIC_BYTES = 721.2e6
 FACTOR_TRAFFIC_BYTES = 10.22e9
-FP64_FACTOR_TRAFFIC_BYTES = 28.04e9
+FP64_FACTOR_TRAFFIC_BYTES = 29.32e9
 # Counter figures of the same runs: FP64 VALU flops counted by
 # SQ_INSTS_VALU_FLOPS_FP64 (per wave instruction, x 64 lanes) per
 # k_dyn_offdiag_h launch, and MFMA flops (SQ_INSTS_VALU_MFMA_MOPS_* x 512)
