@@ -882,6 +882,9 @@ int ctx_solve_items(bemx_ctx* c, int nitem, SolveItem* items, int mode,
                                    mixed_workspace_bytes((int)nn, nbat) > fr / 2)
             --nbat;
     }
+    // batches as even as the count allows (20 problems in batches of at most
+    // 8: 7, 7 and 6 rather than 8, 8 and 4)
+    nbat = (nitem + (nitem + nbat - 1) / nbat - 1) / ((nitem + nbat - 1) / nbat);
     const int nslot = std::min(nitem, halves * nbat);
     HIPCHK(c, c->colflag.ensure(sizeof(int) * nbe));
     HIPCHK(c, c->flist.ensure(sizeof(int) * (nbe + 1)));

@@ -150,7 +150,8 @@ int bemx_solve(bemx_ctx *ctx, const double *zge, const double *zcs, const double
  * Replaces the frequency loop of the reference's main program
  * (Main.for:167-200: GHMATECD + LINSOLVE per frequency) for one material.
  * Frequencies go in batches of up to BEMX_SWEEP_BATCH (environment, 1..16,
- * default 8; fewer when device memory is short): a batch is assembled two
+ * default 8; fewer when device memory is short; sizes as even as the count
+ * allows, e.g. 7, 7, 6 for 20 frequencies): a batch is assembled two
  * frequencies per kernel launch on a second stream while the previous batch
  * is solved, and in BEMX_MODE_MIXED the batch's factorisations and
  * refinements share every launch.  Each frequency's solution is
