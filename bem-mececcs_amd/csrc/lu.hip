@@ -257,21 +257,30 @@ __device__ inline void tpoll(const u64* p, unsigned tag, u64 (&w)[N], bool activ
     }
 }
 
-// tpoll for two groups of N consecutive words at once (their loads in flight
-// together), into wa and wb.
-template <int N>
-__device__ inline void tpoll_pair(const u64* pa, const u64* pb, unsigned tag, u64 (&wa)[N],
-                                  u64 (&wb)[N], bool active, int* status) {
+// tpoll for two groups of consecutive words at once (their loads in flight
+// together): NA words at pa into wa (if active_a), NB_ at pb into wb (if
+// active_b).
+template <int NA, int NB_>
+__device__ inline void tpoll_pair(const u64* pa, const u64* pb, unsigned tag, u64 (&wa)[NA],
+                                  u64 (&wb)[NB_], bool active_a, bool active_b, int* status) {
     int it = 0;
     for (;;) {
         bool ok = true;
-        if (active) {
+        if (active_a) {
 #pragma unroll
-            for (int i = 0; i < N; ++i) wa[i] = tget(pa + i);
+            for (int i = 0; i < NA; ++i) wa[i] = tget(pa + i);
+        }
+        if (active_b) {
 #pragma unroll
-            for (int i = 0; i < N; ++i) wb[i] = tget(pb + i);
+            for (int i = 0; i < NB_; ++i) wb[i] = tget(pb + i);
+        }
+        if (active_a) {
 #pragma unroll
-            for (int i = 0; i < N; ++i) ok = ok && tok(wa[i], tag) && tok(wb[i], tag);
+            for (int i = 0; i < NA; ++i) ok = ok && tok(wa[i], tag);
+        }
+        if (active_b) {
+#pragma unroll
+            for (int i = 0; i < NB_; ++i) ok = ok && tok(wb[i], tag);
         }
         if (__all(ok)) return;
         if ((++it & 63) == 0 &&
@@ -412,9 +421,10 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
     const int r = tid & (PANEL_ROWS - 1), half = tid >> 7;  // update mapping: row, column parity
     if constexpr (!TALL) {
     // Plain form (at most 128 rows, all in LDS), two barriers per column:
-    //   poll   wave 0: the slots, then the winner's row and row d together,
-    //          into the buffers of the step's parity (waves 1-3 may still run
-    //          the previous step's rank-1 update, which reads the other one);
+    //   poll   wave 0: the slots and row d together, then the winner's row
+    //          unless it is row d, into the buffers of the step's parity
+    //          (waves 1-3 may still run the previous step's rank-1 update,
+    //          which reads the other one);
     //   [A]    swap: row d <- pivot row; row piv <- old row d except its
     //          columns j, j+1, which the thread of that row takes from dbuf;
     //          thread r (< 128) scales column j of its row and updates
@@ -432,12 +442,18 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
         if (wave == 0) {
             double wv = -1.0;
             int wr = 0x7fffffff;
+            u64 wd[4];  // row d, published a column earlier, polled with the first slots
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int q = lane + 64 * u;
                 if (64 * u >= P) break;  // wave-uniform
                 u64 w3[3];
-                tpoll<3>(slots + ((size_t)j * P + min(q, P - 1)) * 3, tag, w3, q < P, status);
+                if (u == 0)
+                    tpoll_pair<3, 4>(slots + ((size_t)j * P + min(q, P - 1)) * 3,
+                                     drow + ((size_t)j * NB + min(lane, nb - 1)) * 4, tag, w3, wd,
+                                     q < P, lane < nb, status);
+                else
+                    tpoll<3>(slots + ((size_t)j * P + min(q, P - 1)) * 3, tag, w3, q < P, status);
                 const double cv = tval_d(w3[0], w3[1]);
                 const int cr = (int)(unsigned)w3[2];
                 if (q < P && better(cv, cr, wv, wr)) {
@@ -451,14 +467,20 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
             wr = (int)wave_min_u32(cand_key(wv) == kmax ? (unsigned)wr : 0xffffffffu);
             wv = key_val(kmax);
             const int wp = (wr - k0) / rpw;
-            // the winner's row and row d together
-            u64 w4[4], wd[4];
-            tpoll_pair<4>(cand + (((size_t)j * P + wp) * NB + min(lane, nb - 1)) * 4,
-                          drow + ((size_t)j * NB + min(lane, nb - 1)) * 4, tag, w4, wd, lane < nb,
-                          status);
+            // the winner's row: row d itself when the diagonal wins (no
+            // second round trip; its owner published it both as its
+            // candidate and as row d, from the same values), else polled
+            const cd rd = mk(tval_d(wd[0], wd[1]), tval_d(wd[2], wd[3]));
+            cd ru = rd;
+            if (wr != d) {
+                u64 w4[4];
+                tpoll<4>(cand + (((size_t)j * P + wp) * NB + min(lane, nb - 1)) * 4, tag, w4,
+                         lane < nb, status);
+                ru = mk(tval_d(w4[0], w4[1]), tval_d(w4[2], w4[3]));
+            }
             if (lane < nb) {
-                U[lane] = mk(tval_d(w4[0], w4[1]), tval_d(w4[2], w4[3]));
-                dbuf[lane] = mk(tval_d(wd[0], wd[1]), tval_d(wd[2], wd[3]));
+                U[lane] = ru;
+                dbuf[lane] = rd;
             }
             if (lane == 0) {
                 s_piv2[buf] = wr;
