@@ -13,12 +13,14 @@
 //                  (above 32768 on 256 CUs) use the tall form, whose
 //                  workgroups own 256 or more rows: the first 128 in LDS,
 //                  the rest updated in place in the matrix.
-//   k_rowperm      applies that permutation to other columns: inside the
+//   k_perm_trsm    applies that permutation to other columns (inside the
 //                  group to its earlier L columns and its later columns,
-//                  after the group to the columns right of it (L columns of
+//                  after the group to the columns right of it; L columns of
 //                  earlier groups are left in place: the forward solve
-//                  applies a group's permutations before its L blocks).
-//   k_trsm_lower_cols  U12 = L11^-1 A12, one lane per column.
+//                  applies a group's permutations before its L blocks) and
+//                  solves U12 = L11^-1 A12 for the columns right of the
+//                  panel in the same launch (k_trsm_lower_cols alone for the
+//                  group's later blocks), one lane per column.
 //   k_zgemm_mfma, k_zgemm_mfma2  A22 -= L21 U12 as four real FP64 MFMA
 //                  products (v_mfma_f64_16x16x4f64): Cr -= Ar Br - Ai Bi,
 //                  Ci -= Ar Bi + Ai Br; K = 64 inside a group, K = 256 for
@@ -771,32 +773,66 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
     }
 }
 
-// Row changes of cnt consecutive panels on the columns [c0, c0 + nc) except
-// [s0, s1), in panel order: row dst[i] <- old row src[i], with panel p's
-// perm at perm + p (1 + 4 NB) = {m, dst[0..2NB), src[0..2NB)}.  A wave
-// takes one column (a workgroup four); every source of a panel is read
-// before any destination is written.
-__global__ __launch_bounds__(256) void k_rowperm(cd* __restrict__ A, size_t lda, int c0, int nc,
-                                                 int s0, int s1, const int* __restrict__ perm,
-                                                 int cnt) {
-    const int c = c0 + blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int s = threadIdx.x & 63;
-    const bool colok = c < c0 + nc && (c < s0 || c >= s1);
+// The row changes of panels pk .. pk + cnt - 1, then U12 = L11^-1 A12 for the
+// columns [c0, c0 + nc) (k_trsm_lower_cols' body, 64 columns per workgroup),
+// in one launch; workgroups beyond those of the solve apply the changes alone
+// to the columns [l0, l0 + nl), four per workgroup.  In a solve workgroup
+// wave w permutes the block's columns w, w + 4, ..., all sixteen columns'
+// sources read before any destination is written, and the solve reads them
+// after a barrier.
+__global__ __launch_bounds__(256) void k_perm_trsm(cd* __restrict__ A, size_t lda, int k0, int nb,
+                                                   int c0, int nc, int l0, int nl,
+                                                   const int* __restrict__ perm, int cnt) {
+    __shared__ cd Ls[TB * TB];
+    __shared__ cd X[TB * (TB + 1)];
+    const int ntr = (nc + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG;
+    const int w = threadIdx.x >> 6, s = threadIdx.x & 63;
+    if ((int)blockIdx.x >= ntr) {
+        const int c = l0 + ((int)blockIdx.x - ntr) * 4 + w;
+        const bool colok = c < l0 + nl;
+        for (int p = 0; p < cnt; ++p) {
+            const int* __restrict__ pm = perm + (size_t)p * (1 + 4 * NB);
+            const int m = pm[0];
+            cd v[2];
+            for (int h = 0; h < 2; ++h) {
+                const int i = s + 64 * h;
+                if (colok && i < m) v[h] = A[(size_t)c * lda + pm[1 + 2 * NB + i]];
+            }
+            __syncthreads();
+            for (int h = 0; h < 2; ++h) {
+                const int i = s + 64 * h;
+                if (colok && i < m) A[(size_t)c * lda + pm[1 + i]] = v[h];
+            }
+            __syncthreads();
+        }
+        return;
+    }
+    const int cb = c0 + blockIdx.x * TRSM_COLS_PER_WG;
     for (int p = 0; p < cnt; ++p) {
         const int* __restrict__ pm = perm + (size_t)p * (1 + 4 * NB);
         const int m = pm[0];
-        cd v[2];
-        for (int h = 0; h < 2; ++h) {
-            const int i = s + 64 * h;
-            if (colok && i < m) v[h] = A[(size_t)c * lda + pm[1 + 2 * NB + i]];
+        cd v[16][2];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int c = cb + w + 4 * u;
+            for (int h = 0; h < 2; ++h) {
+                const int i = s + 64 * h;
+                if (c < c0 + nc && i < m) v[u][h] = A[(size_t)c * lda + pm[1 + 2 * NB + i]];
+            }
         }
         __syncthreads();
-        for (int h = 0; h < 2; ++h) {
-            const int i = s + 64 * h;
-            if (colok && i < m) A[(size_t)c * lda + pm[1 + i]] = v[h];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int c = cb + w + 4 * u;
+            for (int h = 0; h < 2; ++h) {
+                const int i = s + 64 * h;
+                if (c < c0 + nc && i < m) A[(size_t)c * lda + pm[1 + i]] = v[u][h];
+            }
         }
         __syncthreads();
     }
+    trsm_lower_cols_body<cd, false>(blockIdx.x, A, lda, k0, nb, c0, nc, nullptr, nullptr, 0, Ls,
+                                    X);
 }
 
 // ---------------------------------------------------------------------------
@@ -1352,12 +1388,14 @@ static int factor_fp64(const Fp64Ws& ws, hipStream_t st, int n, cd* A, size_t ld
         LU_CHK(hipEventRecord(tw->ev_f0, st));
     }
     auto permk_of = [&](int pk) { return perm + (size_t)pk * (1 + 4 * NB); };
-    // the row changes of panels pk .. pk + cnt - 1 on the columns [c0, c0 + nc)
-    // except [s0, s1)
-    auto rowperm = [&](hipStream_t s, int c0, int nc, int s0, int s1, int pk, int cnt) {
-        if (nc > 0)
-            hipLaunchKernelGGL(k_rowperm, dim3((nc + 3) / 4), dim3(256), 0, s, A, lda, c0, nc, s0,
-                               s1, (const int*)permk_of(pk), cnt);
+    // panels pk .. pk + cnt - 1's row changes on the columns [c0, c0 + nc),
+    // which are then solved for with panel k0's L11, and on [l0, l0 + nl)
+    auto perm_trsm = [&](hipStream_t s, int k0, int nb, int c0, int nc, int l0, int nl, int pk,
+                         int cnt) {
+        const int wgs = (nc + TRSM_COLS_PER_WG - 1) / TRSM_COLS_PER_WG + (nl + 3) / 4;
+        if (wgs > 0)
+            hipLaunchKernelGGL(k_perm_trsm, dim3(wgs), dim3(256), 0, s, A, lda, k0, nb, c0, nc,
+                               l0, nl, (const int*)permk_of(pk), cnt);
     };
     auto trsm = [&](hipStream_t s, int k0, int nb, int c0, int nc) {
         hipLaunchKernelGGL((k_trsm_lower_cols<cd, false>),
@@ -1372,10 +1410,12 @@ static int factor_fp64(const Fp64Ws& ws, hipStream_t st, int n, cd* A, size_t ld
     auto apply_group = [&](hipStream_t s, int g0, int pend, int cr, int nr, int cap) {
         if (nr <= 0) return;
         const int G0 = g0 * NB, Ge = std::min(n, pend * NB);
-        rowperm(s, cr, nr, 0, 0, g0, pend - g0);
         for (int pk = g0; pk < pend; ++pk) {
             const int k0 = pk * NB, nb = std::min(NB, n - k0);
-            trsm(s, k0, nb, cr, nr);
+            if (pk == g0)  // the group's row changes, then the first block's solve
+                perm_trsm(s, k0, nb, cr, nr, 0, 0, g0, pend - g0);
+            else
+                trsm(s, k0, nb, cr, nr);
             const int mrow = Ge - (k0 + nb);
             if (mrow > 0)
                 launch_zgemm(s, mrow, nr, nb, A + (size_t)k0 * lda + k0 + nb, lda,
@@ -1439,12 +1479,10 @@ static int factor_fp64(const Fp64Ws& ws, hipStream_t st, int n, cd* A, size_t ld
             // group's L rows stay in one order) and on its later columns,
             // which are then solved for and updated with K = 64
             const int c0 = k0 + nb, ncg = Ge - c0;
-            rowperm(st, G0, Ge - G0, k0, c0, pk, 1);
-            if (ncg > 0) {
-                trsm(st, k0, nb, c0, ncg);
+            perm_trsm(st, k0, nb, c0, ncg, G0, k0 - G0, pk, 1);
+            if (ncg > 0)
                 launch_zgemm(st, n - c0, ncg, nb, A + (size_t)k0 * lda + c0, lda,
                              A + (size_t)c0 * lda + k0, lda, A + (size_t)c0 * lda + c0, lda);
-            }
             LU_CHK(hipGetLastError());
         }
         const int nr = n - Ge;
