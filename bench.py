@@ -67,9 +67,9 @@ HBM_PEAK_GBS = 8000.0
 # factors that a streaming copy of known size, k_cvt_z2c, reads back), per
 # k_dyn_offdiag_h launch and per factorisation (every kernel of it), recorded
 # under profiles/.
-OFFDIAG_TRAFFIC_BYTES = 721.2e6
+OFFDIAG_TRAFFIC_BYTES = 735.6e6
 FACTOR_TRAFFIC_BYTES = 10.22e9
-FP64_FACTOR_TRAFFIC_BYTES = 29.32e9
+FP64_FACTOR_TRAFFIC_BYTES = 29.26e9
 # Counter figures of the same runs: FP64 VALU flops counted by
 # SQ_INSTS_VALU_FLOPS_FP64 (per wave instruction, x 64 lanes) per
 # k_dyn_offdiag_h launch, and MFMA flops (SQ_INSTS_VALU_MFMA_MOPS_* x 512)
