@@ -422,293 +422,297 @@ __global__ __launch_bounds__(256) void k_panel_fp64(cd* __restrict__ A, size_t l
     publish(0);
     const int r = tid & (PANEL_ROWS - 1), half = tid >> 7;  // update mapping: row, column parity
     if constexpr (!TALL) {
-    // Plain form (at most 128 rows, all in LDS), two barriers per column:
-    //   poll   wave 0: the slots and row d together, then the winner's row
-    //          unless it is row d, into the buffers of the step's parity
-    //          (waves 1-3 may still run the previous step's rank-1 update,
-    //          which reads the other one);
-    //   [A]    swap: row d <- pivot row; row piv <- old row d except its
-    //          columns j, j+1, which the thread of that row takes from dbuf;
-    //          thread r (< 128) scales column j of its row and updates
-    //          column j+1, and its row's |re|+|im| there is its candidate
-    //          key; a wave maximum per wave;
-    //   [E]    every thread combines the two waves' candidates; thread c of
-    //          wave 0 (wave 1) completes entry c of the candidate row (row
-    //          d+1) for columns j+2.. and publishes it with no further
-    //          barrier; then waves 1-3 do the rest of the rank-1 update, in
-    //          batches of eight independent LDS reads, ordered before the
-    //          next column by the next step's [A].
-    for (int j = 0; j < nb; ++j) {
-        const int d = k0 + j, buf = j & 1;
-        cd* __restrict__ U = urow2[buf];
-        if (wave == 0) {
-            double wv = -1.0;
-            int wr = 0x7fffffff;
-            u64 wd[4];  // row d, published a column earlier, polled with the first slots
+        // Plain form (at most 128 rows, all in LDS), two barriers per column:
+        //   poll   wave 0: the slots and row d together, then the winner's row
+        //          unless it is row d, into the buffers of the step's parity
+        //          (waves 1-3 may still run the previous step's rank-1 update,
+        //          which reads the other one);
+        //   [A]    swap: row d <- pivot row; row piv <- old row d except its
+        //          columns j, j+1, which the thread of that row takes from dbuf;
+        //          thread r (< 128) scales column j of its row and updates
+        //          column j+1, and its row's |re|+|im| there is its candidate
+        //          key; a wave maximum per wave;
+        //   [E]    every thread combines the two waves' candidates; thread c of
+        //          wave 0 (wave 1) completes entry c of the candidate row (row
+        //          d+1) for columns j+2.. and publishes it with no further
+        //          barrier; then waves 1-3 do the rest of the rank-1 update, in
+        //          batches of eight independent LDS reads, ordered before the
+        //          next column by the next step's [A].
+        for (int j = 0; j < nb; ++j) {
+            const int d = k0 + j, buf = j & 1;
+            cd* __restrict__ U = urow2[buf];
+            if (wave == 0) {
+                double wv = -1.0;
+                int wr = 0x7fffffff;
+                u64 wd[4];  // row d, published a column earlier, polled with the first slots
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int q = lane + 64 * u;
-                if (64 * u >= P) break;  // wave-uniform
-                u64 w3[3];
-                if (u == 0)
-                    tpoll_pair<3, 4>(slots + ((size_t)j * P + min(q, P - 1)) * 3,
-                                     drow + ((size_t)j * NB + min(lane, nb - 1)) * 4, tag, w3, wd,
-                                     q < P, lane < nb, status);
-                else
-                    tpoll<3>(slots + ((size_t)j * P + min(q, P - 1)) * 3, tag, w3, q < P, status);
-                const double cv = tval_d(w3[0], w3[1]);
-                const int cr = (int)(unsigned)w3[2];
-                if (q < P && better(cv, cr, wv, wr)) {
-                    wv = cv;
-                    wr = cr;
+                for (int u = 0; u < 4; ++u) {
+                    const int q = lane + 64 * u;
+                    if (64 * u >= P) break;  // wave-uniform
+                    u64 w3[3];
+                    if (u == 0)
+                        tpoll_pair<3, 4>(slots + ((size_t)j * P + min(q, P - 1)) * 3,
+                                         drow + ((size_t)j * NB + min(lane, nb - 1)) * 4, tag,
+                                         w3, wd, q < P, lane < nb, status);
+                    else
+                        tpoll<3>(slots + ((size_t)j * P + min(q, P - 1)) * 3, tag, w3, q < P,
+                                 status);
+                    const double cv = tval_d(w3[0], w3[1]);
+                    const int cr = (int)(unsigned)w3[2];
+                    if (q < P && better(cv, cr, wv, wr)) {
+                        wv = cv;
+                        wr = cr;
+                    }
+                }
+                // the largest value, then the lowest row holding it; workgroup
+                // q owns the rows from k0 + q rpw on
+                const unsigned long long kmax = wave_max_u64(cand_key(wv));
+                wr = (int)wave_min_u32(cand_key(wv) == kmax ? (unsigned)wr : 0xffffffffu);
+                wv = key_val(kmax);
+                const int wp = (wr - k0) / rpw;
+                // the winner's row: row d itself when the diagonal wins (no
+                // second round trip; its owner published it both as its
+                // candidate and as row d, from the same values), else polled
+                const cd rd = mk(tval_d(wd[0], wd[1]), tval_d(wd[2], wd[3]));
+                cd ru = rd;
+                if (wr != d) {
+                    u64 w4[4];
+                    tpoll<4>(cand + (((size_t)j * P + wp) * NB + min(lane, nb - 1)) * 4, tag, w4,
+                             lane < nb, status);
+                    ru = mk(tval_d(w4[0], w4[1]), tval_d(w4[2], w4[3]));
+                }
+                if (lane < nb) {
+                    U[lane] = ru;
+                    dbuf[lane] = rd;
+                }
+                if (lane == 0) {
+                    s_piv2[buf] = wr;
+                    s_pval2[buf] = wv;
                 }
             }
-            // the largest value, then the lowest row holding it; workgroup
-            // q owns the rows from k0 + q rpw on
-            const unsigned long long kmax = wave_max_u64(cand_key(wv));
-            wr = (int)wave_min_u32(cand_key(wv) == kmax ? (unsigned)wr : 0xffffffffu);
-            wv = key_val(kmax);
-            const int wp = (wr - k0) / rpw;
-            // the winner's row: row d itself when the diagonal wins (no
-            // second round trip; its owner published it both as its
-            // candidate and as row d, from the same values), else polled
-            const cd rd = mk(tval_d(wd[0], wd[1]), tval_d(wd[2], wd[3]));
-            cd ru = rd;
-            if (wr != d) {
+            __syncthreads();  // [A]
+            const int piv = s_piv2[buf];
+            const double pval = s_pval2[buf];
+            if (tid == 0) s_pivs[j] = piv;
+            if (p == 0 && tid == 0) {
+                ipiv[d] = piv;
+                if (pval == 0.0 && *info == 0) *info = d + 1;
+            }
+            if (piv != d && tid < nb) {
+                if (d >= r0 && d < r1) sA[tid * PANEL_ROWS + d - r0] = U[tid];
+                if (piv >= r0 && piv < r1 && tid != j && tid != j + 1)
+                    sA[tid * PANEL_ROWS + piv - r0] = dbuf[tid];
+            }
+            const bool more = j + 1 < nb;
+            double bv = -1.0;
+            int br = 0x7fffffff;
+            if (tid < nr && r0 + tid > d) {
+                const bool moved = r0 + tid == piv && piv != d;
+                cd aj = moved ? dbuf[j] : sA[j * PANEL_ROWS + tid];
+                cd aj1 = more ? (moved ? dbuf[j + 1] : sA[(j + 1) * PANEL_ROWS + tid])
+                              : mk(0.0, 0.0);
+                if (pval != 0.0) {
+                    aj = panel_scale(aj, U[j]);
+                    if (more) aj1 = cfms(aj1, aj, U[j + 1]);
+                }
+                sA[j * PANEL_ROWS + tid] = aj;
+                if (more) {
+                    sA[(j + 1) * PANEL_ROWS + tid] = aj1;
+                    bv = cabs1(aj1);
+                    br = r0 + tid;
+                }
+            }
+            if (wave < 2) {
+                // lane order is row order: the lowest lane holding the largest
+                // value is the lowest row
+                const unsigned long long k = cand_key(bv), kmax = wave_max_u64(k);
+                const unsigned long long tie = __ballot(k == kmax);
+                if (lane == 0) {
+                    rv[wave] = key_val(kmax);
+                    rr[wave] = kmax ? r0 + 64 * wave + __ffsll((long long)tie) - 1 : 0x7fffffff;
+                }
+            }
+            __syncthreads();  // [E]
+            int sp1 = -1, sp2 = -1;  // local rows already completed for columns j+2..
+            if (more) {
+                double cv = rv[0];
+                int cr = rr[0];
+                if (better(rv[1], rr[1], cv, cr)) {
+                    cv = rv[1];
+                    cr = rr[1];
+                }
+                if (cr != 0x7fffffff) sp1 = cr - r0;
+                const bool own_d1 = d + 1 >= r0 && d + 1 < r1;
+                if (own_d1 && d + 1 != cr) sp2 = d + 1 - r0;
+                // wave 0: the candidate row (and row d+1 when it is the
+                // candidate); wave 1: row d+1 otherwise; thread c of the wave
+                // owns entry c
+                const int sr = wave == 0 ? sp1 : (wave == 1 ? sp2 : -1);
+                const int c = lane;
+                if (sr >= 0 && c < nb) {
+                    cd v = sA[c * PANEL_ROWS + sr];
+                    if (c >= j + 2 && pval != 0.0) {
+                        v = cfms(v, sA[j * PANEL_ROWS + sr], U[c]);
+                        sA[c * PANEL_ROWS + sr] = v;
+                    }
+                    if (wave == 0) {
+                        u64* q = cand + (((size_t)(j + 1) * P + p) * NB + c) * 4;
+                        tput_d(q, tag, v.x);
+                        tput_d(q + 2, tag, v.y);
+                    }
+                    if (wave == 1 || (own_d1 && d + 1 == cr)) {
+                        u64* q = drow + ((size_t)(j + 1) * NB + c) * 4;
+                        tput_d(q, tag, v.x);
+                        tput_d(q + 2, tag, v.y);
+                    }
+                }
+                if (tid == 0) {
+                    u64* q = slots + ((size_t)(j + 1) * P + p) * 3;
+                    tput_d(q, tag, cv);
+                    tput(q + 2, tag, (unsigned)cr);
+                }
+            }
+            // the rest of the rank-1 update (columns j+2..) by waves 1-3, while
+            // wave 0 goes on to the next column's polls: item (row, third) of
+            // the 128 x 3 items, the third q taking the columns j+2+q, step 3
+            if (wave > 0 && pval != 0.0) {
+#pragma unroll
+                for (int it = 0; it < 2; ++it) {
+                    const int item = tid - 64 + 192 * it, rw = item & (PANEL_ROWS - 1);
+                    if (rw < nr && r0 + rw > d && rw != sp1 && rw != sp2) {
+                        const cd l = sA[j * PANEL_ROWS + rw];
+                        int c = j + 2 + (item >> 7);
+                        for (; c + 21 < nb; c += 24) {
+                            cd v[8], u[8];
+#pragma unroll
+                            for (int k = 0; k < 8; ++k) {
+                                v[k] = sA[(c + 3 * k) * PANEL_ROWS + rw];
+                                u[k] = U[c + 3 * k];
+                            }
+#pragma unroll
+                            for (int k = 0; k < 8; ++k)
+                                sA[(c + 3 * k) * PANEL_ROWS + rw] = cfms(v[k], l, u[k]);
+                        }
+                        for (; c < nb; c += 3)
+                            sA[c * PANEL_ROWS + rw] = cfms(sA[c * PANEL_ROWS + rw], l, U[c]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    } else {
+        for (int j = 0; j < nb; ++j) {
+            const int d = k0 + j;
+            if (wave == 0) {
+                // the P slots (lane q: slots q, q + 64, ...; P <= 4 * 64), then
+                // the winner's row
+                double wv = -1.0;
+                int wr = 0x7fffffff, wp = 0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int q = lane + 64 * u;
+                    if (64 * u >= P) break;  // wave-uniform
+                    u64 w3[3];
+                    tpoll<3>(slots + ((size_t)j * P + min(q, P - 1)) * 3, tag, w3, q < P, status);
+                    const double cv = tval_d(w3[0], w3[1]);
+                    const int cr = (int)(unsigned)w3[2];
+                    if (q < P && better(cv, cr, wv, wr)) {
+                        wv = cv;
+                        wr = cr;
+                        wp = q;
+                    }
+                }
+                for (int off = 32; off >= 1; off >>= 1) {
+                    double ov = __shfl_xor(wv, off, 64);
+                    int orr = __shfl_xor(wr, off, 64);
+                    int op = __shfl_xor(wp, off, 64);
+                    if (better(ov, orr, wv, wr)) {
+                        wv = ov;
+                        wr = orr;
+                        wp = op;
+                    }
+                }
                 u64 w4[4];
                 tpoll<4>(cand + (((size_t)j * P + wp) * NB + min(lane, nb - 1)) * 4, tag, w4,
                          lane < nb, status);
-                ru = mk(tval_d(w4[0], w4[1]), tval_d(w4[2], w4[3]));
-            }
-            if (lane < nb) {
-                U[lane] = ru;
-                dbuf[lane] = rd;
-            }
-            if (lane == 0) {
-                s_piv2[buf] = wr;
-                s_pval2[buf] = wv;
-            }
-        }
-        __syncthreads();  // [A]
-        const int piv = s_piv2[buf];
-        const double pval = s_pval2[buf];
-        if (tid == 0) s_pivs[j] = piv;
-        if (p == 0 && tid == 0) {
-            ipiv[d] = piv;
-            if (pval == 0.0 && *info == 0) *info = d + 1;
-        }
-        if (piv != d && tid < nb) {
-            if (d >= r0 && d < r1) sA[tid * PANEL_ROWS + d - r0] = U[tid];
-            if (piv >= r0 && piv < r1 && tid != j && tid != j + 1)
-                sA[tid * PANEL_ROWS + piv - r0] = dbuf[tid];
-        }
-        const bool more = j + 1 < nb;
-        double bv = -1.0;
-        int br = 0x7fffffff;
-        if (tid < nr && r0 + tid > d) {
-            const bool moved = r0 + tid == piv && piv != d;
-            cd aj = moved ? dbuf[j] : sA[j * PANEL_ROWS + tid];
-            cd aj1 = more ? (moved ? dbuf[j + 1] : sA[(j + 1) * PANEL_ROWS + tid]) : mk(0.0, 0.0);
-            if (pval != 0.0) {
-                aj = panel_scale(aj, U[j]);
-                if (more) aj1 = cfms(aj1, aj, U[j + 1]);
-            }
-            sA[j * PANEL_ROWS + tid] = aj;
-            if (more) {
-                sA[(j + 1) * PANEL_ROWS + tid] = aj1;
-                bv = cabs1(aj1);
-                br = r0 + tid;
-            }
-        }
-        if (wave < 2) {
-            // lane order is row order: the lowest lane holding the largest
-            // value is the lowest row
-            const unsigned long long k = cand_key(bv), kmax = wave_max_u64(k);
-            const unsigned long long tie = __ballot(k == kmax);
-            if (lane == 0) {
-                rv[wave] = key_val(kmax);
-                rr[wave] = kmax ? r0 + 64 * wave + __ffsll((long long)tie) - 1 : 0x7fffffff;
-            }
-        }
-        __syncthreads();  // [E]
-        int sp1 = -1, sp2 = -1;  // local rows already completed for columns j+2..
-        if (more) {
-            double cv = rv[0];
-            int cr = rr[0];
-            if (better(rv[1], rr[1], cv, cr)) {
-                cv = rv[1];
-                cr = rr[1];
-            }
-            if (cr != 0x7fffffff) sp1 = cr - r0;
-            const bool own_d1 = d + 1 >= r0 && d + 1 < r1;
-            if (own_d1 && d + 1 != cr) sp2 = d + 1 - r0;
-            // wave 0: the candidate row (and row d+1 when it is the
-            // candidate); wave 1: row d+1 otherwise; thread c of the wave
-            // owns entry c
-            const int sr = wave == 0 ? sp1 : (wave == 1 ? sp2 : -1);
-            const int c = lane;
-            if (sr >= 0 && c < nb) {
-                cd v = sA[c * PANEL_ROWS + sr];
-                if (c >= j + 2 && pval != 0.0) {
-                    v = cfms(v, sA[j * PANEL_ROWS + sr], U[c]);
-                    sA[c * PANEL_ROWS + sr] = v;
+                if (lane < nb) urow[lane] = mk(tval_d(w4[0], w4[1]), tval_d(w4[2], w4[3]));
+                if (lane == 0) {
+                    s_piv = wr;
+                    s_pval = wv;
                 }
-                if (wave == 0) {
-                    u64* q = cand + (((size_t)(j + 1) * P + p) * NB + c) * 4;
-                    tput_d(q, tag, v.x);
-                    tput_d(q + 2, tag, v.y);
-                }
-                if (wave == 1 || (own_d1 && d + 1 == cr)) {
-                    u64* q = drow + ((size_t)(j + 1) * NB + c) * 4;
-                    tput_d(q, tag, v.x);
-                    tput_d(q + 2, tag, v.y);
-                }
+            } else if (wave == 1) {
+                u64 w4[4];
+                tpoll<4>(drow + ((size_t)j * NB + min(lane, nb - 1)) * 4, tag, w4, lane < nb,
+                         status);
+                if (lane < nb) dbuf[lane] = mk(tval_d(w4[0], w4[1]), tval_d(w4[2], w4[3]));
             }
-            if (tid == 0) {
-                u64* q = slots + ((size_t)(j + 1) * P + p) * 3;
-                tput_d(q, tag, cv);
-                tput(q + 2, tag, (unsigned)cr);
-            }
-        }
-        // the rest of the rank-1 update (columns j+2..) by waves 1-3, while
-        // wave 0 goes on to the next column's polls: item (row, third) of
-        // the 128 x 3 items, the third q taking the columns j+2+q, step 3
-        if (wave > 0 && pval != 0.0) {
-#pragma unroll
-            for (int it = 0; it < 2; ++it) {
-                const int item = tid - 64 + 192 * it, rw = item & (PANEL_ROWS - 1);
-                if (rw < nr && r0 + rw > d && rw != sp1 && rw != sp2) {
-                    const cd l = sA[j * PANEL_ROWS + rw];
-                    int c = j + 2 + (item >> 7);
-                    for (; c + 21 < nb; c += 24) {
-                        cd v[8], u[8];
-#pragma unroll
-                        for (int k = 0; k < 8; ++k) {
-                            v[k] = sA[(c + 3 * k) * PANEL_ROWS + rw];
-                            u[k] = U[c + 3 * k];
-                        }
-#pragma unroll
-                        for (int k = 0; k < 8; ++k)
-                            sA[(c + 3 * k) * PANEL_ROWS + rw] = cfms(v[k], l, u[k]);
-                    }
-                    for (; c < nb; c += 3)
-                        sA[c * PANEL_ROWS + rw] = cfms(sA[c * PANEL_ROWS + rw], l, U[c]);
-                }
-            }
-        }
-    }
-    __syncthreads();
-    } else {
-    for (int j = 0; j < nb; ++j) {
-        const int d = k0 + j;
-        if (wave == 0) {
-            // the P slots (lane q: slots q, q + 64, ...; P <= 4 * 64), then
-            // the winner's row
-            double wv = -1.0;
-            int wr = 0x7fffffff, wp = 0;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int q = lane + 64 * u;
-                if (64 * u >= P) break;  // wave-uniform
-                u64 w3[3];
-                tpoll<3>(slots + ((size_t)j * P + min(q, P - 1)) * 3, tag, w3, q < P, status);
-                const double cv = tval_d(w3[0], w3[1]);
-                const int cr = (int)(unsigned)w3[2];
-                if (q < P && better(cv, cr, wv, wr)) {
-                    wv = cv;
-                    wr = cr;
-                    wp = q;
-                }
-            }
-            for (int off = 32; off >= 1; off >>= 1) {
-                double ov = __shfl_xor(wv, off, 64);
-                int orr = __shfl_xor(wr, off, 64);
-                int op = __shfl_xor(wp, off, 64);
-                if (better(ov, orr, wv, wr)) {
-                    wv = ov;
-                    wr = orr;
-                    wp = op;
-                }
-            }
-            u64 w4[4];
-            tpoll<4>(cand + (((size_t)j * P + wp) * NB + min(lane, nb - 1)) * 4, tag, w4, lane < nb,
-                     status);
-            if (lane < nb) urow[lane] = mk(tval_d(w4[0], w4[1]), tval_d(w4[2], w4[3]));
-            if (lane == 0) {
-                s_piv = wr;
-                s_pval = wv;
-            }
-        } else if (wave == 1) {
-            u64 w4[4];
-            tpoll<4>(drow + ((size_t)j * NB + min(lane, nb - 1)) * 4, tag, w4, lane < nb, status);
-            if (lane < nb) dbuf[lane] = mk(tval_d(w4[0], w4[1]), tval_d(w4[2], w4[3]));
-        }
-        __syncthreads();
-        const int piv = s_piv;
-        const double pval = s_pval;
-        if (tid == 0) s_pivs[j] = piv;
-        if (p == 0 && tid == 0) {
-            ipiv[d] = piv;
-            if (pval == 0.0 && *info == 0) *info = d + 1;
-        }
-        __syncthreads();
-        // apply: swap, scale column j, update column j+1
-        if (piv != d) {
-            if (d >= r0 && d < r1 && tid < nb) el(tid, d - r0) = urow[tid];
-            if (piv >= r0 && piv < r1 && tid < nb) el(tid, piv - r0) = dbuf[tid];
-        }
-        __syncthreads();
-
-        for (int ri = tid; ri < (TALL ? nr : min(nr, 256)); ri += 256) {
-            if (r0 + ri > d && pval != 0.0) {
-                const cd l = panel_scale(el(j, ri), urow[j]);
-                el(j, ri) = l;
-                if (j + 1 < nb) el(j + 1, ri) = cfms(el(j + 1, ri), l, urow[j + 1]);
-            }
-        }
-        __syncthreads();
-        int sp1 = -1, sp2 = -1;  // local rows already completed for columns j+2..
-        if (j + 1 < nb) {
-            local_cand(j + 1, d + 1);
-            const int cr = s_cand;
-            if (cr != 0x7fffffff) sp1 = cr - r0;
-            if (d + 1 >= r0 && d + 1 < r1 && d + 1 != cr) sp2 = d + 1 - r0;
-            // complete the candidate row (threads 0..63) and row d+1 (64..127)
-            const int sr = tid < 64 ? sp1 : (tid < 128 ? sp2 : -1);
-            const int c = j + 2 + lane;
-            if (sr >= 0 && c < nb && pval != 0.0) el(c, sr) = cfms(el(c, sr), el(j, sr), urow[c]);
             __syncthreads();
-            publish(j + 1);
-        }
-        // the rest of the rank-1 update (columns j+2..), overlapping the exchange
-        if (TALL) {
-            // one thread per row; a row's entries in the matrix are loaded
-            // eight at a time so their latencies overlap
-            for (int rr = tid; rr < nr; rr += 256) {
-                if (r0 + rr > d && rr != sp1 && rr != sp2 && pval != 0.0) {
-                    const cd l = el(j, rr);
-                    int c = j + 2;
-                    if (rr >= PANEL_ROWS) {
-                        cd* col = &A[(size_t)k0 * lda + r0 + rr];
-                        for (; c + 8 <= nb; c += 8) {
-                            cd v[8];
-#pragma unroll
-                            for (int u = 0; u < 8; ++u) v[u] = col[(size_t)(c + u) * lda];
-#pragma unroll
-                            for (int u = 0; u < 8; ++u)
-                                col[(size_t)(c + u) * lda] = cfms(v[u], l, urow[c + u]);
-                        }
-                    }
-                    for (; c < nb; ++c) el(c, rr) = cfms(el(c, rr), l, urow[c]);
+            const int piv = s_piv;
+            const double pval = s_pval;
+            if (tid == 0) s_pivs[j] = piv;
+            if (p == 0 && tid == 0) {
+                ipiv[d] = piv;
+                if (pval == 0.0 && *info == 0) *info = d + 1;
+            }
+            __syncthreads();
+            // apply: swap, scale column j, update column j+1
+            if (piv != d) {
+                if (d >= r0 && d < r1 && tid < nb) el(tid, d - r0) = urow[tid];
+                if (piv >= r0 && piv < r1 && tid < nb) el(tid, piv - r0) = dbuf[tid];
+            }
+            __syncthreads();
+
+            for (int ri = tid; ri < (TALL ? nr : min(nr, 256)); ri += 256) {
+                if (r0 + ri > d && pval != 0.0) {
+                    const cd l = panel_scale(el(j, ri), urow[j]);
+                    el(j, ri) = l;
+                    if (j + 1 < nb) el(j + 1, ri) = cfms(el(j + 1, ri), l, urow[j + 1]);
                 }
             }
-        } else if (r < nr && r0 + r > d && r != sp1 && r != sp2 && pval != 0.0) {
-            const cd l = sA[j * PANEL_ROWS + r];
-            for (int c = j + 2 + half; c < nb; c += 2)
-                sA[c * PANEL_ROWS + r] = cfms(sA[c * PANEL_ROWS + r], l, urow[c]);
+            __syncthreads();
+            int sp1 = -1, sp2 = -1;  // local rows already completed for columns j+2..
+            if (j + 1 < nb) {
+                local_cand(j + 1, d + 1);
+                const int cr = s_cand;
+                if (cr != 0x7fffffff) sp1 = cr - r0;
+                if (d + 1 >= r0 && d + 1 < r1 && d + 1 != cr) sp2 = d + 1 - r0;
+                // complete the candidate row (threads 0..63) and row d+1 (64..127)
+                const int sr = tid < 64 ? sp1 : (tid < 128 ? sp2 : -1);
+                const int c = j + 2 + lane;
+                if (sr >= 0 && c < nb && pval != 0.0)
+                    el(c, sr) = cfms(el(c, sr), el(j, sr), urow[c]);
+                __syncthreads();
+                publish(j + 1);
+            }
+            // the rest of the rank-1 update (columns j+2..), overlapping the exchange
+            if (TALL) {
+                // one thread per row; a row's entries in the matrix are loaded
+                // eight at a time so their latencies overlap
+                for (int rr = tid; rr < nr; rr += 256) {
+                    if (r0 + rr > d && rr != sp1 && rr != sp2 && pval != 0.0) {
+                        const cd l = el(j, rr);
+                        int c = j + 2;
+                        if (rr >= PANEL_ROWS) {
+                            cd* col = &A[(size_t)k0 * lda + r0 + rr];
+                            for (; c + 8 <= nb; c += 8) {
+                                cd v[8];
+#pragma unroll
+                                for (int u = 0; u < 8; ++u) v[u] = col[(size_t)(c + u) * lda];
+#pragma unroll
+                                for (int u = 0; u < 8; ++u)
+                                    col[(size_t)(c + u) * lda] = cfms(v[u], l, urow[c + u]);
+                            }
+                        }
+                        for (; c < nb; ++c) el(c, rr) = cfms(el(c, rr), l, urow[c]);
+                    }
+                }
+            } else if (r < nr && r0 + r > d && r != sp1 && r != sp2 && pval != 0.0) {
+                const cd l = sA[j * PANEL_ROWS + r];
+                for (int c = j + 2 + half; c < nb; c += 2)
+                    sA[c * PANEL_ROWS + r] = cfms(sA[c * PANEL_ROWS + r], l, urow[c]);
+            }
+            __syncthreads();
         }
-        __syncthreads();
-    }
     }
     for (int t = tid; t < nl * nb; t += 256) {
         int c = t / nl, r2 = t - c * nl;
