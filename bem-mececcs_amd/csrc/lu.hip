@@ -918,6 +918,90 @@ __global__ __launch_bounds__(256) void k_zgemm_mfma(int m, int nc, int K, const 
         }
 }
 
+// The same update for K <= 64 with every load issued at once: the 64 x K
+// block of A, the K x 64 block of B and the C tile (48 loads per thread in
+// flight), A and B into LDS (the whole K), one barrier, then the MFMAs in
+// the order k_zgemm_mfma issues them (the same sums, rounded alike).  The
+// small updates on the factorisation's chain (inside a group, K = 64) are
+// load-latency bound in the chunked form.
+constexpr int G64K = 64;
+__global__ __launch_bounds__(256) void k_zgemm64(int m, int nc, int K, const cd* __restrict__ A,
+                                                 size_t lda, const cd* __restrict__ B, size_t ldb,
+                                                 cd* __restrict__ C, size_t ldc) {
+    __shared__ double sAr[G64K][GB + 1], sAi[G64K][GB + 1];
+    __shared__ double sBr[G64K][GB + 1], sBi[G64K][GB + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int bm = blockIdx.x * GB, bn = blockIdx.y * GB;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    cd av[16], bv[16], cv[16];
+    // staging: entry t = tid + 256 u is (k = t / 64, i = t % 64) of A and
+    // (j = t / 64, k = t % 64) of B
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int t = tid + 256 * u, k = t >> 6, i = t & 63;
+        av[u] = (k < K && bm + i < m) ? A[(size_t)k * lda + bm + i] : mk(0.0, 0.0);
+        bv[u] = (i < K && bn + k < nc) ? B[(size_t)(bn + k) * ldb + i] : mk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            const int col = bn + wn + 16 * bj + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = bm + wm + 16 * bi + 4 * r + lk;
+                cv[(bi * 2 + bj) * 4 + r] =
+                    (row < m && col < nc) ? C[(size_t)col * ldc + row] : mk(0.0, 0.0);
+            }
+        }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int t = tid + 256 * u, k = t >> 6, i = t & 63;
+        sAr[k][i] = av[u].x;
+        sAi[k][i] = av[u].y;
+        sBr[i][k] = bv[u].x;
+        sBi[i][k] = bv[u].y;
+    }
+    v4d accR[2][2], accI[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                accR[bi][bj][r] = cv[(bi * 2 + bj) * 4 + r].x;
+                accI[bi][bj][r] = cv[(bi * 2 + bj) * 4 + r].y;
+            }
+    __syncthreads();
+    for (int k = 0; k < K; k += 4) {
+        const int kk = k + lk;
+        double ar[2], ai[2], br[2], bi_[2];
+        for (int b = 0; b < 2; ++b) {
+            ar[b] = sAr[kk][wm + 16 * b + li];
+            ai[b] = sAi[kk][wm + 16 * b + li];
+            br[b] = sBr[kk][wn + 16 * b + li];
+            bi_[b] = sBi[kk][wn + 16 * b + li];
+        }
+        for (int x = 0; x < 2; ++x)
+            for (int y = 0; y < 2; ++y) {
+                accR[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[x], br[y], accR[x][y], 0, 0, 0);
+                accR[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], bi_[y], accR[x][y], 0, 0, 0);
+                accI[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[x], bi_[y], accI[x][y], 0, 0, 0);
+                accI[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[x], br[y], accI[x][y], 0, 0, 0);
+            }
+    }
+    for (int bi = 0; bi < 2; ++bi)
+        for (int bj = 0; bj < 2; ++bj) {
+            const int col = bn + wn + 16 * bj + li;
+            for (int r = 0; r < 4; ++r) {
+                const int row = bm + wm + 16 * bi + 4 * r + lk;
+                if (row < m && col < nc)
+                    C[(size_t)col * ldc + row] = mk(accR[bi][bj][r], accI[bi][bj][r]);
+            }
+        }
+}
+
 // Trailing update, second form: C[m x nc] -= A[m x K] B[K x nc] (K <= NB)
 // with 128 x 128 workgroup tiles.  512 threads = 8 waves of 64 x 32 (4 x 2
 // MFMA blocks of 16 x 16), the product accumulated from zero and C read and
@@ -1331,6 +1415,9 @@ static void launch_zgemm(hipStream_t st, int m, int nc, int K, const cd* A, size
         const int wgs = cap > 0 ? std::min(per * ZXCD, cap) : per * ZXCD;
         hipLaunchKernelGGL(k_zgemm_mfma2, dim3(wgs), dim3(512), 0, st, m, nc, K, A, lda, B, ldb,
                            C, ldc);
+    } else if (K <= G64K) {
+        hipLaunchKernelGGL(k_zgemm64, dim3((m + GB - 1) / GB, (nc + GB - 1) / GB), dim3(256), 0,
+                           st, m, nc, K, A, lda, B, ldb, C, ldc);
     } else {
         hipLaunchKernelGGL(k_zgemm_mfma, dim3((m + GB - 1) / GB, (nc + GB - 1) / GB), dim3(256), 0,
                            st, m, nc, K, A, lda, B, ldb, C, ldc);
