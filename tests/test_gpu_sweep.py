@@ -7,8 +7,8 @@ on a second stream while the previous batch is solved, and the batch's
 mixed solves share every launch of the factorisation and the refinement.
 Each frequency's arithmetic is that of bemx_solve, so every solution must be
 bit-identical to solving that frequency alone: five frequencies (one batch)
-in both modes, and eleven (a batch of eight, then one of three assembled
-beside it), down to omega = 0.1 where the near-field series kernel carries
+in both modes, and eleven (batches of six and five, the second assembled
+beside the first one's solve), down to omega = 0.1 where the near-field series kernel carries
 most of the element pairs near the diagonal, with two load cases.
 """
 import numpy as np
