@@ -388,13 +388,13 @@ class Server:
         return rid.value
 
     def stats(self) -> dict:
-        out = np.zeros(9)
-        code = self._lib.bemx_server_stats(self.h, _p(out), 9)
+        out = np.zeros(10)
+        code = self._lib.bemx_server_stats(self.h, _p(out), 10)
         if code:
             raise BemxError(code, "bemx_server_stats")
         keys = ("completed", "graph_captures", "graph_replays", "charged_bytes",
                 "peak_charged_bytes", "budget_bytes", "workspace_releases", "over_charge",
-                "batches")
+                "batches", "graph_precaptures")
         return {k: (int(v) if k != "budget_bytes" else float(v)) for k, v in zip(keys, out)}
 
     def wait(self, rid: int):

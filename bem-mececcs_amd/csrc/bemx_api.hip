@@ -280,10 +280,14 @@ size_t ctx_solve_bytes_bound(int n, int nbe, int np, int npg, int nrhs, int mode
     return b;
 }
 
-void ctx_graph_counts(const bemx_ctx* c, long long* captures, long long* replays) {
+void ctx_graph_counts(const bemx_ctx* c, long long* captures, long long* replays,
+                      long long* precaptures) {
     *captures = c->lu.captures;
     *replays = c->lu.replays;
+    *precaptures = c->lu.precaptures;
 }
+
+void ctx_capture_ahead(bemx_ctx* c, int ahead) { c->lu.ahead = ahead; }
 
 }  // namespace bemx
 
@@ -885,6 +889,9 @@ int ctx_solve_items(bemx_ctx* c, int nitem, SolveItem* items, int mode,
     // batches as even as the count allows (20 problems in batches of at most
     // 8: 7, 7 and 6 rather than 8, 8 and 4)
     nbat = (nitem + (nitem + nbat - 1) / nbat - 1) / ((nitem + nbat - 1) / nbat);
+    // the first factorisation of this order also captures the graphs of the
+    // other batch sizes a later call may run
+    c->lu.ahead = sweep_batch();
     const int nslot = std::min(nitem, halves * nbat);
     HIPCHK(c, c->colflag.ensure(sizeof(int) * nbe));
     HIPCHK(c, c->flist.ensure(sizeof(int) * (nbe + 1)));

@@ -38,7 +38,7 @@ struct LuWork {
     // Captured factorisation graphs, one per size class and batch size:
     // keyed by the order, the workspace generation and the batch size (and
     // unused zeros); least recently used entries are evicted beyond GRAPHS.
-    static constexpr int GRAPHS = 32, KEYS = 6;
+    static constexpr int GRAPHS = 64, KEYS = 6;
     struct GraphEntry {
         const void* key[KEYS] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
         hipGraphExec_t exec = nullptr;
@@ -47,6 +47,12 @@ struct LuWork {
     GraphEntry graphs[GRAPHS];
     unsigned long long tick = 0;
     long long captures = 0, replays = 0;  // graph statistics since creation
+    // Batch sizes 1 .. ahead of an order are captured together at the
+    // order's first capture (those the arena holds; 1: only the size asked
+    // for); precaptures counts the graphs captured that way.  A pre-captured
+    // graph's first launch counts as a replay.
+    int ahead = 1;
+    long long precaptures = 0;
     bool use_graphs = true;
     hipGraphExec_t find_graph(const void* const key[KEYS]);
     // takes ownership of exec; evicts the least recently used entry if full

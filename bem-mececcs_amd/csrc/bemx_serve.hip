@@ -53,7 +53,9 @@ int ctx_solve_items(bemx_ctx* c, int nitem, SolveItem* items, int mode,
 int batch_size();
 size_t ctx_device_bytes(bemx_ctx* c);
 void ctx_release_workspace(bemx_ctx* c);
-void ctx_graph_counts(const bemx_ctx* c, long long* captures, long long* replays);
+void ctx_graph_counts(const bemx_ctx* c, long long* captures, long long* replays,
+                      long long* precaptures);
+void ctx_capture_ahead(bemx_ctx* c, int ahead);
 }  // namespace bemx
 
 namespace {
@@ -149,8 +151,8 @@ struct bemx_server {
     std::vector<Slot> slots;
     // statistics
     long long completed = 0, releases = 0, over_charge = 0, batches = 0;
-    long long captures = 0, replays = 0;
-    std::vector<long long> slot_captures, slot_replays;
+    long long captures = 0, replays = 0, precaptures = 0;
+    std::vector<long long> slot_captures, slot_replays, slot_precaptures;
     std::string err;
 };
 
@@ -244,15 +246,17 @@ void run_batch(bemx_server* s, int slot, bemx_ctx* c, uint64_t* slot_key,
     }
 }
 
-// Adds the graph captures and replays of the slot's context since the last
-// count to the server's statistics (the server's lock held).
+// Adds the graph captures, replays and captures ahead of the slot's context
+// since the last count to the server's statistics (the server's lock held).
 void count_graphs(bemx_server* s, bemx_ctx* c, int slot) {
-    long long cap = 0, rep = 0;
-    bemx::ctx_graph_counts(c, &cap, &rep);
+    long long cap = 0, rep = 0, pre = 0;
+    bemx::ctx_graph_counts(c, &cap, &rep, &pre);
     s->captures += cap - s->slot_captures[slot];
     s->replays += rep - s->slot_replays[slot];
+    s->precaptures += pre - s->slot_precaptures[slot];
     s->slot_captures[slot] = cap;
     s->slot_replays[slot] = rep;
+    s->slot_precaptures[slot] = pre;
 }
 
 // Hands request r back to its waiter (the server's lock held).
@@ -408,12 +412,16 @@ int bemx_server_create(int device, int nslots, double hbm_budget_bytes, int mode
             delete s;
             return st;
         }
+        // a slot's first solve of a mesh size, batched or not, captures the
+        // factorisation graphs of every batch size up to the batch limit
+        bemx::ctx_capture_ahead(c, s->batch);
         Slot sl;
         sl.ctx = c;
         s->slots.push_back(sl);
     }
     s->slot_captures.assign(nslots, 0);
     s->slot_replays.assign(nslots, 0);
+    s->slot_precaptures.assign(nslots, 0);
     for (int i = 0; i < nslots; ++i) s->workers.emplace_back(worker_main, s, i);
     *out = s;
     return BEMX_OK;
@@ -508,7 +516,7 @@ int bemx_server_wait(bemx_server* s, long long id, double* x, double* info, char
 
 int bemx_server_stats(bemx_server* s, double* out, int nout) {
     if (!s || !out || nout < 1) return BEMX_ERR_ARG;
-    double v[9];
+    double v[10];
     {
         std::lock_guard<std::mutex> lk(s->mu);
         v[8] = (double)s->batches;
@@ -520,8 +528,9 @@ int bemx_server_stats(bemx_server* s, double* out, int nout) {
         v[5] = (double)s->budget;
         v[6] = (double)s->releases;
         v[7] = (double)s->over_charge;
+        v[9] = (double)s->precaptures;
     }
-    for (int i = 0; i < nout && i < 9; ++i) out[i] = v[i];
+    for (int i = 0; i < nout && i < 10; ++i) out[i] = v[i];
     return BEMX_OK;
 }
 

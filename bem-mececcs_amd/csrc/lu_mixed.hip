@@ -1791,7 +1791,12 @@ static ArenaLayout arena_layout(int n) {
     return L;
 }
 
-static size_t rrec_bytes(int nz) { return std::max<size_t>(sizeof(ReadRec) * nz, 256); }
+// The host-read records are sized for the largest batch whatever the batch,
+// so that a larger batch later never reallocates them (a reallocation drops
+// every captured graph of the workspace).
+static size_t rrec_bytes(int) {
+    return std::max<size_t>(sizeof(ReadRec) * MIXED_MAX_BATCH, 256);
+}
 
 size_t mixed_workspace_bytes(int n, int nz) {
     return (size_t)nz * arena_layout(n).bytes + rrec_bytes(nz);
@@ -1995,27 +2000,60 @@ static int factor_mixed(LuWork& w, hipStream_t st, int n, const cd* A64, size_t 
     MX_CHK(hipEventRecord(w.ev_f0, st));
     enqueue_convert(st, n, A64, lda, sa, F);
     if (w.use_graphs) {
-        const void* key[LuWork::KEYS] = {(const void*)(size_t)n, (const void*)(size_t)w.gen,
-                                         (const void*)(size_t)nz, nullptr, nullptr, nullptr};
-        hipGraphExec_t ex = w.find_graph(key);
-        if (!ex) {
+        auto key_of = [&](int z, const void** key) {
+            key[0] = (const void*)(size_t)n;
+            key[1] = (const void*)(size_t)w.gen;
+            key[2] = (const void*)(size_t)z;
+            for (int i = 3; i < LuWork::KEYS; ++i) key[i] = nullptr;
+        };
+        // captures the factorisation of batch size Fc.nz (nothing runs)
+        auto capture = [&](const MixedFactor& Fc, hipGraphExec_t* out) -> hipError_t {
             hipGraph_t g = nullptr;
-            MX_CHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            const hipError_t e1 = enqueue_factor(w, st, n, F, planes, pstride);
+            hipError_t e0 = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+            if (e0 != hipSuccess) return e0;
+            const hipError_t e1 = enqueue_factor(w, st, n, Fc, planes, pstride);
             const hipError_t e2 = hipStreamEndCapture(st, &g);
             if (e1 != hipSuccess || e2 != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
-                MX_CHK(e1 != hipSuccess ? e1 : e2);
+                return e1 != hipSuccess ? e1 : e2;
             }
-            const hipError_t e3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            const hipError_t e3 = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
-            MX_CHK(e3);
+            return e3;
+        };
+        const void* key[LuWork::KEYS];
+        key_of(nz, key);
+        hipGraphExec_t ex = w.find_graph(key);
+        const bool miss = !ex;
+        if (miss) {
+            MX_CHK(capture(F, &ex));
             w.add_graph(key, ex);
             ++w.captures;
         } else {
             ++w.replays;
         }
         MX_CHK(hipGraphLaunch(ex, st));
+        if (miss && w.ahead > 1) {
+            // The first capture of this order also captures the other batch
+            // sizes up to w.ahead that the arena holds, while this batch runs:
+            // a later batch of any of them replays (a request loop then
+            // captures nothing after its first request of each mesh size).
+            // Every buffer enqueue_factor touches is problem 0's or an offset
+            // of it by the arena stride, so one arena serves every batch size.
+            const int room = std::min(std::min(w.ahead, MIXED_MAX_BATCH),
+                                      (int)(w.cap[M_ARENA] / L.bytes));
+            for (int z = 1; z <= room; ++z) {
+                if (z == nz) continue;
+                key_of(z, key);
+                if (w.find_graph(key)) continue;
+                MixedFactor Fz = F;
+                Fz.nz = z;
+                hipGraphExec_t ez = nullptr;
+                MX_CHK(capture(Fz, &ez));
+                w.add_graph(key, ez);
+                ++w.precaptures;
+            }
+        }
     } else {
         MX_CHK(enqueue_factor(w, st, n, F, planes, pstride));
     }

@@ -256,7 +256,8 @@ def serve_slice(args, api, decks, n_requests, rate, check):
                       for _ in range(args.slots)]:
                 srv.wait(w)
         warm = srv.stats()
-        log("serve: warm-up done (%d graphs captured)" % warm["graph_captures"])
+        log("serve: warm-up done (%d graphs captured, %d ahead)" %
+            (warm["graph_captures"], warm["graph_precaptures"]))
         ids = [None] * len(reqs)
 
         def feeder():
@@ -316,6 +317,7 @@ def serve_slice(args, api, decks, n_requests, rate, check):
         "service_p99_ms": percentile([o[1]["service_ms"] for o in outs], 99),
         "graph_captures_after_warmup": end["graph_captures"] - warm["graph_captures"],
         "graph_replays": end["graph_replays"] - warm["graph_replays"],
+        "graph_precaptures_after_warmup": end["graph_precaptures"] - warm["graph_precaptures"],
         "peak_charged_bytes": end["peak_charged_bytes"], "budget_bytes": end["budget_bytes"],
         "over_charge": end["over_charge"],
         "max_rel_err_mixed_vs_fp64": max(errs), "checked_requests": len(errs),

@@ -226,11 +226,14 @@ int bemx_server_submit(bemx_server *srv, const double *cx, const double *cy, con
  * (may be NULL) receives the error text.  Each id can be waited once. */
 int bemx_server_wait(bemx_server *srv, long long id, double *x, double *info, char *msg,
                      int msglen);
-/* Statistics of the request loop (nout <= 9 doubles): requests completed,
- * factorisation graphs captured, graph replays, bytes charged now, peak
- * bytes charged, the budget, workspaces given back to make room, the
- * number of batches after which a slot held more device bytes than it was
- * charged (0 unless the bound is wrong), and batches run. */
+/* Statistics of the request loop (nout <= 10 doubles): requests completed,
+ * factorisation graphs captured for a batch, graph replays, bytes charged
+ * now, peak bytes charged, the budget, workspaces given back to make room,
+ * the number of batches after which a slot held more device bytes than it
+ * was charged (0 unless the bound is wrong), batches run, and graphs
+ * captured ahead (a slot's first batch of a mesh size also captures the
+ * other batch sizes its workspace holds, up to the batch limit; a later
+ * batch of such a size replays).  Captures plus replays equal batches. */
 int bemx_server_stats(bemx_server *srv, double *out, int nout);
 /* Stops the workers after their current request; requests still queued
  * are dropped.  Each worker releases its slot's workspace before it
