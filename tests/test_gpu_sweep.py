@@ -49,6 +49,30 @@ def test_sweep_equals_each_frequency_alone(ctx, mode, freqs):
         assert st[k, 0] == s1[0]  # same GMRES iterations
 
 
+def test_sweep_captures_one_graph(ctx):
+    """A fresh context's first mixed sweep of eleven frequencies (batches of
+    six and five) captures one factorisation graph, the batch of six's; the
+    other batch sizes its workspace holds are captured ahead with it, so the
+    batch of five and a later single solve of the same mesh replay."""
+    _, mesh, dyn = ctx
+    c = api.Context(0)
+    try:
+        c.send_shared_data(mesh)
+        c.ghmatece(mesh.ge, mesh.rnu)
+        c.send_load_cases(dyn.load_cases.T.astype(np.complex128), dyn.kode)
+        zmat = api.material(mesh.ge, mesh.rnu, dyn.dam, dyn.rho)
+        c.solve_sweep(zmat, FREQS11, api.MODE_MIXED)
+        s = c.last_stats()
+        print("after the sweep", s, flush=True)
+        assert (s["graph_captures"], s["graph_replays"]) == (1, 1), s
+        c.solve(zmat, FREQS11[4], api.MODE_MIXED)
+        s = c.last_stats()
+        print("after one solve", s, flush=True)
+        assert (s["graph_captures"], s["graph_replays"]) == (1, 2), s
+    finally:
+        c.close()
+
+
 def test_sweep_refuses_per_case_kode(ctx):
     c, mesh, dyn = ctx
     zdfi = dyn.load_cases.T.astype(np.complex128)
